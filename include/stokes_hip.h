@@ -1,0 +1,89 @@
+/* stokes_hip.h -- entry points of the MI355X libstokes operator that have
+ * no counterpart in the reference API: int status codes, device-resident
+ * buffers, several right-hand sides per launch, mixed precision, Lanczos
+ * bounds and the device-resident Chebyshev recursion.
+ *
+ * Every function returns 0 on success or a negative STOKES_HIP_E* code;
+ * stokes_hip_last_error() gives the message.  Pointers named *_dev are
+ * device (HBM) pointers, e.g. from stokes_hip_malloc().  Vectors are laid
+ * out as in the reference (particle-major, 3/6/11 components per particle,
+ * extracted-form S/E); a block of nrhs vectors is stored column after
+ * column (vector c at offset c*n).
+ */
+#ifndef LIBSTOKES_HIP_STOKES_HIP_H
+#define LIBSTOKES_HIP_STOKES_HIP_H
+
+#include <stddef.h>
+#include "stokes.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  STOKES_HIP_OK = 0,
+  STOKES_HIP_EHIP = -1,
+  STOKES_HIP_EARG = -2,
+  STOKES_HIP_ENOMEM = -3,
+  STOKES_HIP_EUNSUP = -4,
+  STOKES_HIP_ENODEV = -5
+};
+
+/* arithmetic of the operator of sys:
+ *  FP64  -- every part in FP64 (parity path, reference to ~1e-14)
+ *  MIXED -- real-space pairs and reciprocal phases/sums in FP32 with FP64
+ *           displacements, partial sums and self term */
+enum { STOKES_HIP_FP64 = 0, STOKES_HIP_MIXED = 1 };
+
+const char *stokes_hip_last_error (void);
+
+int stokes_hip_set_precision (struct stokes *sys, int mode);
+/* record per-part GPU times (ms) into sys->cpu1/cpu2/cpu3 */
+int stokes_hip_set_timing (struct stokes *sys, int enable);
+
+/* device memory on the GPU of the library */
+int stokes_hip_malloc (void **p, size_t bytes);
+int stokes_hip_free (void *p);
+int stokes_hip_memcpy_h2d (void *dst_dev, const void *src, size_t bytes);
+int stokes_hip_memcpy_d2h (void *dst, const void *src_dev, size_t bytes);
+/* wait for all work queued for sys */
+int stokes_hip_sync (struct stokes *sys);
+
+/* take positions [3*np] from device memory for the following calls
+ * (instead of re-reading sys->pos on every call) until the next call of
+ * this function or stokes_hip_use_host_pos() */
+int stokes_hip_set_pos_dev (struct stokes *sys, const double *pos_dev);
+int stokes_hip_use_host_pos (struct stokes *sys);
+
+/* y = M x for nrhs vectors, device pointers */
+int stokes_hip_atimes_dev (struct stokes *sys, int nrhs, const double *x_dev,
+                           double *y_dev);
+/* int-returning form of atimes_3all (host pointers) */
+int stokes_hip_atimes (struct stokes *sys, int n, const double *x, double *y);
+
+/* dense extracted-form matrix (n x n, row-major) into device memory */
+int stokes_hip_make_matrix_dev (struct stokes *sys, double *mat_dev);
+/* int-returning form of make_matrix_mob_3all (host pointer) */
+int stokes_hip_make_matrix (struct stokes *sys, double *mat);
+/* y = mat . T(x) for nrhs vectors (FTS) or mat . x (F/FT) */
+int stokes_hip_extmat_dev (struct stokes *sys, const double *mat_dev,
+                           int nrhs, const double *x_dev, double *y_dev);
+
+/* Chebyshev-Clenshaw z = f(M) y for nrhs vectors held on the device
+ * (chebyshev.c:163-228 with the whole recursion on the GPU) */
+int stokes_hip_cheb_eval_dev (struct stokes *sys, int ncheb, const double *a,
+                              int nrhs, const double *y_dev, double *z_dev,
+                              double eig0, double eig1);
+
+/* FP64 Lanczos estimate of the extreme eigenvalues of the F-version (or
+ * FT) mobility of sys; replaces the ARPACK bounds of dnaupd_c.c:909-1117
+ * as used by brownian.c:1690-1699.  lmin/lmax are widened by the given
+ * relative margin. */
+int stokes_hip_lanczos_bounds (struct stokes *sys, int niter,
+                               unsigned long seed, double margin,
+                               double *lmin, double *lmax);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,319 @@
+"""libstokes_amd -- Python view of the MI355X libstokes operator.
+
+The product is the C-ABI library ``libstokes_amd/libstokes_hip.so`` (C host
+code + HIP kernels for gfx950) with the reference's ``struct stokes`` /
+``atimes_3all`` interface (include/*.h).  This module binds it with ctypes
+so that tests and the benchmark drive exactly the entry points a C caller
+uses.  There is no fallback: if the library is missing, or it cannot reach a
+GPU, every call raises.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libstokes_hip.so")
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+_vp = C.c_void_p
+
+FP64 = 0
+MIXED = 1
+
+# symbols declared in include/*.h (checked by the CPU tests)
+EXPORTS = [
+    # stokes.h
+    "stokes_init", "stokes_free", "stokes_set_np", "stokes_set_Ui",
+    "stokes_set_Oi", "stokes_set_Ei", "stokes_set_shear",
+    "stokes_get_shear_shift", "stokes_set_shear_shift", "stokes_set_l",
+    "stokes_set_xi", "xi_by_tratio", "stokes_set_iter", "stokes_set_pos",
+    "stokes_set_pos_mobile", "stokes_set_pos_fixed", "stokes_set_radius",
+    "stokes_unset_radius",
+    # ewald.h
+    "scalars_ewald_real", "scalars_ewald_real_poly",
+    "scalars_mob_poly_scale_SD_", "atimes_3all", "make_matrix_mob_3all",
+    "atimes_3all_matrix", "atimes_ewald_3all", "make_matrix_mob_ewald_3all",
+    # non-ewald.h
+    "scalars_nonewald", "atimes_nonewald_3all",
+    "make_matrix_mob_nonewald_3all",
+    # matrix.h
+    "dot_prod_matrix", "multiply_extmat_with_extvec_3fts",
+    # chebyshev.h
+    "chebyshev_coef", "chebyshev_eval", "chebyshev_eval_atimes",
+    "chebyshev_error_minvsqrt", "chebyshev_error_Rsqrt",
+    # stokes_hip.h
+    "stokes_hip_last_error", "stokes_hip_set_precision",
+    "stokes_hip_set_timing", "stokes_hip_malloc", "stokes_hip_free",
+    "stokes_hip_memcpy_h2d", "stokes_hip_memcpy_d2h", "stokes_hip_sync",
+    "stokes_hip_set_pos_dev", "stokes_hip_use_host_pos",
+    "stokes_hip_atimes_dev", "stokes_hip_atimes", "stokes_hip_make_matrix_dev",
+    "stokes_hip_make_matrix", "stokes_hip_extmat_dev",
+    "stokes_hip_cheb_eval_dev", "stokes_hip_lanczos_bounds",
+]
+
+
+class StokesStruct(C.Structure):
+    """struct stokes of include/stokes.h (same layout as the reference)."""
+    _fields_ = [
+        ("version", C.c_int), ("np", C.c_int), ("nm", C.c_int),
+        ("pos", _dp), ("a", _dp), ("rmin", C.c_double),
+        ("twobody_nmax", C.c_int), ("twobody_lub", C.c_int),
+        ("poly_table", _ip), ("twobody_f_list", _vp),
+        ("slip", _dp), ("slip_a", _dp), ("slip_G32", _dp),
+        ("slip_G30", _dp), ("slip_G52", _dp), ("slip_table", _ip),
+        ("twobody_slip_f_list", _vp),
+        ("Ui", C.c_double * 3), ("Oi", C.c_double * 3), ("Ei", C.c_double * 5),
+        ("shear_mode", C.c_int), ("shear_rate", C.c_double),
+        ("shear_shift", C.c_double),
+        ("periodic", C.c_int),
+        ("ewald_eps", C.c_double), ("rmax2", C.c_double),
+        ("rmaxx", C.c_int), ("rmaxy", C.c_int), ("rmaxz", C.c_int),
+        ("kmax", C.c_double),
+        ("kmaxx", C.c_int), ("kmaxy", C.c_int), ("kmaxz", C.c_int),
+        ("xi", C.c_double), ("xi2", C.c_double), ("xiaspi", C.c_double),
+        ("xia2", C.c_double), ("pivol", C.c_double),
+        ("lx", C.c_double), ("ly", C.c_double), ("lz", C.c_double),
+        ("ilx", C.c_int * 27), ("ily", C.c_int * 27), ("ilz", C.c_int * 27),
+        ("self_a", C.c_double), ("self_c", C.c_double), ("self_m", C.c_double),
+        ("flag_table", C.c_int), ("nr", C.c_int),
+        ("rlx", _dp), ("rly", _dp), ("rlz", _dp),
+        ("rmx", _ip), ("rmy", _ip), ("rmz", _ip),
+        ("nk", C.c_int),
+        ("ex", _dp), ("ey", _dp), ("ez", _dp), ("k", _dp), ("k1", _dp),
+        ("k2", _dp), ("k3", _dp), ("ya", _dp), ("yb", _dp), ("yc", _dp),
+        ("yg", _dp), ("yh", _dp), ("ym", _dp),
+        ("lubmin2", C.c_double), ("lubmax", C.c_double),
+        ("cpu1", C.c_double), ("cpu2", C.c_double), ("cpu3", C.c_double),
+        ("it", _vp),
+    ]
+
+
+class StokesError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def lib():
+    """Load libstokes_hip.so (raises if it is not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise StokesError(
+            f"{LIB_PATH} is not built; run `make -C libstokes_amd` "
+            "(or __graft_entry__.build())")
+    L = C.CDLL(LIB_PATH)
+    sp = C.POINTER(StokesStruct)
+    L.stokes_init.restype = sp
+    L.stokes_free.argtypes = [sp]
+    L.stokes_set_np.argtypes = [sp, C.c_int, C.c_int]
+    L.stokes_set_l.argtypes = [sp, C.c_double, C.c_double, C.c_double]
+    L.stokes_set_xi.argtypes = [sp, C.c_double, C.c_double]
+    L.xi_by_tratio.restype = C.c_double
+    L.xi_by_tratio.argtypes = [sp, C.c_double]
+    L.stokes_set_pos.argtypes = [sp, _dp]
+    L.stokes_set_radius.argtypes = [sp, _dp]
+    L.stokes_unset_radius.argtypes = [sp]
+    L.stokes_get_shear_shift.restype = C.c_double
+    L.stokes_hip_last_error.restype = C.c_char_p
+    for f in ("stokes_hip_set_precision", "stokes_hip_set_timing"):
+        getattr(L, f).argtypes = [sp, C.c_int]
+    L.stokes_hip_atimes.argtypes = [sp, C.c_int, _dp, _dp]
+    L.stokes_hip_atimes_dev.argtypes = [sp, C.c_int, _vp, _vp]
+    L.stokes_hip_make_matrix.argtypes = [sp, _dp]
+    L.stokes_hip_make_matrix_dev.argtypes = [sp, _vp]
+    L.stokes_hip_extmat_dev.argtypes = [sp, _vp, C.c_int, _vp, _vp]
+    L.stokes_hip_cheb_eval_dev.argtypes = [sp, C.c_int, _dp, C.c_int, _vp,
+                                           _vp, C.c_double, C.c_double]
+    L.stokes_hip_lanczos_bounds.argtypes = [sp, C.c_int, C.c_ulong,
+                                            C.c_double, _dp, _dp]
+    L.stokes_hip_malloc.argtypes = [C.POINTER(_vp), C.c_size_t]
+    L.stokes_hip_free.argtypes = [_vp]
+    L.stokes_hip_memcpy_h2d.argtypes = [_vp, _vp, C.c_size_t]
+    L.stokes_hip_memcpy_d2h.argtypes = [_vp, _vp, C.c_size_t]
+    L.stokes_hip_sync.argtypes = [sp]
+    L.stokes_hip_set_pos_dev.argtypes = [sp, _vp]
+    L.stokes_hip_use_host_pos.argtypes = [sp]
+    L.chebyshev_eval.restype = C.c_double
+    L.chebyshev_eval.argtypes = [C.c_int, _dp, C.c_double, C.c_double,
+                                 C.c_double]
+    L.scalars_nonewald.argtypes = [C.c_int, C.c_double, _dp]
+    _lib = L
+    return L
+
+
+def _p(a):
+    return a.ctypes.data_as(_dp)
+
+
+def check(rc, what):
+    if rc != 0:
+        msg = lib().stokes_hip_last_error()
+        raise StokesError(f"{what} failed ({rc}): "
+                          f"{msg.decode() if msg else ''}")
+
+
+def dim(version):
+    return (3, 6, 11)[version]
+
+
+class DeviceArray:
+    """A float64 buffer in GPU memory owned by the library's allocator."""
+
+    def __init__(self, n):
+        self.n = int(n)
+        self.ptr = _vp()
+        check(lib().stokes_hip_malloc(C.byref(self.ptr), 8 * max(self.n, 1)),
+              "stokes_hip_malloc")
+
+    @classmethod
+    def from_host(cls, a):
+        a = np.ascontiguousarray(a, dtype=np.float64).reshape(-1)
+        d = cls(a.size)
+        check(lib().stokes_hip_memcpy_h2d(d.ptr, a.ctypes.data_as(_vp),
+                                          8 * a.size), "h2d")
+        return d
+
+    def to_host(self):
+        out = np.empty(self.n)
+        check(lib().stokes_hip_memcpy_d2h(out.ctypes.data_as(_vp), self.ptr,
+                                          8 * self.n), "d2h")
+        return out
+
+    def free(self):
+        if self.ptr:
+            check(lib().stokes_hip_free(self.ptr), "stokes_hip_free")
+            self.ptr = _vp()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Stokes:
+    """A struct stokes created and configured through the C API."""
+
+    def __init__(self, version, pos, box=None, tr=None, xi=None, eps=1e-12,
+                 a=None, rmin=0.0, shear_mode=0, shear_shift=0.0,
+                 precision=FP64):
+        L = lib()
+        self.L = L
+        self.sys = L.stokes_init()
+        s = self.sys.contents
+        pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1)
+        self.np = pos.size // 3
+        s.version = version
+        self.version = version
+        L.stokes_set_np(self.sys, self.np, self.np)
+        L.stokes_set_pos(self.sys, _p(pos))
+        if a is not None:
+            a = np.ascontiguousarray(a, dtype=np.float64)
+            L.stokes_set_radius(self.sys, _p(a))
+        s.rmin = rmin
+        if box is not None:
+            s.periodic = 1
+            L.stokes_set_l(self.sys, *[float(b) for b in box])
+            if xi is None:
+                xi = L.xi_by_tratio(self.sys, float(tr if tr else 1.0))
+            L.stokes_set_xi(self.sys, float(xi), float(eps))
+        s.shear_mode = shear_mode
+        s.shear_shift = shear_shift
+        self.n = dim(version) * self.np
+        if precision != FP64:
+            self.set_precision(precision)
+
+    @property
+    def s(self):
+        return self.sys.contents
+
+    def set_pos(self, pos):
+        pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1)
+        self.L.stokes_set_pos(self.sys, _p(pos))
+
+    def set_precision(self, mode):
+        check(self.L.stokes_hip_set_precision(self.sys, mode),
+              "set_precision")
+
+    def set_timing(self, on=True):
+        check(self.L.stokes_hip_set_timing(self.sys, 1 if on else 0),
+              "set_timing")
+
+    def params(self):
+        s = self.s
+        return dict(xi=s.xi, rmax2=s.rmax2, rmaxx=s.rmaxx, rmaxy=s.rmaxy,
+                    rmaxz=s.rmaxz, kmax=s.kmax, kmaxx=s.kmaxx, kmaxy=s.kmaxy,
+                    kmaxz=s.kmaxz, nr=s.nr, self_a=s.self_a, self_c=s.self_c,
+                    self_m=s.self_m, pivol=s.pivol)
+
+    def image_table(self):
+        s = self.s
+        rl = np.array([[s.rlx[m], s.rly[m], s.rlz[m]] for m in range(s.nr)])
+        rm = np.array([[s.rmx[m], s.rmy[m], s.rmz[m]] for m in range(s.nr)],
+                      dtype=np.int32)
+        return rl, rm
+
+    def atimes(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.empty(self.n)
+        check(self.L.stokes_hip_atimes(self.sys, self.n, _p(x), _p(y)),
+              "atimes")
+        return y
+
+    def atimes_dev(self, x_dev, y_dev, nrhs=1):
+        check(self.L.stokes_hip_atimes_dev(self.sys, nrhs, x_dev.ptr,
+                                           y_dev.ptr), "atimes_dev")
+
+    def matrix(self):
+        m = np.empty((self.n, self.n))
+        check(self.L.stokes_hip_make_matrix(self.sys, _p(m)), "make_matrix")
+        return m
+
+    def matrix_dev(self, mat_dev):
+        check(self.L.stokes_hip_make_matrix_dev(self.sys, mat_dev.ptr),
+              "make_matrix_dev")
+
+    def extmat_dev(self, mat_dev, x_dev, y_dev, nrhs=1):
+        check(self.L.stokes_hip_extmat_dev(self.sys, mat_dev.ptr, nrhs,
+                                           x_dev.ptr, y_dev.ptr), "extmat")
+
+    def cheb_dev(self, acoef, y_dev, z_dev, e0, e1, nrhs=1):
+        a = np.ascontiguousarray(acoef, dtype=np.float64)
+        check(self.L.stokes_hip_cheb_eval_dev(self.sys, a.size, _p(a), nrhs,
+                                              y_dev.ptr, z_dev.ptr, e0, e1),
+              "cheb_eval_dev")
+
+    def lanczos_bounds(self, niter=40, seed=1, margin=0.0):
+        lo, hi = C.c_double(), C.c_double()
+        check(self.L.stokes_hip_lanczos_bounds(self.sys, niter, seed, margin,
+                                               C.byref(lo), C.byref(hi)),
+              "lanczos")
+        return lo.value, hi.value
+
+    def sync(self):
+        check(self.L.stokes_hip_sync(self.sys), "sync")
+
+    def times(self):
+        s = self.s
+        return s.cpu1, s.cpu2, s.cpu3
+
+    def close(self):
+        if self.sys:
+            self.L.stokes_free(self.sys)
+            self.sys = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def exported_symbols():
+    """Names declared in include/*.h that the library must export."""
+    return list(EXPORTS)

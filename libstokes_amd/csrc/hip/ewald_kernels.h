@@ -1,0 +1,401 @@
+// ewald_kernels.h -- device kernels of the Ewald/RPY mobility operator.
+//
+//   k_pairs     real-space pair sum (all pairs x periodic images inside
+//               r_max), FP64 or FP32, moment accumulation per pair and one
+//               11x11 contraction per pair; ewald.c:649-789.
+//   k_recip_fwd structure factors C(k) = sum_j cos(k.x_j) z_j and
+//               S(k) = sum_j sin(k.x_j) z_j and the per-k blocks
+//               P = Ac C - As S, Q = Ac S + As C; ewald.c:923-1075 recast.
+//   k_finalize  y_i = self_i z_i + sum_k cos(k.x_i) P + sin(k.x_i) Q
+//               + sum over j-chunks of the real-space partials.
+//   k_matrix    dense (11N)^2 extracted-form matrix, ewald.c:1092-1677.
+//   k_extmat    y = mat . T(x), matrix.c:447-478.
+#pragma once
+#include <hip/hip_runtime.h>
+#include "pair_math.h"
+
+namespace shk {
+
+struct DevParams {
+  int np, nr, nk;
+  int poly;
+  double xi, rmax2, rmin;
+  double self_a, self_c, self_m, xiaspi, xia2;
+};
+
+// self scalars of particle i (ewald.c:540-643)
+template <typename T>
+__device__ __forceinline__ void self_scalars (const DevParams &p, double ai,
+                                              bool periodic, T *s)
+{
+#pragma unroll
+  for (int k = 0; k < 11; k++) s[k] = T (0);
+  double sa, sc, sm;
+  if (!periodic)
+    { sa = 1.0; sc = 0.75; sm = 0.9; }
+  else if (!p.poly)
+    { sa = p.self_a; sc = p.self_c; sm = p.self_m; }
+  else
+    {
+      double a = ai, a2 = a * a, a3 = a2 * a;
+      sa = 1.0 - a * p.xiaspi * (6.0 - 40.0 / 3.0 * p.xia2 * a2);
+      sc = 0.75 - a3 * p.xiaspi * p.xia2 * 10.0;
+      sm = 0.9 - a3 * p.xiaspi * p.xia2 * (12.0 - 30.24 * p.xia2 * a2);
+    }
+  s[0] = s[1] = T (sa);
+  s[3] = s[4] = T (sc);
+  s[8] = s[9] = s[10] = T (sm);
+}
+
+template <int V, typename T>
+__device__ __forceinline__ void load_z (const double *x, int j, T *z)
+{
+  constexpr int D = Dim<V>::D;
+#pragma unroll
+  for (int q = 0; q < D; q++) z[q] = T (x[(size_t) j * D + q]);
+  if (V == 2) s_transform (z);
+}
+
+// acc[p] += sum_q block(M)[p][q] * z[q]
+template <int V, typename T, typename A>
+__device__ __forceinline__ void block_apply (const T *M, const T *z, A *acc)
+{
+  constexpr int D = Dim<V>::D;
+#pragma unroll
+  for (int p = 0; p < D; p++)
+    {
+      T s = T (0);
+#pragma unroll
+      for (int q = 0; q < D; q++) s += block_entry<V, T> (M, p, q, false) * z[q];
+      acc[p] += A (s);
+    }
+}
+
+// image vector (the shear fold rm*shift is already in .x)
+template <typename T> struct Img;
+template <> struct Img<double> { using type = double4; };
+template <> struct Img<float> { using type = float4; };
+
+// KIND: 0 Ewald, 1 open-BC RPY
+template <int V, typename T, int KIND>
+__global__ __launch_bounds__ (256) void
+k_pairs (const double4 *__restrict__ pos, const double *__restrict__ x,
+         const typename Img<T>::type *__restrict__ img, DevParams p,
+         int jchunk, double *__restrict__ partial)
+{
+  constexpr int D = Dim<V>::D, NM = Dim<V>::NMOM;
+  const int ti = threadIdx.x >> 4, tj = threadIdx.x & 15;
+  const int i = blockIdx.x * 16 + ti;
+  const int jbeg = blockIdx.y * jchunk;
+  const int jend = min (p.np, jbeg + jchunk);
+  T acc[D];
+#pragma unroll
+  for (int q = 0; q < D; q++) acc[q] = T (0);
+  double4 pi = pos[min (i, p.np - 1)];
+  const T rmax2 = T (p.rmax2);
+  const T xi = T (p.xi);
+
+  for (int jb = jbeg; jb < jend; jb += 16)
+    {
+      const int j = jb + tj;
+      if (i >= p.np || j >= jend) continue;
+      double4 pj = pos[j];
+      T M[NM];
+#pragma unroll
+      for (int k = 0; k < NM; k++) M[k] = T (0);
+      const double d0x = pj.x - pi.x, d0y = pj.y - pi.y, d0z = pj.z - pi.z;
+      const T rminij = T ((pi.w + pj.w) * p.rmin);
+      if (KIND == 0)
+        {
+          const T dx0 = T (d0x), dy0 = T (d0y), dz0 = T (d0z);
+          for (int m = 0; m < p.nr; m++)
+            {
+              const typename Img<T>::type im = img[m];
+              const T dx = dx0 + T (im.x), dy = dy0 + T (im.y), dz = dz0 + T (im.z);
+              const T rr = dx * dx + dy * dy + dz * dz;
+              if (rr == T (0) || rr > rmax2) continue;
+              T r = m_sqrt (rr);
+              if (r < rminij) r = rminij;
+              const T ir = T (1) / r;
+              T s[11];
+              if (p.poly)
+                scalars_real_poly_sd<V, T> (xi, r, T (pi.w), T (pj.w), s);
+              else
+                scalars_real<V, T> (xi, r, s);
+              moments_add<V, T> (M, dx * ir, dy * ir, dz * ir, s);
+            }
+        }
+      else
+        {
+          if (i == j) continue;
+          const T dx = T (d0x), dy = T (d0y), dz = T (d0z);
+          T r = m_sqrt (dx * dx + dy * dy + dz * dz);
+          if (r < rminij) r = rminij;
+          const T ir = T (1) / r;
+          T s[11];
+          scalars_rpy_sd<V, T> (r, T (pi.w), T (pj.w), p.poly != 0, s);
+          moments_add<V, T> (M, dx * ir, dy * ir, dz * ir, s);
+        }
+      T z[D];
+      load_z<V, T> (x, j, z);
+      block_apply<V, T, T> (M, z, acc);
+    }
+#pragma unroll
+  for (int q = 0; q < D; q++)
+    {
+      T v = acc[q];
+      v += __shfl_xor (v, 8, 16);
+      v += __shfl_xor (v, 4, 16);
+      v += __shfl_xor (v, 2, 16);
+      v += __shfl_xor (v, 1, 16);
+      acc[q] = v;
+    }
+  if (tj == 0 && i < p.np)
+    {
+      double *out = partial + ((size_t) blockIdx.y * p.np + i) * D;
+#pragma unroll
+      for (int q = 0; q < D; q++) out[q] = double (acc[q]);
+    }
+}
+
+// phase -> (cos, sin) in precision T; FP32 reduces the argument in FP64
+__device__ __forceinline__ void phase_cs (double ph, double &c, double &s)
+{
+  sincos (ph, &s, &c);
+}
+__device__ __forceinline__ void phase_cs (double ph, float &c, float &s)
+{
+  const double inv2pi = 0.15915494309189533577, twopi = 6.28318530717958647692;
+  double red = ph - twopi * rint (ph * inv2pi);
+  sincosf (float (red), &s, &c);
+}
+
+// Ac / As moment sets of one k (unit direction kh, paired-k weights w)
+template <int V, typename T>
+__device__ __forceinline__ void kblock_moments (const double4 kvec, const double *w,
+                                                T *Mc, T *Ms)
+{
+  constexpr int NM = Dim<V>::NMOM;
+#pragma unroll
+  for (int k = 0; k < NM; k++) { Mc[k] = T (0); Ms[k] = T (0); }
+  const T ik = T (1.0 / kvec.w);
+  const T ex = T (kvec.x) * ik, ey = T (kvec.y) * ik, ez = T (kvec.z) * ik;
+  T sc[11] = {T (0), T (w[0]), T (0), T (0), T (w[2]), T (0), T (0), T (w[4]), T (0), T (w[5]), T (0)};
+  T ss[11] = {T (0), T (0), T (w[1]), T (0), T (0), T (0), T (w[3]), T (0), T (0), T (0), T (0)};
+  moments_add<V, T> (Mc, ex, ey, ez, sc);
+  moments_add<V, T> (Ms, ex, ey, ez, ss);
+}
+
+template <int V, typename T>
+__global__ __launch_bounds__ (256) void
+k_recip_fwd (const double4 *__restrict__ pos, const double *__restrict__ x,
+             const double4 *__restrict__ kv, const double *__restrict__ ks,
+             int np, double *__restrict__ PQ)
+{
+  constexpr int D = Dim<V>::D, NM = Dim<V>::NMOM;
+  __shared__ double red[256];
+  __shared__ double CS[2 * D];
+  const int k = blockIdx.x;
+  const double4 kk = kv[k];
+  T C[D], S[D];
+#pragma unroll
+  for (int q = 0; q < D; q++) { C[q] = T (0); S[q] = T (0); }
+  for (int j = threadIdx.x; j < np; j += blockDim.x)
+    {
+      const double4 pj = pos[j];
+      T c, s;
+      phase_cs (kk.x * pj.x + kk.y * pj.y + kk.z * pj.z, c, s);
+      T z[D];
+      load_z<V, T> (x, j, z);
+#pragma unroll
+      for (int q = 0; q < D; q++) { C[q] += c * z[q]; S[q] += s * z[q]; }
+    }
+  // block reduction, one component at a time (deterministic order)
+#pragma unroll
+  for (int q = 0; q < 2 * D; q++)
+    {
+      const T v = q < D ? C[q] : S[q - D];
+      red[threadIdx.x] = double (v);
+      __syncthreads ();
+      for (int st = blockDim.x / 2; st > 0; st >>= 1)
+        {
+          if (threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+          __syncthreads ();
+        }
+      if (threadIdx.x == 0) CS[q] = red[0];
+      __syncthreads ();
+    }
+  if (threadIdx.x == 0)
+    {
+      double Mc[NM], Ms[NM];
+      kblock_moments<V, double> (kk, ks + (size_t) k * 8, Mc, Ms);
+#pragma unroll
+      for (int pr = 0; pr < D; pr++)
+        {
+          double P = 0.0, Q = 0.0;
+#pragma unroll
+          for (int q = 0; q < D; q++)
+            {
+              const double ac = block_entry<V, double> (Mc, pr, q, false);
+              const double as = block_entry<V, double> (Ms, pr, q, false);
+              P += ac * CS[q] - as * CS[D + q];
+              Q += ac * CS[D + q] + as * CS[q];
+            }
+          PQ[((size_t) k * 2 + 0) * D + pr] = P;
+          PQ[((size_t) k * 2 + 1) * D + pr] = Q;
+        }
+    }
+}
+
+template <int V, typename T>
+__global__ __launch_bounds__ (256) void
+k_finalize (const double4 *__restrict__ pos, const double *__restrict__ x,
+            const double4 *__restrict__ kv, const double *__restrict__ PQ,
+            const double *__restrict__ partial, int nchunk, DevParams p,
+            int periodic, double *__restrict__ y)
+{
+  constexpr int D = Dim<V>::D, NM = Dim<V>::NMOM;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= p.np) return;
+  const double4 pi = pos[i];
+  double acc[D];
+#pragma unroll
+  for (int q = 0; q < D; q++) acc[q] = 0.0;
+  {
+    double s[11], M[NM];
+#pragma unroll
+    for (int k = 0; k < NM; k++) M[k] = 0.0;
+    self_scalars<double> (p, pi.w, periodic != 0, s);
+    moments_add<V, double> (M, 0.0, 0.0, 0.0, s);
+    double z[D];
+    load_z<V, double> (x, i, z);
+    block_apply<V, double, double> (M, z, acc);
+  }
+  for (int k = 0; k < p.nk; k++)
+    {
+      const double4 kk = kv[k];
+      T c, s;
+      phase_cs (kk.x * pi.x + kk.y * pi.y + kk.z * pi.z, c, s);
+      const double *P = PQ + (size_t) k * 2 * D;
+#pragma unroll
+      for (int q = 0; q < D; q++) acc[q] += double (c) * P[q] + double (s) * P[D + q];
+    }
+  for (int ch = 0; ch < nchunk; ch++)
+    {
+      const double *pp = partial + ((size_t) ch * p.np + i) * D;
+#pragma unroll
+      for (int q = 0; q < D; q++) acc[q] += pp[q];
+    }
+#pragma unroll
+  for (int q = 0; q < D; q++) y[(size_t) i * D + q] = acc[q];
+}
+
+// Dense build: one thread per (i, j) block.
+template <int V, int KIND>
+__global__ __launch_bounds__ (64) void
+k_matrix (const double4 *__restrict__ pos, const double4 *__restrict__ img,
+          const double4 *__restrict__ kv, const double *__restrict__ ks,
+          DevParams p, double *__restrict__ mat)
+{
+  constexpr int D = Dim<V>::D, NM = Dim<V>::NMOM;
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = blockIdx.y;
+  if (j >= p.np) return;
+  const double4 pi = pos[i], pj = pos[j];
+  double M[NM];
+#pragma unroll
+  for (int k = 0; k < NM; k++) M[k] = 0.0;
+  double s[11];
+  if (i == j)
+    {
+      self_scalars<double> (p, pi.w, KIND == 0, s);
+      moments_add<V, double> (M, 0.0, 0.0, 0.0, s);
+    }
+  const double d0x = pj.x - pi.x, d0y = pj.y - pi.y, d0z = pj.z - pi.z;
+  const double rminij = (pi.w + pj.w) * p.rmin;
+  if (KIND == 0)
+    {
+      for (int m = 0; m < p.nr; m++)
+        {
+          const double4 im = img[m];
+          const double dx = d0x + im.x, dy = d0y + im.y, dz = d0z + im.z;
+          const double rr = dx * dx + dy * dy + dz * dz;
+          if (rr == 0.0 || rr > p.rmax2) continue;
+          double r = sqrt (rr);
+          if (r < rminij) r = rminij;
+          if (p.poly) scalars_real_poly_sd<V, double> (p.xi, r, pi.w, pj.w, s);
+          else scalars_real<V, double> (p.xi, r, s);
+          moments_add<V, double> (M, dx / r, dy / r, dz / r, s);
+        }
+      for (int k = 0; k < p.nk; k++)
+        {
+          const double4 kk = kv[k];
+          const double *w = ks + (size_t) k * 8;
+          double sf, cf;
+          sincos (kk.x * d0x + kk.y * d0y + kk.z * d0z, &sf, &cf);
+          sf = -sf;
+          double ya = w[0], yb = w[1], yc = w[2], yg = w[3], yh = w[4], ym = w[5];
+          if (p.poly)
+            {
+              // ewald.c:985-1025, SD form: a_i powers and a_i^2, a_j^2 terms
+              const double aa = pi.w, aa2 = aa * aa, aa3 = aa2 * aa, ab2 = pj.w * pj.w;
+              const double kk2 = w[6], kexp2 = 2.0 * w[7], kn = kk.w;
+              ya = aa * 6.0 * (1.0 - kk2 * (aa2 + ab2) / 6.0) * kexp2;
+              yb = aa2 * 3.0 * kn * kexp2;
+              yc = aa3 * 1.5 * kk2 * kexp2;
+              yg = aa2 * 3.0 * (1.0 - kk2 * (aa2 / 10.0 + ab2 / 6.0)) * kn * kexp2;
+              yh = aa3 * 1.5 * kk2 * kexp2;
+              ym = aa3 * 3.0 * (1.0 - kk2 * (aa2 + ab2) / 10.0) * kk2 * kexp2;
+            }
+          const double ik = 1.0 / kk.w;
+          double sc[11] = {0.0, cf * ya, sf * yb, 0.0, cf * yc, 0.0, sf * yg, cf * yh, 0.0, cf * ym, 0.0};
+          moments_add<V, double> (M, kk.x * ik, kk.y * ik, kk.z * ik, sc);
+        }
+    }
+  else if (i != j)
+    {
+      double r = sqrt (d0x * d0x + d0y * d0y + d0z * d0z);
+      if (r < rminij) r = rminij;
+      scalars_rpy_sd<V, double> (r, pi.w, pj.w, p.poly != 0, s);
+      moments_add<V, double> (M, d0x / r, d0y / r, d0z / r, s);
+    }
+  const size_t n = (size_t) p.np * D;
+  const bool self = (i == j);
+#pragma unroll
+  for (int r = 0; r < D; r++)
+    {
+      double *row = mat + (size_t) (i * D + r) * n + (size_t) j * D;
+#pragma unroll
+      for (int c = 0; c < D; c++) row[c] = block_entry<V, double> (M, r, c, self);
+    }
+}
+
+// y[row] = sum_col mat[row][col] * z[col], one wave per row
+__global__ __launch_bounds__ (256) void
+k_gemv_rows (const double *__restrict__ mat, const double *__restrict__ z,
+             size_t n, double *__restrict__ y)
+{
+  const size_t row = (size_t) blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (row >= n) return;
+  const double *mr = mat + row * n;
+  double acc = 0.0;
+  for (size_t c = lane; c < n; c += 64) acc += mr[c] * z[c];
+  for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor (acc, off, 64);
+  if (lane == 0) y[row] = acc;
+}
+
+// z = T(x) per particle (FTS S transform); copies for F/FT
+template <int V>
+__global__ void k_transform (const double *__restrict__ x, int np, double *__restrict__ z)
+{
+  constexpr int D = Dim<V>::D;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= np) return;
+  double v[D];
+  load_z<V, double> (x, i, v);
+#pragma unroll
+  for (int q = 0; q < D; q++) z[(size_t) i * D + q] = v[q];
+}
+
+}  // namespace shk

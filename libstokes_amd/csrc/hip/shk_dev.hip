@@ -1,0 +1,548 @@
+// shk_dev.hip -- device context, buffers and kernel launches behind the
+// internal C ABI of shk.h.  One context per struct stokes, one HIP stream
+// per context (default priority), deterministic reduction order.
+#include <hip/hip_runtime.h>
+#include <cstdio>
+#include <cstring>
+#include <cstdarg>
+#include <vector>
+#include <type_traits>
+#include "../shk.h"
+#include "ewald_kernels.h"
+
+using namespace shk;
+
+static thread_local char g_err[512] = "";
+
+static void set_err (const char *fmt, ...)
+{
+  va_list ap;
+  va_start (ap, fmt);
+  vsnprintf (g_err, sizeof (g_err), fmt, ap);
+  va_end (ap);
+}
+
+extern "C" const char *shk_last_error (void) { return g_err; }
+
+#define HIPCHK(call)                                                          \
+  do {                                                                        \
+    hipError_t e_ = (call);                                                   \
+    if (e_ != hipSuccess)                                                     \
+      {                                                                       \
+        set_err ("%s:%d %s: %s", __FILE__, __LINE__, #call,                   \
+                 hipGetErrorString (e_));                                     \
+        return SHK_EHIP;                                                      \
+      }                                                                       \
+  } while (0)
+
+struct DevBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+};
+
+struct shk_dev {
+  hipStream_t stream = nullptr;
+  shk_params prm{};
+  int have_sys = 0;
+  DevBuf pos, img, imgf, kv, ks, partial, pq, xs, ys, zt;
+  int timing = 0;
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  double last_ms[3] = {0, 0, 0};
+};
+
+static int ensure (shk_dev *d, DevBuf &b, size_t bytes)
+{
+  if (b.bytes >= bytes && b.p) return SHK_OK;
+  if (b.p) { HIPCHK (hipFree (b.p)); b.p = nullptr; b.bytes = 0; }
+  if (bytes == 0) return SHK_OK;
+  hipError_t e = hipMalloc (&b.p, bytes);
+  if (e != hipSuccess)
+    {
+      set_err ("hipMalloc(%zu): %s", bytes, hipGetErrorString (e));
+      return SHK_ENOMEM;
+    }
+  b.bytes = bytes;
+  (void) d;
+  return SHK_OK;
+}
+
+static void release (DevBuf &b)
+{
+  if (b.p) (void) hipFree (b.p);
+  b.p = nullptr;
+  b.bytes = 0;
+}
+
+extern "C" shk_dev *shk_dev_create (void)
+{
+  int ndev = 0;
+  if (hipGetDeviceCount (&ndev) != hipSuccess || ndev < 1)
+    {
+      set_err ("no HIP device available");
+      return nullptr;
+    }
+  shk_dev *d = new shk_dev ();
+  if (hipStreamCreateWithFlags (&d->stream, hipStreamNonBlocking) != hipSuccess)
+    {
+      set_err ("hipStreamCreate failed");
+      delete d;
+      return nullptr;
+    }
+  for (auto &e : d->ev)
+    if (hipEventCreate (&e) != hipSuccess)
+      {
+        set_err ("hipEventCreate failed");
+        return nullptr;
+      }
+  return d;
+}
+
+extern "C" void shk_dev_destroy (shk_dev *d)
+{
+  if (!d) return;
+  if (d->stream) (void) hipStreamSynchronize (d->stream);
+  for (DevBuf *b : {&d->pos, &d->img, &d->imgf, &d->kv, &d->ks, &d->partial,
+                    &d->pq, &d->xs, &d->ys, &d->zt})
+    release (*b);
+  for (auto &e : d->ev)
+    if (e) (void) hipEventDestroy (e);
+  if (d->stream) (void) hipStreamDestroy (d->stream);
+  delete d;
+}
+
+extern "C" void *shk_dev_stream (shk_dev *d) { return d ? (void *) d->stream : nullptr; }
+
+extern "C" int shk_dev_malloc (shk_dev *d, void **p, size_t bytes)
+{
+  (void) d;
+  hipError_t e = hipMalloc (p, bytes);
+  if (e != hipSuccess)
+    {
+      set_err ("hipMalloc(%zu): %s", bytes, hipGetErrorString (e));
+      return SHK_ENOMEM;
+    }
+  return SHK_OK;
+}
+
+extern "C" int shk_dev_free (shk_dev *d, void *p)
+{
+  if (d && d->stream) HIPCHK (hipStreamSynchronize (d->stream));
+  if (p) HIPCHK (hipFree (p));
+  return SHK_OK;
+}
+
+extern "C" int shk_dev_h2d (shk_dev *d, void *dst, const void *src, size_t bytes)
+{
+  HIPCHK (hipMemcpyAsync (dst, src, bytes, hipMemcpyHostToDevice, d->stream));
+  return SHK_OK;
+}
+
+extern "C" int shk_dev_d2h (shk_dev *d, void *dst, const void *src, size_t bytes)
+{
+  HIPCHK (hipMemcpyAsync (dst, src, bytes, hipMemcpyDeviceToHost, d->stream));
+  HIPCHK (hipStreamSynchronize (d->stream));
+  return SHK_OK;
+}
+
+extern "C" int shk_dev_d2d (shk_dev *d, void *dst, const void *src, size_t bytes)
+{
+  HIPCHK (hipMemcpyAsync (dst, src, bytes, hipMemcpyDeviceToDevice, d->stream));
+  return SHK_OK;
+}
+
+extern "C" int shk_dev_sync (shk_dev *d)
+{
+  HIPCHK (hipStreamSynchronize (d->stream));
+  return SHK_OK;
+}
+
+extern "C" void shk_dev_timing (shk_dev *d, int enable) { d->timing = enable; }
+
+extern "C" double shk_dev_last_ms (shk_dev *d, int which)
+{
+  if (which < 0 || which > 2) return -1.0;
+  return d->last_ms[which];
+}
+
+// positions AoS [3np] + radii -> double4 (x, y, z, a)
+// (keep_a: leave the radius component as it is)
+__global__ void k_pack_pos (const double *__restrict__ pos, const double *__restrict__ a,
+                            int np, int keep_a, double4 *__restrict__ out)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= np) return;
+  const double r = keep_a ? out[i].w : (a ? a[i] : 1.0);
+  out[i] = make_double4 (pos[3 * i], pos[3 * i + 1], pos[3 * i + 2], r);
+}
+
+__global__ void k_img_to_f32 (const double4 *__restrict__ in, int n, float4 *__restrict__ out)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double4 v = in[i];
+  out[i] = make_float4 (float (v.x), float (v.y), float (v.z), float (v.w));
+}
+
+static DevParams dev_params (const shk_params &p)
+{
+  DevParams q;
+  q.np = p.np; q.nr = p.nr; q.nk = p.nk; q.poly = p.poly;
+  q.xi = p.xi; q.rmax2 = p.rmax2; q.rmin = p.rmin;
+  q.self_a = p.self_a; q.self_c = p.self_c; q.self_m = p.self_m;
+  q.xiaspi = p.xiaspi; q.xia2 = p.xia2;
+  return q;
+}
+
+static int upload_pos (shk_dev *d, const double *pos, const double *a, int on_device,
+                       int keep_a)
+{
+  const int np = d->prm.np;
+  int rc = ensure (d, d->pos, sizeof (double4) * np);
+  if (rc) return rc;
+  // staging for AoS positions and radii
+  rc = ensure (d, d->zt, sizeof (double) * 4 * (size_t) np);
+  if (rc) return rc;
+  double *st = (double *) d->zt.p;
+  if (on_device)
+    HIPCHK (hipMemcpyAsync (st, pos, sizeof (double) * 3 * np, hipMemcpyDeviceToDevice, d->stream));
+  else
+    HIPCHK (hipMemcpyAsync (st, pos, sizeof (double) * 3 * np, hipMemcpyHostToDevice, d->stream));
+  double *ad = nullptr;
+  if (a)
+    {
+      ad = st + 3 * (size_t) np;
+      HIPCHK (hipMemcpyAsync (ad, a, sizeof (double) * np, hipMemcpyHostToDevice, d->stream));
+    }
+  hipLaunchKernelGGL (k_pack_pos, dim3 ((np + 255) / 256), dim3 (256), 0, d->stream,
+                      st, ad, np, keep_a, (double4 *) d->pos.p);
+  HIPCHK (hipGetLastError ());
+  // the staging buffer is reused by later calls on the same stream: order is kept
+  return SHK_OK;
+}
+
+extern "C" int shk_dev_set_system (shk_dev *d, const shk_params *p, const double *pos,
+                                   const double *a, const double *img, const double *kv,
+                                   const double *ks)
+{
+  if (!d || !p || p->np <= 0 || p->version < 0 || p->version > 2)
+    {
+      set_err ("shk_dev_set_system: invalid arguments");
+      return SHK_EARG;
+    }
+  d->prm = *p;
+  int rc = upload_pos (d, pos, a, 0, 0);
+  if (rc) return rc;
+  if (p->periodic)
+    {
+      if ((rc = ensure (d, d->img, sizeof (double4) * (p->nr > 0 ? p->nr : 1)))) return rc;
+      if ((rc = ensure (d, d->imgf, sizeof (float4) * (p->nr > 0 ? p->nr : 1)))) return rc;
+      if (p->nr > 0)
+        {
+          HIPCHK (hipMemcpyAsync (d->img.p, img, sizeof (double4) * p->nr,
+                                  hipMemcpyHostToDevice, d->stream));
+          hipLaunchKernelGGL (k_img_to_f32, dim3 ((p->nr + 255) / 256), dim3 (256), 0,
+                              d->stream, (const double4 *) d->img.p, p->nr, (float4 *) d->imgf.p);
+          HIPCHK (hipGetLastError ());
+        }
+      const int nk = p->nk > 0 ? p->nk : 1;
+      if ((rc = ensure (d, d->kv, sizeof (double4) * nk))) return rc;
+      if ((rc = ensure (d, d->ks, sizeof (double) * 8 * nk))) return rc;
+      if (p->nk > 0)
+        {
+          HIPCHK (hipMemcpyAsync (d->kv.p, kv, sizeof (double4) * p->nk,
+                                  hipMemcpyHostToDevice, d->stream));
+          HIPCHK (hipMemcpyAsync (d->ks.p, ks, sizeof (double) * 8 * p->nk,
+                                  hipMemcpyHostToDevice, d->stream));
+        }
+    }
+  // the host arrays may be freed by the caller right after return
+  HIPCHK (hipStreamSynchronize (d->stream));
+  d->have_sys = 1;
+  return SHK_OK;
+}
+
+extern "C" int shk_dev_set_pos (shk_dev *d, const double *pos, int on_device)
+{
+  if (!d || !d->have_sys) { set_err ("shk_dev_set_pos: no system"); return SHK_EARG; }
+  int rc = upload_pos (d, pos, nullptr, on_device, 1);
+  if (rc) return rc;
+  if (!on_device) HIPCHK (hipStreamSynchronize (d->stream));
+  return SHK_OK;
+}
+
+static int dimv (int v) { return v == 0 ? 3 : (v == 1 ? 6 : 11); }
+
+// j-chunking of the pair kernel: enough workgroups to fill 256 CUs
+static void pair_grid (int np, int &nit, int &nchunk, int &jchunk)
+{
+  nit = (np + 15) / 16;
+  int njt = (np + 15) / 16;
+  nchunk = (2048 + nit - 1) / nit;
+  if (nchunk > njt) nchunk = njt;
+  if (nchunk < 1) nchunk = 1;
+  int jt_per = (njt + nchunk - 1) / nchunk;
+  jchunk = jt_per * 16;
+  nchunk = (np + jchunk - 1) / jchunk;
+}
+
+template <int V, typename T, int KIND>
+static void launch_pairs (shk_dev *d, const DevParams &q, const double *x, int nit,
+                          int nchunk, int jchunk)
+{
+  const void *imgp = std::is_same<T, double>::value ? d->img.p : d->imgf.p;
+  hipLaunchKernelGGL ((k_pairs<V, T, KIND>), dim3 (nit, nchunk), dim3 (256), 0, d->stream,
+                      (const double4 *) d->pos.p, x,
+                      (const typename Img<T>::type *) imgp, q, jchunk,
+                      (double *) d->partial.p);
+}
+
+template <int V>
+static int atimes_v (shk_dev *d, const double *x, double *y, int prec)
+{
+  constexpr int D = Dim<V>::D;
+  const shk_params &p = d->prm;
+  DevParams q = dev_params (p);
+  const int periodic = p.periodic;
+  if (!periodic) { q.nk = 0; q.nr = 0; }
+  int nit, nchunk, jchunk;
+  pair_grid (p.np, nit, nchunk, jchunk);
+  int rc = ensure (d, d->partial, sizeof (double) * (size_t) nchunk * p.np * D);
+  if (rc) return rc;
+  if (periodic && q.nk > 0)
+    if ((rc = ensure (d, d->pq, sizeof (double) * (size_t) q.nk * 2 * D))) return rc;
+  if (d->timing) HIPCHK (hipEventRecord (d->ev[0], d->stream));
+  if (periodic)
+    {
+      if (prec == SHK_MIXED) launch_pairs<V, float, 0> (d, q, x, nit, nchunk, jchunk);
+      else launch_pairs<V, double, 0> (d, q, x, nit, nchunk, jchunk);
+    }
+  else
+    {
+      if (prec == SHK_MIXED) launch_pairs<V, float, 1> (d, q, x, nit, nchunk, jchunk);
+      else launch_pairs<V, double, 1> (d, q, x, nit, nchunk, jchunk);
+    }
+  HIPCHK (hipGetLastError ());
+  if (d->timing) HIPCHK (hipEventRecord (d->ev[1], d->stream));
+  if (periodic && q.nk > 0)
+    {
+      if (prec == SHK_MIXED)
+        hipLaunchKernelGGL ((k_recip_fwd<V, float>), dim3 (q.nk), dim3 (256), 0, d->stream,
+                            (const double4 *) d->pos.p, x, (const double4 *) d->kv.p,
+                            (const double *) d->ks.p, p.np, (double *) d->pq.p);
+      else
+        hipLaunchKernelGGL ((k_recip_fwd<V, double>), dim3 (q.nk), dim3 (256), 0, d->stream,
+                            (const double4 *) d->pos.p, x, (const double4 *) d->kv.p,
+                            (const double *) d->ks.p, p.np, (double *) d->pq.p);
+      HIPCHK (hipGetLastError ());
+    }
+  if (prec == SHK_MIXED)
+    hipLaunchKernelGGL ((k_finalize<V, float>), dim3 ((p.np + 127) / 128), dim3 (128), 0,
+                        d->stream, (const double4 *) d->pos.p, x, (const double4 *) d->kv.p,
+                        (const double *) d->pq.p, (const double *) d->partial.p, nchunk, q,
+                        periodic, y);
+  else
+    hipLaunchKernelGGL ((k_finalize<V, double>), dim3 ((p.np + 127) / 128), dim3 (128), 0,
+                        d->stream, (const double4 *) d->pos.p, x, (const double4 *) d->kv.p,
+                        (const double *) d->pq.p, (const double *) d->partial.p, nchunk, q,
+                        periodic, y);
+  HIPCHK (hipGetLastError ());
+  if (d->timing)
+    {
+      HIPCHK (hipEventRecord (d->ev[2], d->stream));
+      HIPCHK (hipEventSynchronize (d->ev[2]));
+      float a = 0, b = 0;
+      HIPCHK (hipEventElapsedTime (&a, d->ev[0], d->ev[1]));
+      HIPCHK (hipEventElapsedTime (&b, d->ev[1], d->ev[2]));
+      d->last_ms[0] = a;
+      d->last_ms[1] = b;
+      d->last_ms[2] = a + b;
+    }
+  return SHK_OK;
+}
+
+extern "C" int shk_dev_atimes (shk_dev *d, const double *x, double *y, int nrhs, int prec)
+{
+  if (!d || !d->have_sys) { set_err ("shk_dev_atimes: no system"); return SHK_EARG; }
+  if (nrhs < 1) { set_err ("shk_dev_atimes: nrhs < 1"); return SHK_EARG; }
+  const shk_params &p = d->prm;
+  if (p.periodic && p.poly)
+    {
+      set_err ("polydisperse Ewald matvec: reciprocal part not implemented on the device");
+      return SHK_EUNSUP;
+    }
+  const size_t n = (size_t) p.np * dimv (p.version);
+  for (int c = 0; c < nrhs; c++)
+    {
+      int rc;
+      if (p.version == 0) rc = atimes_v<0> (d, x + c * n, y + c * n, prec);
+      else if (p.version == 1) rc = atimes_v<1> (d, x + c * n, y + c * n, prec);
+      else rc = atimes_v<2> (d, x + c * n, y + c * n, prec);
+      if (rc) return rc;
+    }
+  return SHK_OK;
+}
+
+template <int V>
+static int matrix_v (shk_dev *d, double *mat)
+{
+  const shk_params &p = d->prm;
+  DevParams q = dev_params (p);
+  dim3 grid ((p.np + 63) / 64, p.np);
+  if (d->timing) HIPCHK (hipEventRecord (d->ev[0], d->stream));
+  if (p.periodic)
+    hipLaunchKernelGGL ((k_matrix<V, 0>), grid, dim3 (64), 0, d->stream,
+                        (const double4 *) d->pos.p, (const double4 *) d->img.p,
+                        (const double4 *) d->kv.p, (const double *) d->ks.p, q, mat);
+  else
+    hipLaunchKernelGGL ((k_matrix<V, 1>), grid, dim3 (64), 0, d->stream,
+                        (const double4 *) d->pos.p, (const double4 *) d->img.p,
+                        (const double4 *) d->kv.p, (const double *) d->ks.p, q, mat);
+  HIPCHK (hipGetLastError ());
+  if (d->timing)
+    {
+      HIPCHK (hipEventRecord (d->ev[1], d->stream));
+      HIPCHK (hipEventSynchronize (d->ev[1]));
+      float a = 0;
+      HIPCHK (hipEventElapsedTime (&a, d->ev[0], d->ev[1]));
+      d->last_ms[0] = a; d->last_ms[1] = 0; d->last_ms[2] = a;
+    }
+  return SHK_OK;
+}
+
+extern "C" int shk_dev_matrix (shk_dev *d, double *mat)
+{
+  if (!d || !d->have_sys) { set_err ("shk_dev_matrix: no system"); return SHK_EARG; }
+  if (d->prm.version == 0) return matrix_v<0> (d, mat);
+  if (d->prm.version == 1) return matrix_v<1> (d, mat);
+  return matrix_v<2> (d, mat);
+}
+
+extern "C" int shk_dev_extmat (shk_dev *d, const double *mat, const double *x, double *y,
+                               int nrhs)
+{
+  if (!d || !d->have_sys) { set_err ("shk_dev_extmat: no system"); return SHK_EARG; }
+  const shk_params &p = d->prm;
+  const int D = dimv (p.version);
+  const size_t n = (size_t) p.np * D;
+  int rc = ensure (d, d->zt, sizeof (double) * (n > 4 * (size_t) p.np ? n : 4 * (size_t) p.np));
+  if (rc) return rc;
+  for (int c = 0; c < nrhs; c++)
+    {
+      const double *xc = x + c * n;
+      double *zc = (double *) d->zt.p;
+      if (p.version == 2)
+        hipLaunchKernelGGL (k_transform<2>, dim3 ((p.np + 255) / 256), dim3 (256), 0, d->stream,
+                            xc, p.np, zc);
+      else
+        HIPCHK (hipMemcpyAsync (zc, xc, sizeof (double) * n, hipMemcpyDeviceToDevice, d->stream));
+      hipLaunchKernelGGL (k_gemv_rows, dim3 ((unsigned) ((n + 3) / 4)), dim3 (256), 0, d->stream,
+                          mat, zc, n, y + c * n);
+      HIPCHK (hipGetLastError ());
+    }
+  return SHK_OK;
+}
+
+__global__ void k_axpby4 (double *__restrict__ b2, const double *__restrict__ x,
+                          const double *__restrict__ b1, const double *__restrict__ b0,
+                          const double *__restrict__ y, double ca, double cb, double cy,
+                          size_t len)
+{
+  const size_t i = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= len) return;
+  b2[i] = ca * x[i] + cb * b1[i] - b0[i] + cy * y[i];
+}
+
+extern "C" int shk_dev_axpby4 (shk_dev *d, double *b2, const double *x, const double *b1,
+                               const double *b0, const double *y, double ca, double cb,
+                               double cy, size_t len)
+{
+  hipLaunchKernelGGL (k_axpby4, dim3 ((unsigned) ((len + 255) / 256)), dim3 (256), 0,
+                      d->stream, b2, x, b1, b0, y, ca, cb, cy, len);
+  HIPCHK (hipGetLastError ());
+  return SHK_OK;
+}
+
+__global__ void k_coldot (const double *__restrict__ u, const double *__restrict__ v, int n,
+                          double *__restrict__ out)
+{
+  __shared__ double red[256];
+  const int c = blockIdx.x;
+  double acc = 0.0;
+  for (int k = threadIdx.x; k < n; k += blockDim.x)
+    acc += u[(size_t) c * n + k] * v[(size_t) c * n + k];
+  red[threadIdx.x] = acc;
+  __syncthreads ();
+  for (int st = blockDim.x / 2; st > 0; st >>= 1)
+    {
+      if (threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+      __syncthreads ();
+    }
+  if (threadIdx.x == 0) out[c] = red[0];
+}
+
+extern "C" int shk_dev_coldot (shk_dev *d, const double *u, const double *v, int n, int m,
+                               double *out_host)
+{
+  int rc = ensure (d, d->ys, sizeof (double) * (m > 0 ? m : 1));
+  if (rc) return rc;
+  hipLaunchKernelGGL (k_coldot, dim3 (m), dim3 (256), 0, d->stream, u, v, n, (double *) d->ys.p);
+  HIPCHK (hipGetLastError ());
+  HIPCHK (hipMemcpyAsync (out_host, d->ys.p, sizeof (double) * m, hipMemcpyDeviceToHost,
+                          d->stream));
+  HIPCHK (hipStreamSynchronize (d->stream));
+  return SHK_OK;
+}
+
+// y[row] = sum_col mat[row][col] * x[col] for a general (n1 x n2) matrix
+__global__ __launch_bounds__ (256) void
+k_gemv_general (const double *__restrict__ mat, const double *__restrict__ x, int n1, int n2,
+                double *__restrict__ y)
+{
+  const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (row >= n1) return;
+  const double *mr = mat + (size_t) row * n2;
+  double acc = 0.0;
+  for (int c = lane; c < n2; c += 64) acc += mr[c] * x[c];
+  for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor (acc, off, 64);
+  if (lane == 0) y[row] = acc;
+}
+
+extern "C" int shk_dev_gemv (shk_dev *d, const double *mat, int n1, int n2, const double *x,
+                             int extracted, double *y)
+{
+  if (n1 <= 0 || n2 <= 0) { set_err ("shk_dev_gemv: empty matrix"); return SHK_EARG; }
+  const double *xx = x;
+  if (extracted)
+    {
+      if (n1 != n2 || n2 % 11) { set_err ("shk_dev_gemv: extracted form needs 11np"); return SHK_EARG; }
+      int rc = ensure (d, d->zt, sizeof (double) * (size_t) n2);
+      if (rc) return rc;
+      hipLaunchKernelGGL (k_transform<2>, dim3 ((n2 / 11 + 255) / 256), dim3 (256), 0, d->stream,
+                          x, n2 / 11, (double *) d->zt.p);
+      HIPCHK (hipGetLastError ());
+      xx = (const double *) d->zt.p;
+    }
+  hipLaunchKernelGGL (k_gemv_general, dim3 ((n1 + 3) / 4), dim3 (256), 0, d->stream, mat, xx,
+                      n1, n2, y);
+  HIPCHK (hipGetLastError ());
+  return SHK_OK;
+}
+
+__global__ void k_scal (double *__restrict__ dst, const double *__restrict__ src, double alpha,
+                        size_t len)
+{
+  const size_t i = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= len) return;
+  dst[i] = src ? alpha * src[i] : 0.0;
+}
+
+extern "C" int shk_dev_scal (shk_dev *d, double *dst, const double *src, double alpha,
+                             size_t len)
+{
+  if (len == 0) return SHK_OK;
+  hipLaunchKernelGGL (k_scal, dim3 ((unsigned) ((len + 255) / 256)), dim3 (256), 0, d->stream,
+                      dst, src, alpha, len);
+  HIPCHK (hipGetLastError ());
+  return SHK_OK;
+}

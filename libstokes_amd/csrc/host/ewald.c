@@ -1,0 +1,654 @@
+/* ewald.c -- the mobility operator entry points (host side, C).
+ *
+ * atimes_3all and friends keep the reference's prototypes (ewald.h) and
+ * hand the work to the HIP layer: positions, image table and reciprocal
+ * vectors go to the device, the operator runs there, and the result comes
+ * back.  One device context is kept per struct stokes; the parameters are
+ * re-uploaded when they change and the positions on every call, because
+ * callers move sys->pos in place between calls (brownian.c:2392, 2466).
+ */
+#include <math.h>
+#include <pthread.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include "stokes.h"
+#include "ewald.h"
+#include "non-ewald.h"
+#include "matrix.h"
+#include "stokes_hip.h"
+#include "stokes_internal.h"
+
+#ifndef M_PI
+#define M_PI 3.14159265358979323846
+#endif
+
+/* ------------------------------------------------------------------ */
+/* errors                                                               */
+/* ------------------------------------------------------------------ */
+
+static __thread char g_msg[512];
+
+void
+stokes_hip_set_error (const char *fmt, ...)
+{
+  va_list ap;
+  va_start (ap, fmt);
+  vsnprintf (g_msg, sizeof (g_msg), fmt, ap);
+  va_end (ap);
+}
+
+const char *
+stokes_hip_last_error (void)
+{
+  return g_msg;
+}
+
+static int
+dev_fail (int rc, const char *where)
+{
+  stokes_hip_set_error ("%s: %s", where, shk_last_error ());
+  return rc;
+}
+
+/* the reference's void entry points abort on failure (memory-check.h) */
+static void
+die_on (int rc, const char *where)
+{
+  if (rc == 0) return;
+  fprintf (stderr, "libstokes-hip: %s failed (%d): %s\n", where, rc,
+           stokes_hip_last_error ());
+  exit (1);
+}
+
+int
+stokes_dim (int version)
+{
+  return version == 0 ? 3 : (version == 1 ? 6 : 11);
+}
+
+/* ------------------------------------------------------------------ */
+/* contexts                                                             */
+/* ------------------------------------------------------------------ */
+
+static pthread_mutex_t g_lock = PTHREAD_MUTEX_INITIALIZER;
+static stokes_ctx **g_ctx = NULL;
+static int g_nctx = 0, g_cap = 0;
+static shk_dev *g_free_dev = NULL;
+
+stokes_ctx *
+stokes_hip_ctx (struct stokes *sys)
+{
+  stokes_ctx *c = NULL;
+  pthread_mutex_lock (&g_lock);
+  for (int i = 0; i < g_nctx; i++)
+    if (g_ctx[i]->sys == sys) { c = g_ctx[i]; break; }
+  if (c == NULL)
+    {
+      shk_dev *d = shk_dev_create ();
+      if (d == NULL)
+        stokes_hip_set_error ("no GPU context: %s", shk_last_error ());
+      else
+        {
+          c = (stokes_ctx *) calloc (1, sizeof (stokes_ctx));
+          c->sys = sys;
+          c->dev = d;
+          c->prec = SHK_FP64;
+          if (g_nctx == g_cap)
+            {
+              g_cap = g_cap ? 2 * g_cap : 16;
+              g_ctx = realloc (g_ctx, sizeof (stokes_ctx *) * g_cap);
+            }
+          g_ctx[g_nctx++] = c;
+        }
+    }
+  pthread_mutex_unlock (&g_lock);
+  return c;
+}
+
+static void
+ctx_free (stokes_ctx *c)
+{
+  if (c->x_dev) shk_dev_free (c->dev, c->x_dev);
+  if (c->y_dev) shk_dev_free (c->dev, c->y_dev);
+  if (c->mat_dev) shk_dev_free (c->dev, c->mat_dev);
+  if (c->cheb_dev) shk_dev_free (c->dev, c->cheb_dev);
+  shk_dev_destroy (c->dev);
+  free (c);
+}
+
+void
+stokes_hip_release_context (struct stokes *sys)
+{
+  pthread_mutex_lock (&g_lock);
+  for (int i = 0; i < g_nctx; i++)
+    if (g_ctx[i]->sys == sys)
+      {
+        stokes_ctx *c = g_ctx[i];
+        g_ctx[i] = g_ctx[--g_nctx];
+        pthread_mutex_unlock (&g_lock);
+        ctx_free (c);
+        return;
+      }
+  pthread_mutex_unlock (&g_lock);
+}
+
+void
+stokes_hip_invalidate (struct stokes *sys)
+{
+  pthread_mutex_lock (&g_lock);
+  for (int i = 0; i < g_nctx; i++)
+    if (g_ctx[i]->sys == sys) g_ctx[i]->valid = 0;
+  pthread_mutex_unlock (&g_lock);
+}
+
+shk_dev *
+stokes_hip_global_dev (void)
+{
+  pthread_mutex_lock (&g_lock);
+  if (g_free_dev == NULL)
+    {
+      g_free_dev = shk_dev_create ();
+      if (g_free_dev == NULL)
+        stokes_hip_set_error ("no GPU context: %s", shk_last_error ());
+    }
+  pthread_mutex_unlock (&g_lock);
+  return g_free_dev;
+}
+
+/* ------------------------------------------------------------------ */
+/* parameter upload                                                     */
+/* ------------------------------------------------------------------ */
+
+/* Reciprocal vectors of ewald.c:923-958 restricted to the half space
+ * m1 > 0 | (m1 == 0 & m2 > 0) | (m1 == m2 == 0 & m3 > 0); every k is paired
+ * with -k (identical contribution: cos is even, and the odd blocks B, G
+ * come with the odd factor sin), so the weights carry a factor 2. */
+static int
+build_kvectors (const struct stokes *s, double **kv_out, double **ks_out)
+{
+  int cap = 64, nk = 0;
+  double *kv = malloc (sizeof (double) * 4 * cap);
+  double *ks = malloc (sizeof (double) * 8 * cap);
+  double k0x = 2.0 * M_PI / s->lx;
+  double k0y = 2.0 * M_PI / s->ly;
+  double k0z = 2.0 * M_PI / s->lz;
+  double sl = s->shear_shift / s->lx;
+  for (int m1 = 0; m1 <= s->kmaxx; m1++)
+    for (int m2 = -s->kmaxy; m2 <= s->kmaxy; m2++)
+      for (int m3 = -s->kmaxz; m3 <= s->kmaxz; m3++)
+        {
+          if (m1 == 0 && (m2 < 0 || (m2 == 0 && m3 <= 0))) continue;
+          double k1 = k0x * (double) m1;
+          double k2 = k0y * (double) m2;
+          double k3 = k0z * (double) m3;
+          if (s->shear_mode == 1)
+            k2 = k0y * ((double) m2 - sl * (double) m1);
+          else if (s->shear_mode == 2)
+            k3 = k0z * ((double) m3 - sl * (double) m1);
+          double kk = k1 * k1 + k2 * k2 + k3 * k3;
+          double k = sqrt (kk);
+          if (s->kmax != 0.0 && k > s->kmax) continue;
+          double k4z = kk / 4.0 / s->xi2;
+          double kexp = s->pivol * (1.0 + k4z * (1.0 + 2.0 * k4z)) / kk
+            * exp (-k4z);
+          if (nk == cap)
+            {
+              cap *= 2;
+              kv = realloc (kv, sizeof (double) * 4 * cap);
+              ks = realloc (ks, sizeof (double) * 8 * cap);
+            }
+          double *v = kv + 4 * nk, *w = ks + 8 * nk;
+          v[0] = k1; v[1] = k2; v[2] = k3; v[3] = k;
+          w[0] = 2.0 * 6.0 * (1.0 - kk / 3.0) * kexp;              /* ya */
+          w[1] = 2.0 * 3.0 * k * kexp;                             /* yb */
+          w[2] = 2.0 * 3.0 / 2.0 * kk * kexp;                      /* yc */
+          w[3] = 2.0 * 3.0 * (1.0 - 4.0 / 15.0 * kk) * k * kexp;   /* yg */
+          w[4] = 2.0 * 3.0 / 2.0 * kk * kexp;                      /* yh */
+          w[5] = 2.0 * 3.0 * (1.0 - kk / 5.0) * kk * kexp;         /* ym */
+          w[6] = kk;
+          w[7] = kexp;
+          nk++;
+        }
+  *kv_out = kv;
+  *ks_out = ks;
+  return nk;
+}
+
+static int
+same_params (const shk_params *a, const shk_params *b)
+{
+  return memcmp (a, b, sizeof (shk_params)) == 0;
+}
+
+int
+stokes_hip_prepare (stokes_ctx *c, int positions_on_device)
+{
+  struct stokes *s = c->sys;
+  if (s->np <= 0 || s->pos == NULL)
+    {
+      stokes_hip_set_error ("system has no particles");
+      return STOKES_HIP_EARG;
+    }
+  if (s->version < 0 || s->version > 2)
+    {
+      stokes_hip_set_error ("invalid version %d", s->version);
+      return STOKES_HIP_EARG;
+    }
+  if (s->slip != NULL)
+    {
+      stokes_hip_set_error ("slip particles are not supported");
+      return STOKES_HIP_EUNSUP;
+    }
+  shk_params p;
+  memset (&p, 0, sizeof (p));
+  p.version = s->version;
+  p.np = s->np;
+  p.periodic = s->periodic ? 1 : 0;
+  p.poly = s->a != NULL;
+  p.lx = s->lx; p.ly = s->ly; p.lz = s->lz; p.pivol = s->pivol;
+  p.xi = s->xi; p.xi2 = s->xi2; p.xiaspi = s->xiaspi; p.xia2 = s->xia2;
+  p.rmax2 = s->rmax2; p.rmin = s->rmin;
+  p.self_a = s->self_a; p.self_c = s->self_c; p.self_m = s->self_m;
+  p.nr = p.periodic ? s->nr : 0;
+  int full = !c->valid || c->shear_mode != s->shear_mode
+    || c->shear_shift != s->shear_shift || p.poly
+    || c->a_ptr != s->a;
+  /* nk is only known after enumeration; compare the rest */
+  p.nk = c->prm.nk;
+  if (!full && !same_params (&p, &c->prm)) full = 1;
+  if (full)
+    {
+      double *img = NULL, *kv = NULL, *ks = NULL;
+      if (p.periodic)
+        {
+          if (p.nr <= 0 || s->rlx == NULL)
+            {
+              stokes_hip_set_error ("periodic system without image table"
+                                    " (call stokes_set_xi)");
+              return STOKES_HIP_EARG;
+            }
+          img = malloc (sizeof (double) * 4 * p.nr);
+          for (int m = 0; m < p.nr; m++)
+            {
+              double sx = 0.0;
+              if (s->shear_mode == 1) sx = (double) s->rmy[m] * s->shear_shift;
+              else if (s->shear_mode == 2) sx = (double) s->rmz[m] * s->shear_shift;
+              img[4 * m + 0] = s->rlx[m] + sx;
+              img[4 * m + 1] = s->rly[m];
+              img[4 * m + 2] = s->rlz[m];
+              img[4 * m + 3] = 0.0;
+            }
+          p.nk = build_kvectors (s, &kv, &ks);
+        }
+      else
+        p.nk = 0;
+      int rc = shk_dev_set_system (c->dev, &p, s->pos, s->a, img, kv, ks);
+      free (img); free (kv); free (ks);
+      if (rc) return dev_fail (rc, "upload");
+      c->prm = p;
+      c->shear_mode = s->shear_mode;
+      c->shear_shift = s->shear_shift;
+      c->a_ptr = s->a;
+      c->valid = 1;
+      if (positions_on_device && c->pos_dev)
+        {
+          rc = shk_dev_set_pos (c->dev, c->pos_dev, 1);
+          if (rc) return dev_fail (rc, "positions");
+        }
+      return 0;
+    }
+  int rc;
+  if (positions_on_device && c->pos_dev)
+    rc = shk_dev_set_pos (c->dev, c->pos_dev, 1);
+  else
+    rc = shk_dev_set_pos (c->dev, s->pos, 0);
+  if (rc) return dev_fail (rc, "positions");
+  return 0;
+}
+
+static int
+ensure_xy (stokes_ctx *c, size_t bytes)
+{
+  if (c->xy_bytes >= bytes) return 0;
+  if (c->x_dev) shk_dev_free (c->dev, c->x_dev);
+  if (c->y_dev) shk_dev_free (c->dev, c->y_dev);
+  c->x_dev = c->y_dev = NULL;
+  c->xy_bytes = 0;
+  int rc = shk_dev_malloc (c->dev, (void **) &c->x_dev, bytes);
+  if (!rc) rc = shk_dev_malloc (c->dev, (void **) &c->y_dev, bytes);
+  if (rc) return dev_fail (rc, "staging buffers");
+  c->xy_bytes = bytes;
+  return 0;
+}
+
+static void
+record_times (stokes_ctx *c)
+{
+  if (!c->timing) return;
+  c->sys->cpu2 = shk_dev_last_ms (c->dev, 0);
+  c->sys->cpu3 = shk_dev_last_ms (c->dev, 1);
+  c->sys->cpu1 = c->sys->cpu2 + c->sys->cpu3;
+}
+
+/* ------------------------------------------------------------------ */
+/* new entry points                                                     */
+/* ------------------------------------------------------------------ */
+
+static stokes_ctx *
+ctx_or_error (struct stokes *sys)
+{
+  if (sys == NULL)
+    {
+      stokes_hip_set_error ("NULL struct stokes");
+      return NULL;
+    }
+  return stokes_hip_ctx (sys);
+}
+
+int
+stokes_hip_set_precision (struct stokes *sys, int mode)
+{
+  stokes_ctx *c = ctx_or_error (sys);
+  if (!c) return STOKES_HIP_ENODEV;
+  if (mode != STOKES_HIP_FP64 && mode != STOKES_HIP_MIXED)
+    {
+      stokes_hip_set_error ("invalid precision mode %d", mode);
+      return STOKES_HIP_EARG;
+    }
+  c->prec = mode;
+  return 0;
+}
+
+int
+stokes_hip_set_timing (struct stokes *sys, int enable)
+{
+  stokes_ctx *c = ctx_or_error (sys);
+  if (!c) return STOKES_HIP_ENODEV;
+  c->timing = enable ? 1 : 0;
+  shk_dev_timing (c->dev, c->timing);
+  return 0;
+}
+
+int
+stokes_hip_malloc (void **p, size_t bytes)
+{
+  shk_dev *d = stokes_hip_global_dev ();
+  if (!d) return STOKES_HIP_ENODEV;
+  int rc = shk_dev_malloc (d, p, bytes);
+  return rc ? dev_fail (rc, "stokes_hip_malloc") : 0;
+}
+
+int
+stokes_hip_free (void *p)
+{
+  shk_dev *d = stokes_hip_global_dev ();
+  if (!d) return STOKES_HIP_ENODEV;
+  int rc = shk_dev_free (d, p);
+  return rc ? dev_fail (rc, "stokes_hip_free") : 0;
+}
+
+int
+stokes_hip_memcpy_h2d (void *dst, const void *src, size_t bytes)
+{
+  shk_dev *d = stokes_hip_global_dev ();
+  if (!d) return STOKES_HIP_ENODEV;
+  int rc = shk_dev_h2d (d, dst, src, bytes);
+  if (!rc) rc = shk_dev_sync (d);
+  return rc ? dev_fail (rc, "stokes_hip_memcpy_h2d") : 0;
+}
+
+int
+stokes_hip_memcpy_d2h (void *dst, const void *src, size_t bytes)
+{
+  shk_dev *d = stokes_hip_global_dev ();
+  if (!d) return STOKES_HIP_ENODEV;
+  int rc = shk_dev_d2h (d, dst, src, bytes);
+  return rc ? dev_fail (rc, "stokes_hip_memcpy_d2h") : 0;
+}
+
+int
+stokes_hip_sync (struct stokes *sys)
+{
+  stokes_ctx *c = ctx_or_error (sys);
+  if (!c) return STOKES_HIP_ENODEV;
+  int rc = shk_dev_sync (c->dev);
+  return rc ? dev_fail (rc, "stokes_hip_sync") : 0;
+}
+
+int
+stokes_hip_set_pos_dev (struct stokes *sys, const double *pos_dev)
+{
+  stokes_ctx *c = ctx_or_error (sys);
+  if (!c) return STOKES_HIP_ENODEV;
+  c->pos_dev = pos_dev;
+  return 0;
+}
+
+int
+stokes_hip_use_host_pos (struct stokes *sys)
+{
+  stokes_ctx *c = ctx_or_error (sys);
+  if (!c) return STOKES_HIP_ENODEV;
+  c->pos_dev = NULL;
+  return 0;
+}
+
+int
+stokes_hip_atimes_dev (struct stokes *sys, int nrhs, const double *x_dev,
+                       double *y_dev)
+{
+  stokes_ctx *c = ctx_or_error (sys);
+  if (!c) return STOKES_HIP_ENODEV;
+  int rc = stokes_hip_prepare (c, c->pos_dev != NULL);
+  if (rc) return rc;
+  rc = shk_dev_atimes (c->dev, x_dev, y_dev, nrhs, c->prec);
+  if (rc) return dev_fail (rc, "atimes");
+  record_times (c);
+  return 0;
+}
+
+int
+stokes_hip_atimes (struct stokes *sys, int n, const double *x, double *y)
+{
+  stokes_ctx *c = ctx_or_error (sys);
+  if (!c) return STOKES_HIP_ENODEV;
+  int nn = sys->np * stokes_dim (sys->version);
+  if (n != nn)
+    {
+      stokes_hip_set_error ("n = %d but np * %d = %d", n,
+                            stokes_dim (sys->version), nn);
+      return STOKES_HIP_EARG;
+    }
+  int rc = stokes_hip_prepare (c, c->pos_dev != NULL);
+  if (rc) return rc;
+  size_t bytes = sizeof (double) * (size_t) n;
+  if ((rc = ensure_xy (c, bytes))) return rc;
+  rc = shk_dev_h2d (c->dev, c->x_dev, x, bytes);
+  if (!rc) rc = shk_dev_atimes (c->dev, c->x_dev, c->y_dev, 1, c->prec);
+  if (!rc) rc = shk_dev_d2h (c->dev, y, c->y_dev, bytes);
+  if (rc) return dev_fail (rc, "atimes");
+  record_times (c);
+  return 0;
+}
+
+static int
+ensure_mat (stokes_ctx *c, size_t bytes)
+{
+  if (c->mat_bytes >= bytes) return 0;
+  if (c->mat_dev) shk_dev_free (c->dev, c->mat_dev);
+  c->mat_dev = NULL;
+  c->mat_bytes = 0;
+  int rc = shk_dev_malloc (c->dev, (void **) &c->mat_dev, bytes);
+  if (rc) return dev_fail (rc, "matrix buffer");
+  c->mat_bytes = bytes;
+  return 0;
+}
+
+int
+stokes_hip_make_matrix_dev (struct stokes *sys, double *mat_dev)
+{
+  stokes_ctx *c = ctx_or_error (sys);
+  if (!c) return STOKES_HIP_ENODEV;
+  int rc = stokes_hip_prepare (c, c->pos_dev != NULL);
+  if (rc) return rc;
+  rc = shk_dev_matrix (c->dev, mat_dev);
+  if (rc) return dev_fail (rc, "make_matrix");
+  record_times (c);
+  return 0;
+}
+
+int
+stokes_hip_extmat_dev (struct stokes *sys, const double *mat_dev, int nrhs,
+                       const double *x_dev, double *y_dev)
+{
+  stokes_ctx *c = ctx_or_error (sys);
+  if (!c) return STOKES_HIP_ENODEV;
+  int rc = stokes_hip_prepare (c, c->pos_dev != NULL);
+  if (rc) return rc;
+  rc = shk_dev_extmat (c->dev, mat_dev, x_dev, y_dev, nrhs);
+  return rc ? dev_fail (rc, "extmat") : 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* reference API                                                        */
+/* ------------------------------------------------------------------ */
+
+void
+atimes_ewald_3all (int n, const double *x, double *y, void *user_data)
+{
+  struct stokes *sys = (struct stokes *) user_data;
+  if (!sys->periodic)
+    {
+      stokes_hip_set_error ("atimes_ewald_3all on a non-periodic system");
+      die_on (STOKES_HIP_EARG, "atimes_ewald_3all");
+    }
+  die_on (stokes_hip_atimes (sys, n, x, y), "atimes_ewald_3all");
+}
+
+void
+atimes_nonewald_3all (int n, const double *x, double *y, void *user_data)
+{
+  struct stokes *sys = (struct stokes *) user_data;
+  if (sys->periodic)
+    {
+      stokes_hip_set_error ("atimes_nonewald_3all on a periodic system");
+      die_on (STOKES_HIP_EARG, "atimes_nonewald_3all");
+    }
+  die_on (stokes_hip_atimes (sys, n, x, y), "atimes_nonewald_3all");
+}
+
+void
+atimes_3all (int n, const double *x, double *y, void *user_data)
+{
+  die_on (stokes_hip_atimes ((struct stokes *) user_data, n, x, y),
+          "atimes_3all");
+}
+
+int
+stokes_hip_make_matrix (struct stokes *sys, double *mat)
+{
+  stokes_ctx *c = ctx_or_error (sys);
+  if (!c) return STOKES_HIP_ENODEV;
+  size_t n = (size_t) sys->np * stokes_dim (sys->version);
+  int rc = ensure_mat (c, sizeof (double) * n * n);
+  if (rc) return rc;
+  if ((rc = stokes_hip_make_matrix_dev (sys, c->mat_dev))) return rc;
+  rc = shk_dev_d2h (c->dev, mat, c->mat_dev, sizeof (double) * n * n);
+  return rc ? dev_fail (rc, "make_matrix") : 0;
+}
+
+void
+make_matrix_mob_3all (struct stokes *sys, double *mat)
+{
+  die_on (stokes_hip_make_matrix (sys, mat), "make_matrix_mob_3all");
+}
+
+void
+make_matrix_mob_ewald_3all (struct stokes *sys, double *mat)
+{
+  if (!sys->periodic)
+    {
+      stokes_hip_set_error ("non-periodic system");
+      die_on (STOKES_HIP_EARG, "make_matrix_mob_ewald_3all");
+    }
+  die_on (stokes_hip_make_matrix (sys, mat), "make_matrix_mob_ewald_3all");
+}
+
+void
+make_matrix_mob_nonewald_3all (struct stokes *sys, double *mat)
+{
+  if (sys->periodic)
+    {
+      stokes_hip_set_error ("periodic system");
+      die_on (STOKES_HIP_EARG, "make_matrix_mob_nonewald_3all");
+    }
+  die_on (stokes_hip_make_matrix (sys, mat), "make_matrix_mob_nonewald_3all");
+}
+
+/* ewald.c:484-505: build the matrix and multiply, all on the device */
+static int
+atimes_matrix_impl (struct stokes *sys, int n, const double *x, double *y)
+{
+  stokes_ctx *c = ctx_or_error (sys);
+  if (!c) return STOKES_HIP_ENODEV;
+  size_t nn = (size_t) sys->np * stokes_dim (sys->version);
+  if ((size_t) n != nn)
+    {
+      stokes_hip_set_error ("n = %d does not match the system", n);
+      return STOKES_HIP_EARG;
+    }
+  int rc = ensure_mat (c, sizeof (double) * nn * nn);
+  if (!rc) rc = ensure_xy (c, sizeof (double) * nn);
+  if (rc) return rc;
+  if ((rc = stokes_hip_make_matrix_dev (sys, c->mat_dev))) return rc;
+  rc = shk_dev_h2d (c->dev, c->x_dev, x, sizeof (double) * nn);
+  if (!rc) rc = shk_dev_extmat (c->dev, c->mat_dev, c->x_dev, c->y_dev, 1);
+  if (!rc) rc = shk_dev_d2h (c->dev, y, c->y_dev, sizeof (double) * nn);
+  return rc ? dev_fail (rc, "atimes_3all_matrix") : 0;
+}
+
+void
+atimes_3all_matrix (int n, const double *x, double *y, void *user_data)
+{
+  die_on (atimes_matrix_impl ((struct stokes *) user_data, n, x, y),
+          "atimes_3all_matrix");
+}
+
+/* free-standing dense products on the library's global context */
+static int
+dense_product (int extracted, const double *mat, int n1, int n2,
+               const double *x, double *y)
+{
+  shk_dev *d = stokes_hip_global_dev ();
+  if (!d) return STOKES_HIP_ENODEV;
+  size_t mb = sizeof (double) * (size_t) n1 * (size_t) n2;
+  double *md = NULL, *xd = NULL, *yd = NULL;
+  int rc = shk_dev_malloc (d, (void **) &md, mb);
+  if (!rc) rc = shk_dev_malloc (d, (void **) &xd, sizeof (double) * n2);
+  if (!rc) rc = shk_dev_malloc (d, (void **) &yd, sizeof (double) * n1);
+  if (!rc) rc = shk_dev_h2d (d, md, mat, mb);
+  if (!rc) rc = shk_dev_h2d (d, xd, x, sizeof (double) * n2);
+  if (!rc) rc = shk_dev_gemv (d, md, n1, n2, xd, extracted, yd);
+  if (!rc) rc = shk_dev_d2h (d, y, yd, sizeof (double) * n1);
+  if (md) shk_dev_free (d, md);
+  if (xd) shk_dev_free (d, xd);
+  if (yd) shk_dev_free (d, yd);
+  return rc ? dev_fail (rc, "dense product") : 0;
+}
+
+void
+dot_prod_matrix (const double *mat, int n1, int n2, const double *x,
+                 double *y)
+{
+  die_on (dense_product (0, mat, n1, n2, x, y), "dot_prod_matrix");
+}
+
+void
+multiply_extmat_with_extvec_3fts (int np, const double *m, const double *x,
+                                  double *y)
+{
+  die_on (dense_product (1, m, 11 * np, 11 * np, x, y),
+          "multiply_extmat_with_extvec_3fts");
+}

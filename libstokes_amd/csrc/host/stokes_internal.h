@@ -1,0 +1,44 @@
+/* stokes_internal.h -- host-side helpers shared by the C files of the
+ * library (not installed). */
+#ifndef LIBSTOKES_HIP_INTERNAL_H
+#define LIBSTOKES_HIP_INTERNAL_H
+
+#include "stokes.h"
+#include "../shk.h"
+
+/* per-struct-stokes device context */
+typedef struct stokes_ctx {
+  struct stokes *sys;
+  shk_dev *dev;
+  int prec;                 /* SHK_FP64 / SHK_MIXED */
+  int timing;
+  int valid;                /* parameters uploaded and current */
+  shk_params prm;           /* last uploaded parameters */
+  double shear_shift;       /* shift the k table and images were built for */
+  int shear_mode;
+  int version;
+  const double *a_ptr;      /* radii pointer seen at upload */
+  const double *pos_dev;    /* device positions given by the caller */
+  /* host-side staging for the legacy (host pointer) API */
+  double *x_dev, *y_dev;
+  size_t xy_bytes;
+  double *mat_dev;
+  size_t mat_bytes;
+  /* device-resident Chebyshev work space */
+  double *cheb_dev;
+  size_t cheb_bytes;
+} stokes_ctx;
+
+/* find or create the context of sys; NULL and a message on failure */
+stokes_ctx *stokes_hip_ctx (struct stokes *sys);
+/* upload the parameters / positions of sys for the next operator call */
+int stokes_hip_prepare (stokes_ctx *c, int positions_on_device);
+void stokes_hip_release_context (struct stokes *sys);
+void stokes_hip_invalidate (struct stokes *sys);
+/* context not tied to a system (free-standing dense products) */
+shk_dev *stokes_hip_global_dev (void);
+/* record an error message for stokes_hip_last_error() */
+void stokes_hip_set_error (const char *fmt, ...);
+int stokes_dim (int version);
+
+#endif

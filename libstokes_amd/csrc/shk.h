@@ -1,0 +1,99 @@
+/* shk.h -- internal C ABI between the C host layer (csrc/host/ C files) and
+ * the HIP layer (csrc/hip/ HIP files).  Plain C types only; every HIP-side entry
+ * returns 0 on success or a negative SHK_E* code.
+ */
+#ifndef SHK_H
+#define SHK_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  SHK_OK = 0,
+  SHK_EHIP = -1,      /* HIP runtime error (message via shk_last_error) */
+  SHK_EARG = -2,      /* invalid argument / size */
+  SHK_ENOMEM = -3,    /* device allocation failed */
+  SHK_EUNSUP = -4,    /* configuration not supported by the device path */
+  SHK_ENODEV = -5     /* no usable GPU */
+};
+
+enum { SHK_FP64 = 0, SHK_MIXED = 1 };
+
+/* Ewald / RPY parameters of one system, host-prepared.  The field meanings
+ * follow struct stokes (stokes.h:27-154). */
+typedef struct shk_params {
+  int version;          /* 0 F, 1 FT, 2 FTS */
+  int np;
+  int periodic;         /* 0 open-BC RPY, 1 Ewald */
+  int poly;             /* radii present */
+  double lx, ly, lz, pivol;
+  double xi, xi2, xiaspi, xia2;
+  double rmax2, rmin;
+  double self_a, self_c, self_m;
+  int nr;               /* real-space images */
+  int nk;               /* reciprocal half-space vectors (k and -k paired) */
+} shk_params;
+
+typedef struct shk_dev shk_dev;
+
+const char *shk_last_error (void);
+
+shk_dev *shk_dev_create (void);
+void     shk_dev_destroy (shk_dev *d);
+
+/* Upload positions [3np] (AoS), radii [np] or NULL, image table
+ * img[nr*4] = (x + shear fold, y, z, 0), k table kv[nk*4] = (k1,k2,k3,|k|)
+ * and ks[nk*8] = 2*(ya, yb, yc, yg, yh, ym) at unit radius, kk, kexp
+ * (the factor 2 accounts for the paired -k). */
+int shk_dev_set_system (shk_dev *d, const shk_params *p, const double *pos,
+                        const double *a, const double *img, const double *kv,
+                        const double *ks);
+/* Replace only the positions (same np). */
+int shk_dev_set_pos (shk_dev *d, const double *pos, int on_device);
+
+/* y = M x for nrhs right-hand sides (column-major n x nrhs), device
+ * pointers.  prec: SHK_FP64 or SHK_MIXED. */
+int shk_dev_atimes (shk_dev *d, const double *x, double *y, int nrhs, int prec);
+/* dense (n x n) row-major extracted-form matrix, device pointer */
+int shk_dev_matrix (shk_dev *d, double *mat);
+/* y = mat . T(x) (FTS, extracted form) or mat . x (F/FT), device pointers */
+int shk_dev_extmat (shk_dev *d, const double *mat, const double *x, double *y,
+                    int nrhs);
+
+/* y[n1] = mat[n1 x n2] . x (extracted != 0: FTS S transform of x first,
+ * n1 = n2 = 11 np), device pointers */
+int shk_dev_gemv (shk_dev *d, const double *mat, int n1, int n2,
+                  const double *x, int extracted, double *y);
+
+/* device memory and copies on the context stream */
+int  shk_dev_malloc (shk_dev *d, void **p, size_t bytes);
+int  shk_dev_free (shk_dev *d, void *p);
+int  shk_dev_h2d (shk_dev *d, void *dst, const void *src, size_t bytes);
+int  shk_dev_d2h (shk_dev *d, void *dst, const void *src, size_t bytes);
+int  shk_dev_d2d (shk_dev *d, void *dst, const void *src, size_t bytes);
+int  shk_dev_sync (shk_dev *d);
+void *shk_dev_stream (shk_dev *d);
+
+/* per-part device timers of the last operator call (ms): 0 real space,
+ * 1 reciprocal space, 2 whole call.  enable != 0 records events. */
+void   shk_dev_timing (shk_dev *d, int enable);
+double shk_dev_last_ms (shk_dev *d, int which);
+
+/* Chebyshev block helpers (device): b2 = ca*x + cb*b1 - b0 + cy*y, all n*m */
+int shk_dev_axpby4 (shk_dev *d, double *b2, const double *x, const double *b1,
+                    const double *b0, const double *y, double ca, double cb,
+                    double cy, size_t len);
+/* dst = alpha * src (src == NULL: dst = 0), len doubles */
+int shk_dev_scal (shk_dev *d, double *dst, const double *src, double alpha,
+                  size_t len);
+/* out[c] = sum_k u[k + c*n] * v[k + c*n] for c < m (column dots) */
+int shk_dev_coldot (shk_dev *d, const double *u, const double *v, int n, int m,
+                    double *out_host);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

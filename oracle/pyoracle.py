@@ -1,0 +1,297 @@
+"""ctypes bindings to the test oracle (oracle/liboracle.so) and to the
+compiled reference hot path (oracle/_ref/libref.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, as the checker.  The product (libstokes_amd)
+never imports this module.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_SO = os.path.join(_HERE, "liboracle.so")
+REF_SO = os.path.join(_HERE, "_ref", "libref.so")
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+
+
+class OrcSys(C.Structure):
+    """Mirror of orc_sys (oracle.h)."""
+    _fields_ = [
+        ("version", C.c_int), ("np", C.c_int),
+        ("pos", _dp), ("a", _dp), ("rmin", C.c_double), ("periodic", C.c_int),
+        ("lx", C.c_double), ("ly", C.c_double), ("lz", C.c_double),
+        ("pivol", C.c_double),
+        ("xi", C.c_double), ("xi2", C.c_double), ("xiaspi", C.c_double),
+        ("xia2", C.c_double), ("ewald_eps", C.c_double),
+        ("self_a", C.c_double), ("self_c", C.c_double), ("self_m", C.c_double),
+        ("rmax2", C.c_double), ("rmaxx", C.c_int), ("rmaxy", C.c_int),
+        ("rmaxz", C.c_int),
+        ("kmax", C.c_double), ("kmaxx", C.c_int), ("kmaxy", C.c_int),
+        ("kmaxz", C.c_int),
+        ("nr", C.c_int), ("rlx", _dp), ("rly", _dp), ("rlz", _dp),
+        ("rmx", _ip), ("rmy", _ip), ("rmz", _ip),
+        ("shear_mode", C.c_int), ("shear_shift", C.c_double),
+    ]
+
+
+def _p(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _load(path):
+    if not os.path.exists(path):
+        raise FileNotFoundError(
+            f"{path} is not built (run `make -C oracle` / `make -C oracle ref`)")
+    return C.CDLL(path)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        _lib = _load(ORACLE_SO)
+        L = _lib
+        L.orc_xi_by_tratio.restype = C.c_double
+        L.orc_ewald_real_bound.restype = C.c_double
+        L.orc_ewald_recip_bound.restype = C.c_double
+        L.orc_cheb_eval.restype = C.c_double
+        L.orc_cheb_error_minvsqrt.restype = C.c_double
+        L.orc_cheb_error_Rsqrt.restype = C.c_double
+        L.orc_func.restype = C.c_double
+        L.orc_rng_int32.restype = C.c_ulong
+        L.orc_rng_real2.restype = C.c_double
+        L.orc_rng_gaussian.restype = C.c_double
+        L.orc_xi_by_tratio.argtypes = [C.POINTER(OrcSys), C.c_double]
+        L.orc_set_xi.argtypes = [C.POINTER(OrcSys), C.c_double, C.c_double]
+        L.orc_set_l.argtypes = [C.POINTER(OrcSys), C.c_double, C.c_double,
+                                C.c_double]
+        L.orc_ewald_real_bound.argtypes = [C.c_double, C.c_double]
+        L.orc_ewald_recip_bound.argtypes = [C.c_double, C.c_double]
+    return _lib
+
+
+_ref = None
+
+
+def ref_available():
+    return os.path.exists(REF_SO)
+
+
+def ref():
+    global _ref
+    if _ref is None:
+        _ref = _load(REF_SO)
+        R = _ref
+        R.ref_sys_new.restype = C.c_void_p
+        R.ref_sys_new.argtypes = [C.POINTER(OrcSys)]
+        R.ref_sys_free.argtypes = [C.c_void_p]
+        R.ref_atimes.argtypes = [C.c_void_p, C.c_int, _dp, _dp]
+        R.ref_matrix.argtypes = [C.c_void_p, _dp]
+        R.ref_cpu_split.restype = C.c_double
+        R.ref_cpu_split.argtypes = [C.c_void_p, C.c_int]
+        R.ref_cheb_eval.restype = C.c_double
+        R.ref_cheb_error_minvsqrt_dense.restype = C.c_double
+        R.ref_rng_new.restype = C.c_void_p
+        R.ref_rng_new.argtypes = [C.c_ulong]
+        R.ref_rng_free.argtypes = [C.c_void_p]
+        R.ref_rng_int32.restype = C.c_ulong
+        R.ref_rng_int32.argtypes = [C.c_void_p]
+        R.ref_rng_gaussian.restype = C.c_double
+        R.ref_rng_gaussian.argtypes = [C.c_void_p]
+        R.ref_cheb_eval_sys.argtypes = [C.c_int, _dp, C.c_void_p, _dp, _dp,
+                                        C.c_double, C.c_double]
+    return _ref
+
+
+def dim(version):
+    return (3, 6, 11)[version]
+
+
+class System:
+    """Oracle-side system: holds numpy buffers alive for the orc_sys."""
+
+    def __init__(self, version, pos, box=None, xi=None, tr=None, eps=1e-12,
+                 a=None, rmin=0.0, shear_mode=0, shear_shift=0.0):
+        L = lib()
+        self.pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1)
+        self.np = self.pos.size // 3
+        self.a = None if a is None else np.ascontiguousarray(a, np.float64)
+        s = OrcSys()
+        L.orc_sys_init(C.byref(s))
+        s.version = version
+        s.np = self.np
+        s.pos = _p(self.pos)
+        s.a = _p(self.a) if self.a is not None else None
+        s.rmin = rmin
+        s.shear_mode = shear_mode
+        s.shear_shift = shear_shift
+        if box is not None:
+            s.periodic = 1
+            L.orc_set_l(C.byref(s), *[float(b) for b in box])
+            if xi is None:
+                xi = L.orc_xi_by_tratio(C.byref(s), float(tr if tr else 1.0))
+            L.orc_set_xi(C.byref(s), float(xi), float(eps))
+        self.s = s
+        self.version = version
+        self.n = dim(version) * self.np
+
+    def __del__(self):
+        try:
+            lib().orc_sys_free_tables(C.byref(self.s))
+        except Exception:
+            pass
+
+    def params(self):
+        s = self.s
+        return dict(xi=s.xi, rmax2=s.rmax2, rmaxx=s.rmaxx, rmaxy=s.rmaxy,
+                    rmaxz=s.rmaxz, kmax=s.kmax, kmaxx=s.kmaxx, kmaxy=s.kmaxy,
+                    kmaxz=s.kmaxz, nr=s.nr, self_a=s.self_a, self_c=s.self_c,
+                    self_m=s.self_m, pivol=s.pivol)
+
+    def image_table(self):
+        s = self.s
+        nr = s.nr
+        rl = np.array([[s.rlx[m], s.rly[m], s.rlz[m]] for m in range(nr)])
+        rm = np.array([[s.rmx[m], s.rmy[m], s.rmz[m]] for m in range(nr)],
+                      dtype=np.int32)
+        return rl, rm
+
+    def atimes(self, x, rows=None):
+        x = np.ascontiguousarray(x, np.float64)
+        y = np.zeros(self.n)
+        if rows is None:
+            lib().orc_atimes_3all(C.byref(self.s), _p(x), _p(y))
+        else:
+            lib().orc_atimes_ewald_rows(C.byref(self.s), _p(x), _p(y),
+                                        int(rows[0]), int(rows[1]))
+        return y
+
+    def matrix(self):
+        m = np.zeros((self.n, self.n))
+        if self.s.periodic:
+            lib().orc_matrix_ewald(C.byref(self.s), _p(m))
+        else:
+            lib().orc_matrix_nonewald(C.byref(self.s), _p(m))
+        return m
+
+    # ---- compiled reference on the same system --------------------------
+    def ref_atimes(self, x):
+        R = ref()
+        h = R.ref_sys_new(C.byref(self.s))
+        x = np.ascontiguousarray(x, np.float64)
+        y = np.zeros(self.n)
+        R.ref_atimes(h, self.n, _p(x), _p(y))
+        R.ref_sys_free(h)
+        return y
+
+    def ref_matrix(self):
+        R = ref()
+        h = R.ref_sys_new(C.byref(self.s))
+        m = np.zeros((self.n, self.n))
+        R.ref_matrix(h, _p(m))
+        R.ref_sys_free(h)
+        return m
+
+
+def extmat_fts(np_, m, x):
+    y = np.zeros(11 * np_)
+    m = np.ascontiguousarray(m, np.float64)
+    x = np.ascontiguousarray(x, np.float64)
+    lib().orc_extmat_fts(int(np_), _p(m), _p(x), _p(y))
+    return y
+
+
+def scalars_ewald_real(version, xi, r):
+    o = np.zeros(11)
+    lib().orc_scalars_ewald_real(C.c_int(version), C.c_double(xi),
+                                 C.c_double(r), _p(o))
+    return o
+
+
+def scalars_ewald_real_poly(version, xi, r, aa, ab, scale_sd=False):
+    o = np.zeros(11)
+    lib().orc_scalars_ewald_real_poly(C.c_int(version), C.c_double(xi),
+                                      C.c_double(r), C.c_double(aa),
+                                      C.c_double(ab), _p(o))
+    if scale_sd:
+        lib().orc_scalars_scale_SD(C.c_int(version), C.c_double(aa), _p(o))
+    return o
+
+
+def ref_scalars_ewald_real(version, xi, r, poly=None):
+    R = ref()
+    o = [C.c_double() for _ in range(11)]
+    if poly is None:
+        R.scalars_ewald_real(C.c_int(version), C.c_double(xi), C.c_double(r),
+                             *[C.byref(v) for v in o])
+    else:
+        R.scalars_ewald_real_poly(C.c_int(version), C.c_double(xi),
+                                  C.c_double(r), C.c_double(poly[0]),
+                                  C.c_double(poly[1]),
+                                  *[C.byref(v) for v in o])
+    return np.array([v.value for v in o])
+
+
+def cheb_coef(ncheb, func_id, x0, x1):
+    a = np.zeros(ncheb)
+    lib().orc_cheb_coef(C.c_int(ncheb), C.c_int(func_id), C.c_double(x0),
+                        C.c_double(x1), _p(a))
+    return a
+
+
+def cheb_eval(a, x0, x1, x):
+    return lib().orc_cheb_eval(C.c_int(len(a)), _p(np.ascontiguousarray(a)),
+                               C.c_double(x0), C.c_double(x1), C.c_double(x))
+
+
+class _Dense(C.Structure):
+    _fields_ = [("n", C.c_int), ("m", _dp)]
+
+
+def cheb_eval_dense(a, mat, y, e0, e1):
+    mat = np.ascontiguousarray(mat, np.float64)
+    y = np.ascontiguousarray(y, np.float64)
+    n = y.size
+    z = np.zeros(n)
+    d = _Dense(n, _p(mat))
+    L = lib()
+    cb = C.cast(L.orc_atimes_dense_cb, C.c_void_p)
+    L.orc_cheb_eval_atimes(C.c_int(len(a)), _p(np.ascontiguousarray(a)),
+                           C.c_int(n), _p(y), _p(z), C.c_double(e0),
+                           C.c_double(e1), cb, C.byref(d))
+    return z
+
+
+def cheb_eval_sys(a, system, y, e0, e1):
+    y = np.ascontiguousarray(y, np.float64)
+    z = np.zeros(y.size)
+    L = lib()
+    cb = C.cast(L.orc_atimes_sys_cb, C.c_void_p)
+    L.orc_cheb_eval_atimes(C.c_int(len(a)), _p(np.ascontiguousarray(a)),
+                           C.c_int(y.size), _p(y), _p(z), C.c_double(e0),
+                           C.c_double(e1), cb, C.byref(system.s))
+    return z
+
+
+class Rng(C.Structure):
+    _fields_ = [("mt", C.c_ulong * 624), ("mti", C.c_int),
+                ("has_saved", C.c_int), ("saved", C.c_double)]
+
+    def __init__(self, seed):
+        super().__init__()
+        lib().orc_rng_seed(C.byref(self), C.c_ulong(seed))
+
+    def int32(self):
+        return lib().orc_rng_int32(C.byref(self))
+
+    def gaussian(self, n=None):
+        if n is None:
+            return lib().orc_rng_gaussian(C.byref(self))
+        return np.array([lib().orc_rng_gaussian(C.byref(self))
+                         for _ in range(n)])

@@ -1,0 +1,156 @@
+/* ref_harness.c -- glue that drives the compiled reference hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Built by oracle/Makefile (target `ref`) together
+ * with the reference's own ewald.c, f.c, ft.c, fts.c, non-ewald.c, bench.c,
+ * chebyshev.c and KIrand.c, compiled where they lie under the reference tree.
+ * The reference's stokes.c needs the absent libiter header, so the harness
+ * fills struct stokes itself from an orc_sys prepared by the oracle's
+ * restatement of stokes_set_l / stokes_set_xi (oracle.c).  No reference
+ * source is copied; only its headers are included for the struct layout.
+ */
+#include <stdlib.h>
+#include <string.h>
+#include "stokes.h"      /* reference: struct stokes */
+#include "ewald.h"       /* reference: atimes_ewald_3all, ... */
+#include "non-ewald.h"
+#include "chebyshev.h"
+#include "KIrand.h"
+#include "oracle.h"
+
+struct stokes *
+ref_sys_new (const orc_sys *o)
+{
+  struct stokes *s = calloc (1, sizeof (struct stokes));
+  s->version = o->version;
+  s->np = s->nm = o->np;
+  s->pos = malloc (sizeof (double) * 3 * o->np);
+  memcpy (s->pos, o->pos, sizeof (double) * 3 * o->np);
+  if (o->a)
+    {
+      s->a = malloc (sizeof (double) * o->np);
+      memcpy (s->a, o->a, sizeof (double) * o->np);
+    }
+  s->rmin = o->rmin;
+  s->periodic = o->periodic;
+  s->shear_mode = o->shear_mode;
+  s->shear_shift = o->shear_shift;
+  s->ewald_eps = o->ewald_eps;
+  s->rmax2 = o->rmax2;
+  s->rmaxx = o->rmaxx; s->rmaxy = o->rmaxy; s->rmaxz = o->rmaxz;
+  s->kmax = o->kmax;
+  s->kmaxx = o->kmaxx; s->kmaxy = o->kmaxy; s->kmaxz = o->kmaxz;
+  s->xi = o->xi; s->xi2 = o->xi2; s->xiaspi = o->xiaspi; s->xia2 = o->xia2;
+  s->pivol = o->pivol;
+  s->lx = o->lx; s->ly = o->ly; s->lz = o->lz;
+  s->self_a = o->self_a; s->self_c = o->self_c; s->self_m = o->self_m;
+  s->nr = o->nr;
+  if (o->nr > 0)
+    {
+      size_t nd = sizeof (double) * o->nr, ni = sizeof (int) * o->nr;
+      s->rlx = malloc (nd); memcpy (s->rlx, o->rlx, nd);
+      s->rly = malloc (nd); memcpy (s->rly, o->rly, nd);
+      s->rlz = malloc (nd); memcpy (s->rlz, o->rlz, nd);
+      s->rmx = malloc (ni); memcpy (s->rmx, o->rmx, ni);
+      s->rmy = malloc (ni); memcpy (s->rmy, o->rmy, ni);
+      s->rmz = malloc (ni); memcpy (s->rmz, o->rmz, ni);
+    }
+  return s;
+}
+
+void
+ref_sys_free (struct stokes *s)
+{
+  free (s->pos); free (s->a);
+  free (s->rlx); free (s->rly); free (s->rlz);
+  free (s->rmx); free (s->rmy); free (s->rmz);
+  free (s);
+}
+
+void
+ref_atimes (struct stokes *s, int n, const double *x, double *y)
+{
+  atimes_3all (n, x, y, s);
+}
+
+void
+ref_matrix (struct stokes *s, double *mat)
+{
+  if (s->periodic) make_matrix_mob_ewald_3all (s, mat);
+  else make_matrix_mob_nonewald_3all (s, mat);
+}
+
+double
+ref_cpu_split (struct stokes *s, int which)
+{
+  return which == 1 ? s->cpu1 : (which == 2 ? s->cpu2 : s->cpu3);
+}
+
+static double ref_inv_sqrt (double x) { return 1.0 / __builtin_sqrt (x); }
+static double ref_sqrt (double x) { return __builtin_sqrt (x); }
+static double ref_cos (double x) { return __builtin_cos (x); }
+static double ref_sin (double x) { return __builtin_sin (x); }
+
+void
+ref_cheb_coef (int ncheb, int func_id, double x0, double x1, double *a)
+{
+  double (*f) (double) = func_id == 0 ? ref_inv_sqrt
+    : func_id == 1 ? ref_sqrt : func_id == 2 ? ref_cos : ref_sin;
+  chebyshev_coef (ncheb, f, x0, x1, a);
+}
+
+double
+ref_cheb_eval (int ncheb, const double *a, double x0, double x1, double x)
+{
+  return chebyshev_eval (ncheb, a, x0, x1, x);
+}
+
+typedef struct { int n; const double *m; } ref_dense;
+
+static void
+ref_dense_atimes (int n, const double *x, double *y, void *ud)
+{
+  const ref_dense *d = ud;
+  for (int i = 0; i < n; i++)
+    {
+      double acc = 0.0;
+      for (int j = 0; j < n; j++) acc += d->m[(size_t) i * n + j] * x[j];
+      y[i] = acc;
+    }
+}
+
+void
+ref_cheb_eval_dense (int ncheb, const double *a, int n, const double *m,
+                     const double *y, double *z, double e0, double e1)
+{
+  ref_dense d = {n, m};
+  chebyshev_eval_atimes (ncheb, a, n, y, z, e0, e1, ref_dense_atimes, &d);
+}
+
+double
+ref_cheb_error_minvsqrt_dense (int n, const double *m, const double *y,
+                               const double *z)
+{
+  ref_dense d = {n, m};
+  return chebyshev_error_minvsqrt (n, y, z, ref_dense_atimes, &d);
+}
+
+void
+ref_cheb_eval_sys (int ncheb, const double *a, struct stokes *s,
+                   const double *y, double *z, double e0, double e1)
+{
+  int n = s->np * (s->version == 0 ? 3 : (s->version == 1 ? 6 : 11));
+  chebyshev_eval_atimes (ncheb, a, n, y, z, e0, e1, atimes_3all, s);
+}
+
+/* KIrand */
+void *
+ref_rng_new (unsigned long seed)
+{
+  struct KIrand *r = KIrand_init ();
+  KIrand_init_genrand (r, seed);
+  return r;
+}
+
+void ref_rng_free (void *r) { KIrand_free (r); }
+unsigned long ref_rng_int32 (void *r) { return KIrand_genrand_int32 (r); }
+double ref_rng_gaussian (void *r) { return KIrand_Gaussian (r); }

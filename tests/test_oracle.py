@@ -1,0 +1,348 @@
+"""Pinning the oracle (CPU only).
+
+The oracle (oracle/oracle.c) is checked against
+  * the golden vectors written by the compiled reference (tests/golden/),
+  * the compiled reference itself where oracle/_ref is built,
+  * what the mathematics fixes: the documented parameter-selection rule
+    re-evaluated here with scipy, the Hasimoto self-mobility of a simple
+    cubic array, xi independence, the reference's own invariants
+    (atimes == extmat . x, mono == poly(a=1), a shear shift of L == no
+    shift), symmetry and positivity of the F mobility, Chebyshev fits of
+    closed-form functions and f(A)y by eigendecomposition, and the published
+    first MT19937 output.
+Each check is chosen so that a dropped term, a sign or an index error in
+the oracle fails it.
+"""
+import math
+
+import numpy as np
+import pytest
+from scipy.special import erfc
+
+import pyoracle as po
+from conftest import relerr, maxrel
+
+FIX9_L = 5.733683
+
+
+def _system(meta, data):
+    k = meta["name"]
+    a = data[f"{k}/a"] if meta["poly"] else None
+    return po.System(meta["version"], data[f"{k}/pos"], box=meta["box"],
+                     tr=meta["tr"], a=a, rmin=meta["rmin"],
+                     shear_mode=meta["shear_mode"],
+                     shear_shift=meta["shear_shift"])
+
+
+def test_oracle_matches_reference_golden(golden):
+    data, cases = golden
+    worst = 0.0
+    for meta in cases:
+        k = meta["name"]
+        s = _system(meta, data)
+        y = s.atimes(data[f"{k}/x"])
+        e = relerr(y, data[f"{k}/y"])
+        worst = max(worst, e)
+        # the N = 1 cases cancel self, real and reciprocal parts to ~1e-1,
+        # so summation order alone moves them by ~1e-13
+        assert e < 5e-13, (k, e)
+        if meta["matrix"]:
+            m = s.matrix()
+            em = maxrel(m, data[f"{k}/mat"])
+            assert em < 1e-13, (k, em)
+    assert worst < 5e-13
+
+
+def test_oracle_params_match_golden(golden):
+    data, cases = golden
+    for meta in cases:
+        if meta.get("params") is None:
+            continue
+        s = _system(meta, data)
+        assert s.params() == meta["params"], meta["name"]
+
+
+@pytest.mark.skipif(not po.ref_available(), reason="oracle/_ref not built")
+def test_oracle_matches_compiled_reference_random():
+    rng = np.random.default_rng(7)
+    for v in (0, 1, 2):
+        for trial in range(3):
+            L = rng.uniform(7.0, 11.0)
+            box = (L, L * rng.uniform(0.8, 1.2), L * rng.uniform(0.8, 1.2))
+            pos = rng.uniform(0, 1, (7, 3)) * np.array(box)
+            a = rng.uniform(0.5, 1.5, 7) if trial == 1 else None
+            sm = trial % 3
+            s = po.System(v, pos, box=box, tr=rng.uniform(0.5, 20.0), a=a,
+                          shear_mode=sm, shear_shift=0.21 * L if sm else 0.0)
+            x = rng.standard_normal(s.n)
+            assert relerr(s.atimes(x), s.ref_atimes(x)) < 1e-13
+            assert maxrel(s.matrix(), s.ref_matrix()) < 1e-13
+
+
+def _tail_real(r, xi):
+    """stokes.c:654-682 restated independently with scipy"""
+    xr = xi * r / 1.15
+    return (erfc(xr) * (0.75 + 1.5 / r ** 2) / r
+            + math.exp(-xr * xr) * xi / math.sqrt(math.pi)
+            * (4.5 + 3 * xr ** 2 + (3 + xr ** 2 * (14 + xr ** 2 * (4 + xr ** 2)))
+               / r ** 2))
+
+
+def _tail_recip(k, xi):
+    kx = k / xi / 1.15
+    return (6 * math.pi * math.exp(-kx * kx / 4) * (1 + k * k / 3) / (k * k)
+            * (1 + kx * kx * (0.25 + kx * kx / 8)))
+
+
+@pytest.mark.parametrize("box,tr,eps", [
+    ((10.0, 10.0, 10.0), 1.0, 1e-12), ((13.89, 13.89, 13.89), 1.0, 1e-12),
+    ((24.27, 24.27, 24.27), 1e3, 1e-12), ((8.0, 9.0, 10.0), 30.0, 1e-8),
+    ((FIX9_L,) * 3, 1.0, 1e-12)])
+def test_setup_rule(box, tr, eps):
+    """r_max / k_max are the last 0.99-shrinks whose tail stays below eps,
+    and the image table is exactly the cell set of the documented rule."""
+    s = po.System(0, np.zeros((1, 3)), box=box, tr=tr, eps=eps)
+    p = s.params()
+    xi = p["xi"]
+    assert xi == pytest.approx(tr ** (1 / 6) * math.sqrt(math.pi)
+                               / (box[0] * box[1] * box[2]) ** (1 / 3),
+                               rel=1e-15)
+    lmin, lmax = min(box), max(box)
+    rmax = math.sqrt(p["rmax2"])
+    r0 = 10 * math.sqrt(-math.log(eps)) / xi
+    n = round(math.log(rmax / r0) / math.log(0.99))
+    assert rmax == pytest.approx(r0 * 0.99 ** n, rel=1e-12)
+    # every shrink taken satisfied the criterion, the next one would not
+    assert _tail_real(rmax * 0.99, xi) + 6 * _tail_real(rmax * 0.99 + lmin, xi) >= eps
+    assert _tail_real(rmax, xi) + 6 * _tail_real(rmax + lmin, xi) < eps or n == 0
+    kmax = p["kmax"]
+    k0 = 10 * math.sqrt(-math.log(eps)) * 2 * xi
+    dk = 2 * math.pi / lmax
+    assert _tail_recip(kmax * 0.99, xi) + 6 * _tail_recip(kmax * 0.99 + dk, xi) >= eps
+    nk = round(math.log(kmax / k0) / math.log(0.99))
+    assert kmax == pytest.approx(k0 * 0.99 ** nk, rel=1e-12)
+    for ax, key in zip(box, ("rmaxx", "rmaxy", "rmaxz")):
+        assert p[key] == int(rmax / ax) + 1
+    for ax, key in zip(box, ("kmaxx", "kmaxy", "kmaxz")):
+        assert p[key] == int(kmax * ax / 2 / math.pi)
+    # image table by brute force
+    diag = math.sqrt(sum(b * b for b in box))
+    want = []
+    for m1 in range(-p["rmaxx"], p["rmaxx"] + 1):
+        for m2 in range(-p["rmaxy"], p["rmaxy"] + 1):
+            for m3 in range(-p["rmaxz"], p["rmaxz"] + 1):
+                c = [m * b - 0.5 * b * np.sign(m) for m, b in zip((m1, m2, m3), box)]
+                if math.sqrt(sum(v * v for v in c)) <= rmax + diag:
+                    want.append((m1, m2, m3))
+    rl, rm = s.image_table()
+    assert [tuple(r) for r in rm] == want
+    assert np.allclose(rl, rm * np.array(box), rtol=0, atol=0)
+
+
+@pytest.mark.parametrize("L", [5.0, 8.0, 13.0, 20.0])
+@pytest.mark.parametrize("tr", [1.0, 10.0, 1e3])
+def test_hasimoto_simple_cubic(L, tr):
+    """N = 1 in a cubic box: the RPY-Ewald translational self-mobility is
+    1 - 2.837297479 a/L + (4 pi / 3)(a/L)^3 (Hasimoto 1959; Beenakker 1986)"""
+    s = po.System(0, np.zeros((1, 3)), box=(L, L, L), tr=tr)
+    y = s.atimes(np.array([0.0, 1.0, 0.0]))
+    x = 1.0 / L
+    want = 1 - 2.837297479 * x + 4 * math.pi / 3 * x ** 3
+    assert y[1] == pytest.approx(want, rel=1e-9)
+    assert abs(y[0]) < 1e-13 and abs(y[2]) < 1e-13
+
+
+@pytest.mark.parametrize("version", [0, 1])
+def test_xi_independence(version, golden):
+    data, _ = golden
+    pos = data["fix9_FTS/pos"]
+    x = np.random.default_rng(3).standard_normal(9 * po.dim(version))
+    ys = [po.System(version, pos, box=(FIX9_L,) * 3, tr=tr).atimes(x)
+          for tr in (1.0, 4.0, 10.0, 100.0, 1e3)]
+    for y in ys[1:]:
+        assert relerr(y, ys[0]) < 1e-11
+
+
+def _fts_blocks(pos, box, tr, rng_seed):
+    """FTS responses split by input block: returns dict (in, out) -> array"""
+    rng = np.random.default_rng(rng_seed)
+    npart = len(pos)
+    sl = {"F": slice(0, 3), "T": slice(3, 6), "S": slice(6, 11)}
+    out = {}
+    for name, s in sl.items():
+        x = np.zeros((npart, 11))
+        x[:, s] = rng.standard_normal((npart, s.stop - s.start))
+        y = po.System(2, pos, box=box, tr=tr).atimes(x.ravel()).reshape(npart, 11)
+        for oname, os_ in (("U", slice(0, 3)), ("O", slice(3, 6)), ("E", slice(6, 11))):
+            out[(name, oname)] = y[:, os_]
+    return out
+
+
+def test_xi_independence_fts(golden):
+    """FTS: every coupling except the G / G-tilde pair (F -> E, S -> U) is
+    independent of xi.  The reference's own G / G-tilde real + reciprocal
+    sum is not (pinned below as it is, since the library must reproduce
+    the reference)."""
+    data, _ = golden
+    pos = data["fix9_FTS/pos"]
+    base = _fts_blocks(pos, (FIX9_L,) * 3, 1.0, 11)
+    for tr in (4.0, 10.0, 1e3):
+        other = _fts_blocks(pos, (FIX9_L,) * 3, tr, 11)
+        for key, y0 in base.items():
+            if key in (("F", "E"), ("S", "U")):
+                continue
+            scale = max(np.abs(y0).max(), 1.0)
+            assert np.abs(other[key] - y0).max() < 1e-11 * scale, (tr, key)
+
+
+def test_reference_fts_g_block_depends_on_xi(golden):
+    """Documented property of the reference (reproduced by the oracle,
+    which equals the compiled reference to 1e-13): the F -> E and S -> U
+    couplings change with xi, at O(1) for this fixture."""
+    data, _ = golden
+    pos = data["fix9_FTS/pos"]
+    b1 = _fts_blocks(pos, (FIX9_L,) * 3, 1.0, 12)
+    b2 = _fts_blocks(pos, (FIX9_L,) * 3, 10.0, 12)
+    assert np.abs(b1[("F", "E")] - b2[("F", "E")]).max() > 1e-3
+    assert np.abs(b1[("S", "U")] - b2[("S", "U")]).max() > 1e-3
+
+
+@pytest.mark.parametrize("version", [0, 1, 2])
+def test_atimes_equals_matrix(version, golden):
+    """check-ewald.c:46-197: atimes == extmat . x"""
+    data, _ = golden
+    pos = data["fix9_FTS/pos"]
+    s = po.System(version, pos, box=(FIX9_L,) * 3, tr=1.0, shear_mode=1,
+                  shear_shift=1.1)
+    x = np.random.default_rng(4).standard_normal(s.n)
+    m = s.matrix()
+    if version == 2:
+        ym = po.extmat_fts(9, m, x)
+    else:
+        ym = m @ x
+    assert relerr(s.atimes(x), ym) < 1e-13
+
+
+@pytest.mark.parametrize("version", [0, 1, 2])
+def test_mono_equals_poly_a1(version, golden):
+    """check-ewald-poly.c: the poly path with a = 1 equals the mono path"""
+    data, _ = golden
+    pos = data["fix9_FTS/pos"]
+    x = np.random.default_rng(5).standard_normal(9 * po.dim(version))
+    y1 = po.System(version, pos, box=(FIX9_L,) * 3, tr=1.0).atimes(x)
+    y2 = po.System(version, pos, box=(FIX9_L,) * 3, tr=1.0,
+                   a=np.ones(9)).atimes(x)
+    assert relerr(y2, y1) < 1e-13
+
+
+@pytest.mark.parametrize("mode", [1, 2])
+@pytest.mark.parametrize("version", [0, 1, 2])
+def test_shear_shift_by_L(mode, version):
+    """check-ewald-shear.c:51-160: two spheres at (0,0,0) and (l,0,0) in a
+    (2l, l, l) box (period l along x), x = 1: a Lees-Edwards shift of +-l
+    maps the image lattice onto itself, so the result equals no shift.
+    Tolerances of test-all.c:204-224."""
+    l = (math.pi / 0.75 / 0.2) ** (1 / 3)
+    pos = np.array([[0.0, 0.0, 0.0], [l, 0.0, 0.0]])
+    box = (2 * l, l, l)
+    x = np.ones(2 * po.dim(version))
+    tol = {0: 6e-14, 1: 2e-13, 2: 5e-12}[version]
+    y0 = po.System(version, pos, box=box, tr=1.0, shear_mode=mode,
+                   shear_shift=0.0).atimes(x)
+    for shift in (l, -l):
+        y = po.System(version, pos, box=box, tr=1.0, shear_mode=mode,
+                      shear_shift=shift).atimes(x)
+        assert maxrel(y, y0) < tol
+
+
+def test_f_mobility_symmetric_positive(golden):
+    data, _ = golden
+    s = po.System(0, data["fix9_F/pos"], box=(FIX9_L,) * 3, tr=1.0)
+    m = s.matrix()
+    assert np.abs(m - m.T).max() < 1e-14 * np.abs(m).max()
+    assert np.linalg.eigvalsh(m).min() > 0
+
+
+def test_open_bc_is_ewald_limit():
+    """a pair in a very large box approaches the open-boundary RPY result
+    (corrections O(a/L))"""
+    pos = np.array([[0.0, 0.0, 0.0], [2.5, 0.7, -0.3]])
+    x = np.random.default_rng(8).standard_normal(22)
+    y_open = po.System(2, pos).atimes(x)
+    errs = []
+    for L in (200.0, 400.0):
+        y = po.System(2, pos, box=(L, L, L), tr=1.0).atimes(x)
+        errs.append(relerr(y, y_open))
+    assert errs[1] < errs[0] < 0.02
+    assert errs[0] / errs[1] == pytest.approx(2.0, rel=0.05)
+
+
+def test_rpy_far_field_closed_form():
+    """open-BC F mobility of a pair: xa = 3/(2r) - 1/r^3, ya = 3/(4r) + 1/(2r^3)"""
+    r = 3.7
+    pos = np.array([[0.0, 0.0, 0.0], [0.0, 0.0, r]])
+    s = po.System(0, pos)
+    m = s.matrix()
+    assert m[2, 5] == pytest.approx(1.5 / r - 1 / r ** 3, rel=1e-15)
+    assert m[0, 3] == pytest.approx(0.75 / r + 0.5 / r ** 3, rel=1e-15)
+    assert m[0, 0] == 1.0
+
+
+def test_scalars_match_reference_golden_r():
+    """scalars_ewald_real == scalars_ewald_real_poly(a=1) (check-poly.c:216)"""
+    for r in (2.0, 2.5, 4.0):
+        for xi in (0.3, 1.0):
+            m = po.scalars_ewald_real(2, xi, r)
+            p = po.scalars_ewald_real_poly(2, xi, r, 1.0, 1.0)
+            assert np.abs(m - p).max() < 1.3e-15 * max(1, np.abs(m).max()) * 10
+
+
+@pytest.mark.skipif(not po.ref_available(), reason="oracle/_ref not built")
+def test_scalars_match_compiled_reference():
+    for r in (0.5, 2.0, 2.5, 7.0):
+        for xi in (0.1, 0.3, 1.0):
+            assert np.array_equal(po.scalars_ewald_real(2, xi, r),
+                                  po.ref_scalars_ewald_real(2, xi, r))
+            assert np.allclose(po.scalars_ewald_real_poly(2, xi, r, 1.3, 0.7),
+                               po.ref_scalars_ewald_real(2, xi, r, (1.3, 0.7)),
+                               rtol=1e-13, atol=1e-16)
+
+
+def test_chebyshev_golden(golden):
+    data, _ = golden
+    for fid, nch in ((0, 50), (1, 30)):
+        A = data[f"cheb{fid}/A"]
+        e0, e1 = data[f"cheb{fid}/bounds"]
+        a = po.cheb_coef(nch, fid, e0, e1)
+        assert np.allclose(a, data[f"cheb{fid}/a"], rtol=0, atol=1e-15)
+        z = po.cheb_eval_dense(a, A, np.ones(10), e0, e1)
+        assert relerr(z, data[f"cheb{fid}/z"]) < 1e-14
+        # closed form: f(A) y by eigendecomposition
+        w, V = np.linalg.eigh(A)
+        f = 1 / np.sqrt(w) if fid == 0 else np.sqrt(w)
+        assert relerr(z, V @ (f * (V.T @ np.ones(10)))) < 1e-9
+    # check-brownian.c:69-200: z.z = y.A^-1.y
+    z = po.cheb_eval_dense(po.cheb_coef(50, 0, e0, e1), A, np.ones(10), e0, e1)
+    yAy = np.ones(10) @ np.linalg.solve(A, np.ones(10))
+    assert z @ z == pytest.approx(yAy, rel=2e-10)
+
+
+def test_chebyshev_scalar_fits():
+    """check-chebyshev.c:72: cos, sin and sqrt fitted with n = 100"""
+    for fid, f, lo, hi in ((2, math.cos, -1.0, 2.0), (3, math.sin, 0.0, 3.0),
+                           (1, math.sqrt, 0.5, 4.0)):
+        a = po.cheb_coef(100, fid, lo, hi)
+        for x in np.linspace(lo, hi, 17):
+            assert po.cheb_eval(a, lo, hi, x) == pytest.approx(f(x), abs=1e-10)
+
+
+def test_mt19937_known_answer(golden):
+    data, _ = golden
+    r = po.Rng(5489)
+    first = [r.int32() for _ in range(16)]
+    assert first[0] == 3499211612   # published first output of MT19937
+    assert first == list(data["rng5489/int32"])
+    r0 = po.Rng(0)
+    assert [r0.int32() for _ in range(16)] == list(data["rng0/int32"])
+    g = po.Rng(0).gaussian(16)
+    assert np.array_equal(g, data["rng0/gauss"])
