@@ -41,7 +41,9 @@ int stokes_hip_set_precision (struct stokes *sys, int mode);
 /* record per-part GPU times (ms) into sys->cpu1/cpu2/cpu3 */
 int stokes_hip_set_timing (struct stokes *sys, int enable);
 
-/* device memory on the GPU of the library */
+/* device memory on the GPU of the library; the copies wait for all queued
+ * work first, and operator calls on device pointers are asynchronous
+ * (stokes_hip_sync or a copy orders them) */
 int stokes_hip_malloc (void **p, size_t bytes);
 int stokes_hip_free (void *p);
 int stokes_hip_memcpy_h2d (void *dst_dev, const void *src, size_t bytes);

@@ -186,20 +186,85 @@ __device__ __forceinline__ void kblock_moments (const double4 kvec, const double
   moments_add<V, T> (Ms, ex, ey, ez, ss);
 }
 
-template <int V, typename T>
+// Polydisperse reciprocal scalars (ewald.c:985-1025) expanded in powers of
+// the radii: every scalar of pair (i, j) is a sum of c(k) a_i^p a_j^q with
+// q in {0, 2}.  For power p of a_i this returns the paired-k (x2) scalars
+// acting on the q = 0 sums (sc1 even / ss1 odd) and on the q = 2 sums
+// (sc2 / ss2), in the order xa ya yb xc yc xg yg yh xm ym zm.
+__device__ __forceinline__ void poly_kscalars (int p, double k, double kk, double kexp,
+                                               double *sc1, double *ss1, double *sc2,
+                                               double *ss2)
+{
+  for (int t = 0; t < 11; t++) sc1[t] = ss1[t] = sc2[t] = ss2[t] = 0.0;
+  const double e = 2.0 * kexp;
+  switch (p)
+    {
+    case 1:                                   // ya: 6 a_i - kk a_i a_j^2
+      sc1[1] = 6.0 * e;
+      sc2[1] = -kk * e;
+      break;
+    case 2:                                   // yb; yg: 3k a_i^2 - 0.5 kk k a_i^2 a_j^2
+      ss1[2] = 3.0 * k * e;
+      ss1[6] = 3.0 * k * e;
+      ss2[6] = -0.5 * kk * k * e;
+      break;
+    case 3:                                   // ya, yc, yh, ym
+      sc1[1] = -kk * e;
+      sc1[4] = 1.5 * kk * e;
+      sc1[7] = 1.5 * kk * e;
+      sc1[9] = 3.0 * kk * e;
+      sc2[9] = -0.3 * kk * kk * e;
+      break;
+    case 4:                                   // yg: -0.3 kk k a_i^4
+      ss1[6] = -0.3 * kk * k * e;
+      break;
+    default:                                  // ym: -0.3 kk^2 a_i^5
+      sc1[9] = -0.3 * kk * kk * e;
+      break;
+    }
+}
+
+// P = Ac C - As S, Q = Ac S + As C for the even (Mc) / odd (Ms) blocks
+template <int V>
+__device__ __forceinline__ void kblock_apply (const double *Mc, const double *Ms,
+                                              const double *C, const double *S,
+                                              double *P, double *Q)
+{
+  constexpr int D = Dim<V>::D;
+#pragma unroll
+  for (int pr = 0; pr < D; pr++)
+    {
+      double a = 0.0, b = 0.0;
+#pragma unroll
+      for (int q = 0; q < D; q++)
+        {
+          const double ac = block_entry<V, double> (Mc, pr, q, false);
+          const double as = block_entry<V, double> (Ms, pr, q, false);
+          a += ac * C[q] - as * S[q];
+          b += ac * S[q] + as * C[q];
+        }
+      P[pr] += a;
+      Q[pr] += b;
+    }
+}
+
+// NPOW = 1: monodisperse, PQ[k][2][D]; NPOW = 5: polydisperse, PQ[k][5][2][D]
+// holding the coefficients of a_i^1 .. a_i^5.
+template <int V, typename T, int NPOW>
 __global__ __launch_bounds__ (256) void
 k_recip_fwd (const double4 *__restrict__ pos, const double *__restrict__ x,
              const double4 *__restrict__ kv, const double *__restrict__ ks,
              int np, double *__restrict__ PQ)
 {
   constexpr int D = Dim<V>::D, NM = Dim<V>::NMOM;
+  constexpr int NS = (NPOW > 1) ? 4 : 2;      // C S [C2 S2]
   __shared__ double red[256];
-  __shared__ double CS[2 * D];
+  __shared__ double CS[NS * D];
   const int k = blockIdx.x;
   const double4 kk = kv[k];
-  T C[D], S[D];
+  T A[NS * D];
 #pragma unroll
-  for (int q = 0; q < D; q++) { C[q] = T (0); S[q] = T (0); }
+  for (int q = 0; q < NS * D; q++) A[q] = T (0);
   for (int j = threadIdx.x; j < np; j += blockDim.x)
     {
       const double4 pj = pos[j];
@@ -208,14 +273,20 @@ k_recip_fwd (const double4 *__restrict__ pos, const double *__restrict__ x,
       T z[D];
       load_z<V, T> (x, j, z);
 #pragma unroll
-      for (int q = 0; q < D; q++) { C[q] += c * z[q]; S[q] += s * z[q]; }
+      for (int q = 0; q < D; q++) { A[q] += c * z[q]; A[D + q] += s * z[q]; }
+      if (NPOW > 1)
+        {
+          const T a2 = T (pj.w * pj.w);
+#pragma unroll
+          for (int q = 0; q < D; q++)
+            { A[2 * D + q] += a2 * c * z[q]; A[3 * D + q] += a2 * s * z[q]; }
+        }
     }
   // block reduction, one component at a time (deterministic order)
 #pragma unroll
-  for (int q = 0; q < 2 * D; q++)
+  for (int q = 0; q < NS * D; q++)
     {
-      const T v = q < D ? C[q] : S[q - D];
-      red[threadIdx.x] = double (v);
+      red[threadIdx.x] = double (A[q]);
       __syncthreads ();
       for (int st = blockDim.x / 2; st > 0; st >>= 1)
         {
@@ -225,29 +296,50 @@ k_recip_fwd (const double4 *__restrict__ pos, const double *__restrict__ x,
       if (threadIdx.x == 0) CS[q] = red[0];
       __syncthreads ();
     }
-  if (threadIdx.x == 0)
+  if (threadIdx.x != 0) return;
+  const double *w = ks + (size_t) k * 8;
+  double Mc[NM], Ms[NM];
+  if (NPOW == 1)
     {
-      double Mc[NM], Ms[NM];
-      kblock_moments<V, double> (kk, ks + (size_t) k * 8, Mc, Ms);
+      double P[D], Q[D];
 #pragma unroll
-      for (int pr = 0; pr < D; pr++)
+      for (int q = 0; q < D; q++) P[q] = Q[q] = 0.0;
+      kblock_moments<V, double> (kk, w, Mc, Ms);
+      kblock_apply<V> (Mc, Ms, CS, CS + D, P, Q);
+#pragma unroll
+      for (int q = 0; q < D; q++)
         {
-          double P = 0.0, Q = 0.0;
-#pragma unroll
-          for (int q = 0; q < D; q++)
-            {
-              const double ac = block_entry<V, double> (Mc, pr, q, false);
-              const double as = block_entry<V, double> (Ms, pr, q, false);
-              P += ac * CS[q] - as * CS[D + q];
-              Q += ac * CS[D + q] + as * CS[q];
-            }
-          PQ[((size_t) k * 2 + 0) * D + pr] = P;
-          PQ[((size_t) k * 2 + 1) * D + pr] = Q;
+          PQ[((size_t) k * 2 + 0) * D + q] = P[q];
+          PQ[((size_t) k * 2 + 1) * D + q] = Q[q];
         }
+      return;
+    }
+  const double ik = 1.0 / kk.w;
+  const double ex = kk.x * ik, ey = kk.y * ik, ez = kk.z * ik;
+  for (int p = 1; p <= NPOW; p++)
+    {
+      double sc1[11], ss1[11], sc2[11], ss2[11];
+      poly_kscalars (p, kk.w, w[6], w[7], sc1, ss1, sc2, ss2);
+      double P[D], Q[D];
+#pragma unroll
+      for (int q = 0; q < D; q++) P[q] = Q[q] = 0.0;
+#pragma unroll
+      for (int t = 0; t < NM; t++) Mc[t] = Ms[t] = 0.0;
+      moments_add<V, double> (Mc, ex, ey, ez, sc1);
+      moments_add<V, double> (Ms, ex, ey, ez, ss1);
+      kblock_apply<V> (Mc, Ms, CS, CS + D, P, Q);
+#pragma unroll
+      for (int t = 0; t < NM; t++) Mc[t] = Ms[t] = 0.0;
+      moments_add<V, double> (Mc, ex, ey, ez, sc2);
+      moments_add<V, double> (Ms, ex, ey, ez, ss2);
+      kblock_apply<V> (Mc, Ms, CS + 2 * D, CS + 3 * D, P, Q);
+      double *out = PQ + (((size_t) k * NPOW + (p - 1)) * 2) * D;
+#pragma unroll
+      for (int q = 0; q < D; q++) { out[q] = P[q]; out[D + q] = Q[q]; }
     }
 }
 
-template <int V, typename T>
+template <int V, typename T, int NPOW>
 __global__ __launch_bounds__ (256) void
 k_finalize (const double4 *__restrict__ pos, const double *__restrict__ x,
             const double4 *__restrict__ kv, const double *__restrict__ PQ,
@@ -271,14 +363,23 @@ k_finalize (const double4 *__restrict__ pos, const double *__restrict__ x,
     load_z<V, double> (x, i, z);
     block_apply<V, double, double> (M, z, acc);
   }
+  double apow[NPOW];
+  apow[0] = NPOW > 1 ? pi.w : 1.0;
+#pragma unroll
+  for (int t = 1; t < NPOW; t++) apow[t] = apow[t - 1] * pi.w;
   for (int k = 0; k < p.nk; k++)
     {
       const double4 kk = kv[k];
       T c, s;
       phase_cs (kk.x * pi.x + kk.y * pi.y + kk.z * pi.z, c, s);
-      const double *P = PQ + (size_t) k * 2 * D;
 #pragma unroll
-      for (int q = 0; q < D; q++) acc[q] += double (c) * P[q] + double (s) * P[D + q];
+      for (int t = 0; t < NPOW; t++)
+        {
+          const double *P = PQ + ((size_t) k * NPOW + t) * 2 * D;
+          const double cc = apow[t] * double (c), ss = apow[t] * double (s);
+#pragma unroll
+          for (int q = 0; q < D; q++) acc[q] += cc * P[q] + ss * P[D + q];
+        }
     }
   for (int ch = 0; ch < nchunk; ch++)
     {

@@ -406,4 +406,75 @@ SHK_HD void s_transform (T *z)
   z[10] = T (2) * s10 + s6;
 }
 
+// ---------------------------------------------------------------------------
+// Direct contraction of one pair-image block with one transformed 11-vector
+// z (acc += block(e, s) . z, non-self convention), written with vector
+// identities instead of the 121 entries:
+//   U += a1 F + a2 e(e.F) + b1 (T x e) - [g1 e tr + 2 g2 (Z e) + g3 e Q]
+//   W += b1 (F x e) + c1 T + c2 e(e.T) + 2 h1 e x (Z e)
+//   E_ab += g1 d_ab (e.F) + g2 (e_a F_b + e_b F_a) + g3 e_a e_b (e.F)
+//         + h1 (e_a w_b + e_b w_a)                       (w = T x e)
+//         + mm1 e_a e_b Q + mm2 (d_ab Q + e_a e_b tr) + mm3 d_ab tr
+//         + 2 mm4 (e_a (Ze)_b + e_b (Ze)_a) + 2 mm5 Z_ab
+// with Z the symmetric 3x3 whose diagonal is (z6, z10, 0) and off-diagonal
+// xy, xz, yz are z7/2, z8/2, z9/2 (the extracted, transformed stresslet),
+// Q = e.Z.e and tr = z6 + z10.  Same operator as fts.c:625-1010.
+// ---------------------------------------------------------------------------
+template <int V, typename T>
+SHK_HD void contract_item (T ex, T ey, T ez, const T *s, const T *z, T *acc)
+{
+  const T eF = ex * z[0] + ey * z[1] + ez * z[2];
+  const T a1 = s[1], a2 = s[0] - s[1];
+  acc[0] += a1 * z[0] + a2 * ex * eF;
+  acc[1] += a1 * z[1] + a2 * ey * eF;
+  acc[2] += a1 * z[2] + a2 * ez * eF;
+  if (V == 0) return;
+  const T b1 = s[2];
+  // T x e and F x e
+  const T tex = z[4] * ez - z[5] * ey, tey = z[5] * ex - z[3] * ez, tez = z[3] * ey - z[4] * ex;
+  const T fex = z[1] * ez - z[2] * ey, fey = z[2] * ex - z[0] * ez, fez = z[0] * ey - z[1] * ex;
+  acc[0] += b1 * tex; acc[1] += b1 * tey; acc[2] += b1 * tez;
+  const T eT = ex * z[3] + ey * z[4] + ez * z[5];
+  const T c1 = s[4], c2 = s[3] - s[4];
+  acc[3] += b1 * fex + c1 * z[3] + c2 * ex * eT;
+  acc[4] += b1 * fey + c1 * z[4] + c2 * ey * eT;
+  acc[5] += b1 * fez + c1 * z[5] + c2 * ez * eT;
+  if (V == 1) return;
+  const T zxx = z[6], zxy = T (0.5) * z[7], zxz = T (0.5) * z[8], zyz = T (0.5) * z[9];
+  const T zyy = z[10];
+  const T Zx = zxx * ex + zxy * ey + zxz * ez;
+  const T Zy = zxy * ex + zyy * ey + zyz * ez;
+  const T Zz = zxz * ex + zyz * ey;
+  const T Q = Zx * ex + Zy * ey + Zz * ez;
+  const T tr = zxx + zyy;
+  const T g1 = T (-1.0 / 3.0) * s[5], g2 = s[6], g3 = s[5] - T (2) * s[6];
+  const T gu = g1 * tr + g3 * Q;
+  acc[0] -= gu * ex + T (2) * g2 * Zx;
+  acc[1] -= gu * ey + T (2) * g2 * Zy;
+  acc[2] -= gu * ez + T (2) * g2 * Zz;
+  const T h2 = T (2) * s[7];
+  acc[3] += h2 * (ey * Zz - ez * Zy);
+  acc[4] += h2 * (ez * Zx - ex * Zz);
+  acc[5] += h2 * (ex * Zy - ey * Zx);
+  const T xm = s[8], ym = s[9], zm = s[10];
+  const T mm1 = T (1.5) * xm - T (2) * ym + T (0.5) * zm;
+  const T mm2 = T (-0.5) * xm + T (0.5) * zm;
+  const T mm3 = T (1.0 / 6.0) * xm - T (0.5) * zm;
+  const T mm4x2 = ym - zm;          // 2 mm4
+  const T mm5x2 = zm;               // 2 mm5
+  const T h1 = s[7];
+  // coefficient of e_a e_b and of d_ab
+  const T cee = g3 * eF + mm1 * Q + mm2 * tr;
+  const T cdd = g1 * eF + mm2 * Q + mm3 * tr;
+  // vector multiplying e_b in the symmetric (e_a v_b + e_b v_a) part
+  const T vx = g2 * z[0] + h1 * tex + mm4x2 * Zx;
+  const T vy = g2 * z[1] + h1 * tey + mm4x2 * Zy;
+  const T vz = g2 * z[2] + h1 * tez + mm4x2 * Zz;
+  acc[6] += cee * ex * ex + cdd + T (2) * ex * vx + mm5x2 * zxx;   // xx
+  acc[7] += cee * ex * ey + ex * vy + ey * vx + mm5x2 * zxy;       // xy
+  acc[8] += cee * ex * ez + ex * vz + ez * vx + mm5x2 * zxz;       // xz
+  acc[9] += cee * ey * ez + ey * vz + ez * vy + mm5x2 * zyz;       // yz
+  acc[10] += cee * ey * ey + cdd + T (2) * ey * vy + mm5x2 * zyy;  // yy
+}
+
 }  // namespace shk

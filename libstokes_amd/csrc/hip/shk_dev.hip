@@ -9,6 +9,8 @@
 #include <type_traits>
 #include "../shk.h"
 #include "ewald_kernels.h"
+#include "real_kernels.h"
+#include "recip_kernels.h"
 
 using namespace shk;
 
@@ -45,6 +47,10 @@ struct shk_dev {
   shk_params prm{};
   int have_sys = 0;
   DevBuf pos, img, imgf, kv, ks, partial, pq, xs, ys, zt;
+  DevBuf cand, candf, candm, tbits, kpart, kpq, kback;
+  int ncand = 0, ncore = 0, tbits_words = 0;
+  int tx = 0, ty = 0, tz = 0;
+  double s2x = 0.0, s3x = 0.0;
   int timing = 0;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   double last_ms[3] = {0, 0, 0};
@@ -102,7 +108,8 @@ extern "C" void shk_dev_destroy (shk_dev *d)
   if (!d) return;
   if (d->stream) (void) hipStreamSynchronize (d->stream);
   for (DevBuf *b : {&d->pos, &d->img, &d->imgf, &d->kv, &d->ks, &d->partial,
-                    &d->pq, &d->xs, &d->ys, &d->zt})
+                    &d->pq, &d->xs, &d->ys, &d->zt, &d->cand, &d->candf, &d->candm,
+                    &d->tbits, &d->kpart, &d->kpq, &d->kback})
     release (*b);
   for (auto &e : d->ev)
     if (e) (void) hipEventDestroy (e);
@@ -153,6 +160,12 @@ extern "C" int shk_dev_d2d (shk_dev *d, void *dst, const void *src, size_t bytes
 extern "C" int shk_dev_sync (shk_dev *d)
 {
   HIPCHK (hipStreamSynchronize (d->stream));
+  return SHK_OK;
+}
+
+extern "C" int shk_device_sync (void)
+{
+  HIPCHK (hipDeviceSynchronize ());
   return SHK_OK;
 }
 
@@ -220,9 +233,24 @@ static int upload_pos (shk_dev *d, const double *pos, const double *a, int on_de
   return SHK_OK;
 }
 
+static int upload (shk_dev *d, DevBuf &b, const void *src, size_t bytes)
+{
+  int rc = ensure (d, b, bytes > 0 ? bytes : 16);
+  if (rc) return rc;
+  if (bytes) HIPCHK (hipMemcpyAsync (b.p, src, bytes, hipMemcpyHostToDevice, d->stream));
+  return SHK_OK;
+}
+
+__global__ void k_d4_to_f4 (const double4 *__restrict__ in, int n, float4 *__restrict__ out)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double4 v = in[i];
+  out[i] = make_float4 (float (v.x), float (v.y), float (v.z), float (v.w));
+}
+
 extern "C" int shk_dev_set_system (shk_dev *d, const shk_params *p, const double *pos,
-                                   const double *a, const double *img, const double *kv,
-                                   const double *ks)
+                                   const double *a, const shk_tables *t)
 {
   if (!d || !p || p->np <= 0 || p->version < 0 || p->version > 2)
     {
@@ -234,26 +262,23 @@ extern "C" int shk_dev_set_system (shk_dev *d, const shk_params *p, const double
   if (rc) return rc;
   if (p->periodic)
     {
-      if ((rc = ensure (d, d->img, sizeof (double4) * (p->nr > 0 ? p->nr : 1)))) return rc;
-      if ((rc = ensure (d, d->imgf, sizeof (float4) * (p->nr > 0 ? p->nr : 1)))) return rc;
-      if (p->nr > 0)
+      if (!t) { set_err ("shk_dev_set_system: periodic system without tables"); return SHK_EARG; }
+      if ((rc = upload (d, d->img, t->img, sizeof (double4) * p->nr))) return rc;
+      if ((rc = upload (d, d->cand, t->cand, sizeof (double4) * t->ncand))) return rc;
+      if ((rc = upload (d, d->candm, t->candm, sizeof (int4) * t->ncand))) return rc;
+      if ((rc = upload (d, d->tbits, t->tbits, sizeof (unsigned) * t->tbits_words))) return rc;
+      if ((rc = upload (d, d->kv, t->kv, sizeof (double4) * p->nk))) return rc;
+      if ((rc = upload (d, d->ks, t->ks, sizeof (double) * 8 * p->nk))) return rc;
+      if ((rc = ensure (d, d->candf, sizeof (float4) * (t->ncand > 0 ? t->ncand : 1)))) return rc;
+      if (t->ncand > 0)
         {
-          HIPCHK (hipMemcpyAsync (d->img.p, img, sizeof (double4) * p->nr,
-                                  hipMemcpyHostToDevice, d->stream));
-          hipLaunchKernelGGL (k_img_to_f32, dim3 ((p->nr + 255) / 256), dim3 (256), 0,
-                              d->stream, (const double4 *) d->img.p, p->nr, (float4 *) d->imgf.p);
+          hipLaunchKernelGGL (k_d4_to_f4, dim3 ((t->ncand + 255) / 256), dim3 (256), 0, d->stream,
+                              (const double4 *) d->cand.p, t->ncand, (float4 *) d->candf.p);
           HIPCHK (hipGetLastError ());
         }
-      const int nk = p->nk > 0 ? p->nk : 1;
-      if ((rc = ensure (d, d->kv, sizeof (double4) * nk))) return rc;
-      if ((rc = ensure (d, d->ks, sizeof (double) * 8 * nk))) return rc;
-      if (p->nk > 0)
-        {
-          HIPCHK (hipMemcpyAsync (d->kv.p, kv, sizeof (double4) * p->nk,
-                                  hipMemcpyHostToDevice, d->stream));
-          HIPCHK (hipMemcpyAsync (d->ks.p, ks, sizeof (double) * 8 * p->nk,
-                                  hipMemcpyHostToDevice, d->stream));
-        }
+      d->ncand = t->ncand; d->ncore = t->ncore; d->tbits_words = t->tbits_words;
+      d->tx = t->tx; d->ty = t->ty; d->tz = t->tz;
+      d->s2x = t->s2x; d->s3x = t->s3x;
     }
   // the host arrays may be freed by the caller right after return
   HIPCHK (hipStreamSynchronize (d->stream));
@@ -296,68 +321,124 @@ static void launch_pairs (shk_dev *d, const DevParams &q, const double *x, int n
                       (double *) d->partial.p);
 }
 
-template <int V>
-static int atimes_v (shk_dev *d, const double *x, double *y, int prec)
+// reciprocal-space launch shape: enough workgroups for 256 CUs, bounded
+// partial-sum buffers
+static void recip_shape (int np, int nk, int &sj, int &jper, int &sk, int &kper)
+{
+  const int nkt = (nk + 255) / 256;
+  sj = 1024 / nkt;
+  if (sj > 32) sj = 32;
+  if (sj > (np + 63) / 64) sj = (np + 63) / 64;
+  if (sj < 1) sj = 1;
+  jper = (np + sj - 1) / sj;
+  sj = (np + jper - 1) / jper;
+  const int nit = (np + 255) / 256;
+  sk = 1024 / nit;
+  if (sk > 64) sk = 64;
+  if (sk > (nk + 31) / 32) sk = (nk + 31) / 32;
+  if (sk < 1) sk = 1;
+  kper = (nk + sk - 1) / sk;
+  sk = (nk + kper - 1) / kper;
+}
+
+template <int V, typename T, int NPOW>
+static int recip_launch (shk_dev *d, const double *x, int &nrecip, const void *&kpart)
+{
+  constexpr int D = Dim<V>::D;
+  constexpr int NS = NPOW > 1 ? 4 : 2;
+  const shk_params &p = d->prm;
+  const int nk = p.nk;
+  int sj, jper, sk, kper;
+  recip_shape (p.np, nk, sj, jper, sk, kper);
+  int rc;
+  if ((rc = ensure (d, d->kpart, sizeof (T) * (size_t) sj * nk * NS * D))) return rc;
+  if ((rc = ensure (d, d->kpq, sizeof (T) * (size_t) nk * NPOW * 2 * D))) return rc;
+  if ((rc = ensure (d, d->kback, sizeof (T) * (size_t) sk * p.np * D))) return rc;
+  const double4 *pos4 = (const double4 *) d->pos.p;
+  const double4 *kv = (const double4 *) d->kv.p;
+  hipLaunchKernelGGL ((k_recip_sf<V, T, NPOW>), dim3 ((nk + 255) / 256, sj), dim3 (256), 0,
+                      d->stream, pos4, x, kv, p.np, nk, jper, (T *) d->kpart.p);
+  HIPCHK (hipGetLastError ());
+  hipLaunchKernelGGL ((k_recip_blocks<V, T, NPOW>), dim3 ((nk + 127) / 128), dim3 (128), 0,
+                      d->stream, kv, (const double *) d->ks.p, nk, sj, (const T *) d->kpart.p,
+                      (T *) d->kpq.p);
+  HIPCHK (hipGetLastError ());
+  hipLaunchKernelGGL ((k_recip_back<V, T, NPOW>), dim3 ((p.np + 255) / 256, sk), dim3 (256), 0,
+                      d->stream, pos4, kv, (const T *) d->kpq.p, p.np, nk, kper,
+                      (T *) d->kback.p);
+  HIPCHK (hipGetLastError ());
+  nrecip = sk;
+  kpart = d->kback.p;
+  return SHK_OK;
+}
+
+template <int V, typename T>
+static int atimes_vt (shk_dev *d, const double *x, double *y)
 {
   constexpr int D = Dim<V>::D;
   const shk_params &p = d->prm;
   DevParams q = dev_params (p);
-  const int periodic = p.periodic;
-  if (!periodic) { q.nk = 0; q.nr = 0; }
-  int nit, nchunk, jchunk;
-  pair_grid (p.np, nit, nchunk, jchunk);
-  int rc = ensure (d, d->partial, sizeof (double) * (size_t) nchunk * p.np * D);
-  if (rc) return rc;
-  if (periodic && q.nk > 0)
-    if ((rc = ensure (d, d->pq, sizeof (double) * (size_t) q.nk * 2 * D))) return rc;
+  const double4 *pos4 = (const double4 *) d->pos.p;
+  int rc, nreal = 0, nrecip = 0;
+  const void *kpart = nullptr;
   if (d->timing) HIPCHK (hipEventRecord (d->ev[0], d->stream));
-  if (periodic)
+  if (p.periodic)
     {
-      if (prec == SHK_MIXED) launch_pairs<V, float, 0> (d, q, x, nit, nchunk, jchunk);
-      else launch_pairs<V, double, 0> (d, q, x, nit, nchunk, jchunk);
-    }
-  else
-    {
-      if (prec == SHK_MIXED) launch_pairs<V, float, 1> (d, q, x, nit, nchunk, jchunk);
-      else launch_pairs<V, double, 1> (d, q, x, nit, nchunk, jchunk);
-    }
-  HIPCHK (hipGetLastError ());
-  if (d->timing) HIPCHK (hipEventRecord (d->ev[1], d->stream));
-  if (periodic && q.nk > 0)
-    {
-      if (prec == SHK_MIXED)
-        hipLaunchKernelGGL ((k_recip_fwd<V, float>), dim3 (q.nk), dim3 (256), 0, d->stream,
-                            (const double4 *) d->pos.p, x, (const double4 *) d->kv.p,
-                            (const double *) d->ks.p, p.np, (double *) d->pq.p);
-      else
-        hipLaunchKernelGGL ((k_recip_fwd<V, double>), dim3 (q.nk), dim3 (256), 0, d->stream,
-                            (const double4 *) d->pos.p, x, (const double4 *) d->kv.p,
-                            (const double *) d->ks.p, p.np, (double *) d->pq.p);
+      const int nj = (p.np + 63) / 64;
+      nreal = nj;
+      if ((rc = ensure (d, d->partial, sizeof (double) * (size_t) nj * p.np * D))) return rc;
+      RealParams rp;
+      rp.np = p.np; rp.ncand = d->ncand; rp.ncore = d->ncore; rp.poly = p.poly;
+      rp.xi = p.xi; rp.rmax2 = p.rmax2; rp.rmin = p.rmin;
+      rp.lx = p.lx; rp.ly = p.ly; rp.lz = p.lz; rp.s2x = d->s2x; rp.s3x = d->s3x;
+      rp.tx = d->tx; rp.ty = d->ty; rp.tz = d->tz; rp.tbits_words = d->tbits_words;
+      const long waves = (long) p.np * nj;
+      const void *cand = std::is_same<T, double>::value ? d->cand.p : d->candf.p;
+      hipLaunchKernelGGL ((k_real_compact<V, T>), dim3 ((unsigned) ((waves + 3) / 4)), dim3 (256),
+                          0, d->stream, pos4, x, (const typename Vec4<T>::type *) cand,
+                          (const int4 *) d->candm.p, (const unsigned *) d->tbits.p, rp,
+                          (double *) d->partial.p);
       HIPCHK (hipGetLastError ());
+      if (d->timing) HIPCHK (hipEventRecord (d->ev[1], d->stream));
+      if (p.nk > 0)
+        {
+          rc = p.poly ? recip_launch<V, T, 5> (d, x, nrecip, kpart)
+                      : recip_launch<V, T, 1> (d, x, nrecip, kpart);
+          if (rc) return rc;
+        }
     }
-  if (prec == SHK_MIXED)
-    hipLaunchKernelGGL ((k_finalize<V, float>), dim3 ((p.np + 127) / 128), dim3 (128), 0,
-                        d->stream, (const double4 *) d->pos.p, x, (const double4 *) d->kv.p,
-                        (const double *) d->pq.p, (const double *) d->partial.p, nchunk, q,
-                        periodic, y);
   else
-    hipLaunchKernelGGL ((k_finalize<V, double>), dim3 ((p.np + 127) / 128), dim3 (128), 0,
-                        d->stream, (const double4 *) d->pos.p, x, (const double4 *) d->kv.p,
-                        (const double *) d->pq.p, (const double *) d->partial.p, nchunk, q,
-                        periodic, y);
+    {
+      int nit, nchunk, jchunk;
+      pair_grid (p.np, nit, nchunk, jchunk);
+      nreal = nchunk;
+      if ((rc = ensure (d, d->partial, sizeof (double) * (size_t) nchunk * p.np * D))) return rc;
+      launch_pairs<V, T, 1> (d, q, x, nit, nchunk, jchunk);
+      HIPCHK (hipGetLastError ());
+      if (d->timing) HIPCHK (hipEventRecord (d->ev[1], d->stream));
+    }
+  hipLaunchKernelGGL ((k_combine<V, T>), dim3 ((p.np + 127) / 128), dim3 (128), 0, d->stream,
+                      pos4, x, (const double *) d->partial.p, nreal, (const T *) kpart, nrecip, q,
+                      p.periodic, y);
   HIPCHK (hipGetLastError ());
   if (d->timing)
     {
       HIPCHK (hipEventRecord (d->ev[2], d->stream));
       HIPCHK (hipEventSynchronize (d->ev[2]));
-      float a = 0, b = 0;
-      HIPCHK (hipEventElapsedTime (&a, d->ev[0], d->ev[1]));
-      HIPCHK (hipEventElapsedTime (&b, d->ev[1], d->ev[2]));
-      d->last_ms[0] = a;
-      d->last_ms[1] = b;
-      d->last_ms[2] = a + b;
+      float ta = 0, tb = 0;
+      HIPCHK (hipEventElapsedTime (&ta, d->ev[0], d->ev[1]));
+      HIPCHK (hipEventElapsedTime (&tb, d->ev[1], d->ev[2]));
+      d->last_ms[0] = ta;
+      d->last_ms[1] = tb;
+      d->last_ms[2] = ta + tb;
     }
   return SHK_OK;
+}
+
+template <int V>
+static int atimes_v (shk_dev *d, const double *x, double *y, int prec)
+{
+  return prec == SHK_MIXED ? atimes_vt<V, float> (d, x, y) : atimes_vt<V, double> (d, x, y);
 }
 
 extern "C" int shk_dev_atimes (shk_dev *d, const double *x, double *y, int nrhs, int prec)
@@ -365,11 +446,6 @@ extern "C" int shk_dev_atimes (shk_dev *d, const double *x, double *y, int nrhs,
   if (!d || !d->have_sys) { set_err ("shk_dev_atimes: no system"); return SHK_EARG; }
   if (nrhs < 1) { set_err ("shk_dev_atimes: nrhs < 1"); return SHK_EARG; }
   const shk_params &p = d->prm;
-  if (p.periodic && p.poly)
-    {
-      set_err ("polydisperse Ewald matvec: reciprocal part not implemented on the device");
-      return SHK_EUNSUP;
-    }
   const size_t n = (size_t) p.np * dimv (p.version);
   for (int c = 0; c < nrhs; c++)
     {

@@ -216,6 +216,99 @@ build_kvectors (const struct stokes *s, double **kv_out, double **ks_out)
   return nk;
 }
 
+/* Lattice vectors that can hold an image inside r_max once x_j - x_i is
+ * wrapped into the primary cell (|components| <= half the box): those with
+ * |v| <= r_max + R, R the half diagonal, sorted by |v| (ties by index);
+ * *ncore counts the leading ones with |v| <= r_max - R, inside r_max for
+ * every pair.  Returns the count, or -1. */
+typedef struct { double v[4]; int m[4]; } cand_t;
+
+static int
+cand_cmp (const void *a, const void *b)
+{
+  const cand_t *x = a, *y = b;
+  if (x->v[3] < y->v[3]) return -1;
+  if (x->v[3] > y->v[3]) return 1;
+  for (int k = 0; k < 3; k++)
+    if (x->m[k] != y->m[k]) return x->m[k] < y->m[k] ? -1 : 1;
+  return 0;
+}
+
+static int
+build_candidates (const struct stokes *s, double s2x, double s3x,
+                  double **cand_out, int **candm_out, int *ncore)
+{
+  double rmax = sqrt (s->rmax2);
+  double rh = 0.5 * sqrt (s->lx * s->lx + s->ly * s->ly + s->lz * s->lz);
+  double R = rmax + rh;
+  int M3 = (int) ceil (R / s->lz), M2 = (int) ceil (R / s->ly);
+  int cap = 256, n = 0;
+  cand_t *c = malloc (sizeof (cand_t) * cap);
+  if (!c) return -1;
+  for (int m3 = -M3; m3 <= M3; m3++)
+    for (int m2 = -M2; m2 <= M2; m2++)
+      {
+        double off = m2 * s2x + m3 * s3x;
+        int lo = (int) floor ((-R - off) / s->lx), hi = (int) ceil ((R - off) / s->lx);
+        for (int m1 = lo; m1 <= hi; m1++)
+          {
+            double x = m1 * s->lx + off, y = m2 * s->ly, z = m3 * s->lz;
+            double r = sqrt (x * x + y * y + z * z);
+            if (r > R) continue;
+            if (n == cap)
+              {
+                cap *= 2;
+                cand_t *cc = realloc (c, sizeof (cand_t) * cap);
+                if (!cc) { free (c); return -1; }
+                c = cc;
+              }
+            c[n].v[0] = x; c[n].v[1] = y; c[n].v[2] = z; c[n].v[3] = r;
+            c[n].m[0] = m1; c[n].m[1] = m2; c[n].m[2] = m3; c[n].m[3] = 0;
+            n++;
+          }
+      }
+  qsort (c, n, sizeof (cand_t), cand_cmp);
+  double *cv = malloc (sizeof (double) * 4 * (n > 0 ? n : 1));
+  int *cm = malloc (sizeof (int) * 4 * (n > 0 ? n : 1));
+  if (!cv || !cm) { free (c); free (cv); free (cm); return -1; }
+  int nc = 0;
+  for (int k = 0; k < n; k++)
+    {
+      memcpy (cv + 4 * k, c[k].v, sizeof (double) * 4);
+      memcpy (cm + 4 * k, c[k].m, sizeof (int) * 4);
+      if (c[k].v[3] <= rmax - rh) nc = k + 1;
+    }
+  free (c);
+  *cand_out = cv;
+  *candm_out = cm;
+  *ncore = nc;
+  return n;
+}
+
+/* bitmap of the reference image table (stokes.c:173-233) over
+ * [-rmaxx, rmaxx] x [-rmaxy, rmaxy] x [-rmaxz, rmaxz], z fastest */
+static int
+build_table_bits (const struct stokes *s, unsigned **bits_out, int *tx, int *ty,
+                  int *tz)
+{
+  *tx = s->rmaxx; *ty = s->rmaxy; *tz = s->rmaxz;
+  long nb = (long) (2 * *tx + 1) * (2 * *ty + 1) * (2 * *tz + 1);
+  int words = (int) ((nb + 31) / 32);
+  unsigned *b = calloc (words > 0 ? words : 1, sizeof (unsigned));
+  if (!b) return -1;
+  for (int m = 0; m < s->nr; m++)
+    {
+      int m1 = s->rmx[m], m2 = s->rmy[m], m3 = s->rmz[m];
+      if (m1 < -*tx || m1 > *tx || m2 < -*ty || m2 > *ty || m3 < -*tz || m3 > *tz)
+        continue;
+      long idx = ((long) (m1 + *tx) * (2 * *ty + 1) + (m2 + *ty)) * (2 * *tz + 1)
+        + (m3 + *tz);
+      b[idx >> 5] |= 1u << (idx & 31);
+    }
+  *bits_out = b;
+  return words;
+}
+
 static int
 same_params (const shk_params *a, const shk_params *b)
 {
@@ -260,7 +353,11 @@ stokes_hip_prepare (stokes_ctx *c, int positions_on_device)
   if (!full && !same_params (&p, &c->prm)) full = 1;
   if (full)
     {
-      double *img = NULL, *kv = NULL, *ks = NULL;
+      double *img = NULL, *kv = NULL, *ks = NULL, *cand = NULL;
+      int *candm = NULL;
+      unsigned *tbits = NULL;
+      shk_tables t;
+      memset (&t, 0, sizeof (t));
       if (p.periodic)
         {
           if (p.nr <= 0 || s->rlx == NULL)
@@ -281,11 +378,29 @@ stokes_hip_prepare (stokes_ctx *c, int positions_on_device)
               img[4 * m + 3] = 0.0;
             }
           p.nk = build_kvectors (s, &kv, &ks);
+          t.img = img;
+          t.kv = kv;
+          t.ks = ks;
+          t.s2x = s->shear_mode == 1 ? s->shear_shift : 0.0;
+          t.s3x = s->shear_mode == 2 ? s->shear_shift : 0.0;
+          t.ncand = build_candidates (s, t.s2x, t.s3x, &cand, &candm, &t.ncore);
+          t.cand = cand;
+          t.candm = candm;
+          t.tbits_words = build_table_bits (s, &tbits, &t.tx, &t.ty, &t.tz);
+          t.tbits = tbits;
+          if (t.ncand < 0 || t.tbits_words < 0)
+            {
+              free (img); free (kv); free (ks); free (cand); free (candm);
+              free (tbits);
+              stokes_hip_set_error ("lattice tables: allocation failed");
+              return STOKES_HIP_ENOMEM;
+            }
         }
       else
         p.nk = 0;
-      int rc = shk_dev_set_system (c->dev, &p, s->pos, s->a, img, kv, ks);
-      free (img); free (kv); free (ks);
+      int rc = shk_dev_set_system (c->dev, &p, s->pos, s->a,
+                                   p.periodic ? &t : NULL);
+      free (img); free (kv); free (ks); free (cand); free (candm); free (tbits);
       if (rc) return dev_fail (rc, "upload");
       c->prm = p;
       c->shear_mode = s->shear_mode;
@@ -394,7 +509,10 @@ stokes_hip_memcpy_h2d (void *dst, const void *src, size_t bytes)
 {
   shk_dev *d = stokes_hip_global_dev ();
   if (!d) return STOKES_HIP_ENODEV;
-  int rc = shk_dev_h2d (d, dst, src, bytes);
+  /* the copy may overwrite memory that work queued on a system's stream
+   * still reads: order it after all queued work */
+  int rc = shk_device_sync ();
+  if (!rc) rc = shk_dev_h2d (d, dst, src, bytes);
   if (!rc) rc = shk_dev_sync (d);
   return rc ? dev_fail (rc, "stokes_hip_memcpy_h2d") : 0;
 }
@@ -404,7 +522,9 @@ stokes_hip_memcpy_d2h (void *dst, const void *src, size_t bytes)
 {
   shk_dev *d = stokes_hip_global_dev ();
   if (!d) return STOKES_HIP_ENODEV;
-  int rc = shk_dev_d2h (d, dst, src, bytes);
+  /* results may still be in flight on a system's stream */
+  int rc = shk_device_sync ();
+  if (!rc) rc = shk_dev_d2h (d, dst, src, bytes);
   return rc ? dev_fail (rc, "stokes_hip_memcpy_d2h") : 0;
 }
 

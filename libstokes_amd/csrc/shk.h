@@ -44,13 +44,32 @@ const char *shk_last_error (void);
 shk_dev *shk_dev_create (void);
 void     shk_dev_destroy (shk_dev *d);
 
-/* Upload positions [3np] (AoS), radii [np] or NULL, image table
- * img[nr*4] = (x + shear fold, y, z, 0), k table kv[nk*4] = (k1,k2,k3,|k|)
- * and ks[nk*8] = 2*(ya, yb, yc, yg, yh, ym) at unit radius, kk, kexp
- * (the factor 2 accounts for the paired -k). */
+/* Host-prepared lattice tables of a periodic system:
+ *  img[nr*4]     reference image vectors (x + shear fold, y, z, 0)
+ *  cand[ncand*4] lattice vectors v = m1 a1 + m2 a2 + m3 a3 (a1 = (lx,0,0),
+ *                a2 = (s2x,ly,0), a3 = (s3x,0,lz)) with |v| <= r_max +
+ *                half the cell diagonal, sorted by |v| (x, y, z, |v|); the
+ *                first ncore have |v| <= r_max - half diagonal
+ *  candm[ncand*4] their (m1, m2, m3, 0)
+ *  tbits         bitmap of the reference image table over
+ *                [-tx,tx] x [-ty,ty] x [-tz,tz] (z fastest)
+ *  kv[nk*4]      half-space reciprocal vectors (k1, k2, k3, |k|)
+ *  ks[nk*8]      2*(ya, yb, yc, yg, yh, ym) at unit radius, kk, kexp (the
+ *                factor 2 accounts for the paired -k) */
+typedef struct shk_tables {
+  const double *img;
+  const double *cand;
+  const int *candm;
+  int ncand, ncore;
+  const unsigned *tbits;
+  int tbits_words, tx, ty, tz;
+  double s2x, s3x;
+  const double *kv, *ks;
+} shk_tables;
+
+/* Upload positions [3np] (AoS), radii [np] or NULL and the tables. */
 int shk_dev_set_system (shk_dev *d, const shk_params *p, const double *pos,
-                        const double *a, const double *img, const double *kv,
-                        const double *ks);
+                        const double *a, const shk_tables *t);
 /* Replace only the positions (same np). */
 int shk_dev_set_pos (shk_dev *d, const double *pos, int on_device);
 
@@ -75,6 +94,8 @@ int  shk_dev_h2d (shk_dev *d, void *dst, const void *src, size_t bytes);
 int  shk_dev_d2h (shk_dev *d, void *dst, const void *src, size_t bytes);
 int  shk_dev_d2d (shk_dev *d, void *dst, const void *src, size_t bytes);
 int  shk_dev_sync (shk_dev *d);
+/* wait for all work on the GPU (every stream of every context) */
+int  shk_device_sync (void);
 void *shk_dev_stream (shk_dev *d);
 
 /* per-part device timers of the last operator call (ms): 0 real space,

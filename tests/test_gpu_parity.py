@@ -50,16 +50,10 @@ def _oracle(meta, data):
                      shear_shift=meta["shear_shift"])
 
 
-def _unsupported(meta):
-    return meta["poly"] and meta["box"] is not None
-
-
 def test_golden_atimes_fp64(golden):
     data, cases = golden
     for meta in cases:
         k = meta["name"]
-        if _unsupported(meta):
-            continue
         _say(f"case {k}")
         s = _gpu(meta, data)
         y = s.atimes(data[f"{k}/x"])
@@ -84,15 +78,25 @@ def test_golden_matrix_fp64(golden):
         assert maxrel(m, data[f"{k}/mat"]) <= TOL64, k
 
 
-def test_poly_ewald_matrix_and_unsupported_matvec(golden):
-    """the polydisperse Ewald matvec's reciprocal part is not on the device
-    yet: it must fail loudly, never fall back"""
-    data, cases = golden
-    meta = [m for m in cases if m["name"] == "fix9_poly_FTS"][0]
-    s = _gpu(meta, data)
-    with pytest.raises(ls.StokesError):
-        s.atimes(data["fix9_poly_FTS/x"])
+@pytest.mark.parametrize("version", [0, 1, 2])
+def test_poly_random_radii(version):
+    """polydisperse Ewald (radii 0.5..2, the ensemble workload's range):
+    FP64 against the oracle, MIXED within 1e-6"""
+    rng = np.random.default_rng(30 + version)
+    n = 40
+    a = np.exp(rng.uniform(np.log(0.5), np.log(2.0), n))
+    L = (np.sum(4.0 / 3.0 * np.pi * a ** 3) / 0.25) ** (1.0 / 3.0)
+    pos = rng.uniform(0, L, (n, 3))
+    x = rng.standard_normal(n * ls.dim(version))
+    ref = po.System(version, pos, box=(L,) * 3, tr=3.0, a=a, rmin=1.0).atimes(x)
+    s = ls.Stokes(version, pos, box=(L,) * 3, tr=3.0, a=a, rmin=1.0)
+    e64 = relerr(s.atimes(x), ref)
+    s.set_precision(ls.MIXED)
+    emix = relerr(s.atimes(x), ref)
     s.close()
+    _say(f"poly v{version}: fp64 {e64:.2e} mixed {emix:.2e}")
+    assert e64 <= TOL64
+    assert emix <= TOLMIX
 
 
 @pytest.mark.parametrize("version", [0, 1, 2])
