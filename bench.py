@@ -99,6 +99,30 @@ def cpu_baseline(pos, L, x, tr, budget_s):
                        f"matvec (tr={tr:g}), {t:.1f} s; rate = rows/N/time")
 
 
+def profile_run(args):
+    """kernels for the counter runs: k_copy_calibrate on 256 MiB (a known
+    byte count), then the matvec kernels of both precisions"""
+    n = args.n
+    pos, L = workloads.configuration(n, args.phi, seed=1)
+    x = np.random.default_rng(2).standard_normal(11 * n)
+    nb = 256 << 20
+    a = ls.DeviceArray(nb // 8)
+    b = ls.DeviceArray(nb // 8)
+    ls.check(ls.lib().stokes_hip_copy_calibrate(b.ptr, a.ptr, nb), "copy")
+    posd = ls.DeviceArray.from_host(pos)
+    xd = ls.DeviceArray.from_host(x)
+    yd = ls.DeviceArray(11 * n)
+    tr = args.tr if args.tr else 1000.0
+    for prec in (ls.MIXED, ls.FP64):
+        s = ls.Stokes(2, pos, box=(L, L, L), tr=tr, precision=prec)
+        ls.check(s.L.stokes_hip_set_pos_dev(s.sys, posd.ptr), "set_pos_dev")
+        t = time_matvec(s, xd, yd, args.steps, args.warmup)
+        print(json.dumps({"precision": "mixed" if prec else "fp64", "tr": tr,
+                          "ms_per_matvec": t * 1e3,
+                          "copy_calibration_bytes": nb}), flush=True)
+        s.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -110,7 +134,12 @@ def main():
                     help="Ewald splitting for the tuned runs (default: tune)")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--profile", action="store_true",
+                    help="short run for rocprofv3: a 256 MiB calibration "
+                         "copy, then --steps mixed and FP64 matvecs at --tr")
     args = ap.parse_args()
+    if args.profile:
+        return profile_run(args)
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; "
                           "--gpus must be 1"}))

@@ -38,6 +38,14 @@ enum { STOKES_HIP_FP64 = 0, STOKES_HIP_MIXED = 1 };
 const char *stokes_hip_last_error (void);
 
 int stokes_hip_set_precision (struct stokes *sys, int mode);
+
+/* The reference sums the reciprocal part of the FTS G / G-tilde couplings
+ * (F -> E, S -> U) with sin(k.(x_j - x_i)) where the real-space part
+ * implies the opposite sign (ewald.c:1037-1068 pass sf * yg with
+ * sf = -sin), so those two couplings change with xi.  By default the
+ * library reproduces the reference; enable != 0 uses the other sign, which
+ * makes the FTS sum independent of xi (to the Ewald tolerance). */
+int stokes_hip_set_recip_g_fix (struct stokes *sys, int enable);
 /* record per-part GPU times (ms) into sys->cpu1/cpu2/cpu3 */
 int stokes_hip_set_timing (struct stokes *sys, int enable);
 
@@ -50,6 +58,10 @@ int stokes_hip_memcpy_h2d (void *dst_dev, const void *src, size_t bytes);
 int stokes_hip_memcpy_d2h (void *dst, const void *src_dev, size_t bytes);
 /* wait for all work queued for sys */
 int stokes_hip_sync (struct stokes *sys);
+/* copy `bytes` (a multiple of 32) between device buffers with a plain
+ * streaming kernel; a known-size reference for hardware byte counters */
+int stokes_hip_copy_calibrate (void *dst_dev, const void *src_dev,
+                               size_t bytes);
 
 /* take positions [3*np] from device memory for the following calls
  * (instead of re-reading sys->pos on every call) until the next call of

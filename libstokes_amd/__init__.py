@@ -45,8 +45,10 @@ EXPORTS = [
     "chebyshev_error_minvsqrt", "chebyshev_error_Rsqrt",
     # stokes_hip.h
     "stokes_hip_last_error", "stokes_hip_set_precision",
+    "stokes_hip_set_recip_g_fix",
     "stokes_hip_set_timing", "stokes_hip_malloc", "stokes_hip_free",
     "stokes_hip_memcpy_h2d", "stokes_hip_memcpy_d2h", "stokes_hip_sync",
+    "stokes_hip_copy_calibrate",
     "stokes_hip_set_pos_dev", "stokes_hip_use_host_pos",
     "stokes_hip_atimes_dev", "stokes_hip_atimes", "stokes_hip_make_matrix_dev",
     "stokes_hip_make_matrix", "stokes_hip_extmat_dev",
@@ -120,7 +122,8 @@ def lib():
     L.stokes_unset_radius.argtypes = [sp]
     L.stokes_get_shear_shift.restype = C.c_double
     L.stokes_hip_last_error.restype = C.c_char_p
-    for f in ("stokes_hip_set_precision", "stokes_hip_set_timing"):
+    for f in ("stokes_hip_set_precision", "stokes_hip_set_timing",
+              "stokes_hip_set_recip_g_fix"):
         getattr(L, f).argtypes = [sp, C.c_int]
     L.stokes_hip_atimes.argtypes = [sp, C.c_int, _dp, _dp]
     L.stokes_hip_atimes_dev.argtypes = [sp, C.c_int, _vp, _vp]
@@ -136,6 +139,7 @@ def lib():
     L.stokes_hip_memcpy_h2d.argtypes = [_vp, _vp, C.c_size_t]
     L.stokes_hip_memcpy_d2h.argtypes = [_vp, _vp, C.c_size_t]
     L.stokes_hip_sync.argtypes = [sp]
+    L.stokes_hip_copy_calibrate.argtypes = [_vp, _vp, C.c_size_t]
     L.stokes_hip_set_pos_dev.argtypes = [sp, _vp]
     L.stokes_hip_use_host_pos.argtypes = [sp]
     L.chebyshev_eval.restype = C.c_double
@@ -239,6 +243,10 @@ class Stokes:
     def set_precision(self, mode):
         check(self.L.stokes_hip_set_precision(self.sys, mode),
               "set_precision")
+
+    def set_recip_g_fix(self, on=True):
+        check(self.L.stokes_hip_set_recip_g_fix(self.sys, 1 if on else 0),
+              "set_recip_g_fix")
 
     def set_timing(self, on=True):
         check(self.L.stokes_hip_set_timing(self.sys, 1 if on else 0),

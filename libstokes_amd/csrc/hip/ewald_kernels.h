@@ -1,15 +1,12 @@
 // ewald_kernels.h -- device kernels of the Ewald/RPY mobility operator.
 //
-//   k_pairs     real-space pair sum (all pairs x periodic images inside
-//               r_max), FP64 or FP32, moment accumulation per pair and one
-//               11x11 contraction per pair; ewald.c:649-789.
-//   k_recip_fwd structure factors C(k) = sum_j cos(k.x_j) z_j and
-//               S(k) = sum_j sin(k.x_j) z_j and the per-k blocks
-//               P = Ac C - As S, Q = Ac S + As C; ewald.c:923-1075 recast.
-//   k_finalize  y_i = self_i z_i + sum_k cos(k.x_i) P + sin(k.x_i) Q
-//               + sum over j-chunks of the real-space partials.
-//   k_matrix    dense (11N)^2 extracted-form matrix, ewald.c:1092-1677.
-//   k_extmat    y = mat . T(x), matrix.c:447-478.
+//   k_pairs       pair sum with per-pair moment accumulation and one 11x11
+//                 contraction per pair: the open-boundary RPY operator
+//                 (non-ewald.c:330-493), KIND 1.
+//   kblock_*      per-k reciprocal blocks shared with recip_kernels.h.
+//   k_matrix      dense (11N)^2 extracted-form matrix, ewald.c:1092-1677.
+//   k_gemv_rows   y = mat . T(x), matrix.c:447-478.
+// The periodic matvec lives in real_kernels.h and recip_kernels.h.
 #pragma once
 #include <hip/hip_runtime.h>
 #include "pair_math.h"
@@ -21,6 +18,7 @@ struct DevParams {
   int poly;
   double xi, rmax2, rmin;
   double self_a, self_c, self_m, xiaspi, xia2;
+  double gsign;   // sign of the reciprocal G term (+1: as the reference)
 };
 
 // self scalars of particle i (ewald.c:540-643)
@@ -192,11 +190,11 @@ __device__ __forceinline__ void kblock_moments (const double4 kvec, const double
 // acting on the q = 0 sums (sc1 even / ss1 odd) and on the q = 2 sums
 // (sc2 / ss2), in the order xa ya yb xc yc xg yg yh xm ym zm.
 __device__ __forceinline__ void poly_kscalars (int p, double k, double kk, double kexp,
-                                               double *sc1, double *ss1, double *sc2,
-                                               double *ss2)
+                                               double gsign, double *sc1, double *ss1,
+                                               double *sc2, double *ss2)
 {
   for (int t = 0; t < 11; t++) sc1[t] = ss1[t] = sc2[t] = ss2[t] = 0.0;
-  const double e = 2.0 * kexp;
+  const double e = 2.0 * kexp, eg = gsign * e;
   switch (p)
     {
     case 1:                                   // ya: 6 a_i - kk a_i a_j^2
@@ -205,8 +203,8 @@ __device__ __forceinline__ void poly_kscalars (int p, double k, double kk, doubl
       break;
     case 2:                                   // yb; yg: 3k a_i^2 - 0.5 kk k a_i^2 a_j^2
       ss1[2] = 3.0 * k * e;
-      ss1[6] = 3.0 * k * e;
-      ss2[6] = -0.5 * kk * k * e;
+      ss1[6] = 3.0 * k * eg;
+      ss2[6] = -0.5 * kk * k * eg;
       break;
     case 3:                                   // ya, yc, yh, ym
       sc1[1] = -kk * e;
@@ -216,7 +214,7 @@ __device__ __forceinline__ void poly_kscalars (int p, double k, double kk, doubl
       sc2[9] = -0.3 * kk * kk * e;
       break;
     case 4:                                   // yg: -0.3 kk k a_i^4
-      ss1[6] = -0.3 * kk * k * e;
+      ss1[6] = -0.3 * kk * k * eg;
       break;
     default:                                  // ym: -0.3 kk^2 a_i^5
       sc1[9] = -0.3 * kk * kk * e;
@@ -246,149 +244,6 @@ __device__ __forceinline__ void kblock_apply (const double *Mc, const double *Ms
       P[pr] += a;
       Q[pr] += b;
     }
-}
-
-// NPOW = 1: monodisperse, PQ[k][2][D]; NPOW = 5: polydisperse, PQ[k][5][2][D]
-// holding the coefficients of a_i^1 .. a_i^5.
-template <int V, typename T, int NPOW>
-__global__ __launch_bounds__ (256) void
-k_recip_fwd (const double4 *__restrict__ pos, const double *__restrict__ x,
-             const double4 *__restrict__ kv, const double *__restrict__ ks,
-             int np, double *__restrict__ PQ)
-{
-  constexpr int D = Dim<V>::D, NM = Dim<V>::NMOM;
-  constexpr int NS = (NPOW > 1) ? 4 : 2;      // C S [C2 S2]
-  __shared__ double red[256];
-  __shared__ double CS[NS * D];
-  const int k = blockIdx.x;
-  const double4 kk = kv[k];
-  T A[NS * D];
-#pragma unroll
-  for (int q = 0; q < NS * D; q++) A[q] = T (0);
-  for (int j = threadIdx.x; j < np; j += blockDim.x)
-    {
-      const double4 pj = pos[j];
-      T c, s;
-      phase_cs (kk.x * pj.x + kk.y * pj.y + kk.z * pj.z, c, s);
-      T z[D];
-      load_z<V, T> (x, j, z);
-#pragma unroll
-      for (int q = 0; q < D; q++) { A[q] += c * z[q]; A[D + q] += s * z[q]; }
-      if (NPOW > 1)
-        {
-          const T a2 = T (pj.w * pj.w);
-#pragma unroll
-          for (int q = 0; q < D; q++)
-            { A[2 * D + q] += a2 * c * z[q]; A[3 * D + q] += a2 * s * z[q]; }
-        }
-    }
-  // block reduction, one component at a time (deterministic order)
-#pragma unroll
-  for (int q = 0; q < NS * D; q++)
-    {
-      red[threadIdx.x] = double (A[q]);
-      __syncthreads ();
-      for (int st = blockDim.x / 2; st > 0; st >>= 1)
-        {
-          if (threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
-          __syncthreads ();
-        }
-      if (threadIdx.x == 0) CS[q] = red[0];
-      __syncthreads ();
-    }
-  if (threadIdx.x != 0) return;
-  const double *w = ks + (size_t) k * 8;
-  double Mc[NM], Ms[NM];
-  if (NPOW == 1)
-    {
-      double P[D], Q[D];
-#pragma unroll
-      for (int q = 0; q < D; q++) P[q] = Q[q] = 0.0;
-      kblock_moments<V, double> (kk, w, Mc, Ms);
-      kblock_apply<V> (Mc, Ms, CS, CS + D, P, Q);
-#pragma unroll
-      for (int q = 0; q < D; q++)
-        {
-          PQ[((size_t) k * 2 + 0) * D + q] = P[q];
-          PQ[((size_t) k * 2 + 1) * D + q] = Q[q];
-        }
-      return;
-    }
-  const double ik = 1.0 / kk.w;
-  const double ex = kk.x * ik, ey = kk.y * ik, ez = kk.z * ik;
-  for (int p = 1; p <= NPOW; p++)
-    {
-      double sc1[11], ss1[11], sc2[11], ss2[11];
-      poly_kscalars (p, kk.w, w[6], w[7], sc1, ss1, sc2, ss2);
-      double P[D], Q[D];
-#pragma unroll
-      for (int q = 0; q < D; q++) P[q] = Q[q] = 0.0;
-#pragma unroll
-      for (int t = 0; t < NM; t++) Mc[t] = Ms[t] = 0.0;
-      moments_add<V, double> (Mc, ex, ey, ez, sc1);
-      moments_add<V, double> (Ms, ex, ey, ez, ss1);
-      kblock_apply<V> (Mc, Ms, CS, CS + D, P, Q);
-#pragma unroll
-      for (int t = 0; t < NM; t++) Mc[t] = Ms[t] = 0.0;
-      moments_add<V, double> (Mc, ex, ey, ez, sc2);
-      moments_add<V, double> (Ms, ex, ey, ez, ss2);
-      kblock_apply<V> (Mc, Ms, CS + 2 * D, CS + 3 * D, P, Q);
-      double *out = PQ + (((size_t) k * NPOW + (p - 1)) * 2) * D;
-#pragma unroll
-      for (int q = 0; q < D; q++) { out[q] = P[q]; out[D + q] = Q[q]; }
-    }
-}
-
-template <int V, typename T, int NPOW>
-__global__ __launch_bounds__ (256) void
-k_finalize (const double4 *__restrict__ pos, const double *__restrict__ x,
-            const double4 *__restrict__ kv, const double *__restrict__ PQ,
-            const double *__restrict__ partial, int nchunk, DevParams p,
-            int periodic, double *__restrict__ y)
-{
-  constexpr int D = Dim<V>::D, NM = Dim<V>::NMOM;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= p.np) return;
-  const double4 pi = pos[i];
-  double acc[D];
-#pragma unroll
-  for (int q = 0; q < D; q++) acc[q] = 0.0;
-  {
-    double s[11], M[NM];
-#pragma unroll
-    for (int k = 0; k < NM; k++) M[k] = 0.0;
-    self_scalars<double> (p, pi.w, periodic != 0, s);
-    moments_add<V, double> (M, 0.0, 0.0, 0.0, s);
-    double z[D];
-    load_z<V, double> (x, i, z);
-    block_apply<V, double, double> (M, z, acc);
-  }
-  double apow[NPOW];
-  apow[0] = NPOW > 1 ? pi.w : 1.0;
-#pragma unroll
-  for (int t = 1; t < NPOW; t++) apow[t] = apow[t - 1] * pi.w;
-  for (int k = 0; k < p.nk; k++)
-    {
-      const double4 kk = kv[k];
-      T c, s;
-      phase_cs (kk.x * pi.x + kk.y * pi.y + kk.z * pi.z, c, s);
-#pragma unroll
-      for (int t = 0; t < NPOW; t++)
-        {
-          const double *P = PQ + ((size_t) k * NPOW + t) * 2 * D;
-          const double cc = apow[t] * double (c), ss = apow[t] * double (s);
-#pragma unroll
-          for (int q = 0; q < D; q++) acc[q] += cc * P[q] + ss * P[D + q];
-        }
-    }
-  for (int ch = 0; ch < nchunk; ch++)
-    {
-      const double *pp = partial + ((size_t) ch * p.np + i) * D;
-#pragma unroll
-      for (int q = 0; q < D; q++) acc[q] += pp[q];
-    }
-#pragma unroll
-  for (int q = 0; q < D; q++) y[(size_t) i * D + q] = acc[q];
 }
 
 // Dense build: one thread per (i, j) block.
@@ -444,7 +299,7 @@ k_matrix (const double4 *__restrict__ pos, const double4 *__restrict__ img,
               ya = aa * 6.0 * (1.0 - kk2 * (aa2 + ab2) / 6.0) * kexp2;
               yb = aa2 * 3.0 * kn * kexp2;
               yc = aa3 * 1.5 * kk2 * kexp2;
-              yg = aa2 * 3.0 * (1.0 - kk2 * (aa2 / 10.0 + ab2 / 6.0)) * kn * kexp2;
+              yg = p.gsign * aa2 * 3.0 * (1.0 - kk2 * (aa2 / 10.0 + ab2 / 6.0)) * kn * kexp2;
               yh = aa3 * 1.5 * kk2 * kexp2;
               ym = aa3 * 3.0 * (1.0 - kk2 * (aa2 + ab2) / 10.0) * kk2 * kexp2;
             }

@@ -86,7 +86,7 @@ k_recip_sf (const double4 *__restrict__ pos, const double *__restrict__ x,
 template <int V, typename T, int NPOW>
 __global__ __launch_bounds__ (128) void
 k_recip_blocks (const double4 *__restrict__ kv, const double *__restrict__ ks, int nk,
-                int nsplit, const T *__restrict__ part, T *__restrict__ PQ)
+                int nsplit, double gsign, const T *__restrict__ part, T *__restrict__ PQ)
 {
   constexpr int D = Dim<V>::D, NM = Dim<V>::NMOM;
   constexpr int NS = (NPOW > 1) ? 4 : 2;
@@ -121,7 +121,7 @@ k_recip_blocks (const double4 *__restrict__ kv, const double *__restrict__ ks, i
   for (int pw = 1; pw <= NPOW; pw++)
     {
       double sc1[11], ss1[11], sc2[11], ss2[11];
-      poly_kscalars (pw, kk.w, w[6], w[7], sc1, ss1, sc2, ss2);
+      poly_kscalars (pw, kk.w, w[6], w[7], gsign, sc1, ss1, sc2, ss2);
       double P[D], Q[D];
 #pragma unroll
       for (int q = 0; q < D; q++) P[q] = Q[q] = 0.0;

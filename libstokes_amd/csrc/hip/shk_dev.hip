@@ -203,6 +203,7 @@ static DevParams dev_params (const shk_params &p)
   q.xi = p.xi; q.rmax2 = p.rmax2; q.rmin = p.rmin;
   q.self_a = p.self_a; q.self_c = p.self_c; q.self_m = p.self_m;
   q.xiaspi = p.xiaspi; q.xia2 = p.xia2;
+  q.gsign = p.gsign;
   return q;
 }
 
@@ -360,8 +361,8 @@ static int recip_launch (shk_dev *d, const double *x, int &nrecip, const void *&
                       d->stream, pos4, x, kv, p.np, nk, jper, (T *) d->kpart.p);
   HIPCHK (hipGetLastError ());
   hipLaunchKernelGGL ((k_recip_blocks<V, T, NPOW>), dim3 ((nk + 127) / 128), dim3 (128), 0,
-                      d->stream, kv, (const double *) d->ks.p, nk, sj, (const T *) d->kpart.p,
-                      (T *) d->kpq.p);
+                      d->stream, kv, (const double *) d->ks.p, nk, sj, p.gsign,
+                      (const T *) d->kpart.p, (T *) d->kpq.p);
   HIPCHK (hipGetLastError ());
   hipLaunchKernelGGL ((k_recip_back<V, T, NPOW>), dim3 ((p.np + 255) / 256, sk), dim3 (256), 0,
                       d->stream, pos4, kv, (const T *) d->kpq.p, p.np, nk, kper,
@@ -620,5 +621,25 @@ extern "C" int shk_dev_scal (shk_dev *d, double *dst, const double *src, double 
   hipLaunchKernelGGL (k_scal, dim3 ((unsigned) ((len + 255) / 256)), dim3 (256), 0, d->stream,
                       dst, src, alpha, len);
   HIPCHK (hipGetLastError ());
+  return SHK_OK;
+}
+
+// plain streaming copy of known size: the reference point that converts
+// hardware fetch/write counters into bytes for the roofline report
+__global__ __launch_bounds__ (256) void
+k_copy_calibrate (const double4 *__restrict__ src, double4 *__restrict__ dst, size_t n4)
+{
+  for (size_t i = (size_t) blockIdx.x * blockDim.x + threadIdx.x; i < n4;
+       i += (size_t) gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
+extern "C" int shk_dev_copy_calibrate (shk_dev *d, void *dst, const void *src, size_t bytes)
+{
+  const size_t n4 = bytes / sizeof (double4);
+  hipLaunchKernelGGL (k_copy_calibrate, dim3 (2048), dim3 (256), 0, d->stream,
+                      (const double4 *) src, (double4 *) dst, n4);
+  HIPCHK (hipGetLastError ());
+  HIPCHK (hipStreamSynchronize (d->stream));
   return SHK_OK;
 }

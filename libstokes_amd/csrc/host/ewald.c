@@ -166,7 +166,8 @@ stokes_hip_global_dev (void)
  * with -k (identical contribution: cos is even, and the odd blocks B, G
  * come with the odd factor sin), so the weights carry a factor 2. */
 static int
-build_kvectors (const struct stokes *s, double **kv_out, double **ks_out)
+build_kvectors (const struct stokes *s, double gsign, double **kv_out,
+                double **ks_out)
 {
   int cap = 64, nk = 0;
   double *kv = malloc (sizeof (double) * 4 * cap);
@@ -204,7 +205,7 @@ build_kvectors (const struct stokes *s, double **kv_out, double **ks_out)
           w[0] = 2.0 * 6.0 * (1.0 - kk / 3.0) * kexp;              /* ya */
           w[1] = 2.0 * 3.0 * k * kexp;                             /* yb */
           w[2] = 2.0 * 3.0 / 2.0 * kk * kexp;                      /* yc */
-          w[3] = 2.0 * 3.0 * (1.0 - 4.0 / 15.0 * kk) * k * kexp;   /* yg */
+          w[3] = gsign * 2.0 * 3.0 * (1.0 - 4.0 / 15.0 * kk) * k * kexp; /* yg */
           w[4] = 2.0 * 3.0 / 2.0 * kk * kexp;                      /* yh */
           w[5] = 2.0 * 3.0 * (1.0 - kk / 5.0) * kk * kexp;         /* ym */
           w[6] = kk;
@@ -345,6 +346,7 @@ stokes_hip_prepare (stokes_ctx *c, int positions_on_device)
   p.rmax2 = s->rmax2; p.rmin = s->rmin;
   p.self_a = s->self_a; p.self_c = s->self_c; p.self_m = s->self_m;
   p.nr = p.periodic ? s->nr : 0;
+  p.gsign = c->fix_recip_g ? -1.0 : 1.0;
   int full = !c->valid || c->shear_mode != s->shear_mode
     || c->shear_shift != s->shear_shift || p.poly
     || c->a_ptr != s->a;
@@ -377,7 +379,7 @@ stokes_hip_prepare (stokes_ctx *c, int positions_on_device)
               img[4 * m + 2] = s->rlz[m];
               img[4 * m + 3] = 0.0;
             }
-          p.nk = build_kvectors (s, &kv, &ks);
+          p.nk = build_kvectors (s, p.gsign, &kv, &ks);
           t.img = img;
           t.kv = kv;
           t.ks = ks;
@@ -463,6 +465,16 @@ ctx_or_error (struct stokes *sys)
 }
 
 int
+stokes_hip_set_recip_g_fix (struct stokes *sys, int enable)
+{
+  stokes_ctx *c = ctx_or_error (sys);
+  if (!c) return STOKES_HIP_ENODEV;
+  c->fix_recip_g = enable ? 1 : 0;
+  c->valid = 0;
+  return 0;
+}
+
+int
 stokes_hip_set_precision (struct stokes *sys, int mode)
 {
   stokes_ctx *c = ctx_or_error (sys);
@@ -526,6 +538,20 @@ stokes_hip_memcpy_d2h (void *dst, const void *src, size_t bytes)
   int rc = shk_device_sync ();
   if (!rc) rc = shk_dev_d2h (d, dst, src, bytes);
   return rc ? dev_fail (rc, "stokes_hip_memcpy_d2h") : 0;
+}
+
+int
+stokes_hip_copy_calibrate (void *dst, const void *src, size_t bytes)
+{
+  shk_dev *d = stokes_hip_global_dev ();
+  if (!d) return STOKES_HIP_ENODEV;
+  if (bytes % 32)
+    {
+      stokes_hip_set_error ("copy size must be a multiple of 32 bytes");
+      return STOKES_HIP_EARG;
+    }
+  int rc = shk_dev_copy_calibrate (d, dst, src, bytes);
+  return rc ? dev_fail (rc, "stokes_hip_copy_calibrate") : 0;
 }
 
 int

@@ -11,6 +11,7 @@ typedef struct stokes_ctx {
   struct stokes *sys;
   shk_dev *dev;
   int prec;                 /* SHK_FP64 / SHK_MIXED */
+  int fix_recip_g;          /* reciprocal G term with the xi-independent sign */
   int timing;
   int valid;                /* parameters uploaded and current */
   shk_params prm;           /* last uploaded parameters */

@@ -35,6 +35,7 @@ typedef struct shk_params {
   double self_a, self_c, self_m;
   int nr;               /* real-space images */
   int nk;               /* reciprocal half-space vectors (k and -k paired) */
+  double gsign;         /* sign of the reciprocal G term (+1: as the reference) */
 } shk_params;
 
 typedef struct shk_dev shk_dev;
@@ -107,6 +108,9 @@ double shk_dev_last_ms (shk_dev *d, int which);
 int shk_dev_axpby4 (shk_dev *d, double *b2, const double *x, const double *b1,
                     const double *b0, const double *y, double ca, double cb,
                     double cy, size_t len);
+/* plain device copy of `bytes` (a multiple of 32) by a streaming kernel */
+int shk_dev_copy_calibrate (shk_dev *d, void *dst, const void *src,
+                            size_t bytes);
 /* dst = alpha * src (src == NULL: dst = 0), len doubles */
 int shk_dev_scal (shk_dev *d, double *dst, const double *src, double alpha,
                   size_t len);

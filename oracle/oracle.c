@@ -653,7 +653,8 @@ orc_ewald_visit (const orc_sys *s, int i0, int i1, orc_visit_fn visit,
                     sc[0] = 0.0; sc[1] = cf * y6[0];
                     sc[2] = sf * y6[1];
                     sc[3] = 0.0; sc[4] = cf * y6[2];
-                    sc[5] = 0.0; sc[6] = sf * y6[3];
+                    sc[5] = 0.0;
+                    sc[6] = (s->fix_recip_g ? -sf : sf) * y6[3];
                     sc[7] = cf * y6[4];
                     sc[8] = 0.0; sc[9] = cf * y6[5]; sc[10] = 0.0;
                     visit (ctx, i, j, ex, ey, ez, sc);

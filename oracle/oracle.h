@@ -39,6 +39,9 @@ typedef struct orc_sys {
 
   int shear_mode;       /* 0 none, 1 x-flow/y-grad, 2 x-flow/z-grad */
   double shear_shift;
+
+  int fix_recip_g;      /* 1: reciprocal G / G-tilde with the sign that makes
+                           the FTS sum xi-independent (0: as the reference) */
 } orc_sys;
 
 /* set-up (stokes.c) */

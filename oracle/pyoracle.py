@@ -35,6 +35,7 @@ class OrcSys(C.Structure):
         ("nr", C.c_int), ("rlx", _dp), ("rly", _dp), ("rlz", _dp),
         ("rmx", _ip), ("rmy", _ip), ("rmz", _ip),
         ("shear_mode", C.c_int), ("shear_shift", C.c_double),
+        ("fix_recip_g", C.c_int),
     ]
 
 
@@ -117,7 +118,8 @@ class System:
     """Oracle-side system: holds numpy buffers alive for the orc_sys."""
 
     def __init__(self, version, pos, box=None, xi=None, tr=None, eps=1e-12,
-                 a=None, rmin=0.0, shear_mode=0, shear_shift=0.0):
+                 a=None, rmin=0.0, shear_mode=0, shear_shift=0.0,
+                 fix_recip_g=False):
         L = lib()
         self.pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1)
         self.np = self.pos.size // 3
@@ -131,6 +133,7 @@ class System:
         s.rmin = rmin
         s.shear_mode = shear_mode
         s.shear_shift = shear_shift
+        s.fix_recip_g = 1 if fix_recip_g else 0
         if box is not None:
             s.periodic = 1
             L.orc_set_l(C.byref(s), *[float(b) for b in box])

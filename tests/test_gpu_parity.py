@@ -114,6 +114,26 @@ def test_c1_n64_fp64_and_mixed(version):
     assert emix <= TOLMIX
 
 
+def test_recip_g_fix_mode():
+    """corrected reciprocal G sign: GPU == oracle in that mode, and the FTS
+    result no longer depends on xi (FP64 and MIXED)"""
+    pos, L = workloads.configuration(64, 0.2, seed=15)
+    x = np.random.default_rng(16).standard_normal(64 * 11)
+    ys = {}
+    for tr in (1.0, 100.0):
+        s = ls.Stokes(2, pos, box=(L,) * 3, tr=tr)
+        s.set_recip_g_fix(True)
+        ys[tr] = s.atimes(x)
+        s.set_precision(ls.MIXED)
+        ys[(tr, "mix")] = s.atimes(x)
+        s.close()
+    ref = po.System(2, pos, box=(L,) * 3, tr=1.0, fix_recip_g=True).atimes(x)
+    assert relerr(ys[1.0], ref) <= TOL64
+    assert relerr(ys[100.0], ys[1.0]) <= 1e-10
+    assert relerr(ys[(100.0, "mix")], ys[1.0]) <= TOLMIX
+    assert relerr(ys[(1.0, "mix")], ys[1.0]) <= TOLMIX
+
+
 def test_determinism_bitwise():
     pos, L = workloads.configuration(200, 0.2, seed=3)
     x = np.random.default_rng(4).standard_normal(200 * 11)

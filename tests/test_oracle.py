@@ -195,6 +195,31 @@ def test_xi_independence_fts(golden):
             assert np.abs(other[key] - y0).max() < 1e-11 * scale, (tr, key)
 
 
+def test_corrected_recip_g_is_xi_independent(golden):
+    """With the reciprocal G / G-tilde sign flipped (fix_recip_g), every
+    FTS coupling is independent of xi: this pins which sign is wrong in
+    the reference, and that nothing else is."""
+    data, _ = golden
+    pos = data["fix9_FTS/pos"]
+    x = np.random.default_rng(13).standard_normal(99)
+    ys = [po.System(2, pos, box=(FIX9_L,) * 3, tr=tr,
+                    fix_recip_g=True).atimes(x)
+          for tr in (1.0, 10.0, 100.0, 1e3)]
+    for y in ys[1:]:
+        assert relerr(y, ys[0]) < 1e-11
+    # only the G / G-tilde couplings differ from the reference mode
+    b_ref = _fts_blocks(pos, (FIX9_L,) * 3, 1.0, 14)
+    rng = np.random.default_rng(14)
+    for name, s in (("F", slice(0, 3)), ("T", slice(3, 6)), ("S", slice(6, 11))):
+        xx = np.zeros((9, 11))
+        xx[:, s] = rng.standard_normal((9, s.stop - s.start))
+        y = po.System(2, pos, box=(FIX9_L,) * 3, tr=1.0,
+                      fix_recip_g=True).atimes(xx.ravel()).reshape(9, 11)
+        for oname, os_ in (("U", slice(0, 3)), ("O", slice(3, 6)), ("E", slice(6, 11))):
+            same = np.abs(y[:, os_] - b_ref[(name, oname)]).max() < 1e-13
+            assert same == ((name, oname) not in (("F", "E"), ("S", "U")))
+
+
 def test_reference_fts_g_block_depends_on_xi(golden):
     """Documented property of the reference (reproduced by the oracle,
     which equals the compiled reference to 1e-13): the F -> E and S -> U
