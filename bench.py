@@ -134,6 +134,8 @@ def main():
                     help="Ewald splitting for the tuned runs (default: tune)")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--batch", type=int, default=32,
+                    help="right-hand sides per launch for the batched figures")
     ap.add_argument("--profile", action="store_true",
                     help="short run for rocprofv3: a 256 MiB calibration "
                          "copy, then --steps mixed and FP64 matvecs at --tr")
@@ -166,7 +168,8 @@ def main():
 
     # splitting: tuned on the FP64 and mixed paths (results are xi-independent
     # to the Ewald tolerance)
-    trs = [args.tr] if args.tr else [1.0, 10.0, 100.0, 1000.0]
+    trs = [args.tr] if args.tr else [10.0, 100.0, 300.0, 1000.0, 3000.0,
+                                     1e4]
     best = None
     for tr in trs:
         s = make(tr, ls.MIXED)
@@ -185,6 +188,18 @@ def main():
     ymix = yd.to_host()
     err = float(np.linalg.norm(ymix - y64) / np.linalg.norm(y64))
     err_ref_split = float(np.linalg.norm(y64 - y64_ref) / np.linalg.norm(y64_ref))
+
+    # batched right-hand sides (one launch per block of columns)
+    nb = args.batch
+    xb = ls.DeviceArray.from_host(
+        np.random.default_rng(3).standard_normal(nb * 11 * n))
+    yb = ls.DeviceArray(nb * 11 * n)
+    tb_mix = time_matvec(smix, xb, yb, max(3, args.steps // 4), 1, nrhs=nb)
+    yb_mix = yb.to_host()
+    tb_64 = time_matvec(s64, xb, yb, max(3, args.steps // 4), 1, nrhs=nb)
+    yb_64 = yb.to_host()
+    err_b = float(np.linalg.norm(yb_mix - yb_64) / np.linalg.norm(yb_64))
+    log(f"batch {nb}: mixed {tb_mix * 1e3:.3f} ms, fp64 {tb_64 * 1e3:.3f} ms")
 
     # roofline of the dominant kernel (mixed real-space pairs)
     real_ms, recip_ms = kernel_times(smix, xd, yd, 5)
@@ -217,6 +232,10 @@ def main():
         "fp64_matvecs_per_s_tr1": 1.0 / t64_ref,
         "mixed_over_fp64": t64 / tmix,
         "rel_err_fp64_tuned_vs_tr1": err_ref_split,
+        "batched": {"nrhs": nb, "mixed_matvecs_per_s": nb / tb_mix,
+                    "fp64_matvecs_per_s": nb / tb_64,
+                    "mixed_over_fp64": tb_64 / tb_mix,
+                    "rel_err_mixed_vs_fp64": err_b},
         "roofline": roof,
     }
     if not args.no_cpu:

@@ -45,6 +45,20 @@ __device__ __forceinline__ void self_scalars (const DevParams &p, double ai,
   s[8] = s[9] = s[10] = T (sm);
 }
 
+// The self block (e = 0) acts on the extracted vector as a diagonal: with
+// a1 = self_a, c1 = self_c and mm3 = -self_m/3, mm5 = self_m/2 the
+// transformed stresslet gives E = self_m S component by component
+// (ewald.c:540-643 with fts.c:625-1010 at e = 0).
+__device__ __forceinline__ double self_diag (const DevParams &p, double ai, bool periodic,
+                                             int V, int q)
+{
+  double s[11];
+  self_scalars<double> (p, ai, periodic, s);
+  if (q < 3) return s[1];
+  if (q < 6 && V > 0) return s[4];
+  return s[10];
+}
+
 template <int V, typename T>
 __device__ __forceinline__ void load_z (const double *x, int j, T *z)
 {

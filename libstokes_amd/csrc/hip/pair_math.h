@@ -34,113 +34,133 @@ SHK_HD double m_exp (double x)  { return exp (x); }
 SHK_HD float  m_exp (float x)   { return expf (x); }
 SHK_HD double m_sqrt (double x) { return sqrt (x); }
 SHK_HD float  m_sqrt (float x)  { return sqrtf (x); }
+SHK_HD double m_rsqrt (double x) { return 1.0 / sqrt (x); }
+SHK_HD float  m_rsqrt (float x)  { return rsqrtf (x); }
 
 template <typename T> SHK_HD T inv_sqrt_pi () { return T (0.56418958354775628695); }
 
 // ---------------------------------------------------------------------------
 // real-space Ewald scalars, monodisperse (ewald.c:48-186)
 // ---------------------------------------------------------------------------
+// `u` is 1/r, supplied by the caller (the pair kernels have it already):
+// every division of the reference's expressions becomes a product with a
+// power of u, which keeps the scalar evaluation to one reciprocal.
+template <int V, typename T>
+SHK_HD void scalars_real_u (T xi, T r, T u, T *o)
+{
+  const T xr = xi * r, x2 = xr * xr;
+  const T u2 = u * u, u3 = u2 * u;
+  const T ec = m_erfc (xr);
+  const T ex = xi * inv_sqrt_pi<T> () * m_exp (-x2);
+  o[1] = (T (0.75) + T (0.5) * u2) * u * ec
+    + ((T (1) + x2 * (T (14) + T (4) * x2 * (T (-5) + x2))) * u2 - T (4.5) + T (3) * x2) * ex;
+  const T a2 = (T (0.75) - T (1.5) * u2) * u * ec
+    + ((T (-3) + x2 * (T (-2) + T (4) * x2 * (T (4) - x2))) * u2 + T (1.5) - T (3) * x2) * ex;
+  o[0] = a2 + o[1];
+  if (V == 0) return;
+  o[2] = T (-0.75) * u2 * ec - T (1.5) * (T (1) + x2 * (T (-6) + x2 * T (2))) * u * ex;
+  o[4] = T (-0.375) * u3 * ec
+    - T (0.75) * (T (1) + x2 * (T (14) + x2 * (T (-20) + x2 * T (4)))) * u2 * ex;
+  const T c2 = T (1.125) * u3 * ec
+    - T (0.75) * (T (-3) + x2 * (T (-2) + x2 * (T (16) + x2 * T (-4)))) * u2 * ex;
+  o[3] = c2 + o[4];
+  if (V == 1) return;
+  o[5] = (T (2.25) - T (3.6) * u2) * u2 * ec
+    + (T (-1.5) * (T (-3) + x2 * T (6))
+       - T (0.8) * (T (9) + x2 * (T (6) + x2 * (T (-48) + x2 * T (12)))) * u2) * u * ex;
+  o[6] = T (1.2) * u2 * u2 * ec
+    + (T (-3) * (x2 * (T (2) - x2))
+       - T (0.8) * (T (-3) + x2 * (T (-2) + x2 * (T (-26) + x2 * (T (26) + x2 * T (-4))))) * u2)
+      * u * ex;
+  o[7] = T (-1.125) * u3 * ec
+    + T (1.5) * (T (-1.5) + x2 * (T (-1) + x2 * (T (8) + x2 * T (-2)))) * u2 * ex;
+  o[8] = (T (-4.5) + T (10.8) * u2) * u3 * ec
+    + (T (1.5) * (T (-6) + x2 * (T (-12) + x2 * T (12)))
+       + T (1.2) * (T (18) + x2 * (T (12) + x2 * (T (30) + x2 * (T (-66) + x2 * T (12))))) * u2)
+      * u2 * ex;
+  o[9] = (T (2.25) - T (7.2) * u2) * u3 * ec
+    + (T (-1.5) * (T (-3) + x2 * (T (6) + x2 * (T (-12) + x2 * T (4))))
+       - T (1.2) * (T (12) + x2 * (T (8) + x2 * (T (-22) + x2 * (T (58) + x2 * (T (-34) + x2 * T (4))))))
+         * u2) * u2 * ex;
+  o[10] = T (1.8) * u2 * u3 * ec
+    + (T (-1.5) * (x2 * (T (8) + x2 * T (-4)))
+       - T (1.2) * (T (-3) + x2 * (T (-2) + x2 * (T (-26) + x2 * (T (26) + x2 * T (-4))))) * u2)
+      * u2 * ex;
+}
+
 template <int V, typename T>
 SHK_HD void scalars_real (T xi, T r, T *o)
 {
-  T xr = xi * r, x2 = xr * xr;
-  T s = r, s2 = r * r;
-  T ec = m_erfc (xr);
-  T ex = xi * inv_sqrt_pi<T> () * m_exp (-x2);
-  o[1] = (T (0.75) + T (0.5) / s2) / s * ec
-    + ((T (1) + x2 * (T (14) + T (4) * x2 * (T (-5) + x2))) / s2 - T (4.5) + T (3) * x2) * ex;
-  T a2 = (T (0.75) - T (1.5) / s2) / s * ec
-    + ((T (-3) + x2 * (T (-2) + T (4) * x2 * (T (4) - x2))) / s2 + T (1.5) - T (3) * x2) * ex;
-  o[0] = a2 + o[1];
-  if (V == 0) return;
-  o[2] = T (-0.75) / s2 * ec - T (1.5) * (T (1) + x2 * (T (-6) + x2 * T (2))) / s * ex;
-  o[4] = T (-0.375) / s2 / s * ec
-    - T (0.75) * (T (1) + x2 * (T (14) + x2 * (T (-20) + x2 * T (4)))) / s2 * ex;
-  T c2 = T (1.125) / s2 / s * ec
-    - T (0.75) * (T (-3) + x2 * (T (-2) + x2 * (T (16) + x2 * T (-4)))) / s2 * ex;
-  o[3] = c2 + o[4];
-  if (V == 1) return;
-  o[5] = (T (2.25) - T (3.6) / s2) / s2 * ec
-    + (T (-1.5) * (T (-3) + x2 * T (6))
-       - T (0.8) * (T (9) + x2 * (T (6) + x2 * (T (-48) + x2 * T (12)))) / s2) / s * ex;
-  o[6] = T (1.2) / s2 / s2 * ec
-    + (T (-3) * (x2 * (T (2) - x2))
-       - T (0.8) * (T (-3) + x2 * (T (-2) + x2 * (T (-26) + x2 * (T (26) + x2 * T (-4))))) / s2)
-      / s * ex;
-  o[7] = T (-1.125) / s2 / s * ec
-    + T (1.5) * (T (-1.5) + x2 * (T (-1) + x2 * (T (8) + x2 * T (-2)))) / s2 * ex;
-  o[8] = (T (-4.5) + T (10.8) / s2) / s / s2 * ec
-    + (T (1.5) * (T (-6) + x2 * (T (-12) + x2 * T (12)))
-       + T (1.2) * (T (18) + x2 * (T (12) + x2 * (T (30) + x2 * (T (-66) + x2 * T (12))))) / s2)
-      / s2 * ex;
-  o[9] = (T (2.25) - T (7.2) / s2) / s / s2 * ec
-    + (T (-1.5) * (T (-3) + x2 * (T (6) + x2 * (T (-12) + x2 * T (4))))
-       - T (1.2) * (T (12) + x2 * (T (8) + x2 * (T (-22) + x2 * (T (58) + x2 * (T (-34) + x2 * T (4))))))
-         / s2) / s2 * ex;
-  o[10] = T (1.8) / s2 / s / s2 * ec
-    + (T (-1.5) * (x2 * (T (8) + x2 * T (-4)))
-       - T (1.2) * (T (-3) + x2 * (T (-2) + x2 * (T (-26) + x2 * (T (26) + x2 * T (-4))))) / s2)
-      / s2 * ex;
+  scalars_real_u<V, T> (xi, r, T (1) / r, o);
 }
 
 // ---------------------------------------------------------------------------
 // real-space Ewald scalars, unequal spheres: ewald.c:203-359, followed by
 // the SD scaling of ewald.c:382-420 when SD is true
 // ---------------------------------------------------------------------------
+// `u` is 1/r (one reciprocal per pair-image, as in scalars_real_u)
 template <int V, typename T, bool SD = true>
-SHK_HD void scalars_real_poly_sd (T xi, T r, T aa_, T ab, T *o)
+SHK_HD void scalars_real_poly_u (T xi, T r, T u, T aa, T ab, T *o)
 {
-  const T aa = aa_;
-  const T sa = SD ? aa_ : T (1);
-  T x = xi * r, x2 = x * x;
-  T r2 = r * r, r3 = r2 * r, r4 = r2 * r2, r5 = r3 * r2;
-  T aa2 = aa * aa, ab2 = ab * ab;
-  T ec = m_erfc (x);
-  T ex = xi * inv_sqrt_pi<T> () * m_exp (-x2);
-  T a0 = T (2) * (T (-3) + x2 * T (2)) * ex + ec / r;
-  T b0 = T (2) * (T (1) + x2 * T (-2)) * ex + ec / r;
-  T a2 = T (4) / r2 * (T (1) + x2 * (T (14) + x2 * (T (-20) + x2 * T (4)))) * ex + T (2) / r3 * ec;
-  T b2 = T (-4) / r2 * (T (3) + x2 * (T (2) + x2 * (T (-16) + x2 * T (4)))) * ex - T (6) / r3 * ec;
-  o[1] = T (0.75) * (a0 + a2 * (aa2 + ab2) / T (6));
-  o[0] = T (0.75) * (b0 + b2 * (aa2 + ab2) / T (6)) + o[1];
+  const T sa = SD ? aa : T (1);
+  const T x = xi * r, x2 = x * x;
+  const T u2 = u * u, u3 = u2 * u, u4 = u2 * u2, u5 = u3 * u2;
+  const T aa2 = aa * aa, ab2 = ab * ab;
+  const T ec = m_erfc (x);
+  const T ex = xi * inv_sqrt_pi<T> () * m_exp (-x2);
+  const T sixth = T (1.0 / 6.0), tenth = T (0.1);
+  const T a0 = T (2) * (T (-3) + x2 * T (2)) * ex + ec * u;
+  const T b0 = T (2) * (T (1) + x2 * T (-2)) * ex + ec * u;
+  const T a2 = T (4) * u2 * (T (1) + x2 * (T (14) + x2 * (T (-20) + x2 * T (4)))) * ex + T (2) * u3 * ec;
+  const T b2 = T (-4) * u2 * (T (3) + x2 * (T (2) + x2 * (T (-16) + x2 * T (4)))) * ex - T (6) * u3 * ec;
+  o[1] = T (0.75) * (a0 + a2 * (aa2 + ab2) * sixth);
+  o[0] = T (0.75) * (b0 + b2 * (aa2 + ab2) * sixth) + o[1];
   o[0] *= sa; o[1] *= sa;
   if (V == 0) return;
-  T ad0 = T (-2) / r * (T (1) + x2 * (T (-10) + x2 * T (4))) * ex - ec / r2;
-  T add0 = T (4) / r2 * (T (1) + x2 * (T (6) + x2 * (T (-16) + x2 * T (4)))) * ex + T (2) / r3 * ec;
-  T bd0 = T (-2) / r * (T (1) + x2 * (T (6) + x2 * T (-4))) * ex - ec / r2;
+  const T ad0 = T (-2) * u * (T (1) + x2 * (T (-10) + x2 * T (4))) * ex - ec * u2;
+  const T add0 = T (4) * u2 * (T (1) + x2 * (T (6) + x2 * (T (-16) + x2 * T (4)))) * ex + T (2) * u3 * ec;
+  const T bd0 = T (-2) * u * (T (1) + x2 * (T (6) + x2 * T (-4))) * ex - ec * u2;
   const T sa2 = sa * sa, sa3 = sa2 * sa;
-  o[2] = T (-0.375) * (-ad0 + b0 / r) * sa2;
-  T c1 = T (0.1875) * (-add0 + (-ad0 + bd0) / r);
-  T c2 = T (0.1875) * (add0 + (-ad0 - bd0 + T (2) * b0 / r) / r);
+  const T b0u = b0 * u;
+  o[2] = T (-0.375) * (-ad0 + b0u) * sa2;
+  const T c1 = T (0.1875) * (-add0 + (-ad0 + bd0) * u);
+  const T c2 = T (0.1875) * (add0 + (-ad0 - bd0 + T (2) * b0u) * u);
   o[4] = c1 * sa3;
   o[3] = (c2 + c1) * sa3;
   if (V == 1) return;
-  T ad2 = T (-4) / r3 * (T (3) + x2 * (T (2) + x2 * (T (68) + x2 * (T (-56) + x2 * T (8))))) * ex
-    - T (6) / r4 * ec;
-  T add2 = T (16) / r4
+  const T ad2 = T (-4) * u3 * (T (3) + x2 * (T (2) + x2 * (T (68) + x2 * (T (-56) + x2 * T (8))))) * ex
+    - T (6) * u4 * ec;
+  const T add2 = T (16) * u4
     * (T (3) + x2 * (T (2) + x2 * (T (-16) + x2 * (T (76) + x2 * (T (-38) + x2 * T (4)))))) * ex
-    + T (24) / r5 * ec;
-  T bd2 = T (4) / r3 * (T (9) + x2 * (T (6) + x2 * (T (36) + x2 * (T (-48) + x2 * T (8))))) * ex
-    + T (18) / r4 * ec;
-  T wg = aa2 / T (10) + ab2 / T (6);
-  T g10 = T (-0.75) / r * b0;
-  T g12 = T (-0.75) / r * b2 * wg;
-  T g20 = T (-0.375) * (ad0 + b0 / r);
-  T g22 = T (-0.375) * (ad2 + b2 / r) * wg;
+    + T (24) * u5 * ec;
+  const T bd2 = T (4) * u3 * (T (9) + x2 * (T (6) + x2 * (T (36) + x2 * (T (-48) + x2 * T (8))))) * ex
+    + T (18) * u4 * ec;
+  const T b2u = b2 * u;
+  const T wg = aa2 * tenth + ab2 * sixth;
+  const T g10 = T (-0.75) * b0u;
+  const T g12 = T (-0.75) * b2u * wg;
+  const T g20 = T (-0.375) * (ad0 + b0u);
+  const T g22 = T (-0.375) * (ad2 + b2u) * wg;
   o[5] = T (-3) * (g10 + g12) * sa2;
   o[6] = (g20 + g22) * sa2;
-  o[7] = T (-0.1875) * (add0 + (-ad0 - bd0 + T (2) * b0 / r) / r) * sa3;
-  T wm = (aa2 + ab2) / T (10);
-  T m50 = T (-0.375) * (ad0 + b0 / r) / r;
-  T m52 = T (-0.375) * (ad2 + b2 / r) / r * wm;
-  T zm = T (2) * (m50 + m52);
-  T m30 = T (-0.75) * b0 / r2;
-  T m32 = T (-0.75) * b2 / r2 * wm;
-  T xm = T (6) * (m30 + m32) + T (3) * zm;
-  T m40 = T (-0.1875) * (add0 + (-ad0 + T (3) * bd0 - T (6) * b0 / r) / r);
-  T m42 = T (-0.1875) * (add2 + (-ad2 + T (3) * bd2 - T (6) * b2 / r) / r) * wm;
-  T ym = T (2) * (m40 + m42) + zm;
+  o[7] = T (-0.1875) * (add0 + (-ad0 - bd0 + T (2) * b0u) * u) * sa3;
+  const T wm = (aa2 + ab2) * tenth;
+  const T m50 = T (-0.375) * (ad0 + b0u) * u;
+  const T m52 = T (-0.375) * (ad2 + b2u) * u * wm;
+  const T zm = T (2) * (m50 + m52);
+  const T m30 = T (-0.75) * b0 * u2;
+  const T m32 = T (-0.75) * b2 * u2 * wm;
+  const T xm = T (6) * (m30 + m32) + T (3) * zm;
+  const T m40 = T (-0.1875) * (add0 + (-ad0 + T (3) * bd0 - T (6) * b0u) * u);
+  const T m42 = T (-0.1875) * (add2 + (-ad2 + T (3) * bd2 - T (6) * b2u) * u) * wm;
+  const T ym = T (2) * (m40 + m42) + zm;
   o[8] = xm * sa3; o[9] = ym * sa3; o[10] = zm * sa3;
+}
+
+template <int V, typename T, bool SD = true>
+SHK_HD void scalars_real_poly_sd (T xi, T r, T aa, T ab, T *o)
+{
+  scalars_real_poly_u<V, T, SD> (xi, r, T (1) / r, aa, ab, o);
 }
 
 // open-BC RPY scalars, SD form (non-ewald.c:49-90, 110-145 + scaling)

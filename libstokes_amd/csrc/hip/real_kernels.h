@@ -53,14 +53,19 @@ __device__ __forceinline__ void real_item (const RealParams &p, T ai, T xi,
                                            const RealLds<V, T> &L, int src, T dx,
                                            T dy, T dz, T rr, T *acc)
 {
-  T r = m_sqrt (rr);
+  // one reciprocal square root per pair-image: u = 1/r, r = rr u
+  T ir = m_rsqrt (rr);
+  T r = rr * ir;
   const T aj = L.aj[src];
   const T rmn = (ai + aj) * T (p.rmin);
-  if (r < rmn) r = rmn;
-  const T ir = T (1) / r;
+  if (r < rmn)
+    {
+      r = rmn;
+      ir = T (1) / rmn;
+    }
   T s[11];
-  if (p.poly) scalars_real_poly_sd<V, T> (xi, r, ai, aj, s);
-  else scalars_real<V, T> (xi, r, s);
+  if (p.poly) scalars_real_poly_u<V, T> (xi, r, ir, ai, aj, s);
+  else scalars_real_u<V, T> (xi, r, ir, s);
   contract_item<V, T> (dx * ir, dy * ir, dz * ir, s, L.z[src], acc);
 }
 

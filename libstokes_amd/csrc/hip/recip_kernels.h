@@ -195,43 +195,22 @@ k_recip_back (const double4 *__restrict__ pos, const double4 *__restrict__ kv,
     }
 }
 
+// one thread per output component (i, q)
 template <int V, typename T>
-__global__ __launch_bounds__ (128) void
+__global__ __launch_bounds__ (256) void
 k_combine (const double4 *__restrict__ pos, const double *__restrict__ x,
            const double *__restrict__ rpart, int nreal, const T *__restrict__ kpart,
            int nrecip, DevParams p, int periodic, double *__restrict__ y)
 {
-  constexpr int D = Dim<V>::D, NM = Dim<V>::NMOM;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= p.np) return;
-  const double4 pi = pos[i];
-  double acc[D];
-#pragma unroll
-  for (int q = 0; q < D; q++) acc[q] = 0.0;
-  {
-    double s[11], M[NM];
-#pragma unroll
-    for (int k = 0; k < NM; k++) M[k] = 0.0;
-    self_scalars<double> (p, pi.w, periodic != 0, s);
-    moments_add<V, double> (M, 0.0, 0.0, 0.0, s);
-    double z[D];
-    load_z<V, double> (x, i, z);
-    block_apply<V, double, double> (M, z, acc);
-  }
-  for (int ch = 0; ch < nreal; ch++)
-    {
-      const double *pp = rpart + ((size_t) ch * p.np + i) * D;
-#pragma unroll
-      for (int q = 0; q < D; q++) acc[q] += pp[q];
-    }
-  for (int ch = 0; ch < nrecip; ch++)
-    {
-      const T *pp = kpart + ((size_t) ch * p.np + i) * D;
-#pragma unroll
-      for (int q = 0; q < D; q++) acc[q] += double (pp[q]);
-    }
-#pragma unroll
-  for (int q = 0; q < D; q++) y[(size_t) i * D + q] = acc[q];
+  constexpr int D = Dim<V>::D;
+  const size_t t = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t n = (size_t) p.np * D;
+  if (t >= n) return;
+  const int i = (int) (t / D), q = (int) (t - (size_t) i * D);
+  double acc = self_diag (p, pos[i].w, periodic != 0, V, q) * x[t];
+  for (int ch = 0; ch < nreal; ch++) acc += rpart[(size_t) ch * n + t];
+  for (int ch = 0; ch < nrecip; ch++) acc += double (kpart[(size_t) ch * n + t]);
+  y[t] = acc;
 }
 
 }  // namespace shk

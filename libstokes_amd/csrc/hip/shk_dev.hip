@@ -11,6 +11,7 @@
 #include "ewald_kernels.h"
 #include "real_kernels.h"
 #include "recip_kernels.h"
+#include "block_kernels.h"
 
 using namespace shk;
 
@@ -327,15 +328,15 @@ static void launch_pairs (shk_dev *d, const DevParams &q, const double *x, int n
 static void recip_shape (int np, int nk, int &sj, int &jper, int &sk, int &kper)
 {
   const int nkt = (nk + 255) / 256;
-  sj = 1024 / nkt;
-  if (sj > 32) sj = 32;
+  sj = 4096 / nkt;
+  if (sj > 64) sj = 64;
   if (sj > (np + 63) / 64) sj = (np + 63) / 64;
   if (sj < 1) sj = 1;
   jper = (np + sj - 1) / sj;
   sj = (np + jper - 1) / jper;
   const int nit = (np + 255) / 256;
-  sk = 1024 / nit;
-  if (sk > 64) sk = 64;
+  sk = 4096 / nit;
+  if (sk > 128) sk = 128;
   if (sk > (nk + 31) / 32) sk = (nk + 31) / 32;
   if (sk < 1) sk = 1;
   kper = (nk + sk - 1) / sk;
@@ -418,7 +419,7 @@ static int atimes_vt (shk_dev *d, const double *x, double *y)
       HIPCHK (hipGetLastError ());
       if (d->timing) HIPCHK (hipEventRecord (d->ev[1], d->stream));
     }
-  hipLaunchKernelGGL ((k_combine<V, T>), dim3 ((p.np + 127) / 128), dim3 (128), 0, d->stream,
+  hipLaunchKernelGGL ((k_combine<V, T>), dim3 ((p.np * D + 255) / 256), dim3 (256), 0, d->stream,
                       pos4, x, (const double *) d->partial.p, nreal, (const T *) kpart, nrecip, q,
                       p.periodic, y);
   HIPCHK (hipGetLastError ());
@@ -442,12 +443,133 @@ static int atimes_v (shk_dev *d, const double *x, double *y, int prec)
   return prec == SHK_MIXED ? atimes_vt<V, float> (d, x, y) : atimes_vt<V, double> (d, x, y);
 }
 
+// ---- block of right-hand sides (periodic) --------------------------------
+
+template <int V, typename T, int NPOW>
+static int recip_block_launch (shk_dev *d, const T *XT, int B, int &nrecip,
+                               const void *&kpart)
+{
+  constexpr int D = Dim<V>::D;
+  constexpr int NS = NPOW > 1 ? 4 : 2;
+  const shk_params &p = d->prm;
+  const int nk = p.nk, nbg = (B + 7) / 8;
+  int sj = 1024 / (((nk + 31) / 32) * nbg);
+  if (sj > 8) sj = 8;
+  if (sj < 1) sj = 1;
+  int jper = (p.np + sj - 1) / sj;
+  sj = (p.np + jper - 1) / jper;
+  int sk = 1024 / (((p.np + 31) / 32) * nbg);
+  if (sk > 16) sk = 16;
+  if (sk > (nk + 31) / 32) sk = (nk + 31) / 32;
+  if (sk < 1) sk = 1;
+  int kper = (nk + sk - 1) / sk;
+  sk = (nk + kper - 1) / kper;
+  int rc;
+  if ((rc = ensure (d, d->kpart, sizeof (T) * (size_t) sj * nk * B * NS * D))) return rc;
+  if ((rc = ensure (d, d->kpq, sizeof (T) * (size_t) nk * B * NPOW * 2 * D))) return rc;
+  if ((rc = ensure (d, d->kback, sizeof (T) * (size_t) sk * p.np * D * B))) return rc;
+  const double4 *pos4 = (const double4 *) d->pos.p;
+  const double4 *kv = (const double4 *) d->kv.p;
+  hipLaunchKernelGGL ((k_recip_sf_block<V, T, NPOW>), dim3 ((nk + 31) / 32, nbg, sj), dim3 (256),
+                      0, d->stream, pos4, XT, B, kv, p.np, nk, jper, (T *) d->kpart.p);
+  HIPCHK (hipGetLastError ());
+  const size_t nkb = (size_t) nk * B;
+  hipLaunchKernelGGL ((k_recip_blk_block<V, T, NPOW>), dim3 ((unsigned) ((nkb + 127) / 128)),
+                      dim3 (128), 0, d->stream, kv, (const double *) d->ks.p, nk, B, sj, p.gsign,
+                      (const T *) d->kpart.p, (T *) d->kpq.p);
+  HIPCHK (hipGetLastError ());
+  hipLaunchKernelGGL ((k_recip_back_block<V, T, NPOW>), dim3 ((p.np + 31) / 32, nbg, sk),
+                      dim3 (256), 0, d->stream, pos4, kv, (const T *) d->kpq.p, p.np, nk, B, kper,
+                      (T *) d->kback.p);
+  HIPCHK (hipGetLastError ());
+  nrecip = sk;
+  kpart = d->kback.p;
+  return SHK_OK;
+}
+
+template <int V, typename T>
+static int atimes_block_vt (shk_dev *d, const double *x, double *y, int B)
+{
+  constexpr int D = Dim<V>::D;
+  constexpr int JW = 256;
+  const shk_params &p = d->prm;
+  DevParams q = dev_params (p);
+  const double4 *pos4 = (const double4 *) d->pos.p;
+  int rc, nrecip = 0;
+  const void *kpart = nullptr;
+  if ((rc = ensure (d, d->xs, sizeof (T) * (size_t) p.np * D * B))) return rc;
+  const int nj = (p.np + JW - 1) / JW;
+  if ((rc = ensure (d, d->partial, sizeof (T) * (size_t) nj * p.np * D * B))) return rc;
+  T *XT = (T *) d->xs.p;
+  const size_t nb = (size_t) p.np * B;
+  if (d->timing) HIPCHK (hipEventRecord (d->ev[0], d->stream));
+  hipLaunchKernelGGL ((k_block_prep<V, T>), dim3 ((unsigned) ((nb + 255) / 256)), dim3 (256), 0,
+                      d->stream, x, p.np, B, XT);
+  HIPCHK (hipGetLastError ());
+  RealParams rp;
+  rp.np = p.np; rp.ncand = d->ncand; rp.ncore = d->ncore; rp.poly = p.poly;
+  rp.xi = p.xi; rp.rmax2 = p.rmax2; rp.rmin = p.rmin;
+  rp.lx = p.lx; rp.ly = p.ly; rp.lz = p.lz; rp.s2x = d->s2x; rp.s3x = d->s3x;
+  rp.tx = d->tx; rp.ty = d->ty; rp.tz = d->tz; rp.tbits_words = d->tbits_words;
+  const long waves = (long) p.np * nj;
+  const void *cand = std::is_same<T, double>::value ? d->cand.p : d->candf.p;
+  hipLaunchKernelGGL ((k_real_block<V, T, JW>), dim3 ((unsigned) ((waves + 3) / 4)), dim3 (256), 0,
+                      d->stream, pos4, (const T *) XT, B, (const typename Vec4<T>::type *) cand,
+                      (const int4 *) d->candm.p, (const unsigned *) d->tbits.p, rp,
+                      (T *) d->partial.p);
+  HIPCHK (hipGetLastError ());
+  if (d->timing) HIPCHK (hipEventRecord (d->ev[1], d->stream));
+  if (p.nk > 0)
+    {
+      rc = p.poly ? recip_block_launch<V, T, 5> (d, XT, B, nrecip, kpart)
+                  : recip_block_launch<V, T, 1> (d, XT, B, nrecip, kpart);
+      if (rc) return rc;
+    }
+  hipLaunchKernelGGL ((k_combine_block<V, T>), dim3 ((unsigned) ((nb + 127) / 128)), dim3 (128), 0,
+                      d->stream, pos4, x, B, (const T *) d->partial.p, nj, (const T *) kpart,
+                      nrecip, q, 1, y);
+  HIPCHK (hipGetLastError ());
+  if (d->timing)
+    {
+      HIPCHK (hipEventRecord (d->ev[2], d->stream));
+      HIPCHK (hipEventSynchronize (d->ev[2]));
+      float ta = 0, tb = 0;
+      HIPCHK (hipEventElapsedTime (&ta, d->ev[0], d->ev[1]));
+      HIPCHK (hipEventElapsedTime (&tb, d->ev[1], d->ev[2]));
+      d->last_ms[0] = ta;
+      d->last_ms[1] = tb;
+      d->last_ms[2] = ta + tb;
+    }
+  return SHK_OK;
+}
+
+template <int V>
+static int atimes_block_v (shk_dev *d, const double *x, double *y, int B, int prec)
+{
+  return prec == SHK_MIXED ? atimes_block_vt<V, float> (d, x, y, B)
+                           : atimes_block_vt<V, double> (d, x, y, B);
+}
+
 extern "C" int shk_dev_atimes (shk_dev *d, const double *x, double *y, int nrhs, int prec)
 {
   if (!d || !d->have_sys) { set_err ("shk_dev_atimes: no system"); return SHK_EARG; }
   if (nrhs < 1) { set_err ("shk_dev_atimes: nrhs < 1"); return SHK_EARG; }
   const shk_params &p = d->prm;
   const size_t n = (size_t) p.np * dimv (p.version);
+  if (nrhs >= 2 && p.periodic)
+    {
+      // block path, at most 128 columns per pass
+      for (int c0 = 0; c0 < nrhs; c0 += 128)
+        {
+          const int B = nrhs - c0 < 128 ? nrhs - c0 : 128;
+          int rc;
+          if (p.version == 0) rc = atimes_block_v<0> (d, x + c0 * n, y + c0 * n, B, prec);
+          else if (p.version == 1) rc = atimes_block_v<1> (d, x + c0 * n, y + c0 * n, B, prec);
+          else rc = atimes_block_v<2> (d, x + c0 * n, y + c0 * n, B, prec);
+          if (rc) return rc;
+        }
+      return SHK_OK;
+    }
   for (int c = 0; c < nrhs; c++)
     {
       int rc;

@@ -170,6 +170,37 @@ def test_device_api_multi_rhs_and_extmat(golden):
     s.close()
 
 
+@pytest.mark.parametrize("version,poly,nrhs", [
+    (0, False, 70), (1, False, 9), (2, False, 5), (2, True, 3), (0, True, 16)])
+def test_block_rhs(version, poly, nrhs):
+    """several right-hand sides in one call (block kernels): every column
+    equals the oracle's single matvec; MIXED within 1e-6"""
+    rng = np.random.default_rng(40 + version + nrhs)
+    n = 150
+    a = np.exp(rng.uniform(np.log(0.6), np.log(1.6), n)) if poly else None
+    pos, L = workloads.configuration(n, 0.15, seed=41)
+    d = ls.dim(version)
+    X = rng.standard_normal((nrhs, n * d))
+    s = ls.Stokes(version, pos, box=(L,) * 3, tr=20.0, a=a, shear_mode=1,
+                  shear_shift=0.3 * L)
+    o = po.System(version, pos, box=(L,) * 3, tr=20.0, a=a, shear_mode=1,
+                  shear_shift=0.3 * L)
+    xd = ls.DeviceArray.from_host(X)
+    yd = ls.DeviceArray(nrhs * n * d)
+    s.atimes_dev(xd, yd, nrhs=nrhs)
+    Y = yd.to_host().reshape(nrhs, n * d)
+    s.set_precision(ls.MIXED)
+    s.atimes_dev(xd, yd, nrhs=nrhs)
+    Ym = yd.to_host().reshape(nrhs, n * d)
+    for c in sorted({0, nrhs // 2, nrhs - 1}):
+        yo = o.atimes(X[c])
+        assert relerr(Y[c], yo) <= TOL64, c
+        assert relerr(Ym[c], yo) <= TOLMIX, c
+    xd.free()
+    yd.free()
+    s.close()
+
+
 def test_open_boundary_fp64(golden):
     data, cases = golden
     for meta in cases:
