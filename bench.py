@@ -99,6 +99,43 @@ def cpu_baseline(pos, L, x, tr, budget_s):
                        f"matvec (tr={tr:g}), {t:.1f} s; rate = rows/N/time")
 
 
+def bench_bd(args, log):
+    """BD steps/s: mid-point steps (brownian.c:2352-2560), F version,
+    N = 2048 spheres at phi = 0.2, Chebyshev order 30, dt = 1e-3, Pe = 1;
+    spectral bounds by Lanczos on the first (untimed) step"""
+    n, phi = 2048, 0.2
+    pos, L = workloads.configuration(n, phi, seed=2)
+    out = {"workload": f"BD mid-point F N={n} phi={phi} n_cheb=30 dt=1e-3 "
+                       f"Pe=1 (31 operator products per attempt)"}
+    for name, prec in (("mixed", ls.MIXED), ("fp64", ls.FP64)):
+        s = ls.Stokes(0, pos, box=(L, L, L), tr=1.0, precision=prec)
+        xd = ls.DeviceArray.from_host(pos.reshape(-1))
+        bd = ls.BD(s, peclet=1.0, n_cheb=30, eps=1e-6, dt_lim=1e-6,
+                   rmin_ol=1.0, seed=11)
+        for _ in range(2):
+            bd.step(xd, 1e-3)
+        s.sync()
+        att = halv = redr = 0
+        t0 = time.perf_counter()
+        for _ in range(args.bd_steps):
+            info = bd.step(xd, 1e-3)
+            att += info.attempts
+            halv += info.halvings
+            redr += info.redraws
+        s.sync()
+        t = time.perf_counter() - t0
+        out[name] = {"steps_per_s": args.bd_steps / t,
+                     "ms_per_step": t / args.bd_steps * 1e3,
+                     "attempts_per_step": att / args.bd_steps,
+                     "halvings": halv, "redraws": redr,
+                     "err_cheb_last": info.err_cheb,
+                     "eig": [info.eig[0], info.eig[1]]}
+        log(f"BD {name}: {out[name]['ms_per_step']:.3f} ms/step")
+        bd.close()
+        s.close()
+    return out
+
+
 def profile_run(args):
     """kernels for the counter runs: k_copy_calibrate on 256 MiB (a known
     byte count), then the matvec kernels of both precisions"""
@@ -136,6 +173,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--batch", type=int, default=32,
                     help="right-hand sides per launch for the batched figures")
+    ap.add_argument("--bd-steps", type=int, default=50,
+                    help="timed BD steps per precision (0: skip BD)")
     ap.add_argument("--profile", action="store_true",
                     help="short run for rocprofv3: a 256 MiB calibration "
                          "copy, then --steps mixed and FP64 matvecs at --tr")
@@ -238,6 +277,8 @@ def main():
                     "rel_err_mixed_vs_fp64": err_b},
         "roofline": roof,
     }
+    if args.bd_steps > 0:
+        out["bd"] = bench_bd(args, log)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(pos, L, x, 1.0, args.cpu_budget)
     print(json.dumps(out))

@@ -97,6 +97,60 @@ int stokes_hip_lanczos_bounds (struct stokes *sys, int niter,
                                unsigned long seed, double margin,
                                double *lmin, double *lmax);
 
+/* ---- Brownian dynamics ------------------------------------------------
+ * The mid-point step of BD_evolve_mid (brownian.c:2352-2560) for the F
+ * version with every particle mobile (the BD_params fields it reads are
+ * the arguments of stokes_hip_bd_create), run on the GPU as one captured
+ * graph per attempt: Gaussian draw y, z = M^{-1/2} y by the Chebyshev
+ * recursion (BD_calc_FB, brownian.c:1545-1874), its accuracy
+ * |z.Mz - y.y| / z.Mz, drift at x, Euler half step, overlap count, drift
+ * at the mid point, Euler full step, overlap count.  The host then takes
+ * the reference's decisions: refit the spectral bounds once when the
+ * accuracy exceeds eps (brownian.c:1707-1727, FP64 Lanczos in place of
+ * ARPACK), halve dt on overlap and retry with the same noise, or redraw
+ * the noise and restore dt once dt falls below dt_lim
+ * (brownian.c:2435-2459, 2501-2525).
+ *
+ * Noise: Philox4x32-10 with Box-Muller (the reference draws from MT19937
+ * with the polar method, KIrand.c:108-135), so trajectories follow the
+ * reference's distribution, not its random stream. */
+typedef struct stokes_hip_bd stokes_hip_bd;
+
+typedef struct stokes_hip_bd_info {
+  double dt;          /* step taken (the return value of BD_evolve_mid) */
+  double err_cheb;    /* Chebyshev accuracy of the accepted noise */
+  double eig[2];      /* spectral bounds in use */
+  int attempts;       /* step graphs run */
+  int halvings;       /* dt halvings after overlaps */
+  int redraws;        /* noise redraws */
+  int refits;         /* spectral-bound refits */
+} stokes_hip_bd_info;
+
+/* sys: version 0 (F), equal radii (a == NULL), no shear, set up as for
+ * atimes_3all.  F_dev: constant force [3 np] (BD->F) or NULL.  peclet:
+ * fact = sqrt(2 / (Pe dt)), 0 when Pe <= 0 (brownian.c:2177-2187).
+ * n_cheb: order of the 1/sqrt(x) fit (BD->n_minv > 0).  eps: refit
+ * threshold (BD->eps).  dt_lim, rmin_ol: BD->dt_lim and BD->rmin (pairs
+ * with r^2 < rmin_ol (a_i+a_j)^2 overlap; checked only when sys->rmin is
+ * 0, as in the reference).  seed: key of the noise generator. */
+int stokes_hip_bd_create (struct stokes *sys, const double *F_dev,
+                          double peclet, int n_cheb, double eps,
+                          double dt_lim, double rmin_ol,
+                          unsigned long long seed, stokes_hip_bd **out);
+/* spectral bounds of M for the Chebyshev fit (BD->eig_minv); without this
+ * call the first step estimates them by Lanczos */
+int stokes_hip_bd_set_bounds (stokes_hip_bd *bd, double lmin, double lmax);
+/* enable != 0: recompute fact when a redraw restores dt (the reference
+ * keeps the value of the halved dt, brownian.c:2446-2449, 2512-2515) */
+int stokes_hip_bd_set_fact_fix (stokes_hip_bd *bd, int enable);
+/* one step from the positions x_dev [3 np] (updated in place), step dt */
+int stokes_hip_bd_step (stokes_hip_bd *bd, double *x_dev, double dt,
+                        stokes_hip_bd_info *info);
+/* copies of the Gaussian draw and of z = M^{-1/2} y of the last attempt
+ * (device pointers, 3 np each; either may be NULL) */
+int stokes_hip_bd_last_noise (stokes_hip_bd *bd, double *y_dev, double *z_dev);
+void stokes_hip_bd_destroy (stokes_hip_bd *bd);
+
 #ifdef __cplusplus
 }
 #endif

@@ -53,6 +53,9 @@ EXPORTS = [
     "stokes_hip_atimes_dev", "stokes_hip_atimes", "stokes_hip_make_matrix_dev",
     "stokes_hip_make_matrix", "stokes_hip_extmat_dev",
     "stokes_hip_cheb_eval_dev", "stokes_hip_lanczos_bounds",
+    "stokes_hip_bd_create", "stokes_hip_bd_set_bounds",
+    "stokes_hip_bd_set_fact_fix", "stokes_hip_bd_step",
+    "stokes_hip_bd_last_noise", "stokes_hip_bd_destroy",
 ]
 
 
@@ -90,6 +93,14 @@ class StokesStruct(C.Structure):
         ("cpu1", C.c_double), ("cpu2", C.c_double), ("cpu3", C.c_double),
         ("it", _vp),
     ]
+
+
+class BDInfo(C.Structure):
+    """stokes_hip_bd_info of include/stokes_hip.h"""
+    _fields_ = [("dt", C.c_double), ("err_cheb", C.c_double),
+                ("eig", C.c_double * 2), ("attempts", C.c_int),
+                ("halvings", C.c_int), ("redraws", C.c_int),
+                ("refits", C.c_int)]
 
 
 class StokesError(RuntimeError):
@@ -146,6 +157,16 @@ def lib():
     L.chebyshev_eval.argtypes = [C.c_int, _dp, C.c_double, C.c_double,
                                  C.c_double]
     L.scalars_nonewald.argtypes = [C.c_int, C.c_double, _dp]
+    L.stokes_hip_bd_create.argtypes = [sp, _vp, C.c_double, C.c_int,
+                                       C.c_double, C.c_double, C.c_double,
+                                       C.c_ulonglong, C.POINTER(_vp)]
+    L.stokes_hip_bd_set_bounds.argtypes = [_vp, C.c_double, C.c_double]
+    L.stokes_hip_bd_set_fact_fix.argtypes = [_vp, C.c_int]
+    L.stokes_hip_bd_step.argtypes = [_vp, _vp, C.c_double,
+                                     C.POINTER(BDInfo)]
+    L.stokes_hip_bd_last_noise.argtypes = [_vp, _vp, _vp]
+    L.stokes_hip_bd_destroy.argtypes = [_vp]
+    L.stokes_hip_bd_destroy.restype = None
     _lib = L
     return L
 
@@ -314,6 +335,54 @@ class Stokes:
         if self.sys:
             self.L.stokes_free(self.sys)
             self.sys = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class BD:
+    """Mid-point Brownian-dynamics integrator of one F-version system
+    (stokes_hip_bd_* of include/stokes_hip.h; brownian.c:2352-2560)."""
+
+    def __init__(self, stokes, peclet=1.0, n_cheb=30, eps=1e-6, dt_lim=1e-6,
+                 rmin_ol=1.0, seed=1, force_dev=None, bounds=None,
+                 fact_fix=False):
+        self.L = lib()
+        self.stokes = stokes
+        h = _vp()
+        check(self.L.stokes_hip_bd_create(
+            stokes.sys, force_dev.ptr if force_dev is not None else None,
+            float(peclet), int(n_cheb), float(eps), float(dt_lim),
+            float(rmin_ol), int(seed), C.byref(h)), "bd_create")
+        self.h = h
+        if bounds is not None:
+            check(self.L.stokes_hip_bd_set_bounds(self.h, *map(float, bounds)),
+                  "bd_set_bounds")
+        if fact_fix:
+            check(self.L.stokes_hip_bd_set_fact_fix(self.h, 1),
+                  "bd_set_fact_fix")
+
+    def step(self, x_dev, dt):
+        info = BDInfo()
+        check(self.L.stokes_hip_bd_step(self.h, x_dev.ptr, float(dt),
+                                        C.byref(info)), "bd_step")
+        return info
+
+    def last_noise(self):
+        n = 3 * self.stokes.np
+        y = DeviceArray(n)
+        z = DeviceArray(n)
+        check(self.L.stokes_hip_bd_last_noise(self.h, y.ptr, z.ptr),
+              "bd_last_noise")
+        return y.to_host(), z.to_host()
+
+    def close(self):
+        if self.h:
+            self.L.stokes_hip_bd_destroy(self.h)
+            self.h = None
 
     def __del__(self):
         try:

@@ -27,6 +27,14 @@ static void set_err (const char *fmt, ...)
 
 extern "C" const char *shk_last_error (void) { return g_err; }
 
+extern "C" void shk_set_error (const char *fmt, ...)
+{
+  va_list ap;
+  va_start (ap, fmt);
+  vsnprintf (g_err, sizeof (g_err), fmt, ap);
+  va_end (ap);
+}
+
 #define HIPCHK(call)                                                          \
   do {                                                                        \
     hipError_t e_ = (call);                                                   \
@@ -55,6 +63,7 @@ struct shk_dev {
   int timing = 0;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   double last_ms[3] = {0, 0, 0};
+  unsigned long generation = 0;   // bumped whenever a work buffer moves
 };
 
 static int ensure (shk_dev *d, DevBuf &b, size_t bytes)
@@ -69,7 +78,7 @@ static int ensure (shk_dev *d, DevBuf &b, size_t bytes)
       return SHK_ENOMEM;
     }
   b.bytes = bytes;
-  (void) d;
+  d->generation++;
   return SHK_OK;
 }
 
@@ -120,6 +129,8 @@ extern "C" void shk_dev_destroy (shk_dev *d)
 
 extern "C" void *shk_dev_stream (shk_dev *d) { return d ? (void *) d->stream : nullptr; }
 
+extern "C" unsigned long shk_dev_generation (shk_dev *d) { return d ? d->generation : 0; }
+
 extern "C" int shk_dev_malloc (shk_dev *d, void **p, size_t bytes)
 {
   (void) d;
@@ -149,6 +160,18 @@ extern "C" int shk_dev_d2h (shk_dev *d, void *dst, const void *src, size_t bytes
 {
   HIPCHK (hipMemcpyAsync (dst, src, bytes, hipMemcpyDeviceToHost, d->stream));
   HIPCHK (hipStreamSynchronize (d->stream));
+  return SHK_OK;
+}
+
+extern "C" int shk_dev_d2h_async (shk_dev *d, void *dst, const void *src, size_t bytes)
+{
+  HIPCHK (hipMemcpyAsync (dst, src, bytes, hipMemcpyDeviceToHost, d->stream));
+  return SHK_OK;
+}
+
+extern "C" int shk_dev_h2d_async (shk_dev *d, void *dst, const void *src, size_t bytes)
+{
+  HIPCHK (hipMemcpyAsync (dst, src, bytes, hipMemcpyHostToDevice, d->stream));
   return SHK_OK;
 }
 

@@ -93,37 +93,45 @@ stokes_hip_cheb_eval_dev (struct stokes *sys, int ncheb, const double *a,
         }
       c->cheb_bytes = need;
     }
-  double *x = c->cheb_dev, *b0 = x + len, *b1 = b0 + len, *b2 = b1 + len;
-  double da = 2.0 / (eig1 - eig0);
-  double db = -(eig1 + eig0) / (eig1 - eig0);
   int rc = stokes_hip_prepare (c, c->pos_dev != NULL);
   if (rc) return rc;
-  /* b_{n} = b_{n+1} = 0; the first operator product M.0 = 0 is skipped,
-   * so b_{n-1} = a_{n-1} y */
-  if (ncheb == 1)
-    {
-      rc = shk_dev_scal (c->dev, z, y, a[0], len);
-      return rc ? STOKES_HIP_EHIP : 0;
-    }
-  rc = shk_dev_scal (c->dev, b0, NULL, 0.0, len);
-  if (!rc) rc = shk_dev_scal (c->dev, b1, y, a[ncheb - 1], len);
-  if (rc) return STOKES_HIP_EHIP;
-  for (int j = ncheb - 2; j >= 1; j--)
-    {
-      rc = shk_dev_atimes (c->dev, b1, x, nrhs, c->prec);
-      if (rc) return STOKES_HIP_EHIP;
-      rc = shk_dev_axpby4 (c->dev, b2, x, b1, b0, y, 2.0 * da, 2.0 * db, a[j], len);
-      if (rc) return STOKES_HIP_EHIP;
-      double *t = b0; b0 = b1; b1 = b2; b2 = t;
-    }
-  rc = shk_dev_atimes (c->dev, b1, x, nrhs, c->prec);
-  if (!rc) rc = shk_dev_axpby4 (c->dev, z, x, b1, b0, y, da, db, a[0], len);
+  rc = stokes_hip_cheb_launch (c->dev, c->prec, ncheb, a, nrhs, len, y, z,
+                               eig0, eig1, c->cheb_dev);
   if (rc)
     {
       stokes_hip_set_error ("chebyshev: %s", shk_last_error ());
       return STOKES_HIP_EHIP;
     }
   return 0;
+}
+
+int
+stokes_hip_cheb_launch (shk_dev *dev, int prec, int ncheb, const double *a,
+                        int nrhs, size_t len, const double *y, double *z,
+                        double eig0, double eig1, double *work)
+{
+  double *x = work, *b0 = x + len, *b1 = b0 + len, *b2 = b1 + len;
+  double da = 2.0 / (eig1 - eig0);
+  double db = -(eig1 + eig0) / (eig1 - eig0);
+  int rc;
+  /* b_{n} = b_{n+1} = 0; the first operator product M.0 = 0 is skipped,
+   * so b_{n-1} = a_{n-1} y */
+  if (ncheb == 1)
+    return shk_dev_scal (dev, z, y, a[0], len);
+  rc = shk_dev_scal (dev, b0, NULL, 0.0, len);
+  if (!rc) rc = shk_dev_scal (dev, b1, y, a[ncheb - 1], len);
+  if (rc) return rc;
+  for (int j = ncheb - 2; j >= 1; j--)
+    {
+      rc = shk_dev_atimes (dev, b1, x, nrhs, prec);
+      if (rc) return rc;
+      rc = shk_dev_axpby4 (dev, b2, x, b1, b0, y, 2.0 * da, 2.0 * db, a[j], len);
+      if (rc) return rc;
+      double *t = b0; b0 = b1; b1 = b2; b2 = t;
+    }
+  rc = shk_dev_atimes (dev, b1, x, nrhs, prec);
+  if (!rc) rc = shk_dev_axpby4 (dev, z, x, b1, b0, y, da, db, a[0], len);
+  return rc;
 }
 
 void

@@ -409,6 +409,7 @@ stokes_hip_prepare (stokes_ctx *c, int positions_on_device)
       c->shear_shift = s->shear_shift;
       c->a_ptr = s->a;
       c->valid = 1;
+      c->uploads++;
       if (positions_on_device && c->pos_dev)
         {
           rc = shk_dev_set_pos (c->dev, c->pos_dev, 1);

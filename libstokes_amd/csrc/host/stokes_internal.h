@@ -14,6 +14,7 @@ typedef struct stokes_ctx {
   int fix_recip_g;          /* reciprocal G term with the xi-independent sign */
   int timing;
   int valid;                /* parameters uploaded and current */
+  unsigned long uploads;    /* count of full parameter uploads */
   shk_params prm;           /* last uploaded parameters */
   double shear_shift;       /* shift the k table and images were built for */
   int shear_mode;
@@ -41,5 +42,13 @@ shk_dev *stokes_hip_global_dev (void);
 /* record an error message for stokes_hip_last_error() */
 void stokes_hip_set_error (const char *fmt, ...);
 int stokes_dim (int version);
+/* Clenshaw recursion of chebyshev.c:163-228 on the device operator of dev
+ * (positions already uploaded): z = sum_j a_j T_j(M') y for nrhs columns of
+ * length len / nrhs; work holds 4 len doubles.  Launches only (no
+ * allocation, no wait), so it may be captured into a graph.  Returns an
+ * SHK_E* code. */
+int stokes_hip_cheb_launch (shk_dev *dev, int prec, int ncheb, const double *a,
+                            int nrhs, size_t len, const double *y, double *z,
+                            double eig0, double eig1, double *work);
 
 #endif

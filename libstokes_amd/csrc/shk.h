@@ -98,6 +98,9 @@ int  shk_dev_sync (shk_dev *d);
 /* wait for all work on the GPU (every stream of every context) */
 int  shk_device_sync (void);
 void *shk_dev_stream (shk_dev *d);
+/* changes whenever a work buffer of d is (re)allocated: a captured graph
+ * holding its addresses is stale once this moves */
+unsigned long shk_dev_generation (shk_dev *d);
 
 /* per-part device timers of the last operator call (ms): 0 real space,
  * 1 reciprocal space, 2 whole call.  enable != 0 records events. */
@@ -117,6 +120,57 @@ int shk_dev_scal (shk_dev *d, double *dst, const double *src, double alpha,
 /* out[c] = sum_k u[k + c*n] * v[k + c*n] for c < m (column dots) */
 int shk_dev_coldot (shk_dev *d, const double *u, const double *v, int n, int m,
                     double *out_host);
+
+/* copies that do not wait (host side must be pinned for overlap) */
+int shk_dev_d2h_async (shk_dev *d, void *dst, const void *src, size_t bytes);
+int shk_dev_h2d_async (shk_dev *d, void *dst, const void *src, size_t bytes);
+void shk_set_error (const char *fmt, ...);
+
+/* ---- Brownian-dynamics step pieces (bd_kernels.hip) ---------------- */
+
+/* step scalars read by the kernels from device memory */
+typedef struct shk_bd_scalars {
+  double fact;     /* sqrt(2 / (Pe dt)) */
+  double dt;       /* step length */
+} shk_bd_scalars;
+
+/* per-step results written by the kernels */
+typedef struct shk_bd_status {
+  double yy;                         /* y.y of the Gaussian vector */
+  double zmz;                        /* z.(M z) of the Chebyshev result */
+  unsigned long long overlap[2];     /* overlap counts at x_mid, x_new */
+} shk_bd_status;
+
+/* out[0..n) ~ N(0,1) from (seed, *counter_dev) */
+int shk_dev_gauss (shk_dev *d, unsigned long long seed,
+                   const unsigned long long *counter_dev, size_t n, double *out);
+int shk_dev_counter_add (shk_dev *d, unsigned long long *counter_dev,
+                         unsigned long long inc);
+int shk_dev_bd_status_clear (shk_dev *d, shk_bd_status *st);
+/* f = F + fact z (F may be NULL) */
+int shk_dev_bd_force (shk_dev *d, double *f, const double *F, const double *z,
+                      const shk_bd_scalars *sc, size_t n);
+/* xo = xi + frac dt u */
+int shk_dev_bd_euler (shk_dev *d, double *xo, const double *xi, const double *u,
+                      const shk_bd_scalars *sc, double frac, size_t n);
+/* st->yy = y.y, st->zmz = z.w */
+int shk_dev_bd_dots (shk_dev *d, const double *y, const double *z,
+                     const double *w, size_t n, shk_bd_status *st);
+/* *count += overlapping (pair, image) entries, brownian.c:299-375 */
+int shk_dev_bd_overlap (shk_dev *d, const double *pos, const double *a, int np,
+                        int periodic, double lx, double ly, double lz,
+                        int shear_mode, double shift, double rfac,
+                        unsigned long long *count);
+
+/* stream capture into an executable graph */
+int  shk_dev_capture_begin (shk_dev *d);
+int  shk_dev_capture_end (shk_dev *d, void **exec);
+void shk_dev_capture_abort (shk_dev *d);
+int  shk_dev_graph_launch (shk_dev *d, void *exec);
+void shk_dev_graph_destroy (void *exec);
+/* pinned host memory */
+int  shk_host_alloc (void **p, size_t bytes);
+void shk_host_free (void *p);
 
 #ifdef __cplusplus
 }

@@ -1004,3 +1004,119 @@ orc_bd_noise_F (const orc_sys *s, int ncheb, const double *acoef,
   orc_cheb_eval_atimes (ncheb, acoef, n, y, z, e0, e1, orc_atimes_sys_cb,
                         (void *) s);
 }
+
+/* brownian.c:299-375 (check_overlap): number of (pair, image) entries with
+ * r^2 < rmin (a_i + a_j)^2 over pairs i <= j, the primary cell for i != j
+ * and the 26 neighbouring cells of the raw separation when periodic. */
+int
+orc_check_overlap (const orc_sys *s, const double *pos, double rmin)
+{
+  int flag = 0;
+  for (int i = 0; i < s->np; i++)
+    for (int j = i; j < s->np; j++)
+      {
+        double x = pos[3 * i] - pos[3 * j];
+        double y = pos[3 * i + 1] - pos[3 * j + 1];
+        double z = pos[3 * i + 2] - pos[3 * j + 2];
+        double a2 = s->a ? (s->a[i] + s->a[j]) * (s->a[i] + s->a[j]) : 4.0;
+        if (i != j && x * x + y * y + z * z < rmin * a2) flag++;
+        if (!s->periodic) continue;
+        for (int mx = -1; mx <= 1; mx++)
+          for (int my = -1; my <= 1; my++)
+            for (int mz = -1; mz <= 1; mz++)
+              {
+                if (mx == 0 && my == 0 && mz == 0) continue;
+                double xx = x + mx * s->lx, yy = y + my * s->ly, zz = z + mz * s->lz;
+                if (s->shear_mode == 1) xx += my * s->shear_shift;
+                else if (s->shear_mode == 2) xx += mz * s->shear_shift;
+                if (xx * xx + yy * yy + zz * zz < rmin * a2) flag++;
+              }
+      }
+  return flag;
+}
+
+/* brownian.c:2177-2187 */
+static double
+orc_bd_fact (double peclet, double dt)
+{
+  return peclet > 0.0 ? sqrt (2.0 / (peclet * dt)) : 0.0;
+}
+
+/* One mid-point step, F version, every particle mobile, no imposed flow,
+ * no interaction forces: brownian.c:2352-2560 with BD_calc_FB's atimes
+ * route (1545-1874, spectral bounds and coefficients given, so the refit
+ * of 1707-1727 does not happen), BD_calc_forces (2273-2311) and
+ * evolve_Euler_3all (1889-1930).  The mobility of the F version is one
+ * atimes_3all (brownian.c:512-520, ewald-3f.c:281-285).
+ *
+ * draws[ndraw][3np] are the Gaussian vectors BD_calc_FB consumes, one per
+ * (re)start of the noise.  fix_fact != 0 recomputes fact after a redraw
+ * resets dt (the reference keeps the halved-dt value).  x is updated in
+ * place; the return value is the dt taken, or -1 when more draws are
+ * needed.  Counts go to *used / *halvings; *err gets the Chebyshev
+ * accuracy of the last noise. */
+double
+orc_bd_mid_F (orc_sys *s, double *x, const double *F, double peclet,
+              double dt, double dt_lim, double rmin_ol, int ncheb,
+              const double *acoef, double e0, double e1, int ndraw,
+              const double *draws, int fix_fact, int *used, int *halvings,
+              double *err)
+{
+  const int n = 3 * s->np;
+  double *z = malloc (sizeof (double) * n);
+  double *f = malloc (sizeof (double) * n);
+  double *u = malloc (sizeof (double) * n);
+  double *xmid = malloc (sizeof (double) * n);
+  double dt_local = dt;
+  double fact = orc_bd_fact (peclet, dt_local);
+  int k = 0, nh = 0, stage;
+  double ret = -1.0;
+  const double *pos0 = s->pos;
+
+redo:                              /* BD_evolve_mid_REDO */
+  s->pos = x;
+redo_fb:                           /* BD_evolve_mid_REDO_FB */
+  if (k >= ndraw) goto done;
+  orc_bd_noise_F (s, ncheb, acoef, e0, e1, draws + (size_t) k * n, z);
+  *err = orc_cheb_error_minvsqrt (n, draws + (size_t) k * n, z,
+                                  orc_atimes_sys_cb, (void *) s);
+  k++;
+redo_scale:                        /* BD_evolve_mid_REDO_scale */
+  for (int i = 0; i < n; i++) xmid[i] = x[i];
+  for (int i = 0; i < n; i++) f[i] = (F ? F[i] : 0.0) + fact * z[i];
+  s->pos = x;
+  orc_atimes_3all (s, f, u);
+  for (int i = 0; i < n; i++) xmid[i] += dt_local * 0.5 * u[i];
+  stage = 1;
+  if (s->rmin == 0.0 && orc_check_overlap (s, xmid, rmin_ol) > 0)
+    goto overlap;
+  s->pos = xmid;
+  for (int i = 0; i < n; i++) f[i] = (F ? F[i] : 0.0) + fact * z[i];
+  orc_atimes_3all (s, f, u);
+  for (int i = 0; i < n; i++) xmid[i] = x[i];
+  for (int i = 0; i < n; i++) xmid[i] += dt_local * u[i];
+  stage = 2;
+  if (s->rmin == 0.0 && orc_check_overlap (s, xmid, rmin_ol) > 0)
+    goto overlap;
+  for (int i = 0; i < n; i++) x[i] = xmid[i];
+  ret = dt_local;
+  goto done;
+overlap:
+  dt_local *= 0.5;
+  nh++;
+  if (dt_local < dt_lim)
+    {
+      dt_local = dt;
+      if (fix_fact) fact = orc_bd_fact (peclet, dt_local);
+      if (stage == 1) goto redo_fb;
+      goto redo;
+    }
+  fact = orc_bd_fact (peclet, dt_local);
+  goto redo_scale;
+done:
+  s->pos = pos0;
+  *used = k;
+  *halvings = nh;
+  free (z); free (f); free (u); free (xmid);
+  return ret;
+}

@@ -117,6 +117,16 @@ double        orc_rng_gaussian (orc_rng *r);
 void orc_bd_noise_F (const orc_sys *s, int ncheb, const double *acoef,
                      double e0, double e1, const double *y, double *z);
 
+/* overlapping (pair, image) count (brownian.c:299-375) */
+int orc_check_overlap (const orc_sys *s, const double *pos, double rmin);
+
+/* one mid-point BD step, F version (brownian.c:2352-2560); see oracle.c */
+double orc_bd_mid_F (orc_sys *s, double *x, const double *F, double peclet,
+                     double dt, double dt_lim, double rmin_ol, int ncheb,
+                     const double *acoef, double e0, double e1, int ndraw,
+                     const double *draws, int fix_fact, int *used,
+                     int *halvings, double *err);
+
 #ifdef __cplusplus
 }
 #endif

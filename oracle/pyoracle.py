@@ -68,6 +68,8 @@ def lib():
         L.orc_rng_int32.restype = C.c_ulong
         L.orc_rng_real2.restype = C.c_double
         L.orc_rng_gaussian.restype = C.c_double
+        L.orc_bd_mid_F.restype = C.c_double
+        L.orc_check_overlap.restype = C.c_int
         L.orc_xi_by_tratio.argtypes = [C.POINTER(OrcSys), C.c_double]
         L.orc_set_xi.argtypes = [C.POINTER(OrcSys), C.c_double, C.c_double]
         L.orc_set_l.argtypes = [C.POINTER(OrcSys), C.c_double, C.c_double,
@@ -182,6 +184,31 @@ class System:
         else:
             lib().orc_matrix_nonewald(C.byref(self.s), _p(m))
         return m
+
+    def check_overlap(self, pos, rmin=1.0):
+        """overlapping (pair, image) count at positions pos (brownian.c:299-375)"""
+        pos = np.ascontiguousarray(pos, np.float64).reshape(-1)
+        return lib().orc_check_overlap(C.byref(self.s), _p(pos),
+                                       C.c_double(rmin))
+
+    def bd_mid_F(self, x, draws, peclet, dt, ncheb, acoef, e0, e1, F=None,
+                 dt_lim=1e-6, rmin_ol=1.0, fix_fact=False):
+        """one mid-point BD step (brownian.c:2352-2560), F version; returns
+        (x_new, dt_taken, draws_used, halvings, err_cheb)"""
+        x = np.array(x, dtype=np.float64).reshape(-1)
+        draws = np.ascontiguousarray(draws, np.float64).reshape(-1)
+        nd = draws.size // x.size
+        a = np.ascontiguousarray(acoef, np.float64)
+        used, nh, err = C.c_int(), C.c_int(), C.c_double()
+        Fp = None if F is None else _p(np.ascontiguousarray(F, np.float64))
+        L = lib()
+        dtt = L.orc_bd_mid_F(C.byref(self.s), _p(x), Fp, C.c_double(peclet),
+                             C.c_double(dt), C.c_double(dt_lim),
+                             C.c_double(rmin_ol), C.c_int(ncheb), _p(a),
+                             C.c_double(e0), C.c_double(e1), C.c_int(nd),
+                             _p(draws), C.c_int(1 if fix_fact else 0),
+                             C.byref(used), C.byref(nh), C.byref(err))
+        return x, dtt, used.value, nh.value, err.value
 
     # ---- compiled reference on the same system --------------------------
     def ref_atimes(self, x):

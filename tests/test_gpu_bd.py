@@ -1,0 +1,224 @@
+"""The device Brownian-dynamics step (stokes_hip_bd_*, F version, mid-point
+scheme of brownian.c:2352-2560) against the oracle's restatement of the
+same step, fed the Gaussian draw the device used.
+
+Bars: FP64 operator -- positions after a step within 1e-10 of the oracle
+relative to the displacement (the step is 31 FP64 operator products plus
+the Chebyshev recursion); z = M^{-1/2} y within the fit accuracy of the
+eigendecomposition closed form.  Each step is one replay of a captured
+graph; replays alternate with eager operator calls and host syncs.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import libstokes_amd as ls
+from libstokes_amd import workloads
+import pyoracle as po
+from conftest import relerr
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+def _setup(n, phi, seed=3, precision=ls.FP64):
+    pos, L = workloads.configuration(n, phi, seed=seed)
+    s = ls.Stokes(0, pos, box=(L, L, L), tr=1.0, precision=precision)
+    o = po.System(0, pos, box=(L, L, L), tr=1.0)
+    return pos, L, s, o
+
+
+def _spectrum(o):
+    M = o.matrix()
+    return np.linalg.eigh(0.5 * (M + M.T))
+
+
+def test_bd_steps_match_oracle():
+    n, phi = 27, 0.1
+    _say(f"bd parity N={n} phi={phi}")
+    pos, L, s, o = _setup(n, phi)
+    w, V = _spectrum(o)
+    e0, e1 = 0.95 * w[0], 1.05 * w[-1]
+    nch, pe, dt = 30, 1.0, 1e-3
+    a = po.cheb_coef(nch, 0, e0, e1)
+    rng = np.random.default_rng(2)
+    F = rng.standard_normal(3 * n)
+    Fd = ls.DeviceArray.from_host(F)
+    bd = ls.BD(s, peclet=pe, n_cheb=nch, eps=1e-6, dt_lim=1e-6, rmin_ol=1.0,
+               seed=7, force_dev=Fd, bounds=(e0, e1))
+    xd = ls.DeviceArray.from_host(pos.reshape(-1))
+    x = pos.reshape(-1).copy()
+    ys = []
+    for step in range(4):
+        info = bd.step(xd, dt)
+        y, z = bd.last_noise()
+        ys.append(y)
+        assert info.attempts == 1 and info.dt == dt
+        assert info.err_cheb < 1e-6
+        o.pos[:] = x
+        zt = V @ ((V.T @ y) / np.sqrt(w)) if step == 0 else None
+        if zt is not None:
+            assert relerr(z, zt) < 1e-7
+        xo, dto, used, nh, err = o.bd_mid_F(x, y[None], pe, dt, nch, a, e0,
+                                            e1, F=F)
+        assert used == 1 and nh == 0 and dto == dt
+        xg = xd.to_host()
+        disp = np.abs(xo - x).max()
+        _say(f"  step {step}: |x_gpu - x_oracle| / disp = "
+             f"{np.abs(xg - xo).max() / disp:.2e}")
+        assert np.abs(xg - xo).max() < 1e-10 * disp
+        x = xo
+        # eager operator work between replays: still the oracle's operator
+        s.set_pos(x.reshape(-1, 3))
+        v = rng.standard_normal(3 * n)
+        o.pos[:] = x
+        assert relerr(s.atimes(v), o.atimes(v)) < 1e-12
+    # fresh draws each step, standard normal
+    assert not np.array_equal(ys[0], ys[1])
+    bd.close()
+
+
+def test_bd_noise_statistics_and_seed():
+    n = 2048
+    pos, L = workloads.configuration(n, 0.2, seed=1)
+    s = ls.Stokes(0, pos, box=(L, L, L), tr=1.0, precision=ls.MIXED)
+    xd = ls.DeviceArray.from_host(pos.reshape(-1))
+    x0 = pos.reshape(-1).copy()
+    draws = []
+    for rep in range(2):
+        xd = ls.DeviceArray.from_host(x0)
+        bd = ls.BD(s, peclet=1.0, n_cheb=16, eps=1.0, seed=99,
+                   bounds=(0.2, 3.0))
+        bd.step(xd, 1e-4)
+        draws.append(bd.last_noise()[0])
+        bd.close()
+    y = draws[0]
+    assert np.array_equal(draws[0], draws[1])          # same seed, same draw
+    m, v = y.mean(), y.var()
+    _say(f"  N(0,1) check on {y.size}: mean {m:.4f} var {v:.4f}")
+    assert abs(m) < 5 / math.sqrt(y.size)
+    assert abs(v - 1) < 6 * math.sqrt(2 / y.size)
+    # fourth moment of a normal is 3
+    assert abs((y**4).mean() - 3) < 0.3
+
+
+def _pair(gap=2.05, L=10.0):
+    pos = np.array([[2.0, 2.0, 2.0], [2.0 + gap, 2.0, 2.0]])
+    s = ls.Stokes(0, pos, box=(L, L, L), tr=1.0)
+    o = po.System(0, pos, box=(L, L, L), tr=1.0)
+    return pos, s, o
+
+
+def test_bd_overlap_halving_matches_oracle():
+    pos, s, o = _pair()
+    F = np.array([50.0, 0, 0, -50.0, 0, 0])
+    nch, e0, e1 = 20, 0.3, 1.5
+    a = po.cheb_coef(nch, 0, e0, e1)
+    bd = ls.BD(s, peclet=1e6, n_cheb=nch, eps=1.0, dt_lim=1e-8, seed=1,
+               force_dev=ls.DeviceArray.from_host(F), bounds=(e0, e1))
+    xd = ls.DeviceArray.from_host(pos.reshape(-1))
+    info = bd.step(xd, 0.01)
+    y, _ = bd.last_noise()
+    xo, dto, used, nh, _ = o.bd_mid_F(pos.reshape(-1), y[None], 1e6, 0.01, nch,
+                                      a, e0, e1, F=F, dt_lim=1e-8)
+    _say(f"  halvings gpu {info.halvings} oracle {nh}, dt {info.dt} {dto}")
+    assert info.halvings == nh >= 1 and info.redraws == 0
+    assert info.dt == dto
+    assert info.attempts == nh + 1
+    xg = xd.to_host()
+    assert np.abs(xg - xo).max() < 1e-10 * np.abs(xo - pos.reshape(-1)).max()
+    bd.close()
+
+
+def test_bd_redraw_below_dt_lim():
+    """F = 10 closes the gap 0.05 within dt = 0.01 but not within 0.005,
+    and dt_lim = 0.006: an overlap halves dt below dt_lim, so the draw is
+    rejected and dt restored (brownian.c:2501-2525)"""
+    F = np.array([10.0, 0, 0, -10.0, 0, 0])
+    # without noise every attempt overlaps: the step gives up
+    pos, s, o = _pair()
+    bd = ls.BD(s, peclet=1e12, n_cheb=20, eps=1.0, dt_lim=0.006, seed=5,
+               force_dev=ls.DeviceArray.from_host(F), bounds=(0.3, 1.5))
+    xd = ls.DeviceArray.from_host(pos.reshape(-1))
+    with pytest.raises(ls.StokesError, match="attempts"):
+        bd.step(xd, 0.01)
+    bd.close()
+    # with noise some draw separates the pair; the accepted step is the
+    # plain step of that draw at the restored dt
+    nch, e0, e1 = 20, 0.3, 1.5
+    a = po.cheb_coef(nch, 0, e0, e1)
+    seen = 0
+    for seed in range(1, 21):
+        pos, s, o = _pair()
+        bd = ls.BD(s, peclet=1.0, n_cheb=nch, eps=1.0, dt_lim=0.006,
+                   seed=seed, force_dev=ls.DeviceArray.from_host(F),
+                   bounds=(e0, e1))
+        xd = ls.DeviceArray.from_host(pos.reshape(-1))
+        info = bd.step(xd, 0.01)
+        y, _ = bd.last_noise()
+        bd.close()
+        assert info.dt == 0.01 and info.redraws == info.halvings
+        xg = xd.to_host()
+        assert o.check_overlap(xg, 1.0) == 0
+        xo, dto, used, nh, _ = o.bd_mid_F(pos.reshape(-1), y[None], 1.0, 0.01,
+                                          nch, a, e0, e1, F=F, dt_lim=0.006)
+        assert used == 1 and nh == 0 and dto == 0.01
+        assert np.abs(xg - xo).max() < 1e-10 * np.abs(xo - pos.reshape(-1)).max()
+        if info.redraws:
+            seen += 1
+            _say(f"  seed {seed}: redraws {info.redraws} attempts "
+                 f"{info.attempts}")
+        s.close()
+    assert seen >= 1
+
+
+def test_bd_lanczos_bounds_and_refit():
+    n, phi = 27, 0.1
+    pos, L, s, o = _setup(n, phi, seed=4)
+    w, V = _spectrum(o)
+    # no bounds: the first step estimates them by Lanczos
+    bd = ls.BD(s, peclet=1.0, n_cheb=30, eps=1e-6, seed=3)
+    xd = ls.DeviceArray.from_host(pos.reshape(-1))
+    info = bd.step(xd, 1e-4)
+    _say(f"  lanczos bounds {info.eig[0]:.6f} {info.eig[1]:.6f} "
+         f"exact {w[0]:.6f} {w[-1]:.6f}, err {info.err_cheb:.2e}")
+    assert info.eig[0] <= w[0] * (1 + 1e-6) and info.eig[1] >= w[-1] * (1 - 1e-6)
+    assert info.eig[0] > 0.5 * w[0] and info.eig[1] < 1.5 * w[-1]
+    assert info.refits == 0 and info.err_cheb < 1e-6
+    bd.close()
+    # bounds far off: the accuracy check refits once and re-evaluates
+    xd = ls.DeviceArray.from_host(pos.reshape(-1))
+    bd = ls.BD(s, peclet=1.0, n_cheb=30, eps=1e-6, seed=3,
+               bounds=(0.01 * w[0], 0.02 * w[0]))
+    info = bd.step(xd, 1e-4)
+    y, z = bd.last_noise()
+    _say(f"  refit: refits {info.refits} err {info.err_cheb:.2e}")
+    assert info.refits == 1 and info.err_cheb < 1e-6
+    zt = V @ ((V.T @ y) / np.sqrt(w))
+    assert relerr(z, zt) < 1e-6
+    bd.close()
+
+
+def test_bd_mixed_precision_step():
+    n, phi = 64, 0.2
+    pos, L, s, o = _setup(n, phi, seed=6, precision=ls.MIXED)
+    w, V = _spectrum(o)
+    e0, e1 = 0.95 * w[0], 1.05 * w[-1]
+    nch = 30
+    a = po.cheb_coef(nch, 0, e0, e1)
+    bd = ls.BD(s, peclet=1.0, n_cheb=nch, eps=1e-5, seed=8, bounds=(e0, e1))
+    xd = ls.DeviceArray.from_host(pos.reshape(-1))
+    info = bd.step(xd, 1e-3)
+    y, z = bd.last_noise()
+    xo, dto, used, nh, _ = o.bd_mid_F(pos.reshape(-1), y[None], 1.0, 1e-3,
+                                      nch, a, e0, e1)
+    disp = xo - pos.reshape(-1)
+    e = relerr(xd.to_host() - pos.reshape(-1), disp)
+    _say(f"  mixed step displacement rel err {e:.2e}")
+    assert info.attempts == 1
+    assert e < 1e-5
+    bd.close()

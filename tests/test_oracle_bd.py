@@ -1,0 +1,190 @@
+"""Pins of the oracle's Brownian-dynamics step (F version, mid-point
+scheme, brownian.c:2352-2560) and of its overlap count (brownian.c:299-375).
+
+* the overlap count against a brute-force enumeration of the 27 cells;
+* one sphere in a periodic box, where M = mu I: the step must equal
+  x + dt (mu F + fact sqrt(mu) y) with mu the self-mobility (Hasimoto's
+  closed form to 1e-6, the operator's own value to 1e-10);
+* eight spheres: the step against the same composition built from the
+  compiled reference's own chebyshev_eval_atimes and atimes_3all;
+* the overlap branches: halving dt keeps the draw, a redraw below dt_lim
+  takes the next draw and restores dt, with the reference's stale fact.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import pyoracle as po
+
+
+def _brute_overlap(pos, L, rmin, periodic=True, shear_mode=0, shift=0.0):
+    pos = pos.reshape(-1, 3)
+    n = len(pos)
+    cnt = 0
+    cells = [(0, 0, 0)]
+    if periodic:
+        cells = [(a, b, c) for a in (-1, 0, 1) for b in (-1, 0, 1)
+                 for c in (-1, 0, 1)]
+    for i in range(n):
+        for j in range(i, n):
+            d = pos[i] - pos[j]
+            for (a, b, c) in cells:
+                if i == j and (a, b, c) == (0, 0, 0):
+                    continue
+                v = d + np.array([a * L, b * L, c * L])
+                if shear_mode == 1:
+                    v[0] += b * shift
+                elif shear_mode == 2:
+                    v[0] += c * shift
+                cnt += v @ v < rmin * 4.0
+    return cnt
+
+
+@pytest.mark.parametrize("shear_mode", [0, 1, 2])
+def test_overlap_count_brute_force(shear_mode):
+    rng = np.random.default_rng(5)
+    L = 6.0
+    pos = rng.uniform(0, L, (12, 3))
+    pos[3] = pos[2] + [1.0, 0.5, 0.2]          # planted primary overlap
+    pos[7] = pos[6] + [L - 0.9, 0.0, 0.3]      # overlap through a cell wall
+    s = po.System(0, pos, box=(L, L, L), tr=1.0, shear_mode=shear_mode,
+                  shear_shift=0.7 if shear_mode else 0.0)
+    for rmin in (1.0, 0.5):
+        want = _brute_overlap(pos, L, rmin, True, shear_mode,
+                              0.7 if shear_mode else 0.0)
+        assert s.check_overlap(pos, rmin) == want
+    assert s.check_overlap(pos, 1.0) >= 2
+
+
+def test_bd_single_sphere_closed_form():
+    L = 12.0
+    s = po.System(0, np.zeros((1, 3)), box=(L, L, L), tr=1.0)
+    mu = s.atimes(np.array([1.0, 0.0, 0.0]))[0]
+    x = 1.0 / L
+    assert mu == pytest.approx(1 - 2.837297479 * x + 4 * math.pi / 3 * x**3,
+                               rel=1e-6)
+    F = np.array([0.3, -1.2, 0.5])
+    y = np.array([0.7, 1.1, -0.4])
+    pe, dt, nch = 2.0, 1e-2, 30
+    e0, e1 = 0.8 * mu, 1.2 * mu
+    a = po.cheb_coef(nch, 0, e0, e1)
+    x0 = np.array([0.5, 1.0, -2.0])
+    xn, dtt, used, nh, err = s.bd_mid_F(x0, y[None], pe, dt, nch, a, e0, e1,
+                                        F=F)
+    fact = math.sqrt(2 / (pe * dt))
+    want = x0 + dt * (mu * F + fact * math.sqrt(mu) * y)
+    assert dtt == dt and used == 1 and nh == 0
+    assert err < 1e-12
+    assert np.abs(xn - want).max() < 1e-10 * np.abs(want - x0).max()
+
+
+@pytest.mark.skipif(not po.ref_available(), reason="oracle/_ref not built")
+def test_bd_step_vs_reference_composition():
+    """brownian.c:2352-2560 composed from the compiled reference's own
+    chebyshev_eval_atimes and atimes_3all (no overlap in this step)"""
+    L = 8.0
+    pos = np.array([[1, 1, 1], [4, 1.2, 0.8], [1.1, 4.2, 1], [4, 4, 1.3],
+                    [1, 1.1, 4], [4.2, 1, 4], [1, 4, 4.1], [4, 4.1, 4.2]],
+                   dtype=np.float64) + 0.3
+    s = po.System(0, pos, box=(L, L, L), tr=1.0)
+    n = pos.size
+    rng = np.random.default_rng(11)
+    y = rng.standard_normal(n)
+    F = rng.standard_normal(n)
+    pe, dt, nch = 1.0, 1e-3, 30
+    M = s.matrix()
+    w = np.linalg.eigvalsh(0.5 * (M + M.T))
+    e0, e1 = 0.95 * w[0], 1.05 * w[-1]
+    a = po.cheb_coef(nch, 0, e0, e1)
+    xn, dtt, used, nh, err = s.bd_mid_F(pos, y[None], pe, dt, nch, a, e0, e1,
+                                        F=F)
+    assert used == 1 and nh == 0 and dtt == dt
+
+    import ctypes as C
+    R = po.ref()
+
+    def ref_sys(p):
+        t = po.System(0, p, box=(L, L, L), tr=1.0)
+        return t, R.ref_sys_new(C.byref(t.s))
+
+    t0, h0 = ref_sys(pos)
+    z = np.zeros(n)
+    R.ref_cheb_eval_sys(nch, po._p(a), h0, po._p(y), po._p(z), e0, e1)
+    fact = math.sqrt(2 / (pe * dt))
+    f = F + fact * z
+    u = np.zeros(n)
+    R.ref_atimes(h0, n, po._p(f), po._p(u))
+    xmid = pos.reshape(-1) + 0.5 * dt * u
+    t1, h1 = ref_sys(xmid)
+    R.ref_atimes(h1, n, po._p(f), po._p(u))
+    want = pos.reshape(-1) + dt * u
+    R.ref_sys_free(h0)
+    R.ref_sys_free(h1)
+    disp = want - pos.reshape(-1)
+    assert np.abs(xn - want).max() < 1e-12 * np.abs(disp).max()
+    # z = M^{-1/2} y to the fit accuracy
+    wv, V = np.linalg.eigh(0.5 * (M + M.T))
+    zt = V @ ((V.T @ y) / np.sqrt(wv))
+    assert np.linalg.norm(z - zt) < 1e-6 * np.linalg.norm(zt)
+
+
+def _two_spheres(gap, L=10.0):
+    pos = np.array([[2.0, 2.0, 2.0], [2.0 + gap, 2.0, 2.0]])
+    return po.System(0, pos, box=(L, L, L), tr=1.0), pos
+
+
+def test_bd_overlap_halving_keeps_draw():
+    """a large first step that overlaps: dt is halved (fact recomputed) and
+    the same draw is reused until the step is accepted"""
+    s, pos = _two_spheres(2.05)
+    y = np.zeros(6)
+    F = np.array([50.0, 0, 0, -50.0, 0, 0])    # pushes the pair together
+    nch = 20
+    a = po.cheb_coef(nch, 0, 0.3, 1.5)
+    xn, dtt, used, nh, err = s.bd_mid_F(pos, y[None], 1.0, 0.01, nch, a,
+                                        0.3, 1.5, F=F, dt_lim=1e-8)
+    assert used == 1 and nh >= 1
+    assert dtt == pytest.approx(0.01 / 2**nh)
+    assert s.check_overlap(xn, 1.0) == 0
+    # the accepted step is the plain mid-point step at the final dt
+    x2, dt2, _, nh2, _ = s.bd_mid_F(pos, y[None], 1.0, dtt, nch, a, 0.3, 1.5,
+                                    F=F, dt_lim=1e-8)
+    assert nh2 == 0 and dt2 == dtt
+    assert np.array_equal(xn, x2)
+
+
+def test_bd_overlap_redraw_below_dt_lim():
+    """below dt_lim the draw is rejected and dt restored (REDO_FB); the
+    reference keeps the halved-dt fact, fix_fact recomputes it"""
+    s, pos = _two_spheres(2.05)
+    F = np.array([50.0, 0, 0, -50.0, 0, 0])
+    nch = 20
+    a = po.cheb_coef(nch, 0, 0.3, 1.5)
+    draws = np.zeros((3, 6))
+    draws[1, 0] = -6.0        # second draw pulls the pair apart
+    draws[1, 3] = 6.0
+    draws[2] = draws[1]
+    dt, dt_lim = 0.01, 0.003
+    # dt 0.01 overlaps, 0.005 (>= dt_lim, same draw) overlaps, 0.0025 is
+    # below dt_lim: redraw at dt = 0.01 with fact still that of dt = 0.005
+    xn, dtt, used, nh, _ = s.bd_mid_F(pos, draws, 1.0, dt, nch, a, 0.3, 1.5,
+                                      F=F, dt_lim=dt_lim)
+    assert used == 2 and nh == 2 and dtt == dt
+    fact_stale, fact_ok = math.sqrt(2 / 0.005), math.sqrt(2 / 0.01)
+    xf, dtf, usedf, nhf, _ = s.bd_mid_F(pos, draws, 1.0, dt, nch, a, 0.3, 1.5,
+                                        F=F, dt_lim=dt_lim, fix_fact=True)
+    assert usedf == 2 and nhf == 2 and dtf == dt
+    # both equal a plain step with the second draw, fact scaled accordingly:
+    # Pe' = Pe fact_ok^2 / fact^2 gives the same fact at dt
+    for xx, fact in ((xn, fact_stale), (xf, fact_ok)):
+        pe = 2 / (fact**2 * dt)
+        xw, dtw, uw, nw, _ = s.bd_mid_F(pos, draws[1:2], pe, dt, nch, a, 0.3,
+                                        1.5, F=F, dt_lim=dt_lim)
+        assert nw == 0 and uw == 1
+        assert np.allclose(xx, xw, rtol=0, atol=1e-14)
+    assert np.abs(xn - xf).max() > 1e-3
+    # running out of draws reports -1
+    _, dtn, usedn, _, _ = s.bd_mid_F(pos, draws[:1], 1.0, dt, nch, a, 0.3, 1.5,
+                                     F=F, dt_lim=dt_lim)
+    assert dtn == -1.0 and usedn == 1
