@@ -46,6 +46,21 @@ int stokes_hip_set_precision (struct stokes *sys, int mode);
  * library reproduces the reference; enable != 0 uses the other sign, which
  * makes the FTS sum independent of xi (to the Ewald tolerance). */
 int stokes_hip_set_recip_g_fix (struct stokes *sys, int enable);
+/* Compute the operator at the Ewald splitting xi_by_tratio(sys, tratio)
+ * (stokes.c:765-776; r_max, k_max and the image table from the same
+ * set-up rule and sys->ewald_eps) instead of sys->xi.  The Ewald sum does
+ * not depend on the splitting, to the Ewald tolerance, except for the
+ * reference's reciprocal G sign of the FTS version; that term is restored
+ * at sys->xi for equal spheres (FTS with unequal radii and the reference
+ * sign keeps sys->xi).  Results therefore match the reference at sys->xi
+ * within the Ewald tolerance while the real/reciprocal balance is chosen
+ * for speed.  tratio = 0 computes at sys->xi (the default). */
+int stokes_hip_set_split (struct stokes *sys, double tratio);
+/* time reps operator calls of nrhs columns at each candidate tratio (NULL:
+ * 0 (= sys->xi), 10, 100, 300, 1000, 3000, 1e4) and keep the fastest as
+ * the splitting of sys */
+int stokes_hip_autotune (struct stokes *sys, int nrhs, const double *trs,
+                         int ntr, int reps, double *best_tr, double *best_ms);
 /* record per-part GPU times (ms) into sys->cpu1/cpu2/cpu3 */
 int stokes_hip_set_timing (struct stokes *sys, int enable);
 

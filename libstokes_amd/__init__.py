@@ -45,7 +45,8 @@ EXPORTS = [
     "chebyshev_error_minvsqrt", "chebyshev_error_Rsqrt",
     # stokes_hip.h
     "stokes_hip_last_error", "stokes_hip_set_precision",
-    "stokes_hip_set_recip_g_fix",
+    "stokes_hip_set_recip_g_fix", "stokes_hip_set_split",
+    "stokes_hip_autotune",
     "stokes_hip_set_timing", "stokes_hip_malloc", "stokes_hip_free",
     "stokes_hip_memcpy_h2d", "stokes_hip_memcpy_d2h", "stokes_hip_sync",
     "stokes_hip_copy_calibrate",
@@ -136,6 +137,9 @@ def lib():
     for f in ("stokes_hip_set_precision", "stokes_hip_set_timing",
               "stokes_hip_set_recip_g_fix"):
         getattr(L, f).argtypes = [sp, C.c_int]
+    L.stokes_hip_set_split.argtypes = [sp, C.c_double]
+    L.stokes_hip_autotune.argtypes = [sp, C.c_int, _dp, C.c_int, C.c_int,
+                                      _dp, _dp]
     L.stokes_hip_atimes.argtypes = [sp, C.c_int, _dp, _dp]
     L.stokes_hip_atimes_dev.argtypes = [sp, C.c_int, _vp, _vp]
     L.stokes_hip_make_matrix.argtypes = [sp, _dp]
@@ -268,6 +272,25 @@ class Stokes:
     def set_recip_g_fix(self, on=True):
         check(self.L.stokes_hip_set_recip_g_fix(self.sys, 1 if on else 0),
               "set_recip_g_fix")
+
+    def set_split(self, tratio):
+        """compute at the splitting of tratio (0: sys->xi); results stay
+        those of sys->xi to the Ewald tolerance"""
+        check(self.L.stokes_hip_set_split(self.sys, float(tratio)),
+              "set_split")
+
+    def autotune(self, nrhs=1, trs=None, reps=3):
+        """time candidate splittings, keep the fastest; returns (tr, ms)"""
+        bt, bm = C.c_double(), C.c_double()
+        if trs is None:
+            tp, nt = None, 0
+        else:
+            arr = np.ascontiguousarray(trs, dtype=np.float64)
+            tp, nt = _p(arr), arr.size
+        check(self.L.stokes_hip_autotune(self.sys, nrhs, tp, nt, reps,
+                                         C.byref(bt), C.byref(bm)),
+              "autotune")
+        return bt.value, bm.value
 
     def set_timing(self, on=True):
         check(self.L.stokes_hip_set_timing(self.sys, 1 if on else 0),

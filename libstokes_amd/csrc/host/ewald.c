@@ -13,6 +13,7 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <time.h>
 #include "stokes.h"
 #include "ewald.h"
 #include "non-ewald.h"
@@ -167,11 +168,12 @@ stokes_hip_global_dev (void)
  * come with the odd factor sin), so the weights carry a factor 2. */
 static int
 build_kvectors (const struct stokes *s, double gsign, double **kv_out,
-                double **ks_out)
+                double **ks_out, int **km_out)
 {
   int cap = 64, nk = 0;
   double *kv = malloc (sizeof (double) * 4 * cap);
   double *ks = malloc (sizeof (double) * 8 * cap);
+  int *km = malloc (sizeof (int) * 4 * cap);
   double k0x = 2.0 * M_PI / s->lx;
   double k0y = 2.0 * M_PI / s->ly;
   double k0z = 2.0 * M_PI / s->lz;
@@ -199,8 +201,11 @@ build_kvectors (const struct stokes *s, double gsign, double **kv_out,
               cap *= 2;
               kv = realloc (kv, sizeof (double) * 4 * cap);
               ks = realloc (ks, sizeof (double) * 8 * cap);
+              km = realloc (km, sizeof (int) * 4 * cap);
             }
           double *v = kv + 4 * nk, *w = ks + 8 * nk;
+          km[4 * nk] = m1; km[4 * nk + 1] = m2; km[4 * nk + 2] = m3;
+          km[4 * nk + 3] = 0;
           v[0] = k1; v[1] = k2; v[2] = k3; v[3] = k;
           w[0] = 2.0 * 6.0 * (1.0 - kk / 3.0) * kexp;              /* ya */
           w[1] = 2.0 * 3.0 * k * kexp;                             /* yb */
@@ -214,6 +219,70 @@ build_kvectors (const struct stokes *s, double gsign, double **kv_out,
         }
   *kv_out = kv;
   *ks_out = ks;
+  if (km_out) *km_out = km;
+  else free (km);
+  return nk;
+}
+
+/* Reference-sign G term at the splitting of sys, added to reciprocal
+ * tables built at another splitting with the xi-independent sign.
+ *
+ * With either sign s_G of the reciprocal G / G-tilde weight, the FTS sum is
+ * y(xi; s_G) = y(xi; -1) + (1 + s_G) K_G(xi), K_G the reciprocal G part
+ * with weight +yg.  y(xi; -1) does not depend on xi (to the Ewald
+ * tolerance), so the reference's y(xi_1; +1) = y(xi_t; -1) + 2 K_G(xi_1):
+ * 2 yg(xi_1) goes onto the G weight of each k of the xi_1 sphere -- the
+ * same lattice vector in the xi_t tables, or a new G-only entry.  Returns
+ * the new count (the tables are reallocated), or -1. */
+static int
+add_g_correction (const struct stokes *s, double **kv_io, double **ks_io,
+                  int *km_t, int nk_t)
+{
+  double *kv1 = NULL, *ks1 = NULL;
+  int *km1 = NULL;
+  int nk1 = build_kvectors (s, 1.0, &kv1, &ks1, &km1);
+  int KX = 0, KY = 0, KZ = 0;
+  for (int k = 0; k < nk_t; k++)
+    {
+      KX = km_t[4 * k] > KX ? km_t[4 * k] : KX;
+      KY = abs (km_t[4 * k + 1]) > KY ? abs (km_t[4 * k + 1]) : KY;
+      KZ = abs (km_t[4 * k + 2]) > KZ ? abs (km_t[4 * k + 2]) : KZ;
+    }
+  long ny = 2 * KY + 1, nz = 2 * KZ + 1;
+  long ncell = (KX + 1) * ny * nz;
+  int *slot = malloc (sizeof (int) * (ncell > 0 ? ncell : 1));
+  double *kv = realloc (*kv_io, sizeof (double) * 4 * (nk_t + nk1 + 1));
+  double *ks = realloc (*ks_io, sizeof (double) * 8 * (nk_t + nk1 + 1));
+  if (!slot || !kv || !ks || !kv1 || !ks1 || !km1)
+    {
+      free (slot); free (kv1); free (ks1); free (km1);
+      if (kv) *kv_io = kv;
+      if (ks) *ks_io = ks;
+      return -1;
+    }
+  *kv_io = kv;
+  *ks_io = ks;
+  for (long c = 0; c < ncell; c++) slot[c] = -1;
+  for (int k = 0; k < nk_t; k++)
+    slot[(km_t[4 * k] * ny + km_t[4 * k + 1] + KY) * nz + km_t[4 * k + 2] + KZ] = k;
+  int nk = nk_t;
+  for (int k = 0; k < nk1; k++)
+    {
+      int m1 = km1[4 * k], m2 = km1[4 * k + 1], m3 = km1[4 * k + 2];
+      int t = -1;
+      if (m1 <= KX && abs (m2) <= KY && abs (m3) <= KZ)
+        t = slot[(m1 * ny + m2 + KY) * nz + m3 + KZ];
+      if (t < 0)
+        {
+          t = nk++;
+          memcpy (kv + 4 * t, kv1 + 4 * k, sizeof (double) * 4);
+          for (int q = 0; q < 8; q++) ks[8 * t + q] = 0.0;
+          ks[8 * t + 6] = ks1[8 * k + 6];
+          ks[8 * t + 7] = ks1[8 * k + 7];
+        }
+      ks[8 * t + 3] += 2.0 * ks1[8 * k + 3];
+    }
+  free (slot); free (kv1); free (ks1); free (km1);
   return nk;
 }
 
@@ -335,35 +404,55 @@ stokes_hip_prepare (stokes_ctx *c, int positions_on_device)
       stokes_hip_set_error ("slip particles are not supported");
       return STOKES_HIP_EUNSUP;
     }
-  shk_params p;
-  memset (&p, 0, sizeof (p));
-  p.version = s->version;
-  p.np = s->np;
-  p.periodic = s->periodic ? 1 : 0;
-  p.poly = s->a != NULL;
-  p.lx = s->lx; p.ly = s->ly; p.lz = s->lz; p.pivol = s->pivol;
-  p.xi = s->xi; p.xi2 = s->xi2; p.xiaspi = s->xiaspi; p.xia2 = s->xia2;
-  p.rmax2 = s->rmax2; p.rmin = s->rmin;
-  p.self_a = s->self_a; p.self_c = s->self_c; p.self_m = s->self_m;
-  p.nr = p.periodic ? s->nr : 0;
-  p.gsign = c->fix_recip_g ? -1.0 : 1.0;
+  shk_params pu;
+  memset (&pu, 0, sizeof (pu));
+  pu.version = s->version;
+  pu.np = s->np;
+  pu.periodic = s->periodic ? 1 : 0;
+  pu.poly = s->a != NULL;
+  pu.lx = s->lx; pu.ly = s->ly; pu.lz = s->lz; pu.pivol = s->pivol;
+  pu.xi = s->xi; pu.xi2 = s->xi2; pu.xiaspi = s->xiaspi; pu.xia2 = s->xia2;
+  pu.rmax2 = s->rmax2; pu.rmin = s->rmin;
+  pu.self_a = s->self_a; pu.self_c = s->self_c; pu.self_m = s->self_m;
+  pu.nr = pu.periodic ? s->nr : 0;
+  pu.gsign = c->fix_recip_g ? -1.0 : 1.0;
+  /* another splitting computes the same sum (to the Ewald tolerance) except
+   * for the reference-sign G term of FTS, which add_g_correction restores;
+   * that correction is written for equal spheres */
+  const int gcorr = s->version == 2 && !c->fix_recip_g;
+  const int split = c->split_tr > 0.0 && pu.periodic && s->shear_mode == 0
+    && !(gcorr && pu.poly);
   int full = !c->valid || c->shear_mode != s->shear_mode
-    || c->shear_shift != s->shear_shift || p.poly
-    || c->a_ptr != s->a;
+    || c->shear_shift != s->shear_shift || pu.poly
+    || c->a_ptr != s->a || (split ? c->split_tr : 0.0) != c->split_used_tr;
   /* nk is only known after enumeration; compare the rest */
-  p.nk = c->prm.nk;
-  if (!full && !same_params (&p, &c->prm)) full = 1;
+  pu.nk = c->prm_user.nk;
+  if (!full && !same_params (&pu, &c->prm_user)) full = 1;
   if (full)
     {
       double *img = NULL, *kv = NULL, *ks = NULL, *cand = NULL;
-      int *candm = NULL;
+      int *candm = NULL, *km = NULL;
       unsigned *tbits = NULL;
       shk_tables t;
       memset (&t, 0, sizeof (t));
+      struct stokes sh;
+      const struct stokes *src = s;
+      shk_params p = pu;
+      if (split)
+        {
+          stokes_hip_shadow_split (s, c->split_tr, &sh);
+          src = &sh;
+          p.xi = sh.xi; p.xi2 = sh.xi2; p.xiaspi = sh.xiaspi; p.xia2 = sh.xia2;
+          p.rmax2 = sh.rmax2;
+          p.self_a = sh.self_a; p.self_c = sh.self_c; p.self_m = sh.self_m;
+          p.nr = sh.nr;
+          if (gcorr) p.gsign = -1.0;
+        }
       if (p.periodic)
         {
-          if (p.nr <= 0 || s->rlx == NULL)
+          if (p.nr <= 0 || src->rlx == NULL)
             {
+              if (split) stokes_hip_shadow_free (&sh);
               stokes_hip_set_error ("periodic system without image table"
                                     " (call stokes_set_xi)");
               return STOKES_HIP_EARG;
@@ -372,28 +461,31 @@ stokes_hip_prepare (stokes_ctx *c, int positions_on_device)
           for (int m = 0; m < p.nr; m++)
             {
               double sx = 0.0;
-              if (s->shear_mode == 1) sx = (double) s->rmy[m] * s->shear_shift;
-              else if (s->shear_mode == 2) sx = (double) s->rmz[m] * s->shear_shift;
-              img[4 * m + 0] = s->rlx[m] + sx;
-              img[4 * m + 1] = s->rly[m];
-              img[4 * m + 2] = s->rlz[m];
+              if (s->shear_mode == 1) sx = (double) src->rmy[m] * s->shear_shift;
+              else if (s->shear_mode == 2) sx = (double) src->rmz[m] * s->shear_shift;
+              img[4 * m + 0] = src->rlx[m] + sx;
+              img[4 * m + 1] = src->rly[m];
+              img[4 * m + 2] = src->rlz[m];
               img[4 * m + 3] = 0.0;
             }
-          p.nk = build_kvectors (s, p.gsign, &kv, &ks);
+          p.nk = build_kvectors (src, p.gsign, &kv, &ks, &km);
+          if (split && gcorr)
+            p.nk = add_g_correction (s, &kv, &ks, km, p.nk);
           t.img = img;
           t.kv = kv;
           t.ks = ks;
           t.s2x = s->shear_mode == 1 ? s->shear_shift : 0.0;
           t.s3x = s->shear_mode == 2 ? s->shear_shift : 0.0;
-          t.ncand = build_candidates (s, t.s2x, t.s3x, &cand, &candm, &t.ncore);
+          t.ncand = build_candidates (src, t.s2x, t.s3x, &cand, &candm, &t.ncore);
           t.cand = cand;
           t.candm = candm;
-          t.tbits_words = build_table_bits (s, &tbits, &t.tx, &t.ty, &t.tz);
+          t.tbits_words = build_table_bits (src, &tbits, &t.tx, &t.ty, &t.tz);
           t.tbits = tbits;
-          if (t.ncand < 0 || t.tbits_words < 0)
+          if (t.ncand < 0 || t.tbits_words < 0 || p.nk < 0)
             {
-              free (img); free (kv); free (ks); free (cand); free (candm);
-              free (tbits);
+              free (img); free (kv); free (ks); free (km); free (cand);
+              free (candm); free (tbits);
+              if (split) stokes_hip_shadow_free (&sh);
               stokes_hip_set_error ("lattice tables: allocation failed");
               return STOKES_HIP_ENOMEM;
             }
@@ -402,9 +494,14 @@ stokes_hip_prepare (stokes_ctx *c, int positions_on_device)
         p.nk = 0;
       int rc = shk_dev_set_system (c->dev, &p, s->pos, s->a,
                                    p.periodic ? &t : NULL);
-      free (img); free (kv); free (ks); free (cand); free (candm); free (tbits);
+      free (img); free (kv); free (ks); free (km); free (cand); free (candm);
+      free (tbits);
+      if (split) stokes_hip_shadow_free (&sh);
       if (rc) return dev_fail (rc, "upload");
       c->prm = p;
+      pu.nk = p.nk;
+      c->prm_user = pu;
+      c->split_used_tr = split ? c->split_tr : 0.0;
       c->shear_mode = s->shear_mode;
       c->shear_shift = s->shear_shift;
       c->a_ptr = s->a;
@@ -472,6 +569,89 @@ stokes_hip_set_recip_g_fix (struct stokes *sys, int enable)
   if (!c) return STOKES_HIP_ENODEV;
   c->fix_recip_g = enable ? 1 : 0;
   c->valid = 0;
+  return 0;
+}
+
+int
+stokes_hip_set_split (struct stokes *sys, double tratio)
+{
+  stokes_ctx *c = ctx_or_error (sys);
+  if (!c) return STOKES_HIP_ENODEV;
+  if (tratio < 0.0 || tratio != tratio)
+    {
+      stokes_hip_set_error ("set_split: tratio must be >= 0");
+      return STOKES_HIP_EARG;
+    }
+  c->split_tr = tratio;
+  return 0;
+}
+
+static double
+now_s (void)
+{
+  struct timespec ts;
+  clock_gettime (CLOCK_MONOTONIC, &ts);
+  return (double) ts.tv_sec + 1e-9 * (double) ts.tv_nsec;
+}
+
+int
+stokes_hip_autotune (struct stokes *sys, int nrhs, const double *trs, int ntr,
+                     int reps, double *best_tr, double *best_ms)
+{
+  static const double dflt[] = {0.0, 10.0, 100.0, 300.0, 1000.0, 3000.0,
+                                10000.0};
+  stokes_ctx *c = ctx_or_error (sys);
+  if (!c) return STOKES_HIP_ENODEV;
+  if (nrhs < 1 || reps < 1)
+    {
+      stokes_hip_set_error ("autotune: nrhs and reps must be >= 1");
+      return STOKES_HIP_EARG;
+    }
+  if (trs == NULL || ntr <= 0)
+    {
+      trs = dflt;
+      ntr = (int) (sizeof (dflt) / sizeof (dflt[0]));
+    }
+  size_t n = (size_t) sys->np * stokes_dim (sys->version) * (size_t) nrhs;
+  double *x = NULL, *y = NULL;
+  int rc = shk_dev_malloc (c->dev, (void **) &x, sizeof (double) * n);
+  if (!rc) rc = shk_dev_malloc (c->dev, (void **) &y, sizeof (double) * n);
+  if (!rc) rc = shk_dev_scal (c->dev, x, NULL, 0.0, n);
+  if (rc)
+    {
+      if (x) shk_dev_free (c->dev, x);
+      stokes_hip_set_error ("autotune: %s", shk_last_error ());
+      return STOKES_HIP_ENOMEM;
+    }
+  const double keep = c->split_tr;
+  double bt = keep, bms = -1.0;
+  for (int k = 0; k < ntr && !rc; k++)
+    {
+      c->split_tr = trs[k];
+      rc = stokes_hip_prepare (c, c->pos_dev != NULL);
+      if (!rc) rc = shk_dev_atimes (c->dev, x, y, nrhs, c->prec);
+      if (!rc) rc = shk_dev_sync (c->dev);
+      double t0 = now_s ();
+      for (int r = 0; r < reps && !rc; r++)
+        rc = shk_dev_atimes (c->dev, x, y, nrhs, c->prec);
+      if (!rc) rc = shk_dev_sync (c->dev);
+      double ms = (now_s () - t0) * 1e3 / reps;
+      if (!rc && (bms < 0.0 || ms < bms))
+        {
+          bms = ms;
+          bt = trs[k];
+        }
+    }
+  shk_dev_free (c->dev, x);
+  shk_dev_free (c->dev, y);
+  if (rc)
+    {
+      c->split_tr = keep;
+      return dev_fail (rc, "autotune");
+    }
+  c->split_tr = bt;
+  if (best_tr) *best_tr = bt;
+  if (best_ms) *best_ms = bms;
   return 0;
 }
 

@@ -272,6 +272,23 @@ stokes_set_xi (struct stokes *s, double xi, double ewald_eps)
   stokes_hip_invalidate (s);
 }
 
+void
+stokes_hip_shadow_split (const struct stokes *s, double tratio,
+                         struct stokes *sh)
+{
+  *sh = *s;
+  sh->rlx = sh->rly = sh->rlz = NULL;
+  sh->rmx = sh->rmy = sh->rmz = NULL;
+  sh->nr = 0;
+  stokes_set_xi (sh, xi_by_tratio (sh, tratio), s->ewald_eps);
+}
+
+void
+stokes_hip_shadow_free (struct stokes *sh)
+{
+  table_free (sh);
+}
+
 double
 xi_by_tratio (struct stokes *s, double tratio)
 {

@@ -15,6 +15,9 @@ typedef struct stokes_ctx {
   int timing;
   int valid;                /* parameters uploaded and current */
   unsigned long uploads;    /* count of full parameter uploads */
+  shk_params prm_user;      /* parameters of sys itself at the last upload */
+  double split_tr;          /* > 0: compute at the splitting of this tratio */
+  double split_used_tr;     /* splitting of the uploaded tables (0: sys->xi) */
   shk_params prm;           /* last uploaded parameters */
   double shear_shift;       /* shift the k table and images were built for */
   int shear_mode;
@@ -39,6 +42,12 @@ void stokes_hip_release_context (struct stokes *sys);
 void stokes_hip_invalidate (struct stokes *sys);
 /* context not tied to a system (free-standing dense products) */
 shk_dev *stokes_hip_global_dev (void);
+/* sh = s with the Ewald splitting of tratio (stokes_set_xi at
+ * xi_by_tratio, same ewald_eps) and its own image table; release the table
+ * with stokes_hip_shadow_free */
+void stokes_hip_shadow_split (const struct stokes *s, double tratio,
+                              struct stokes *sh);
+void stokes_hip_shadow_free (struct stokes *sh);
 /* record an error message for stokes_hip_last_error() */
 void stokes_hip_set_error (const char *fmt, ...);
 int stokes_dim (int version);

@@ -1,0 +1,92 @@
+"""Computing at another Ewald splitting (stokes_hip_set_split / autotune)
+returns the reference's result at the system's own xi.
+
+The oracle at tr = 1 (the reference's default splitting, reference sign of
+the reciprocal G term) is the target; the GPU computes at tr = 10 ... 3000
+with the xi-independent sign plus the G-term correction at the system's xi.
+The corrected-sign sum is xi-independent to ~3e-14 (normwise, checked on the
+oracle for N = 64, tr = 1 ... 1000), so the FP64 bar stays 1e-12.
+"""
+import numpy as np
+import pytest
+
+import libstokes_amd as ls
+from libstokes_amd import workloads
+import pyoracle as po
+from conftest import relerr
+
+pytestmark = pytest.mark.gpu
+
+TOL64 = 1e-12
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+@pytest.mark.parametrize("version", [0, 1, 2])
+def test_split_matches_reference_splitting(version):
+    n = 64
+    pos, L = workloads.configuration(n, 0.1, seed=1)
+    o = po.System(version, pos, box=(L, L, L), tr=1.0)
+    x = np.random.default_rng(version).standard_normal(ls.dim(version) * n)
+    y_ref = o.atimes(x)
+    s = ls.Stokes(version, pos, box=(L, L, L), tr=1.0)
+    for tr in (10.0, 300.0, 3000.0):
+        s.set_split(tr)
+        e = relerr(s.atimes(x), y_ref)
+        _say(f"  version {version} split tr={tr:g}: rel err {e:.2e}")
+        assert e < TOL64
+    s.set_split(0.0)
+    assert relerr(s.atimes(x), y_ref) < TOL64
+    s.close()
+
+
+def test_split_matrix_and_fix_mode():
+    n = 27
+    pos, L = workloads.configuration(n, 0.1, seed=2)
+    o = po.System(2, pos, box=(L, L, L), tr=1.0)
+    m_ref = o.matrix()
+    s = ls.Stokes(2, pos, box=(L, L, L), tr=1.0)
+    s.set_split(1000.0)
+    m = s.matrix()
+    e = np.abs(m - m_ref).max() / np.abs(m_ref).max()
+    _say(f"  split matrix max rel err {e:.2e}")
+    assert e < 1e-12
+    # the xi-independent sign: no correction, the oracle's fix mode at tr=1
+    of = po.System(2, pos, box=(L, L, L), tr=1.0, fix_recip_g=True)
+    x = np.random.default_rng(5).standard_normal(11 * n)
+    s.set_recip_g_fix(True)
+    assert relerr(s.atimes(x), of.atimes(x)) < TOL64
+    s.close()
+
+
+def test_split_mixed_and_block_paths():
+    n = 64
+    pos, L = workloads.configuration(n, 0.1, seed=3)
+    o = po.System(2, pos, box=(L, L, L), tr=1.0)
+    rng = np.random.default_rng(9)
+    X = rng.standard_normal((5, 11 * n))
+    Y = np.stack([o.atimes(v) for v in X])
+    s = ls.Stokes(2, pos, box=(L, L, L), tr=1.0, precision=ls.MIXED)
+    s.set_split(1000.0)
+    xd = ls.DeviceArray.from_host(X.reshape(-1))
+    yd = ls.DeviceArray(X.size)
+    s.atimes_dev(xd, yd, nrhs=5)
+    Yg = yd.to_host().reshape(5, -1)
+    for c in range(5):
+        assert relerr(Yg[c], Y[c]) < 1e-6
+    s.close()
+
+
+def test_autotune_keeps_results():
+    n = 256
+    pos, L = workloads.configuration(n, 0.2, seed=4)
+    s = ls.Stokes(2, pos, box=(L, L, L), tr=1.0)
+    x = np.random.default_rng(1).standard_normal(11 * n)
+    y0 = s.atimes(x)                     # the system's own splitting
+    tr, ms = s.autotune(nrhs=1, reps=2)
+    _say(f"  autotune: tr={tr:g} {ms:.3f} ms")
+    assert tr in (0.0, 10.0, 100.0, 300.0, 1000.0, 3000.0, 10000.0)
+    assert relerr(s.atimes(x), y0) < TOL64
+    s.close()
