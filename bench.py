@@ -55,6 +55,23 @@ def pair_images_in_cutoff(pos, s):
     return count
 
 
+def measured_traffic(kernel, n, tr):
+    """HBM bytes per launch of `kernel` at (n, tr) from the committed counter
+    summary profiles/traffic.txt (FETCH_SIZE and WRITE_SIZE per dispatch,
+    each scaled by the calibration copy), or None"""
+    path = os.path.join(ROOT, "profiles", "traffic.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        f = line.split()
+        if len(f) >= 4 and not line.startswith("#"):
+            kv = dict(x.split("=", 1) for x in f[1:] if "=" in x)
+            if (f[0] == kernel.replace(" ", "") and int(kv.get("n", -1)) == n
+                    and float(kv.get("tr", -1)) == float(tr)):
+                return float(kv["bytes"])
+    return None
+
+
 def time_matvec(s, xd, yd, steps, warmup, nrhs=1):
     for _ in range(warmup):
         s.atimes_dev(xd, yd, nrhs)
@@ -132,13 +149,34 @@ def bench_bd(args, log):
                      "eig": [info.eig[0], info.eig[1]]}
         log(f"BD {name}: {out[name]['ms_per_step']:.3f} ms/step")
         bd.close()
+        # 64 noise vectors as one 3N x 64 block per Chebyshev recursion
+        # (z = M^{-1/2} y, order 30, the bounds of the run above)
+        e0, e1 = info.eig
+        from libstokes_amd import cheb_coef_inv_sqrt
+        acoef = cheb_coef_inv_sqrt(30, e0, e1)
+        nb = 64
+        yb = ls.DeviceArray.from_host(
+            np.random.default_rng(4).standard_normal(nb * 3 * n))
+        zb = ls.DeviceArray(nb * 3 * n)
+        s.cheb_dev(acoef, yb, zb, e0, e1, nrhs=nb)
+        s.sync()
+        t0 = time.perf_counter()
+        reps = 3
+        for _ in range(reps):
+            s.cheb_dev(acoef, yb, zb, e0, e1, nrhs=nb)
+        s.sync()
+        tc = (time.perf_counter() - t0) / reps
+        out[name]["cheb_block64_ms"] = tc * 1e3
+        out[name]["noise_vectors_per_s_block64"] = nb / tc
+        log(f"cheb block 64 {name}: {tc * 1e3:.3f} ms")
         s.close()
     return out
 
 
 def profile_run(args):
     """kernels for the counter runs: k_copy_calibrate on 256 MiB (a known
-    byte count), then the matvec kernels of both precisions"""
+    byte count), then --steps matvecs of both precisions at the splitting
+    of --tr (system at the reference's tr = 1, as in the timed run)"""
     n = args.n
     pos, L = workloads.configuration(n, args.phi, seed=1)
     x = np.random.default_rng(2).standard_normal(11 * n)
@@ -149,9 +187,10 @@ def profile_run(args):
     posd = ls.DeviceArray.from_host(pos)
     xd = ls.DeviceArray.from_host(x)
     yd = ls.DeviceArray(11 * n)
-    tr = args.tr if args.tr else 1000.0
+    tr = args.tr if args.tr else 3000.0
     for prec in (ls.MIXED, ls.FP64):
-        s = ls.Stokes(2, pos, box=(L, L, L), tr=tr, precision=prec)
+        s = ls.Stokes(2, pos, box=(L, L, L), tr=1.0, precision=prec)
+        s.set_split(0.0 if tr == 1.0 else tr)
         ls.check(s.L.stokes_hip_set_pos_dev(s.sys, posd.ptr), "set_pos_dev")
         t = time_matvec(s, xd, yd, args.steps, args.warmup)
         print(json.dumps({"precision": "mixed" if prec else "fp64", "tr": tr,
@@ -244,15 +283,20 @@ def main():
     err_b = float(np.linalg.norm(yb_mix - yb_64) / np.linalg.norm(yb_64))
     log(f"batch {nb}: mixed {tb_mix * 1e3:.3f} ms, fp64 {tb_64 * 1e3:.3f} ms")
 
-    # roofline of the dominant kernel (mixed real-space pairs)
+    # roofline of the dominant kernel (mixed real-space pairs) at the
+    # splitting it ran with: P_in pair-images inside r_max x 300 flop
     real_ms, recip_ms = kernel_times(smix, xd, yd, 5)
-    pin = pair_images_in_cutoff(pos, smix)
-    flops = pin * FLOP_PER_IMAGE_FTS + n * n * (2 * 121 + 150)
+    shadow = ls.Stokes(2, pos, box=(L, L, L), tr=tr)
+    pin = pair_images_in_cutoff(pos, shadow)
+    shadow.close()
+    flops = pin * FLOP_PER_IMAGE_FTS
     achieved = flops / (real_ms * 1e-3) / 1e12
     roof = dict(bound="mfma", achieved=achieved, peak=PEAK_FP32,
-                unit="TFLOP/s", frac=achieved / PEAK_FP32, traffic=None,
-                kernel="k_pairs<FTS,float> (real-space pairs, VALU FP32)",
-                kernel_ms=real_ms, recip_ms=recip_ms, pair_images=pin)
+                unit="TFLOP/s", frac=achieved / PEAK_FP32,
+                traffic=measured_traffic("k_real_compact<2, float>", n, tr),
+                kernel="k_real_compact<2, float> (real-space pairs, FP32 VALU)",
+                kernel_ms=real_ms, recip_ms=recip_ms, pair_images=pin,
+                flop_per_pair_image=FLOP_PER_IMAGE_FTS)
 
     out = {
         "metric": METRIC,

@@ -142,7 +142,8 @@ typedef struct stokes_hip_bd_info {
 } stokes_hip_bd_info;
 
 /* sys: version 0 (F), equal radii (a == NULL), no shear, set up as for
- * atimes_3all.  F_dev: constant force [3 np] (BD->F) or NULL.  peclet:
+ * atimes_3all.  F_dev: constant force [3 np] (BD->F) or NULL; it is read on
+ * every step and must stay allocated until stokes_hip_bd_destroy.  peclet:
  * fact = sqrt(2 / (Pe dt)), 0 when Pe <= 0 (brownian.c:2177-2187).
  * n_cheb: order of the 1/sqrt(x) fit (BD->n_minv > 0).  eps: refit
  * threshold (BD->eps).  dt_lim, rmin_ol: BD->dt_lim and BD->rmin (pairs

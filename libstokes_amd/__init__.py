@@ -375,6 +375,8 @@ class BD:
                  fact_fix=False):
         self.L = lib()
         self.stokes = stokes
+        # the captured step reads the force buffer on every replay: keep it
+        self.force_dev = force_dev
         h = _vp()
         check(self.L.stokes_hip_bd_create(
             stokes.sys, force_dev.ptr if force_dev is not None else None,
@@ -412,6 +414,20 @@ class BD:
             self.close()
         except Exception:
             pass
+
+
+def cheb_coef_inv_sqrt(n_cheb, e0, e1):
+    """Chebyshev coefficients of 1/sqrt(x) on [e0, e1] (chebyshev_coef of
+    the library, chebyshev.c:33-61)"""
+    L = lib()
+    a = np.zeros(n_cheb)
+    f = _INV_SQRT_CB
+    L.chebyshev_coef(int(n_cheb), f, C.c_double(e0), C.c_double(e1), _p(a))
+    return a
+
+
+_FUNC_CB = C.CFUNCTYPE(C.c_double, C.c_double)
+_INV_SQRT_CB = _FUNC_CB(lambda x: 1.0 / np.sqrt(x))
 
 
 def exported_symbols():
