@@ -98,6 +98,17 @@ int stokes_hip_make_matrix (struct stokes *sys, double *mat);
 int stokes_hip_extmat_dev (struct stokes *sys, const double *mat_dev,
                            int nrhs, const double *x_dev, double *y_dev);
 
+/* Dense FP64 mixed problem on the device: particles [0, nm) are mobile
+ * with known force / torque (and, FTS, known strain E), particles
+ * [nm, np) are fixed with known velocity, angular velocity (and E).
+ * b_dev[np * dim] holds the known values per particle in the reference's
+ * order (mobile: F T E, fixed: U O E), x_dev the unknowns (mobile: U O S,
+ * fixed: F T S); dim = 3, 6, 11 for F, FT, FTS.  The same problem as
+ * solve_mix_3fts_matrix_0 (ewald-3fts-matrix.c:1192-1290) for every
+ * version; LU on the GPU in place of the explicit LAPACK inverse. */
+int stokes_hip_solve_mix_dev (struct stokes *sys, const double *b_dev,
+                              double *x_dev);
+
 /* Chebyshev-Clenshaw z = f(M) y for nrhs vectors held on the device
  * (chebyshev.c:163-228 with the whole recursion on the GPU) */
 int stokes_hip_cheb_eval_dev (struct stokes *sys, int ncheb, const double *a,

@@ -54,6 +54,8 @@ EXPORTS = [
     "stokes_hip_atimes_dev", "stokes_hip_atimes", "stokes_hip_make_matrix_dev",
     "stokes_hip_make_matrix", "stokes_hip_extmat_dev",
     "stokes_hip_cheb_eval_dev", "stokes_hip_lanczos_bounds",
+    "stokes_hip_solve_mix_dev", "solve_mob_3fts_matrix_0",
+    "solve_mix_3fts_matrix_0",
     "stokes_hip_bd_create", "stokes_hip_bd_set_bounds",
     "stokes_hip_bd_set_fact_fix", "stokes_hip_bd_step",
     "stokes_hip_bd_last_noise", "stokes_hip_bd_destroy",
@@ -138,6 +140,9 @@ def lib():
               "stokes_hip_set_recip_g_fix"):
         getattr(L, f).argtypes = [sp, C.c_int]
     L.stokes_hip_set_split.argtypes = [sp, C.c_double]
+    L.stokes_hip_solve_mix_dev.argtypes = [sp, _vp, _vp]
+    L.solve_mob_3fts_matrix_0.argtypes = [sp] + [_dp] * 6
+    L.solve_mix_3fts_matrix_0.argtypes = [sp] + [_dp] * 12
     L.stokes_hip_autotune.argtypes = [sp, C.c_int, _dp, C.c_int, C.c_int,
                                       _dp, _dp]
     L.stokes_hip_atimes.argtypes = [sp, C.c_int, _dp, _dp]
@@ -291,6 +296,18 @@ class Stokes:
                                          C.byref(bt), C.byref(bm)),
               "autotune")
         return bt.value, bm.value
+
+    def solve_mix(self, b, nm=None):
+        """dense FP64 mixed problem (stokes_hip_solve_mix_dev): b holds the
+        known values per particle (mobile: F T E, fixed: U O E), the result
+        the unknowns (mobile: U O S, fixed: F T S)"""
+        if nm is not None:
+            self.s.nm = nm
+        bd = DeviceArray.from_host(np.ascontiguousarray(b, np.float64))
+        xd = DeviceArray(self.n)
+        check(self.L.stokes_hip_solve_mix_dev(self.sys, bd.ptr, xd.ptr),
+              "solve_mix")
+        return xd.to_host()
 
     def set_timing(self, on=True):
         check(self.L.stokes_hip_set_timing(self.sys, 1 if on else 0),

@@ -126,6 +126,13 @@ int shk_dev_d2h_async (shk_dev *d, void *dst, const void *src, size_t bytes);
 int shk_dev_h2d_async (shk_dev *d, void *dst, const void *src, size_t bytes);
 void shk_set_error (const char *fmt, ...);
 
+/* Dense FP64 solve with known forces on the mobile particles' force /
+ * torque components and known velocities elsewhere (solve.hip); mat is the
+ * n x n operator matrix (extracted form for FTS, transformed in place), b
+ * and x device vectors of n = np * dim; waits for the result. */
+int shk_dev_solve_mixed (shk_dev *d, double *mat, int np, int nm, int version,
+                         const double *b, double *x);
+
 /* ---- Brownian-dynamics step pieces (bd_kernels.hip) ---------------- */
 
 /* step scalars read by the kernels from device memory */

@@ -229,6 +229,52 @@ class System:
         return m
 
 
+def trans_mat_ext2ext(np_, mat):
+    """mat . T on every 11-column particle block, T(F, T, S) = extracted
+    form (ewald-3fts-matrix.c:383-437)"""
+    t = np.eye(11)
+    t[6, 6] = t[10, 10] = 2.0
+    t[6, 10] = t[10, 6] = 1.0
+    t[7, 7] = t[8, 8] = t[9, 9] = 2.0
+    return mat @ np.kron(np.eye(np_), t)
+
+
+def solve_mix_dense(system, nm, b):
+    """Dense mixed problem of ewald-3fts-matrix.c:570-652 / 1192-1290 (and
+    the same partition for F, FT): the true map A = M T split into known
+    force components (mobile F, T) and known velocity components (mobile E,
+    every component of the fixed particles), solve_linear of
+    matrix.c:266-312 with numpy's LAPACK inverse for lapack_inv_.
+    b: known values per particle (mobile F T E, fixed U O E); returns the
+    unknowns (mobile U O S, fixed F T S).  Numpy restatement: LAPACK is not
+    part of the reference tree."""
+    npart, v = system.np, system.version
+    d = dim(v)
+    mat = system.matrix()
+    if v == 2:
+        mat = trans_mat_ext2ext(npart, mat)
+    isH = np.zeros(npart * d, dtype=bool)
+    for i in range(npart):
+        for q in range(d):
+            isH[i * d + q] = i >= nm or q >= 6
+    h = np.flatnonzero(isH)
+    l = np.flatnonzero(~isH)
+    b = np.asarray(b, np.float64)
+    x = np.empty_like(b)
+    if h.size == 0:
+        return mat @ b
+    A = mat[np.ix_(h, h)]
+    B = mat[np.ix_(h, l)]
+    Cm = mat[np.ix_(l, h)]
+    D = mat[np.ix_(l, l)]
+    Ai = np.linalg.inv(A)
+    I_, J_ = Ai, -Ai @ B
+    K_, L_ = Cm @ Ai, D - Cm @ Ai @ B
+    x[l] = L_ @ b[l] + K_ @ b[h]
+    x[h] = J_ @ b[l] + I_ @ b[h]
+    return x
+
+
 def extmat_fts(np_, m, x):
     y = np.zeros(11 * np_)
     m = np.ascontiguousarray(m, np.float64)

@@ -188,3 +188,23 @@ def test_bd_overlap_redraw_below_dt_lim():
     _, dtn, usedn, _, _ = s.bd_mid_F(pos, draws[:1], 1.0, dt, nch, a, 0.3, 1.5,
                                      F=F, dt_lim=dt_lim)
     assert dtn == -1.0 and usedn == 1
+
+
+@pytest.mark.parametrize("version,nm", [(2, 8), (2, 5), (1, 6), (0, 5)])
+def test_dense_mixed_solve_residual(version, nm):
+    """the restated solve (ewald-3fts-matrix.c:570-652, 1192-1290) returns
+    generalised forces g and velocities v with v = M g on every component:
+    the known ones reproduced, the unknown ones consistent"""
+    L = 7.0
+    rng = np.random.default_rng(version * 10 + nm)
+    pos = np.array([[1, 1, 1], [4, 1.2, 0.8], [1.1, 4.2, 1], [4, 4, 1.3],
+                    [1, 1.1, 4], [4.2, 1, 4], [1, 4, 4.1], [4, 4.1, 4.2]],
+                   dtype=np.float64) + 0.2
+    s = po.System(version, pos, box=(L, L, L), tr=1.0)
+    d = po.dim(version)
+    b = rng.standard_normal(8 * d)
+    x = po.solve_mix_dense(s, nm, b)
+    isH = np.array([i >= nm or q >= 6 for i in range(8) for q in range(d)])
+    g = np.where(isH, x, b)          # forces / torques / stresslets
+    v = np.where(isH, b, x)          # velocities / angular velocities / E
+    assert np.abs(s.atimes(g) - v).max() < 1e-12 * np.abs(v).max()
