@@ -129,8 +129,15 @@ def bench_bd(args, log):
         xd = ls.DeviceArray.from_host(pos.reshape(-1))
         bd = ls.BD(s, peclet=1.0, n_cheb=30, eps=1e-6, dt_lim=1e-6,
                    rmin_ol=1.0, seed=11)
-        for _ in range(2):
-            bd.step(xd, 1e-3)
+        try:
+            for _ in range(2):
+                bd.step(xd, 1e-3)
+        except ls.StokesError as e:
+            out[name] = {"error": str(e)}
+            log(f"BD {name}: {e}")
+            bd.close()
+            s.close()
+            continue
         s.sync()
         att = halv = redr = 0
         t0 = time.perf_counter()

@@ -20,8 +20,13 @@
 #include "stokes_internal.h"
 
 #define BD_MAX_ATTEMPTS 64
-#define BD_LANCZOS_ITER 120
-#define BD_LANCZOS_MARGIN 0.02
+#define BD_LANCZOS_ITER 160
+/* Lanczos Ritz values approach the extreme eigenvalues from inside, the
+ * lower one slowly; a Chebyshev fit of 1/sqrt(x) grows fast below its
+ * interval, so the lower bound is taken well below the Ritz value, more so
+ * at each refit (level = refits so far, at most 3) */
+#define BD_LOW_MARGIN(level) (0.1 * ((level) + 1))
+#define BD_HIGH_MARGIN(level) (0.02 * ((level) + 1))
 
 struct stokes_hip_bd {
   struct stokes *sys;
@@ -32,6 +37,7 @@ struct stokes_hip_bd {
   unsigned long long seed;
   int fact_fix;
   int have_bounds;
+  int fits;                    /* Lanczos fits so far */
   double eig[2];
   double *acoef;               /* host [ncheb] */
   size_t n;                    /* 3 np */
@@ -307,11 +313,14 @@ static int
 bd_fit_bounds (stokes_hip_bd *bd, stokes_ctx *c, double *x)
 {
   double l0, l1;
+  int level = bd->fits < 3 ? bd->fits : 3;
   c->pos_dev = x;
-  int rc = stokes_hip_lanczos_bounds (bd->sys, BD_LANCZOS_ITER, 1 + bd->seed,
-                                      BD_LANCZOS_MARGIN, &l0, &l1);
+  int rc = stokes_hip_lanczos_bounds (bd->sys, BD_LANCZOS_ITER * (level + 1),
+                                      1 + bd->seed + bd->fits, 0.0, &l0, &l1);
   if (rc) return rc;
-  return stokes_hip_bd_set_bounds (bd, l0, l1);
+  bd->fits++;
+  return stokes_hip_bd_set_bounds (bd, l0 * (1.0 - BD_LOW_MARGIN (level)),
+                                   l1 * (1.0 + BD_HIGH_MARGIN (level)));
 }
 
 int

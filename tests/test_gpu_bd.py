@@ -228,3 +228,27 @@ def test_bd_mixed_precision_step():
     assert info.attempts == 1
     assert e < 1e-5
     bd.close()
+
+
+def test_bd_bounds_enclose_spectrum_n512():
+    """N = 512 (n = 1536): Ritz values lie inside the spectrum; the
+    integrator's bounds (Ritz values widened) must enclose it, and steps
+    then run with an accurate Chebyshev noise"""
+    n, phi = 512, 0.2
+    pos, L = workloads.configuration(n, phi, seed=7)
+    s = ls.Stokes(0, pos, box=(L, L, L), tr=1.0)
+    M = s.matrix()
+    w = np.linalg.eigvalsh(0.5 * (M + M.T))
+    t0, t1 = s.lanczos_bounds(niter=160, seed=1, margin=0.0)
+    _say(f"  spectrum [{w[0]:.5f}, {w[-1]:.5f}]  Ritz [{t0:.5f}, {t1:.5f}]")
+    assert w[0] * (1 - 1e-9) <= t0 and t1 <= w[-1] * (1 + 1e-9)
+    bd = ls.BD(s, peclet=1.0, n_cheb=30, eps=1e-6, seed=2)
+    xd = ls.DeviceArray.from_host(pos.reshape(-1))
+    for step in range(3):
+        info = bd.step(xd, 1e-3)
+        _say(f"  step {step}: bounds [{info.eig[0]:.5f}, {info.eig[1]:.5f}] "
+             f"err {info.err_cheb:.2e} attempts {info.attempts} "
+             f"halvings {info.halvings} refits {info.refits}")
+        assert info.err_cheb < 1e-5
+    assert info.eig[0] <= w[0] and info.eig[1] >= w[-1] * (1 - 1e-9)
+    bd.close()
