@@ -127,11 +127,14 @@ def bench_bd(args, log):
     for name, prec in (("mixed", ls.MIXED), ("fp64", ls.FP64)):
         s = ls.Stokes(0, pos, box=(L, L, L), tr=1.0, precision=prec)
         xd = ls.DeviceArray.from_host(pos.reshape(-1))
-        bd = ls.BD(s, peclet=1.0, n_cheb=30, eps=1e-6, dt_lim=1e-6,
+        # order 30 on this spectrum (condition ~10^2) is accurate to ~1e-2
+        # in z.Mz: eps is set so the fit is not redone every step
+        bd = ls.BD(s, peclet=1.0, n_cheb=30, eps=5e-2, dt_lim=1e-6,
                    rmin_ol=1.0, seed=11)
         try:
             for _ in range(2):
-                bd.step(xd, 1e-3)
+                info = bd.step(xd, 1e-3)
+                log(f"BD {name} warm-up: {info.summary()}")
         except ls.StokesError as e:
             out[name] = {"error": str(e)}
             log(f"BD {name}: {e}")

@@ -98,6 +98,22 @@ int stokes_hip_make_matrix (struct stokes *sys, double *mat);
 int stokes_hip_extmat_dev (struct stokes *sys, const double *mat_dev,
                            int nrhs, const double *x_dev, double *y_dev);
 
+/* Gradients of the scalar g . M(x) f for the periodic F version with equal
+ * spheres (device vectors of 3 np): dx_dev = d/dx (adjoint kernels that
+ * recompute the pair scalars and their radial derivatives; the real-space
+ * sum of ewald.c:649-789 and the reciprocal sum of ewald.c:923-1075
+ * differentiated), df_dev = d/df = M g.  Either output may be NULL.  Runs
+ * in the precision of sys (stokes_hip_set_precision).  No counterpart in
+ * the reference. */
+int stokes_hip_adjoint_dev (struct stokes *sys, const double *g_dev,
+                            const double *f_dev, double *dx_dev,
+                            double *df_dev);
+
+/* xa, ya of scalars_ewald_real for the F version (ewald.c:80-99) and
+ * their derivatives in r: out = (xa, ya, dxa/dr, dya/dr), as the adjoint
+ * evaluates them */
+void stokes_hip_scalars_real_f_deriv (double xi, double r, double *out);
+
 /* Dense FP64 mixed problem on the device: particles [0, nm) are mobile
  * with known force / torque (and, FTS, known strain E), particles
  * [nm, np) are fixed with known velocity, angular velocity (and E).
@@ -150,6 +166,8 @@ typedef struct stokes_hip_bd_info {
   int halvings;       /* dt halvings after overlaps */
   int redraws;        /* noise redraws */
   int refits;         /* spectral-bound refits */
+  unsigned long long overlaps[2];  /* overlap counts of the last attempt at
+                                      x_mid and x_new */
 } stokes_hip_bd_info;
 
 /* sys: version 0 (F), equal radii (a == NULL), no shear, set up as for

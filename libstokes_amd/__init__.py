@@ -54,7 +54,7 @@ EXPORTS = [
     "stokes_hip_atimes_dev", "stokes_hip_atimes", "stokes_hip_make_matrix_dev",
     "stokes_hip_make_matrix", "stokes_hip_extmat_dev",
     "stokes_hip_cheb_eval_dev", "stokes_hip_lanczos_bounds",
-    "stokes_hip_solve_mix_dev", "solve_mob_3fts_matrix_0",
+    "stokes_hip_adjoint_dev", "stokes_hip_scalars_real_f_deriv", "stokes_hip_solve_mix_dev", "solve_mob_3fts_matrix_0",
     "solve_mix_3fts_matrix_0",
     "stokes_hip_bd_create", "stokes_hip_bd_set_bounds",
     "stokes_hip_bd_set_fact_fix", "stokes_hip_bd_step",
@@ -103,7 +103,14 @@ class BDInfo(C.Structure):
     _fields_ = [("dt", C.c_double), ("err_cheb", C.c_double),
                 ("eig", C.c_double * 2), ("attempts", C.c_int),
                 ("halvings", C.c_int), ("redraws", C.c_int),
-                ("refits", C.c_int)]
+                ("refits", C.c_int), ("overlaps", C.c_ulonglong * 2)]
+
+    def summary(self):
+        return (f"dt {self.dt:g} err {self.err_cheb:.2e} bounds "
+                f"[{self.eig[0]:.4g}, {self.eig[1]:.4g}] attempts "
+                f"{self.attempts} halvings {self.halvings} redraws "
+                f"{self.redraws} refits {self.refits} last overlaps "
+                f"{self.overlaps[0]}/{self.overlaps[1]}")
 
 
 class StokesError(RuntimeError):
@@ -141,6 +148,10 @@ def lib():
         getattr(L, f).argtypes = [sp, C.c_int]
     L.stokes_hip_set_split.argtypes = [sp, C.c_double]
     L.stokes_hip_solve_mix_dev.argtypes = [sp, _vp, _vp]
+    L.stokes_hip_adjoint_dev.argtypes = [sp, _vp, _vp, _vp, _vp]
+    L.stokes_hip_scalars_real_f_deriv.argtypes = [C.c_double, C.c_double,
+                                                  _dp]
+    L.stokes_hip_scalars_real_f_deriv.restype = None
     L.solve_mob_3fts_matrix_0.argtypes = [sp] + [_dp] * 6
     L.solve_mix_3fts_matrix_0.argtypes = [sp] + [_dp] * 12
     L.stokes_hip_autotune.argtypes = [sp, C.c_int, _dp, C.c_int, C.c_int,
@@ -309,6 +320,13 @@ class Stokes:
               "solve_mix")
         return xd.to_host()
 
+    def adjoint_dev(self, g_dev, f_dev, dx_dev=None, df_dev=None):
+        """d(g . M(x) f)/dx and /df into device arrays (F version)"""
+        check(self.L.stokes_hip_adjoint_dev(
+            self.sys, g_dev.ptr, f_dev.ptr if f_dev is not None else None,
+            dx_dev.ptr if dx_dev is not None else None,
+            df_dev.ptr if df_dev is not None else None), "adjoint")
+
     def set_timing(self, on=True):
         check(self.L.stokes_hip_set_timing(self.sys, 1 if on else 0),
               "set_timing")
@@ -409,8 +427,13 @@ class BD:
 
     def step(self, x_dev, dt):
         info = BDInfo()
-        check(self.L.stokes_hip_bd_step(self.h, x_dev.ptr, float(dt),
-                                        C.byref(info)), "bd_step")
+        rc = self.L.stokes_hip_bd_step(self.h, x_dev.ptr, float(dt),
+                                       C.byref(info))
+        if rc != 0:
+            msg = self.L.stokes_hip_last_error()
+            raise StokesError(f"bd_step failed ({rc}): "
+                              f"{msg.decode() if msg else ''} "
+                              f"[{info.summary()}]")
         return info
 
     def last_noise(self):

@@ -94,6 +94,42 @@ SHK_HD void scalars_real (T xi, T r, T *o)
   scalars_real_u<V, T> (xi, r, T (1) / r, o);
 }
 
+// F-version real-space scalars xa, ya (ewald.c:80-99) and their radial
+// derivatives, for the adjoint; `u` is 1/r.  With z = (xi r)^2,
+// E = xi/sqrt(pi) exp(-z), d erfc(xi r)/dr = -2E, dE/dr = -2 xi^2 r E:
+//   ya = (3/4 u + 1/2 u^3) erfc + (Pa(z) u^2 - 4.5 + 3z) E,
+//   a2 = (3/4 u - 3/2 u^3) erfc + (Pb(z) u^2 + 1.5 - 3z) E,  xa = a2 + ya,
+// Pa = 1 + 14z - 20z^2 + 4z^3, Pb = -3 - 2z + 16z^2 - 4z^3.
+template <typename T>
+SHK_HD void scalars_real_f_deriv (T xi, T r, T u, T *xa, T *ya, T *dxa, T *dya)
+{
+  const T xr = xi * r, z = xr * xr;
+  const T u2 = u * u, u3 = u2 * u, u4 = u2 * u2;
+  const T ec = m_erfc (xr);
+  const T E = xi * inv_sqrt_pi<T> () * m_exp (-z);
+  const T dz = T (2) * xi * xi * r;                 // dz/dr
+  const T pa = T (1) + z * (T (14) + z * (T (-20) + z * T (4)));
+  const T dpa = T (14) + z * (T (-40) + z * T (12));
+  const T pb = T (-3) + z * (T (-2) + z * (T (16) + z * T (-4)));
+  const T dpb = T (-2) + z * (T (32) + z * T (-12));
+  const T ca = T (0.75) * u + T (0.5) * u3;          // erfc coefficients
+  const T cb = T (0.75) * u - T (1.5) * u3;
+  const T qa = pa * u2 - T (4.5) + T (3) * z;        // E coefficients
+  const T qb = pb * u2 + T (1.5) - T (3) * z;
+  const T ya_ = ca * ec + qa * E;
+  const T a2 = cb * ec + qb * E;
+  const T dca = T (-0.75) * u2 - T (1.5) * u4;
+  const T dcb = T (-0.75) * u2 + T (4.5) * u4;
+  const T dqa = dpa * dz * u2 - T (2) * pa * u3 + T (3) * dz;
+  const T dqb = dpb * dz * u2 - T (2) * pb * u3 - T (3) * dz;
+  const T dya_ = dca * ec - T (2) * ca * E + (dqa - qa * dz) * E;
+  const T da2 = dcb * ec - T (2) * cb * E + (dqb - qb * dz) * E;
+  *ya = ya_;
+  *xa = a2 + ya_;
+  *dya = dya_;
+  *dxa = da2 + dya_;
+}
+
 // ---------------------------------------------------------------------------
 // real-space Ewald scalars, unequal spheres: ewald.c:203-359, followed by
 // the SD scaling of ewald.c:382-420 when SD is true

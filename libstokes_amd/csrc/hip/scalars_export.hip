@@ -68,3 +68,11 @@ scalars_nonewald (int version, double r, double *scalar)
     shk::scalars_rpy_sd<decltype (v)::value, double> (r, 1.0, 1.0, false, scalar);
   });
 }
+
+// F-version scalars with their radial derivatives (the adjoint's inputs):
+// out = (xa, ya, dxa/dr, dya/dr)
+extern "C" void
+stokes_hip_scalars_real_f_deriv (double xi, double r, double *out)
+{
+  shk::scalars_real_f_deriv<double> (xi, r, 1.0 / r, out, out + 1, out + 2, out + 3);
+}

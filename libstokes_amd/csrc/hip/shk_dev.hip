@@ -12,6 +12,7 @@
 #include "real_kernels.h"
 #include "recip_kernels.h"
 #include "block_kernels.h"
+#include "adj_kernels.h"
 
 using namespace shk;
 
@@ -57,6 +58,7 @@ struct shk_dev {
   int have_sys = 0;
   DevBuf pos, img, imgf, kv, ks, partial, pq, xs, ys, zt;
   DevBuf cand, candf, candm, tbits, kpart, kpq, kback;
+  DevBuf adjfg, adjcs, adjpart;
   int ncand = 0, ncore = 0, tbits_words = 0;
   int tx = 0, ty = 0, tz = 0;
   double s2x = 0.0, s3x = 0.0;
@@ -119,7 +121,8 @@ extern "C" void shk_dev_destroy (shk_dev *d)
   if (d->stream) (void) hipStreamSynchronize (d->stream);
   for (DevBuf *b : {&d->pos, &d->img, &d->imgf, &d->kv, &d->ks, &d->partial,
                     &d->pq, &d->xs, &d->ys, &d->zt, &d->cand, &d->candf, &d->candm,
-                    &d->tbits, &d->kpart, &d->kpq, &d->kback})
+                    &d->tbits, &d->kpart, &d->kpq, &d->kback, &d->adjfg, &d->adjcs,
+                    &d->adjpart})
     release (*b);
   for (auto &e : d->ev)
     if (e) (void) hipEventDestroy (e);
@@ -787,4 +790,75 @@ extern "C" int shk_dev_copy_calibrate (shk_dev *d, void *dst, const void *src, s
   HIPCHK (hipGetLastError ());
   HIPCHK (hipStreamSynchronize (d->stream));
   return SHK_OK;
+}
+
+// ---- adjoint of the F-version product with respect to the positions ----
+
+template <typename T>
+static int adjoint_t (shk_dev *d, const double *f, const double *g, double *dx)
+{
+  const shk_params &p = d->prm;
+  const int np = p.np, nk = p.nk;
+  int rc;
+  if ((rc = ensure (d, d->adjfg, sizeof (double) * 6 * (size_t) np))) return rc;
+  hipLaunchKernelGGL (k_adj_pack, dim3 ((np + 255) / 256), dim3 (256), 0, d->stream, f, g, np,
+                      (double *) d->adjfg.p);
+  HIPCHK (hipGetLastError ());
+  const double *fg = (const double *) d->adjfg.p;
+  const double4 *pos4 = (const double4 *) d->pos.p;
+  // real space
+  const int nj = (np + 63) / 64;
+  if ((rc = ensure (d, d->partial, sizeof (double) * (size_t) nj * np * 3))) return rc;
+  RealParams rp;
+  rp.np = np; rp.ncand = d->ncand; rp.ncore = d->ncore; rp.poly = 0;
+  rp.xi = p.xi; rp.rmax2 = p.rmax2; rp.rmin = p.rmin;
+  rp.lx = p.lx; rp.ly = p.ly; rp.lz = p.lz; rp.s2x = d->s2x; rp.s3x = d->s3x;
+  rp.tx = d->tx; rp.ty = d->ty; rp.tz = d->tz; rp.tbits_words = d->tbits_words;
+  const long waves = (long) np * nj;
+  const void *cand = std::is_same<T, double>::value ? d->cand.p : d->candf.p;
+  hipLaunchKernelGGL ((k_real_adj<T>), dim3 ((unsigned) ((waves + 3) / 4)), dim3 (256), 0,
+                      d->stream, pos4, fg, (const typename Vec4<T>::type *) cand,
+                      (const int4 *) d->candm.p, (const unsigned *) d->tbits.p, rp,
+                      (double *) d->partial.p);
+  HIPCHK (hipGetLastError ());
+  // reciprocal space
+  int nrecip = 0;
+  if (nk > 0)
+    {
+      int sj, jper, sk, kper;
+      recip_shape (np, nk, sj, jper, sk, kper);
+      if ((rc = ensure (d, d->kpart, sizeof (T) * (size_t) sj * nk * 12))) return rc;
+      if ((rc = ensure (d, d->adjcs, sizeof (T) * (size_t) nk * 12))) return rc;
+      if ((rc = ensure (d, d->adjpart, sizeof (T) * (size_t) sk * np * 3))) return rc;
+      const double4 *kv = (const double4 *) d->kv.p;
+      hipLaunchKernelGGL ((k_recip_sf<1, T, 1>), dim3 ((nk + 255) / 256, sj), dim3 (256), 0,
+                          d->stream, pos4, fg, kv, np, nk, jper, (T *) d->kpart.p);
+      HIPCHK (hipGetLastError ());
+      hipLaunchKernelGGL ((k_adj_sf_sum<T>), dim3 ((nk + 127) / 128), dim3 (128), 0, d->stream,
+                          (const T *) d->kpart.p, nk, sj, (T *) d->adjcs.p);
+      HIPCHK (hipGetLastError ());
+      hipLaunchKernelGGL ((k_adj_back<T>), dim3 ((np + 255) / 256, sk), dim3 (256), 0, d->stream,
+                          pos4, fg, kv, (const double *) d->ks.p, (const T *) d->adjcs.p, np, nk,
+                          kper, (T *) d->adjpart.p);
+      HIPCHK (hipGetLastError ());
+      nrecip = sk;
+    }
+  hipLaunchKernelGGL ((k_adj_combine<T>), dim3 ((np * 3 + 255) / 256), dim3 (256), 0, d->stream,
+                      (const double *) d->partial.p, nj, (const T *) d->adjpart.p, nrecip, np,
+                      dx);
+  HIPCHK (hipGetLastError ());
+  return SHK_OK;
+}
+
+extern "C" int shk_dev_adjoint_x (shk_dev *d, const double *f, const double *g, double *dx,
+                                  int prec)
+{
+  if (!d || !d->have_sys) { set_err ("shk_dev_adjoint_x: no system"); return SHK_EARG; }
+  const shk_params &p = d->prm;
+  if (p.version != 0 || !p.periodic || p.poly)
+    {
+      set_err ("adjoint: implemented for the periodic F version with equal spheres");
+      return SHK_EUNSUP;
+    }
+  return prec == SHK_MIXED ? adjoint_t<float> (d, f, g, dx) : adjoint_t<double> (d, f, g, dx);
 }

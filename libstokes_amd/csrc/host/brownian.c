@@ -390,6 +390,8 @@ stokes_hip_bd_step (stokes_hip_bd *bd, double *x_dev, double dt,
       if (rc) { rc = bd_fail (rc, "bd_step (graph)"); goto out; }
       inf.attempts++;
       const shk_bd_status st = *bd->st_host;
+      inf.overlaps[0] = st.overlap[0];
+      inf.overlaps[1] = st.overlap[1];
       if (!(st.zmz > 0.0))
         {
           stokes_hip_set_error ("bd_step: z.Mz = %g is not positive", st.zmz);

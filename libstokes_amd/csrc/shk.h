@@ -126,6 +126,11 @@ int shk_dev_d2h_async (shk_dev *d, void *dst, const void *src, size_t bytes);
 int shk_dev_h2d_async (shk_dev *d, void *dst, const void *src, size_t bytes);
 void shk_set_error (const char *fmt, ...);
 
+/* dx = d(g . M(x) f)/dx (3 np) for the periodic F version with equal
+ * spheres; f, g, dx device vectors */
+int shk_dev_adjoint_x (shk_dev *d, const double *f, const double *g,
+                       double *dx, int prec);
+
 /* Dense FP64 solve with known forces on the mobile particles' force /
  * torque components and known velocities elsewhere (solve.hip); mat is the
  * n x n operator matrix (extracted form for FTS, transformed in place), b

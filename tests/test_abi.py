@@ -135,6 +135,25 @@ def test_scalars_exported_match_oracle():
     assert np.allclose(sc, want, rtol=1e-15, atol=0)
 
 
+def test_scalar_derivatives_of_the_adjoint():
+    """the radial derivatives of xa, ya used by the adjoint kernels equal
+    central differences (h = 1e-5) of the oracle's scalars_ewald_real,
+    itself bitwise equal to the compiled reference's"""
+    lib = ls.lib()
+    out = np.zeros(4)
+    h = 1e-5
+    for xi in (0.1, 0.4, 1.0):
+        for r in (0.7, 2.0, 3.3, 6.0):
+            lib.stokes_hip_scalars_real_f_deriv(
+                C.c_double(xi), C.c_double(r),
+                out.ctypes.data_as(C.POINTER(C.c_double)))
+            s0 = po.scalars_ewald_real(0, xi, r)[:2]
+            fd = (po.scalars_ewald_real(0, xi, r + h)[:2]
+                  - po.scalars_ewald_real(0, xi, r - h)[:2]) / (2 * h)
+            assert np.allclose(out[:2], s0, rtol=1e-13, atol=1e-16)
+            assert np.allclose(out[2:], fd, rtol=1e-7, atol=1e-10)
+
+
 def test_chebyshev_host_functions_match_oracle():
     lib = ls.lib()
     FUNC = C.CFUNCTYPE(C.c_double, C.c_double)
