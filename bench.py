@@ -123,9 +123,13 @@ def bench_bd(args, log):
     n, phi = 2048, 0.2
     pos, L = workloads.configuration(n, phi, seed=2)
     out = {"workload": f"BD mid-point F N={n} phi={phi} n_cheb=30 dt=1e-3 "
-                       f"Pe=1 (31 operator products per attempt)"}
+                       f"Pe=1 rmin=1 (31 operator products per step)"}
     for name, prec in (("mixed", ls.MIXED), ("fp64", ls.FP64)):
-        s = ls.Stokes(0, pos, box=(L, L, L), tr=1.0, precision=prec)
+        # r clamped at contact (sys->rmin = 1): without an excluded-volume
+        # force this is how the reference runs dense BD; it also skips the
+        # overlap-driven dt control (brownian.c:2436, 2502)
+        s = ls.Stokes(0, pos, box=(L, L, L), tr=1.0, precision=prec,
+                      rmin=1.0)
         xd = ls.DeviceArray.from_host(pos.reshape(-1))
         # order 30 on this spectrum (condition ~10^2) is accurate to ~1e-2
         # in z.Mz: eps is set so the fit is not redone every step

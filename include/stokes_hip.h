@@ -51,8 +51,8 @@ int stokes_hip_set_recip_g_fix (struct stokes *sys, int enable);
  * set-up rule and sys->ewald_eps) instead of sys->xi.  The Ewald sum does
  * not depend on the splitting, to the Ewald tolerance, except for the
  * reference's reciprocal G sign of the FTS version; that term is restored
- * at sys->xi for equal spheres (FTS with unequal radii and the reference
- * sign keeps sys->xi).  Results therefore match the reference at sys->xi
+ * at sys->xi (equal and unequal spheres).  Results therefore match the
+ * reference at sys->xi
  * within the Ewald tolerance while the real/reciprocal balance is chosen
  * for speed.  tratio = 0 computes at sys->xi (the default). */
 int stokes_hip_set_split (struct stokes *sys, double tratio);
@@ -138,6 +138,25 @@ int stokes_hip_cheb_eval_dev (struct stokes *sys, int ncheb, const double *a,
 int stokes_hip_lanczos_bounds (struct stokes *sys, int niter,
                                unsigned long seed, double margin,
                                double *lmin, double *lmax);
+
+/* ---- ensembles of independent systems -----------------------------------
+ * The operator of nbox systems (uneven np, radii, boxes, precisions) as one
+ * graph launch: every box runs on one of a pool of streams forked from and
+ * joined back to the launch stream.  Box b reads positions at
+ * pos_dev + 3 off_b and x, y at x_dev / y_dev + dim_b off_b, off_b the sum
+ * of np over the boxes before b (stokes_hip_ensemble_offset).  Positions
+ * are re-read on every launch; the other parameters of the systems are
+ * those at the first launch until stokes_hip_ensemble_refresh().  Calls on
+ * the member systems must not overlap an ensemble launch.  No counterpart
+ * in the reference. */
+typedef struct stokes_hip_ensemble stokes_hip_ensemble;
+int stokes_hip_ensemble_create (struct stokes *const *sys, int nbox,
+                                stokes_hip_ensemble **out);
+int stokes_hip_ensemble_atimes (stokes_hip_ensemble *e, const double *pos_dev,
+                                const double *x_dev, double *y_dev);
+int stokes_hip_ensemble_refresh (stokes_hip_ensemble *e);
+size_t stokes_hip_ensemble_offset (const stokes_hip_ensemble *e, int box);
+void stokes_hip_ensemble_destroy (stokes_hip_ensemble *e);
 
 /* ---- Brownian dynamics ------------------------------------------------
  * The mid-point step of BD_evolve_mid (brownian.c:2352-2560) for the F

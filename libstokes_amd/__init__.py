@@ -56,6 +56,9 @@ EXPORTS = [
     "stokes_hip_cheb_eval_dev", "stokes_hip_lanczos_bounds",
     "stokes_hip_adjoint_dev", "stokes_hip_scalars_real_f_deriv", "stokes_hip_solve_mix_dev", "solve_mob_3fts_matrix_0",
     "solve_mix_3fts_matrix_0",
+    "stokes_hip_ensemble_create", "stokes_hip_ensemble_atimes",
+    "stokes_hip_ensemble_refresh", "stokes_hip_ensemble_offset",
+    "stokes_hip_ensemble_destroy",
     "stokes_hip_bd_create", "stokes_hip_bd_set_bounds",
     "stokes_hip_bd_set_fact_fix", "stokes_hip_bd_step",
     "stokes_hip_bd_last_noise", "stokes_hip_bd_destroy",
@@ -177,6 +180,14 @@ def lib():
     L.chebyshev_eval.argtypes = [C.c_int, _dp, C.c_double, C.c_double,
                                  C.c_double]
     L.scalars_nonewald.argtypes = [C.c_int, C.c_double, _dp]
+    L.stokes_hip_ensemble_create.argtypes = [C.POINTER(sp), C.c_int,
+                                             C.POINTER(_vp)]
+    L.stokes_hip_ensemble_atimes.argtypes = [_vp, _vp, _vp, _vp]
+    L.stokes_hip_ensemble_refresh.argtypes = [_vp]
+    L.stokes_hip_ensemble_offset.argtypes = [_vp, C.c_int]
+    L.stokes_hip_ensemble_offset.restype = C.c_size_t
+    L.stokes_hip_ensemble_destroy.argtypes = [_vp]
+    L.stokes_hip_ensemble_destroy.restype = None
     L.stokes_hip_bd_create.argtypes = [sp, _vp, C.c_double, C.c_int,
                                        C.c_double, C.c_double, C.c_double,
                                        C.c_ulonglong, C.POINTER(_vp)]
@@ -393,6 +404,41 @@ class Stokes:
         if self.sys:
             self.L.stokes_free(self.sys)
             self.sys = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Ensemble:
+    """The operator of several independent systems as one graph launch
+    (stokes_hip_ensemble_* of include/stokes_hip.h)."""
+
+    def __init__(self, systems):
+        self.L = lib()
+        self.systems = list(systems)
+        arr = (C.POINTER(StokesStruct) * len(self.systems))(
+            *[s.sys for s in self.systems])
+        h = _vp()
+        check(self.L.stokes_hip_ensemble_create(arr, len(self.systems),
+                                                C.byref(h)), "ensemble_create")
+        self.h = h
+        self.offsets = [self.L.stokes_hip_ensemble_offset(h, b)
+                        for b in range(len(self.systems) + 1)]
+
+    def atimes_dev(self, pos_dev, x_dev, y_dev):
+        check(self.L.stokes_hip_ensemble_atimes(self.h, pos_dev.ptr, x_dev.ptr,
+                                                y_dev.ptr), "ensemble_atimes")
+
+    def refresh(self):
+        check(self.L.stokes_hip_ensemble_refresh(self.h), "ensemble_refresh")
+
+    def close(self):
+        if self.h:
+            self.L.stokes_hip_ensemble_destroy(self.h)
+            self.h = None
 
     def __del__(self):
         try:

@@ -202,13 +202,15 @@ __device__ __forceinline__ void kblock_moments (const double4 kvec, const double
 // the radii: every scalar of pair (i, j) is a sum of c(k) a_i^p a_j^q with
 // q in {0, 2}.  For power p of a_i this returns the paired-k (x2) scalars
 // acting on the q = 0 sums (sc1 even / ss1 odd) and on the q = 2 sums
-// (sc2 / ss2), in the order xa ya yb xc yc xg yg yh xm ym zm.
+// (sc2 / ss2), in the order xa ya yb xc yc xg yg yh xm ym zm.  The G
+// terms take their own (signed) kexp, kexpg: sign * kexp, plus the
+// reference-sign correction of another splitting where one is applied.
 __device__ __forceinline__ void poly_kscalars (int p, double k, double kk, double kexp,
-                                               double gsign, double *sc1, double *ss1,
+                                               double kexpg, double *sc1, double *ss1,
                                                double *sc2, double *ss2)
 {
   for (int t = 0; t < 11; t++) sc1[t] = ss1[t] = sc2[t] = ss2[t] = 0.0;
-  const double e = 2.0 * kexp, eg = gsign * e;
+  const double e = 2.0 * kexp, eg = 2.0 * kexpg;
   switch (p)
     {
     case 1:                                   // ya: 6 a_i - kk a_i a_j^2
@@ -313,7 +315,7 @@ k_matrix (const double4 *__restrict__ pos, const double4 *__restrict__ img,
               ya = aa * 6.0 * (1.0 - kk2 * (aa2 + ab2) / 6.0) * kexp2;
               yb = aa2 * 3.0 * kn * kexp2;
               yc = aa3 * 1.5 * kk2 * kexp2;
-              yg = p.gsign * aa2 * 3.0 * (1.0 - kk2 * (aa2 / 10.0 + ab2 / 6.0)) * kn * kexp2;
+              yg = aa2 * 3.0 * (1.0 - kk2 * (aa2 / 10.0 + ab2 / 6.0)) * kn * 2.0 * w[3];
               yh = aa3 * 1.5 * kk2 * kexp2;
               ym = aa3 * 3.0 * (1.0 - kk2 * (aa2 + ab2) / 10.0) * kk2 * kexp2;
             }

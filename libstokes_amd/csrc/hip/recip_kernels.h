@@ -121,7 +121,7 @@ k_recip_blocks (const double4 *__restrict__ kv, const double *__restrict__ ks, i
   for (int pw = 1; pw <= NPOW; pw++)
     {
       double sc1[11], ss1[11], sc2[11], ss2[11];
-      poly_kscalars (pw, kk.w, w[6], w[7], gsign, sc1, ss1, sc2, ss2);
+      poly_kscalars (pw, kk.w, w[6], w[7], w[3], sc1, ss1, sc2, ss2);
       double P[D], Q[D];
 #pragma unroll
       for (int q = 0; q < D; q++) P[q] = Q[q] = 0.0;

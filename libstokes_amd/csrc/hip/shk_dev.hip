@@ -134,6 +134,13 @@ extern "C" void *shk_dev_stream (shk_dev *d) { return d ? (void *) d->stream : n
 
 extern "C" unsigned long shk_dev_generation (shk_dev *d) { return d ? d->generation : 0; }
 
+extern "C" void *shk_dev_swap_stream (shk_dev *d, void *stream)
+{
+  void *old = (void *) d->stream;
+  d->stream = (hipStream_t) stream;
+  return old;
+}
+
 extern "C" int shk_dev_malloc (shk_dev *d, void **p, size_t bytes)
 {
   (void) d;

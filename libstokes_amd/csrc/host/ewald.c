@@ -210,7 +210,9 @@ build_kvectors (const struct stokes *s, double gsign, double **kv_out,
           w[0] = 2.0 * 6.0 * (1.0 - kk / 3.0) * kexp;              /* ya */
           w[1] = 2.0 * 3.0 * k * kexp;                             /* yb */
           w[2] = 2.0 * 3.0 / 2.0 * kk * kexp;                      /* yc */
-          w[3] = gsign * 2.0 * 3.0 * (1.0 - 4.0 / 15.0 * kk) * k * kexp; /* yg */
+          /* yg; unequal spheres: the signed kexp of the G terms */
+          w[3] = s->a ? gsign * kexp
+            : gsign * 2.0 * 3.0 * (1.0 - 4.0 / 15.0 * kk) * k * kexp;
           w[4] = 2.0 * 3.0 / 2.0 * kk * kexp;                      /* yh */
           w[5] = 2.0 * 3.0 * (1.0 - kk / 5.0) * kk * kexp;         /* ym */
           w[6] = kk;
@@ -231,9 +233,11 @@ build_kvectors (const struct stokes *s, double gsign, double **kv_out,
  * y(xi; s_G) = y(xi; -1) + (1 + s_G) K_G(xi), K_G the reciprocal G part
  * with weight +yg.  y(xi; -1) does not depend on xi (to the Ewald
  * tolerance), so the reference's y(xi_1; +1) = y(xi_t; -1) + 2 K_G(xi_1):
- * 2 yg(xi_1) goes onto the G weight of each k of the xi_1 sphere -- the
- * same lattice vector in the xi_t tables, or a new G-only entry.  Returns
- * the new count (the tables are reallocated), or -1. */
+ * twice the G weight at xi_1 (yg for equal spheres, kexp for unequal ones,
+ * whose G scalars are formed in the kernels) goes onto the G weight of each
+ * k of the xi_1 sphere -- the same lattice vector in the xi_t tables, or a
+ * new G-only entry.  Returns the new count (the tables are reallocated), or
+ * -1. */
 static int
 add_g_correction (const struct stokes *s, double **kv_io, double **ks_io,
                   int *km_t, int nk_t)
@@ -277,8 +281,7 @@ add_g_correction (const struct stokes *s, double **kv_io, double **ks_io,
           t = nk++;
           memcpy (kv + 4 * t, kv1 + 4 * k, sizeof (double) * 4);
           for (int q = 0; q < 8; q++) ks[8 * t + q] = 0.0;
-          ks[8 * t + 6] = ks1[8 * k + 6];
-          ks[8 * t + 7] = ks1[8 * k + 7];
+          ks[8 * t + 6] = ks1[8 * k + 6];   /* kk; kexp 0: G terms only */
         }
       ks[8 * t + 3] += 2.0 * ks1[8 * k + 3];
     }
@@ -417,11 +420,9 @@ stokes_hip_prepare (stokes_ctx *c, int positions_on_device)
   pu.nr = pu.periodic ? s->nr : 0;
   pu.gsign = c->fix_recip_g ? -1.0 : 1.0;
   /* another splitting computes the same sum (to the Ewald tolerance) except
-   * for the reference-sign G term of FTS, which add_g_correction restores;
-   * that correction is written for equal spheres */
+   * for the reference-sign G term of FTS, which add_g_correction restores */
   const int gcorr = s->version == 2 && !c->fix_recip_g;
-  const int split = c->split_tr > 0.0 && pu.periodic && s->shear_mode == 0
-    && !(gcorr && pu.poly);
+  const int split = c->split_tr > 0.0 && pu.periodic && s->shear_mode == 0;
   int full = !c->valid || c->shear_mode != s->shear_mode
     || c->shear_shift != s->shear_shift || pu.poly
     || c->a_ptr != s->a || (split ? c->split_tr : 0.0) != c->split_used_tr;

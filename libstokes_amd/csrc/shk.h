@@ -180,6 +180,20 @@ int  shk_dev_capture_end (shk_dev *d, void **exec);
 void shk_dev_capture_abort (shk_dev *d);
 int  shk_dev_graph_launch (shk_dev *d, void *exec);
 void shk_dev_graph_destroy (void *exec);
+/* launches of d go to `stream` until swapped back; returns the previous */
+void *shk_dev_swap_stream (shk_dev *d, void *stream);
+/* plain streams, events and graphs (streams.hip) */
+int  shk_stream_create (void **s);
+void shk_stream_destroy (void *s);
+int  shk_stream_sync (void *s);
+int  shk_event_create (void **e);
+void shk_event_destroy (void *e);
+int  shk_event_record (void *e, void *s);
+int  shk_stream_wait (void *s, void *e);
+int  shk_stream_capture_begin (void *s);
+int  shk_stream_capture_end (void *s, void **exec);
+void shk_stream_capture_abort (void *s);
+int  shk_graph_launch_on (void *exec, void *s);
 /* pinned host memory */
 int  shk_host_alloc (void **p, size_t bytes);
 void shk_host_free (void *p);

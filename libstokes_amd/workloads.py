@@ -72,6 +72,37 @@ def configuration(n, phi, seed=0, gap=2.1):
     return pos, L
 
 
+def configuration_poly(n, phi, seed=0, a_lo=0.5, a_hi=2.0, gap=1.05,
+                       max_tries=2000000):
+    """(positions [n,3], radii [n], box length) for n spheres with radii
+    log-uniform on [a_lo, a_hi] at volume fraction phi; centres at least
+    gap (a_i + a_j) apart (minimum image), placed largest first by random
+    sequential addition; deterministic in seed"""
+    rng = np.random.default_rng(seed)
+    a = np.exp(rng.uniform(math.log(a_lo), math.log(a_hi), n))
+    L = (np.sum(4.0 * math.pi / 3.0 * a ** 3) / phi) ** (1.0 / 3.0)
+    order = np.argsort(-a)
+    pos = np.empty((n, 3))
+    placed = []
+    tries = 0
+    for idx in order:
+        while True:
+            tries += 1
+            if tries > max_tries:
+                raise RuntimeError("random sequential addition did not converge")
+            p = rng.uniform(0.0, L, 3)
+            if placed:
+                q = np.array(placed)
+                d = _min_image(pos[q] - p, L)
+                lim = gap * (a[q] + a[idx])
+                if np.any(np.einsum("ij,ij->i", d, d) < lim * lim):
+                    continue
+            pos[idx] = p
+            placed.append(idx)
+            break
+    return pos, a, L
+
+
 def min_distance(pos, L):
     best = np.inf
     for i in range(len(pos) - 1):

@@ -70,18 +70,16 @@ def test_adjoint_c5_mixed_autograd_vs_fp64_fd():
     rng = np.random.default_rng(7)
     f = rng.standard_normal(3 * n)
     ut = rng.standard_normal(3 * n)
-    dev = torch.device("cuda:0")
     smix = ls.Stokes(0, pos, box=(L, L, L), tr=1.0, precision=ls.MIXED)
     smix.set_split(300.0)
-    x = torch.tensor(pos.reshape(-1), dtype=torch.float64, device=dev,
-                     requires_grad=True)
-    ft = torch.tensor(f, dtype=torch.float64, device=dev, requires_grad=True)
-    ut_t = torch.tensor(ut, dtype=torch.float64, device=dev)
+    x = torch.tensor(pos.reshape(-1), dtype=torch.float64, requires_grad=True)
+    ft = torch.tensor(f, dtype=torch.float64, requires_grad=True)
+    ut_t = torch.tensor(ut, dtype=torch.float64)
     y = mobility_product(x, ft, smix)
     loss = 0.5 * ((y - ut_t) ** 2).sum()
     loss.backward()
-    gx = x.grad.detach().cpu().numpy()
-    gf = ft.grad.detach().cpu().numpy()
+    gx = x.grad.detach().numpy()
+    gf = ft.grad.detach().numpy()
 
     # FP64 references: dL/df = M r exactly, dL/dx by central differences
     s64 = ls.Stokes(0, pos, box=(L, L, L), tr=1.0)

@@ -42,6 +42,21 @@ def test_split_matches_reference_splitting(version):
     s.close()
 
 
+def test_split_unequal_spheres_fts():
+    n = 40
+    pos, a, L = workloads.configuration_poly(n, 0.2, seed=8)
+    o = po.System(2, pos, box=(L, L, L), tr=1.0, a=a)
+    x = np.random.default_rng(4).standard_normal(11 * n)
+    y_ref = o.atimes(x)
+    s = ls.Stokes(2, pos, box=(L, L, L), tr=1.0, a=a)
+    for tr in (30.0, 1000.0):
+        s.set_split(tr)
+        e = relerr(s.atimes(x), y_ref)
+        _say(f"  unequal spheres split tr={tr:g}: rel err {e:.2e}")
+        assert e < TOL64
+    s.close()
+
+
 def test_split_matrix_and_fix_mode():
     n = 27
     pos, L = workloads.configuration(n, 0.1, seed=2)
