@@ -187,6 +187,58 @@ def bench_bd(args, log):
     return out
 
 
+def _poly_box(args):
+    n, seed = args
+    return workloads.configuration_poly(n, 0.25, seed=seed)
+
+
+def bench_ensemble(args, log):
+    """Batched ensemble: nbox independent FTS boxes, N log-uniform on
+    [50, 2000], radii log-uniform on [0.5, 2], phi = 0.25, MIXED precision,
+    one graph launch per ensemble product (each box at the splitting of
+    tr = 1000, results those of the box's own tr = 1 splitting)"""
+    import multiprocessing as mp
+    nbox = args.ens_boxes
+    rng = np.random.default_rng(0)
+    ns = np.exp(rng.uniform(math.log(50), math.log(2000), nbox)).astype(int)
+    t0 = time.perf_counter()
+    with mp.get_context("spawn").Pool(8) as pool:
+        boxes = pool.map(_poly_box, [(int(n), 1000 + k) for k, n in
+                                     enumerate(ns)])
+    log(f"ensemble: {nbox} configurations in {time.perf_counter() - t0:.1f} s")
+    systems = []
+    for pos, a, L in boxes:
+        s = ls.Stokes(2, pos, box=(L, L, L), tr=1.0, a=a, precision=ls.MIXED)
+        s.set_split(1000.0)
+        systems.append(s)
+    ens = ls.Ensemble(systems)
+    tot = ens.offsets[-1]
+    posd = ls.DeviceArray.from_host(
+        np.concatenate([b[0].reshape(-1) for b in boxes]))
+    xd = ls.DeviceArray.from_host(
+        np.random.default_rng(1).standard_normal(11 * tot))
+    yd = ls.DeviceArray(11 * tot)
+    t0 = time.perf_counter()
+    ens.atimes_dev(posd, xd, yd)          # capture + first launch
+    log(f"ensemble: capture + first launch {time.perf_counter() - t0:.2f} s")
+    reps = max(3, args.steps // 4)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        ens.atimes_dev(posd, xd, yd)
+    t = (time.perf_counter() - t0) / reps
+    log(f"ensemble: {t * 1e3:.2f} ms per launch")
+    out = {"workload": f"{nbox} FTS boxes, N log-uniform [50, 2000] "
+                       f"(sum {tot}), radii log-uniform [0.5, 2], phi 0.25, "
+                       f"MIXED, one graph launch",
+           "nbox": nbox, "sum_n": int(tot), "ms_per_launch": t * 1e3,
+           "box_matvecs_per_s": nbox / t,
+           "particle_matvecs_per_s": tot / t}
+    ens.close()
+    for s in systems:
+        s.close()
+    return out
+
+
 def profile_run(args):
     """kernels for the counter runs: k_copy_calibrate on 256 MiB (a known
     byte count), then --steps matvecs of both precisions at the splitting
@@ -226,6 +278,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--batch", type=int, default=32,
                     help="right-hand sides per launch for the batched figures")
+    ap.add_argument("--ens-boxes", type=int, default=256,
+                    help="boxes of the batched-ensemble figure (0: skip)")
     ap.add_argument("--bd-steps", type=int, default=50,
                     help="timed BD steps per precision (0: skip BD)")
     ap.add_argument("--profile", action="store_true",
@@ -342,6 +396,8 @@ def main():
     }
     if args.bd_steps > 0:
         out["bd"] = bench_bd(args, log)
+    if args.ens_boxes > 0:
+        out["ensemble"] = bench_ensemble(args, log)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(pos, L, x, 1.0, args.cpu_budget)
     print(json.dumps(out))
