@@ -195,7 +195,11 @@ k_recip_back (const double4 *__restrict__ pos, const double4 *__restrict__ kv,
     }
 }
 
-// one thread per output component (i, q)
+// COMBINE_LANES lanes per output component (i, q): lane g sums the partial
+// chunks g, g + 16, ...; the lanes then combine by a fixed butterfly, so
+// the summation order (and the result) does not depend on scheduling
+constexpr int COMBINE_LANES = 16;
+
 template <int V, typename T>
 __global__ __launch_bounds__ (256) void
 k_combine (const double4 *__restrict__ pos, const double *__restrict__ x,
@@ -203,14 +207,230 @@ k_combine (const double4 *__restrict__ pos, const double *__restrict__ x,
            int nrecip, DevParams p, int periodic, double *__restrict__ y)
 {
   constexpr int D = Dim<V>::D;
-  const size_t t = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t gt = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t t = gt / COMBINE_LANES;
+  const int g = (int) (gt % COMBINE_LANES);
   const size_t n = (size_t) p.np * D;
-  if (t >= n) return;
-  const int i = (int) (t / D), q = (int) (t - (size_t) i * D);
-  double acc = self_diag (p, pos[i].w, periodic != 0, V, q) * x[t];
-  for (int ch = 0; ch < nreal; ch++) acc += rpart[(size_t) ch * n + t];
-  for (int ch = 0; ch < nrecip; ch++) acc += double (kpart[(size_t) ch * n + t]);
-  y[t] = acc;
+  const bool live = t < n;
+  const size_t tt = live ? t : 0;
+  double acc = 0.0;
+  if (live)
+    {
+      for (int ch = g; ch < nreal; ch += COMBINE_LANES) acc += rpart[(size_t) ch * n + tt];
+      for (int ch = g; ch < nrecip; ch += COMBINE_LANES)
+        acc += double (kpart[(size_t) ch * n + tt]);
+    }
+#pragma unroll
+  for (int off = COMBINE_LANES / 2; off > 0; off >>= 1) acc += __shfl_xor (acc, off, 64);
+  if (live && g == 0)
+    {
+      const int i = (int) (tt / D), q = (int) (tt - (size_t) i * D);
+      y[tt] = self_diag (p, pos[i].w, periodic != 0, V, q) * x[tt] + acc;
+    }
+}
+
+// ---- separable phases ---------------------------------------------------
+//
+// For k = m1 b1 + m2 b2 + m3 b3 on the (possibly sheared) reciprocal
+// lattice, k.x = m1 th1 + m2 th2 + m3 th3 with th2 = k0y y, th3 = k0z z and
+// th1 = k0x x - s1 k0y y - s2 k0z z (s1 or s2 the shear shift / lx of mode 1
+// or 2).  Every particle gets a table of e^{i m th} for m1 in [0, K1],
+// m2 in [-K2, K2], m3 in [-K3, K3] (FP64 sincos, stored in T); the
+// structure factors then need two complex products per (k, j) instead of a
+// sincos, and the back-projection walks each lattice row (m1, m2 fixed, m3
+// consecutive) by multiplying with e^{i th3} in FP64.
+
+template <typename T> struct Cplx { T re, im; };
+
+struct PhaseBasis {
+  double k0x, k0y, k0z, s1, s2;
+  int K1, K2, K3, W;           // W = (K1 + 1) + (2 K2 + 1) + (2 K3 + 1)
+};
+
+__device__ __forceinline__ void phase_angles (const PhaseBasis &b, const double4 p, double &t1,
+                                              double &t2, double &t3)
+{
+  t2 = b.k0y * p.y;
+  t3 = b.k0z * p.z;
+  t1 = b.k0x * p.x - b.s1 * t2 - b.s2 * t3;
+}
+
+template <typename T>
+__global__ void k_phase_tab (const double4 *__restrict__ pos, int np, PhaseBasis b,
+                             Cplx<T> *__restrict__ tab)
+{
+  const size_t t = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (size_t) np * b.W) return;
+  const int j = (int) (t / b.W), e = (int) (t - (size_t) j * b.W);
+  double t1, t2, t3;
+  phase_angles (b, pos[j], t1, t2, t3);
+  double th;
+  int m;
+  if (e <= b.K1) { m = e; th = t1; }
+  else if (e <= b.K1 + 2 * b.K2 + 1) { m = e - (b.K1 + 1) - b.K2; th = t2; }
+  else { m = e - (b.K1 + 2 * b.K2 + 2) - b.K3; th = t3; }
+  double s, c;
+  sincos (double (m) * th, &s, &c);
+  tab[t] = Cplx<T>{T (c), T (s)};
+}
+
+template <typename T>
+__device__ __forceinline__ Cplx<T> cmul (const Cplx<T> a, const Cplx<T> b)
+{
+  return Cplx<T>{a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+
+// structure factors with tabulated phases: thread per k, j tiles of JT
+// particles (tables, z and a_j^2) in dynamic LDS
+template <int V, typename T, int NPOW>
+__global__ __launch_bounds__ (256) void
+k_recip_sf_tab (const double4 *__restrict__ pos, const double *__restrict__ x,
+                const int4 *__restrict__ km, const Cplx<T> *__restrict__ tab, PhaseBasis b,
+                int np, int nk, int jper, int JT, T *__restrict__ part)
+{
+  constexpr int D = Dim<V>::D;
+  constexpr int NS = (NPOW > 1) ? 4 : 2;
+  extern __shared__ __align__ (16) unsigned char smem_sf[];
+  Cplx<T> *stab = (Cplx<T> *) smem_sf;
+  T *sz = (T *) (stab + (size_t) JT * b.W);
+  T *sa2 = sz + (size_t) JT * D;
+  const int k = blockIdx.x * 256 + threadIdx.x;
+  const int j0 = blockIdx.y * jper;
+  const int j1 = min (np, j0 + jper);
+  const int4 m = km[min (k, nk - 1)];
+  const int o1 = m.x, o2 = b.K1 + 1 + b.K2 + m.y, o3 = b.K1 + 2 * b.K2 + 2 + b.K3 + m.z;
+  T acc[NS * D];
+#pragma unroll
+  for (int q = 0; q < NS * D; q++) acc[q] = T (0);
+  for (int jb = j0; jb < j1; jb += JT)
+    {
+      const int n = min (JT, j1 - jb);
+      const Cplx<T> *src = tab + (size_t) jb * b.W;
+      for (int e = threadIdx.x; e < n * b.W; e += 256) stab[e] = src[e];
+      if (threadIdx.x < n)
+        {
+          T zz[D];
+          load_z<V, T> (x, jb + threadIdx.x, zz);
+#pragma unroll
+          for (int q = 0; q < D; q++) sz[threadIdx.x * D + q] = zz[q];
+          if (NPOW > 1)
+            {
+              const double a = pos[jb + threadIdx.x].w;
+              sa2[threadIdx.x] = T (a * a);
+            }
+        }
+      __syncthreads ();
+      for (int t = 0; t < n; t++)
+        {
+          const Cplx<T> *row = stab + (size_t) t * b.W;
+          const Cplx<T> e = cmul (cmul (row[o1], row[o2]), row[o3]);
+          const T c = e.re, s = e.im;
+          const T *zq = sz + t * D;
+#pragma unroll
+          for (int q = 0; q < D; q++)
+            {
+              acc[q] += c * zq[q];
+              acc[D + q] += s * zq[q];
+            }
+          if (NPOW > 1)
+            {
+              const T c2 = sa2[t] * c, s2 = sa2[t] * s;
+#pragma unroll
+              for (int q = 0; q < D; q++)
+                {
+                  acc[2 * D + q] += c2 * zq[q];
+                  acc[3 * D + q] += s2 * zq[q];
+                }
+            }
+        }
+      __syncthreads ();
+    }
+  if (k < nk)
+    {
+      T *out = part + ((size_t) blockIdx.y * nk + k) * (NS * D);
+#pragma unroll
+      for (int q = 0; q < NS * D; q++) out[q] = acc[q];
+    }
+}
+
+// back-projection walking lattice rows: thread per i, k split over
+// blockIdx.y; km[k].w != 0 marks the first k of a row (direct phase), the
+// others rotate the FP64 phase by e^{i th3}
+template <int V, typename T, int NPOW>
+__global__ __launch_bounds__ (256) void
+k_recip_back_rows (const double4 *__restrict__ pos, const double4 *__restrict__ kv,
+                   const int4 *__restrict__ km, PhaseBasis b, const T *__restrict__ PQ, int np,
+                   int nk, int kper, T *__restrict__ part)
+{
+  constexpr int D = Dim<V>::D;
+  constexpr int KT = 32;
+  __shared__ double4 skv[KT];
+  __shared__ int srow[KT];
+  __shared__ T spq[KT][NPOW * 2 * D];
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  const int k0 = blockIdx.y * kper;
+  const int k1 = min (nk, k0 + kper);
+  const double4 pi = pos[min (i, np - 1)];
+  double t1, t2, t3;
+  phase_angles (b, pi, t1, t2, t3);
+  double rs, rc;
+  sincos (t3, &rs, &rc);
+  double pc = 1.0, ps = 0.0;
+  T apow[NPOW];
+  apow[0] = NPOW > 1 ? T (pi.w) : T (1);
+#pragma unroll
+  for (int t = 1; t < NPOW; t++) apow[t] = apow[t - 1] * T (pi.w);
+  T acc[D];
+#pragma unroll
+  for (int q = 0; q < D; q++) acc[q] = T (0);
+  for (int kb = k0; kb < k1; kb += KT)
+    {
+      const int n = min (KT, k1 - kb);
+      for (int e = threadIdx.x; e < n * (NPOW * 2 * D); e += 256)
+        {
+          const int kt = e / (NPOW * 2 * D), q = e - kt * (NPOW * 2 * D);
+          spq[kt][q] = PQ[(size_t) (kb + kt) * (NPOW * 2 * D) + q];
+        }
+      if (threadIdx.x < n)
+        {
+          skv[threadIdx.x] = kv[kb + threadIdx.x];
+          srow[threadIdx.x] = km[kb + threadIdx.x].w;
+        }
+      __syncthreads ();
+      for (int t = 0; t < n; t++)
+        {
+          if (srow[t] || kb + t == k0)
+            {
+              const double4 kk = skv[t];
+              T c0, s0;
+              phase_cs (kk.x * pi.x + kk.y * pi.y + kk.z * pi.z, c0, s0);
+              pc = double (c0);
+              ps = double (s0);
+            }
+          else
+            {
+              const double nc = pc * rc - ps * rs;
+              ps = ps * rc + pc * rs;
+              pc = nc;
+            }
+          const T c = T (pc), s = T (ps);
+#pragma unroll
+          for (int pw = 0; pw < NPOW; pw++)
+            {
+              const T cc = apow[pw] * c, ss = apow[pw] * s;
+              const T *P = spq[t] + pw * 2 * D;
+#pragma unroll
+              for (int q = 0; q < D; q++) acc[q] += cc * P[q] + ss * P[D + q];
+            }
+        }
+      __syncthreads ();
+    }
+  if (i < np)
+    {
+      T *out = part + ((size_t) blockIdx.y * np + i) * D;
+#pragma unroll
+      for (int q = 0; q < D; q++) out[q] = acc[q];
+    }
 }
 
 }  // namespace shk

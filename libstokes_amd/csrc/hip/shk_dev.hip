@@ -59,6 +59,8 @@ struct shk_dev {
   DevBuf pos, img, imgf, kv, ks, partial, pq, xs, ys, zt;
   DevBuf cand, candf, candm, tbits, kpart, kpq, kback;
   DevBuf adjfg, adjcs, adjpart;
+  DevBuf km, ptab;                // k lattice indices, per-particle phase tables
+  PhaseBasis basis{};
   int ncand = 0, ncore = 0, tbits_words = 0;
   int tx = 0, ty = 0, tz = 0;
   double s2x = 0.0, s3x = 0.0;
@@ -122,7 +124,7 @@ extern "C" void shk_dev_destroy (shk_dev *d)
   for (DevBuf *b : {&d->pos, &d->img, &d->imgf, &d->kv, &d->ks, &d->partial,
                     &d->pq, &d->xs, &d->ys, &d->zt, &d->cand, &d->candf, &d->candm,
                     &d->tbits, &d->kpart, &d->kpq, &d->kback, &d->adjfg, &d->adjcs,
-                    &d->adjpart})
+                    &d->adjpart, &d->km, &d->ptab})
     release (*b);
   for (auto &e : d->ev)
     if (e) (void) hipEventDestroy (e);
@@ -304,6 +306,22 @@ extern "C" int shk_dev_set_system (shk_dev *d, const shk_params *p, const double
       if ((rc = upload (d, d->tbits, t->tbits, sizeof (unsigned) * t->tbits_words))) return rc;
       if ((rc = upload (d, d->kv, t->kv, sizeof (double4) * p->nk))) return rc;
       if ((rc = upload (d, d->ks, t->ks, sizeof (double) * 8 * p->nk))) return rc;
+      if (p->nk > 0 && !t->km)
+        {
+          set_err ("shk_dev_set_system: k table without lattice indices");
+          return SHK_EARG;
+        }
+      if ((rc = upload (d, d->km, t->km, sizeof (int4) * p->nk))) return rc;
+      const double twopi = 6.28318530717958647692;
+      d->basis.k0x = twopi / p->lx;
+      d->basis.k0y = twopi / p->ly;
+      d->basis.k0z = twopi / p->lz;
+      d->basis.s1 = t->kbasis_s1;
+      d->basis.s2 = t->kbasis_s2;
+      d->basis.K1 = t->kmax1;
+      d->basis.K2 = t->kmax2;
+      d->basis.K3 = t->kmax3;
+      d->basis.W = (t->kmax1 + 1) + (2 * t->kmax2 + 1) + (2 * t->kmax3 + 1);
       if ((rc = ensure (d, d->candf, sizeof (float4) * (t->ncand > 0 ? t->ncand : 1)))) return rc;
       if (t->ncand > 0)
         {
@@ -376,6 +394,15 @@ static void recip_shape (int np, int nk, int &sj, int &jper, int &sk, int &kper)
   sk = (nk + kper - 1) / kper;
 }
 
+// particles per LDS tile of the table-phase structure-factor kernel
+template <int V, typename T>
+static int sf_tile (const PhaseBasis &b)
+{
+  const size_t per = sizeof (Cplx<T>) * b.W + sizeof (T) * (Dim<V>::D + 1);
+  int jt = (int) ((48 * 1024) / per);
+  return jt > 64 ? 64 : (jt < 1 ? 1 : jt);
+}
+
 template <int V, typename T, int NPOW>
 static int recip_launch (shk_dev *d, const double *x, int &nrecip, const void *&kpart)
 {
@@ -389,17 +416,26 @@ static int recip_launch (shk_dev *d, const double *x, int &nrecip, const void *&
   if ((rc = ensure (d, d->kpart, sizeof (T) * (size_t) sj * nk * NS * D))) return rc;
   if ((rc = ensure (d, d->kpq, sizeof (T) * (size_t) nk * NPOW * 2 * D))) return rc;
   if ((rc = ensure (d, d->kback, sizeof (T) * (size_t) sk * p.np * D))) return rc;
+  const PhaseBasis &b = d->basis;
+  if ((rc = ensure (d, d->ptab, sizeof (Cplx<T>) * (size_t) p.np * b.W))) return rc;
   const double4 *pos4 = (const double4 *) d->pos.p;
   const double4 *kv = (const double4 *) d->kv.p;
-  hipLaunchKernelGGL ((k_recip_sf<V, T, NPOW>), dim3 ((nk + 255) / 256, sj), dim3 (256), 0,
-                      d->stream, pos4, x, kv, p.np, nk, jper, (T *) d->kpart.p);
+  const int4 *km = (const int4 *) d->km.p;
+  hipLaunchKernelGGL ((k_phase_tab<T>), dim3 ((unsigned) (((size_t) p.np * b.W + 255) / 256)),
+                      dim3 (256), 0, d->stream, pos4, p.np, b, (Cplx<T> *) d->ptab.p);
+  HIPCHK (hipGetLastError ());
+  const int jt = sf_tile<V, T> (b);
+  const size_t lds = sizeof (Cplx<T>) * (size_t) jt * b.W + sizeof (T) * (size_t) jt * (D + 1);
+  hipLaunchKernelGGL ((k_recip_sf_tab<V, T, NPOW>), dim3 ((nk + 255) / 256, sj), dim3 (256), lds,
+                      d->stream, pos4, x, km, (const Cplx<T> *) d->ptab.p, b, p.np, nk, jper, jt,
+                      (T *) d->kpart.p);
   HIPCHK (hipGetLastError ());
   hipLaunchKernelGGL ((k_recip_blocks<V, T, NPOW>), dim3 ((nk + 127) / 128), dim3 (128), 0,
                       d->stream, kv, (const double *) d->ks.p, nk, sj, p.gsign,
                       (const T *) d->kpart.p, (T *) d->kpq.p);
   HIPCHK (hipGetLastError ());
-  hipLaunchKernelGGL ((k_recip_back<V, T, NPOW>), dim3 ((p.np + 255) / 256, sk), dim3 (256), 0,
-                      d->stream, pos4, kv, (const T *) d->kpq.p, p.np, nk, kper,
+  hipLaunchKernelGGL ((k_recip_back_rows<V, T, NPOW>), dim3 ((p.np + 255) / 256, sk), dim3 (256),
+                      0, d->stream, pos4, kv, km, b, (const T *) d->kpq.p, p.np, nk, kper,
                       (T *) d->kback.p);
   HIPCHK (hipGetLastError ());
   nrecip = sk;
@@ -452,7 +488,9 @@ static int atimes_vt (shk_dev *d, const double *x, double *y)
       HIPCHK (hipGetLastError ());
       if (d->timing) HIPCHK (hipEventRecord (d->ev[1], d->stream));
     }
-  hipLaunchKernelGGL ((k_combine<V, T>), dim3 ((p.np * D + 255) / 256), dim3 (256), 0, d->stream,
+  hipLaunchKernelGGL ((k_combine<V, T>),
+                      dim3 ((unsigned) (((size_t) p.np * D * COMBINE_LANES + 255) / 256)),
+                      dim3 (256), 0, d->stream,
                       pos4, x, (const double *) d->partial.p, nreal, (const T *) kpart, nrecip, q,
                       p.periodic, y);
   HIPCHK (hipGetLastError ());

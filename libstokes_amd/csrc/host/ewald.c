@@ -205,7 +205,9 @@ build_kvectors (const struct stokes *s, double gsign, double **kv_out,
             }
           double *v = kv + 4 * nk, *w = ks + 8 * nk;
           km[4 * nk] = m1; km[4 * nk + 1] = m2; km[4 * nk + 2] = m3;
-          km[4 * nk + 3] = 0;
+          /* first k of a lattice row (m1, m2 fixed, m3 consecutive) */
+          km[4 * nk + 3] = !(nk > 0 && km[4 * nk - 4] == m1 && km[4 * nk - 3] == m2
+                             && km[4 * nk - 2] == m3 - 1);
           v[0] = k1; v[1] = k2; v[2] = k3; v[3] = k;
           w[0] = 2.0 * 6.0 * (1.0 - kk / 3.0) * kexp;              /* ya */
           w[1] = 2.0 * 3.0 * k * kexp;                             /* yb */
@@ -240,8 +242,9 @@ build_kvectors (const struct stokes *s, double gsign, double **kv_out,
  * -1. */
 static int
 add_g_correction (const struct stokes *s, double **kv_io, double **ks_io,
-                  int *km_t, int nk_t)
+                  int **km_io, int nk_t)
 {
+  int *km_t = *km_io;
   double *kv1 = NULL, *ks1 = NULL;
   int *km1 = NULL;
   int nk1 = build_kvectors (s, 1.0, &kv1, &ks1, &km1);
@@ -257,15 +260,16 @@ add_g_correction (const struct stokes *s, double **kv_io, double **ks_io,
   int *slot = malloc (sizeof (int) * (ncell > 0 ? ncell : 1));
   double *kv = realloc (*kv_io, sizeof (double) * 4 * (nk_t + nk1 + 1));
   double *ks = realloc (*ks_io, sizeof (double) * 8 * (nk_t + nk1 + 1));
-  if (!slot || !kv || !ks || !kv1 || !ks1 || !km1)
+  int *km = realloc (km_t, sizeof (int) * 4 * (nk_t + nk1 + 1));
+  if (kv) *kv_io = kv;
+  if (ks) *ks_io = ks;
+  if (km) *km_io = km;
+  if (!slot || !kv || !ks || !km || !kv1 || !ks1 || !km1)
     {
       free (slot); free (kv1); free (ks1); free (km1);
-      if (kv) *kv_io = kv;
-      if (ks) *ks_io = ks;
       return -1;
     }
-  *kv_io = kv;
-  *ks_io = ks;
+  km_t = km;
   for (long c = 0; c < ncell; c++) slot[c] = -1;
   for (int k = 0; k < nk_t; k++)
     slot[(km_t[4 * k] * ny + km_t[4 * k + 1] + KY) * nz + km_t[4 * k + 2] + KZ] = k;
@@ -282,6 +286,10 @@ add_g_correction (const struct stokes *s, double **kv_io, double **ks_io,
           memcpy (kv + 4 * t, kv1 + 4 * k, sizeof (double) * 4);
           for (int q = 0; q < 8; q++) ks[8 * t + q] = 0.0;
           ks[8 * t + 6] = ks1[8 * k + 6];   /* kk; kexp 0: G terms only */
+          memcpy (km + 4 * t, km1 + 4 * k, sizeof (int) * 3);
+          km[4 * t + 3] = !(t > 0 && km[4 * t - 4] == km[4 * t]
+                            && km[4 * t - 3] == km[4 * t + 1]
+                            && km[4 * t - 2] == km[4 * t + 2] - 1);
         }
       ks[8 * t + 3] += 2.0 * ks1[8 * k + 3];
     }
@@ -471,7 +479,16 @@ stokes_hip_prepare (stokes_ctx *c, int positions_on_device)
             }
           p.nk = build_kvectors (src, p.gsign, &kv, &ks, &km);
           if (split && gcorr)
-            p.nk = add_g_correction (s, &kv, &ks, km, p.nk);
+            p.nk = add_g_correction (s, &kv, &ks, &km, p.nk);
+          t.km = km;
+          t.kbasis_s1 = s->shear_mode == 1 ? s->shear_shift / s->lx : 0.0;
+          t.kbasis_s2 = s->shear_mode == 2 ? s->shear_shift / s->lx : 0.0;
+          for (int k = 0; k < p.nk && km; k++)
+            {
+              if (km[4 * k] > t.kmax1) t.kmax1 = km[4 * k];
+              if (abs (km[4 * k + 1]) > t.kmax2) t.kmax2 = abs (km[4 * k + 1]);
+              if (abs (km[4 * k + 2]) > t.kmax3) t.kmax3 = abs (km[4 * k + 2]);
+            }
           t.img = img;
           t.kv = kv;
           t.ks = ks;

@@ -66,6 +66,9 @@ typedef struct shk_tables {
   int tbits_words, tx, ty, tz;
   double s2x, s3x;
   const double *kv, *ks;
+  const int *km;        /* (m1, m2, m3, first-of-row) per k */
+  int kmax1, kmax2, kmax3;          /* max |m| over the k table */
+  double kbasis_s1, kbasis_s2;      /* shear shift / lx of mode 1 / 2 */
 } shk_tables;
 
 /* Upload positions [3np] (AoS), radii [np] or NULL and the tables. */
