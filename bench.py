@@ -130,6 +130,8 @@ def bench_bd(args, log):
         # overlap-driven dt control (brownian.c:2436, 2502)
         s = ls.Stokes(0, pos, box=(L, L, L), tr=1.0, precision=prec,
                       rmin=1.0)
+        # splitting by the autotuner (the result stays that of tr = 1)
+        tr_bd, _ = s.autotune(nrhs=1, reps=3)
         xd = ls.DeviceArray.from_host(pos.reshape(-1))
         # order 30 on this spectrum (condition ~10^2) is accurate to ~1e-2
         # in z.Mz: eps is set so the fit is not redone every step
@@ -159,7 +161,7 @@ def bench_bd(args, log):
                      "ms_per_step": t / args.bd_steps * 1e3,
                      "attempts_per_step": att / args.bd_steps,
                      "halvings": halv, "redraws": redr,
-                     "err_cheb_last": info.err_cheb,
+                     "err_cheb_last": info.err_cheb, "tr": tr_bd,
                      "eig": [info.eig[0], info.eig[1]]}
         log(f"BD {name}: {out[name]['ms_per_step']:.3f} ms/step")
         bd.close()
