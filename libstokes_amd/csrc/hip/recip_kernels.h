@@ -280,8 +280,12 @@ __device__ __forceinline__ Cplx<T> cmul (const Cplx<T> a, const Cplx<T> b)
   return Cplx<T>{a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
 }
 
-// structure factors with tabulated phases: thread per k, j tiles of JT
-// particles (tables, z and a_j^2) in dynamic LDS
+// structure factors with tabulated phases.  A block is 64 k (one per lane)
+// x 4 waves: the block's j range goes in tiles of JT particles (tables, z
+// and a_j^2 in dynamic LDS), wave w takes every fourth particle of a tile,
+// and the four partial sums are added in LDS in wave order at the end --
+// four times the waves of a thread-per-k layout for the same partial
+// buffer, and a fixed summation order.
 template <int V, typename T, int NPOW>
 __global__ __launch_bounds__ (256) void
 k_recip_sf_tab (const double4 *__restrict__ pos, const double *__restrict__ x,
@@ -294,7 +298,9 @@ k_recip_sf_tab (const double4 *__restrict__ pos, const double *__restrict__ x,
   Cplx<T> *stab = (Cplx<T> *) smem_sf;
   T *sz = (T *) (stab + (size_t) JT * b.W);
   T *sa2 = sz + (size_t) JT * D;
-  const int k = blockIdx.x * 256 + threadIdx.x;
+  T *red = (T *) smem_sf;                        // reused after the j loop
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int k = blockIdx.x * 64 + lane;
   const int j0 = blockIdx.y * jper;
   const int j1 = min (np, j0 + jper);
   const int4 m = km[min (k, nk - 1)];
@@ -320,7 +326,7 @@ k_recip_sf_tab (const double4 *__restrict__ pos, const double *__restrict__ x,
             }
         }
       __syncthreads ();
-      for (int t = 0; t < n; t++)
+      for (int t = wave; t < n; t += 4)
         {
           const Cplx<T> *row = stab + (size_t) t * b.W;
           const Cplx<T> e = cmul (cmul (row[o1], row[o2]), row[o3]);
@@ -345,7 +351,23 @@ k_recip_sf_tab (const double4 *__restrict__ pos, const double *__restrict__ x,
         }
       __syncthreads ();
     }
-  if (k < nk)
+  // waves 1..3 hand their sums to wave 0, D components at a time (fixed
+  // order); the region holds 3 x 64 x D values
+#pragma unroll
+  for (int c = 0; c < NS; c++)
+    {
+      if (wave > 0)
+#pragma unroll
+        for (int q = 0; q < D; q++) red[((wave - 1) * D + q) * 64 + lane] = acc[c * D + q];
+      __syncthreads ();
+      if (wave == 0)
+#pragma unroll
+        for (int q = 0; q < D; q++)
+          acc[c * D + q] += red[q * 64 + lane] + red[(D + q) * 64 + lane]
+            + red[(2 * D + q) * 64 + lane];
+      __syncthreads ();
+    }
+  if (wave == 0 && k < nk)
     {
       T *out = part + ((size_t) blockIdx.y * nk + k) * (NS * D);
 #pragma unroll
@@ -353,9 +375,11 @@ k_recip_sf_tab (const double4 *__restrict__ pos, const double *__restrict__ x,
     }
 }
 
-// back-projection walking lattice rows: thread per i, k split over
-// blockIdx.y; km[k].w != 0 marks the first k of a row (direct phase), the
-// others rotate the FP64 phase by e^{i th3}
+// back-projection walking lattice rows.  A block is 64 i (one per lane) x
+// 4 waves; the block's k range [k0, k1) is cut into four consecutive
+// quarters, one per wave, each with its own LDS tiles.  km[k].w != 0 marks
+// the first k of a row (direct phase); the other k rotate the FP64 phase by
+// e^{i th3}.  The four partial sums are added in LDS in wave order.
 template <int V, typename T, int NPOW>
 __global__ __launch_bounds__ (256) void
 k_recip_back_rows (const double4 *__restrict__ pos, const double4 *__restrict__ kv,
@@ -363,13 +387,19 @@ k_recip_back_rows (const double4 *__restrict__ pos, const double4 *__restrict__ 
                    int nk, int kper, T *__restrict__ part)
 {
   constexpr int D = Dim<V>::D;
-  constexpr int KT = 32;
-  __shared__ double4 skv[KT];
-  __shared__ int srow[KT];
-  __shared__ T spq[KT][NPOW * 2 * D];
-  const int i = blockIdx.x * 256 + threadIdx.x;
-  const int k0 = blockIdx.y * kper;
-  const int k1 = min (nk, k0 + kper);
+  constexpr int KT = NPOW > 1 ? 8 : 16;          // LDS stays under 64 KB
+  constexpr int PW = NPOW * 2 * D;
+  __shared__ double4 skv[4][KT];
+  __shared__ int srow[4][KT];
+  __shared__ T spq[4][KT][PW];
+  __shared__ T red[3][D][64];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int i = blockIdx.x * 64 + lane;
+  const int kb0 = blockIdx.y * kper;
+  const int kb1 = min (nk, kb0 + kper);
+  const int kq = (kb1 - kb0 + 3) / 4;
+  const int k0 = min (kb1, kb0 + wave * kq);
+  const int k1 = min (kb1, k0 + kq);
   const double4 pi = pos[min (i, np - 1)];
   double t1, t2, t3;
   phase_angles (b, pi, t1, t2, t3);
@@ -386,22 +416,22 @@ k_recip_back_rows (const double4 *__restrict__ pos, const double4 *__restrict__ 
   for (int kb = k0; kb < k1; kb += KT)
     {
       const int n = min (KT, k1 - kb);
-      for (int e = threadIdx.x; e < n * (NPOW * 2 * D); e += 256)
+      for (int e = lane; e < n * PW; e += 64)
         {
-          const int kt = e / (NPOW * 2 * D), q = e - kt * (NPOW * 2 * D);
-          spq[kt][q] = PQ[(size_t) (kb + kt) * (NPOW * 2 * D) + q];
+          const int kt = e / PW, q = e - kt * PW;
+          spq[wave][kt][q] = PQ[(size_t) (kb + kt) * PW + q];
         }
-      if (threadIdx.x < n)
+      if (lane < n)
         {
-          skv[threadIdx.x] = kv[kb + threadIdx.x];
-          srow[threadIdx.x] = km[kb + threadIdx.x].w;
+          skv[wave][lane] = kv[kb + lane];
+          srow[wave][lane] = km[kb + lane].w;
         }
-      __syncthreads ();
+      wave_sync ();
       for (int t = 0; t < n; t++)
         {
-          if (srow[t] || kb + t == k0)
+          if (srow[wave][t] || kb + t == k0)
             {
-              const double4 kk = skv[t];
+              const double4 kk = skv[wave][t];
               T c0, s0;
               phase_cs (kk.x * pi.x + kk.y * pi.y + kk.z * pi.z, c0, s0);
               pc = double (c0);
@@ -418,18 +448,22 @@ k_recip_back_rows (const double4 *__restrict__ pos, const double4 *__restrict__ 
           for (int pw = 0; pw < NPOW; pw++)
             {
               const T cc = apow[pw] * c, ss = apow[pw] * s;
-              const T *P = spq[t] + pw * 2 * D;
+              const T *P = spq[wave][t] + pw * 2 * D;
 #pragma unroll
               for (int q = 0; q < D; q++) acc[q] += cc * P[q] + ss * P[D + q];
             }
         }
-      __syncthreads ();
+      wave_sync ();
     }
-  if (i < np)
+  if (wave > 0)
+#pragma unroll
+    for (int q = 0; q < D; q++) red[wave - 1][q][lane] = acc[q];
+  __syncthreads ();
+  if (wave == 0 && i < np)
     {
       T *out = part + ((size_t) blockIdx.y * np + i) * D;
 #pragma unroll
-      for (int q = 0; q < D; q++) out[q] = acc[q];
+      for (int q = 0; q < D; q++) out[q] = acc[q] + red[0][q][lane] + red[1][q][lane] + red[2][q][lane];
     }
 }
 

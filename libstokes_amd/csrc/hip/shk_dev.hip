@@ -394,13 +394,44 @@ static void recip_shape (int np, int nk, int &sj, int &jper, int &sk, int &kper)
   sk = (nk + kper - 1) / kper;
 }
 
-// particles per LDS tile of the table-phase structure-factor kernel
+// particles per LDS tile of the table-phase structure-factor kernel (the
+// tile region also holds the 3 x 64 x D end-of-block reduction)
 template <int V, typename T>
 static int sf_tile (const PhaseBasis &b)
 {
   const size_t per = sizeof (Cplx<T>) * b.W + sizeof (T) * (Dim<V>::D + 1);
-  int jt = (int) ((48 * 1024) / per);
-  return jt > 64 ? 64 : (jt < 1 ? 1 : jt);
+  int jt = (int) ((40 * 1024) / per);
+  return jt > 64 ? 64 : (jt < 4 ? 4 : jt);
+}
+
+template <int V, typename T, int NPOW>
+static size_t sf_lds (const PhaseBasis &b, int jt)
+{
+  constexpr int NS = NPOW > 1 ? 4 : 2;
+  const size_t tile = sizeof (Cplx<T>) * (size_t) jt * b.W
+    + sizeof (T) * (size_t) jt * (Dim<V>::D + 1);
+  const size_t red = sizeof (T) * 3 * 64 * Dim<V>::D;
+  return tile > red ? tile : red;
+}
+
+// splits of the reciprocal kernels: about 8192 waves in flight, at least 16
+// particles / 16 k per block
+static void recip_shape_tab (int np, int nk, int &sj, int &jper, int &sk, int &kper)
+{
+  const int nkb = (nk + 63) / 64;
+  sj = (8192 + 4 * nkb - 1) / (4 * nkb);
+  if (sj > 64) sj = 64;
+  if (sj > (np + 15) / 16) sj = (np + 15) / 16;
+  if (sj < 1) sj = 1;
+  jper = (np + sj - 1) / sj;
+  sj = (np + jper - 1) / jper;
+  const int nib = (np + 63) / 64;
+  sk = (8192 + 4 * nib - 1) / (4 * nib);
+  if (sk > 256) sk = 256;
+  if (sk > (nk + 15) / 16) sk = (nk + 15) / 16;
+  if (sk < 1) sk = 1;
+  kper = (nk + sk - 1) / sk;
+  sk = (nk + kper - 1) / kper;
 }
 
 template <int V, typename T, int NPOW>
@@ -411,7 +442,7 @@ static int recip_launch (shk_dev *d, const double *x, int &nrecip, const void *&
   const shk_params &p = d->prm;
   const int nk = p.nk;
   int sj, jper, sk, kper;
-  recip_shape (p.np, nk, sj, jper, sk, kper);
+  recip_shape_tab (p.np, nk, sj, jper, sk, kper);
   int rc;
   if ((rc = ensure (d, d->kpart, sizeof (T) * (size_t) sj * nk * NS * D))) return rc;
   if ((rc = ensure (d, d->kpq, sizeof (T) * (size_t) nk * NPOW * 2 * D))) return rc;
@@ -425,8 +456,8 @@ static int recip_launch (shk_dev *d, const double *x, int &nrecip, const void *&
                       dim3 (256), 0, d->stream, pos4, p.np, b, (Cplx<T> *) d->ptab.p);
   HIPCHK (hipGetLastError ());
   const int jt = sf_tile<V, T> (b);
-  const size_t lds = sizeof (Cplx<T>) * (size_t) jt * b.W + sizeof (T) * (size_t) jt * (D + 1);
-  hipLaunchKernelGGL ((k_recip_sf_tab<V, T, NPOW>), dim3 ((nk + 255) / 256, sj), dim3 (256), lds,
+  const size_t lds = sf_lds<V, T, NPOW> (b, jt);
+  hipLaunchKernelGGL ((k_recip_sf_tab<V, T, NPOW>), dim3 ((nk + 63) / 64, sj), dim3 (256), lds,
                       d->stream, pos4, x, km, (const Cplx<T> *) d->ptab.p, b, p.np, nk, jper, jt,
                       (T *) d->kpart.p);
   HIPCHK (hipGetLastError ());
@@ -434,7 +465,7 @@ static int recip_launch (shk_dev *d, const double *x, int &nrecip, const void *&
                       d->stream, kv, (const double *) d->ks.p, nk, sj, p.gsign,
                       (const T *) d->kpart.p, (T *) d->kpq.p);
   HIPCHK (hipGetLastError ());
-  hipLaunchKernelGGL ((k_recip_back_rows<V, T, NPOW>), dim3 ((p.np + 255) / 256, sk), dim3 (256),
+  hipLaunchKernelGGL ((k_recip_back_rows<V, T, NPOW>), dim3 ((p.np + 63) / 64, sk), dim3 (256),
                       0, d->stream, pos4, kv, km, b, (const T *) d->kpq.p, p.np, nk, kper,
                       (T *) d->kback.p);
   HIPCHK (hipGetLastError ());
