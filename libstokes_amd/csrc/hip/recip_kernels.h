@@ -95,11 +95,12 @@ k_recip_blocks (const double4 *__restrict__ kv, const double *__restrict__ ks, i
   double CS[NS * D];
 #pragma unroll
   for (int q = 0; q < NS * D; q++) CS[q] = 0.0;
+  // part[(split * NS*D + q) * nk + k] (k fastest: coalesced)
   for (int sp = 0; sp < nsplit; sp++)
     {
-      const T *src = part + ((size_t) sp * nk + k) * (NS * D);
+      const T *src = part + (size_t) sp * (NS * D) * nk + k;
 #pragma unroll
-      for (int q = 0; q < NS * D; q++) CS[q] += double (src[q]);
+      for (int q = 0; q < NS * D; q++) CS[q] += double (src[(size_t) q * nk]);
     }
   const double4 kk = kv[k];
   const double *w = ks + (size_t) k * 8;
@@ -286,6 +287,12 @@ __device__ __forceinline__ Cplx<T> cmul (const Cplx<T> a, const Cplx<T> b)
 // and the four partial sums are added in LDS in wave order at the end --
 // four times the waves of a thread-per-k layout for the same partial
 // buffer, and a fixed summation order.
+// z rows padded to a multiple of 4 for vector LDS reads
+template <int V> struct ZPad { static constexpr int P = (Dim<V>::D + 3) & ~3; };
+template <typename T> struct Quad;
+template <> struct Quad<float> { using type = float4; };
+template <> struct Quad<double> { using type = double4; };
+
 template <int V, typename T, int NPOW>
 __global__ __launch_bounds__ (256) void
 k_recip_sf_tab (const double4 *__restrict__ pos, const double *__restrict__ x,
@@ -293,11 +300,13 @@ k_recip_sf_tab (const double4 *__restrict__ pos, const double *__restrict__ x,
                 int np, int nk, int jper, int JT, T *__restrict__ part)
 {
   constexpr int D = Dim<V>::D;
+  constexpr int DP = ZPad<V>::P;
   constexpr int NS = (NPOW > 1) ? 4 : 2;
-  extern __shared__ __align__ (16) unsigned char smem_sf[];
-  Cplx<T> *stab = (Cplx<T> *) smem_sf;
-  T *sz = (T *) (stab + (size_t) JT * b.W);
-  T *sa2 = sz + (size_t) JT * D;
+  extern __shared__ __align__ (32) unsigned char smem_sf[];
+  // [JT][DP] z rows, [JT] a_j^2, then [JT][W] phase tables (JT a multiple of 4)
+  T *sz = (T *) smem_sf;
+  T *sa2 = sz + (size_t) JT * DP;
+  Cplx<T> *stab = (Cplx<T> *) (sa2 + JT);
   T *red = (T *) smem_sf;                        // reused after the j loop
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int k = blockIdx.x * 64 + lane;
@@ -318,7 +327,7 @@ k_recip_sf_tab (const double4 *__restrict__ pos, const double *__restrict__ x,
           T zz[D];
           load_z<V, T> (x, jb + threadIdx.x, zz);
 #pragma unroll
-          for (int q = 0; q < D; q++) sz[threadIdx.x * D + q] = zz[q];
+          for (int q = 0; q < DP; q++) sz[threadIdx.x * DP + q] = q < D ? zz[q] : T (0);
           if (NPOW > 1)
             {
               const double a = pos[jb + threadIdx.x].w;
@@ -331,7 +340,14 @@ k_recip_sf_tab (const double4 *__restrict__ pos, const double *__restrict__ x,
           const Cplx<T> *row = stab + (size_t) t * b.W;
           const Cplx<T> e = cmul (cmul (row[o1], row[o2]), row[o3]);
           const T c = e.re, s = e.im;
-          const T *zq = sz + t * D;
+          T zq[DP];
+          const typename Quad<T>::type *zv = (const typename Quad<T>::type *) (sz + t * DP);
+#pragma unroll
+          for (int v = 0; v < DP / 4; v++)
+            {
+              const typename Quad<T>::type z4 = zv[v];
+              zq[4 * v] = z4.x; zq[4 * v + 1] = z4.y; zq[4 * v + 2] = z4.z; zq[4 * v + 3] = z4.w;
+            }
 #pragma unroll
           for (int q = 0; q < D; q++)
             {
@@ -369,9 +385,10 @@ k_recip_sf_tab (const double4 *__restrict__ pos, const double *__restrict__ x,
     }
   if (wave == 0 && k < nk)
     {
-      T *out = part + ((size_t) blockIdx.y * nk + k) * (NS * D);
+      // part[(split * NS*D + q) * nk + k]: consecutive lanes, consecutive k
+      T *out = part + (size_t) blockIdx.y * (NS * D) * nk + k;
 #pragma unroll
-      for (int q = 0; q < NS * D; q++) out[q] = acc[q];
+      for (int q = 0; q < NS * D; q++) out[(size_t) q * nk] = acc[q];
     }
 }
 

@@ -399,17 +399,16 @@ static void recip_shape (int np, int nk, int &sj, int &jper, int &sk, int &kper)
 template <int V, typename T>
 static int sf_tile (const PhaseBasis &b)
 {
-  const size_t per = sizeof (Cplx<T>) * b.W + sizeof (T) * (Dim<V>::D + 1);
-  int jt = (int) ((40 * 1024) / per);
+  const size_t per = sizeof (Cplx<T>) * b.W + sizeof (T) * (ZPad<V>::P + 1);
+  int jt = (int) ((40 * 1024) / per) & ~3;
   return jt > 64 ? 64 : (jt < 4 ? 4 : jt);
 }
 
 template <int V, typename T, int NPOW>
 static size_t sf_lds (const PhaseBasis &b, int jt)
 {
-  constexpr int NS = NPOW > 1 ? 4 : 2;
   const size_t tile = sizeof (Cplx<T>) * (size_t) jt * b.W
-    + sizeof (T) * (size_t) jt * (Dim<V>::D + 1);
+    + sizeof (T) * (size_t) jt * (ZPad<V>::P + 1);
   const size_t red = sizeof (T) * 3 * 64 * Dim<V>::D;
   return tile > red ? tile : red;
 }
