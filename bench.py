@@ -256,13 +256,18 @@ def profile_run(args):
     xd = ls.DeviceArray.from_host(x)
     yd = ls.DeviceArray(11 * n)
     tr = args.tr if args.tr else 3000.0
+    nrhs = args.profile_batch
+    if nrhs > 1:
+        xd = ls.DeviceArray.from_host(
+            np.random.default_rng(3).standard_normal(nrhs * 11 * n))
+        yd = ls.DeviceArray(nrhs * 11 * n)
     for prec in (ls.MIXED, ls.FP64):
         s = ls.Stokes(2, pos, box=(L, L, L), tr=1.0, precision=prec)
         s.set_split(0.0 if tr == 1.0 else tr)
         ls.check(s.L.stokes_hip_set_pos_dev(s.sys, posd.ptr), "set_pos_dev")
-        t = time_matvec(s, xd, yd, args.steps, args.warmup)
+        t = time_matvec(s, xd, yd, args.steps, args.warmup, nrhs=nrhs)
         print(json.dumps({"precision": "mixed" if prec else "fp64", "tr": tr,
-                          "ms_per_matvec": t * 1e3,
+                          "nrhs": nrhs, "ms_per_call": t * 1e3,
                           "copy_calibration_bytes": nb}), flush=True)
         s.close()
 
@@ -284,6 +289,8 @@ def main():
                     help="boxes of the batched-ensemble figure (0: skip)")
     ap.add_argument("--bd-steps", type=int, default=50,
                     help="timed BD steps per precision (0: skip BD)")
+    ap.add_argument("--profile-batch", type=int, default=1,
+                    help="right-hand sides per call in --profile mode")
     ap.add_argument("--profile", action="store_true",
                     help="short run for rocprofv3: a 256 MiB calibration "
                          "copy, then --steps mixed and FP64 matvecs at --tr")
