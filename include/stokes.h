@@ -133,8 +133,9 @@ void stokes_set_l (struct stokes *sys, double lx, double ly, double lz);
 void stokes_set_xi (struct stokes *sys, double xi, double ewald_eps);
 /* replaces stokes.c:765-776 */
 double xi_by_tratio (struct stokes *sys, double tratio);
-/* replaces stokes.c:790-813; the iterative solvers are not part of this
- * library, so the parameters are accepted and sys->it stays NULL */
+/* replaces stokes.c:790-813: keeps solver name, max, restart, eps, debug
+ * and out in sys->it (freed by stokes_free); the library's GMRES
+ * (solve_mob_3fts_0) uses max, restart and eps */
 void stokes_set_iter (struct stokes *sys, const char *solver, int max,
                       int restart, double eps, int debug, FILE *out);
 /* replaces stokes.c:822-868 */

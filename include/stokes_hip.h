@@ -125,6 +125,15 @@ void stokes_hip_scalars_real_f_deriv (double xi, double r, double *out);
 int stokes_hip_solve_mix_dev (struct stokes *sys, const double *b_dev,
                               double *x_dev);
 
+/* FTS natural mobility problem by GMRES on the device (the libiter route
+ * of solve_mob_3fts_0, ewald-3fts.c:263-305): fte_dev[11 np] holds F, T, E
+ * per particle, uos_dev[11 np] receives U, O, S; iterations and the final
+ * relative residual |b - A x| / |b| are returned when the pointers are not
+ * NULL.  Parameters from sys->it (stokes_set_iter). */
+int stokes_hip_solve_mob_3fts_dev (struct stokes *sys, const double *fte_dev,
+                                   double *uos_dev, int *iters,
+                                   double *relres);
+
 /* Chebyshev-Clenshaw z = f(M) y for nrhs vectors held on the device
  * (chebyshev.c:163-228 with the whole recursion on the GPU) */
 int stokes_hip_cheb_eval_dev (struct stokes *sys, int ncheb, const double *a,

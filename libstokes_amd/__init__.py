@@ -55,7 +55,8 @@ EXPORTS = [
     "stokes_hip_make_matrix", "stokes_hip_extmat_dev",
     "stokes_hip_cheb_eval_dev", "stokes_hip_lanczos_bounds",
     "stokes_hip_adjoint_dev", "stokes_hip_scalars_real_f_deriv", "stokes_hip_solve_mix_dev", "solve_mob_3fts_matrix_0",
-    "solve_mix_3fts_matrix_0",
+    "solve_mix_3fts_matrix_0", "stokes_hip_solve_mob_3fts_dev",
+    "solve_mob_3fts_0",
     "stokes_hip_ensemble_create", "stokes_hip_ensemble_atimes",
     "stokes_hip_ensemble_refresh", "stokes_hip_ensemble_offset",
     "stokes_hip_ensemble_destroy",
@@ -151,6 +152,12 @@ def lib():
         getattr(L, f).argtypes = [sp, C.c_int]
     L.stokes_hip_set_split.argtypes = [sp, C.c_double]
     L.stokes_hip_solve_mix_dev.argtypes = [sp, _vp, _vp]
+    L.stokes_hip_solve_mob_3fts_dev.argtypes = [sp, _vp, _vp,
+                                                C.POINTER(C.c_int), _dp]
+    L.solve_mob_3fts_0.argtypes = [sp, _vp] + [_dp] * 6
+    L.solve_mob_3fts_0.restype = None
+    L.stokes_set_iter.argtypes = [sp, C.c_char_p, C.c_int, C.c_int,
+                                  C.c_double, C.c_int, _vp]
     L.stokes_hip_adjoint_dev.argtypes = [sp, _vp, _vp, _vp, _vp]
     L.stokes_hip_scalars_real_f_deriv.argtypes = [C.c_double, C.c_double,
                                                   _dp]
@@ -337,6 +344,22 @@ class Stokes:
             self.sys, g_dev.ptr, f_dev.ptr if f_dev is not None else None,
             dx_dev.ptr if dx_dev is not None else None,
             df_dev.ptr if df_dev is not None else None), "adjoint")
+
+    def solve_mob_gmres(self, fte):
+        """FTS mobility problem by device GMRES: fte per particle (F T E)
+        -> (U O S); returns (x, iterations, relative residual)"""
+        bd = DeviceArray.from_host(np.ascontiguousarray(fte, np.float64))
+        xd = DeviceArray(bd.n)
+        it, rr = C.c_int(), C.c_double()
+        check(self.L.stokes_hip_solve_mob_3fts_dev(self.sys, bd.ptr, xd.ptr,
+                                                   C.byref(it),
+                                                   C.byref(rr)),
+              "solve_mob_3fts")
+        return xd.to_host(), it.value, rr.value
+
+    def set_iter(self, solver="gmres", max_it=2000, restart=30, eps=1e-12):
+        self.L.stokes_set_iter(self.sys, solver.encode(), max_it, restart,
+                               eps, 0, None)
 
     def set_timing(self, on=True):
         check(self.L.stokes_hip_set_timing(self.sys, 1 if on else 0),

@@ -1,0 +1,134 @@
+// linalg.hip -- small dense helpers for the device GMRES (Arnoldi basis
+// products) and the FTS mobility-problem operator pieces
+// (ewald-3fts.c:140-253).
+#include <hip/hip_runtime.h>
+#include "../shk.h"
+
+namespace {
+
+// out[c] = sum_k V[c n + k] w[k] for c < m: one block per column, fixed
+// tree order (deterministic)
+__global__ __launch_bounds__ (256) void
+k_vtw (const double *__restrict__ V, const double *__restrict__ w, size_t n,
+       double *__restrict__ out)
+{
+  __shared__ double red[256];
+  const double *v = V + (size_t) blockIdx.x * n;
+  double acc = 0.0;
+  for (size_t k = threadIdx.x; k < n; k += 256) acc += v[k] * w[k];
+  red[threadIdx.x] = acc;
+  __syncthreads ();
+  for (int s = 128; s > 0; s >>= 1)
+    {
+      if ((int) threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+      __syncthreads ();
+    }
+  if (threadIdx.x == 0) out[blockIdx.x] = red[0];
+}
+
+// w[k] += sign * sum_c V[c n + k] h[c]
+__global__ void k_vh (const double *__restrict__ V, const double *__restrict__ h, int m,
+                      size_t n, double sign, double *__restrict__ w)
+{
+  const size_t k = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  double acc = 0.0;
+  for (int c = 0; c < m; c++) acc += V[(size_t) c * n + k] * h[c];
+  w[k] += sign * acc;
+}
+
+// y = a x + b y
+__global__ void k_axpby (double *__restrict__ y, const double *__restrict__ x, double a,
+                         double b, size_t n)
+{
+  const size_t k = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) y[k] = a * x[k] + b * y[k];
+}
+
+// per particle of 11: lo = (x_0..5, 0), hi = (0, x_6..10); either may be NULL
+__global__ void k_fts_split (const double *__restrict__ x, int np, double *__restrict__ lo,
+                             double *__restrict__ hi)
+{
+  const size_t t = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (size_t) np * 11) return;
+  const int q = (int) (t % 11);
+  const double v = x[t];
+  if (lo) lo[t] = q < 6 ? v : 0.0;
+  if (hi) hi[t] = q < 6 ? 0.0 : v;
+}
+
+// y = lo(a) - z (mode 0) or y = z - hi(a) (mode 1), per particle of 11
+__global__ void k_fts_combine (const double *__restrict__ a, const double *__restrict__ z,
+                               int np, int mode, double *__restrict__ y)
+{
+  const size_t t = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (size_t) np * 11) return;
+  const int q = (int) (t % 11);
+  if (mode == 0) y[t] = (q < 6 ? a[t] : 0.0) - z[t];
+  else y[t] = z[t] - (q < 6 ? 0.0 : a[t]);
+}
+
+}  // namespace
+
+#define CHK(call)                                                           \
+  do {                                                                      \
+    hipError_t e_ = (call);                                                 \
+    if (e_ != hipSuccess)                                                   \
+      {                                                                     \
+        shk_set_error ("%s: %s", #call, hipGetErrorString (e_));            \
+        return SHK_EHIP;                                                    \
+      }                                                                     \
+  } while (0)
+
+extern "C" int shk_dev_axpby (shk_dev *d, double *y, const double *x, double a, double b,
+                              size_t len)
+{
+  if (len == 0) return SHK_OK;
+  hipStream_t s = (hipStream_t) shk_dev_stream (d);
+  hipLaunchKernelGGL (k_axpby, dim3 ((unsigned) ((len + 255) / 256)), dim3 (256), 0, s, y, x, a,
+                      b, len);
+  CHK (hipGetLastError ());
+  return SHK_OK;
+}
+
+extern "C" int shk_dev_vtw (shk_dev *d, const double *V, int m, size_t n, const double *w,
+                            double *out_dev)
+{
+  if (m <= 0) return SHK_OK;
+  hipStream_t s = (hipStream_t) shk_dev_stream (d);
+  hipLaunchKernelGGL (k_vtw, dim3 (m), dim3 (256), 0, s, V, w, n, out_dev);
+  CHK (hipGetLastError ());
+  return SHK_OK;
+}
+
+extern "C" int shk_dev_vh (shk_dev *d, const double *V, const double *h_dev, int m, size_t n,
+                           double sign, double *w)
+{
+  if (m <= 0) return SHK_OK;
+  hipStream_t s = (hipStream_t) shk_dev_stream (d);
+  hipLaunchKernelGGL (k_vh, dim3 ((unsigned) ((n + 255) / 256)), dim3 (256), 0, s, V, h_dev, m,
+                      n, sign, w);
+  CHK (hipGetLastError ());
+  return SHK_OK;
+}
+
+extern "C" int shk_dev_fts_split (shk_dev *d, const double *x, int np, double *lo, double *hi)
+{
+  hipStream_t s = (hipStream_t) shk_dev_stream (d);
+  const size_t n = (size_t) np * 11;
+  hipLaunchKernelGGL (k_fts_split, dim3 ((unsigned) ((n + 255) / 256)), dim3 (256), 0, s, x, np,
+                      lo, hi);
+  CHK (hipGetLastError ());
+  return SHK_OK;
+}
+
+extern "C" int shk_dev_fts_combine (shk_dev *d, const double *a, const double *z, int np,
+                                    int mode, double *y)
+{
+  hipStream_t s = (hipStream_t) shk_dev_stream (d);
+  const size_t n = (size_t) np * 11;
+  hipLaunchKernelGGL (k_fts_combine, dim3 ((unsigned) ((n + 255) / 256)), dim3 (256), 0, s, a,
+                      z, np, mode, y);
+  CHK (hipGetLastError ());
+  return SHK_OK;
+}

@@ -62,6 +62,7 @@ stokes_free (struct stokes *s)
 {
   if (s == NULL) return;
   stokes_hip_release_context (s);
+  free (s->it);
   free (s->pos);
   free (s->a);
   table_free (s);
@@ -300,9 +301,15 @@ void
 stokes_set_iter (struct stokes *s, const char *solver, int max, int restart,
                  double eps, int debug, FILE *out)
 {
-  (void) solver; (void) max; (void) restart; (void) eps; (void) debug;
-  (void) out;
-  s->it = NULL;
+  free (s->it);
+  s->it = calloc (1, sizeof (struct iter));
+  if (s->it == NULL) return;
+  if (solver) snprintf (s->it->solver, sizeof (s->it->solver), "%s", solver);
+  s->it->max = max;
+  s->it->restart = restart;
+  s->it->eps = eps;
+  s->it->debug = debug;
+  s->it->out = out;
 }
 
 void

@@ -3,8 +3,21 @@
 #ifndef LIBSTOKES_HIP_INTERNAL_H
 #define LIBSTOKES_HIP_INTERNAL_H
 
+#include <stdio.h>
 #include "stokes.h"
 #include "../shk.h"
+
+/* iterative-solver parameters kept by stokes_set_iter (the fields libiter's
+ * struct iter carries: solver name, maximum iterations, restart, tolerance,
+ * debug flag and stream); the GMRES of iterative.c reads max, restart and
+ * eps */
+struct iter {
+  char solver[32];
+  int max, restart;
+  double eps;
+  int debug;
+  FILE *out;
+};
 
 /* per-struct-stokes device context */
 typedef struct stokes_ctx {

@@ -134,6 +134,23 @@ void shk_set_error (const char *fmt, ...);
 int shk_dev_adjoint_x (shk_dev *d, const double *f, const double *g,
                        double *dx, int prec);
 
+/* y = a x + b y (len doubles) */
+int shk_dev_axpby (shk_dev *d, double *y, const double *x, double a, double b,
+                   size_t len);
+/* Arnoldi helpers (linalg.hip): out_dev[c] = V[:, c] . w (c < m, V column-
+ * major n x m); w += sign V h (h_dev[m]) */
+int shk_dev_vtw (shk_dev *d, const double *V, int m, size_t n, const double *w,
+                 double *out_dev);
+int shk_dev_vh (shk_dev *d, const double *V, const double *h_dev, int m,
+                size_t n, double sign, double *w);
+/* FTS mobility-problem pieces per particle of 11 (ewald-3fts.c:140-253):
+ * lo = (x_0..5, 0), hi = (0, x_6..10) (either may be NULL);
+ * y = lo(a) - z (mode 0) or z - hi(a) (mode 1) */
+int shk_dev_fts_split (shk_dev *d, const double *x, int np, double *lo,
+                       double *hi);
+int shk_dev_fts_combine (shk_dev *d, const double *a, const double *z, int np,
+                         int mode, double *y);
+
 /* Dense FP64 solve with known forces on the mobile particles' force /
  * torque components and known velocities elsewhere (solve.hip); mat is the
  * n x n operator matrix (extracted form for FTS, transformed in place), b
