@@ -48,10 +48,11 @@ template <int V, typename T> struct BlockLds {
   T blk[16][D * D];
 };
 
-// JW = source particles per wave (multiple of 64)
+// JW = source particles per wave (multiple of 64); XT rows of zs values
+// (particle j at XT + j zs, component q of column b at q B + b)
 template <int V, typename T, int JW>
 __global__ __launch_bounds__ (256) void
-k_real_block (const double4 *__restrict__ pos, const T *__restrict__ XT, int B,
+k_real_block (const double4 *__restrict__ pos, const T *__restrict__ XT, int B, int zs,
               const typename Vec4<T>::type *__restrict__ cand,
               const int4 *__restrict__ candm, const unsigned *__restrict__ tbits,
               RealParams p, T *__restrict__ partial)
@@ -146,7 +147,7 @@ k_real_block (const double4 *__restrict__ pos, const T *__restrict__ XT, int B,
                   const int jj = j0 + g * 16 + t;
                   T z[D];
 #pragma unroll
-                  for (int q = 0; q < D; q++) z[q] = XT[((size_t) jj * D + q) * B + b];
+                  for (int q = 0; q < D; q++) z[q] = XT[(size_t) jj * zs + (size_t) q * B + b];
 #pragma unroll
                   for (int r = 0; r < D; r++)
                     {

@@ -1,0 +1,545 @@
+// mfma_kernels.h -- reciprocal-space Ewald sum (ewald.c:923-1075 recast) for
+// a block of B right-hand sides as FP32 matrix-core contractions
+// (v_mfma_f32_16x16x4_f32, FP32 products accumulated in FP32).
+//
+// With the phases e^{i k.x_j} of the half space as a complex nk x np matrix
+// E and the (S-transformed) forces as an np x C matrix Z (C = D B columns,
+// twice that with the a_j^2-weighted copy for unequal spheres):
+//
+//   k_sf_mfma    Cf = Re(E) Z, Sf = Im(E) Z              (nk x C, twice)
+//   k_blk_mfma   per (k, b): P, Q from Cf, Sf by the FP64 k-blocks
+//   k_back_mfma  y = sum_p a^p [Re(E)^T P_p + Im(E)^T Q_p]  (np x D B)
+//
+// and the real-space pair sum as Y = A Z with A the (D np) x (D np) matrix
+// of the image-summed pair blocks, built on the fly 16 x 4 particles at a
+// time (k_real_mfma).
+//
+// The phase matrix never goes to memory: the lanes that feed an A operand
+// build their entry from the per-particle tables e^{i m th} (two complex
+// products, as in k_recip_sf_tab).  Operand layouts of the 16x16x4 FP32
+// instruction: A[i][k] in lane i + 16 k, B[k][j] in lane j + 16 k, and
+// D[i][j] in element i % 4 of lane j + 16 (i / 4).
+//
+// A workgroup is four waves on one 64 x 64 output tile; each wave takes
+// every fourth quad of the reduction index, and the four partial tiles are
+// added in LDS in wave order (deterministic).
+#pragma once
+#include <hip/hip_runtime.h>
+#include "pair_math.h"
+#include "ewald_kernels.h"
+#include "recip_kernels.h"
+#include "real_kernels.h"
+
+namespace shk {
+
+typedef float mf4 __attribute__ ((ext_vector_type (4)));
+
+__device__ __forceinline__ mf4 mfma4 (float a, float b, mf4 c)
+{
+  return __builtin_amdgcn_mfma_f32_16x16x4f32 (a, b, c, 0, 0, 0);
+}
+
+__device__ __forceinline__ int4 phase_offsets (const PhaseBasis &b, const int4 m)
+{
+  return make_int4 (m.x, b.K1 + 1 + b.K2 + m.y, b.K1 + 2 * b.K2 + 2 + b.K3 + m.z, 0);
+}
+
+// column-major x (n x B) -> Zt[j][h][q][b] (h = 1: a_j^2 z, unequal spheres)
+template <int V, int NH>
+__global__ void k_block_prep_f (const double *__restrict__ x, const double4 *__restrict__ pos,
+                                int np, int B, float *__restrict__ Zt)
+{
+  constexpr int D = Dim<V>::D;
+  const size_t n = (size_t) np * D;
+  const size_t t = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (size_t) np * B) return;
+  const int b = (int) (t % B);
+  const int j = (int) (t / B);
+  float z[D];
+#pragma unroll
+  for (int q = 0; q < D; q++) z[q] = float (x[(size_t) b * n + (size_t) j * D + q]);
+  if (V == 2) s_transform (z);
+  float *row = Zt + (size_t) j * NH * D * B + b;
+#pragma unroll
+  for (int q = 0; q < D; q++) row[(size_t) q * B] = z[q];
+  if (NH > 1)
+    {
+      const double a = pos[j].w;
+      const float a2 = float (a * a);
+#pragma unroll
+      for (int q = 0; q < D; q++) row[(size_t) (D + q) * B] = a2 * z[q];
+    }
+}
+
+// 4 waves x 4 x 4 tiles; 64 KB of LDS for the end-of-block reduction
+constexpr int MF_RED_BYTES = 4 * 64 * 64 * 4;
+
+// copy n Cplx from global to LDS with eight loads in flight per thread
+__device__ __forceinline__ void stage_cplx (Cplx<float> *__restrict__ dst,
+                                            const Cplx<float> *__restrict__ src, int n)
+{
+  for (int e0 = threadIdx.x; e0 < n; e0 += 256 * 8)
+    {
+      Cplx<float> v[8];
+#pragma unroll
+      for (int u = 0; u < 8; u++)
+        {
+          const int e = e0 + 256 * u;
+          if (e < n) v[u] = src[e];
+        }
+#pragma unroll
+      for (int u = 0; u < 8; u++)
+        {
+          const int e = e0 + 256 * u;
+          if (e < n) dst[e] = v[u];
+        }
+    }
+}
+
+// structure factors: part[((sj nk + k) 2 + s) cz + c], s = 0 cos, 1 sin;
+// Zt rows of cz columns.  grid (ceil(cz/64), ceil(nk/64), splits of j) --
+// the column blocks of one k block run side by side and share the tables
+// through L2; JT particle tables per LDS tile; the B operands (z) come
+// straight from memory, one quad ahead.
+__global__ __launch_bounds__ (256, 2) void
+k_sf_mfma (const float *__restrict__ Zt, int cz, const int4 *__restrict__ km,
+           const Cplx<float> *__restrict__ tab, PhaseBasis b, int np, int nk, int jper, int JT,
+           float *__restrict__ part)
+{
+  extern __shared__ __align__ (16) unsigned char smem_mf[];
+  Cplx<float> *stab = (Cplx<float> *) smem_mf;            // [JT][W]
+  float *red = (float *) smem_mf;                         // [4][64][64], after the loop
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int ir = lane & 15, kq = lane >> 4;
+  const int cbase = blockIdx.x * 64, kbase = blockIdx.y * 64;
+  const int j0 = blockIdx.z * jper, j1 = min (np, j0 + jper);
+  int4 o[4];
+#pragma unroll
+  for (int kt = 0; kt < 4; kt++) o[kt] = phase_offsets (b, km[min (kbase + 16 * kt + ir, nk - 1)]);
+  bool cok[4];
+#pragma unroll
+  for (int ct = 0; ct < 4; ct++) cok[ct] = cbase + 16 * ct + ir < cz;
+  const float *zc = Zt + cbase + ir;
+  mf4 accc[4][4], accs[4][4];
+#pragma unroll
+  for (int kt = 0; kt < 4; kt++)
+#pragma unroll
+    for (int ct = 0; ct < 4; ct++)
+      {
+        accc[kt][ct] = mf4{0.f, 0.f, 0.f, 0.f};
+        accs[kt][ct] = mf4{0.f, 0.f, 0.f, 0.f};
+      }
+  for (int jb = j0; jb < j1; jb += JT)
+    {
+      const int n = min (JT, j1 - jb);
+      stage_cplx (stab, tab + (size_t) jb * b.W, n * b.W);
+      __syncthreads ();
+      const int nq = (n + 3) / 4;
+      float bz[4];
+      {
+        const int j = jb + 4 * wave + kq;
+#pragma unroll
+        for (int ct = 0; ct < 4; ct++)
+          bz[ct] = (wave < nq && j < j1 && cok[ct]) ? zc[(size_t) j * cz + 16 * ct] : 0.f;
+      }
+      for (int qd = wave; qd < nq; qd += 4)
+        {
+          const int jt = 4 * qd + kq;
+          const bool ok = jt < n;
+          float bn[4];
+          {
+            const int j = jb + jt + 16;
+            const bool nok = qd + 4 < nq && j < j1;
+#pragma unroll
+            for (int ct = 0; ct < 4; ct++)
+              bn[ct] = (nok && cok[ct]) ? zc[(size_t) j * cz + 16 * ct] : 0.f;
+          }
+          const Cplx<float> *row = stab + (size_t) (ok ? jt : 0) * b.W;
+          float ac[4], as[4];
+#pragma unroll
+          for (int kt = 0; kt < 4; kt++)
+            {
+              const Cplx<float> e = cmul (cmul (row[o[kt].x], row[o[kt].y]), row[o[kt].z]);
+              ac[kt] = ok ? e.re : 0.f;
+              as[kt] = ok ? e.im : 0.f;
+            }
+#pragma unroll
+          for (int kt = 0; kt < 4; kt++)
+#pragma unroll
+            for (int ct = 0; ct < 4; ct++)
+              {
+                accc[kt][ct] = mfma4 (ac[kt], bz[ct], accc[kt][ct]);
+                accs[kt][ct] = mfma4 (as[kt], bz[ct], accs[kt][ct]);
+              }
+#pragma unroll
+          for (int ct = 0; ct < 4; ct++) bz[ct] = bn[ct];
+        }
+      __syncthreads ();
+    }
+#pragma unroll
+  for (int s = 0; s < 2; s++)
+    {
+#pragma unroll
+      for (int kt = 0; kt < 4; kt++)
+#pragma unroll
+        for (int ct = 0; ct < 4; ct++)
+#pragma unroll
+          for (int r = 0; r < 4; r++)
+            red[(wave * 64 + 16 * kt + 4 * kq + r) * 64 + 16 * ct + ir]
+              = s ? accs[kt][ct][r] : accc[kt][ct][r];
+      __syncthreads ();
+      for (int e = threadIdx.x; e < 64 * 64; e += 256)
+        {
+          const int k = kbase + (e >> 6), c = cbase + (e & 63);
+          const float v = red[e] + red[4096 + e] + red[8192 + e] + red[12288 + e];
+          if (k < nk && c < cz) part[(((size_t) blockIdx.z * nk + k) * 2 + s) * cz + c] = v;
+        }
+      __syncthreads ();
+    }
+}
+
+// P, Q per (k, b) from the split structure factors (FP64 k-blocks, as
+// k_recip_blocks); PQ[((k NPOW + p) 2 + s) C + q B + b], C = D B
+template <int V, int NPOW>
+__global__ __launch_bounds__ (128) void
+k_blk_mfma (const double4 *__restrict__ kv, const double *__restrict__ ks, int nk, int B,
+            int nsplit, const float *__restrict__ part, float *__restrict__ PQ)
+{
+  constexpr int D = Dim<V>::D, NM = Dim<V>::NMOM;
+  constexpr int NS = (NPOW > 1) ? 4 : 2;
+  constexpr int NH = NS / 2;
+  const size_t t = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (size_t) nk * B) return;
+  const int k = (int) (t / B), b = (int) (t % B);
+  const size_t cz = (size_t) NH * D * B, C = (size_t) D * B;
+  // CS: [cos z | sin z | cos a^2 z | sin a^2 z], D each
+  double CS[NS * D];
+#pragma unroll
+  for (int q = 0; q < NS * D; q++) CS[q] = 0.0;
+  for (int sp = 0; sp < nsplit; sp++)
+#pragma unroll
+    for (int s = 0; s < 2; s++)
+      {
+        const float *src = part + (((size_t) sp * nk + k) * 2 + s) * cz + b;
+#pragma unroll
+        for (int h = 0; h < NH; h++)
+#pragma unroll
+          for (int q = 0; q < D; q++)
+            CS[(2 * h + s) * D + q] += double (src[(size_t) (h * D + q) * B]);
+      }
+  const double4 kk = kv[k];
+  const double *w = ks + (size_t) k * 8;
+  double Mc[NM], Ms[NM];
+  if (NPOW == 1)
+    {
+      double P[D], Q[D];
+#pragma unroll
+      for (int q = 0; q < D; q++) P[q] = Q[q] = 0.0;
+      kblock_moments<V, double> (kk, w, Mc, Ms);
+      kblock_apply<V> (Mc, Ms, CS, CS + D, P, Q);
+      float *out = PQ + (size_t) k * 2 * C + b;
+#pragma unroll
+      for (int q = 0; q < D; q++)
+        {
+          out[(size_t) q * B] = float (P[q]);
+          out[C + (size_t) q * B] = float (Q[q]);
+        }
+      return;
+    }
+  const double ik = 1.0 / kk.w;
+  const double ex = kk.x * ik, ey = kk.y * ik, ez = kk.z * ik;
+  for (int pw = 1; pw <= NPOW; pw++)
+    {
+      double sc1[11], ss1[11], sc2[11], ss2[11];
+      poly_kscalars (pw, kk.w, w[6], w[7], w[3], sc1, ss1, sc2, ss2);
+      double P[D], Q[D];
+#pragma unroll
+      for (int q = 0; q < D; q++) P[q] = Q[q] = 0.0;
+#pragma unroll
+      for (int u = 0; u < NM; u++) Mc[u] = Ms[u] = 0.0;
+      moments_add<V, double> (Mc, ex, ey, ez, sc1);
+      moments_add<V, double> (Ms, ex, ey, ez, ss1);
+      kblock_apply<V> (Mc, Ms, CS, CS + D, P, Q);
+#pragma unroll
+      for (int u = 0; u < NM; u++) Mc[u] = Ms[u] = 0.0;
+      moments_add<V, double> (Mc, ex, ey, ez, sc2);
+      moments_add<V, double> (Ms, ex, ey, ez, ss2);
+      kblock_apply<V> (Mc, Ms, CS + 2 * D, CS + 3 * D, P, Q);
+      float *out = PQ + ((size_t) k * NPOW + (pw - 1)) * 2 * C + b;
+#pragma unroll
+      for (int q = 0; q < D; q++)
+        {
+          out[(size_t) q * B] = float (P[q]);
+          out[C + (size_t) q * B] = float (Q[q]);
+        }
+    }
+}
+
+// back-projection: part[(sk np + i) C + c] = sum_k sum_p a_i^p [cos P_p +
+// sin Q_p] (a_i^p = 1 for equal spheres, p = 1..NPOW otherwise).  grid
+// (ceil(C/64), ceil(np/64), splits of k).  The block's 64 particle tables
+// sit in LDS for the whole k range; k offsets and the B operands (P, Q)
+// come straight from memory, one quad ahead.
+template <int NPOW>
+__global__ __launch_bounds__ (256, 2) void
+k_back_mfma (const double4 *__restrict__ pos, const int4 *__restrict__ km,
+             const Cplx<float> *__restrict__ tab, PhaseBasis b, const float *__restrict__ PQ,
+             int C, int np, int nk, int kper, float *__restrict__ part)
+{
+  extern __shared__ __align__ (16) unsigned char smem_mf[];
+  Cplx<float> *stab = (Cplx<float> *) smem_mf;            // [64][W]
+  float *red = (float *) smem_mf;                         // [4][64][64], after the loop
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int ir = lane & 15, kq = lane >> 4;
+  const int cbase = blockIdx.x * 64, ibase = blockIdx.y * 64;
+  const int k0 = blockIdx.z * kper, k1 = min (nk, k0 + kper);
+  {
+    const int ni = min (64, np - ibase);
+    stage_cplx (stab, tab + (size_t) ibase * b.W, ni * b.W);
+    for (int e = ni * b.W + threadIdx.x; e < 64 * b.W; e += 256) stab[e] = Cplx<float>{0.f, 0.f};
+  }
+  float apow[4][NPOW];
+#pragma unroll
+  for (int it = 0; it < 4; it++)
+    {
+      const float a = NPOW > 1 ? float (pos[min (ibase + 16 * it + ir, np - 1)].w) : 1.f;
+      apow[it][0] = a;
+#pragma unroll
+      for (int p = 1; p < NPOW; p++) apow[it][p] = apow[it][p - 1] * a;
+    }
+  bool cok[4];
+#pragma unroll
+  for (int ct = 0; ct < 4; ct++) cok[ct] = cbase + 16 * ct + ir < C;
+  const float *pc = PQ + cbase + ir;
+  mf4 acc[4][4];
+#pragma unroll
+  for (int it = 0; it < 4; it++)
+#pragma unroll
+    for (int ct = 0; ct < 4; ct++) acc[it][ct] = mf4{0.f, 0.f, 0.f, 0.f};
+  __syncthreads ();
+  const int nq = (k1 - k0 + 3) / 4;
+  // operands of quad qd: offsets of k = k0 + 4 qd + kq and its P, Q rows
+  int4 o;
+  float bpq[NPOW][2][4];
+  auto load = [&] (int qd, int4 &oo, float (&v)[NPOW][2][4]) {
+    const int k = k0 + 4 * qd + kq;
+    const bool ok = qd < nq && k < k1;
+    oo = phase_offsets (b, km[ok ? k : k0]);
+#pragma unroll
+    for (int p = 0; p < NPOW; p++)
+#pragma unroll
+      for (int s = 0; s < 2; s++)
+#pragma unroll
+        for (int ct = 0; ct < 4; ct++)
+          v[p][s][ct] = (ok && cok[ct]) ? pc[((size_t) k * NPOW * 2 + 2 * p + s) * C + 16 * ct]
+                                        : 0.f;
+  };
+  load (wave, o, bpq);
+  for (int qd = wave; qd < nq; qd += 4)
+    {
+      int4 on;
+      float bn[NPOW][2][4];
+      load (qd + 4, on, bn);
+      float ac[4], as[4];
+#pragma unroll
+      for (int it = 0; it < 4; it++)
+        {
+          const Cplx<float> *row = stab + (size_t) (16 * it + ir) * b.W;
+          const Cplx<float> e = cmul (cmul (row[o.x], row[o.y]), row[o.z]);
+          ac[it] = e.re;
+          as[it] = e.im;
+        }
+#pragma unroll
+      for (int p = 0; p < NPOW; p++)
+#pragma unroll
+        for (int it = 0; it < 4; it++)
+          {
+            const float cc = NPOW > 1 ? apow[it][p] * ac[it] : ac[it];
+            const float ss = NPOW > 1 ? apow[it][p] * as[it] : as[it];
+#pragma unroll
+            for (int ct = 0; ct < 4; ct++)
+              {
+                acc[it][ct] = mfma4 (cc, bpq[p][0][ct], acc[it][ct]);
+                acc[it][ct] = mfma4 (ss, bpq[p][1][ct], acc[it][ct]);
+              }
+          }
+      o = on;
+#pragma unroll
+      for (int p = 0; p < NPOW; p++)
+#pragma unroll
+        for (int s = 0; s < 2; s++)
+#pragma unroll
+          for (int ct = 0; ct < 4; ct++) bpq[p][s][ct] = bn[p][s][ct];
+    }
+  __syncthreads ();
+#pragma unroll
+  for (int it = 0; it < 4; it++)
+#pragma unroll
+    for (int ct = 0; ct < 4; ct++)
+#pragma unroll
+      for (int r = 0; r < 4; r++)
+        red[(wave * 64 + 16 * it + 4 * kq + r) * 64 + 16 * ct + ir] = acc[it][ct][r];
+  __syncthreads ();
+  for (int e = threadIdx.x; e < 64 * 64; e += 256)
+    {
+      const int i = ibase + (e >> 6), c = cbase + (e & 63);
+      const float v = red[e] + red[4096 + e] + red[8192 + e] + red[12288 + e];
+      if (i < np && c < C) part[((size_t) blockIdx.z * np + i) * C + c] = v;
+    }
+}
+
+// real-space pair sum for a block of columns.  One wave per (16 target
+// particles i, range of source particles j, 16 CT columns).  Per chunk of 4
+// source particles: phase A, lane = pair (i_l = lane % 16, j_l = lane / 16)
+// sums the moments of its pair over the images inside r_max -- each lane
+// first marks its images among the next 64 candidates in a bit mask, then
+// evaluates its own marked images, so a lane idles only while another
+// still has images left -- and writes the D x D block to LDS; phase B
+// multiplies the (16 D) x (4 D) block matrix of the chunk into the
+// (4 D) x 16 CT slice of z on the matrix cores.  No padding: 16 D rows are
+// D row tiles of 16 and 4 D columns are D quads.
+// partial[((split np + i) D + q) B + b]; XT rows of zs values.
+template <int V, int CT>
+__global__ __launch_bounds__ (64) void
+k_real_mfma (const double4 *__restrict__ pos, const float *__restrict__ XT, int B, int zs,
+             const float4 *__restrict__ cand, const int4 *__restrict__ candm,
+             const unsigned *__restrict__ tbits, RealParams p, int jper,
+             float *__restrict__ partial)
+{
+  constexpr int D = Dim<V>::D, NM = Dim<V>::NMOM, DD = D * D;
+  __shared__ float blk[64 * DD];
+  const int lane = threadIdx.x;
+  const int ir = lane & 15, kq = lane >> 4;
+  const int ibase = blockIdx.x * 16;
+  const int j0 = blockIdx.y * jper, j1 = min (p.np, j0 + jper);
+  const int cbase = blockIdx.z * 16 * CT;
+  // phase A pair of this lane
+  const int ia = ibase + ir;
+  const bool ivalid = ia < p.np;
+  const double4 pi = pos[ivalid ? ia : 0];
+  const float ai = float (pi.w), xi = float (p.xi), rmax2 = float (p.rmax2);
+  // phase B: LDS offsets of the A entries (row tile t, quad u)
+  int rowoff[D], koff[D];
+#pragma unroll
+  for (int t = 0; t < D; t++)
+    {
+      const int R = 16 * t + ir, il = R / D, r = R - il * D;
+      rowoff[t] = il * 4 * DD + r * D;
+    }
+#pragma unroll
+  for (int u = 0; u < D; u++)
+    {
+      const int K = 4 * u + kq, jl = K / D, q = K - jl * D;
+      koff[u] = jl * DD + q;
+    }
+  bool cok[CT];
+#pragma unroll
+  for (int ct = 0; ct < CT; ct++) cok[ct] = cbase + 16 * ct + ir < B;
+  mf4 acc[D][CT];
+#pragma unroll
+  for (int t = 0; t < D; t++)
+#pragma unroll
+    for (int ct = 0; ct < CT; ct++) acc[t][ct] = mf4{0.f, 0.f, 0.f, 0.f};
+
+  for (int jb = j0; jb < j1; jb += 4)
+    {
+      // ---- phase A
+      const int j = jb + kq;
+      const bool valid = ivalid && j < j1;
+      const double4 pj = pos[valid ? j : 0];
+      double dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
+      const double n3 = rint (dz / p.lz);
+      dx -= n3 * p.s3x; dz -= n3 * p.lz;
+      const double n2 = rint (dy / p.ly);
+      dx -= n2 * p.s2x; dy -= n2 * p.ly;
+      const double n1 = rint (dx / p.lx);
+      dx -= n1 * p.lx;
+      const int w1 = int (n1), w2 = int (n2), w3 = int (n3);
+      const float mdx = float (dx), mdy = float (dy), mdz = float (dz);
+      const float aj = float (pj.w);
+      const float rmn = (ai + aj) * float (p.rmin);
+      float M[NM];
+#pragma unroll
+      for (int t = 0; t < NM; t++) M[t] = 0.f;
+      for (int c0 = 0; c0 < p.ncand; c0 += 64)
+        {
+          const int cn = min (64, p.ncand - c0);
+          unsigned long long mask = 0ull;
+          for (int c = 0; c < cn; c++)
+            {
+              const float4 v = cand[c0 + c];
+              const float ex = mdx + v.x, ey = mdy + v.y, ez = mdz + v.z;
+              const float rr = ex * ex + ey * ey + ez * ez;
+              if (valid && rr != 0.f && rr <= rmax2)
+                {
+                  const int4 mm = candm[c0 + c];
+                  if (in_table (p, tbits, mm.x - w1, mm.y - w2, mm.z - w3)) mask |= 1ull << c;
+                }
+            }
+          while (mask)
+            {
+              const int c = __builtin_ctzll (mask);
+              mask &= mask - 1ull;
+              const float4 v = cand[c0 + c];
+              const float ex = mdx + v.x, ey = mdy + v.y, ez = mdz + v.z;
+              const float rr = ex * ex + ey * ey + ez * ez;
+              float ir2 = m_rsqrt (rr);
+              float r = rr * ir2;
+              if (r < rmn)
+                {
+                  r = rmn;
+                  ir2 = 1.f / rmn;
+                }
+              float sc[11];
+              if (p.poly) scalars_real_poly_u<V, float> (xi, r, ir2, ai, aj, sc);
+              else scalars_real_u<V, float> (xi, r, ir2, sc);
+              moments_add<V, float> (M, ex * ir2, ey * ir2, ez * ir2, sc);
+            }
+        }
+      {
+        float *dst = blk + (ir * 4 + kq) * DD;
+#pragma unroll
+        for (int r = 0; r < D; r++)
+#pragma unroll
+          for (int cq = 0; cq < D; cq++) dst[r * D + cq] = block_entry<V, float> (M, r, cq, false);
+      }
+      __syncthreads ();
+      // ---- phase B
+#pragma unroll
+      for (int u = 0; u < D; u++)
+        {
+          const int K = 4 * u + kq;
+          const int jz = jb + K / D, q = K % D;
+          float bz[CT];
+#pragma unroll
+          for (int ct = 0; ct < CT; ct++)
+            bz[ct] = (jz < j1 && cok[ct]) ? XT[(size_t) jz * zs + (size_t) q * B + cbase + 16 * ct + ir]
+                                          : 0.f;
+#pragma unroll
+          for (int t = 0; t < D; t++)
+            {
+              const float a = blk[rowoff[t] + koff[u]];
+#pragma unroll
+              for (int ct = 0; ct < CT; ct++) acc[t][ct] = mfma4 (a, bz[ct], acc[t][ct]);
+            }
+        }
+      __syncthreads ();
+    }
+  // rows R = 16 t + 4 (lane / 16) + e of the (16 D)-row tile
+#pragma unroll
+  for (int t = 0; t < D; t++)
+#pragma unroll
+    for (int e = 0; e < 4; e++)
+      {
+        const int R = 16 * t + 4 * kq + e, il = R / D, r = R - il * D;
+        const int i = ibase + il;
+        if (i < p.np)
+#pragma unroll
+          for (int ct = 0; ct < CT; ct++)
+            if (cok[ct])
+              partial[(((size_t) blockIdx.y * p.np + i) * D + r) * B + cbase + 16 * ct + ir]
+                = acc[t][ct][e];
+      }
+}
+
+}  // namespace shk

@@ -12,6 +12,7 @@
 #include "real_kernels.h"
 #include "recip_kernels.h"
 #include "block_kernels.h"
+#include "mfma_kernels.h"
 #include "adj_kernels.h"
 
 using namespace shk;
@@ -588,6 +589,89 @@ static int recip_block_launch (shk_dev *d, const T *XT, int B, int &nrecip,
   return SHK_OK;
 }
 
+// FP32 matrix-core reciprocal sums for a block (MIXED precision)
+template <int V, int NPOW>
+static int recip_block_launch_mfma (shk_dev *d, const float *Zt, int B, int &nrecip,
+                                    const void *&kpart)
+{
+  constexpr int D = Dim<V>::D;
+  constexpr int NH = NPOW > 1 ? 2 : 1;
+  const shk_params &p = d->prm;
+  const int nk = p.nk, np = p.np;
+  const int cz = NH * D * B, C = D * B;
+  const PhaseBasis &b = d->basis;
+  // structure factors: JT particle tables per LDS tile (the 64 KB
+  // reduction region bounds the tile), two blocks per CU
+  const size_t perj = sizeof (Cplx<float>) * b.W;
+  int JT = (int) ((size_t) MF_RED_BYTES / perj) & ~15;
+  if (JT > 128) JT = 128;
+  if (JT < 16) JT = 16;
+  size_t lds_sf = (size_t) JT * perj;
+  if (lds_sf < (size_t) MF_RED_BYTES) lds_sf = MF_RED_BYTES;
+  const int nkb = (nk + 63) / 64, ncz = (cz + 63) / 64;
+  int sj = (1024 + nkb * ncz - 1) / (nkb * ncz);
+  if (sj > (np + 63) / 64) sj = (np + 63) / 64;
+  if (sj < 1) sj = 1;
+  int jper = (np + sj - 1) / sj;
+  jper = (jper + 15) & ~15;
+  sj = (np + jper - 1) / jper;
+  // back-projection: the block's 64 particle tables
+  size_t lds_bk = sizeof (Cplx<float>) * 64 * (size_t) b.W;
+  if (lds_bk < (size_t) MF_RED_BYTES) lds_bk = MF_RED_BYTES;
+  if (lds_sf > 80 * 1024 || lds_bk > 80 * 1024)
+    {
+      set_err ("batched reciprocal sum: phase tables of %d entries exceed LDS", b.W);
+      return SHK_EARG;
+    }
+  const int nib = (np + 63) / 64, ncc = (C + 63) / 64;
+  int sk = (1024 + nib * ncc - 1) / (nib * ncc);
+  if (sk > (nk + 63) / 64) sk = (nk + 63) / 64;
+  if (sk < 1) sk = 1;
+  int kper = (nk + sk - 1) / sk;
+  kper = (kper + 3) & ~3;
+  sk = (nk + kper - 1) / kper;
+  int rc;
+  if ((rc = ensure (d, d->kpart, sizeof (float) * (size_t) sj * nk * 2 * cz))) return rc;
+  if ((rc = ensure (d, d->kpq, sizeof (float) * (size_t) nk * NPOW * 2 * C))) return rc;
+  if ((rc = ensure (d, d->kback, sizeof (float) * (size_t) sk * np * C))) return rc;
+  if ((rc = ensure (d, d->ptab, sizeof (Cplx<float>) * (size_t) np * b.W))) return rc;
+  const double4 *pos4 = (const double4 *) d->pos.p;
+  const double4 *kv = (const double4 *) d->kv.p;
+  const int4 *km = (const int4 *) d->km.p;
+  const Cplx<float> *tab = (const Cplx<float> *) d->ptab.p;
+  hipLaunchKernelGGL ((k_phase_tab<float>), dim3 ((unsigned) (((size_t) np * b.W + 255) / 256)),
+                      dim3 (256), 0, d->stream, pos4, np, b, (Cplx<float> *) d->ptab.p);
+  HIPCHK (hipGetLastError ());
+  static bool attr_sf = false, attr_bk = false;
+  if (!attr_sf)
+    {
+      HIPCHK (hipFuncSetAttribute ((const void *) k_sf_mfma,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, 80 * 1024));
+      attr_sf = true;
+    }
+  if (!attr_bk)
+    {
+      HIPCHK (hipFuncSetAttribute ((const void *) k_back_mfma<NPOW>,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, 80 * 1024));
+      attr_bk = true;
+    }
+  hipLaunchKernelGGL (k_sf_mfma, dim3 (ncz, nkb, sj), dim3 (256), lds_sf, d->stream, Zt, cz, km,
+                      tab, b, np, nk, jper, JT, (float *) d->kpart.p);
+  HIPCHK (hipGetLastError ());
+  const size_t nkB = (size_t) nk * B;
+  hipLaunchKernelGGL ((k_blk_mfma<V, NPOW>), dim3 ((unsigned) ((nkB + 127) / 128)), dim3 (128),
+                      0, d->stream, kv, (const double *) d->ks.p, nk, B, sj,
+                      (const float *) d->kpart.p, (float *) d->kpq.p);
+  HIPCHK (hipGetLastError ());
+  hipLaunchKernelGGL ((k_back_mfma<NPOW>), dim3 (ncc, nib, sk), dim3 (256), lds_bk, d->stream,
+                      pos4, km, tab, b, (const float *) d->kpq.p, C, np, nk, kper,
+                      (float *) d->kback.p);
+  HIPCHK (hipGetLastError ());
+  nrecip = sk;
+  kpart = d->kback.p;
+  return SHK_OK;
+}
+
 template <int V, typename T>
 static int atimes_block_vt (shk_dev *d, const double *x, double *y, int B)
 {
@@ -598,14 +682,37 @@ static int atimes_block_vt (shk_dev *d, const double *x, double *y, int B)
   const double4 *pos4 = (const double4 *) d->pos.p;
   int rc, nrecip = 0;
   const void *kpart = nullptr;
-  if ((rc = ensure (d, d->xs, sizeof (T) * (size_t) p.np * D * B))) return rc;
-  const int nj = (p.np + JW - 1) / JW;
+  constexpr bool MF = std::is_same<T, float>::value;
+  const int NH = MF && p.poly ? 2 : 1;
+  if ((rc = ensure (d, d->xs, sizeof (T) * (size_t) p.np * NH * D * B))) return rc;
+  // real-space splits: JW sources per wave (FP64), or for the matrix-core
+  // path (MIXED) one wave per 16 targets x jper sources x 16 CT columns,
+  // about 4096 waves
+  constexpr int CT = V == 2 ? 2 : (V == 1 ? 4 : 8);
+  const int nib = (p.np + 15) / 16, ncg = (B + 16 * CT - 1) / (16 * CT);
+  int nj = (p.np + JW - 1) / JW, jper = 0;
+  if (MF)
+    {
+      nj = (4096 + nib * ncg - 1) / (nib * ncg);
+      if (nj > (p.np + 15) / 16) nj = (p.np + 15) / 16;
+      if (nj < 1) nj = 1;
+      jper = (p.np + nj - 1) / nj;
+      jper = (jper + 3) & ~3;
+      nj = (p.np + jper - 1) / jper;
+    }
   if ((rc = ensure (d, d->partial, sizeof (T) * (size_t) nj * p.np * D * B))) return rc;
   T *XT = (T *) d->xs.p;
   const size_t nb = (size_t) p.np * B;
   if (d->timing) HIPCHK (hipEventRecord (d->ev[0], d->stream));
-  hipLaunchKernelGGL ((k_block_prep<V, T>), dim3 ((unsigned) ((nb + 255) / 256)), dim3 (256), 0,
-                      d->stream, x, p.np, B, XT);
+  if (!MF)
+    hipLaunchKernelGGL ((k_block_prep<V, T>), dim3 ((unsigned) ((nb + 255) / 256)), dim3 (256),
+                        0, d->stream, x, p.np, B, XT);
+  else if (NH == 2)
+    hipLaunchKernelGGL ((k_block_prep_f<V, 2>), dim3 ((unsigned) ((nb + 255) / 256)), dim3 (256),
+                        0, d->stream, x, pos4, p.np, B, (float *) XT);
+  else
+    hipLaunchKernelGGL ((k_block_prep_f<V, 1>), dim3 ((unsigned) ((nb + 255) / 256)), dim3 (256),
+                        0, d->stream, x, pos4, p.np, B, (float *) XT);
   HIPCHK (hipGetLastError ());
   RealParams rp;
   rp.np = p.np; rp.ncand = d->ncand; rp.ncore = d->ncore; rp.poly = p.poly;
@@ -614,16 +721,27 @@ static int atimes_block_vt (shk_dev *d, const double *x, double *y, int B)
   rp.tx = d->tx; rp.ty = d->ty; rp.tz = d->tz; rp.tbits_words = d->tbits_words;
   const long waves = (long) p.np * nj;
   const void *cand = std::is_same<T, double>::value ? d->cand.p : d->candf.p;
-  hipLaunchKernelGGL ((k_real_block<V, T, JW>), dim3 ((unsigned) ((waves + 3) / 4)), dim3 (256), 0,
-                      d->stream, pos4, (const T *) XT, B, (const typename Vec4<T>::type *) cand,
-                      (const int4 *) d->candm.p, (const unsigned *) d->tbits.p, rp,
-                      (T *) d->partial.p);
+  if (MF)
+    hipLaunchKernelGGL ((k_real_mfma<V, CT>), dim3 (nib, nj, ncg), dim3 (64), 0, d->stream, pos4,
+                        (const float *) XT, B, NH * D * B, (const float4 *) d->candf.p,
+                        (const int4 *) d->candm.p, (const unsigned *) d->tbits.p, rp, jper,
+                        (float *) d->partial.p);
+  else
+    hipLaunchKernelGGL ((k_real_block<V, T, JW>), dim3 ((unsigned) ((waves + 3) / 4)), dim3 (256),
+                        0, d->stream, pos4, (const T *) XT, B, NH * D * B,
+                        (const typename Vec4<T>::type *) cand,
+                        (const int4 *) d->candm.p, (const unsigned *) d->tbits.p, rp,
+                        (T *) d->partial.p);
   HIPCHK (hipGetLastError ());
   if (d->timing) HIPCHK (hipEventRecord (d->ev[1], d->stream));
   if (p.nk > 0)
     {
-      rc = p.poly ? recip_block_launch<V, T, 5> (d, XT, B, nrecip, kpart)
-                  : recip_block_launch<V, T, 1> (d, XT, B, nrecip, kpart);
+      if (MF)
+        rc = p.poly ? recip_block_launch_mfma<V, 5> (d, (const float *) XT, B, nrecip, kpart)
+                    : recip_block_launch_mfma<V, 1> (d, (const float *) XT, B, nrecip, kpart);
+      else
+        rc = p.poly ? recip_block_launch<V, T, 5> (d, XT, B, nrecip, kpart)
+                    : recip_block_launch<V, T, 1> (d, XT, B, nrecip, kpart);
       if (rc) return rc;
     }
   hipLaunchKernelGGL ((k_combine_block<V, T>), dim3 ((unsigned) ((nb + 127) / 128)), dim3 (128), 0,

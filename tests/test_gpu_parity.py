@@ -201,6 +201,42 @@ def test_block_rhs(version, poly, nrhs):
     s.close()
 
 
+@pytest.mark.parametrize("tr,B", [(3000.0, 32), (300.0, 7)])
+def test_block_mixed_full_size(tr, B):
+    """N = 1024, phi = 0.3 FTS: the MIXED block path (FP32 matrix-core
+    reciprocal sums) against the FP64 block path column by column, 1e-6;
+    bitwise repeatable"""
+    n = 1024
+    pos, L = workloads.configuration(n, 0.3, seed=1)
+    X = np.random.default_rng(3).standard_normal(B * 11 * n)
+    s = ls.Stokes(2, pos, box=(L,) * 3, tr=1.0)
+    s.set_split(tr)
+    xd = ls.DeviceArray.from_host(X)
+    yd = ls.DeviceArray(X.size)
+    s.atimes_dev(xd, yd, nrhs=B)
+    Y = yd.to_host().reshape(B, -1)
+    s.set_precision(ls.MIXED)
+    s.atimes_dev(xd, yd, nrhs=B)
+    Ym = yd.to_host().reshape(B, -1)
+    s.atimes_dev(xd, yd, nrhs=B)
+    Ym2 = yd.to_host().reshape(B, -1)
+    errs = [relerr(Ym[c], Y[c]) for c in range(B)]
+    _say(f"  N={n} tr={tr:g} B={B}: MIXED block vs FP64 block max "
+         f"{max(errs):.2e}")
+    assert max(errs) <= TOLMIX
+    assert np.array_equal(Ym, Ym2)
+    # one column against the single-vector FP64 path
+    assert relerr(s_single(s, X[:11 * n]), Y[0]) <= TOL64
+    xd.free()
+    yd.free()
+    s.close()
+
+
+def s_single(s, x):
+    s.set_precision(ls.FP64)
+    return s.atimes(x)
+
+
 def test_open_boundary_fp64(golden):
     data, cases = golden
     for meta in cases:
