@@ -388,53 +388,97 @@ k_back_mfma (const double4 *__restrict__ pos, const int4 *__restrict__ km,
     }
 }
 
+// entries (r, q), r < D, q0 <= q < q0 + NQ, of the D x D block of one pair
+// from its moments, stored at dst[r * LD + 4 (q - q0)]; a template
+// recursion keeps every moment index a compile-time constant, so the
+// moments stay in registers
+template <int V, int Q0, int NQ, int E> struct BlockStore {
+  static constexpr int D = Dim<V>::D;
+  template <int LD, typename T>
+  __device__ __forceinline__ static void run (const T *M, T *dst)
+  {
+    BlockStore<V, Q0, NQ, E - 1>::template run<LD, T> (M, dst);
+    constexpr int r = (E - 1) / NQ, u = (E - 1) % NQ;
+    dst[r * LD + 4 * u] = block_entry<V, T> (M, r, Q0 + u, false);
+  }
+};
+template <int V, int Q0, int NQ> struct BlockStore<V, Q0, NQ, 0> {
+  template <int LD, typename T> __device__ __forceinline__ static void run (const T *, T *) {}
+};
+
 // real-space pair sum for a block of columns.  One wave per (16 target
 // particles i, range of source particles j, 16 CT columns).  Per chunk of 4
-// source particles: phase A, lane = pair (i_l = lane % 16, j_l = lane / 16)
-// sums the moments of its pair over the images inside r_max -- each lane
-// first marks its images among the next 64 candidates in a bit mask, then
-// evaluates its own marked images, so a lane idles only while another
-// still has images left -- and writes the D x D block to LDS; phase B
-// multiplies the (16 D) x (4 D) block matrix of the chunk into the
-// (4 D) x 16 CT slice of z on the matrix cores.  No padding: 16 D rows are
-// D row tiles of 16 and 4 D columns are D quads.
+// source particles:
+//   phase A: lane = pair (i_l = lane % 16, j_l = lane / 16) sums the
+//     moments of its pair over the images inside r_max -- each lane first
+//     marks its images among the next 64 candidates in a bit mask, then
+//     evaluates its own marked images, so a lane idles only while another
+//     still has images left;
+//   phase B: Y (16 D x 16 CT) += A (16 D x 4 D) Z (4 D x 16 CT) on the
+//     matrix cores, with row R = D i_l + r and reduction index K = 4 q + j_l,
+//     so quad u of the reduction is component q = u and the lane's k index
+//     is its source particle.  The block matrix goes through LDS in rounds
+//     of QR components (every lane writes its entries of those columns),
+//     which keeps the LDS of a wave near 15 KB.
 // partial[((split np + i) D + q) B + b]; XT rows of zs values.
+constexpr int REAL_QR = 3;
+
+template <int V, int CT, int RND> struct RealRounds {
+  static constexpr int D = Dim<V>::D, Q0 = RND * REAL_QR;
+  static constexpr int NQ = (D - Q0) < REAL_QR ? (D - Q0) : REAL_QR;
+  static constexpr int LDA = 4 * REAL_QR + 1;
+  static constexpr int NCOL = 16 * CT;
+  __device__ __forceinline__ static void run (const float *M, float *blk, const float *zs_l,
+                                              int ir, int kq, mf4 (&acc)[Dim<V>::D][CT])
+  {
+    BlockStore<V, Q0, NQ, D * NQ>::template run<LDA, float> (M, blk + ir * D * LDA + kq);
+    __syncthreads ();
+#pragma unroll
+    for (int u = 0; u < NQ; u++)
+      {
+        float bz[CT];
+#pragma unroll
+        for (int ct = 0; ct < CT; ct++) bz[ct] = zs_l[(4 * (Q0 + u) + kq) * NCOL + 16 * ct + ir];
+#pragma unroll
+        for (int t = 0; t < D; t++)
+          {
+            const float a = blk[(16 * t + ir) * LDA + 4 * u + kq];
+#pragma unroll
+            for (int ct = 0; ct < CT; ct++) acc[t][ct] = mfma4 (a, bz[ct], acc[t][ct]);
+          }
+      }
+    __syncthreads ();
+    if constexpr (Q0 + NQ < D) RealRounds<V, CT, RND + 1>::run (M, blk, zs_l, ir, kq, acc);
+  }
+};
+
 template <int V, int CT>
-__global__ __launch_bounds__ (64) void
+__global__ __launch_bounds__ (64) __attribute__ ((amdgpu_waves_per_eu (2))) void
 k_real_mfma (const double4 *__restrict__ pos, const float *__restrict__ XT, int B, int zs,
              const float4 *__restrict__ cand, const int4 *__restrict__ candm,
              const unsigned *__restrict__ tbits, RealParams p, int jper,
              float *__restrict__ partial)
 {
-  constexpr int D = Dim<V>::D, NM = Dim<V>::NMOM, DD = D * D;
-  __shared__ float blk[64 * DD];
+  constexpr int D = Dim<V>::D, NM = Dim<V>::NMOM;
+  constexpr int LDA = 4 * REAL_QR + 1;
+  constexpr int NCOL = 16 * CT;                 // columns of the z slice
+  constexpr int ZE = 4 * D * NCOL;              // z slice entries
+  __shared__ float blk[16 * D * LDA];
+  __shared__ float zs_l[ZE];                    // [K = 4 q + j_l][c]
+  __shared__ unsigned bits_l[64];
   const int lane = threadIdx.x;
   const int ir = lane & 15, kq = lane >> 4;
   const int ibase = blockIdx.x * 16;
   const int j0 = blockIdx.y * jper, j1 = min (p.np, j0 + jper);
-  const int cbase = blockIdx.z * 16 * CT;
-  // phase A pair of this lane
+  const int cbase = blockIdx.z * NCOL;
+  const bool bits_in_lds = p.tbits_words <= 64;
+  if (bits_in_lds && lane < p.tbits_words) bits_l[lane] = tbits[lane];
+  __syncthreads ();
+  const unsigned *bits = bits_in_lds ? bits_l : tbits;
   const int ia = ibase + ir;
   const bool ivalid = ia < p.np;
   const double4 pi = pos[ivalid ? ia : 0];
   const float ai = float (pi.w), xi = float (p.xi), rmax2 = float (p.rmax2);
-  // phase B: LDS offsets of the A entries (row tile t, quad u)
-  int rowoff[D], koff[D];
-#pragma unroll
-  for (int t = 0; t < D; t++)
-    {
-      const int R = 16 * t + ir, il = R / D, r = R - il * D;
-      rowoff[t] = il * 4 * DD + r * D;
-    }
-#pragma unroll
-  for (int u = 0; u < D; u++)
-    {
-      const int K = 4 * u + kq, jl = K / D, q = K - jl * D;
-      koff[u] = jl * DD + q;
-    }
-  bool cok[CT];
-#pragma unroll
-  for (int ct = 0; ct < CT; ct++) cok[ct] = cbase + 16 * ct + ir < B;
   mf4 acc[D][CT];
 #pragma unroll
   for (int t = 0; t < D; t++)
@@ -443,6 +487,15 @@ k_real_mfma (const double4 *__restrict__ pos, const float *__restrict__ XT, int 
 
   for (int jb = j0; jb < j1; jb += 4)
     {
+      // z slice of the chunk (read after the first barrier of phase B)
+      for (int e = lane; e < ZE; e += 64)
+        {
+          const int K = e / NCOL, c = e - K * NCOL;
+          const int q = K >> 2, jz = jb + (K & 3), b = cbase + c;
+          const bool ok = jz < j1 && b < B;
+          const float v = XT[(size_t) (ok ? jz : 0) * zs + (ok ? q * B + b : 0)];
+          zs_l[e] = ok ? v : 0.f;
+        }
       // ---- phase A
       const int j = jb + kq;
       const bool valid = ivalid && j < j1;
@@ -465,16 +518,16 @@ k_real_mfma (const double4 *__restrict__ pos, const float *__restrict__ XT, int 
         {
           const int cn = min (64, p.ncand - c0);
           unsigned long long mask = 0ull;
+#pragma unroll 4
           for (int c = 0; c < cn; c++)
             {
               const float4 v = cand[c0 + c];
+              const int4 mm = candm[c0 + c];
               const float ex = mdx + v.x, ey = mdy + v.y, ez = mdz + v.z;
               const float rr = ex * ex + ey * ey + ez * ez;
-              if (valid && rr != 0.f && rr <= rmax2)
-                {
-                  const int4 mm = candm[c0 + c];
-                  if (in_table (p, tbits, mm.x - w1, mm.y - w2, mm.z - w3)) mask |= 1ull << c;
-                }
+              const bool hit = valid && rr != 0.f && rr <= rmax2
+                && in_table (p, bits, mm.x - w1, mm.y - w2, mm.z - w3);
+              mask |= (unsigned long long) hit << c;
             }
           while (mask)
             {
@@ -483,47 +536,21 @@ k_real_mfma (const double4 *__restrict__ pos, const float *__restrict__ XT, int 
               const float4 v = cand[c0 + c];
               const float ex = mdx + v.x, ey = mdy + v.y, ez = mdz + v.z;
               const float rr = ex * ex + ey * ey + ez * ez;
-              float ir2 = m_rsqrt (rr);
-              float r = rr * ir2;
+              float iu = m_rsqrt (rr);
+              float r = rr * iu;
               if (r < rmn)
                 {
                   r = rmn;
-                  ir2 = 1.f / rmn;
+                  iu = 1.f / rmn;
                 }
               float sc[11];
-              if (p.poly) scalars_real_poly_u<V, float> (xi, r, ir2, ai, aj, sc);
-              else scalars_real_u<V, float> (xi, r, ir2, sc);
-              moments_add<V, float> (M, ex * ir2, ey * ir2, ez * ir2, sc);
+              if (p.poly) scalars_real_poly_u<V, float> (xi, r, iu, ai, aj, sc);
+              else scalars_real_u<V, float> (xi, r, iu, sc);
+              moments_add<V, float> (M, ex * iu, ey * iu, ez * iu, sc);
             }
         }
-      {
-        float *dst = blk + (ir * 4 + kq) * DD;
-#pragma unroll
-        for (int r = 0; r < D; r++)
-#pragma unroll
-          for (int cq = 0; cq < D; cq++) dst[r * D + cq] = block_entry<V, float> (M, r, cq, false);
-      }
-      __syncthreads ();
       // ---- phase B
-#pragma unroll
-      for (int u = 0; u < D; u++)
-        {
-          const int K = 4 * u + kq;
-          const int jz = jb + K / D, q = K % D;
-          float bz[CT];
-#pragma unroll
-          for (int ct = 0; ct < CT; ct++)
-            bz[ct] = (jz < j1 && cok[ct]) ? XT[(size_t) jz * zs + (size_t) q * B + cbase + 16 * ct + ir]
-                                          : 0.f;
-#pragma unroll
-          for (int t = 0; t < D; t++)
-            {
-              const float a = blk[rowoff[t] + koff[u]];
-#pragma unroll
-              for (int ct = 0; ct < CT; ct++) acc[t][ct] = mfma4 (a, bz[ct], acc[t][ct]);
-            }
-        }
-      __syncthreads ();
+      RealRounds<V, CT, 0>::run (M, blk, zs_l, ir, kq, acc);
     }
   // rows R = 16 t + 4 (lane / 16) + e of the (16 D)-row tile
 #pragma unroll
@@ -536,9 +563,10 @@ k_real_mfma (const double4 *__restrict__ pos, const float *__restrict__ XT, int 
         if (i < p.np)
 #pragma unroll
           for (int ct = 0; ct < CT; ct++)
-            if (cok[ct])
-              partial[(((size_t) blockIdx.y * p.np + i) * D + r) * B + cbase + 16 * ct + ir]
-                = acc[t][ct][e];
+            {
+              const int b = cbase + 16 * ct + ir;
+              if (b < B) partial[(((size_t) blockIdx.y * p.np + i) * D + r) * B + b] = acc[t][ct][e];
+            }
       }
 }
 
