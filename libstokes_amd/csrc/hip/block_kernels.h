@@ -370,32 +370,26 @@ k_recip_back_block (const double4 *__restrict__ pos, const double4 *__restrict__
     }
 }
 
+// one thread per (i, q, b), b fastest: the partial chunks are read with
+// consecutive threads on consecutive addresses and summed in chunk order
 template <int V, typename T>
-__global__ __launch_bounds__ (128) void
+__global__ __launch_bounds__ (256) void
 k_combine_block (const double4 *__restrict__ pos, const double *__restrict__ x, int B,
                  const T *__restrict__ rpart, int nreal, const T *__restrict__ kpart,
                  int nrecip, DevParams p, int periodic, double *__restrict__ y)
 {
   constexpr int D = Dim<V>::D;
-  const size_t t = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (size_t) p.np * B) return;
-  const int b = (int) (t % B), i = (int) (t / B);
   const size_t n = (size_t) p.np * D;
-  const double ai = pos[i].w;
-  double acc[D];
-#pragma unroll
-  for (int q = 0; q < D; q++)
-    acc[q] = self_diag (p, ai, periodic != 0, V, q) * x[(size_t) b * n + (size_t) i * D + q];
-  for (int ch = 0; ch < nreal; ch++)
-#pragma unroll
-    for (int q = 0; q < D; q++)
-      acc[q] += double (rpart[(((size_t) ch * p.np + i) * D + q) * B + b]);
-  for (int ch = 0; ch < nrecip; ch++)
-#pragma unroll
-    for (int q = 0; q < D; q++)
-      acc[q] += double (kpart[(((size_t) ch * p.np + i) * D + q) * B + b]);
-#pragma unroll
-  for (int q = 0; q < D; q++) y[(size_t) b * n + (size_t) i * D + q] = acc[q];
+  const size_t t = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n * B) return;
+  const int b = (int) (t % B);
+  const size_t iq = t / B;
+  const int i = (int) (iq / D), q = (int) (iq - (size_t) i * D);
+  const size_t stride = n * B;
+  double acc = self_diag (p, pos[i].w, periodic != 0, V, q) * x[(size_t) b * n + iq];
+  for (int ch = 0; ch < nreal; ch++) acc += double (rpart[(size_t) ch * stride + t]);
+  for (int ch = 0; ch < nrecip; ch++) acc += double (kpart[(size_t) ch * stride + t]);
+  y[(size_t) b * n + iq] = acc;
 }
 
 }  // namespace shk

@@ -687,13 +687,17 @@ static int atimes_block_vt (shk_dev *d, const double *x, double *y, int B)
   if ((rc = ensure (d, d->xs, sizeof (T) * (size_t) p.np * NH * D * B))) return rc;
   // real-space splits: JW sources per wave (FP64), or for the matrix-core
   // path (MIXED) one wave per 16 targets x jper sources x 16 CT columns,
-  // about 4096 waves
-  constexpr int CT = V == 2 ? 2 : (V == 1 ? 4 : 8);
+  // about 2048 waves (two per SIMD)
+  // 16 CT columns per wave: the smallest power of two covering B, at most
+  // 2 / 4 / 8 tiles for FTS / FT / F (the accumulators of D row tiles)
+  constexpr int CTMAX = V == 2 ? 2 : (V == 1 ? 4 : 8);
+  int CT = 1;
+  while (CT < CTMAX && 16 * CT < B) CT *= 2;
   const int nib = (p.np + 15) / 16, ncg = (B + 16 * CT - 1) / (16 * CT);
   int nj = (p.np + JW - 1) / JW, jper = 0;
   if (MF)
     {
-      nj = (4096 + nib * ncg - 1) / (nib * ncg);
+      nj = (2048 + nib * ncg - 1) / (nib * ncg);
       if (nj > (p.np + 15) / 16) nj = (p.np + 15) / 16;
       if (nj < 1) nj = 1;
       jper = (p.np + nj - 1) / nj;
@@ -703,7 +707,6 @@ static int atimes_block_vt (shk_dev *d, const double *x, double *y, int B)
   if ((rc = ensure (d, d->partial, sizeof (T) * (size_t) nj * p.np * D * B))) return rc;
   T *XT = (T *) d->xs.p;
   const size_t nb = (size_t) p.np * B;
-  if (d->timing) HIPCHK (hipEventRecord (d->ev[0], d->stream));
   if (!MF)
     hipLaunchKernelGGL ((k_block_prep<V, T>), dim3 ((unsigned) ((nb + 255) / 256)), dim3 (256),
                         0, d->stream, x, p.np, B, XT);
@@ -721,11 +724,30 @@ static int atimes_block_vt (shk_dev *d, const double *x, double *y, int B)
   rp.tx = d->tx; rp.ty = d->ty; rp.tz = d->tz; rp.tbits_words = d->tbits_words;
   const long waves = (long) p.np * nj;
   const void *cand = std::is_same<T, double>::value ? d->cand.p : d->candf.p;
+  // timing: ev[0] -> ev[1] brackets the real-space kernel alone
+  if (d->timing) HIPCHK (hipEventRecord (d->ev[0], d->stream));
   if (MF)
-    hipLaunchKernelGGL ((k_real_mfma<V, CT>), dim3 (nib, nj, ncg), dim3 (64), 0, d->stream, pos4,
-                        (const float *) XT, B, NH * D * B, (const float4 *) d->candf.p,
-                        (const int4 *) d->candm.p, (const unsigned *) d->tbits.p, rp, jper,
-                        (float *) d->partial.p);
+    {
+      const dim3 g (nib, nj, ncg);
+      const float4 *cf = (const float4 *) d->candf.p;
+      const int4 *cm = (const int4 *) d->candm.p;
+      const unsigned *tb = (const unsigned *) d->tbits.p;
+      const float *xf = (const float *) XT;
+      float *pf = (float *) d->partial.p;
+      const int zs = NH * D * B;
+      if (CT == 1)
+        hipLaunchKernelGGL ((k_real_mfma<V, 1>), g, dim3 (64), 0, d->stream, pos4, xf, B, zs, cf,
+                            cm, tb, rp, jper, pf);
+      else if (CT == 2)
+        hipLaunchKernelGGL ((k_real_mfma<V, (CTMAX < 2 ? CTMAX : 2)>), g, dim3 (64), 0, d->stream,
+                            pos4, xf, B, zs, cf, cm, tb, rp, jper, pf);
+      else if (CT == 4)
+        hipLaunchKernelGGL ((k_real_mfma<V, (CTMAX < 4 ? CTMAX : 4)>), g, dim3 (64), 0, d->stream,
+                            pos4, xf, B, zs, cf, cm, tb, rp, jper, pf);
+      else
+        hipLaunchKernelGGL ((k_real_mfma<V, CTMAX>), g, dim3 (64), 0, d->stream, pos4, xf, B, zs,
+                            cf, cm, tb, rp, jper, pf);
+    }
   else
     hipLaunchKernelGGL ((k_real_block<V, T, JW>), dim3 ((unsigned) ((waves + 3) / 4)), dim3 (256),
                         0, d->stream, pos4, (const T *) XT, B, NH * D * B,
@@ -744,7 +766,7 @@ static int atimes_block_vt (shk_dev *d, const double *x, double *y, int B)
                     : recip_block_launch<V, T, 1> (d, XT, B, nrecip, kpart);
       if (rc) return rc;
     }
-  hipLaunchKernelGGL ((k_combine_block<V, T>), dim3 ((unsigned) ((nb + 127) / 128)), dim3 (128), 0,
+  hipLaunchKernelGGL ((k_combine_block<V, T>), dim3 ((unsigned) ((nb * D + 255) / 256)), dim3 (256), 0,
                       d->stream, pos4, x, B, (const T *) d->partial.p, nj, (const T *) kpart,
                       nrecip, q, 1, y);
   HIPCHK (hipGetLastError ());
