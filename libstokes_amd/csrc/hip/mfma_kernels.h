@@ -16,7 +16,7 @@
 //
 // The phase matrix never goes to memory: the lanes that feed an A operand
 // build their entry from the per-particle tables e^{i m th} (two complex
-// products, as in k_recip_sf_tab).  Operand layouts of the 16x16x4 FP32
+// products, as in k_recip_sf_tab), held entry-major in LDS.  Operand layouts of the 16x16x4 FP32
 // instruction: A[i][k] in lane i + 16 k, B[k][j] in lane j + 16 k, and
 // D[i][j] in element i % 4 of lane j + 16 (i / 4).
 //
@@ -44,82 +44,131 @@ __device__ __forceinline__ int4 phase_offsets (const PhaseBasis &b, const int4 m
   return make_int4 (m.x, b.K1 + 1 + b.K2 + m.y, b.K1 + 2 * b.K2 + 2 + b.K3 + m.z, 0);
 }
 
-// column-major x (n x B) -> Zt[j][h][q][b] (h = 1: a_j^2 z, unequal spheres)
+// column-major x (n x B) -> Zt[j zs + (h D + q) B + b] (h = 1: a_j^2 z,
+// unequal spheres; zs >= NH D B, a multiple of 4 for 16-byte rows); rows
+// np <= j < npad are zero
 template <int V, int NH>
 __global__ void k_block_prep_f (const double *__restrict__ x, const double4 *__restrict__ pos,
-                                int np, int B, float *__restrict__ Zt)
+                                int np, int npad, int B, int zs, float *__restrict__ Zt)
 {
   constexpr int D = Dim<V>::D;
   const size_t n = (size_t) np * D;
   const size_t t = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (size_t) np * B) return;
+  if (t >= (size_t) npad * B) return;
   const int b = (int) (t % B);
   const int j = (int) (t / B);
   float z[D];
 #pragma unroll
-  for (int q = 0; q < D; q++) z[q] = float (x[(size_t) b * n + (size_t) j * D + q]);
+  for (int q = 0; q < D; q++) z[q] = j < np ? float (x[(size_t) b * n + (size_t) j * D + q]) : 0.f;
   if (V == 2) s_transform (z);
-  float *row = Zt + (size_t) j * NH * D * B + b;
+  float *row = Zt + (size_t) j * zs + b;
 #pragma unroll
   for (int q = 0; q < D; q++) row[(size_t) q * B] = z[q];
   if (NH > 1)
     {
-      const double a = pos[j].w;
+      const double a = j < np ? pos[j].w : 0.0;
       const float a2 = float (a * a);
 #pragma unroll
       for (int q = 0; q < D; q++) row[(size_t) (D + q) * B] = a2 * z[q];
     }
 }
 
+// phase tables in two layouts: tab[j LD + e] (particle-major, for the
+// structure factors) and tabT[e npad + j] (entry-major, for the
+// back-projection), e^{i m th_j} of entry e as in k_phase_tab; zero for
+// np <= j < npad
+template <int LD>
+__global__ void k_phase_tab2 (const double4 *__restrict__ pos, int np, int npad, PhaseBasis b,
+                              Cplx<float> *__restrict__ tab, Cplx<float> *__restrict__ tabT)
+{
+  const size_t t = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (size_t) npad * b.W) return;
+  const int e = (int) (t / npad), j = (int) (t - (size_t) e * npad);
+  Cplx<float> v{0.f, 0.f};
+  if (j < np)
+    {
+      double t1, t2, t3;
+      phase_angles (b, pos[j], t1, t2, t3);
+      double th;
+      int m;
+      if (e <= b.K1) { m = e; th = t1; }
+      else if (e <= b.K1 + 2 * b.K2 + 1) { m = e - (b.K1 + 1) - b.K2; th = t2; }
+      else { m = e - (b.K1 + 2 * b.K2 + 2) - b.K3; th = t3; }
+      double sn, cs;
+      sincos (double (m) * th, &sn, &cs);
+      v = Cplx<float>{float (cs), float (sn)};
+    }
+  tabT[t] = v;
+  tab[(size_t) j * LD + e] = v;
+}
+
 // 4 waves x 4 x 4 tiles; 64 KB of LDS for the end-of-block reduction
 constexpr int MF_RED_BYTES = 4 * 64 * 64 * 4;
 
-// copy n Cplx from global to LDS with eight loads in flight per thread
-__device__ __forceinline__ void stage_cplx (Cplx<float> *__restrict__ dst,
-                                            const Cplx<float> *__restrict__ src, int n)
+typedef __attribute__ ((address_space (3))) void lds_void;
+
+// asynchronous copy global -> LDS of nv 16-byte vectors by the 256 threads
+// of a block (global_load_lds_dwordx4: lane t of a wave writes 16 bytes at
+// the wave's LDS base + 16 t).  src16(v) gives the global address of
+// vector v; dst is 16-byte aligned.  Complete after s_waitcnt vmcnt(0) and
+// a barrier.
+template <typename F>
+__device__ __forceinline__ void async_copy16 (void *dst, int nv, F src16)
 {
-  for (int e0 = threadIdx.x; e0 < n; e0 += 256 * 8)
-    {
-      Cplx<float> v[8];
-#pragma unroll
-      for (int u = 0; u < 8; u++)
-        {
-          const int e = e0 + 256 * u;
-          if (e < n) v[u] = src[e];
-        }
-#pragma unroll
-      for (int u = 0; u < 8; u++)
-        {
-          const int e = e0 + 256 * u;
-          if (e < n) dst[e] = v[u];
-        }
-    }
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  for (int v0 = wave * 64; v0 < nv; v0 += 256)
+    if (v0 + lane < nv)
+      __builtin_amdgcn_global_load_lds (src16 (v0 + lane), (lds_void *) ((char *) dst + 16 * v0), 16,
+                                        0, 0);
 }
 
-// structure factors: part[((sj nk + k) 2 + s) cz + c], s = 0 cos, 1 sin;
-// Zt rows of cz columns.  grid (ceil(cz/64), ceil(nk/64), splits of j) --
-// the column blocks of one k block run side by side and share the tables
-// through L2; JT particle tables per LDS tile; the B operands (z) come
-// straight from memory, one quad ahead.
+__device__ __forceinline__ void async_wait_all ()
+{
+  __builtin_amdgcn_s_waitcnt (0);
+  __syncthreads ();
+}
+
+// LDS read of a complex entry at a byte address
+__device__ __forceinline__ Cplx<float> lds_c (const unsigned char *base, int byte)
+{
+  return *(const Cplx<float> *) (base + byte);
+}
+
+// structure factors: part[((sj nk + k) 2 + s) cz + c], s = 0 cos, 1 sin.
+// Zt rows of zs floats (16-byte rows, npad rows, zero past np), tables
+// tab[j LD + e].  grid (ceil(cz/64), ceil(nk/64), splits of j, each a
+// multiple of 64 particles): the column blocks of one k block run side by
+// side and share the tables through L2.  Per tile of 64 particles the
+// tables ([64][LD]) and the z slice ([64][64]) go to LDS by direct global ->
+// LDS loads; wave w takes quads w, w + 4, w + 8, w + 12, whose LDS
+// addresses differ from the first by constants, so the inner loop spends
+// its vector instructions on the phase products only.  The 16 lanes of a
+// row read consecutive entries (consecutive m3): no bank conflicts.
+template <int LD>
 __global__ __launch_bounds__ (256, 2) void
-k_sf_mfma (const float *__restrict__ Zt, int cz, const int4 *__restrict__ km,
-           const Cplx<float> *__restrict__ tab, PhaseBasis b, int np, int nk, int jper, int JT,
+k_sf_mfma (const float *__restrict__ Zt, int cz, int zs, const int4 *__restrict__ km,
+           const Cplx<float> *__restrict__ tab, int npad, PhaseBasis b, int nk, int jper,
            float *__restrict__ part)
 {
   extern __shared__ __align__ (16) unsigned char smem_mf[];
-  Cplx<float> *stab = (Cplx<float> *) smem_mf;            // [JT][W]
-  float *red = (float *) smem_mf;                         // [4][64][64], after the loop
+  constexpr int TB = 64 * LD * (int) sizeof (Cplx<float>);
+  float *red = (float *) smem_mf;                          // [4][64][64], after the loop
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int ir = lane & 15, kq = lane >> 4;
   const int cbase = blockIdx.x * 64, kbase = blockIdx.y * 64;
-  const int j0 = blockIdx.z * jper, j1 = min (np, j0 + jper);
-  int4 o[4];
+  const int j0 = blockIdx.z * jper, j1 = min (npad, j0 + jper);
+  // byte addresses of the lane's table entries for its first quad
+  int ax[4], ay[4], az[4];
 #pragma unroll
-  for (int kt = 0; kt < 4; kt++) o[kt] = phase_offsets (b, km[min (kbase + 16 * kt + ir, nk - 1)]);
-  bool cok[4];
-#pragma unroll
-  for (int ct = 0; ct < 4; ct++) cok[ct] = cbase + 16 * ct + ir < cz;
-  const float *zc = Zt + cbase + ir;
+  for (int kt = 0; kt < 4; kt++)
+    {
+      const int4 o = phase_offsets (b, km[min (kbase + 16 * kt + ir, nk - 1)]);
+      const int rb = (4 * wave + kq) * LD;
+      ax[kt] = (rb + o.x) * 8;
+      ay[kt] = (rb + o.y) * 8;
+      az[kt] = (rb + o.z) * 8;
+    }
+  const int zb = TB + ((4 * wave + kq) * 64 + ir) * 4;
   mf4 accc[4][4], accs[4][4];
 #pragma unroll
   for (int kt = 0; kt < 4; kt++)
@@ -129,40 +178,32 @@ k_sf_mfma (const float *__restrict__ Zt, int cz, const int4 *__restrict__ km,
         accc[kt][ct] = mf4{0.f, 0.f, 0.f, 0.f};
         accs[kt][ct] = mf4{0.f, 0.f, 0.f, 0.f};
       }
-  for (int jb = j0; jb < j1; jb += JT)
+  const int ncv = min (16, (zs - cbase) >> 2);           // 16-byte vectors per z row
+  for (int jb = j0; jb < j1; jb += 64)
     {
-      const int n = min (JT, j1 - jb);
-      stage_cplx (stab, tab + (size_t) jb * b.W, n * b.W);
-      __syncthreads ();
-      const int nq = (n + 3) / 4;
-      float bz[4];
-      {
-        const int j = jb + 4 * wave + kq;
+      const Cplx<float> *ts = tab + (size_t) jb * LD;
+      async_copy16 (smem_mf, 32 * LD, [&] (int v) { return (const void *) (ts + 2 * v); });
+      async_copy16 (smem_mf + TB, 64 * 16, [&] (int v) -> const void * {
+        const int row = v >> 4, cv = v & 15;
+        return Zt + (size_t) (jb + row) * zs + cbase + 4 * min (cv, ncv - 1);
+      });
+      async_wait_all ();
 #pragma unroll
-        for (int ct = 0; ct < 4; ct++)
-          bz[ct] = (wave < nq && j < j1 && cok[ct]) ? zc[(size_t) j * cz + 16 * ct] : 0.f;
-      }
-      for (int qd = wave; qd < nq; qd += 4)
+      for (int u = 0; u < 4; u++)
         {
-          const int jt = 4 * qd + kq;
-          const bool ok = jt < n;
-          float bn[4];
-          {
-            const int j = jb + jt + 16;
-            const bool nok = qd + 4 < nq && j < j1;
-#pragma unroll
-            for (int ct = 0; ct < 4; ct++)
-              bn[ct] = (nok && cok[ct]) ? zc[(size_t) j * cz + 16 * ct] : 0.f;
-          }
-          const Cplx<float> *row = stab + (size_t) (ok ? jt : 0) * b.W;
-          float ac[4], as[4];
+          float ac[4], as[4], bz[4];
 #pragma unroll
           for (int kt = 0; kt < 4; kt++)
             {
-              const Cplx<float> e = cmul (cmul (row[o[kt].x], row[o[kt].y]), row[o[kt].z]);
-              ac[kt] = ok ? e.re : 0.f;
-              as[kt] = ok ? e.im : 0.f;
+              const Cplx<float> e = cmul (cmul (lds_c (smem_mf, ax[kt] + 128 * LD * u),
+                                                lds_c (smem_mf, ay[kt] + 128 * LD * u)),
+                                          lds_c (smem_mf, az[kt] + 128 * LD * u));
+              ac[kt] = e.re;
+              as[kt] = e.im;
             }
+#pragma unroll
+          for (int ct = 0; ct < 4; ct++)
+            bz[ct] = *(const float *) (smem_mf + zb + 4096 * u + 64 * ct);
 #pragma unroll
           for (int kt = 0; kt < 4; kt++)
 #pragma unroll
@@ -171,8 +212,6 @@ k_sf_mfma (const float *__restrict__ Zt, int cz, const int4 *__restrict__ km,
                 accc[kt][ct] = mfma4 (ac[kt], bz[ct], accc[kt][ct]);
                 accs[kt][ct] = mfma4 (as[kt], bz[ct], accs[kt][ct]);
               }
-#pragma unroll
-          for (int ct = 0; ct < 4; ct++) bz[ct] = bn[ct];
         }
       __syncthreads ();
     }
@@ -199,19 +238,29 @@ k_sf_mfma (const float *__restrict__ Zt, int cz, const int4 *__restrict__ km,
 }
 
 // P, Q per (k, b) from the split structure factors (FP64 k-blocks, as
-// k_recip_blocks); PQ[((k NPOW + p) 2 + s) C + q B + b], C = D B
+// k_recip_blocks); PQ[((k NPOW + p) 2 + s) Cp + q B + b], Cp >= D B;
+// rows nk <= k < nkpad are zero
 template <int V, int NPOW>
 __global__ __launch_bounds__ (128) void
-k_blk_mfma (const double4 *__restrict__ kv, const double *__restrict__ ks, int nk, int B,
-            int nsplit, const float *__restrict__ part, float *__restrict__ PQ)
+k_blk_mfma (const double4 *__restrict__ kv, const double *__restrict__ ks, int nk, int nkpad,
+            int B, int nsplit, int Cp, const float *__restrict__ part, float *__restrict__ PQ)
 {
   constexpr int D = Dim<V>::D, NM = Dim<V>::NMOM;
   constexpr int NS = (NPOW > 1) ? 4 : 2;
   constexpr int NH = NS / 2;
   const size_t t = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (size_t) nk * B) return;
+  if (t >= (size_t) nkpad * B) return;
   const int k = (int) (t / B), b = (int) (t % B);
-  const size_t cz = (size_t) NH * D * B, C = (size_t) D * B;
+  const size_t cz = (size_t) NH * D * B, C = (size_t) Cp;
+  if (k >= nk)
+    {
+      // rows past nk: zero P, Q (the back-projection runs whole quads)
+      float *out = PQ + (size_t) k * NPOW * 2 * C + b;
+      for (int r = 0; r < NPOW * 2; r++)
+#pragma unroll
+        for (int q = 0; q < D; q++) out[r * C + (size_t) q * B] = 0.f;
+      return;
+    }
   // CS: [cos z | sin z | cos a^2 z | sin a^2 z], D each
   double CS[NS * D];
 #pragma unroll
@@ -276,28 +325,30 @@ k_blk_mfma (const double4 *__restrict__ kv, const double *__restrict__ ks, int n
 }
 
 // back-projection: part[(sk np + i) C + c] = sum_k sum_p a_i^p [cos P_p +
-// sin Q_p] (a_i^p = 1 for equal spheres, p = 1..NPOW otherwise).  grid
-// (ceil(C/64), ceil(np/64), splits of k).  The block's 64 particle tables
-// sit in LDS for the whole k range; k offsets and the B operands (P, Q)
-// come straight from memory, one quad ahead.
+// sin Q_p] (a_i^p = 1 for equal spheres, p = 1..NPOW otherwise).  PQ rows
+// of Cp floats (Cp a multiple of 64), zero from nk to the end of the last
+// split, so every quad is whole; tables transposed, tabT[e npad + j].  grid
+// (ceil(C/64), ceil(np/64), splits of kper k).  The block's 64 particle
+// tables sit in LDS ([entry][64]) for the whole k range; the lattice
+// offsets and the B operands (P, Q) come straight from memory, one quad
+// ahead.
 template <int NPOW>
 __global__ __launch_bounds__ (256, 2) void
 k_back_mfma (const double4 *__restrict__ pos, const int4 *__restrict__ km,
-             const Cplx<float> *__restrict__ tab, PhaseBasis b, const float *__restrict__ PQ,
-             int C, int np, int nk, int kper, float *__restrict__ part)
+             const Cplx<float> *__restrict__ tabT, int npad, PhaseBasis b,
+             const float *__restrict__ PQ, int C, int Cp, int np, int nk, int kper,
+             float *__restrict__ part)
 {
   extern __shared__ __align__ (16) unsigned char smem_mf[];
-  Cplx<float> *stab = (Cplx<float> *) smem_mf;            // [64][W]
+  constexpr int PR = NPOW * 2;                            // P, Q rows per k
   float *red = (float *) smem_mf;                         // [4][64][64], after the loop
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int ir = lane & 15, kq = lane >> 4;
   const int cbase = blockIdx.x * 64, ibase = blockIdx.y * 64;
-  const int k0 = blockIdx.z * kper, k1 = min (nk, k0 + kper);
-  {
-    const int ni = min (64, np - ibase);
-    stage_cplx (stab, tab + (size_t) ibase * b.W, ni * b.W);
-    for (int e = ni * b.W + threadIdx.x; e < 64 * b.W; e += 256) stab[e] = Cplx<float>{0.f, 0.f};
-  }
+  const int k0 = blockIdx.z * kper, k1 = k0 + kper;
+  async_copy16 (smem_mf, b.W * 32, [&] (int v) {
+    return (const void *) (tabT + (size_t) (v >> 5) * npad + ibase + 2 * (v & 31));
+  });
   float apow[4][NPOW];
 #pragma unroll
   for (int it = 0; it < 4; it++)
@@ -307,33 +358,30 @@ k_back_mfma (const double4 *__restrict__ pos, const int4 *__restrict__ km,
 #pragma unroll
       for (int p = 1; p < NPOW; p++) apow[it][p] = apow[it][p - 1] * a;
     }
-  bool cok[4];
-#pragma unroll
-  for (int ct = 0; ct < 4; ct++) cok[ct] = cbase + 16 * ct + ir < C;
-  const float *pc = PQ + cbase + ir;
+  const float *pq = PQ + cbase + ir;
   mf4 acc[4][4];
 #pragma unroll
   for (int it = 0; it < 4; it++)
 #pragma unroll
     for (int ct = 0; ct < 4; ct++) acc[it][ct] = mf4{0.f, 0.f, 0.f, 0.f};
-  __syncthreads ();
-  const int nq = (k1 - k0 + 3) / 4;
-  // operands of quad qd: offsets of k = k0 + 4 qd + kq and its P, Q rows
-  int4 o;
-  float bpq[NPOW][2][4];
-  auto load = [&] (int qd, int4 &oo, float (&v)[NPOW][2][4]) {
-    const int k = k0 + 4 * qd + kq;
-    const bool ok = qd < nq && k < k1;
-    oo = phase_offsets (b, km[ok ? k : k0]);
+  async_wait_all ();
+  const int nq = kper / 4;
+  // operands of quad q: table byte offsets of k = k0 + 4 q + kq and its
+  // P, Q rows
+  auto load = [&] (int q, int4 &oo, float (&v)[NPOW][2][4]) {
+    const int k = k0 + 4 * min (q, nq - 1) + kq;
+    const int4 m = phase_offsets (b, km[min (k, nk - 1)]);
+    oo = make_int4 (m.x * 512 + ir * 8, m.y * 512 + ir * 8, m.z * 512 + ir * 8, 0);
+    const float *r = pq + (size_t) k * PR * Cp;
 #pragma unroll
     for (int p = 0; p < NPOW; p++)
 #pragma unroll
       for (int s = 0; s < 2; s++)
 #pragma unroll
-        for (int ct = 0; ct < 4; ct++)
-          v[p][s][ct] = (ok && cok[ct]) ? pc[((size_t) k * NPOW * 2 + 2 * p + s) * C + 16 * ct]
-                                        : 0.f;
+        for (int ct = 0; ct < 4; ct++) v[p][s][ct] = r[(2 * p + s) * Cp + 16 * ct];
   };
+  int4 o;
+  float bpq[NPOW][2][4];
   load (wave, o, bpq);
   for (int qd = wave; qd < nq; qd += 4)
     {
@@ -344,8 +392,9 @@ k_back_mfma (const double4 *__restrict__ pos, const int4 *__restrict__ km,
 #pragma unroll
       for (int it = 0; it < 4; it++)
         {
-          const Cplx<float> *row = stab + (size_t) (16 * it + ir) * b.W;
-          const Cplx<float> e = cmul (cmul (row[o.x], row[o.y]), row[o.z]);
+          const Cplx<float> e = cmul (cmul (lds_c (smem_mf, o.x + 128 * it),
+                                            lds_c (smem_mf, o.y + 128 * it)),
+                                      lds_c (smem_mf, o.z + 128 * it));
           ac[it] = e.re;
           as[it] = e.im;
         }

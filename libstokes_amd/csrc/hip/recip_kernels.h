@@ -16,6 +16,7 @@
 #include <hip/hip_runtime.h>
 #include "pair_math.h"
 #include "ewald_kernels.h"
+#include "real_kernels.h"
 
 namespace shk {
 
@@ -256,8 +257,9 @@ __device__ __forceinline__ void phase_angles (const PhaseBasis &b, const double4
   t1 = b.k0x * p.x - b.s1 * t2 - b.s2 * t3;
 }
 
+// tab[j ld + e], e < W (ld >= W: row stride)
 template <typename T>
-__global__ void k_phase_tab (const double4 *__restrict__ pos, int np, PhaseBasis b,
+__global__ void k_phase_tab (const double4 *__restrict__ pos, int np, PhaseBasis b, int ld,
                              Cplx<T> *__restrict__ tab)
 {
   const size_t t = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
@@ -272,7 +274,7 @@ __global__ void k_phase_tab (const double4 *__restrict__ pos, int np, PhaseBasis
   else { m = e - (b.K1 + 2 * b.K2 + 2) - b.K3; th = t3; }
   double s, c;
   sincos (double (m) * th, &s, &c);
-  tab[t] = Cplx<T>{T (c), T (s)};
+  tab[(size_t) j * ld + e] = Cplx<T>{T (c), T (s)};
 }
 
 template <typename T>

@@ -453,7 +453,7 @@ static int recip_launch (shk_dev *d, const double *x, int &nrecip, const void *&
   const double4 *kv = (const double4 *) d->kv.p;
   const int4 *km = (const int4 *) d->km.p;
   hipLaunchKernelGGL ((k_phase_tab<T>), dim3 ((unsigned) (((size_t) p.np * b.W + 255) / 256)),
-                      dim3 (256), 0, d->stream, pos4, p.np, b, (Cplx<T> *) d->ptab.p);
+                      dim3 (256), 0, d->stream, pos4, p.np, b, b.W, (Cplx<T> *) d->ptab.p);
   HIPCHK (hipGetLastError ());
   const int jt = sf_tile<V, T> (b);
   const size_t lds = sf_lds<V, T, NPOW> (b, jt);
@@ -589,63 +589,72 @@ static int recip_block_launch (shk_dev *d, const T *XT, int B, int &nrecip,
   return SHK_OK;
 }
 
-// FP32 matrix-core reciprocal sums for a block (MIXED precision)
+// FP32 matrix-core reciprocal sums for a block (MIXED precision); Zt rows
+// of zs floats, npad = np rounded up to 64 rows (zero past np)
 template <int V, int NPOW>
-static int recip_block_launch_mfma (shk_dev *d, const float *Zt, int B, int &nrecip,
-                                    const void *&kpart)
+static int recip_block_launch_mfma (shk_dev *d, const float *Zt, int zs, int npad, int B,
+                                    int &nrecip, const void *&kpart)
 {
   constexpr int D = Dim<V>::D;
   constexpr int NH = NPOW > 1 ? 2 : 1;
+  constexpr int PR = NPOW * 2;
   const shk_params &p = d->prm;
   const int nk = p.nk, np = p.np;
   const int cz = NH * D * B, C = D * B;
   const PhaseBasis &b = d->basis;
-  // structure factors: JT particle tables per LDS tile (the 64 KB
-  // reduction region bounds the tile), two blocks per CU
-  const size_t perj = sizeof (Cplx<float>) * b.W;
-  int JT = (int) ((size_t) MF_RED_BYTES / perj) & ~15;
-  if (JT > 128) JT = 128;
-  if (JT < 16) JT = 16;
-  size_t lds_sf = (size_t) JT * perj;
-  if (lds_sf < (size_t) MF_RED_BYTES) lds_sf = MF_RED_BYTES;
-  const int nkb = (nk + 63) / 64, ncz = (cz + 63) / 64;
-  int sj = (1024 + nkb * ncz - 1) / (nkb * ncz);
-  if (sj > (np + 63) / 64) sj = (np + 63) / 64;
-  if (sj < 1) sj = 1;
-  int jper = (np + sj - 1) / sj;
-  jper = (jper + 15) & ~15;
-  sj = (np + jper - 1) / jper;
-  // back-projection: the block's 64 particle tables
-  size_t lds_bk = sizeof (Cplx<float>) * 64 * (size_t) b.W;
-  if (lds_bk < (size_t) MF_RED_BYTES) lds_bk = MF_RED_BYTES;
-  if (lds_sf > 80 * 1024 || lds_bk > 80 * 1024)
+  // LDS: tables of 64 particles (64 x LD entries for the structure factors
+  // with LD = 96 or 128 >= W, W x 64 for the back-projection) [+ the 16 KB
+  // z slice], the 64 KB end-of-block reduction aliased over them
+  const int LD = b.W <= 96 ? 96 : 128;
+  if (b.W > 128)
     {
       set_err ("batched reciprocal sum: phase tables of %d entries exceed LDS", b.W);
       return SHK_EARG;
     }
-  const int nib = (np + 63) / 64, ncc = (C + 63) / 64;
+  size_t lds_sf = (size_t) 64 * LD * 8 + 16384, lds_bk = (size_t) b.W * 512;
+  if (lds_sf < (size_t) MF_RED_BYTES) lds_sf = MF_RED_BYTES;
+  if (lds_bk < (size_t) MF_RED_BYTES) lds_bk = MF_RED_BYTES;
+  // structure factors: splits of j, multiples of 64
+  const int nkb = (nk + 63) / 64, ncz = (cz + 63) / 64;
+  int sj = (1024 + nkb * ncz - 1) / (nkb * ncz);
+  if (sj > npad / 64) sj = npad / 64;
+  if (sj < 1) sj = 1;
+  int jper = (npad + sj - 1) / sj;
+  jper = (jper + 63) & ~63;
+  sj = (npad + jper - 1) / jper;
+  // back-projection: splits of kper k (a multiple of 4); P, Q rows padded
+  // with zeros to sk kper, columns to a multiple of 64
+  const int nib = (np + 63) / 64, ncc = (C + 63) / 64, Cp = ncc * 64;
   int sk = (1024 + nib * ncc - 1) / (nib * ncc);
   if (sk > (nk + 63) / 64) sk = (nk + 63) / 64;
   if (sk < 1) sk = 1;
   int kper = (nk + sk - 1) / sk;
   kper = (kper + 3) & ~3;
   sk = (nk + kper - 1) / kper;
+  const int nkpad = sk * kper;
   int rc;
   if ((rc = ensure (d, d->kpart, sizeof (float) * (size_t) sj * nk * 2 * cz))) return rc;
-  if ((rc = ensure (d, d->kpq, sizeof (float) * (size_t) nk * NPOW * 2 * C))) return rc;
+  if ((rc = ensure (d, d->kpq, sizeof (float) * (size_t) nkpad * PR * Cp))) return rc;
   if ((rc = ensure (d, d->kback, sizeof (float) * (size_t) sk * np * C))) return rc;
-  if ((rc = ensure (d, d->ptab, sizeof (Cplx<float>) * (size_t) np * b.W))) return rc;
+  if ((rc = ensure (d, d->ptab, sizeof (Cplx<float>) * (size_t) npad * (b.W + LD)))) return rc;
   const double4 *pos4 = (const double4 *) d->pos.p;
   const double4 *kv = (const double4 *) d->kv.p;
   const int4 *km = (const int4 *) d->km.p;
-  const Cplx<float> *tab = (const Cplx<float> *) d->ptab.p;
-  hipLaunchKernelGGL ((k_phase_tab<float>), dim3 ((unsigned) (((size_t) np * b.W + 255) / 256)),
-                      dim3 (256), 0, d->stream, pos4, np, b, (Cplx<float> *) d->ptab.p);
+  Cplx<float> *tab = (Cplx<float> *) d->ptab.p, *tabT = tab + (size_t) npad * LD;
+  const dim3 gt ((unsigned) (((size_t) npad * b.W + 255) / 256));
+  if (LD == 96)
+    hipLaunchKernelGGL ((k_phase_tab2<96>), gt, dim3 (256), 0, d->stream, pos4, np, npad, b, tab,
+                        tabT);
+  else
+    hipLaunchKernelGGL ((k_phase_tab2<128>), gt, dim3 (256), 0, d->stream, pos4, np, npad, b, tab,
+                        tabT);
   HIPCHK (hipGetLastError ());
   static bool attr_sf = false, attr_bk = false;
   if (!attr_sf)
     {
-      HIPCHK (hipFuncSetAttribute ((const void *) k_sf_mfma,
+      HIPCHK (hipFuncSetAttribute ((const void *) k_sf_mfma<96>,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, 80 * 1024));
+      HIPCHK (hipFuncSetAttribute ((const void *) k_sf_mfma<128>,
                                    hipFuncAttributeMaxDynamicSharedMemorySize, 80 * 1024));
       attr_sf = true;
     }
@@ -655,16 +664,20 @@ static int recip_block_launch_mfma (shk_dev *d, const float *Zt, int B, int &nre
                                    hipFuncAttributeMaxDynamicSharedMemorySize, 80 * 1024));
       attr_bk = true;
     }
-  hipLaunchKernelGGL (k_sf_mfma, dim3 (ncz, nkb, sj), dim3 (256), lds_sf, d->stream, Zt, cz, km,
-                      tab, b, np, nk, jper, JT, (float *) d->kpart.p);
+  if (LD == 96)
+    hipLaunchKernelGGL ((k_sf_mfma<96>), dim3 (ncz, nkb, sj), dim3 (256), lds_sf, d->stream, Zt,
+                        cz, zs, km, tab, npad, b, nk, jper, (float *) d->kpart.p);
+  else
+    hipLaunchKernelGGL ((k_sf_mfma<128>), dim3 (ncz, nkb, sj), dim3 (256), lds_sf, d->stream, Zt,
+                        cz, zs, km, tab, npad, b, nk, jper, (float *) d->kpart.p);
   HIPCHK (hipGetLastError ());
-  const size_t nkB = (size_t) nk * B;
+  const size_t nkB = (size_t) nkpad * B;
   hipLaunchKernelGGL ((k_blk_mfma<V, NPOW>), dim3 ((unsigned) ((nkB + 127) / 128)), dim3 (128),
-                      0, d->stream, kv, (const double *) d->ks.p, nk, B, sj,
+                      0, d->stream, kv, (const double *) d->ks.p, nk, nkpad, B, sj, Cp,
                       (const float *) d->kpart.p, (float *) d->kpq.p);
   HIPCHK (hipGetLastError ());
   hipLaunchKernelGGL ((k_back_mfma<NPOW>), dim3 (ncc, nib, sk), dim3 (256), lds_bk, d->stream,
-                      pos4, km, tab, b, (const float *) d->kpq.p, C, np, nk, kper,
+                      pos4, km, tabT, npad, b, (const float *) d->kpq.p, C, Cp, np, nk, kper,
                       (float *) d->kback.p);
   HIPCHK (hipGetLastError ());
   nrecip = sk;
@@ -684,7 +697,10 @@ static int atimes_block_vt (shk_dev *d, const double *x, double *y, int B)
   const void *kpart = nullptr;
   constexpr bool MF = std::is_same<T, float>::value;
   const int NH = MF && p.poly ? 2 : 1;
-  if ((rc = ensure (d, d->xs, sizeof (T) * (size_t) p.np * NH * D * B))) return rc;
+  // rows of the transposed block: 16-byte multiples on the FP32 path
+  const int zs = MF ? (NH * D * B + 3) & ~3 : D * B;
+  const int npad = MF ? (p.np + 63) & ~63 : p.np;     // zero rows past np
+  if ((rc = ensure (d, d->xs, sizeof (T) * (size_t) npad * zs))) return rc;
   // real-space splits: JW sources per wave (FP64), or for the matrix-core
   // path (MIXED) one wave per 16 targets x jper sources x 16 CT columns,
   // about 2048 waves (two per SIMD)
@@ -711,11 +727,11 @@ static int atimes_block_vt (shk_dev *d, const double *x, double *y, int B)
     hipLaunchKernelGGL ((k_block_prep<V, T>), dim3 ((unsigned) ((nb + 255) / 256)), dim3 (256),
                         0, d->stream, x, p.np, B, XT);
   else if (NH == 2)
-    hipLaunchKernelGGL ((k_block_prep_f<V, 2>), dim3 ((unsigned) ((nb + 255) / 256)), dim3 (256),
-                        0, d->stream, x, pos4, p.np, B, (float *) XT);
+    hipLaunchKernelGGL ((k_block_prep_f<V, 2>), dim3 ((unsigned) (((size_t) npad * B + 255) / 256)),
+                        dim3 (256), 0, d->stream, x, pos4, p.np, npad, B, zs, (float *) XT);
   else
-    hipLaunchKernelGGL ((k_block_prep_f<V, 1>), dim3 ((unsigned) ((nb + 255) / 256)), dim3 (256),
-                        0, d->stream, x, pos4, p.np, B, (float *) XT);
+    hipLaunchKernelGGL ((k_block_prep_f<V, 1>), dim3 ((unsigned) (((size_t) npad * B + 255) / 256)),
+                        dim3 (256), 0, d->stream, x, pos4, p.np, npad, B, zs, (float *) XT);
   HIPCHK (hipGetLastError ());
   RealParams rp;
   rp.np = p.np; rp.ncand = d->ncand; rp.ncore = d->ncore; rp.poly = p.poly;
@@ -734,7 +750,6 @@ static int atimes_block_vt (shk_dev *d, const double *x, double *y, int B)
       const unsigned *tb = (const unsigned *) d->tbits.p;
       const float *xf = (const float *) XT;
       float *pf = (float *) d->partial.p;
-      const int zs = NH * D * B;
       if (CT == 1)
         hipLaunchKernelGGL ((k_real_mfma<V, 1>), g, dim3 (64), 0, d->stream, pos4, xf, B, zs, cf,
                             cm, tb, rp, jper, pf);
@@ -750,7 +765,7 @@ static int atimes_block_vt (shk_dev *d, const double *x, double *y, int B)
     }
   else
     hipLaunchKernelGGL ((k_real_block<V, T, JW>), dim3 ((unsigned) ((waves + 3) / 4)), dim3 (256),
-                        0, d->stream, pos4, (const T *) XT, B, NH * D * B,
+                        0, d->stream, pos4, (const T *) XT, B, zs,
                         (const typename Vec4<T>::type *) cand,
                         (const int4 *) d->candm.p, (const unsigned *) d->tbits.p, rp,
                         (T *) d->partial.p);
@@ -759,8 +774,10 @@ static int atimes_block_vt (shk_dev *d, const double *x, double *y, int B)
   if (p.nk > 0)
     {
       if (MF)
-        rc = p.poly ? recip_block_launch_mfma<V, 5> (d, (const float *) XT, B, nrecip, kpart)
-                    : recip_block_launch_mfma<V, 1> (d, (const float *) XT, B, nrecip, kpart);
+        rc = p.poly ? recip_block_launch_mfma<V, 5> (d, (const float *) XT, zs, npad, B, nrecip,
+                                                      kpart)
+                    : recip_block_launch_mfma<V, 1> (d, (const float *) XT, zs, npad, B, nrecip,
+                                                      kpart);
       else
         rc = p.poly ? recip_block_launch<V, T, 5> (d, XT, B, nrecip, kpart)
                     : recip_block_launch<V, T, 1> (d, XT, B, nrecip, kpart);
