@@ -129,11 +129,9 @@ k_real_block (const double4 *__restrict__ pos, const T *__restrict__ XT, int B, 
         {
           if ((lane >> 4) == g)
             {
-#pragma unroll
-              for (int r = 0; r < D; r++)
-#pragma unroll
-                for (int cq = 0; cq < D; cq++)
-                  L.blk[lane & 15][r * D + cq] = block_entry<V, T> (M, r, cq, false);
+              T *dst = L.blk[lane & 15];
+              auto put = [&] (int r, int cq, T v) { dst[r * D + cq] = v; };
+              BlockEach<V, 0, D, D * D>::run (M, false, put);
             }
           wave_sync ();
           const int nvalid = min (16, p.np - (j0 + g * 16));
@@ -253,9 +251,11 @@ k_recip_blk_block (const double4 *__restrict__ kv, const double *__restrict__ ks
 {
   constexpr int D = Dim<V>::D, NM = Dim<V>::NMOM;
   constexpr int NS = (NPOW > 1) ? 4 : 2;
+  // one thread per (k, power of a_i, b)
   const size_t t = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (size_t) nk * B) return;
-  const int k = (int) (t / B), b = (int) (t % B);
+  if (t >= (size_t) nk * NPOW * B) return;
+  const int k = (int) (t / ((size_t) NPOW * B));
+  const int r = (int) (t - (size_t) k * NPOW * B), pw = r / B + 1, b = r - (pw - 1) * B;
   double CS[NS * D];
 #pragma unroll
   for (int q = 0; q < NS * D; q++) CS[q] = 0.0;
@@ -267,42 +267,12 @@ k_recip_blk_block (const double4 *__restrict__ kv, const double *__restrict__ ks
     }
   const double4 kk = kv[k];
   const double *w = ks + (size_t) k * 8;
-  double Mc[NM], Ms[NM];
-  T *out = PQ + ((size_t) k * B + b) * (NPOW * 2 * D);
-  if (NPOW == 1)
-    {
-      double P[D], Q[D];
+  T *o = PQ + ((size_t) k * B + b) * (NPOW * 2 * D) + (pw - 1) * 2 * D;
+  double P[D], Q[D];
+  if (NPOW == 1) kblock_pq_mono<V> (kk, w, CS, P, Q);
+  else kblock_pq_poly_rt<V> (pw, kk, w, CS, P, Q);
 #pragma unroll
-      for (int q = 0; q < D; q++) P[q] = Q[q] = 0.0;
-      kblock_moments<V, double> (kk, w, Mc, Ms);
-      kblock_apply<V> (Mc, Ms, CS, CS + D, P, Q);
-#pragma unroll
-      for (int q = 0; q < D; q++) { out[q] = T (P[q]); out[D + q] = T (Q[q]); }
-      return;
-    }
-  const double ik = 1.0 / kk.w;
-  const double ex = kk.x * ik, ey = kk.y * ik, ez = kk.z * ik;
-  for (int pw = 1; pw <= NPOW; pw++)
-    {
-      double sc1[11], ss1[11], sc2[11], ss2[11];
-      poly_kscalars (pw, kk.w, w[6], w[7], w[3], sc1, ss1, sc2, ss2);
-      double P[D], Q[D];
-#pragma unroll
-      for (int q = 0; q < D; q++) P[q] = Q[q] = 0.0;
-#pragma unroll
-      for (int u = 0; u < NM; u++) Mc[u] = Ms[u] = 0.0;
-      moments_add<V, double> (Mc, ex, ey, ez, sc1);
-      moments_add<V, double> (Ms, ex, ey, ez, ss1);
-      kblock_apply<V> (Mc, Ms, CS, CS + D, P, Q);
-#pragma unroll
-      for (int u = 0; u < NM; u++) Mc[u] = Ms[u] = 0.0;
-      moments_add<V, double> (Mc, ex, ey, ez, sc2);
-      moments_add<V, double> (Ms, ex, ey, ez, ss2);
-      kblock_apply<V> (Mc, Ms, CS + 2 * D, CS + 3 * D, P, Q);
-      T *o = out + (pw - 1) * 2 * D;
-#pragma unroll
-      for (int q = 0; q < D; q++) { o[q] = T (P[q]); o[D + q] = T (Q[q]); }
-    }
+  for (int q = 0; q < D; q++) { o[q] = T (P[q]); o[D + q] = T (Q[q]); }
 }
 
 // back-projection, 32 i x 8 columns per workgroup; part[sk][i][q][b]

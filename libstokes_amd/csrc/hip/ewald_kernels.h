@@ -182,22 +182,6 @@ __device__ __forceinline__ void phase_cs (double ph, float &c, float &s)
   sincosf (float (red), &s, &c);
 }
 
-// Ac / As moment sets of one k (unit direction kh, paired-k weights w)
-template <int V, typename T>
-__device__ __forceinline__ void kblock_moments (const double4 kvec, const double *w,
-                                                T *Mc, T *Ms)
-{
-  constexpr int NM = Dim<V>::NMOM;
-#pragma unroll
-  for (int k = 0; k < NM; k++) { Mc[k] = T (0); Ms[k] = T (0); }
-  const T ik = T (1.0 / kvec.w);
-  const T ex = T (kvec.x) * ik, ey = T (kvec.y) * ik, ez = T (kvec.z) * ik;
-  T sc[11] = {T (0), T (w[0]), T (0), T (0), T (w[2]), T (0), T (0), T (w[4]), T (0), T (w[5]), T (0)};
-  T ss[11] = {T (0), T (0), T (w[1]), T (0), T (0), T (0), T (w[3]), T (0), T (0), T (0), T (0)};
-  moments_add<V, T> (Mc, ex, ey, ez, sc);
-  moments_add<V, T> (Ms, ex, ey, ez, ss);
-}
-
 // Polydisperse reciprocal scalars (ewald.c:985-1025) expanded in powers of
 // the radii: every scalar of pair (i, j) is a sum of c(k) a_i^p a_j^q with
 // q in {0, 2}.  For power p of a_i this returns the paired-k (x2) scalars
@@ -205,60 +189,101 @@ __device__ __forceinline__ void kblock_moments (const double4 kvec, const double
 // (sc2 / ss2), in the order xa ya yb xc yc xg yg yh xm ym zm.  The G
 // terms take their own (signed) kexp, kexpg: sign * kexp, plus the
 // reference-sign correction of another splitting where one is applied.
-__device__ __forceinline__ void poly_kscalars (int p, double k, double kk, double kexp,
-                                               double kexpg, double *sc1, double *ss1,
-                                               double *sc2, double *ss2)
+template <int P>
+__device__ __forceinline__ void poly_kscalars (double k, double kk, double kexp, double kexpg,
+                                               double *sc1, double *ss1, double *sc2, double *ss2)
 {
+#pragma unroll
   for (int t = 0; t < 11; t++) sc1[t] = ss1[t] = sc2[t] = ss2[t] = 0.0;
   const double e = 2.0 * kexp, eg = 2.0 * kexpg;
-  switch (p)
+  if (P == 1)                                  // ya: 6 a_i - kk a_i a_j^2
     {
-    case 1:                                   // ya: 6 a_i - kk a_i a_j^2
       sc1[1] = 6.0 * e;
       sc2[1] = -kk * e;
-      break;
-    case 2:                                   // yb; yg: 3k a_i^2 - 0.5 kk k a_i^2 a_j^2
+    }
+  else if (P == 2)                             // yb; yg: 3k a_i^2 - 0.5 kk k a_i^2 a_j^2
+    {
       ss1[2] = 3.0 * k * e;
       ss1[6] = 3.0 * k * eg;
       ss2[6] = -0.5 * kk * k * eg;
-      break;
-    case 3:                                   // ya, yc, yh, ym
+    }
+  else if (P == 3)                             // ya, yc, yh, ym
+    {
       sc1[1] = -kk * e;
       sc1[4] = 1.5 * kk * e;
       sc1[7] = 1.5 * kk * e;
       sc1[9] = 3.0 * kk * e;
       sc2[9] = -0.3 * kk * kk * e;
-      break;
-    case 4:                                   // yg: -0.3 kk k a_i^4
-      ss1[6] = -0.3 * kk * k * eg;
-      break;
-    default:                                  // ym: -0.3 kk^2 a_i^5
-      sc1[9] = -0.3 * kk * kk * e;
-      break;
     }
+  else if (P == 4)                             // yg: -0.3 kk k a_i^4
+    ss1[6] = -0.3 * kk * k * eg;
+  else                                         // ym: -0.3 kk^2 a_i^5
+    sc1[9] = -0.3 * kk * kk * e;
 }
 
-// P = Ac C - As S, Q = Ac S + As C for the even (Mc) / odd (Ms) blocks
+// P += Ac C - As S, Q += Ac S + As C with the even / odd blocks Ac =
+// block(e, sc), As = block(e, ss) of one k: four direct contractions
+// (contract_item, the block is linear in its scalars), no moment arrays
 template <int V>
-__device__ __forceinline__ void kblock_apply (const double *Mc, const double *Ms,
-                                              const double *C, const double *S,
-                                              double *P, double *Q)
+__device__ __forceinline__ void kblock_apply_s (double ex, double ey, double ez, const double *sc,
+                                                const double *ss, const double *C,
+                                                const double *S, double *P, double *Q)
+{
+  double nss[11];
+#pragma unroll
+  for (int t = 0; t < 11; t++) nss[t] = -ss[t];
+  contract_item<V, double> (ex, ey, ez, sc, C, P);
+  contract_item<V, double> (ex, ey, ez, nss, S, P);
+  contract_item<V, double> (ex, ey, ez, sc, S, Q);
+  contract_item<V, double> (ex, ey, ez, ss, C, Q);
+}
+
+// P, Q (D each) of one k from its structure factors CS = [C | S] for equal
+// spheres: the paired-k weights w of the k table (ya yb yc yg yh ym)
+template <int V>
+__device__ __forceinline__ void kblock_pq_mono (const double4 kvec, const double *w,
+                                                const double *CS, double *P, double *Q)
 {
   constexpr int D = Dim<V>::D;
+  const double ik = 1.0 / kvec.w;
+  const double ex = kvec.x * ik, ey = kvec.y * ik, ez = kvec.z * ik;
+  const double sc[11] = {0.0, w[0], 0.0, 0.0, w[2], 0.0, 0.0, w[4], 0.0, w[5], 0.0};
+  const double ss[11] = {0.0, 0.0, w[1], 0.0, 0.0, 0.0, w[3], 0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-  for (int pr = 0; pr < D; pr++)
+  for (int q = 0; q < D; q++) P[q] = Q[q] = 0.0;
+  kblock_apply_s<V> (ex, ey, ez, sc, ss, CS, CS + D, P, Q);
+}
+
+// P, Q of power PW (1..5) of a_i for unequal spheres: CS = [C | S | C2 | S2]
+// with the a_j^2-weighted sums second (PW a compile-time constant, so the
+// zero scalars of poly_kscalars fold away)
+template <int V, int PW>
+__device__ __forceinline__ void kblock_pq_poly (const double4 kvec, const double *w,
+                                                const double *CS, double *P, double *Q)
+{
+  constexpr int D = Dim<V>::D;
+  const double ik = 1.0 / kvec.w;
+  const double ex = kvec.x * ik, ey = kvec.y * ik, ez = kvec.z * ik;
+  double sc1[11], ss1[11], sc2[11], ss2[11];
+  poly_kscalars<PW> (kvec.w, w[6], w[7], w[3], sc1, ss1, sc2, ss2);
+#pragma unroll
+  for (int q = 0; q < D; q++) P[q] = Q[q] = 0.0;
+  kblock_apply_s<V> (ex, ey, ez, sc1, ss1, CS, CS + D, P, Q);
+  kblock_apply_s<V> (ex, ey, ez, sc2, ss2, CS + 2 * D, CS + 3 * D, P, Q);
+}
+
+// the same for a power known only at run time (1..5)
+template <int V>
+__device__ __forceinline__ void kblock_pq_poly_rt (int pw, const double4 kvec, const double *w,
+                                                   const double *CS, double *P, double *Q)
+{
+  switch (pw)
     {
-      double a = 0.0, b = 0.0;
-#pragma unroll
-      for (int q = 0; q < D; q++)
-        {
-          const double ac = block_entry<V, double> (Mc, pr, q, false);
-          const double as = block_entry<V, double> (Ms, pr, q, false);
-          a += ac * C[q] - as * S[q];
-          b += ac * S[q] + as * C[q];
-        }
-      P[pr] += a;
-      Q[pr] += b;
+    case 1: kblock_pq_poly<V, 1> (kvec, w, CS, P, Q); break;
+    case 2: kblock_pq_poly<V, 2> (kvec, w, CS, P, Q); break;
+    case 3: kblock_pq_poly<V, 3> (kvec, w, CS, P, Q); break;
+    case 4: kblock_pq_poly<V, 4> (kvec, w, CS, P, Q); break;
+    default: kblock_pq_poly<V, 5> (kvec, w, CS, P, Q); break;
     }
 }
 
@@ -333,13 +358,9 @@ k_matrix (const double4 *__restrict__ pos, const double4 *__restrict__ img,
     }
   const size_t n = (size_t) p.np * D;
   const bool self = (i == j);
-#pragma unroll
-  for (int r = 0; r < D; r++)
-    {
-      double *row = mat + (size_t) (i * D + r) * n + (size_t) j * D;
-#pragma unroll
-      for (int c = 0; c < D; c++) row[c] = block_entry<V, double> (M, r, c, self);
-    }
+  double *blk0 = mat + (size_t) i * D * n + (size_t) j * D;
+  auto put = [&] (int r, int c, double v) { blk0[(size_t) r * n + c] = v; };
+  BlockEach<V, 0, D, D * D>::run (M, self, put);
 }
 
 // y[row] = sum_col mat[row][col] * z[col], one wave per row

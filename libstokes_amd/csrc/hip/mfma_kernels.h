@@ -237,8 +237,8 @@ k_sf_mfma (const float *__restrict__ Zt, int cz, int zs, const int4 *__restrict_
     }
 }
 
-// P, Q per (k, b) from the split structure factors (FP64 k-blocks, as
-// k_recip_blocks); PQ[((k NPOW + p) 2 + s) Cp + q B + b], Cp >= D B;
+// P, Q per (k, power, b) from the split structure factors (FP64 k-blocks,
+// as k_recip_blocks); PQ[((k NPOW + p) 2 + s) Cp + q B + b], Cp >= D B;
 // rows nk <= k < nkpad are zero
 template <int V, int NPOW>
 __global__ __launch_bounds__ (128) void
@@ -249,16 +249,16 @@ k_blk_mfma (const double4 *__restrict__ kv, const double *__restrict__ ks, int n
   constexpr int NS = (NPOW > 1) ? 4 : 2;
   constexpr int NH = NS / 2;
   const size_t t = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (size_t) nkpad * B) return;
-  const int k = (int) (t / B), b = (int) (t % B);
+  if (t >= (size_t) nkpad * NPOW * B) return;
+  const int k = (int) (t / ((size_t) NPOW * B));
+  const int r = (int) (t - (size_t) k * NPOW * B), pw = r / B + 1, b = r - (pw - 1) * B;
   const size_t cz = (size_t) NH * D * B, C = (size_t) Cp;
+  float *out = PQ + ((size_t) k * NPOW + (pw - 1)) * 2 * C + b;
   if (k >= nk)
     {
       // rows past nk: zero P, Q (the back-projection runs whole quads)
-      float *out = PQ + (size_t) k * NPOW * 2 * C + b;
-      for (int r = 0; r < NPOW * 2; r++)
 #pragma unroll
-        for (int q = 0; q < D; q++) out[r * C + (size_t) q * B] = 0.f;
+      for (int q = 0; q < D; q++) out[(size_t) q * B] = out[C + (size_t) q * B] = 0.f;
       return;
     }
   // CS: [cos z | sin z | cos a^2 z | sin a^2 z], D each
@@ -278,49 +278,14 @@ k_blk_mfma (const double4 *__restrict__ kv, const double *__restrict__ ks, int n
       }
   const double4 kk = kv[k];
   const double *w = ks + (size_t) k * 8;
-  double Mc[NM], Ms[NM];
-  if (NPOW == 1)
+  double P[D], Q[D];
+  if (NPOW == 1) kblock_pq_mono<V> (kk, w, CS, P, Q);
+  else kblock_pq_poly_rt<V> (pw, kk, w, CS, P, Q);
+#pragma unroll
+  for (int q = 0; q < D; q++)
     {
-      double P[D], Q[D];
-#pragma unroll
-      for (int q = 0; q < D; q++) P[q] = Q[q] = 0.0;
-      kblock_moments<V, double> (kk, w, Mc, Ms);
-      kblock_apply<V> (Mc, Ms, CS, CS + D, P, Q);
-      float *out = PQ + (size_t) k * 2 * C + b;
-#pragma unroll
-      for (int q = 0; q < D; q++)
-        {
-          out[(size_t) q * B] = float (P[q]);
-          out[C + (size_t) q * B] = float (Q[q]);
-        }
-      return;
-    }
-  const double ik = 1.0 / kk.w;
-  const double ex = kk.x * ik, ey = kk.y * ik, ez = kk.z * ik;
-  for (int pw = 1; pw <= NPOW; pw++)
-    {
-      double sc1[11], ss1[11], sc2[11], ss2[11];
-      poly_kscalars (pw, kk.w, w[6], w[7], w[3], sc1, ss1, sc2, ss2);
-      double P[D], Q[D];
-#pragma unroll
-      for (int q = 0; q < D; q++) P[q] = Q[q] = 0.0;
-#pragma unroll
-      for (int u = 0; u < NM; u++) Mc[u] = Ms[u] = 0.0;
-      moments_add<V, double> (Mc, ex, ey, ez, sc1);
-      moments_add<V, double> (Ms, ex, ey, ez, ss1);
-      kblock_apply<V> (Mc, Ms, CS, CS + D, P, Q);
-#pragma unroll
-      for (int u = 0; u < NM; u++) Mc[u] = Ms[u] = 0.0;
-      moments_add<V, double> (Mc, ex, ey, ez, sc2);
-      moments_add<V, double> (Ms, ex, ey, ez, ss2);
-      kblock_apply<V> (Mc, Ms, CS + 2 * D, CS + 3 * D, P, Q);
-      float *out = PQ + ((size_t) k * NPOW + (pw - 1)) * 2 * C + b;
-#pragma unroll
-      for (int q = 0; q < D; q++)
-        {
-          out[(size_t) q * B] = float (P[q]);
-          out[C + (size_t) q * B] = float (Q[q]);
-        }
+      out[(size_t) q * B] = float (P[q]);
+      out[C + (size_t) q * B] = float (Q[q]);
     }
 }
 
@@ -437,24 +402,6 @@ k_back_mfma (const double4 *__restrict__ pos, const int4 *__restrict__ km,
     }
 }
 
-// entries (r, q), r < D, q0 <= q < q0 + NQ, of the D x D block of one pair
-// from its moments, stored at dst[r * LD + 4 (q - q0)]; a template
-// recursion keeps every moment index a compile-time constant, so the
-// moments stay in registers
-template <int V, int Q0, int NQ, int E> struct BlockStore {
-  static constexpr int D = Dim<V>::D;
-  template <int LD, typename T>
-  __device__ __forceinline__ static void run (const T *M, T *dst)
-  {
-    BlockStore<V, Q0, NQ, E - 1>::template run<LD, T> (M, dst);
-    constexpr int r = (E - 1) / NQ, u = (E - 1) % NQ;
-    dst[r * LD + 4 * u] = block_entry<V, T> (M, r, Q0 + u, false);
-  }
-};
-template <int V, int Q0, int NQ> struct BlockStore<V, Q0, NQ, 0> {
-  template <int LD, typename T> __device__ __forceinline__ static void run (const T *, T *) {}
-};
-
 // real-space pair sum for a block of columns.  One wave per (16 target
 // particles i, range of source particles j, 16 CT columns).  Per chunk of 4
 // source particles:
@@ -480,7 +427,9 @@ template <int V, int CT, int RND> struct RealRounds {
   __device__ __forceinline__ static void run (const float *M, float *blk, const float *zs_l,
                                               int ir, int kq, mf4 (&acc)[Dim<V>::D][CT])
   {
-    BlockStore<V, Q0, NQ, D * NQ>::template run<LDA, float> (M, blk + ir * D * LDA + kq);
+    float *dst = blk + ir * D * LDA + kq;
+    auto put = [&] (int r, int q, float v) { dst[r * LDA + 4 * (q - Q0)] = v; };
+    BlockEach<V, Q0, NQ, D * NQ>::run (M, false, put);
     __syncthreads ();
 #pragma unroll
     for (int u = 0; u < NQ; u++)

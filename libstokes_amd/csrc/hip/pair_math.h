@@ -450,6 +450,23 @@ SHK_HD T block_entry (const T *M, int p, int q, bool self)
   }
 }
 
+// f(r, q, entry) for the block entries r < D, Q0 <= q < Q0 + NQ, in row
+// order, by a template recursion: every moment index is a compile-time
+// constant after inlining, so a moment array stays in registers
+template <int V, int Q0, int NQ, int E> struct BlockEach {
+  template <typename T, typename F>
+  SHK_HD static __attribute__ ((always_inline)) void run (const T *M, bool self, F &f)
+  {
+    BlockEach<V, Q0, NQ, E - 1>::run (M, self, f);
+    constexpr int r = (E - 1) / NQ, q = Q0 + (E - 1) % NQ;
+    f (r, q, block_entry<V, T> (M, r, q, self));
+  }
+};
+template <int V, int Q0, int NQ> struct BlockEach<V, Q0, NQ, 0> {
+  template <typename T, typename F>
+  SHK_HD static __attribute__ ((always_inline)) void run (const T *, bool, F &) {}
+};
+
 // extracted-form S transform of fts.c:660-670 applied to one 11-vector
 template <typename T>
 SHK_HD void s_transform (T *z)

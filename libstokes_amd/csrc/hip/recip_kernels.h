@@ -89,10 +89,12 @@ __global__ __launch_bounds__ (128) void
 k_recip_blocks (const double4 *__restrict__ kv, const double *__restrict__ ks, int nk,
                 int nsplit, double gsign, const T *__restrict__ part, T *__restrict__ PQ)
 {
-  constexpr int D = Dim<V>::D, NM = Dim<V>::NMOM;
+  constexpr int D = Dim<V>::D;
   constexpr int NS = (NPOW > 1) ? 4 : 2;
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= nk) return;
+  // one thread per (k, power of a_i)
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nk * NPOW) return;
+  const int k = t / NPOW, pw = t - k * NPOW + 1;
   double CS[NS * D];
 #pragma unroll
   for (int q = 0; q < NS * D; q++) CS[q] = 0.0;
@@ -105,42 +107,20 @@ k_recip_blocks (const double4 *__restrict__ kv, const double *__restrict__ ks, i
     }
   const double4 kk = kv[k];
   const double *w = ks + (size_t) k * 8;
-  double Mc[NM], Ms[NM];
   if (NPOW == 1)
     {
       double P[D], Q[D];
-#pragma unroll
-      for (int q = 0; q < D; q++) P[q] = Q[q] = 0.0;
-      kblock_moments<V, double> (kk, w, Mc, Ms);
-      kblock_apply<V> (Mc, Ms, CS, CS + D, P, Q);
+      kblock_pq_mono<V> (kk, w, CS, P, Q);
       T *out = PQ + (size_t) k * 2 * D;
 #pragma unroll
       for (int q = 0; q < D; q++) { out[q] = T (P[q]); out[D + q] = T (Q[q]); }
       return;
     }
-  const double ik = 1.0 / kk.w;
-  const double ex = kk.x * ik, ey = kk.y * ik, ez = kk.z * ik;
-  for (int pw = 1; pw <= NPOW; pw++)
-    {
-      double sc1[11], ss1[11], sc2[11], ss2[11];
-      poly_kscalars (pw, kk.w, w[6], w[7], w[3], sc1, ss1, sc2, ss2);
-      double P[D], Q[D];
+  double P[D], Q[D];
+  kblock_pq_poly_rt<V> (pw, kk, w, CS, P, Q);
+  T *out = PQ + (((size_t) k * NPOW + (pw - 1)) * 2) * D;
 #pragma unroll
-      for (int q = 0; q < D; q++) P[q] = Q[q] = 0.0;
-#pragma unroll
-      for (int t = 0; t < NM; t++) Mc[t] = Ms[t] = 0.0;
-      moments_add<V, double> (Mc, ex, ey, ez, sc1);
-      moments_add<V, double> (Ms, ex, ey, ez, ss1);
-      kblock_apply<V> (Mc, Ms, CS, CS + D, P, Q);
-#pragma unroll
-      for (int t = 0; t < NM; t++) Mc[t] = Ms[t] = 0.0;
-      moments_add<V, double> (Mc, ex, ey, ez, sc2);
-      moments_add<V, double> (Ms, ex, ey, ez, ss2);
-      kblock_apply<V> (Mc, Ms, CS + 2 * D, CS + 3 * D, P, Q);
-      T *out = PQ + (((size_t) k * NPOW + (pw - 1)) * 2) * D;
-#pragma unroll
-      for (int q = 0; q < D; q++) { out[q] = T (P[q]); out[D + q] = T (Q[q]); }
-    }
+  for (int q = 0; q < D; q++) { out[q] = T (P[q]); out[D + q] = T (Q[q]); }
 }
 
 template <int V, typename T, int NPOW>

@@ -461,7 +461,7 @@ static int recip_launch (shk_dev *d, const double *x, int &nrecip, const void *&
                       d->stream, pos4, x, km, (const Cplx<T> *) d->ptab.p, b, p.np, nk, jper, jt,
                       (T *) d->kpart.p);
   HIPCHK (hipGetLastError ());
-  hipLaunchKernelGGL ((k_recip_blocks<V, T, NPOW>), dim3 ((nk + 127) / 128), dim3 (128), 0,
+  hipLaunchKernelGGL ((k_recip_blocks<V, T, NPOW>), dim3 ((nk * NPOW + 127) / 128), dim3 (128), 0,
                       d->stream, kv, (const double *) d->ks.p, nk, sj, p.gsign,
                       (const T *) d->kpart.p, (T *) d->kpq.p);
   HIPCHK (hipGetLastError ());
@@ -575,7 +575,7 @@ static int recip_block_launch (shk_dev *d, const T *XT, int B, int &nrecip,
   hipLaunchKernelGGL ((k_recip_sf_block<V, T, NPOW>), dim3 ((nk + 31) / 32, nbg, sj), dim3 (256),
                       0, d->stream, pos4, XT, B, kv, p.np, nk, jper, (T *) d->kpart.p);
   HIPCHK (hipGetLastError ());
-  const size_t nkb = (size_t) nk * B;
+  const size_t nkb = (size_t) nk * NPOW * B;
   hipLaunchKernelGGL ((k_recip_blk_block<V, T, NPOW>), dim3 ((unsigned) ((nkb + 127) / 128)),
                       dim3 (128), 0, d->stream, kv, (const double *) d->ks.p, nk, B, sj, p.gsign,
                       (const T *) d->kpart.p, (T *) d->kpq.p);
@@ -671,7 +671,7 @@ static int recip_block_launch_mfma (shk_dev *d, const float *Zt, int zs, int npa
     hipLaunchKernelGGL ((k_sf_mfma<128>), dim3 (ncz, nkb, sj), dim3 (256), lds_sf, d->stream, Zt,
                         cz, zs, km, tab, npad, b, nk, jper, (float *) d->kpart.p);
   HIPCHK (hipGetLastError ());
-  const size_t nkB = (size_t) nkpad * B;
+  const size_t nkB = (size_t) nkpad * NPOW * B;
   hipLaunchKernelGGL ((k_blk_mfma<V, NPOW>), dim3 ((unsigned) ((nkB + 127) / 128)), dim3 (128),
                       0, d->stream, kv, (const double *) d->ks.p, nk, nkpad, B, sj, Cp,
                       (const float *) d->kpart.p, (float *) d->kpq.p);
