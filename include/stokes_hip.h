@@ -134,6 +134,19 @@ int stokes_hip_solve_mob_3fts_dev (struct stokes *sys, const double *fte_dev,
                                    double *uos_dev, int *iters,
                                    double *relres);
 
+/* The same route for the mixed problem with the fixed particles
+ * sys->nm..np-1 (solve_mix_3fts_0, ewald-3fts.c:531-593): b_dev[11 np]
+ * holds F, T, E of the mobile and U, O, E of the fixed particles, x_dev
+ * receives U, O, S and F, T, S (the layout of stokes_hip_solve_mix_dev). */
+int stokes_hip_solve_mix_3fts_dev (struct stokes *sys, const double *b_dev,
+                                   double *x_dev, int *iters, double *relres);
+
+/* The resistance problem M (F, T, S) = (U, O, E) by GMRES
+ * (solve_res_3fts_0, ewald-3fts.c:50-80): uoe_dev[11 np] in, fts_dev out. */
+int stokes_hip_solve_res_3fts_dev (struct stokes *sys, const double *uoe_dev,
+                                   double *fts_dev, int *iters,
+                                   double *relres);
+
 /* Chebyshev-Clenshaw z = f(M) y for nrhs vectors held on the device
  * (chebyshev.c:163-228 with the whole recursion on the GPU) */
 int stokes_hip_cheb_eval_dev (struct stokes *sys, int ncheb, const double *a,

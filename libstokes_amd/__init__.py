@@ -56,7 +56,11 @@ EXPORTS = [
     "stokes_hip_cheb_eval_dev", "stokes_hip_lanczos_bounds",
     "stokes_hip_adjoint_dev", "stokes_hip_scalars_real_f_deriv", "stokes_hip_solve_mix_dev", "solve_mob_3fts_matrix_0",
     "solve_mix_3fts_matrix_0", "stokes_hip_solve_mob_3fts_dev",
-    "solve_mob_3fts_0",
+    "solve_mob_3fts_0", "stokes_hip_solve_mix_3fts_dev",
+    "stokes_hip_solve_res_3fts_dev", "solve_res_3fts_0", "solve_res_3fts",
+    "solve_mob_3fts", "solve_mix_3fts_0", "solve_mix_3fts",
+    "shift_labo_to_rest_U", "shift_rest_to_labo_U", "shift_labo_to_rest_O",
+    "shift_rest_to_labo_O", "shift_labo_to_rest_E",
     "stokes_hip_ensemble_create", "stokes_hip_ensemble_atimes",
     "stokes_hip_ensemble_refresh", "stokes_hip_ensemble_offset",
     "stokes_hip_ensemble_destroy",
@@ -140,6 +144,9 @@ def lib():
     L.stokes_set_np.argtypes = [sp, C.c_int, C.c_int]
     L.stokes_set_l.argtypes = [sp, C.c_double, C.c_double, C.c_double]
     L.stokes_set_xi.argtypes = [sp, C.c_double, C.c_double]
+    L.stokes_set_Ui.argtypes = [sp] + [C.c_double] * 3
+    L.stokes_set_Oi.argtypes = [sp] + [C.c_double] * 3
+    L.stokes_set_Ei.argtypes = [sp] + [C.c_double] * 5
     L.xi_by_tratio.restype = C.c_double
     L.xi_by_tratio.argtypes = [sp, C.c_double]
     L.stokes_set_pos.argtypes = [sp, _dp]
@@ -156,6 +163,15 @@ def lib():
                                                 C.POINTER(C.c_int), _dp]
     L.solve_mob_3fts_0.argtypes = [sp, _vp] + [_dp] * 6
     L.solve_mob_3fts_0.restype = None
+    for f in ("stokes_hip_solve_mix_3fts_dev", "stokes_hip_solve_res_3fts_dev"):
+        getattr(L, f).argtypes = [sp, _vp, _vp, C.POINTER(C.c_int), _dp]
+    for f, a in (("solve_res_3fts_0", [sp] + [_dp] * 6),
+                 ("solve_res_3fts", [sp] + [_dp] * 6),
+                 ("solve_mob_3fts", [sp] + [_dp] * 6),
+                 ("solve_mix_3fts_0", [sp, _vp] + [_dp] * 12),
+                 ("solve_mix_3fts", [sp] + [_dp] * 12)):
+        getattr(L, f).argtypes = a
+        getattr(L, f).restype = None
     L.stokes_set_iter.argtypes = [sp, C.c_char_p, C.c_int, C.c_int,
                                   C.c_double, C.c_int, _vp]
     L.stokes_hip_adjoint_dev.argtypes = [sp, _vp, _vp, _vp, _vp]
@@ -355,6 +371,21 @@ class Stokes:
                                                    C.byref(it),
                                                    C.byref(rr)),
               "solve_mob_3fts")
+        return xd.to_host(), it.value, rr.value
+
+    def solve_gmres(self, b, kind="mix", nm=None):
+        """device GMRES: kind "mix" (b per particle: mobile F T E, fixed
+        U O E -> mobile U O S, fixed F T S; nm mobile particles) or "res"
+        (U O E -> F T S); returns (x, iterations, relative residual)"""
+        if nm is not None:
+            self.s.nm = nm
+        bd = DeviceArray.from_host(np.ascontiguousarray(b, np.float64))
+        xd = DeviceArray(bd.n)
+        it, rr = C.c_int(), C.c_double()
+        f = (self.L.stokes_hip_solve_res_3fts_dev if kind == "res"
+             else self.L.stokes_hip_solve_mix_3fts_dev)
+        check(f(self.sys, bd.ptr, xd.ptr, C.byref(it), C.byref(rr)),
+              "solve_" + kind)
         return xd.to_host(), it.value, rr.value
 
     def set_iter(self, solver="gmres", max_it=2000, restart=30, eps=1e-12):

@@ -45,27 +45,31 @@ __global__ void k_axpby (double *__restrict__ y, const double *__restrict__ x, d
   if (k < n) y[k] = a * x[k] + b * y[k];
 }
 
-// per particle of 11: lo = (x_0..5, 0), hi = (0, x_6..10); either may be NULL
-__global__ void k_fts_split (const double *__restrict__ x, int np, double *__restrict__ lo,
-                             double *__restrict__ hi)
+// per particle of 11, mobile when i < nm: lo = (x_0..5, 0) for mobile and
+// 0 for fixed particles, hi = (0, x_6..10) for mobile and x for fixed
+// particles; either may be NULL
+__global__ void k_fts_split (const double *__restrict__ x, int np, int nm,
+                             double *__restrict__ lo, double *__restrict__ hi)
 {
   const size_t t = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (size_t) np * 11) return;
   const int q = (int) (t % 11);
+  const bool mob = (int) (t / 11) < nm;
   const double v = x[t];
-  if (lo) lo[t] = q < 6 ? v : 0.0;
-  if (hi) hi[t] = q < 6 ? 0.0 : v;
+  if (lo) lo[t] = mob && q < 6 ? v : 0.0;
+  if (hi) hi[t] = mob && q < 6 ? 0.0 : v;
 }
 
 // y = lo(a) - z (mode 0) or y = z - hi(a) (mode 1), per particle of 11
 __global__ void k_fts_combine (const double *__restrict__ a, const double *__restrict__ z,
-                               int np, int mode, double *__restrict__ y)
+                               int np, int nm, int mode, double *__restrict__ y)
 {
   const size_t t = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (size_t) np * 11) return;
   const int q = (int) (t % 11);
-  if (mode == 0) y[t] = (q < 6 ? a[t] : 0.0) - z[t];
-  else y[t] = z[t] - (q < 6 ? 0.0 : a[t]);
+  const bool mob = (int) (t / 11) < nm;
+  if (mode == 0) y[t] = (mob && q < 6 ? a[t] : 0.0) - z[t];
+  else y[t] = z[t] - (mob && q < 6 ? 0.0 : a[t]);
 }
 
 }  // namespace
@@ -112,23 +116,24 @@ extern "C" int shk_dev_vh (shk_dev *d, const double *V, const double *h_dev, int
   return SHK_OK;
 }
 
-extern "C" int shk_dev_fts_split (shk_dev *d, const double *x, int np, double *lo, double *hi)
+extern "C" int shk_dev_fts_split (shk_dev *d, const double *x, int np, int nm, double *lo,
+                                  double *hi)
 {
   hipStream_t s = (hipStream_t) shk_dev_stream (d);
   const size_t n = (size_t) np * 11;
   hipLaunchKernelGGL (k_fts_split, dim3 ((unsigned) ((n + 255) / 256)), dim3 (256), 0, s, x, np,
-                      lo, hi);
+                      nm, lo, hi);
   CHK (hipGetLastError ());
   return SHK_OK;
 }
 
 extern "C" int shk_dev_fts_combine (shk_dev *d, const double *a, const double *z, int np,
-                                    int mode, double *y)
+                                    int nm, int mode, double *y)
 {
   hipStream_t s = (hipStream_t) shk_dev_stream (d);
   const size_t n = (size_t) np * 11;
   hipLaunchKernelGGL (k_fts_combine, dim3 ((unsigned) ((n + 255) / 256)), dim3 (256), 0, s, a,
-                      z, np, mode, y);
+                      z, np, nm, mode, y);
   CHK (hipGetLastError ());
   return SHK_OK;
 }

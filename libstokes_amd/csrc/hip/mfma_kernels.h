@@ -516,21 +516,22 @@ k_real_mfma (const double4 *__restrict__ pos, const float *__restrict__ XT, int 
         {
           const int cn = min (64, p.ncand - c0);
           unsigned long long mask = 0ull;
-#pragma unroll 4
+#pragma unroll 8
           for (int c = 0; c < cn; c++)
             {
               const float4 v = cand[c0 + c];
-              const int4 mm = candm[c0 + c];
               const float ex = mdx + v.x, ey = mdy + v.y, ez = mdz + v.z;
               const float rr = ex * ex + ey * ey + ez * ez;
-              const bool hit = valid && rr != 0.f && rr <= rmax2
-                && in_table (p, bits, mm.x - w1, mm.y - w2, mm.z - w3);
+              const bool hit = valid && rr != 0.f && rr <= rmax2;
               mask |= (unsigned long long) hit << c;
             }
+          // the reference-table test only for the images inside r_max
           while (mask)
             {
               const int c = __builtin_ctzll (mask);
               mask &= mask - 1ull;
+              const int4 mm = candm[c0 + c];
+              if (!in_table (p, bits, mm.x - w1, mm.y - w2, mm.z - w3)) continue;
               const float4 v = cand[c0 + c];
               const float ex = mdx + v.x, ey = mdy + v.y, ez = mdz + v.z;
               const float rr = ex * ex + ey * ey + ez * ez;

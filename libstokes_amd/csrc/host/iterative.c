@@ -1,14 +1,18 @@
-/* iterative.c -- restarted GMRES on device vectors and the FTS natural
- * mobility problem solved with it (the libiter route of
- * solve_mob_3fts_0, ewald-3fts.c:263-305, with the operator of
- * atimes_mob_3fts, 200-253, and the right-hand side of calc_b_mob_3fts,
- * 145-188).
- *
- * Unknowns x = (U, O, S) per particle; A x = (U, O, 0) - M (0, 0, S);
- * b = M (F, T, 0) - (0, 0, E).  GMRES(m): Arnoldi with classical
- * Gram-Schmidt applied twice (basis products on the GPU, the small
- * Hessenberg problem and Givens rotations on the host), restarts until the
- * relative residual |b - A x| / |b| falls below eps or max iterations.
+/* iterative.c -- restarted GMRES on device vectors and the FTS problems
+ * solved with it, the libiter route of ewald-3fts.c:
+ *   resistance  M (F, T, S) = (U, O, E)                     (:50-143)
+ *   mobility    unknowns (U, O, S) from (F, T, E)           (:146-360)
+ *   mixed       mobile particles as the mobility problem, fixed ones
+ *               with unknowns (F, T, S) from (U, O, E)      (:363-670)
+ * For the mixed problem (the mobility problem is its nm = np case) the
+ * unknowns are x = [(U, O, S)_mobile, (F, T, S)_fixed],
+ *   A x = [(U, O, 0)_m, 0_f] - M [(0, 0, S)_m, (F, T, S)_f]   (:440-525)
+ *   b   = M [(F, T, 0)_m, 0_f] - [(0, 0, E)_m, (U, O, E)_f]   (:363-438)
+ * GMRES(m): Arnoldi with classical Gram-Schmidt applied twice (basis
+ * products on the GPU, the small Hessenberg problem and Givens rotations on
+ * the host), restarts until the relative residual |b - A x| / |b| falls
+ * below eps or max iterations.  The labo-frame wrappers shift by the
+ * imposed flow (Ui, Oi, Ei) as f.c:569-660, ft.c:901-960, fts.c:2303-2330.
  */
 #include <math.h>
 #include <stdio.h>
@@ -133,19 +137,28 @@ gmres_dev (shk_dev *d, size_t n, dev_op op, void *ud, const double *b,
 
 typedef struct {
   shk_dev *d;
-  int np, prec;
+  int np, nm, prec;
   double *t1, *t2;
-} mob_op;
+} mix_op;
 
-/* A x = (U, O, 0) - M (0, 0, S)  (atimes_mob_3fts, ewald-3fts.c:200-253) */
+/* A x = [(U, O, 0)_m, 0_f] - M [(0, 0, S)_m, (F, T, S)_f]
+ * (atimes_mix_3fts, ewald-3fts.c:440-525; atimes_mob_3fts for nm = np) */
 static int
-mob_apply (void *ud, const double *x, double *y)
+mix_apply (void *ud, const double *x, double *y)
 {
-  mob_op *m = ud;
-  int rc = shk_dev_fts_split (m->d, x, m->np, NULL, m->t1);
+  mix_op *m = ud;
+  int rc = shk_dev_fts_split (m->d, x, m->np, m->nm, NULL, m->t1);
   if (!rc) rc = shk_dev_atimes (m->d, m->t1, m->t2, 1, m->prec);
-  if (!rc) rc = shk_dev_fts_combine (m->d, x, m->t2, m->np, 0, y);
+  if (!rc) rc = shk_dev_fts_combine (m->d, x, m->t2, m->np, m->nm, 0, y);
   return rc;
+}
+
+/* A x = M x (the resistance problem, ewald-3fts.c:50-80) */
+static int
+res_apply (void *ud, const double *x, double *y)
+{
+  mix_op *m = ud;
+  return shk_dev_atimes (m->d, x, y, 1, m->prec);
 }
 
 static void
@@ -165,19 +178,27 @@ iter_params (const struct stokes *sys, const struct iter *it, int *m,
   if (*m > GMRES_MAX_RESTART) *m = GMRES_MAX_RESTART;
 }
 
-int
-stokes_hip_solve_mob_3fts_dev (struct stokes *sys, const double *fte_dev,
-                               double *uos_dev, int *iters, double *relres)
+/* kind 0: resistance, b_dev = (U, O, E) per particle -> x = (F, T, S);
+ * kind 1: mixed with nm mobile particles, b_dev = [(F, T, E)_m,
+ * (U, O, E)_f] -> x = [(U, O, S)_m, (F, T, S)_f] */
+static int
+solve_fts_dev (struct stokes *sys, int kind, int nm, const double *b_dev,
+               double *x_dev, int *iters, double *relres, const char *what)
 {
-  if (sys == NULL || fte_dev == NULL || uos_dev == NULL)
+  if (sys == NULL || b_dev == NULL || x_dev == NULL)
     {
-      stokes_hip_set_error ("solve_mob_3fts: NULL argument");
+      stokes_hip_set_error ("%s: NULL argument", what);
       return STOKES_HIP_EARG;
     }
   if (sys->version != 2)
     {
-      stokes_hip_set_error ("solve_mob_3fts: FTS version only (version %d)",
+      stokes_hip_set_error ("%s: FTS version only (version %d)", what,
                             sys->version);
+      return STOKES_HIP_EARG;
+    }
+  if (nm < 0 || nm > sys->np)
+    {
+      stokes_hip_set_error ("%s: nm = %d outside [0, %d]", what, nm, sys->np);
       return STOKES_HIP_EARG;
     }
   stokes_ctx *c = stokes_hip_ctx (sys);
@@ -193,25 +214,31 @@ stokes_hip_solve_mob_3fts_dev (struct stokes *sys, const double *fte_dev,
   rc = shk_dev_malloc (c->dev, (void **) &buf, sizeof (double) * nd);
   if (rc)
     {
-      stokes_hip_set_error ("solve_mob_3fts: %s", shk_last_error ());
+      stokes_hip_set_error ("%s: %s", what, shk_last_error ());
       return STOKES_HIP_ENOMEM;
     }
   double *V = buf, *w = V + n * (size_t) (m + 1), *b = w + n, *t1 = b + n,
     *t2 = t1 + n, *hd = t2 + n;
-  mob_op op = {c->dev, sys->np, c->prec, t1, t2};
-  /* b = M (F, T, 0) - (0, 0, E)  (calc_b_mob_3fts, ewald-3fts.c:145-188) */
-  rc = shk_dev_fts_split (c->dev, fte_dev, sys->np, t1, NULL);
-  if (!rc) rc = shk_dev_atimes (c->dev, t1, t2, 1, c->prec);
-  if (!rc) rc = shk_dev_fts_combine (c->dev, fte_dev, t2, sys->np, 1, b);
+  mix_op op = {c->dev, sys->np, nm, c->prec, t1, t2};
+  if (kind == 0)
+    rc = shk_dev_scal (c->dev, b, b_dev, 1.0, n);
+  else
+    {
+      /* b = M [(F, T, 0)_m, 0_f] - [(0, 0, E)_m, (U, O, E)_f]
+       * (calc_b_mix_3fts, ewald-3fts.c:363-438) */
+      rc = shk_dev_fts_split (c->dev, b_dev, sys->np, nm, t1, NULL);
+      if (!rc) rc = shk_dev_atimes (c->dev, t1, t2, 1, c->prec);
+      if (!rc) rc = shk_dev_fts_combine (c->dev, b_dev, t2, sys->np, nm, 1, b);
+    }
   int it = 0;
   double rr = 0.0;
   if (!rc)
-    rc = gmres_dev (c->dev, n, mob_apply, &op, b, uos_dev, m, maxit, eps, V, w,
-                    hd, &it, &rr);
+    rc = gmres_dev (c->dev, n, kind == 0 ? res_apply : mix_apply, &op, b,
+                    x_dev, m, maxit, eps, V, w, hd, &it, &rr);
   shk_dev_free (c->dev, buf);
   if (rc)
     {
-      stokes_hip_set_error ("solve_mob_3fts: %s", shk_last_error ());
+      stokes_hip_set_error ("%s: %s", what, shk_last_error ());
       return rc;
     }
   if (iters) *iters = it;
@@ -219,53 +246,266 @@ stokes_hip_solve_mob_3fts_dev (struct stokes *sys, const double *fte_dev,
   return 0;
 }
 
-void
-solve_mob_3fts_0 (struct stokes *sys, struct iter *iter, const double *f,
-                  const double *t, const double *e, double *u, double *o,
-                  double *s)
+int
+stokes_hip_solve_mob_3fts_dev (struct stokes *sys, const double *fte_dev,
+                               double *uos_dev, int *iters, double *relres)
+{
+  return solve_fts_dev (sys, 1, sys ? sys->np : 0, fte_dev, uos_dev, iters,
+                        relres, "solve_mob_3fts");
+}
+
+int
+stokes_hip_solve_mix_3fts_dev (struct stokes *sys, const double *b_dev,
+                               double *x_dev, int *iters, double *relres)
+{
+  return solve_fts_dev (sys, 1, sys ? sys->nm : 0, b_dev, x_dev, iters,
+                        relres, "solve_mix_3fts");
+}
+
+int
+stokes_hip_solve_res_3fts_dev (struct stokes *sys, const double *uoe_dev,
+                               double *fts_dev, int *iters, double *relres)
+{
+  return solve_fts_dev (sys, 0, sys ? sys->np : 0, uoe_dev, fts_dev, iters,
+                        relres, "solve_res_3fts");
+}
+
+/* ---- reference entry points (host arrays; void, exit(1) on failure) ---- */
+
+static void
+check_fts (struct stokes *sys, const char *name)
 {
   if (sys->version != 2)
     {
-      fprintf (stderr, "libstokes solve_mob_3fts_0 :"
-               " the version is wrong. reset to FTS\n");
+      fprintf (stderr, "libstokes %s : the version is wrong. reset to FTS\n",
+               name);
       sys->version = 2;
     }
+}
+
+/* pack per-particle blocks of 3, 3, 5 from three arrays into h[11 n] */
+static void
+pack_335 (int n, double *h, const double *a, const double *b, const double *c)
+{
+  for (int i = 0; i < n; i++)
+    {
+      memcpy (h + 11 * i, a + 3 * i, sizeof (double) * 3);
+      memcpy (h + 11 * i + 3, b + 3 * i, sizeof (double) * 3);
+      memcpy (h + 11 * i + 6, c + 5 * i, sizeof (double) * 5);
+    }
+}
+
+static void
+unpack_335 (int n, const double *h, double *a, double *b, double *c)
+{
+  for (int i = 0; i < n; i++)
+    {
+      memcpy (a + 3 * i, h + 11 * i, sizeof (double) * 3);
+      memcpy (b + 3 * i, h + 11 * i + 3, sizeof (double) * 3);
+      memcpy (c + 5 * i, h + 11 * i + 6, sizeof (double) * 5);
+    }
+}
+
+/* solve on host vectors h[11 np] (in: b, out: x) with sys->it replaced by
+ * iter when not NULL */
+static void
+solve_host (struct stokes *sys, struct iter *iter, int kind, int nm,
+            double *h, const char *name)
+{
   struct iter *keep = sys->it;
   if (iter != NULL) sys->it = iter;
-  int np = sys->np;
-  size_t n = (size_t) np * 11;
-  double *h = malloc (sizeof (double) * n);
+  size_t n = (size_t) sys->np * 11;
   double *bd = NULL, *xd = NULL;
-  if (h == NULL)
-    {
-      fprintf (stderr, "allocation failed in solve_mob_3fts_0\n");
-      exit (1);
-    }
-  for (int i = 0; i < np; i++)
-    {
-      memcpy (h + 11 * i, f + 3 * i, sizeof (double) * 3);
-      memcpy (h + 11 * i + 3, t + 3 * i, sizeof (double) * 3);
-      memcpy (h + 11 * i + 6, e + 5 * i, sizeof (double) * 5);
-    }
   int rc = stokes_hip_malloc ((void **) &bd, sizeof (double) * n);
   if (!rc) rc = stokes_hip_malloc ((void **) &xd, sizeof (double) * n);
   if (!rc) rc = stokes_hip_memcpy_h2d (bd, h, sizeof (double) * n);
-  if (!rc) rc = stokes_hip_solve_mob_3fts_dev (sys, bd, xd, NULL, NULL);
+  if (!rc) rc = solve_fts_dev (sys, kind, nm, bd, xd, NULL, NULL, name);
   if (!rc) rc = stokes_hip_memcpy_d2h (h, xd, sizeof (double) * n);
   if (bd) stokes_hip_free (bd);
   if (xd) stokes_hip_free (xd);
   sys->it = keep;
   if (rc)
     {
-      fprintf (stderr, "libstokes-hip: solve_mob_3fts_0 failed (%d): %s\n",
-               rc, stokes_hip_last_error ());
+      fprintf (stderr, "libstokes-hip: %s failed (%d): %s\n", name, rc,
+               stokes_hip_last_error ());
       exit (1);
     }
-  for (int i = 0; i < np; i++)
+}
+
+static double *
+alloc_h (size_t n, const char *name)
+{
+  double *h = malloc (sizeof (double) * n);
+  if (h == NULL)
     {
-      memcpy (u + 3 * i, h + 11 * i, sizeof (double) * 3);
-      memcpy (o + 3 * i, h + 11 * i + 3, sizeof (double) * 3);
-      memcpy (s + 5 * i, h + 11 * i + 6, sizeof (double) * 5);
+      fprintf (stderr, "allocation failed in %s\n", name);
+      exit (1);
     }
+  return h;
+}
+
+void
+solve_res_3fts_0 (struct stokes *sys, const double *u, const double *o,
+                  const double *e, double *f, double *t, double *s)
+{
+  check_fts (sys, "solve_res_3fts_0");
+  double *h = alloc_h ((size_t) sys->np * 11, "solve_res_3fts_0");
+  pack_335 (sys->np, h, u, o, e);
+  solve_host (sys, NULL, 0, sys->np, h, "solve_res_3fts_0");
+  unpack_335 (sys->np, h, f, t, s);
   free (h);
+}
+
+void
+solve_mob_3fts_0 (struct stokes *sys, struct iter *iter, const double *f,
+                  const double *t, const double *e, double *u, double *o,
+                  double *s)
+{
+  check_fts (sys, "solve_mob_3fts_0");
+  double *h = alloc_h ((size_t) sys->np * 11, "solve_mob_3fts_0");
+  pack_335 (sys->np, h, f, t, e);
+  solve_host (sys, iter, 1, sys->np, h, "solve_mob_3fts_0");
+  unpack_335 (sys->np, h, u, o, s);
+  free (h);
+}
+
+void
+solve_mix_3fts_0 (struct stokes *sys, struct iter *iter, const double *f,
+                  const double *t, const double *e, const double *uf,
+                  const double *of, const double *ef, double *u, double *o,
+                  double *s, double *ff, double *tf, double *sf)
+{
+  check_fts (sys, "solve_mix_3fts_0");
+  int np = sys->np, nm = sys->nm, nf = np - nm;
+  if (nm == np)
+    {
+      solve_mob_3fts_0 (sys, iter, f, t, e, u, o, s);
+      return;
+    }
+  double *h = alloc_h ((size_t) np * 11, "solve_mix_3fts_0");
+  pack_335 (nm, h, f, t, e);
+  pack_335 (nf, h + 11 * (size_t) nm, uf, of, ef);
+  solve_host (sys, iter, 1, nm, h, "solve_mix_3fts_0");
+  unpack_335 (nm, h, u, o, s);
+  unpack_335 (nf, h + 11 * (size_t) nm, ff, tf, sf);
+  free (h);
+}
+
+/* ---- imposed flow: labo frame <-> fluid-rest frame ---- */
+
+/* imposed velocity Ui + Oi x pos_i + Ei . pos_i at particle i (Ei in the
+ * extracted form xx xy xz yz yy, zz = -xx - yy) */
+static void
+imposed_u (const struct stokes *sys, int i, double *v)
+{
+  const double *E = sys->Ei, Ezz = -sys->Ei[0] - sys->Ei[4];
+  const double *x = sys->pos + 3 * (size_t) i;
+  v[0] = sys->Ui[0] + sys->Oi[1] * x[2] - sys->Oi[2] * x[1]
+    + E[0] * x[0] + E[1] * x[1] + E[2] * x[2];
+  v[1] = sys->Ui[1] + sys->Oi[2] * x[0] - sys->Oi[0] * x[2]
+    + E[1] * x[0] + E[4] * x[1] + E[3] * x[2];
+  v[2] = sys->Ui[2] + sys->Oi[0] * x[1] - sys->Oi[1] * x[0]
+    + E[2] * x[0] + E[3] * x[1] + Ezz * x[2];
+}
+
+/* u0 = u - imposed velocity, particles 0..n-1 (f.c:569-610) */
+void
+shift_labo_to_rest_U (struct stokes *sys, int n, const double *u, double *u0)
+{
+  for (int i = 0; i < n; i++)
+    {
+      double v[3];
+      imposed_u (sys, i, v);
+      for (int c = 0; c < 3; c++) u0[3 * i + c] = u[3 * i + c] - v[c];
+    }
+}
+
+/* u += imposed velocity, particles 0..n-1 (f.c:621-660) */
+void
+shift_rest_to_labo_U (struct stokes *sys, int n, double *u)
+{
+  for (int i = 0; i < n; i++)
+    {
+      double v[3];
+      imposed_u (sys, i, v);
+      for (int c = 0; c < 3; c++) u[3 * i + c] += v[c];
+    }
+}
+
+/* o0 = o - Oi (ft.c:901-920) */
+void
+shift_labo_to_rest_O (struct stokes *sys, int n, const double *o, double *o0)
+{
+  for (int i = 0; i < 3 * n; i++) o0[i] = o[i] - sys->Oi[i % 3];
+}
+
+/* o += Oi (ft.c:935-955) */
+void
+shift_rest_to_labo_O (struct stokes *sys, int n, double *o)
+{
+  for (int i = 0; i < 3 * n; i++) o[i] += sys->Oi[i % 3];
+}
+
+/* e0 = e - Ei (fts.c:2303-2320) */
+void
+shift_labo_to_rest_E (struct stokes *sys, int n, const double *e, double *e0)
+{
+  for (int i = 0; i < 5 * n; i++) e0[i] = e[i] - sys->Ei[i % 5];
+}
+
+void
+solve_res_3fts (struct stokes *sys, const double *u, const double *o,
+                const double *e, double *f, double *t, double *s)
+{
+  check_fts (sys, "solve_res_3fts");
+  int np = sys->np;
+  double *u0 = alloc_h ((size_t) 11 * np, "solve_res_3fts");
+  double *o0 = u0 + 3 * (size_t) np, *e0 = o0 + 3 * (size_t) np;
+  shift_labo_to_rest_U (sys, np, u, u0);
+  shift_labo_to_rest_O (sys, np, o, o0);
+  shift_labo_to_rest_E (sys, np, e, e0);
+  solve_res_3fts_0 (sys, u0, o0, e0, f, t, s);
+  free (u0);
+}
+
+void
+solve_mob_3fts (struct stokes *sys, const double *f, const double *t,
+                const double *e, double *u, double *o, double *s)
+{
+  check_fts (sys, "solve_mob_3fts");
+  int np = sys->np;
+  double *e0 = alloc_h ((size_t) 5 * np, "solve_mob_3fts");
+  shift_labo_to_rest_E (sys, np, e, e0);
+  solve_mob_3fts_0 (sys, sys->it, f, t, e0, u, o, s);
+  free (e0);
+  shift_rest_to_labo_U (sys, np, u);
+  shift_rest_to_labo_O (sys, np, o);
+}
+
+void
+solve_mix_3fts (struct stokes *sys, const double *f, const double *t,
+                const double *e, const double *uf, const double *of,
+                const double *ef, double *u, double *o, double *s,
+                double *ff, double *tf, double *sf)
+{
+  check_fts (sys, "solve_mix_3fts");
+  int np = sys->np, nm = sys->nm, nf = np - nm;
+  if (nm == np)
+    {
+      solve_mob_3fts (sys, f, t, e, u, o, s);
+      return;
+    }
+  double *buf = alloc_h ((size_t) 11 * nf + 5 * (size_t) nm, "solve_mix_3fts");
+  double *uf0 = buf, *of0 = uf0 + 3 * (size_t) nf, *ef0 = of0 + 3 * (size_t) nf;
+  double *e0 = ef0 + 5 * (size_t) nf;
+  /* as ewald-3fts.c:652: the fixed particles' velocities are shifted with
+   * the positions of particles 0..nf-1, not nm..np-1 (kept for parity) */
+  shift_labo_to_rest_U (sys, nf, uf, uf0);
+  shift_labo_to_rest_O (sys, nf, of, of0);
+  shift_labo_to_rest_E (sys, nf, ef, ef0);
+  shift_labo_to_rest_E (sys, nm, e, e0);
+  solve_mix_3fts_0 (sys, sys->it, f, t, e0, uf0, of0, ef0, u, o, s, ff, tf, sf);
+  free (buf);
+  shift_rest_to_labo_U (sys, nm, u);
+  shift_rest_to_labo_O (sys, nm, o);
 }

@@ -146,10 +146,14 @@ int shk_dev_vh (shk_dev *d, const double *V, const double *h_dev, int m,
 /* FTS mobility-problem pieces per particle of 11 (ewald-3fts.c:140-253):
  * lo = (x_0..5, 0), hi = (0, x_6..10) (either may be NULL);
  * y = lo(a) - z (mode 0) or z - hi(a) (mode 1) */
-int shk_dev_fts_split (shk_dev *d, const double *x, int np, double *lo,
+/* FTS mixed problems, particle i mobile when i < nm (11 values each):
+ * split: lo = [(x_0..5, 0)_mobile, 0_fixed], hi = [(0, x_6..10)_mobile,
+ * x_fixed] (either may be NULL); combine mode 0: y = [(a_0..5, 0)_mobile,
+ * 0_fixed] - z; mode 1: y = z - [(0, a_6..10)_mobile, a_fixed] */
+int shk_dev_fts_split (shk_dev *d, const double *x, int np, int nm, double *lo,
                        double *hi);
 int shk_dev_fts_combine (shk_dev *d, const double *a, const double *z, int np,
-                         int mode, double *y);
+                         int nm, int mode, double *y);
 
 /* Dense FP64 solve with known forces on the mobile particles' force /
  * torque components and known velocities elsewhere (solve.hip); mat is the

@@ -1,14 +1,21 @@
-"""FTS natural mobility problem by GMRES on the device
-(stokes_hip_solve_mob_3fts_dev, solve_mob_3fts_0) against the dense solve.
+"""FTS problems by GMRES on the device (ewald-3fts.c): the natural
+mobility problem (stokes_hip_solve_mob_3fts_dev, solve_mob_3fts_0), the
+mixed problem with fixed particles (stokes_hip_solve_mix_3fts_dev,
+solve_mix_3fts_0), the resistance problem (stokes_hip_solve_res_3fts_dev,
+solve_res_3fts_0) and the labo-frame wrappers.
 
-Unknowns (U, O, S) from known (F, T, E) for every particle; the same
-problem the dense path (solve_mix with nm = np) solves by LU, and the numpy
-restatement of ewald-3fts-matrix.c solves by explicit inversion.
+Mobility / mixed: the same problems the dense path (solve_mix) solves by LU
+and the numpy restatement of ewald-3fts-matrix.c by explicit inversion.
+Resistance: numpy's dense solve of the oracle's operator assembled column
+by column (N = 16) and the defining property M x = b with the oracle's
+operator (N = 64).
 
 Bars: relative residual below the requested eps (reported by the solver
-from a fresh operator application); solution within 1e-10 of the
-restatement at N = 64 (eps 1e-13) and within 1e-9 of the device LU at
-N = 256 (eps 1e-12); MIXED operator with eps 1e-6 within 1e-5 of FP64.
+from a fresh operator application); solutions within 1e-10 of the
+restatements (eps 1e-13) and within 1e-9 of the device LU at N = 256
+(eps 1e-12); MIXED operator with eps 1e-6 within 1e-5 of FP64; the
+labo-frame wrappers equal the fluid-rest solves with the shifts written
+out in numpy to 1e-12.
 """
 import ctypes as C
 
@@ -113,4 +120,134 @@ def test_reference_entry_point_solve_mob_3fts_0():
     # zero right-hand side: zero solution, no iterations
     x, it, rr = s.solve_mob_gmres(np.zeros(11 * n))
     assert it == 0 and not np.any(x)
+    s.close()
+
+
+def _operator_matrix(o, n):
+    """the oracle's FTS operator as a dense matrix, column by column"""
+    m = np.empty((11 * n, 11 * n))
+    for c in range(11 * n):
+        e = np.zeros(11 * n)
+        e[c] = 1.0
+        m[:, c] = o.atimes(e)
+    return m
+
+
+def test_gmres_resistance_problem():
+    """M (F, T, S) = (U, O, E): against numpy's dense solve of the oracle's
+    operator (N = 16), and the defining property at N = 64"""
+    n = 16
+    pos, L = workloads.configuration(n, 0.1, seed=31)
+    o = po.System(2, pos, box=(L, L, L), tr=1.0)
+    s = ls.Stokes(2, pos, box=(L, L, L), tr=1.0)
+    b = np.random.default_rng(6).standard_normal(11 * n)
+    s.set_iter(max_it=1000, restart=40, eps=1e-13)
+    x, it, rr = s.solve_gmres(b, kind="res")
+    want = np.linalg.solve(_operator_matrix(o, n), b)
+    e = relerr(x, want)
+    _say(f"  resistance N={n}: {it} iterations, residual {rr:.2e}, "
+         f"vs dense solve {e:.2e}")
+    assert rr < 1e-13
+    assert e < 1e-10
+    s.close()
+    n = 64
+    pos, L = workloads.configuration(n, 0.15, seed=32)
+    o = po.System(2, pos, box=(L, L, L), tr=1.0)
+    s = ls.Stokes(2, pos, box=(L, L, L), tr=1.0)
+    b = np.random.default_rng(7).standard_normal(11 * n)
+    s.set_iter(max_it=1000, restart=40, eps=1e-13)
+    x, it, rr = s.solve_gmres(b, kind="res")
+    res = relerr(o.atimes(x), b)
+    _say(f"  resistance N={n}: {it} iterations, |M x - b| / |b| (oracle) "
+         f"{res:.2e}")
+    assert res < 1e-11
+    s.close()
+
+
+def test_gmres_mixed_problem_with_fixed_particles():
+    n, nm = 64, 48
+    pos, L = workloads.configuration(n, 0.1, seed=33)
+    o = po.System(2, pos, box=(L, L, L), tr=1.0)
+    s = ls.Stokes(2, pos, box=(L, L, L), tr=1.0)
+    b = np.random.default_rng(8).standard_normal(11 * n)
+    want = po.solve_mix_dense(o, nm, b)
+    lu = s.solve_mix(b, nm=nm)
+    s.set_iter(max_it=1000, restart=30, eps=1e-13)
+    x, it, rr = s.solve_gmres(b, kind="mix", nm=nm)
+    e = relerr(x, want)
+    _say(f"  mixed N={n} nm={nm}: {it} iterations, residual {rr:.2e}, "
+         f"vs restatement {e:.2e}, vs device LU {relerr(x, lu):.2e}")
+    assert rr < 1e-13
+    assert e < 1e-10
+    assert relerr(x, lu) < 1e-10
+    s.close()
+
+
+def _imposed_u(pos, Ui, Oi, Ei):
+    E = np.array([[Ei[0], Ei[1], Ei[2]], [Ei[1], Ei[4], Ei[3]],
+                  [Ei[2], Ei[3], -Ei[0] - Ei[4]]])
+    return Ui + np.cross(Oi, pos) + pos @ E.T
+
+
+def test_labo_frame_wrappers():
+    """solve_res_3fts / solve_mob_3fts / solve_mix_3fts (labo frame) equal the
+    fluid-rest solve with the imposed-flow shifts written out in numpy; the
+    mixed one shifts the fixed particles with the positions of particles
+    0..nf-1, as ewald-3fts.c:652 does"""
+    n, nm = 24, 18
+    nf = n - nm
+    pos, L = workloads.configuration(n, 0.1, seed=34)
+    s = ls.Stokes(2, pos, box=(L, L, L), tr=1.0)
+    s.set_iter(max_it=1000, restart=30, eps=1e-13)
+    Ui, Oi = np.array([0.3, -0.2, 0.1]), np.array([0.05, 0.1, -0.07])
+    Ei = np.array([0.2, 0.05, -0.1, 0.03, -0.15])
+    s.L.stokes_set_Ui(s.sys, *Ui)
+    s.L.stokes_set_Oi(s.sys, *Oi)
+    s.L.stokes_set_Ei(s.sys, *Ei)
+    rng = np.random.default_rng(9)
+    P = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))
+    pk = lambda a, b, c, k: np.concatenate([a.reshape(k, 3), b.reshape(k, 3),
+                                            c.reshape(k, 5)], 1).reshape(-1)
+    # resistance, labo frame
+    u, om, e = (rng.standard_normal(n * k) for k in (3, 3, 5))
+    f, t, st = np.zeros(3 * n), np.zeros(3 * n), np.zeros(5 * n)
+    s.L.solve_res_3fts(s.sys, P(u), P(om), P(e), P(f), P(t), P(st))
+    u0 = u - _imposed_u(pos, Ui, Oi, Ei).reshape(-1)
+    o0 = om - np.tile(Oi, n)
+    e0 = e - np.tile(Ei, n)
+    x, _, _ = s.solve_gmres(pk(u0, o0, e0, n), kind="res")
+    assert relerr(pk(f, t, st, n), x) < 1e-12
+    # mobility, labo frame
+    f, t, e = (rng.standard_normal(n * k) for k in (3, 3, 5))
+    u, om, st = np.zeros(3 * n), np.zeros(3 * n), np.zeros(5 * n)
+    s.s.nm = n
+    s.L.solve_mob_3fts(s.sys, P(f), P(t), P(e), P(u), P(om), P(st))
+    x, _, _ = s.solve_gmres(pk(f, t, e - np.tile(Ei, n), n), kind="mix", nm=n)
+    xr = x.reshape(n, 11)
+    u_want = xr[:, :3] + _imposed_u(pos, Ui, Oi, Ei)
+    o_want = xr[:, 3:6] + Oi
+    got = np.concatenate([u.reshape(n, 3), om.reshape(n, 3), st.reshape(n, 5)], 1)
+    want = np.concatenate([u_want, o_want, xr[:, 6:]], 1)
+    assert relerr(got, want) < 1e-12
+    # mixed, labo frame
+    s.s.nm = nm
+    f, t, e = (rng.standard_normal(nm * k) for k in (3, 3, 5))
+    uf, of, ef = (rng.standard_normal(nf * k) for k in (3, 3, 5))
+    u, om, st = np.zeros(3 * nm), np.zeros(3 * nm), np.zeros(5 * nm)
+    ff, tf, sf = np.zeros(3 * nf), np.zeros(3 * nf), np.zeros(5 * nf)
+    s.L.solve_mix_3fts(s.sys, P(f), P(t), P(e), P(uf), P(of), P(ef), P(u),
+                       P(om), P(st), P(ff), P(tf), P(sf))
+    uf0 = uf - _imposed_u(pos[:nf], Ui, Oi, Ei).reshape(-1)
+    b = np.concatenate([pk(f, t, e - np.tile(Ei, nm), nm),
+                        pk(uf0, of - np.tile(Oi, nf), ef - np.tile(Ei, nf), nf)])
+    x, _, _ = s.solve_gmres(b, kind="mix", nm=nm)
+    xr = x.reshape(n, 11)
+    want = np.concatenate([
+        (np.concatenate([xr[:nm, :3] + _imposed_u(pos[:nm], Ui, Oi, Ei),
+                         xr[:nm, 3:6] + Oi, xr[:nm, 6:]], 1)).reshape(-1),
+        xr[nm:].reshape(-1)])
+    got = np.concatenate([pk(u, om, st, nm), pk(ff, tf, sf, nf)])
+    err = relerr(got, want)
+    _say(f"  labo-frame wrappers: mixed {err:.2e}")
+    assert err < 1e-12
     s.close()
