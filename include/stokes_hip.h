@@ -56,6 +56,16 @@ int stokes_hip_set_recip_g_fix (struct stokes *sys, int enable);
  * within the Ewald tolerance while the real/reciprocal balance is chosen
  * for speed.  tratio = 0 computes at sys->xi (the default). */
 int stokes_hip_set_split (struct stokes *sys, double tratio);
+
+/* enable: build the real-space part of M at the current positions as a
+ * dense matrix ((D np)^2 values in the current precision) and use it for
+ * the real-space sum of every single-rhs product until disabled or until
+ * the system's parameters change (splitting, precision, box).  The caller
+ * keeps the positions fixed meanwhile (products re-read them but the
+ * matrix is not rebuilt).  Pays off where many products share one
+ * configuration and the pair sum costs more than reading the matrix (the
+ * F version: 9 values per pair).  No counterpart in the reference. */
+int stokes_hip_set_real_cache (struct stokes *sys, int enable);
 /* time reps operator calls of nrhs columns at each candidate tratio (NULL:
  * 0 (= sys->xi), 10, 100, 300, 1000, 3000, 1e4) and keep the fastest as
  * the splitting of sys */
@@ -210,6 +220,15 @@ typedef struct stokes_hip_bd_info {
   unsigned long long overlaps[2];  /* overlap counts of the last attempt at
                                       x_mid and x_new */
 } stokes_hip_bd_info;
+
+/* Hold the real-space part of M(x) as a dense matrix ((3N)^2 values in
+ * the system's precision) for the products at x of each step -- the
+ * Chebyshev recursion, its accuracy check and the drift at x, 32 of the 33
+ * products of a step -- and for the Lanczos products of a refit; the
+ * product at x_mid stays matrix-free.  On by default for periodic systems
+ * whose matrix stays below 4 GiB in FP64.  No counterpart in the reference
+ * (its products are matrix-free). */
+int stokes_hip_bd_set_real_cache (stokes_hip_bd *bd, int enable);
 
 /* sys: version 0 (F), equal radii (a == NULL), no shear, set up as for
  * atimes_3all.  F_dev: constant force [3 np] (BD->F) or NULL; it is read on

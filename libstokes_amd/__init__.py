@@ -66,6 +66,7 @@ EXPORTS = [
     "stokes_hip_ensemble_destroy",
     "stokes_hip_bd_create", "stokes_hip_bd_set_bounds",
     "stokes_hip_bd_set_fact_fix", "stokes_hip_bd_step",
+    "stokes_hip_bd_set_real_cache", "stokes_hip_set_real_cache",
     "stokes_hip_bd_last_noise", "stokes_hip_bd_destroy",
 ]
 
@@ -158,6 +159,7 @@ def lib():
               "stokes_hip_set_recip_g_fix"):
         getattr(L, f).argtypes = [sp, C.c_int]
     L.stokes_hip_set_split.argtypes = [sp, C.c_double]
+    L.stokes_hip_set_real_cache.argtypes = [sp, C.c_int]
     L.stokes_hip_solve_mix_dev.argtypes = [sp, _vp, _vp]
     L.stokes_hip_solve_mob_3fts_dev.argtypes = [sp, _vp, _vp,
                                                 C.POINTER(C.c_int), _dp]
@@ -216,6 +218,7 @@ def lib():
                                        C.c_ulonglong, C.POINTER(_vp)]
     L.stokes_hip_bd_set_bounds.argtypes = [_vp, C.c_double, C.c_double]
     L.stokes_hip_bd_set_fact_fix.argtypes = [_vp, C.c_int]
+    L.stokes_hip_bd_set_real_cache.argtypes = [_vp, C.c_int]
     L.stokes_hip_bd_step.argtypes = [_vp, _vp, C.c_double,
                                      C.POINTER(BDInfo)]
     L.stokes_hip_bd_last_noise.argtypes = [_vp, _vp, _vp]
@@ -388,6 +391,12 @@ class Stokes:
               "solve_" + kind)
         return xd.to_host(), it.value, rr.value
 
+    def set_real_cache(self, on=True):
+        """dense real-space matrix of the current positions for the
+        following single-rhs products (keep the positions fixed)"""
+        check(self.L.stokes_hip_set_real_cache(self.sys, int(bool(on))),
+              "set_real_cache")
+
     def set_iter(self, solver="gmres", max_it=2000, restart=30, eps=1e-12):
         self.L.stokes_set_iter(self.sys, solver.encode(), max_it, restart,
                                eps, 0, None)
@@ -524,6 +533,11 @@ class BD:
         if fact_fix:
             check(self.L.stokes_hip_bd_set_fact_fix(self.h, 1),
                   "bd_set_fact_fix")
+
+    def set_real_cache(self, on=True):
+        """dense real-space matrix for the products at x of each step"""
+        check(self.L.stokes_hip_bd_set_real_cache(self.h, int(bool(on))),
+              "bd_set_real_cache")
 
     def step(self, x_dev, dt):
         info = BDInfo()

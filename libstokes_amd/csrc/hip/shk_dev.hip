@@ -13,6 +13,7 @@
 #include "recip_kernels.h"
 #include "block_kernels.h"
 #include "mfma_kernels.h"
+#include "cache_kernels.h"
 #include "adj_kernels.h"
 
 using namespace shk;
@@ -61,6 +62,8 @@ struct shk_dev {
   DevBuf cand, candf, candm, tbits, kpart, kpq, kback;
   DevBuf adjfg, adjcs, adjpart;
   DevBuf km, ptab;                // k lattice indices, per-particle phase tables
+  DevBuf rcache, rz;              // dense real-space matrix, its vector
+  int rc_use = 0, rc_prec = 0;    // products use rcache (built in rc_prec)
   PhaseBasis basis{};
   int ncand = 0, ncore = 0, tbits_words = 0;
   int tx = 0, ty = 0, tz = 0;
@@ -296,6 +299,7 @@ extern "C" int shk_dev_set_system (shk_dev *d, const shk_params *p, const double
       return SHK_EARG;
     }
   d->prm = *p;
+  d->rc_use = 0;                  // a cached real-space matrix is stale now
   int rc = upload_pos (d, pos, a, 0, 0);
   if (rc) return rc;
   if (p->periodic)
@@ -484,7 +488,29 @@ static int atimes_vt (shk_dev *d, const double *x, double *y)
   int rc, nreal = 0, nrecip = 0;
   const void *kpart = nullptr;
   if (d->timing) HIPCHK (hipEventRecord (d->ev[0], d->stream));
-  if (p.periodic)
+  const int want_prec = std::is_same<T, float>::value ? SHK_MIXED : SHK_FP64;
+  if (p.periodic && d->rc_use && d->rc_prec == want_prec)
+    {
+      // real space from the cached dense matrix of these positions
+      const size_t ND = (size_t) p.np * D;
+      nreal = 1;
+      if ((rc = ensure (d, d->partial, sizeof (double) * ND))) return rc;
+      if ((rc = ensure (d, d->rz, sizeof (T) * ND))) return rc;
+      hipLaunchKernelGGL ((k_z_prep<V, T>), dim3 ((unsigned) ((p.np + 255) / 256)), dim3 (256), 0,
+                          d->stream, x, p.np, (T *) d->rz.p);
+      hipLaunchKernelGGL ((k_real_gemv<T>), dim3 ((unsigned) ((ND + 3) / 4)), dim3 (256), 0,
+                          d->stream, (const T *) d->rcache.p, (const T *) d->rz.p, (int) ND,
+                          (double *) d->partial.p);
+      HIPCHK (hipGetLastError ());
+      if (d->timing) HIPCHK (hipEventRecord (d->ev[1], d->stream));
+      if (p.nk > 0)
+        {
+          rc = p.poly ? recip_launch<V, T, 5> (d, x, nrecip, kpart)
+                      : recip_launch<V, T, 1> (d, x, nrecip, kpart);
+          if (rc) return rc;
+        }
+    }
+  else if (p.periodic)
     {
       const int nj = (p.np + 63) / 64;
       nreal = nj;
@@ -806,6 +832,52 @@ static int atimes_block_v (shk_dev *d, const double *x, double *y, int B, int pr
 {
   return prec == SHK_MIXED ? atimes_block_vt<V, float> (d, x, y, B)
                            : atimes_block_vt<V, double> (d, x, y, B);
+}
+
+template <int V, typename T>
+static int real_cache_build (shk_dev *d)
+{
+  constexpr int D = Dim<V>::D;
+  const shk_params &p = d->prm;
+  const size_t ND = (size_t) p.np * D;
+  int rc;
+  if ((rc = ensure (d, d->rcache, sizeof (T) * ND * ND))) return rc;
+  RealParams rp;
+  rp.np = p.np; rp.ncand = d->ncand; rp.ncore = d->ncore; rp.poly = p.poly;
+  rp.xi = p.xi; rp.rmax2 = p.rmax2; rp.rmin = p.rmin;
+  rp.lx = p.lx; rp.ly = p.ly; rp.lz = p.lz; rp.s2x = d->s2x; rp.s3x = d->s3x;
+  rp.tx = d->tx; rp.ty = d->ty; rp.tz = d->tz; rp.tbits_words = d->tbits_words;
+  const long waves = (long) p.np * ((p.np + 63) / 64);
+  const void *cand = std::is_same<T, double>::value ? d->cand.p : d->candf.p;
+  hipLaunchKernelGGL ((k_real_dense<V, T>), dim3 ((unsigned) ((waves + 3) / 4)), dim3 (256), 0,
+                      d->stream, (const double4 *) d->pos.p,
+                      (const typename Vec4<T>::type *) cand, (const int4 *) d->candm.p,
+                      (const unsigned *) d->tbits.p, rp, (T *) d->rcache.p);
+  HIPCHK (hipGetLastError ());
+  return SHK_OK;
+}
+
+extern "C" int shk_dev_real_cache (shk_dev *d, int on, int prec)
+{
+  if (!d || !d->have_sys) { set_err ("shk_dev_real_cache: no system"); return SHK_EARG; }
+  if (!on)
+    {
+      d->rc_use = 0;
+      return SHK_OK;
+    }
+  const shk_params &p = d->prm;
+  if (!p.periodic) { set_err ("shk_dev_real_cache: periodic systems only"); return SHK_EARG; }
+  int rc;
+  if (p.version == 0)
+    rc = prec == SHK_MIXED ? real_cache_build<0, float> (d) : real_cache_build<0, double> (d);
+  else if (p.version == 1)
+    rc = prec == SHK_MIXED ? real_cache_build<1, float> (d) : real_cache_build<1, double> (d);
+  else
+    rc = prec == SHK_MIXED ? real_cache_build<2, float> (d) : real_cache_build<2, double> (d);
+  if (rc) return rc;
+  d->rc_use = 1;
+  d->rc_prec = prec;
+  return SHK_OK;
 }
 
 extern "C" int shk_dev_atimes (shk_dev *d, const double *x, double *y, int nrhs, int prec)

@@ -36,6 +36,7 @@ struct stokes_hip_bd {
   int ncheb;
   unsigned long long seed;
   int fact_fix;
+  int real_cache;              /* dense real-space matrix at x (32 products) */
   int have_bounds;
   int fits;                    /* Lanczos fits so far */
   double eig[2];
@@ -56,7 +57,7 @@ struct stokes_hip_bd {
   stokes_ctx *graph_c;
   double *graph_x;
   unsigned long graph_gen, graph_uploads;
-  int graph_prec, graph_ol;
+  int graph_prec, graph_ol, graph_rc;
 };
 
 static double
@@ -144,6 +145,10 @@ stokes_hip_bd_create (struct stokes *sys, const double *F_dev, double peclet,
   bd->ncheb = n_cheb;
   bd->seed = seed;
   bd->n = (size_t) sys->np * 3;
+  /* the dense real-space matrix ((3N)^2 values) when it stays below 4 GiB
+   * in FP64 */
+  bd->real_cache = sys->periodic && (double) bd->n * (double) bd->n * 8.0
+    <= 4.0 * 1024.0 * 1024.0 * 1024.0;
   bd->sc_fact = bd->sc_dt = -1.0;
   bd->acoef = calloc ((size_t) n_cheb, sizeof (double));
   const size_t n = bd->n;
@@ -215,6 +220,23 @@ stokes_hip_bd_set_fact_fix (stokes_hip_bd *bd, int enable)
 }
 
 int
+stokes_hip_bd_set_real_cache (stokes_hip_bd *bd, int enable)
+{
+  if (!bd)
+    {
+      stokes_hip_set_error ("bd_set_real_cache: NULL handle");
+      return STOKES_HIP_EARG;
+    }
+  if (enable && !bd->sys->periodic)
+    {
+      stokes_hip_set_error ("bd_set_real_cache: periodic systems only");
+      return STOKES_HIP_EARG;
+    }
+  bd->real_cache = enable != 0;
+  return 0;
+}
+
+int
 stokes_hip_bd_last_noise (stokes_hip_bd *bd, double *y_dev, double *z_dev)
 {
   if (!bd)
@@ -242,6 +264,8 @@ bd_record (stokes_hip_bd *bd, shk_dev *d, int prec, int check_ol, double *x)
   if (!rc) rc = shk_dev_gauss (d, bd->seed, bd->counter, n, bd->y);
   /* BD_calc_FB at x: z = M^{-1/2} y and its accuracy (chebyshev.c:241-269) */
   if (!rc) rc = shk_dev_set_pos (d, x, 1);
+  /* the 32 products at x share one dense real-space matrix */
+  if (!rc && bd->real_cache) rc = shk_dev_real_cache (d, 1, prec);
   if (!rc) rc = stokes_hip_cheb_launch (d, prec, bd->ncheb, bd->acoef, 1, n,
                                         bd->y, bd->z, bd->eig[0], bd->eig[1],
                                         bd->work);
@@ -255,6 +279,7 @@ bd_record (stokes_hip_bd *bd, shk_dev *d, int prec, int check_ol, double *x)
     rc = shk_dev_bd_overlap (d, bd->xm, NULL, s->np, s->periodic, s->lx, s->ly,
                              s->lz, 0, 0.0, bd->rmin_ol, &bd->st_dev->overlap[0]);
   /* step 2: drift at x_mid with the same forces, full step from x */
+  if (!rc && bd->real_cache) rc = shk_dev_real_cache (d, 0, prec);
   if (!rc) rc = shk_dev_set_pos (d, bd->xm, 1);
   if (!rc) rc = shk_dev_atimes (d, bd->f, bd->u, 1, prec);
   if (!rc) rc = shk_dev_bd_euler (d, bd->xn, x, bd->u, bd->sc_dev, 1.0, n);
@@ -278,6 +303,13 @@ bd_capture (stokes_hip_bd *bd, stokes_ctx *c, double *x)
   shk_dev_timing (d, 0);
   int rc = shk_dev_set_pos (d, x, 1);
   if (!rc) rc = shk_dev_atimes (d, bd->z, bd->w, 1, c->prec);
+  if (!rc && bd->real_cache)
+    {
+      rc = shk_dev_real_cache (d, 1, c->prec);
+      if (!rc) rc = shk_dev_atimes (d, bd->z, bd->w, 1, c->prec);
+      int rc2 = shk_dev_real_cache (d, 0, c->prec);
+      if (!rc) rc = rc2;
+    }
   if (!rc) rc = shk_dev_sync (d);
   if (rc)
     {
@@ -306,6 +338,7 @@ bd_capture (stokes_hip_bd *bd, stokes_ctx *c, double *x)
   bd->graph_uploads = c->uploads;
   bd->graph_prec = c->prec;
   bd->graph_ol = check_ol;
+  bd->graph_rc = bd->real_cache;
   return 0;
 }
 
@@ -315,8 +348,17 @@ bd_fit_bounds (stokes_hip_bd *bd, stokes_ctx *c, double *x)
   double l0, l1;
   int level = bd->fits < 3 ? bd->fits : 3;
   c->pos_dev = x;
-  int rc = stokes_hip_lanczos_bounds (bd->sys, BD_LANCZOS_ITER * (level + 1),
-                                      1 + bd->seed + bd->fits, 0.0, &l0, &l1);
+  int rc = 0;
+  if (bd->real_cache)
+    {
+      /* every Lanczos product is at x: one dense real-space matrix */
+      rc = stokes_hip_prepare (c, 1);
+      if (!rc) rc = shk_dev_real_cache (c->dev, 1, c->prec);
+      if (rc) return bd_fail (rc, "bd_fit_bounds");
+    }
+  rc = stokes_hip_lanczos_bounds (bd->sys, BD_LANCZOS_ITER * (level + 1),
+                                  1 + bd->seed + bd->fits, 0.0, &l0, &l1);
+  if (bd->real_cache) shk_dev_real_cache (c->dev, 0, c->prec);
   if (rc) return rc;
   bd->fits++;
   return stokes_hip_bd_set_bounds (bd, l0 * (1.0 - BD_LOW_MARGIN (level)),
@@ -365,7 +407,8 @@ stokes_hip_bd_step (stokes_hip_bd *bd, double *x_dev, double dt,
       if (!bd->graph || bd->graph_c != c || bd->graph_x != x_dev
           || bd->graph_gen != shk_dev_generation (d)
           || bd->graph_uploads != c->uploads || bd->graph_prec != c->prec
-          || bd->graph_ol != (bd->sys->rmin == 0.0))
+          || bd->graph_ol != (bd->sys->rmin == 0.0)
+          || bd->graph_rc != bd->real_cache)
         {
           rc = bd_capture (bd, c, x_dev);
           if (rc) goto out;

@@ -591,6 +591,22 @@ stokes_hip_set_recip_g_fix (struct stokes *sys, int enable)
 }
 
 int
+stokes_hip_set_real_cache (struct stokes *sys, int enable)
+{
+  stokes_ctx *c = ctx_or_error (sys);
+  if (!c) return STOKES_HIP_ENODEV;
+  if (enable && !sys->periodic)
+    {
+      stokes_hip_set_error ("set_real_cache: periodic systems only");
+      return STOKES_HIP_EARG;
+    }
+  int rc = enable ? stokes_hip_prepare (c, c->pos_dev != NULL) : 0;
+  if (rc) return rc;
+  rc = shk_dev_real_cache (c->dev, enable != 0, c->prec);
+  return rc ? dev_fail (rc, "set_real_cache") : 0;
+}
+
+int
 stokes_hip_set_split (struct stokes *sys, double tratio)
 {
   stokes_ctx *c = ctx_or_error (sys);

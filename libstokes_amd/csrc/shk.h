@@ -134,6 +134,12 @@ void shk_set_error (const char *fmt, ...);
 int shk_dev_adjoint_x (shk_dev *d, const double *f, const double *g,
                        double *dx, int prec);
 
+/* on: build the dense real-space matrix of the current positions in the
+ * precision prec (periodic systems) and use it for the real-space part of
+ * every single-rhs product until turned off; off: back to the pair kernels.
+ * The caller keeps the positions fixed while it is on. */
+int shk_dev_real_cache (shk_dev *d, int on, int prec);
+
 /* y = a x + b y (len doubles) */
 int shk_dev_axpby (shk_dev *d, double *y, const double *x, double a, double b,
                    size_t len);

@@ -252,3 +252,54 @@ def test_bd_bounds_enclose_spectrum_n512():
         assert info.err_cheb < 1e-3 and info.refits == 0
     assert info.eig[0] <= w[0] and info.eig[1] >= w[-1] * (1 - 1e-9)
     bd.close()
+
+
+@pytest.mark.parametrize("version", [0, 1, 2])
+def test_real_cache_products_equal_matrix_free(version):
+    """stokes_hip_set_real_cache: the dense real-space matrix gives the
+    matrix-free product (FP64 1e-13, MIXED 1e-6 against FP64)"""
+    n = 96
+    pos, L = workloads.configuration(n, 0.15, seed=12)
+    x = np.random.default_rng(13).standard_normal(ls.dim(version) * n)
+    s = ls.Stokes(version, pos, box=(L, L, L), tr=1.0)
+    s.set_split(30.0)
+    y0 = s.atimes(x)
+    s.set_real_cache(True)
+    y1 = s.atimes(x)
+    s.set_real_cache(False)
+    e64 = relerr(y1, y0)
+    s.set_precision(ls.MIXED)
+    s.set_real_cache(True)
+    y2 = s.atimes(x)
+    s.set_real_cache(False)
+    em = relerr(y2, y0)
+    print(f"  version {version}: cached FP64 {e64:.2e}, cached MIXED {em:.2e}",
+          flush=True)
+    assert e64 < 1e-13
+    assert em < 1e-6
+    s.close()
+
+
+def test_bd_steps_with_and_without_real_cache():
+    """the BD step with the dense real-space matrix for its products at x
+    follows the matrix-free step (FP64: 1e-12 of the displacement)"""
+    n = 128
+    pos, L = workloads.configuration(n, 0.15, seed=14)
+    out = []
+    for cache in (False, True):
+        s = ls.Stokes(0, pos, box=(L, L, L), tr=1.0, rmin=1.0)
+        s.set_split(100.0)
+        xd = ls.DeviceArray.from_host(pos.reshape(-1))
+        bd = ls.BD(s, peclet=1.0, n_cheb=20, eps=1e-2, dt_lim=1e-6,
+                   rmin_ol=1.0, seed=5)
+        bd.set_real_cache(cache)
+        for _ in range(3):
+            bd.step(xd, 1e-3)
+        out.append(xd.to_host())
+        bd.close()
+        s.close()
+    disp = np.linalg.norm(out[0] - pos.reshape(-1))
+    d = np.linalg.norm(out[1] - out[0]) / disp
+    print(f"  BD 3 steps, cached vs matrix-free: {d:.2e} of the displacement",
+          flush=True)
+    assert d < 1e-12
