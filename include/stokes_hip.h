@@ -191,10 +191,14 @@ size_t stokes_hip_ensemble_offset (const stokes_hip_ensemble *e, int box);
 void stokes_hip_ensemble_destroy (stokes_hip_ensemble *e);
 
 /* ---- Brownian dynamics ------------------------------------------------
- * The mid-point step of BD_evolve_mid (brownian.c:2352-2560) for the F
- * version with every particle mobile (the BD_params fields it reads are
- * the arguments of stokes_hip_bd_create), run on the GPU as one captured
- * graph per attempt: Gaussian draw y, z = M^{-1/2} y by the Chebyshev
+ * The mid-point step of BD_evolve_mid (brownian.c:2352-2560) for the F, FT
+ * and FTS versions with every particle mobile (the BD_params fields it
+ * reads are the arguments of stokes_hip_bd_create).  M is the mobility of
+ * BD_atimes_mob_FU (brownian.c:494-739): F -> U (F), (F, T) -> (U, O)
+ * particle-wise (FT), and for FTS the mobility problem with E = 0 solved
+ * by GMRES for each product.  For F and FT an attempt runs on the GPU as
+ * one captured graph (FTS: the same launches, eagerly, the GMRES steered
+ * from the host): Gaussian draw y, z = M^{-1/2} y by the Chebyshev
  * recursion (BD_calc_FB, brownian.c:1545-1874), its accuracy
  * |z.Mz - y.y| / z.Mz, drift at x, Euler half step, overlap count, drift
  * at the mid point, Euler full step, overlap count.  The host then takes
@@ -221,17 +225,19 @@ typedef struct stokes_hip_bd_info {
                                       x_mid and x_new */
 } stokes_hip_bd_info;
 
-/* Hold the real-space part of M(x) as a dense matrix ((3N)^2 values in
- * the system's precision) for the products at x of each step -- the
- * Chebyshev recursion, its accuracy check and the drift at x, 32 of the 33
- * products of a step -- and for the Lanczos products of a refit; the
+/* Hold the real-space part of M(x) as a dense matrix ((D N)^2 values in
+ * the system's precision, D = 3, 6, 11) for the products at x of each
+ * step -- the Chebyshev recursion, its accuracy check and the drift at x,
+ * 32 of the 33 products of a step (F, FT) -- and for the Lanczos products
+ * of a refit; the
  * product at x_mid stays matrix-free.  On by default for periodic systems
  * whose matrix stays below 4 GiB in FP64.  No counterpart in the reference
  * (its products are matrix-free). */
 int stokes_hip_bd_set_real_cache (stokes_hip_bd *bd, int enable);
 
-/* sys: version 0 (F), equal radii (a == NULL), no shear, set up as for
- * atimes_3all.  F_dev: constant force [3 np] (BD->F) or NULL; it is read on
+/* sys: version 0 (F), 1 (FT) or 2 (FTS), equal radii (a == NULL), no
+ * shear, set up as for atimes_3all.  F_dev: constant force [3 np] (BD->F)
+ * or NULL; it is read on
  * every step and must stay allocated until stokes_hip_bd_destroy.  peclet:
  * fact = sqrt(2 / (Pe dt)), 0 when Pe <= 0 (brownian.c:2177-2187).
  * n_cheb: order of the 1/sqrt(x) fit (BD->n_minv > 0).  eps: refit
@@ -251,8 +257,25 @@ int stokes_hip_bd_set_fact_fix (stokes_hip_bd *bd, int enable);
 /* one step from the positions x_dev [3 np] (updated in place), step dt */
 int stokes_hip_bd_step (stokes_hip_bd *bd, double *x_dev, double dt,
                         stokes_hip_bd_info *info);
+/* the same with the orientations (BD_evolve_mid's q, brownian.c:2352):
+ * q_dev [4 np] quaternions updated in place by q += dt dQ/dt(q, O)
+ * (evolve_Euler_3all, brownian.c:1889-1930, quaternion_dQdt,
+ * ode-quaternion.c:89-105) for FT and FTS; NULL, or the F version,
+ * tracks none */
+int stokes_hip_bd_step_q (stokes_hip_bd *bd, double *x_dev, double *q_dev,
+                          double dt, stokes_hip_bd_info *info);
+/* FT / FTS: constant torques [3 np] (BD->T, brownian.c:2296-2299) or NULL,
+ * read on every step */
+int stokes_hip_bd_set_torque (stokes_hip_bd *bd, const double *T_dev);
+/* FTS: GMRES parameters of the noise products and of the Lanczos bounds
+ * (brownian.c:597-600 fixes restart 20, max 2000, eps 1e-6, the default
+ * here); the step's own solves take those of stokes_set_iter, as
+ * solve_mix_3all does */
+int stokes_hip_bd_set_fts_iter (stokes_hip_bd *bd, int restart, int max,
+                                double eps);
 /* copies of the Gaussian draw and of z = M^{-1/2} y of the last attempt
- * (device pointers, 3 np each; either may be NULL) */
+ * (device pointers, 3 np (F) or 6 np (FT, FTS, particle-wise (F, T)) each;
+ * either may be NULL) */
 int stokes_hip_bd_last_noise (stokes_hip_bd *bd, double *y_dev, double *z_dev);
 void stokes_hip_bd_destroy (stokes_hip_bd *bd);
 

@@ -68,6 +68,8 @@ EXPORTS = [
     "stokes_hip_bd_set_fact_fix", "stokes_hip_bd_step",
     "stokes_hip_bd_set_real_cache", "stokes_hip_set_real_cache",
     "stokes_hip_bd_last_noise", "stokes_hip_bd_destroy",
+    "stokes_hip_bd_step_q", "stokes_hip_bd_set_torque",
+    "stokes_hip_bd_set_fts_iter",
 ]
 
 
@@ -221,6 +223,11 @@ def lib():
     L.stokes_hip_bd_set_real_cache.argtypes = [_vp, C.c_int]
     L.stokes_hip_bd_step.argtypes = [_vp, _vp, C.c_double,
                                      C.POINTER(BDInfo)]
+    L.stokes_hip_bd_step_q.argtypes = [_vp, _vp, _vp, C.c_double,
+                                       C.POINTER(BDInfo)]
+    L.stokes_hip_bd_set_torque.argtypes = [_vp, _vp]
+    L.stokes_hip_bd_set_fts_iter.argtypes = [_vp, C.c_int, C.c_int,
+                                             C.c_double]
     L.stokes_hip_bd_last_noise.argtypes = [_vp, _vp, _vp]
     L.stokes_hip_bd_destroy.argtypes = [_vp]
     L.stokes_hip_bd_destroy.restype = None
@@ -511,7 +518,7 @@ class Ensemble:
 
 
 class BD:
-    """Mid-point Brownian-dynamics integrator of one F-version system
+    """Mid-point Brownian-dynamics integrator of one F, FT or FTS system
     (stokes_hip_bd_* of include/stokes_hip.h; brownian.c:2352-2560)."""
 
     def __init__(self, stokes, peclet=1.0, n_cheb=30, eps=1e-6, dt_lim=1e-6,
@@ -539,10 +546,24 @@ class BD:
         check(self.L.stokes_hip_bd_set_real_cache(self.h, int(bool(on))),
               "bd_set_real_cache")
 
-    def step(self, x_dev, dt):
+    def set_torque(self, torque_dev):
+        """constant torques [3 np] (FT, FTS); read on every step: kept"""
+        self.torque_dev = torque_dev
+        check(self.L.stokes_hip_bd_set_torque(
+            self.h, torque_dev.ptr if torque_dev is not None else None),
+            "bd_set_torque")
+
+    def set_fts_iter(self, restart=20, max_iter=2000, eps=1e-6):
+        """FTS: GMRES parameters of the noise products"""
+        check(self.L.stokes_hip_bd_set_fts_iter(self.h, int(restart),
+                                                int(max_iter), float(eps)),
+              "bd_set_fts_iter")
+
+    def step(self, x_dev, dt, q_dev=None):
         info = BDInfo()
-        rc = self.L.stokes_hip_bd_step(self.h, x_dev.ptr, float(dt),
-                                       C.byref(info))
+        rc = self.L.stokes_hip_bd_step_q(
+            self.h, x_dev.ptr, q_dev.ptr if q_dev is not None else None,
+            float(dt), C.byref(info))
         if rc != 0:
             msg = self.L.stokes_hip_last_error()
             raise StokesError(f"bd_step failed ({rc}): "
@@ -551,7 +572,7 @@ class BD:
         return info
 
     def last_noise(self):
-        n = 3 * self.stokes.np
+        n = (3 if self.stokes.version == 0 else 6) * self.stokes.np
         y = DeviceArray(n)
         z = DeviceArray(n)
         check(self.L.stokes_hip_bd_last_noise(self.h, y.ptr, z.ptr),

@@ -1,5 +1,5 @@
 // bd_kernels.hip -- device pieces of the Brownian-dynamics mid-point step
-// (brownian.c:2352-2560, F version): Gaussian noise, forces F + fact z
+// (brownian.c:2352-2560, F, FT and FTS versions): Gaussian noise, forces F + fact z
 // (brownian.c:2273-2311), Euler updates (brownian.c:1889-1930), the
 // Chebyshev accuracy dots (chebyshev.c:241-269) and the overlap count
 // (brownian.c:299-375), plus hipGraph capture / replay of a step.
@@ -106,6 +106,52 @@ __global__ void k_bd_euler (double *__restrict__ xo, const double *__restrict__ 
   const size_t i = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   xo[i] = xi[i] + (frac * sc->dt) * u[i];
+}
+
+// FT / FTS forces per particle of 6: (F + fact z_F, T + fact z_T), F and T
+// of 3 per particle (either may be NULL), z particle-wise of 6
+// (BD_calc_forces, brownian.c:2273-2311, with BD_calc_FB's particle-wise
+// noise, 1851-1871)
+__global__ void k_bd_force_ft (double *__restrict__ f, const double *__restrict__ F,
+                               const double *__restrict__ T, const double *__restrict__ z,
+                               const shk_bd_scalars *__restrict__ sc, int np)
+{
+  const size_t t = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (size_t) np * 6) return;
+  const size_t i = t / 6;
+  const int q = (int) (t % 6);
+  const double *ext = q < 3 ? F : T;
+  const double fz = sc->fact * z[t];
+  f[t] = ext ? ext[i * 3 + (q % 3)] + fz : fz;
+}
+
+// FT / FTS Euler update (evolve_Euler_3all, brownian.c:1889-1930): xo = xi
+// + (frac dt) U and, when qi is given, qo = qi + (frac dt) dQ/dt with
+// dQ/dt = 1/2 W(q)^{-1} (O, 0) (quaternion_dQdt, ode-quaternion.c:89-105;
+// W(q)^{-1} = W(q)^T / |q|^2, quaternion_Winv, :58-79); u holds (U, O) per
+// particle of 6
+__global__ void k_bd_euler_ft (double *__restrict__ xo, const double *__restrict__ xi,
+                               double *__restrict__ qo, const double *__restrict__ qi,
+                               const double *__restrict__ u,
+                               const shk_bd_scalars *__restrict__ sc, double frac, int np)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= np) return;
+  const double h = frac * sc->dt;
+  const double *ui = u + (size_t) i * 6;
+  for (int a = 0; a < 3; a++) xo[i * 3 + a] = xi[i * 3 + a] + h * ui[a];
+  if (!qi) return;
+  const double q0 = qi[i * 4], q1 = qi[i * 4 + 1], q2 = qi[i * 4 + 2], q3 = qi[i * 4 + 3];
+  const double o0 = ui[3], o1 = ui[4], o2 = ui[5];
+  const double hd = 0.5 / (q0 * q0 + q1 * q1 + q2 * q2 + q3 * q3);
+  const double d0 = hd * (q3 * o0 - q2 * o1 + q1 * o2);
+  const double d1 = hd * (q2 * o0 + q3 * o1 - q0 * o2);
+  const double d2 = hd * (-q1 * o0 + q0 * o1 + q3 * o2);
+  const double d3 = hd * (-q0 * o0 - q1 * o1 - q2 * o2);
+  qo[i * 4] = q0 + h * d0;
+  qo[i * 4 + 1] = q1 + h * d1;
+  qo[i * 4 + 2] = q2 + h * d2;
+  qo[i * 4 + 3] = q3 + h * d3;
 }
 
 // y.y and z.w in one block (fixed summation order: deterministic)
@@ -231,6 +277,27 @@ extern "C" int shk_dev_bd_euler (shk_dev *d, double *xo, const double *xi, const
   hipStream_t s = (hipStream_t) shk_dev_stream (d);
   hipLaunchKernelGGL (k_bd_euler, dim3 (blocks_for (n, 256)), dim3 (256), 0, s, xo, xi, u, sc,
                       frac, n);
+  CHK (hipGetLastError ());
+  return SHK_OK;
+}
+
+extern "C" int shk_dev_bd_force_ft (shk_dev *d, double *f, const double *F, const double *T,
+                                    const double *z, const shk_bd_scalars *sc, int np)
+{
+  hipStream_t s = (hipStream_t) shk_dev_stream (d);
+  hipLaunchKernelGGL (k_bd_force_ft, dim3 (blocks_for ((size_t) np * 6, 256)), dim3 (256), 0, s,
+                      f, F, T, z, sc, np);
+  CHK (hipGetLastError ());
+  return SHK_OK;
+}
+
+extern "C" int shk_dev_bd_euler_ft (shk_dev *d, double *xo, const double *xi, double *qo,
+                                    const double *qi, const double *u, const shk_bd_scalars *sc,
+                                    double frac, int np)
+{
+  hipStream_t s = (hipStream_t) shk_dev_stream (d);
+  hipLaunchKernelGGL (k_bd_euler_ft, dim3 (blocks_for ((size_t) np, 256)), dim3 (256), 0, s, xo,
+                      xi, qo, qi, u, sc, frac, np);
   CHK (hipGetLastError ());
   return SHK_OK;
 }

@@ -72,6 +72,20 @@ __global__ void k_fts_combine (const double *__restrict__ a, const double *__res
   else y[t] = z[t] - (mob && q < 6 ? 0.0 : a[t]);
 }
 
+// (F, T) per particle of 6 -> (F, T, 0) per particle of 11 (mode 0), or
+// the first 6 of 11 back (mode 1): the FT view of the FTS mobility problem
+// with E = 0 (BD_atimes_mob_FU, brownian.c:597-634)
+__global__ void k_ft_fts (const double *__restrict__ x, int np, int mode,
+                          double *__restrict__ y)
+{
+  const size_t t = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (size_t) np * 11) return;
+  const size_t i = t / 11;
+  const int q = (int) (t % 11);
+  if (mode == 0) y[t] = q < 6 ? x[i * 6 + q] : 0.0;
+  else if (q < 6) y[i * 6 + q] = x[t];
+}
+
 }  // namespace
 
 #define CHK(call)                                                           \
@@ -134,6 +148,16 @@ extern "C" int shk_dev_fts_combine (shk_dev *d, const double *a, const double *z
   const size_t n = (size_t) np * 11;
   hipLaunchKernelGGL (k_fts_combine, dim3 ((unsigned) ((n + 255) / 256)), dim3 (256), 0, s, a,
                       z, np, nm, mode, y);
+  CHK (hipGetLastError ());
+  return SHK_OK;
+}
+
+extern "C" int shk_dev_ft_fts (shk_dev *d, const double *x, int np, int mode, double *y)
+{
+  hipStream_t s = (hipStream_t) shk_dev_stream (d);
+  const size_t n = (size_t) np * 11;
+  hipLaunchKernelGGL (k_ft_fts, dim3 ((unsigned) ((n + 255) / 256)), dim3 (256), 0, s, x, np,
+                      mode, y);
   CHK (hipGetLastError ());
   return SHK_OK;
 }

@@ -1,15 +1,22 @@
-/* brownian.c -- the mid-point Brownian-dynamics step, F version, on the
- * GPU (restates BD_evolve_mid, brownian.c:2352-2560, with BD_calc_FB's
- * atimes route 1545-1874, BD_calc_forces 2273-2311, evolve_Euler_3all
- * 1889-1930 and check_overlap 299-375).
+/* brownian.c -- the mid-point Brownian-dynamics step on the GPU, F, FT and
+ * FTS versions (restates BD_evolve_mid, brownian.c:2352-2560, with
+ * BD_calc_FB's atimes route 1545-1874, BD_atimes_mob_FU 494-739,
+ * BD_calc_forces 2273-2311, evolve_Euler_3all 1889-1930 and check_overlap
+ * 299-375).
  *
- * One attempt of the step is a captured graph (see bd_record): draw y,
- * z = M^{-1/2} y by Chebyshev, the accuracy dots, f = F + fact z, u = M(x) f,
- * x_mid = x + dt/2 u, overlap count, u = M(x_mid) f, x_new = x + dt u,
- * overlap count, advance the draw counter, copy the status to pinned host
- * memory.  fact and dt are read by the kernels from device memory, so the
- * graph is replayed unchanged for every dt.  The host then takes the
- * reference's decisions on the status (refit, halve dt, redraw, accept).
+ * One attempt of the step (see bd_record): draw y, z = M^{-1/2} y by
+ * Chebyshev, the accuracy dots, f = F + fact z, u = M(x) f, x_mid = x + dt/2
+ * u, overlap count, u = M(x_mid) f, x_new = x + dt u (and the quaternions
+ * q_new = q + dt dQ/dt(q, O) for FT / FTS), overlap count, advance the draw
+ * counter, copy the status to pinned host memory.  M is the mobility of
+ * the forces (F) or of the forces and torques (FT, FTS: particle-wise (F,
+ * T) -> (U, O)); for FTS every product is the mobility problem with E = 0
+ * solved by GMRES (BD_atimes_mob_FU, brownian.c:597-634).  For F and FT an
+ * attempt is one captured graph; fact and dt are read by the kernels from
+ * device memory, so the graph is replayed unchanged for every dt.  For FTS
+ * the GMRES iterations are steered from the host and the attempt runs
+ * eagerly.  The host then takes the reference's decisions on the status
+ * (refit, halve dt, redraw, accept).
  */
 #include <math.h>
 #include <stdio.h>
@@ -31,7 +38,9 @@
 struct stokes_hip_bd {
   struct stokes *sys;
   stokes_ctx *c;
-  const double *F;
+  const double *F, *T;         /* device [3 np] or NULL */
+  int ver;                     /* sys->version */
+  int np;
   double peclet, eps, dt_lim, rmin_ol;
   int ncheb;
   unsigned long long seed;
@@ -41,9 +50,15 @@ struct stokes_hip_bd {
   int fits;                    /* Lanczos fits so far */
   double eig[2];
   double *acoef;               /* host [ncheb] */
-  size_t n;                    /* 3 np */
-  double *buf;                 /* device: y z w f u xm xn work[4n] */
-  double *y, *z, *w, *f, *u, *xm, *xn, *work;
+  size_t n;                    /* operator dimension: 3 np (F), 6 np */
+  double *buf;                 /* device: y z w f u work[4n] xm xn qn */
+  double *y, *z, *w, *f, *u, *xm, *xn, *qn, *work;
+  /* FTS: GMRES of the noise products (brownian.c:597-600: restart 20,
+   * max 2000, eps 1e-6) and work space */
+  int fts_m, fts_max;
+  double fts_eps;
+  double *fts_buf, *fts_b, *fts_x, *fts_ws;
+  long fts_iters;
   void *small;                 /* device: counter, scalars, status */
   unsigned long long *counter;
   shk_bd_scalars *sc_dev;
@@ -55,7 +70,7 @@ struct stokes_hip_bd {
   /* captured step */
   void *graph;
   stokes_ctx *graph_c;
-  double *graph_x;
+  double *graph_x, *graph_q;
   unsigned long graph_gen, graph_uploads;
   int graph_prec, graph_ol, graph_rc;
 };
@@ -93,6 +108,7 @@ stokes_hip_bd_destroy (stokes_hip_bd *bd)
    * the device */
   shk_dev_graph_destroy (bd->graph);
   if (bd->buf) shk_dev_free (NULL, bd->buf);
+  if (bd->fts_buf) shk_dev_free (NULL, bd->fts_buf);
   if (bd->small) shk_dev_free (NULL, bd->small);
   shk_host_free (bd->pinned);
   free (bd->acoef);
@@ -110,11 +126,10 @@ stokes_hip_bd_create (struct stokes *sys, const double *F_dev, double peclet,
       return STOKES_HIP_EARG;
     }
   *out = NULL;
-  if (sys->version != 0)
+  if (sys->version < 0 || sys->version > 2)
     {
-      stokes_hip_set_error ("bd_create: only the F version (version 0) is"
-                            " supported, got %d", sys->version);
-      return STOKES_HIP_EUNSUP;
+      stokes_hip_set_error ("bd_create: version %d", sys->version);
+      return STOKES_HIP_EARG;
     }
   if (sys->a != NULL || sys->shear_mode != 0 || sys->nm != sys->np)
     {
@@ -144,16 +159,26 @@ stokes_hip_bd_create (struct stokes *sys, const double *F_dev, double peclet,
   bd->rmin_ol = rmin_ol;
   bd->ncheb = n_cheb;
   bd->seed = seed;
-  bd->n = (size_t) sys->np * 3;
-  /* the dense real-space matrix ((3N)^2 values) when it stays below 4 GiB
+  bd->ver = sys->version;
+  bd->np = sys->np;
+  bd->n = (size_t) sys->np * (sys->version == 0 ? 3 : 6);
+  bd->fts_m = 20;
+  bd->fts_max = 2000;
+  bd->fts_eps = 1e-6;
+  /* the dense real-space matrix ((D N)^2 values) when it stays below 4 GiB
    * in FP64 */
-  bd->real_cache = sys->periodic && (double) bd->n * (double) bd->n * 8.0
+  const double dn = (double) sys->np * stokes_dim (sys->version);
+  bd->real_cache = sys->periodic && dn * dn * 8.0
     <= 4.0 * 1024.0 * 1024.0 * 1024.0;
   bd->sc_fact = bd->sc_dt = -1.0;
   bd->acoef = calloc ((size_t) n_cheb, sizeof (double));
-  const size_t n = bd->n;
+  const size_t n = bd->n, np = (size_t) sys->np;
+  const size_t nbuf = 9 * n + 10 * np;
   int rc = bd->acoef ? 0 : SHK_ENOMEM;
-  if (!rc) rc = shk_dev_malloc (c->dev, (void **) &bd->buf, sizeof (double) * 11 * n);
+  if (!rc) rc = shk_dev_malloc (c->dev, (void **) &bd->buf, sizeof (double) * nbuf);
+  if (!rc && bd->ver == 2)
+    rc = shk_dev_malloc (c->dev, (void **) &bd->fts_buf, sizeof (double)
+                         * (22 * np + stokes_hip_fts_ws (sys->np, 64)));
   if (!rc) rc = shk_dev_malloc (c->dev, &bd->small, 256);
   if (!rc) rc = shk_host_alloc (&bd->pinned, 256);
   if (rc)
@@ -167,9 +192,16 @@ stokes_hip_bd_create (struct stokes *sys, const double *F_dev, double peclet,
   bd->w = bd->z + n;
   bd->f = bd->w + n;
   bd->u = bd->f + n;
-  bd->xm = bd->u + n;
-  bd->xn = bd->xm + n;
-  bd->work = bd->xn + n;
+  bd->work = bd->u + n;
+  bd->xm = bd->work + 4 * n;
+  bd->xn = bd->xm + 3 * np;
+  bd->qn = bd->xn + 3 * np;
+  if (bd->fts_buf)
+    {
+      bd->fts_b = bd->fts_buf;
+      bd->fts_x = bd->fts_b + 11 * np;
+      bd->fts_ws = bd->fts_x + 11 * np;
+    }
   bd->counter = (unsigned long long *) bd->small;
   bd->sc_dev = (shk_bd_scalars *) ((char *) bd->small + 64);
   bd->st_dev = (shk_bd_status *) ((char *) bd->small + 128);
@@ -179,7 +211,7 @@ stokes_hip_bd_create (struct stokes *sys, const double *F_dev, double peclet,
   /* draw counter = 0, work vectors = 0 */
   unsigned long long zero = 0;
   rc = shk_dev_h2d (c->dev, bd->counter, &zero, sizeof (zero));
-  if (!rc) rc = shk_dev_scal (c->dev, bd->buf, NULL, 0.0, 11 * n);
+  if (!rc) rc = shk_dev_scal (c->dev, bd->buf, NULL, 0.0, nbuf);
   if (!rc) rc = shk_dev_sync (c->dev);
   if (rc)
     {
@@ -237,6 +269,37 @@ stokes_hip_bd_set_real_cache (stokes_hip_bd *bd, int enable)
 }
 
 int
+stokes_hip_bd_set_torque (stokes_hip_bd *bd, const double *T_dev)
+{
+  if (!bd || bd->ver == 0)
+    {
+      stokes_hip_set_error ("bd_set_torque: needs an FT or FTS integrator");
+      return STOKES_HIP_EARG;
+    }
+  bd->T = T_dev;
+  shk_dev_graph_destroy (bd->graph);   /* the pointer is baked in */
+  bd->graph = NULL;
+  return 0;
+}
+
+int
+stokes_hip_bd_set_fts_iter (stokes_hip_bd *bd, int restart, int max,
+                            double eps)
+{
+  if (!bd || bd->ver != 2 || restart < 1 || restart > 64 || max < 1
+      || !(eps > 0.0))
+    {
+      stokes_hip_set_error ("bd_set_fts_iter: needs an FTS integrator, "
+                            "1 <= restart <= 64, max >= 1, eps > 0");
+      return STOKES_HIP_EARG;
+    }
+  bd->fts_m = restart;
+  bd->fts_max = max;
+  bd->fts_eps = eps;
+  return 0;
+}
+
+int
 stokes_hip_bd_last_noise (stokes_hip_bd *bd, double *y_dev, double *z_dev)
 {
   if (!bd)
@@ -254,35 +317,93 @@ stokes_hip_bd_last_noise (stokes_hip_bd *bd, double *y_dev, double *z_dev)
   return rc ? bd_fail (rc, "bd_last_noise") : 0;
 }
 
-/* the kernels of one attempt, in stream order (captured) */
+/* the mobility of the integrator, y = M x on device vectors of bd->n:
+ * F and FT the operator itself; FTS the mobility problem (F, T, E = 0) ->
+ * (U, O) by GMRES (BD_atimes_mob_FU, brownian.c:597-634), with the noise
+ * parameters (noise != 0) or those of stokes_set_iter (the step's solves,
+ * solve_mix_3all) */
 static int
-bd_record (stokes_hip_bd *bd, shk_dev *d, int prec, int check_ol, double *x)
+bd_op (stokes_hip_bd *bd, shk_dev *d, int prec, const double *x, double *y,
+       int noise)
+{
+  if (bd->ver != 2) return shk_dev_atimes (d, x, y, 1, prec);
+  int m = bd->fts_m, maxit = bd->fts_max, it = 0;
+  double eps = bd->fts_eps, rr = 0.0;
+  if (!noise) stokes_hip_iter_params (bd->sys, &m, &maxit, &eps);
+  int rc = shk_dev_ft_fts (d, x, bd->np, 0, bd->fts_b);
+  if (!rc) rc = stokes_hip_fts_solve (bd->c, 1, bd->np, bd->fts_b, bd->fts_x,
+                                      m, maxit, eps, bd->fts_ws, &it, &rr);
+  bd->fts_iters += it;
+  if (!rc) rc = shk_dev_ft_fts (d, bd->fts_x, bd->np, 1, y);
+  return rc;
+}
+
+/* Clenshaw recursion of chebyshev.c:163-228 over bd_op: z = sum_j a_j
+ * T_j(M') y (the same launches as stokes_hip_cheb_launch for F and FT) */
+static int
+bd_cheb (stokes_hip_bd *bd, shk_dev *d, int prec, const double *y, double *z)
+{
+  const size_t len = bd->n;
+  const double *a = bd->acoef;
+  const double e0 = bd->eig[0], e1 = bd->eig[1];
+  double *x = bd->work, *b0 = x + len, *b1 = b0 + len, *b2 = b1 + len;
+  const double da = 2.0 / (e1 - e0), db = -(e1 + e0) / (e1 - e0);
+  const int ncheb = bd->ncheb;
+  int rc = shk_dev_scal (d, b0, NULL, 0.0, len);
+  if (!rc) rc = shk_dev_scal (d, b1, y, a[ncheb - 1], len);
+  for (int j = ncheb - 2; j >= 1 && !rc; j--)
+    {
+      rc = bd_op (bd, d, prec, b1, x, 1);
+      if (!rc) rc = shk_dev_axpby4 (d, b2, x, b1, b0, y, 2.0 * da, 2.0 * db,
+                                    a[j], len);
+      double *t = b0; b0 = b1; b1 = b2; b2 = t;
+    }
+  if (!rc) rc = bd_op (bd, d, prec, b1, x, 1);
+  if (!rc) rc = shk_dev_axpby4 (d, z, x, b1, b0, y, da, db, a[0], len);
+  return rc;
+}
+
+/* the kernels of one attempt, in stream order (captured for F and FT) */
+static int
+bd_record (stokes_hip_bd *bd, shk_dev *d, int prec, int check_ol, double *x,
+           double *q)
 {
   const struct stokes *s = bd->sys;
   const size_t n = bd->n;
+  const size_t n3 = (size_t) bd->np * 3;
   int rc = shk_dev_bd_status_clear (d, bd->st_dev);
   if (!rc) rc = shk_dev_gauss (d, bd->seed, bd->counter, n, bd->y);
   /* BD_calc_FB at x: z = M^{-1/2} y and its accuracy (chebyshev.c:241-269) */
   if (!rc) rc = shk_dev_set_pos (d, x, 1);
-  /* the 32 products at x share one dense real-space matrix */
+  /* the products at x share one dense real-space matrix */
   if (!rc && bd->real_cache) rc = shk_dev_real_cache (d, 1, prec);
-  if (!rc) rc = stokes_hip_cheb_launch (d, prec, bd->ncheb, bd->acoef, 1, n,
-                                        bd->y, bd->z, bd->eig[0], bd->eig[1],
-                                        bd->work);
-  if (!rc) rc = shk_dev_atimes (d, bd->z, bd->w, 1, prec);
+  if (!rc) rc = bd_cheb (bd, d, prec, bd->y, bd->z);
+  if (!rc) rc = bd_op (bd, d, prec, bd->z, bd->w, 1);
   if (!rc) rc = shk_dev_bd_dots (d, bd->y, bd->z, bd->w, n, bd->st_dev);
   /* step 1: drift at x, half step */
-  if (!rc) rc = shk_dev_bd_force (d, bd->f, bd->F, bd->z, bd->sc_dev, n);
-  if (!rc) rc = shk_dev_atimes (d, bd->f, bd->u, 1, prec);
-  if (!rc) rc = shk_dev_bd_euler (d, bd->xm, x, bd->u, bd->sc_dev, 0.5, n);
+  if (!rc)
+    rc = bd->ver == 0
+      ? shk_dev_bd_force (d, bd->f, bd->F, bd->z, bd->sc_dev, n)
+      : shk_dev_bd_force_ft (d, bd->f, bd->F, bd->T, bd->z, bd->sc_dev, bd->np);
+  if (!rc) rc = bd_op (bd, d, prec, bd->f, bd->u, 0);
+  /* the half-step orientation is not needed: no force here depends on it */
+  if (!rc)
+    rc = bd->ver == 0
+      ? shk_dev_bd_euler (d, bd->xm, x, bd->u, bd->sc_dev, 0.5, n3)
+      : shk_dev_bd_euler_ft (d, bd->xm, x, NULL, NULL, bd->u, bd->sc_dev, 0.5,
+                             bd->np);
   if (!rc && check_ol)
     rc = shk_dev_bd_overlap (d, bd->xm, NULL, s->np, s->periodic, s->lx, s->ly,
                              s->lz, 0, 0.0, bd->rmin_ol, &bd->st_dev->overlap[0]);
   /* step 2: drift at x_mid with the same forces, full step from x */
   if (!rc && bd->real_cache) rc = shk_dev_real_cache (d, 0, prec);
   if (!rc) rc = shk_dev_set_pos (d, bd->xm, 1);
-  if (!rc) rc = shk_dev_atimes (d, bd->f, bd->u, 1, prec);
-  if (!rc) rc = shk_dev_bd_euler (d, bd->xn, x, bd->u, bd->sc_dev, 1.0, n);
+  if (!rc) rc = bd_op (bd, d, prec, bd->f, bd->u, 0);
+  if (!rc)
+    rc = bd->ver == 0
+      ? shk_dev_bd_euler (d, bd->xn, x, bd->u, bd->sc_dev, 1.0, n3)
+      : shk_dev_bd_euler_ft (d, bd->xn, x, q ? bd->qn : NULL, q, bd->u,
+                             bd->sc_dev, 1.0, bd->np);
   if (!rc && check_ol)
     rc = shk_dev_bd_overlap (d, bd->xn, NULL, s->np, s->periodic, s->lx, s->ly,
                              s->lz, 0, 0.0, bd->rmin_ol, &bd->st_dev->overlap[1]);
@@ -292,7 +413,7 @@ bd_record (stokes_hip_bd *bd, shk_dev *d, int prec, int check_ol, double *x)
 }
 
 static int
-bd_capture (stokes_hip_bd *bd, stokes_ctx *c, double *x)
+bd_capture (stokes_hip_bd *bd, stokes_ctx *c, double *x, double *q)
 {
   shk_dev *d = c->dev;
   const int check_ol = bd->sys->rmin == 0.0;
@@ -322,7 +443,7 @@ bd_capture (stokes_hip_bd *bd, stokes_ctx *c, double *x)
       shk_dev_timing (d, c->timing);
       return bd_fail (rc, "bd_step (capture)");
     }
-  rc = bd_record (bd, d, c->prec, check_ol, x);
+  rc = bd_record (bd, d, c->prec, check_ol, x, q);
   if (rc)
     {
       shk_dev_capture_abort (d);
@@ -334,6 +455,7 @@ bd_capture (stokes_hip_bd *bd, stokes_ctx *c, double *x)
   if (rc) return bd_fail (rc, "bd_step (instantiate)");
   bd->graph_c = c;
   bd->graph_x = x;
+  bd->graph_q = q;
   bd->graph_gen = shk_dev_generation (d);
   bd->graph_uploads = c->uploads;
   bd->graph_prec = c->prec;
@@ -342,22 +464,38 @@ bd_capture (stokes_hip_bd *bd, stokes_ctx *c, double *x)
   return 0;
 }
 
+/* host-vector form of bd_op (noise parameters) for the Lanczos bounds */
+static int
+bd_host_op (void *ud, const double *x, double *y)
+{
+  stokes_hip_bd *bd = ud;
+  shk_dev *d = bd->c->dev;
+  const size_t bytes = sizeof (double) * bd->n;
+  double *xd = bd->work, *yd = bd->work + bd->n;
+  int rc = shk_dev_h2d (d, xd, x, bytes);
+  if (!rc) rc = bd_op (bd, d, bd->c->prec, xd, yd, 1);
+  if (!rc) rc = shk_dev_d2h (d, y, yd, bytes);
+  return rc ? bd_fail (rc, "bd_fit_bounds") : 0;
+}
+
 static int
 bd_fit_bounds (stokes_hip_bd *bd, stokes_ctx *c, double *x)
 {
   double l0, l1;
   int level = bd->fits < 3 ? bd->fits : 3;
   c->pos_dev = x;
-  int rc = 0;
+  /* the products are at x (an attempt leaves x_mid on the device) */
+  int rc = stokes_hip_prepare (c, 1);
+  if (rc) return rc;
   if (bd->real_cache)
     {
       /* every Lanczos product is at x: one dense real-space matrix */
-      rc = stokes_hip_prepare (c, 1);
-      if (!rc) rc = shk_dev_real_cache (c->dev, 1, c->prec);
+      rc = shk_dev_real_cache (c->dev, 1, c->prec);
       if (rc) return bd_fail (rc, "bd_fit_bounds");
     }
-  rc = stokes_hip_lanczos_bounds (bd->sys, BD_LANCZOS_ITER * (level + 1),
-                                  1 + bd->seed + bd->fits, 0.0, &l0, &l1);
+  rc = stokes_hip_lanczos_op ((int) bd->n, BD_LANCZOS_ITER * (level + 1),
+                              1 + bd->seed + bd->fits, 0.0, bd_host_op, bd,
+                              &l0, &l1);
   if (bd->real_cache) shk_dev_real_cache (c->dev, 0, c->prec);
   if (rc) return rc;
   bd->fits++;
@@ -369,11 +507,19 @@ int
 stokes_hip_bd_step (stokes_hip_bd *bd, double *x_dev, double dt,
                     stokes_hip_bd_info *info)
 {
+  return stokes_hip_bd_step_q (bd, x_dev, NULL, dt, info);
+}
+
+int
+stokes_hip_bd_step_q (stokes_hip_bd *bd, double *x_dev, double *q_dev,
+                      double dt, stokes_hip_bd_info *info)
+{
   if (!bd || !x_dev || !(dt > 0.0))
     {
       stokes_hip_set_error ("bd_step: need a handle, positions and dt > 0");
       return STOKES_HIP_EARG;
     }
+  if (bd->ver == 0) q_dev = NULL;      /* evolve_Euler_3all ignores q for F */
   stokes_ctx *c = stokes_hip_ctx (bd->sys);
   if (!c) return STOKES_HIP_ENODEV;
   shk_dev *d = c->dev;
@@ -404,13 +550,15 @@ stokes_hip_bd_step (stokes_hip_bd *bd, double *x_dev, double dt,
           rc = STOKES_HIP_EARG;
           goto out;
         }
-      if (!bd->graph || bd->graph_c != c || bd->graph_x != x_dev
-          || bd->graph_gen != shk_dev_generation (d)
-          || bd->graph_uploads != c->uploads || bd->graph_prec != c->prec
-          || bd->graph_ol != (bd->sys->rmin == 0.0)
-          || bd->graph_rc != bd->real_cache)
+      if (bd->ver != 2
+          && (!bd->graph || bd->graph_c != c || bd->graph_x != x_dev
+              || bd->graph_q != q_dev
+              || bd->graph_gen != shk_dev_generation (d)
+              || bd->graph_uploads != c->uploads || bd->graph_prec != c->prec
+              || bd->graph_ol != (bd->sys->rmin == 0.0)
+              || bd->graph_rc != bd->real_cache))
         {
-          rc = bd_capture (bd, c, x_dev);
+          rc = bd_capture (bd, c, x_dev, q_dev);
           if (rc) goto out;
         }
       if (fact != bd->sc_fact || dt_local != bd->sc_dt)
@@ -428,7 +576,14 @@ stokes_hip_bd_step (stokes_hip_bd *bd, double *x_dev, double dt,
           rc = shk_dev_counter_add (d, bd->counter, (unsigned long long) -1);
           if (rc) { rc = bd_fail (rc, "bd_step"); goto out; }
         }
-      rc = shk_dev_graph_launch (d, bd->graph);
+      if (bd->ver == 2)
+        {
+          rc = bd_record (bd, d, c->prec, bd->sys->rmin == 0.0, x_dev, q_dev);
+          if (bd->real_cache && rc)
+            shk_dev_real_cache (d, 0, c->prec);
+        }
+      else
+        rc = shk_dev_graph_launch (d, bd->graph);
       if (!rc) rc = shk_dev_sync (d);
       if (rc) { rc = bd_fail (rc, "bd_step (graph)"); goto out; }
       inf.attempts++;
@@ -473,7 +628,9 @@ stokes_hip_bd_step (stokes_hip_bd *bd, double *x_dev, double dt,
         }
     }
   /* accepted: x <- x_new */
-  rc = shk_dev_d2d (d, x_dev, bd->xn, sizeof (double) * bd->n);
+  rc = shk_dev_d2d (d, x_dev, bd->xn, sizeof (double) * 3 * bd->np);
+  if (!rc && q_dev)
+    rc = shk_dev_d2d (d, q_dev, bd->qn, sizeof (double) * 4 * bd->np);
   if (rc) { rc = bd_fail (rc, "bd_step"); goto out; }
   inf.dt = dt_local;
 out:

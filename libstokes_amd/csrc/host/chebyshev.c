@@ -269,11 +269,26 @@ tridiag_eig (int m, const double *al, const double *be, int kth)
   return 0.5 * (lo + hi);
 }
 
+static int
+sys_op (void *ud, const double *x, double *y)
+{
+  struct stokes *sys = ud;
+  return stokes_hip_atimes (sys, sys->np * stokes_dim (sys->version), x, y);
+}
+
 int
 stokes_hip_lanczos_bounds (struct stokes *sys, int niter, unsigned long seed,
                            double margin, double *lmin, double *lmax)
 {
-  int n = sys->np * stokes_dim (sys->version);
+  return stokes_hip_lanczos_op (sys->np * stokes_dim (sys->version), niter,
+                                seed, margin, sys_op, sys, lmin, lmax);
+}
+
+int
+stokes_hip_lanczos_op (int n, int niter, unsigned long seed, double margin,
+                       stokes_hip_host_op op, void *ud, double *lmin,
+                       double *lmax)
+{
   if (niter < 2) niter = 2;
   if (niter > n) niter = n;
   double *V = malloc (sizeof (double) * (size_t) n * (size_t) (niter + 1));
@@ -305,7 +320,7 @@ stokes_hip_lanczos_bounds (struct stokes *sys, int niter, unsigned long seed,
   for (int j = 0; j < niter; j++)
     {
       double *vj = V + (size_t) j * n;
-      rc = stokes_hip_atimes (sys, n, vj, w);
+      rc = op (ud, vj, w);
       if (rc) break;
       double a = 0.0;
       for (int k = 0; k < n; k++) a += w[k] * vj[k];

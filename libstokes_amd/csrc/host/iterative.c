@@ -178,6 +178,53 @@ iter_params (const struct stokes *sys, const struct iter *it, int *m,
   if (*m > GMRES_MAX_RESTART) *m = GMRES_MAX_RESTART;
 }
 
+size_t
+stokes_hip_fts_ws (int np, int m)
+{
+  const size_t n = (size_t) np * 11;
+  if (m > GMRES_MAX_RESTART) m = GMRES_MAX_RESTART;
+  return n * (size_t) (m + 1) + 4 * n + (size_t) (m + 2);
+}
+
+int
+stokes_hip_fts_solve (stokes_ctx *c, int kind, int nm, const double *b_dev,
+                      double *x_dev, int m, int maxit, double eps, double *ws,
+                      int *iters, double *relres)
+{
+  const int np = c->sys->np;
+  const size_t n = (size_t) np * 11;
+  if (m > GMRES_MAX_RESTART) m = GMRES_MAX_RESTART;
+  double *V = ws, *w = V + n * (size_t) (m + 1), *b = w + n, *t1 = b + n,
+    *t2 = t1 + n, *hd = t2 + n;
+  mix_op op = {c->dev, np, nm, c->prec, t1, t2};
+  int rc;
+  if (kind == 0)
+    rc = shk_dev_scal (c->dev, b, b_dev, 1.0, n);
+  else
+    {
+      /* b = M [(F, T, 0)_m, 0_f] - [(0, 0, E)_m, (U, O, E)_f]
+       * (calc_b_mix_3fts, ewald-3fts.c:363-438) */
+      rc = shk_dev_fts_split (c->dev, b_dev, np, nm, t1, NULL);
+      if (!rc) rc = shk_dev_atimes (c->dev, t1, t2, 1, c->prec);
+      if (!rc) rc = shk_dev_fts_combine (c->dev, b_dev, t2, np, nm, 1, b);
+    }
+  int it = 0;
+  double rr = 0.0;
+  if (!rc)
+    rc = gmres_dev (c->dev, n, kind == 0 ? res_apply : mix_apply, &op, b,
+                    x_dev, m, maxit, eps, V, w, hd, &it, &rr);
+  if (iters) *iters = it;
+  if (relres) *relres = rr;
+  return rc;
+}
+
+void
+stokes_hip_iter_params (const struct stokes *sys, int *m, int *maxit,
+                        double *eps)
+{
+  iter_params (sys, NULL, m, maxit, eps);
+}
+
 /* kind 0: resistance, b_dev = (U, O, E) per particle -> x = (F, T, S);
  * kind 1: mixed with nm mobile particles, b_dev = [(F, T, E)_m,
  * (U, O, E)_f] -> x = [(U, O, S)_m, (F, T, S)_f] */
@@ -208,33 +255,18 @@ solve_fts_dev (struct stokes *sys, int kind, int nm, const double *b_dev,
   int m, maxit;
   double eps;
   iter_params (sys, NULL, &m, &maxit, &eps);
-  size_t n = (size_t) sys->np * 11;
   double *buf = NULL;
-  size_t nd = n * (size_t) (m + 1) + 4 * n + (size_t) (m + 2);
-  rc = shk_dev_malloc (c->dev, (void **) &buf, sizeof (double) * nd);
+  rc = shk_dev_malloc (c->dev, (void **) &buf,
+                       sizeof (double) * stokes_hip_fts_ws (sys->np, m));
   if (rc)
     {
       stokes_hip_set_error ("%s: %s", what, shk_last_error ());
       return STOKES_HIP_ENOMEM;
     }
-  double *V = buf, *w = V + n * (size_t) (m + 1), *b = w + n, *t1 = b + n,
-    *t2 = t1 + n, *hd = t2 + n;
-  mix_op op = {c->dev, sys->np, nm, c->prec, t1, t2};
-  if (kind == 0)
-    rc = shk_dev_scal (c->dev, b, b_dev, 1.0, n);
-  else
-    {
-      /* b = M [(F, T, 0)_m, 0_f] - [(0, 0, E)_m, (U, O, E)_f]
-       * (calc_b_mix_3fts, ewald-3fts.c:363-438) */
-      rc = shk_dev_fts_split (c->dev, b_dev, sys->np, nm, t1, NULL);
-      if (!rc) rc = shk_dev_atimes (c->dev, t1, t2, 1, c->prec);
-      if (!rc) rc = shk_dev_fts_combine (c->dev, b_dev, t2, sys->np, nm, 1, b);
-    }
   int it = 0;
   double rr = 0.0;
-  if (!rc)
-    rc = gmres_dev (c->dev, n, kind == 0 ? res_apply : mix_apply, &op, b,
-                    x_dev, m, maxit, eps, V, w, hd, &it, &rr);
+  rc = stokes_hip_fts_solve (c, kind, nm, b_dev, x_dev, m, maxit, eps, buf,
+                             &it, &rr);
   shk_dev_free (c->dev, buf);
   if (rc)
     {

@@ -73,4 +73,25 @@ int stokes_hip_cheb_launch (shk_dev *dev, int prec, int ncheb, const double *a,
                             int nrhs, size_t len, const double *y, double *z,
                             double eig0, double eig1, double *work);
 
+/* restarted GMRES for the FTS problems on the device operator of c
+ * (parameters and positions already uploaded): kind 0 resistance (b = (U,
+ * O, E) -> x = (F, T, S)), kind 1 mixed with nm mobile particles (nm = np:
+ * the mobility problem, b = (F, T, E) -> x = (U, O, S)); ws holds
+ * stokes_hip_fts_ws (np, m) doubles of device memory.  Returns an SHK_E*
+ * code. */
+size_t stokes_hip_fts_ws (int np, int m);
+/* GMRES parameters of stokes_set_iter (restart, max, eps; defaults 30,
+ * 2000, 1e-12) */
+void stokes_hip_iter_params (const struct stokes *sys, int *m, int *maxit,
+                             double *eps);
+int stokes_hip_fts_solve (stokes_ctx *c, int kind, int nm, const double *b_dev,
+                          double *x_dev, int m, int maxit, double eps,
+                          double *ws, int *iters, double *relres);
+/* Lanczos extreme eigenvalues of a symmetric operator of dimension n given
+ * as op (ud, x, y) on host vectors (returns a STOKES_HIP_E* code) */
+typedef int (*stokes_hip_host_op) (void *ud, const double *x, double *y);
+int stokes_hip_lanczos_op (int n, int niter, unsigned long seed, double margin,
+                           stokes_hip_host_op op, void *ud, double *lmin,
+                           double *lmax);
+
 #endif

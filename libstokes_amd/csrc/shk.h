@@ -195,6 +195,18 @@ int shk_dev_bd_force (shk_dev *d, double *f, const double *F, const double *z,
 /* xo = xi + frac dt u */
 int shk_dev_bd_euler (shk_dev *d, double *xo, const double *xi, const double *u,
                       const shk_bd_scalars *sc, double frac, size_t n);
+/* FT / FTS, np particles: f[6 i + q] = (F | T)[3 i + q % 3] + fact z[6 i + q]
+ * (F, T may be NULL) */
+int shk_dev_bd_force_ft (shk_dev *d, double *f, const double *F, const double *T,
+                         const double *z, const shk_bd_scalars *sc, int np);
+/* FT / FTS: xo = xi + frac dt U, and qo = qi + frac dt dQ/dt(qi, O) when qi
+ * is not NULL; u = (U, O) per particle of 6 */
+int shk_dev_bd_euler_ft (shk_dev *d, double *xo, const double *xi, double *qo,
+                         const double *qi, const double *u,
+                         const shk_bd_scalars *sc, double frac, int np);
+/* (F, T) of 6 per particle -> (F, T, 0) of 11 (mode 0), or the first 6 of
+ * 11 back (mode 1) */
+int shk_dev_ft_fts (shk_dev *d, const double *x, int np, int mode, double *y);
 /* st->yy = y.y, st->zmz = z.w */
 int shk_dev_bd_dots (shk_dev *d, const double *y, const double *z,
                      const double *w, size_t n, shk_bd_status *st);

@@ -1042,31 +1042,147 @@ orc_bd_fact (double peclet, double dt)
   return peclet > 0.0 ? sqrt (2.0 / (peclet * dt)) : 0.0;
 }
 
-/* One mid-point step, F version, every particle mobile, no imposed flow,
- * no interaction forces: brownian.c:2352-2560 with BD_calc_FB's atimes
- * route (1545-1874, spectral bounds and coefficients given, so the refit
- * of 1707-1727 does not happen), BD_calc_forces (2273-2311) and
- * evolve_Euler_3all (1889-1930).  The mobility of the F version is one
- * atimes_3all (brownian.c:512-520, ewald-3f.c:281-285).
- *
- * draws[ndraw][3np] are the Gaussian vectors BD_calc_FB consumes, one per
- * (re)start of the noise.  fix_fact != 0 recomputes fact after a redraw
- * resets dt (the reference keeps the halved-dt value).  x is updated in
- * place; the return value is the dt taken, or -1 when more draws are
- * needed.  Counts go to *used / *halvings; *err gets the Chebyshev
- * accuracy of the last noise. */
-double
-orc_bd_mid_F (orc_sys *s, double *x, const double *F, double peclet,
-              double dt, double dt_lim, double rmin_ol, int ncheb,
-              const double *acoef, double e0, double e1, int ndraw,
-              const double *draws, int fix_fact, int *used, int *halvings,
-              double *err)
+/* ode-quaternion.c:58-79 (quaternion_Winv) and 89-105 (quaternion_dQdt):
+ * dQ/dt = 1/2 W(Q)^{-1} (O, 0), W(Q)^{-1} = W(Q)^T / |Q|^2 */
+void
+orc_quat_dqdt (const double *Q, const double *O, double *dQdt)
 {
-  const int n = 3 * s->np;
+  double den = Q[0] * Q[0] + Q[1] * Q[1] + Q[2] * Q[2] + Q[3] * Q[3];
+  double W[16] = {
+    +Q[3] / den, -Q[2] / den, +Q[1] / den, +Q[0] / den,
+    +Q[2] / den, +Q[3] / den, -Q[0] / den, +Q[1] / den,
+    -Q[1] / den, +Q[0] / den, +Q[3] / den, +Q[2] / den,
+    -Q[0] / den, -Q[1] / den, -Q[2] / den, +Q[3] / den };
+  for (int i = 0; i < 4; i++)
+    {
+      dQdt[i] = 0.0;
+      for (int j = 0; j < 3; j++) dQdt[i] += 0.5 * W[i * 4 + j] * O[j];
+    }
+}
+
+/* The FTS mobility problem with E = 0 restricted to (F, T) -> (U, O)
+ * particle-wise (BD_atimes_mob_FU, brownian.c:597-646, which solves it with
+ * GMRES through solve_mob_3fts_0): the true operator M (11 np)^2 is the
+ * extracted-form matrix (orc_matrix_ewald / _nonewald) times the S
+ * transform of fts.c:660-670 (columns 6 and 10 of a particle: 2 m_6 +
+ * m_10 and 2 m_10 + m_6, columns 7-9: 2 m_c), S = -M_ES^{-1} M_EF (F, T)
+ * by Gaussian elimination with partial pivoting, (U, O) = M_UF (F, T) +
+ * M_US S.  Exact to rounding: the limit of the reference's iteration.
+ * O(np^3); small systems only. */
+void
+orc_fts_mob_ft (const orc_sys *s, const double *ft, double *uo)
+{
+  const int np = s->np, n = 11 * np, ne = 5 * np;
+  double *M = malloc (sizeof (double) * (size_t) n * n);
+  double *A = malloc (sizeof (double) * (size_t) ne * ne);
+  double *b = malloc (sizeof (double) * ne);
+  double *S = malloc (sizeof (double) * ne);
+  if (s->periodic) orc_matrix_ewald (s, M);
+  else orc_matrix_nonewald (s, M);
+  for (int r = 0; r < n; r++)
+    for (int i = 0; i < np; i++)
+      {
+        double *m = M + (size_t) r * n + 11 * i + 6;
+        double m6 = m[0], m10 = m[4];
+        m[0] = 2.0 * m6 + m10;
+        m[1] *= 2.0;
+        m[2] *= 2.0;
+        m[3] *= 2.0;
+        m[4] = 2.0 * m10 + m6;
+      }
+  /* rows / columns of E, S: 11 i + 6 + k */
+#define ES(i, k) (11 * (i) + 6 + (k))
+#define FT(i, k) (11 * (i) + (k))
+  for (int a = 0; a < ne; a++)
+    {
+      int ra = ES (a / 5, a % 5);
+      double acc = 0.0;
+      for (int c = 0; c < 6 * np; c++)
+        acc += M[(size_t) ra * n + FT (c / 6, c % 6)] * ft[c];
+      b[a] = -acc;
+      for (int c = 0; c < ne; c++)
+        A[(size_t) a * ne + c] = M[(size_t) ra * n + ES (c / 5, c % 5)];
+    }
+  for (int k = 0; k < ne; k++)
+    {
+      int p = k;
+      for (int r = k + 1; r < ne; r++)
+        if (fabs (A[(size_t) r * ne + k]) > fabs (A[(size_t) p * ne + k])) p = r;
+      if (p != k)
+        {
+          for (int c = 0; c < ne; c++)
+            {
+              double t = A[(size_t) k * ne + c];
+              A[(size_t) k * ne + c] = A[(size_t) p * ne + c];
+              A[(size_t) p * ne + c] = t;
+            }
+          double t = b[k]; b[k] = b[p]; b[p] = t;
+        }
+      for (int r = k + 1; r < ne; r++)
+        {
+          double f = A[(size_t) r * ne + k] / A[(size_t) k * ne + k];
+          for (int c = k; c < ne; c++) A[(size_t) r * ne + c] -= f * A[(size_t) k * ne + c];
+          b[r] -= f * b[k];
+        }
+    }
+  for (int k = ne - 1; k >= 0; k--)
+    {
+      double acc = b[k];
+      for (int c = k + 1; c < ne; c++) acc -= A[(size_t) k * ne + c] * S[c];
+      S[k] = acc / A[(size_t) k * ne + k];
+    }
+  for (int r = 0; r < 6 * np; r++)
+    {
+      int rr = FT (r / 6, r % 6);
+      double acc = 0.0;
+      for (int c = 0; c < 6 * np; c++) acc += M[(size_t) rr * n + FT (c / 6, c % 6)] * ft[c];
+      for (int c = 0; c < ne; c++) acc += M[(size_t) rr * n + ES (c / 5, c % 5)] * S[c];
+      uo[r] = acc;
+    }
+#undef ES
+#undef FT
+  free (M); free (A); free (b); free (S);
+}
+
+/* the mobility of BD_atimes_mob_FU (brownian.c:494-739), every particle
+ * mobile, equal radii (the Global / Particle scalings are identities) */
+void
+orc_bd_op_cb (int n, const double *x, double *y, void *ud)
+{
+  (void) n;
+  const orc_sys *s = (const orc_sys *) ud;
+  if (s->version == 2) orc_fts_mob_ft (s, x, y);
+  else orc_atimes_3all (s, x, y);
+}
+
+/* One mid-point step, every particle mobile, no imposed flow, no
+ * interaction forces: brownian.c:2352-2560 with BD_calc_FB's atimes route
+ * (1545-1874, spectral bounds and coefficients given, so the refit of
+ * 1707-1727 does not happen), BD_calc_forces (2273-2311) and
+ * evolve_Euler_3all (1889-1930).  The mobility is orc_bd_op_cb: one
+ * atimes_3all for F (brownian.c:512-520) and FT (559-587), the E = 0
+ * mobility problem for FTS (597-646).  For FT / FTS the noise is
+ * particle-wise (F, T) (the reference reorders it to [F; T] and back,
+ * 1851-1871 and solve_mix_3all), f = F + fact z_F, t = T + fact z_T.
+ *
+ * draws[ndraw][n] (n = 3 np or 6 np) are the Gaussian vectors BD_calc_FB
+ * consumes, one per (re)start of the noise.  fix_fact != 0 recomputes fact
+ * after a redraw resets dt (the reference keeps the halved-dt value).  x
+ * (and q [4 np] for FT / FTS when not NULL) are updated in place on
+ * success; returns the dt taken, or -1 if the draws ran out. */
+double
+orc_bd_mid (orc_sys *s, double *x, double *q, const double *F,
+            const double *T, double peclet, double dt, double dt_lim,
+            double rmin_ol, int ncheb, const double *acoef, double e0,
+            double e1, int ndraw, const double *draws, int fix_fact,
+            int *used, int *halvings, double *err)
+{
+  const int np = s->np, ver = s->version;
+  const int n = np * (ver == 0 ? 3 : 6), n3 = 3 * np;
   double *z = malloc (sizeof (double) * n);
   double *f = malloc (sizeof (double) * n);
   double *u = malloc (sizeof (double) * n);
-  double *xmid = malloc (sizeof (double) * n);
+  double *xmid = malloc (sizeof (double) * n3);
   double dt_local = dt;
   double fact = orc_bd_fact (peclet, dt_local);
   int k = 0, nh = 0, stage;
@@ -1077,28 +1193,41 @@ redo:                              /* BD_evolve_mid_REDO */
   s->pos = x;
 redo_fb:                           /* BD_evolve_mid_REDO_FB */
   if (k >= ndraw) goto done;
-  orc_bd_noise_F (s, ncheb, acoef, e0, e1, draws + (size_t) k * n, z);
+  orc_cheb_eval_atimes (ncheb, acoef, n, draws + (size_t) k * n, z, e0, e1,
+                        orc_bd_op_cb, (void *) s);
   *err = orc_cheb_error_minvsqrt (n, draws + (size_t) k * n, z,
-                                  orc_atimes_sys_cb, (void *) s);
+                                  orc_bd_op_cb, (void *) s);
   k++;
 redo_scale:                        /* BD_evolve_mid_REDO_scale */
-  for (int i = 0; i < n; i++) xmid[i] = x[i];
-  for (int i = 0; i < n; i++) f[i] = (F ? F[i] : 0.0) + fact * z[i];
+  for (int i = 0; i < n; i++)
+    {
+      const double *ext = ver == 0 ? F : ((i % 6) < 3 ? F : T);
+      int ie = ver == 0 ? i : 3 * (i / 6) + (i % 3);
+      f[i] = (ext ? ext[ie] : 0.0) + fact * z[i];
+    }
+  for (int i = 0; i < n3; i++) xmid[i] = x[i];
   s->pos = x;
-  orc_atimes_3all (s, f, u);
-  for (int i = 0; i < n; i++) xmid[i] += dt_local * 0.5 * u[i];
+  orc_bd_op_cb (n, f, u, (void *) s);
+  for (int i = 0; i < n3; i++)
+    xmid[i] += dt_local * 0.5 * u[ver == 0 ? i : 6 * (i / 3) + (i % 3)];
   stage = 1;
   if (s->rmin == 0.0 && orc_check_overlap (s, xmid, rmin_ol) > 0)
     goto overlap;
   s->pos = xmid;
-  for (int i = 0; i < n; i++) f[i] = (F ? F[i] : 0.0) + fact * z[i];
-  orc_atimes_3all (s, f, u);
-  for (int i = 0; i < n; i++) xmid[i] = x[i];
-  for (int i = 0; i < n; i++) xmid[i] += dt_local * u[i];
+  orc_bd_op_cb (n, f, u, (void *) s);
+  for (int i = 0; i < n3; i++)
+    xmid[i] = x[i] + dt_local * u[ver == 0 ? i : 6 * (i / 3) + (i % 3)];
   stage = 2;
   if (s->rmin == 0.0 && orc_check_overlap (s, xmid, rmin_ol) > 0)
     goto overlap;
-  for (int i = 0; i < n; i++) x[i] = xmid[i];
+  for (int i = 0; i < n3; i++) x[i] = xmid[i];
+  if (ver > 0 && q)
+    for (int i = 0; i < np; i++)
+      {
+        double dq[4];
+        orc_quat_dqdt (q + 4 * i, u + 6 * i + 3, dq);
+        for (int c = 0; c < 4; c++) q[4 * i + c] += dt_local * dq[c];
+      }
   ret = dt_local;
   goto done;
 overlap:
@@ -1119,4 +1248,17 @@ done:
   *halvings = nh;
   free (z); free (f); free (u); free (xmid);
   return ret;
+}
+
+/* the F-version step (orc_bd_mid with no torques or orientations) */
+double
+orc_bd_mid_F (orc_sys *s, double *x, const double *F, double peclet,
+              double dt, double dt_lim, double rmin_ol, int ncheb,
+              const double *acoef, double e0, double e1, int ndraw,
+              const double *draws, int fix_fact, int *used, int *halvings,
+              double *err)
+{
+  return orc_bd_mid (s, x, NULL, F, NULL, peclet, dt, dt_lim, rmin_ol, ncheb,
+                     acoef, e0, e1, ndraw, draws, fix_fact, used, halvings,
+                     err);
 }

@@ -120,6 +120,18 @@ void orc_bd_noise_F (const orc_sys *s, int ncheb, const double *acoef,
 /* overlapping (pair, image) count (brownian.c:299-375) */
 int orc_check_overlap (const orc_sys *s, const double *pos, double rmin);
 
+/* quaternion_dQdt (ode-quaternion.c:89-105) */
+void orc_quat_dqdt (const double *Q, const double *O, double *dQdt);
+/* FTS mobility problem with E = 0, (F, T) -> (U, O) particle-wise */
+void orc_fts_mob_ft (const orc_sys *s, const double *ft, double *uo);
+/* the BD mobility of the system's version (BD_atimes_mob_FU) */
+void orc_bd_op_cb (int n, const double *x, double *y, void *ud);
+/* one mid-point BD step, F / FT / FTS (brownian.c:2352-2560); see oracle.c */
+double orc_bd_mid (orc_sys *s, double *x, double *q, const double *F,
+                   const double *T, double peclet, double dt, double dt_lim,
+                   double rmin_ol, int ncheb, const double *acoef, double e0,
+                   double e1, int ndraw, const double *draws, int fix_fact,
+                   int *used, int *halvings, double *err);
 /* one mid-point BD step, F version (brownian.c:2352-2560); see oracle.c */
 double orc_bd_mid_F (orc_sys *s, double *x, const double *F, double peclet,
                      double dt, double dt_lim, double rmin_ol, int ncheb,

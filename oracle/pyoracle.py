@@ -69,6 +69,7 @@ def lib():
         L.orc_rng_real2.restype = C.c_double
         L.orc_rng_gaussian.restype = C.c_double
         L.orc_bd_mid_F.restype = C.c_double
+        L.orc_bd_mid.restype = C.c_double
         L.orc_check_overlap.restype = C.c_int
         L.orc_xi_by_tratio.argtypes = [C.POINTER(OrcSys), C.c_double]
         L.orc_set_xi.argtypes = [C.POINTER(OrcSys), C.c_double, C.c_double]
@@ -210,6 +211,40 @@ class System:
                              C.byref(used), C.byref(nh), C.byref(err))
         return x, dtt, used.value, nh.value, err.value
 
+    def bd_mid(self, x, draws, peclet, dt, ncheb, acoef, e0, e1, q=None,
+               F=None, T=None, dt_lim=1e-6, rmin_ol=1.0, fix_fact=False):
+        """one mid-point BD step (brownian.c:2352-2560) of the system's
+        version (F, FT, FTS); draws[k] has 3 np (F) or 6 np values; returns
+        (x_new, q_new, dt_taken, draws_used, halvings, err_cheb)"""
+        x = np.array(x, dtype=np.float64).reshape(-1)
+        nop = x.size * (1 if self.version == 0 else 2)
+        draws = np.ascontiguousarray(draws, np.float64).reshape(-1)
+        nd = draws.size // nop
+        a = np.ascontiguousarray(acoef, np.float64)
+        qq = None if q is None else np.array(q, dtype=np.float64).reshape(-1)
+        used, nh, err = C.c_int(), C.c_int(), C.c_double()
+
+        def opt(v):
+            return None if v is None else _p(np.ascontiguousarray(v, np.float64))
+        Fa = None if F is None else np.ascontiguousarray(F, np.float64)
+        Ta = None if T is None else np.ascontiguousarray(T, np.float64)
+        dtt = lib().orc_bd_mid(
+            C.byref(self.s), _p(x), None if qq is None else _p(qq), opt(Fa),
+            opt(Ta), C.c_double(peclet), C.c_double(dt), C.c_double(dt_lim),
+            C.c_double(rmin_ol), C.c_int(ncheb), _p(a), C.c_double(e0),
+            C.c_double(e1), C.c_int(nd), _p(draws),
+            C.c_int(1 if fix_fact else 0), C.byref(used), C.byref(nh),
+            C.byref(err))
+        return x, qq, dtt, used.value, nh.value, err.value
+
+    def bd_op(self, x):
+        """the BD mobility of the system's version (BD_atimes_mob_FU): F ->
+        U, (F, T) -> (U, O), or for FTS the E = 0 mobility problem"""
+        x = np.ascontiguousarray(x, np.float64).reshape(-1)
+        y = np.zeros_like(x)
+        lib().orc_bd_op_cb(C.c_int(x.size), _p(x), _p(y), C.byref(self.s))
+        return y
+
     # ---- compiled reference on the same system --------------------------
     def ref_atimes(self, x):
         R = ref()
@@ -273,6 +308,18 @@ def solve_mix_dense(system, nm, b):
     x[l] = L_ @ b[l] + K_ @ b[h]
     x[h] = J_ @ b[l] + I_ @ b[h]
     return x
+
+
+def ft_mobility_dense(system):
+    """(F, T) -> (U, O) particle-wise of the FTS mobility problem with E =
+    0 (BD_atimes_mob_FU, brownian.c:597-646) as a dense (6 np)^2 matrix,
+    from the restated dense solve (solve_mix_dense)"""
+    n = system.np
+    B = np.zeros((11 * n, 6 * n))
+    for c in range(6 * n):
+        B[11 * (c // 6) + c % 6, c] = 1.0
+    X = solve_mix_dense(system, n, B)
+    return X.reshape(n, 11, 6 * n)[:, :6, :].reshape(6 * n, 6 * n)
 
 
 def extmat_fts(np_, m, x):
@@ -371,3 +418,12 @@ class Rng(C.Structure):
             return lib().orc_rng_gaussian(C.byref(self))
         return np.array([lib().orc_rng_gaussian(C.byref(self))
                          for _ in range(n)])
+
+
+def quat_dqdt(q, o):
+    """quaternion_dQdt (ode-quaternion.c:89-105) of the oracle"""
+    q = np.ascontiguousarray(q, np.float64)
+    o = np.ascontiguousarray(o, np.float64)
+    d = np.zeros(4)
+    lib().orc_quat_dqdt(_p(q), _p(o), _p(d))
+    return d

@@ -303,3 +303,84 @@ def test_bd_steps_with_and_without_real_cache():
     print(f"  BD 3 steps, cached vs matrix-free: {d:.2e} of the displacement",
           flush=True)
     assert d < 1e-12
+
+
+@pytest.mark.parametrize("version,n", [(1, 27), (2, 12)])
+def test_bd_ft_fts_steps_match_oracle(version, n):
+    """FT / FTS steps (positions and quaternions) against the oracle's step
+    fed with the device's draw; FTS with tight GMRES for the noise and the
+    step's solves"""
+    phi = 0.1
+    _say(f"bd parity version={version} N={n} phi={phi}")
+    pos, L = workloads.configuration(n, phi, seed=5)
+    s = ls.Stokes(version, pos, box=(L, L, L), tr=1.0)
+    o = po.System(version, pos, box=(L, L, L), tr=1.0)
+    G = o.matrix() if version == 1 else po.ft_mobility_dense(o)
+    w = np.linalg.eigvalsh(0.5 * (G + G.T))
+    e0, e1 = 0.95 * w[0], 1.05 * w[-1]
+    nch, pe, dt = 30, 1.0, 1e-3
+    a = po.cheb_coef(nch, 0, e0, e1)
+    rng = np.random.default_rng(3 + version)
+    F = rng.standard_normal(3 * n)
+    T = rng.standard_normal(3 * n)
+    q = rng.standard_normal((n, 4))
+    q /= np.linalg.norm(q, axis=1)[:, None]
+    q = q.reshape(-1)
+    Fd = ls.DeviceArray.from_host(F)
+    Td = ls.DeviceArray.from_host(T)
+    bd = ls.BD(s, peclet=pe, n_cheb=nch, eps=1e-6, dt_lim=1e-6, rmin_ol=1.0,
+               seed=9, force_dev=Fd, bounds=(e0, e1))
+    bd.set_torque(Td)
+    if version == 2:
+        s.set_iter(eps=1e-13)
+        bd.set_fts_iter(30, 2000, 1e-13)
+    xd = ls.DeviceArray.from_host(pos.reshape(-1))
+    qd = ls.DeviceArray.from_host(q)
+    x = pos.reshape(-1).copy()
+    for step in range(3):
+        info = bd.step(xd, dt, qd)
+        assert info.attempts == 1 and info.err_cheb < 1e-6
+        y, z = bd.last_noise()
+        assert y.size == 6 * n
+        o.pos[:] = x
+        xo, qo, dto, used, nh, err = o.bd_mid(x, y[None], pe, dt, nch, a, e0,
+                                              e1, q=q, F=F, T=T)
+        assert used == 1 and nh == 0 and dto == info.dt
+        xg, qg = xd.to_host(), qd.to_host()
+        disp = np.abs(xo - x).max()
+        ex = np.abs(xg - xo).max() / disp
+        eq = np.abs(qg - qo).max() / np.abs(qo - q).max()
+        _say(f"  step {step}: x err / disp {ex:.2e}, q err / dq {eq:.2e}")
+        assert ex < (1e-10 if version == 1 else 1e-8)
+        assert eq < (1e-10 if version == 1 else 1e-8)
+        x, q = xo, qo
+    bd.close()
+
+
+def test_bd_fts_reference_iteration_parameters():
+    """FTS step with the reference's GMRES parameters for the noise
+    (brownian.c:597-600: restart 20, eps 1e-6) and the Lanczos bounds of
+    the integrator: close to the exact-solve oracle step"""
+    n, phi = 12, 0.1
+    pos, L = workloads.configuration(n, phi, seed=6)
+    s = ls.Stokes(2, pos, box=(L, L, L), tr=1.0)
+    s.set_iter(eps=1e-12)
+    o = po.System(2, pos, box=(L, L, L), tr=1.0)
+    nch, pe, dt = 30, 1.0, 1e-3
+    bd = ls.BD(s, peclet=pe, n_cheb=nch, eps=1e-2, dt_lim=1e-6, rmin_ol=1.0,
+               seed=4)
+    xd = ls.DeviceArray.from_host(pos.reshape(-1))
+    info = bd.step(xd, dt)
+    _say(f"fts bd default iteration: {info.summary()}")
+    y, _ = bd.last_noise()
+    b0, b1 = info.eig
+    G = po.ft_mobility_dense(o)
+    w = np.linalg.eigvalsh(0.5 * (G + G.T))
+    assert b0 < w[0] and b1 > w[-1] and b0 > 0.5 * w[0]
+    a = po.cheb_coef(nch, 0, b0, b1)
+    xo, _, dto, used, nh, err = o.bd_mid(pos, y[None], pe, dt, nch, a, b0, b1)
+    disp = np.abs(xo - pos.reshape(-1)).max()
+    e = np.abs(xd.to_host() - xo).max() / disp
+    _say(f"  |x_gpu - x_oracle| / disp = {e:.2e}")
+    assert e < 1e-4
+    bd.close()

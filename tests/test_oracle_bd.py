@@ -208,3 +208,130 @@ def test_dense_mixed_solve_residual(version, nm):
     g = np.where(isH, x, b)          # forces / torques / stresslets
     v = np.where(isH, b, x)          # velocities / angular velocities / E
     assert np.abs(s.atimes(g) - v).max() < 1e-12 * np.abs(v).max()
+
+
+def _quat_W(q):
+    """W(Q) of ode-quaternion.c:28-48 (quaternion_W), the matrix whose
+    inverse quaternion_Winv writes out"""
+    return np.array([[+q[3], +q[2], -q[1], -q[0]],
+                     [-q[2], +q[3], +q[0], -q[1]],
+                     [+q[1], -q[0], +q[3], -q[2]],
+                     [+q[0], +q[1], +q[2], +q[3]]])
+
+
+def test_quaternion_rate():
+    """dQ/dt = 1/2 W^{-1} (O, 0) (ode-quaternion.c:89-105): the columns the
+    oracle applies equal numpy's inverse of W (what quaternion_Winv_lapack
+    computes); |Q| is conserved to first order; a rotation about z at rate
+    w turns the Euler-angle quaternion of quaternion_by_Euler
+    (ode-quaternion.c:107-121, theta = 0: Q = (0, 0, sin(a/2), cos(a/2)))
+    at da/dt = w"""
+    rng = np.random.default_rng(4)
+    for _ in range(5):
+        q = rng.standard_normal(4)
+        Winv = np.linalg.inv(_quat_W(q))
+        for j in range(3):
+            e = np.zeros(3)
+            e[j] = 1.0
+            assert np.allclose(2 * po.quat_dqdt(q, e), Winv[:, j],
+                               rtol=0, atol=1e-14 * np.abs(Winv).max())
+        o = rng.standard_normal(3)
+        assert abs(q @ po.quat_dqdt(q, o)) < 1e-14 * (q @ q) * np.abs(o).max()
+    a, w = 0.7, 1.3
+    q = np.array([0.0, 0.0, math.sin(a / 2), math.cos(a / 2)])
+    dq = po.quat_dqdt(q, np.array([0.0, 0.0, w]))
+    want = 0.5 * w * np.array([0.0, 0.0, math.cos(a / 2), -math.sin(a / 2)])
+    assert np.allclose(dq, want, rtol=0, atol=1e-15)
+
+
+def _eight(L=8.0, shift=0.3):
+    return np.array([[1, 1, 1], [4, 1.2, 0.8], [1.1, 4.2, 1], [4, 4, 1.3],
+                     [1, 1.1, 4], [4.2, 1, 4], [1, 4, 4.1], [4, 4.1, 4.2]],
+                    dtype=np.float64) + shift
+
+
+def _ft_of_fts_dense(s):
+    """(F, T) -> (U, O) of the FTS mobility problem with E = 0 from the
+    restated dense solve (ewald-3fts-matrix.c:570-652)"""
+    return po.ft_mobility_dense(s)
+
+
+def test_fts_mobility_of_forces_and_torques():
+    """the oracle's E = 0 mobility problem (BD_atimes_mob_FU,
+    brownian.c:597-646) equals the restated dense solve and is a symmetric
+    positive-definite map"""
+    L = 8.0
+    s = po.System(2, _eight(L), box=(L, L, L), tr=1.0)
+    G = _ft_of_fts_dense(s)
+    rng = np.random.default_rng(2)
+    x = rng.standard_normal(48)
+    y = s.bd_op(x)
+    assert np.abs(y - G @ x).max() < 1e-12 * np.abs(y).max()
+    assert np.abs(G - G.T).max() < 1e-12 * np.abs(G).max()
+    assert np.linalg.eigvalsh(0.5 * (G + G.T)).min() > 0
+
+
+@pytest.mark.parametrize("version", [1, 2])
+def test_bd_step_ft_fts_composition(version):
+    """brownian.c:2352-2560 for FT / FTS: x and q after one step against
+    the composition of the steps with the compiled reference's own
+    chebyshev_eval_atimes and atimes_3all (FT), or with the restated dense
+    solve and the Chebyshev restatement (FTS)"""
+    import ctypes as C
+    L = 8.0
+    pos = _eight(L)
+    s = po.System(version, pos, box=(L, L, L), tr=1.0)
+    n6 = 48
+    rng = np.random.default_rng(20 + version)
+    y = rng.standard_normal(n6)
+    F = rng.standard_normal(24)
+    T = rng.standard_normal(24)
+    q = rng.standard_normal(32)
+    pe, dt, nch = 1.0, 1e-3, 30
+    G0 = s.matrix() if version == 1 else _ft_of_fts_dense(s)
+    w = np.linalg.eigvalsh(0.5 * (G0 + G0.T))
+    e0, e1 = 0.95 * w[0], 1.05 * w[-1]
+    a = po.cheb_coef(nch, 0, e0, e1)
+    xn, qn, dtt, used, nh, err = s.bd_mid(pos, y[None], pe, dt, nch, a, e0,
+                                          e1, q=q, F=F, T=T)
+    assert used == 1 and nh == 0 and dtt == dt
+    fact = math.sqrt(2 / (pe * dt))
+
+    def ft(z):
+        out = np.empty(n6)
+        out.reshape(8, 6)[:, :3] = F.reshape(8, 3) + fact * z.reshape(8, 6)[:, :3]
+        out.reshape(8, 6)[:, 3:] = T.reshape(8, 3) + fact * z.reshape(8, 6)[:, 3:]
+        return out
+    if version == 1:
+        R = po.ref()
+
+        def ref_sys(p):
+            t = po.System(1, p, box=(L, L, L), tr=1.0)
+            return t, R.ref_sys_new(C.byref(t.s))
+        t0, h0 = ref_sys(pos)
+        z = np.zeros(n6)
+        R.ref_cheb_eval_sys(nch, po._p(a), h0, po._p(y), po._p(z), e0, e1)
+        f = ft(z)
+        u = np.zeros(n6)
+        R.ref_atimes(h0, n6, po._p(f), po._p(u))
+        xmid = pos.reshape(-1) + 0.5 * dt * u.reshape(8, 6)[:, :3].reshape(-1)
+        t1, h1 = ref_sys(xmid)
+        R.ref_atimes(h1, n6, po._p(f), po._p(u))
+        R.ref_sys_free(h0)
+        R.ref_sys_free(h1)
+    else:
+        z = po.cheb_eval_dense(a, G0, y, e0, e1)
+        f = ft(z)
+        u = G0 @ f
+        xmid = pos.reshape(-1) + 0.5 * dt * u.reshape(8, 6)[:, :3].reshape(-1)
+        s1 = po.System(2, xmid.reshape(-1, 3), box=(L, L, L), tr=1.0)
+        u = _ft_of_fts_dense(s1) @ f
+    U = u.reshape(8, 6)[:, :3].reshape(-1)
+    O = u.reshape(8, 6)[:, 3:]
+    want = pos.reshape(-1) + dt * U
+    disp = np.abs(want - pos.reshape(-1)).max()
+    assert np.abs(xn - want).max() < 1e-11 * disp
+    qw = q.reshape(8, 4) + dt * np.stack([po.quat_dqdt(q.reshape(8, 4)[i], O[i])
+                                          for i in range(8)])
+    assert np.abs(qn - qw.reshape(-1)).max() < 1e-13 * np.abs(q).max()
+    assert np.abs(qn - q).max() > 1e-6
