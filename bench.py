@@ -125,7 +125,48 @@ def cpu_baseline(pos, L, x, tr, budget_s):
     rate = (rows / n) / t
     return dict(value=rate, unit="matvecs/s", cores=1, kind="port",
                 sample=f"oracle FP64 rows 0..{rows - 1} of the N={n} FTS "
-                       f"matvec (tr={tr:g}), {t:.1f} s; rate = rows/N/time")
+                       f"matvec (tr={tr:g}), {t:.1f} s; rate = rows/N/time",
+                host=host_cpu())
+
+
+def host_cpu():
+    model = "?"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return f"{model}, {os.cpu_count()} logical CPUs visible"
+
+
+def cpu_baseline_threads(pos, L, x, tr, budget_s, threads):
+    """oracle rows spread over host threads (row ranges are independent;
+    the oracle call releases the GIL)"""
+    from concurrent.futures import ThreadPoolExecutor
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as po
+    o = po.System(2, pos, box=(L, L, L), tr=tr)
+    n = len(pos)
+    per = 1
+    with ThreadPoolExecutor(threads) as ex:
+        while True:
+            rows = min(n, per * threads)
+            bounds = [(i * rows // threads, (i + 1) * rows // threads)
+                      for i in range(threads)]
+            t0 = time.perf_counter()
+            list(ex.map(lambda b: o.atimes(x, rows=b), bounds))
+            t = time.perf_counter() - t0
+            if t * 4 > budget_s or rows * 4 > n:
+                break
+            per *= 4
+    return dict(value=(rows / n) / t, unit="matvecs/s", cores=threads,
+                kind="port",
+                sample=f"oracle FP64 rows 0..{rows - 1} of the N={n} FTS "
+                       f"matvec (tr={tr:g}) over {threads} threads, "
+                       f"{t:.1f} s; rate = rows/N/time", host=host_cpu())
 
 
 def bench_bd(args, log):
@@ -144,7 +185,7 @@ def bench_bd(args, log):
         # use the dense real-space matrix at x, so the balance differs from
         # a lone product); the result stays that of tr = 1
         best = None
-        for tr in (100.0, 300.0, 1000.0, 3000.0, 1e4):
+        for tr in (30.0, 100.0, 300.0, 1000.0, 3000.0):
             s = ls.Stokes(0, pos, box=(L, L, L), tr=1.0, precision=prec,
                           rmin=1.0)
             s.set_split(tr)
@@ -221,9 +262,76 @@ def bench_bd(args, log):
             s.cheb_dev(acoef, yb, zb, e0, e1, nrhs=nb)
         s.sync()
         tc = (time.perf_counter() - t0) / reps
+        # the same block with the real-space part as a dense matrix built at
+        # this configuration (the build is inside the timed region): each of
+        # the 30 products is one GEMM plus the matrix-free reciprocal part
+        # the GEMM cost does not depend on the splitting, the reciprocal
+        # part does: the splitting is chosen for this variant on its own
+        def cached_block():
+            s.set_real_cache(True)
+            s.cheb_dev(acoef, yb, zb, e0, e1, nrhs=nb)
+            s.set_real_cache(False)
+        tcc, trc = None, None
+        for trx in (10.0, 30.0, 100.0, 300.0, 1000.0):
+            s.set_split(trx)
+            cached_block()
+            s.sync()
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                cached_block()
+            s.sync()
+            tx = (time.perf_counter() - t0) / reps
+            log(f"cheb block 64 {name} cached tr={trx:g}: {tx * 1e3:.3f} ms")
+            if tcc is None or tx < tcc:
+                tcc, trc = tx, trx
+        s.set_split(trc)
+        cached_block()
+        s.sync()
+        zc = zb.to_host()
+        s.set_split(tr_bd)
+        s.cheb_dev(acoef, yb, zb, e0, e1, nrhs=nb)
+        s.sync()
+        out[name]["cheb_block64_cached_vs_matrix_free"] = float(
+            np.linalg.norm(zc - zb.to_host()) / np.linalg.norm(zb.to_host()))
+        out[name]["cheb_block64_matrix_free_ms"] = tc * 1e3
+        out[name]["cheb_block64_cached_ms"] = tcc * 1e3
+        out[name]["cheb_block64_cached_tr"] = trc
+        tc = min(tc, tcc)
         out[name]["cheb_block64_ms"] = tc * 1e3
         out[name]["noise_vectors_per_s_block64"] = nb / tc
-        log(f"cheb block 64 {name}: {tc * 1e3:.3f} ms")
+        # accuracy of column 0: |z.Mz - y.y| / z.Mz (chebyshev.c:241-269)
+        y0 = yb.to_host()[:3 * n]
+        z0 = zb.to_host()[:3 * n]
+        mz = s.atimes(z0)
+        out[name]["cheb_block64_err_col0"] = abs(z0 @ mz - y0 @ y0) / (z0 @ mz)
+        # the same block for z = M^{1/2} y (the other semantics of the noise:
+        # the drift-free Brownian displacement), same number of products;
+        # accuracy |y.My - z.z| / y.My (chebyshev.c:282-310)
+        from libstokes_amd import cheb_coef_sqrt
+        asq = cheb_coef_sqrt(30, e0, e1)
+        use_cache = tcc <= out[name]["cheb_block64_matrix_free_ms"] * 1e-3
+        s.set_split(trc if use_cache else tr_bd)
+
+        def sqrt_block():
+            if use_cache:
+                s.set_real_cache(True)
+            s.cheb_dev(asq, yb, zb, e0, e1, nrhs=nb)
+            if use_cache:
+                s.set_real_cache(False)
+        sqrt_block()
+        s.sync()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            sqrt_block()
+        s.sync()
+        ts = (time.perf_counter() - t0) / reps
+        s.set_split(tr_bd)
+        z0 = zb.to_host()[:3 * n]
+        my = s.atimes(y0)
+        out[name]["cheb_block64_sqrt_ms"] = ts * 1e3
+        out[name]["cheb_block64_sqrt_err_col0"] = abs(y0 @ my - z0 @ z0) / (y0 @ my)
+        log(f"cheb block 64 {name}: {tc * 1e3:.3f} ms (1/sqrt), "
+            f"{ts * 1e3:.3f} ms (sqrt)")
         s.close()
     return out
 
@@ -475,6 +583,9 @@ def main():
         out["ensemble"] = bench_ensemble(args, log)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(pos, L, x, 1.0, args.cpu_budget)
+        thr = max(1, min(16, os.cpu_count() or 1))
+        out["cpu_baseline_threads"] = cpu_baseline_threads(
+            pos, L, x, 1.0, args.cpu_budget, thr)
     print(json.dumps(out))
     s64_ref.close()
 

@@ -59,7 +59,8 @@ int stokes_hip_set_split (struct stokes *sys, double tratio);
 
 /* enable: build the real-space part of M at the current positions as a
  * dense matrix ((D np)^2 values in the current precision) and use it for
- * the real-space sum of every single-rhs product until disabled or until
+ * the real-space sum of every product (one GEMV per column; several
+ * columns per call: one GEMM, rocBLAS without atomics) until disabled or until
  * the system's parameters change (splitting, precision, box).  The caller
  * keeps the positions fixed meanwhile (products re-read them but the
  * matrix is not rebuilt).  Pays off where many products share one

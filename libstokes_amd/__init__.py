@@ -400,7 +400,7 @@ class Stokes:
 
     def set_real_cache(self, on=True):
         """dense real-space matrix of the current positions for the
-        following single-rhs products (keep the positions fixed)"""
+        following products (keep the positions fixed)"""
         check(self.L.stokes_hip_set_real_cache(self.sys, int(bool(on))),
               "set_real_cache")
 
@@ -601,8 +601,18 @@ def cheb_coef_inv_sqrt(n_cheb, e0, e1):
     return a
 
 
+def cheb_coef_sqrt(n_cheb, e0, e1):
+    """Chebyshev coefficients of sqrt(x) on [e0, e1]"""
+    L = lib()
+    a = np.zeros(n_cheb)
+    L.chebyshev_coef(int(n_cheb), _SQRT_CB, C.c_double(e0), C.c_double(e1),
+                     _p(a))
+    return a
+
+
 _FUNC_CB = C.CFUNCTYPE(C.c_double, C.c_double)
 _INV_SQRT_CB = _FUNC_CB(lambda x: 1.0 / np.sqrt(x))
+_SQRT_CB = _FUNC_CB(lambda x: float(np.sqrt(x)))
 
 
 def exported_symbols():

@@ -2,6 +2,7 @@
 // internal C ABI of shk.h.  One context per struct stokes, one HIP stream
 // per context (default priority), deterministic reduction order.
 #include <hip/hip_runtime.h>
+#include <rocblas/rocblas.h>
 #include <cstdio>
 #include <cstring>
 #include <cstdarg>
@@ -62,7 +63,8 @@ struct shk_dev {
   DevBuf cand, candf, candm, tbits, kpart, kpq, kback;
   DevBuf adjfg, adjcs, adjpart;
   DevBuf km, ptab;                // k lattice indices, per-particle phase tables
-  DevBuf rcache, rz;              // dense real-space matrix, its vector
+  DevBuf rcache, rz;              // dense real-space matrix, its vector(s)
+  rocblas_handle blas = nullptr;  // block products with the cached matrix
   int rc_use = 0, rc_prec = 0;    // products use rcache (built in rc_prec)
   PhaseBasis basis{};
   int ncand = 0, ncore = 0, tbits_words = 0;
@@ -128,8 +130,9 @@ extern "C" void shk_dev_destroy (shk_dev *d)
   for (DevBuf *b : {&d->pos, &d->img, &d->imgf, &d->kv, &d->ks, &d->partial,
                     &d->pq, &d->xs, &d->ys, &d->zt, &d->cand, &d->candf, &d->candm,
                     &d->tbits, &d->kpart, &d->kpq, &d->kback, &d->adjfg, &d->adjcs,
-                    &d->adjpart, &d->km, &d->ptab})
+                    &d->adjpart, &d->km, &d->ptab, &d->rcache, &d->rz})
     release (*b);
+  if (d->blas) (void) rocblas_destroy_handle (d->blas);
   for (auto &e : d->ev)
     if (e) (void) hipEventDestroy (e);
   if (d->stream) (void) hipStreamDestroy (d->stream);
@@ -498,9 +501,14 @@ static int atimes_vt (shk_dev *d, const double *x, double *y)
       if ((rc = ensure (d, d->rz, sizeof (T) * ND))) return rc;
       hipLaunchKernelGGL ((k_z_prep<V, T>), dim3 ((unsigned) ((p.np + 255) / 256)), dim3 (256), 0,
                           d->stream, x, p.np, (T *) d->rz.p);
-      hipLaunchKernelGGL ((k_real_gemv<T>), dim3 ((unsigned) ((ND + 3) / 4)), dim3 (256), 0,
-                          d->stream, (const T *) d->rcache.p, (const T *) d->rz.p, (int) ND,
-                          (double *) d->partial.p);
+      if (ND % Vec16<T>::W == 0)
+        hipLaunchKernelGGL ((k_real_gemv16<T>), dim3 ((unsigned) ((ND + 3) / 4)), dim3 (256), 0,
+                            d->stream, (const T *) d->rcache.p, (const T *) d->rz.p, (int) ND,
+                            (double *) d->partial.p);
+      else
+        hipLaunchKernelGGL ((k_real_gemv<T>), dim3 ((unsigned) ((ND + 3) / 4)), dim3 (256), 0,
+                            d->stream, (const T *) d->rcache.p, (const T *) d->rz.p, (int) ND,
+                            (double *) d->partial.p);
       HIPCHK (hipGetLastError ());
       if (d->timing) HIPCHK (hipEventRecord (d->ev[1], d->stream));
       if (p.nk > 0)
@@ -768,7 +776,85 @@ static int atimes_block_vt (shk_dev *d, const double *x, double *y, int B)
   const void *cand = std::is_same<T, double>::value ? d->cand.p : d->candf.p;
   // timing: ev[0] -> ev[1] brackets the real-space kernel alone
   if (d->timing) HIPCHK (hipEventRecord (d->ev[0], d->stream));
-  if (MF)
+  const int want_prec = MF ? SHK_MIXED : SHK_FP64;
+  if (d->rc_use && d->rc_prec == want_prec)
+    {
+      // real space from the cached dense matrix: Y (ND x B, row-major) =
+      // Mr (ND x ND, row-major) Z (ND x B, row-major).  MIXED: split-K
+      // matrix-core GEMM (k_cache_gemm), one partial per K slice; FP64:
+      // column-major Y^T = Z^T Mr^T, one rocBLAS GEMM without atomics
+      const int ND = p.np * D;
+      const T *Z = XT;
+      if constexpr (MF)
+        {
+          if ((rc = ensure (d, d->rz, sizeof (T) * (size_t) ND * B))) return rc;
+          hipLaunchKernelGGL ((k_block_prep<V, T>), dim3 ((unsigned) ((nb + 255) / 256)),
+                              dim3 (256), 0, d->stream, x, p.np, B, (T *) d->rz.p);
+          HIPCHK (hipGetLastError ());
+          Z = (const T *) d->rz.p;
+          // K slices of a multiple of 64 so that about 768 workgroups run
+          const int ct = (B + 15) / 16;
+          const int rows_wg = 64 * gemm_rt (ct > 4 ? 8 : 4);
+          const int nrg = (ND + rows_wg - 1) / rows_wg;
+          int ns = (768 + nrg / 2) / nrg;
+          if (ns > (ND + 255) / 256) ns = (ND + 255) / 256;
+          if (ns < 1) ns = 1;
+          int ks = (ND + ns - 1) / ns;
+          ks = (ks + 63) & ~63;
+          ns = (ND + ks - 1) / ks;
+          if ((rc = ensure (d, d->partial, sizeof (T) * (size_t) ns * ND * B))) return rc;
+          const dim3 g ((unsigned) nrg, (unsigned) ns);
+          const bool vec = ND % 4 == 0;
+          const float *mr = (const float *) d->rcache.p;
+          float *pf = (float *) d->partial.p;
+#define CACHE_GEMM(CT)                                                                  \
+  do {                                                                                \
+      if (vec)                                                                        \
+        hipLaunchKernelGGL ((k_cache_gemm<CT, true>), g, dim3 (256), 0, d->stream, mr, \
+                            (const float *) Z, ND, B, ks, pf);                        \
+      else                                                                            \
+        hipLaunchKernelGGL ((k_cache_gemm<CT, false>), g, dim3 (256), 0, d->stream,   \
+                            mr, (const float *) Z, ND, B, ks, pf);                    \
+  } while (0)
+          if (ct <= 1) CACHE_GEMM (1);
+          else if (ct <= 2) CACHE_GEMM (2);
+          else if (ct <= 4) CACHE_GEMM (4);
+          else CACHE_GEMM (8);
+#undef CACHE_GEMM
+          HIPCHK (hipGetLastError ());
+          nj = ns;
+        }
+      else
+        {
+      if (!d->blas)
+        {
+          if (rocblas_create_handle (&d->blas) != rocblas_status_success
+              || rocblas_set_atomics_mode (d->blas, rocblas_atomics_not_allowed)
+                   != rocblas_status_success)
+            {
+              set_err ("rocblas handle: creation failed");
+              return SHK_EHIP;
+            }
+        }
+      if (rocblas_set_stream (d->blas, d->stream) != rocblas_status_success)
+        {
+          set_err ("rocblas_set_stream failed");
+          return SHK_EHIP;
+        }
+      const double one = 1, zero = 0;
+      rocblas_status st =
+        rocblas_dgemm (d->blas, rocblas_operation_none, rocblas_operation_none, B, ND, ND,
+                       &one, (const double *) Z, B, (const double *) d->rcache.p, ND, &zero,
+                       (double *) d->partial.p, B);
+      if (st != rocblas_status_success)
+        {
+          set_err ("rocblas gemm (cached real space): status %d", (int) st);
+          return SHK_EHIP;
+        }
+      nj = 1;
+        }
+    }
+  else if (MF)
     {
       const dim3 g (nib, nj, ncg);
       const float4 *cf = (const float4 *) d->candf.p;
@@ -859,12 +945,12 @@ static int real_cache_build (shk_dev *d)
 
 extern "C" int shk_dev_real_cache (shk_dev *d, int on, int prec)
 {
-  if (!d || !d->have_sys) { set_err ("shk_dev_real_cache: no system"); return SHK_EARG; }
-  if (!on)
+  if (d && !on)
     {
       d->rc_use = 0;
       return SHK_OK;
     }
+  if (!d || !d->have_sys) { set_err ("shk_dev_real_cache: no system"); return SHK_EARG; }
   const shk_params &p = d->prm;
   if (!p.periodic) { set_err ("shk_dev_real_cache: periodic systems only"); return SHK_EARG; }
   int rc;

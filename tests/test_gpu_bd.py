@@ -280,6 +280,41 @@ def test_real_cache_products_equal_matrix_free(version):
     s.close()
 
 
+@pytest.mark.parametrize("version,n,nrhs", [(0, 97, 5), (2, 96, 7), (1, 97, 32),
+                                             (2, 97, 1)])
+def test_real_cache_block_products(version, n, nrhs):
+    """products of several columns (and one column with a row length off the
+    16-byte width) with the dense real-space matrix equal the matrix-free
+    products (FP64 1e-13, MIXED 1e-6 against FP64)"""
+    pos, L = workloads.configuration(n, 0.15, seed=14)
+    d = ls.dim(version) * n
+    x = np.random.default_rng(15).standard_normal(d * nrhs)
+    s = ls.Stokes(version, pos, box=(L, L, L), tr=1.0)
+    s.set_split(30.0)
+    xd = ls.DeviceArray.from_host(x)
+    yd = ls.DeviceArray(d * nrhs)
+
+    def run(prec, cache):
+        s.set_precision(prec)
+        s.set_real_cache(cache)
+        s.atimes_dev(xd, yd, nrhs=nrhs)
+        s.sync()
+        s.set_real_cache(False)
+        return yd.to_host().reshape(nrhs, d)
+    y0 = run(ls.FP64, False)
+    y1 = run(ls.FP64, True)
+    y2 = run(ls.MIXED, True)
+    y3 = run(ls.MIXED, True)
+    e64 = max(relerr(y1[b], y0[b]) for b in range(nrhs))
+    em = max(relerr(y2[b], y0[b]) for b in range(nrhs))
+    print(f"  version {version} N={n} nrhs={nrhs}: cached FP64 {e64:.2e}, "
+          f"cached MIXED {em:.2e}", flush=True)
+    assert e64 < 1e-13
+    assert em < 1e-6
+    assert np.array_equal(y2, y3)
+    s.close()
+
+
 def test_bd_steps_with_and_without_real_cache():
     """the BD step with the dense real-space matrix for its products at x
     follows the matrix-free step (FP64: 1e-12 of the displacement)"""
