@@ -129,10 +129,13 @@ template <> struct Vec16<double> {
 
 // y_real = Mr z for ND a multiple of the 16-byte width W: one wave per
 // row, each lane reads 16-byte vectors (four in flight per lane), FP64
-// accumulation, fixed lane order
-template <typename T>
+// accumulation, fixed lane order.  XIN: z is the caller's FP64 vector x
+// itself (F and FT, where the S transform is the identity), rounded to T
+// as k_z_prep would
+template <typename T, bool XIN>
 __global__ __launch_bounds__ (256) void
-k_real_gemv16 (const T *__restrict__ Mr, const T *__restrict__ z, int ND, double *__restrict__ out)
+k_real_gemv16 (const T *__restrict__ Mr, const void *__restrict__ zv, int ND,
+               double *__restrict__ out)
 {
   using V16 = typename Vec16<T>::type;
   constexpr int W = Vec16<T>::W;
@@ -140,7 +143,18 @@ k_real_gemv16 (const T *__restrict__ Mr, const T *__restrict__ z, int ND, double
   const int row = blockIdx.x * 4 + wave;
   if (row >= ND) return;
   const V16 *m = (const V16 *) (Mr + (size_t) row * ND);
-  const V16 *zz = (const V16 *) z;
+  auto zat = [&] (int c) -> V16 {
+    if constexpr (XIN)
+      {
+        const double *xd = (const double *) zv + (size_t) c * W;
+        V16 v;
+#pragma unroll
+        for (int t = 0; t < W; t++) v[t] = T (xd[t]);
+        return v;
+      }
+    else
+      return ((const V16 *) zv)[c];
+  };
   const int nv = ND / W;
   double acc = 0.0;
   int c = lane;
@@ -150,10 +164,10 @@ k_real_gemv16 (const T *__restrict__ Mr, const T *__restrict__ z, int ND, double
       const V16 m1 = __builtin_nontemporal_load (m + c + 64);
       const V16 m2 = __builtin_nontemporal_load (m + c + 128);
       const V16 m3 = __builtin_nontemporal_load (m + c + 192);
-      acc += Vec16<T>::dot (m0, zz[c]) + Vec16<T>::dot (m1, zz[c + 64])
-        + Vec16<T>::dot (m2, zz[c + 128]) + Vec16<T>::dot (m3, zz[c + 192]);
+      acc += Vec16<T>::dot (m0, zat (c)) + Vec16<T>::dot (m1, zat (c + 64))
+        + Vec16<T>::dot (m2, zat (c + 128)) + Vec16<T>::dot (m3, zat (c + 192));
     }
-  for (; c < nv; c += 64) acc += Vec16<T>::dot (m[c], zz[c]);
+  for (; c < nv; c += 64) acc += Vec16<T>::dot (m[c], zat (c));
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor (acc, off, 64);
   if (lane == 0) out[row] = acc;

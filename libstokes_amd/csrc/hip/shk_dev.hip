@@ -63,6 +63,9 @@ struct shk_dev {
   DevBuf cand, candf, candm, tbits, kpart, kpq, kback;
   DevBuf adjfg, adjcs, adjpart;
   DevBuf km, ptab;                // k lattice indices, per-particle phase tables
+  unsigned long pos_gen = 0;      // positions uploads so far
+  unsigned long ptab_gen = 0;     // positions the single-column table was built
+  int ptab_kind = 0;              // for (1 FP32, 2 FP64; 0 none)
   DevBuf rcache, rz;              // dense real-space matrix, its vector(s)
   rocblas_handle blas = nullptr;  // block products with the cached matrix
   int rc_use = 0, rc_prec = 0;    // products use rcache (built in rc_prec)
@@ -256,6 +259,7 @@ static int upload_pos (shk_dev *d, const double *pos, const double *a, int on_de
   const int np = d->prm.np;
   int rc = ensure (d, d->pos, sizeof (double4) * np);
   if (rc) return rc;
+  d->pos_gen++;
   // staging for AoS positions and radii
   rc = ensure (d, d->zt, sizeof (double) * 4 * (size_t) np);
   if (rc) return rc;
@@ -303,6 +307,7 @@ extern "C" int shk_dev_set_system (shk_dev *d, const shk_params *p, const double
     }
   d->prm = *p;
   d->rc_use = 0;                  // a cached real-space matrix is stale now
+  d->ptab_kind = 0;               // so are the phase tables (k basis)
   int rc = upload_pos (d, pos, a, 0, 0);
   if (rc) return rc;
   if (p->periodic)
@@ -455,13 +460,23 @@ static int recip_launch (shk_dev *d, const double *x, int &nrecip, const void *&
   if ((rc = ensure (d, d->kpq, sizeof (T) * (size_t) nk * NPOW * 2 * D))) return rc;
   if ((rc = ensure (d, d->kback, sizeof (T) * (size_t) sk * p.np * D))) return rc;
   const PhaseBasis &b = d->basis;
+  const void *ptab_before = d->ptab.p;
   if ((rc = ensure (d, d->ptab, sizeof (Cplx<T>) * (size_t) p.np * b.W))) return rc;
   const double4 *pos4 = (const double4 *) d->pos.p;
   const double4 *kv = (const double4 *) d->kv.p;
   const int4 *km = (const int4 *) d->km.p;
-  hipLaunchKernelGGL ((k_phase_tab<T>), dim3 ((unsigned) (((size_t) p.np * b.W + 255) / 256)),
-                      dim3 (256), 0, d->stream, pos4, p.np, b, b.W, (Cplx<T> *) d->ptab.p);
-  HIPCHK (hipGetLastError ());
+  // the tables depend on the positions only: kept across products at one
+  // configuration (stream order keeps a captured graph consistent, since
+  // every positions upload is recorded before the products that follow it)
+  const int kind = std::is_same<T, float>::value ? 1 : 2;
+  if (d->ptab.p != ptab_before || d->ptab_kind != kind || d->ptab_gen != d->pos_gen)
+    {
+      hipLaunchKernelGGL ((k_phase_tab<T>), dim3 ((unsigned) (((size_t) p.np * b.W + 255) / 256)),
+                          dim3 (256), 0, d->stream, pos4, p.np, b, b.W, (Cplx<T> *) d->ptab.p);
+      HIPCHK (hipGetLastError ());
+      d->ptab_kind = kind;
+      d->ptab_gen = d->pos_gen;
+    }
   const int jt = sf_tile<V, T> (b);
   const size_t lds = sf_lds<V, T, NPOW> (b, jt);
   hipLaunchKernelGGL ((k_recip_sf_tab<V, T, NPOW>), dim3 ((nk + 63) / 64, sj), dim3 (256), lds,
@@ -499,16 +514,26 @@ static int atimes_vt (shk_dev *d, const double *x, double *y)
       nreal = 1;
       if ((rc = ensure (d, d->partial, sizeof (double) * ND))) return rc;
       if ((rc = ensure (d, d->rz, sizeof (T) * ND))) return rc;
-      hipLaunchKernelGGL ((k_z_prep<V, T>), dim3 ((unsigned) ((p.np + 255) / 256)), dim3 (256), 0,
-                          d->stream, x, p.np, (T *) d->rz.p);
-      if (ND % Vec16<T>::W == 0)
-        hipLaunchKernelGGL ((k_real_gemv16<T>), dim3 ((unsigned) ((ND + 3) / 4)), dim3 (256), 0,
-                            d->stream, (const T *) d->rcache.p, (const T *) d->rz.p, (int) ND,
+      const dim3 grows ((unsigned) ((ND + 3) / 4));
+      const bool w16 = ND % Vec16<T>::W == 0;
+      if (V < 2 && w16)
+        // F, FT: no S transform, the kernel reads x itself
+        hipLaunchKernelGGL ((k_real_gemv16<T, true>), grows, dim3 (256), 0, d->stream,
+                            (const T *) d->rcache.p, (const void *) x, (int) ND,
                             (double *) d->partial.p);
       else
-        hipLaunchKernelGGL ((k_real_gemv<T>), dim3 ((unsigned) ((ND + 3) / 4)), dim3 (256), 0,
-                            d->stream, (const T *) d->rcache.p, (const T *) d->rz.p, (int) ND,
-                            (double *) d->partial.p);
+        {
+          hipLaunchKernelGGL ((k_z_prep<V, T>), dim3 ((unsigned) ((p.np + 255) / 256)),
+                              dim3 (256), 0, d->stream, x, p.np, (T *) d->rz.p);
+          if (w16)
+            hipLaunchKernelGGL ((k_real_gemv16<T, false>), grows, dim3 (256), 0, d->stream,
+                                (const T *) d->rcache.p, (const void *) d->rz.p, (int) ND,
+                                (double *) d->partial.p);
+          else
+            hipLaunchKernelGGL ((k_real_gemv<T>), grows, dim3 (256), 0, d->stream,
+                                (const T *) d->rcache.p, (const T *) d->rz.p, (int) ND,
+                                (double *) d->partial.p);
+        }
       HIPCHK (hipGetLastError ());
       if (d->timing) HIPCHK (hipEventRecord (d->ev[1], d->stream));
       if (p.nk > 0)
@@ -671,6 +696,7 @@ static int recip_block_launch_mfma (shk_dev *d, const float *Zt, int zs, int npa
   if ((rc = ensure (d, d->kpq, sizeof (float) * (size_t) nkpad * PR * Cp))) return rc;
   if ((rc = ensure (d, d->kback, sizeof (float) * (size_t) sk * np * C))) return rc;
   if ((rc = ensure (d, d->ptab, sizeof (Cplx<float>) * (size_t) npad * (b.W + LD)))) return rc;
+  d->ptab_kind = 0;               // the single-column tables are overwritten
   const double4 *pos4 = (const double4 *) d->pos.p;
   const double4 *kv = (const double4 *) d->kv.p;
   const int4 *km = (const int4 *) d->km.p;
