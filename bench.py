@@ -419,6 +419,46 @@ def profile_run(args):
         s.close()
 
 
+def profile_bd_run(args):
+    """short run for rocprofv3: --steps BD steps (F, N=2048, phi=0.2, order
+    30) per precision at the splitting --tr (default 1000), then one
+    64-column Chebyshev block per precision with the cached real-space
+    operator at tr = 10"""
+    n = 2048
+    pos, L = workloads.configuration(n, 0.2, seed=2)
+    tr = args.tr if args.tr else 1000.0
+    for name, prec in (("mixed", ls.MIXED), ("fp64", ls.FP64)):
+        s = ls.Stokes(0, pos, box=(L, L, L), tr=1.0, precision=prec, rmin=1.0)
+        s.set_split(tr)
+        xd = ls.DeviceArray.from_host(pos.reshape(-1))
+        bd = ls.BD(s, peclet=1.0, n_cheb=30, eps=5e-2, dt_lim=1e-6,
+                   rmin_ol=1.0, seed=11)
+        for _ in range(2):
+            bd.step(xd, 1e-3)
+        s.sync()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            info = bd.step(xd, 1e-3)
+        s.sync()
+        t = (time.perf_counter() - t0) / args.steps
+        e0, e1 = info.eig
+        bd.close()
+        from libstokes_amd import cheb_coef_inv_sqrt
+        a = cheb_coef_inv_sqrt(30, e0, e1)
+        yb = ls.DeviceArray.from_host(
+            np.random.default_rng(4).standard_normal(64 * 3 * n))
+        zb = ls.DeviceArray(64 * 3 * n)
+        s.set_split(10.0)
+        s.set_real_cache(True)
+        s.cheb_dev(a, yb, zb, e0, e1, nrhs=64)
+        s.set_real_cache(False)
+        s.sync()
+        print(json.dumps({"precision": name, "bd_tr": tr,
+                          "bd_ms_per_step": t * 1e3, "bd_steps": args.steps,
+                          "block_tr": 10.0}), flush=True)
+        s.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -441,9 +481,15 @@ def main():
     ap.add_argument("--profile", action="store_true",
                     help="short run for rocprofv3: a 256 MiB calibration "
                          "copy, then --steps mixed and FP64 matvecs at --tr")
+    ap.add_argument("--profile-bd", action="store_true",
+                    help="short run for rocprofv3: --steps BD steps of each "
+                         "precision at --tr, one cached 64-column Chebyshev "
+                         "block of each")
     args = ap.parse_args()
     if args.profile:
         return profile_run(args)
+    if args.profile_bd:
+        return profile_bd_run(args)
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; "
                           "--gpus must be 1"}))
