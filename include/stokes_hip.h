@@ -188,6 +188,12 @@ int stokes_hip_ensemble_create (struct stokes *const *sys, int nbox,
 int stokes_hip_ensemble_atimes (stokes_hip_ensemble *e, const double *pos_dev,
                                 const double *x_dev, double *y_dev);
 int stokes_hip_ensemble_refresh (stokes_hip_ensemble *e);
+/* the autotuner per box: stokes_hip_autotune of each member system (one
+ * column, candidates trs[ntr] or the default list when NULL, reps timed
+ * calls each), best_tr[nbox] (may be NULL) receives the splittings kept;
+ * the next launch recaptures */
+int stokes_hip_ensemble_autotune (stokes_hip_ensemble *e, const double *trs,
+                                  int ntr, int reps, double *best_tr);
 size_t stokes_hip_ensemble_offset (const stokes_hip_ensemble *e, int box);
 void stokes_hip_ensemble_destroy (stokes_hip_ensemble *e);
 

@@ -69,7 +69,7 @@ EXPORTS = [
     "stokes_hip_bd_set_real_cache", "stokes_hip_set_real_cache",
     "stokes_hip_bd_last_noise", "stokes_hip_bd_destroy",
     "stokes_hip_bd_step_q", "stokes_hip_bd_set_torque",
-    "stokes_hip_bd_set_fts_iter",
+    "stokes_hip_bd_set_fts_iter", "stokes_hip_ensemble_autotune",
 ]
 
 
@@ -211,6 +211,8 @@ def lib():
                                              C.POINTER(_vp)]
     L.stokes_hip_ensemble_atimes.argtypes = [_vp, _vp, _vp, _vp]
     L.stokes_hip_ensemble_refresh.argtypes = [_vp]
+    L.stokes_hip_ensemble_autotune.argtypes = [_vp, _dp, C.c_int, C.c_int,
+                                               _dp]
     L.stokes_hip_ensemble_offset.argtypes = [_vp, C.c_int]
     L.stokes_hip_ensemble_offset.restype = C.c_size_t
     L.stokes_hip_ensemble_destroy.argtypes = [_vp]
@@ -504,6 +506,20 @@ class Ensemble:
 
     def refresh(self):
         check(self.L.stokes_hip_ensemble_refresh(self.h), "ensemble_refresh")
+
+    def autotune(self, trs=None, reps=3):
+        """per-box splitting autotuner; returns the splittings kept"""
+        nb = len(self.systems)
+        best = np.zeros(nb)
+        if trs is None:
+            tp, nt = None, 0
+        else:
+            arr = np.ascontiguousarray(trs, dtype=np.float64)
+            tp, nt = _p(arr), arr.size
+        check(self.L.stokes_hip_ensemble_autotune(self.h, tp, nt, reps,
+                                                  _p(best)),
+              "ensemble_autotune")
+        return best
 
     def close(self):
         if self.h:

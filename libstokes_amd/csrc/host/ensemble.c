@@ -112,6 +112,25 @@ stokes_hip_ensemble_refresh (stokes_hip_ensemble *e)
   return 0;
 }
 
+int
+stokes_hip_ensemble_autotune (stokes_hip_ensemble *e, const double *trs,
+                              int ntr, int reps, double *best_tr)
+{
+  if (!e)
+    {
+      stokes_hip_set_error ("ensemble: NULL handle");
+      return STOKES_HIP_EARG;
+    }
+  for (int b = 0; b < e->nbox; b++)
+    {
+      double bt = 0.0;
+      int rc = stokes_hip_autotune (e->sys[b], 1, trs, ntr, reps, &bt, NULL);
+      if (rc) return rc;
+      if (best_tr) best_tr[b] = bt;
+    }
+  return stokes_hip_ensemble_refresh (e);
+}
+
 static int
 ens_capture (stokes_hip_ensemble *e, const double *pos, const double *x,
              double *y)

@@ -68,3 +68,29 @@ def test_ensemble_matches_oracle_per_box(precision, tol):
     ens.close()
     for s in systems:
         s.close()
+
+
+def test_ensemble_autotune_keeps_results():
+    """the per-box splitting autotuner (stokes_hip_ensemble_autotune)
+    changes only the splitting: every box still equals the oracle at the
+    reference's tr = 1 (FP64 1e-12)"""
+    boxes, systems = _boxes(ls.FP64)
+    ens = ls.Ensemble(systems)
+    off = ens.offsets
+    pos_all = np.concatenate([b[0].reshape(-1) for b in boxes])
+    x_all = np.random.default_rng(4).standard_normal(11 * off[-1])
+    posd = ls.DeviceArray.from_host(pos_all)
+    xd = ls.DeviceArray.from_host(x_all)
+    yd = ls.DeviceArray(11 * off[-1])
+    trs = ens.autotune(trs=[10.0, 100.0, 1000.0], reps=2)
+    _say(f"  tuned splittings {trs}")
+    assert set(trs) <= {10.0, 100.0, 1000.0}
+    ens.atimes_dev(posd, xd, yd)
+    y = yd.to_host()
+    for b, (pos, a, L) in enumerate(boxes):
+        o = po.System(2, pos, box=(L, L, L), tr=1.0, a=a)
+        sl = slice(11 * off[b], 11 * off[b + 1])
+        assert relerr(y[sl], o.atimes(x_all[sl])) < 1e-12
+    ens.close()
+    for s in systems:
+        s.close()
