@@ -271,6 +271,13 @@ int stokes_hip_bd_step (stokes_hip_bd *bd, double *x_dev, double dt,
  * tracks none */
 int stokes_hip_bd_step_q (stokes_hip_bd *bd, double *x_dev, double *q_dev,
                           double dt, stokes_hip_bd_info *info);
+/* the time-stepping scheme of BD->scheme (brownian.h:77-85, the dispatch
+ * of brownian.c:2108-2125): 0 the mid-point algorithm (BD_evolve_mid,
+ * the default), 1 Banchio-Brady 2003 with step parameter bb_n = BD->BB_n
+ * (BD_evolve_BB03, brownian.c:2579-2920; its deterministic velocity is
+ * evaluated at the intermediate configuration, as the reference does,
+ * 2770-2816), 2 Ball-Melrose 1997 (BD_evolve_BM97, 2970-3190) */
+int stokes_hip_bd_set_scheme (stokes_hip_bd *bd, int scheme, double bb_n);
 /* FT / FTS: constant torques [3 np] (BD->T, brownian.c:2296-2299) or NULL,
  * read on every step */
 int stokes_hip_bd_set_torque (stokes_hip_bd *bd, const double *T_dev);

@@ -70,6 +70,7 @@ EXPORTS = [
     "stokes_hip_bd_last_noise", "stokes_hip_bd_destroy",
     "stokes_hip_bd_step_q", "stokes_hip_bd_set_torque",
     "stokes_hip_bd_set_fts_iter", "stokes_hip_ensemble_autotune",
+    "stokes_hip_bd_set_scheme",
 ]
 
 
@@ -228,6 +229,7 @@ def lib():
     L.stokes_hip_bd_step_q.argtypes = [_vp, _vp, _vp, C.c_double,
                                        C.POINTER(BDInfo)]
     L.stokes_hip_bd_set_torque.argtypes = [_vp, _vp]
+    L.stokes_hip_bd_set_scheme.argtypes = [_vp, C.c_int, C.c_double]
     L.stokes_hip_bd_set_fts_iter.argtypes = [_vp, C.c_int, C.c_int,
                                              C.c_double]
     L.stokes_hip_bd_last_noise.argtypes = [_vp, _vp, _vp]
@@ -568,6 +570,12 @@ class BD:
         check(self.L.stokes_hip_bd_set_torque(
             self.h, torque_dev.ptr if torque_dev is not None else None),
             "bd_set_torque")
+
+    def set_scheme(self, scheme, bb_n=1.0):
+        """0 mid-point, 1 Banchio-Brady 2003 (step parameter bb_n), 2
+        Ball-Melrose 1997"""
+        check(self.L.stokes_hip_bd_set_scheme(self.h, int(scheme),
+                                              float(bb_n)), "bd_set_scheme")
 
     def set_fts_iter(self, restart=20, max_iter=2000, eps=1e-6):
         """FTS: GMRES parameters of the noise products"""

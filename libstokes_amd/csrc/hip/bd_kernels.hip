@@ -87,14 +87,14 @@ __global__ void k_status_clear (shk_bd_status *st)
     }
 }
 
-// f = F + fact z (F may be NULL)
+// f = F + fact z (F or z may be NULL)
 __global__ void k_bd_force (double *__restrict__ f, const double *__restrict__ F,
                             const double *__restrict__ z, const shk_bd_scalars *__restrict__ sc,
                             size_t n)
 {
   const size_t i = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const double fz = sc->fact * z[i];
+  const double fz = z ? sc->fact * z[i] : 0.0;
   f[i] = F ? F[i] + fz : fz;
 }
 
@@ -121,7 +121,7 @@ __global__ void k_bd_force_ft (double *__restrict__ f, const double *__restrict_
   const size_t i = t / 6;
   const int q = (int) (t % 6);
   const double *ext = q < 3 ? F : T;
-  const double fz = sc->fact * z[t];
+  const double fz = z ? sc->fact * z[t] : 0.0;
   f[t] = ext ? ext[i * 3 + (q % 3)] + fz : fz;
 }
 

@@ -6,8 +6,10 @@
  *
  * One attempt of the step (see bd_record): draw y, z = M^{-1/2} y by
  * Chebyshev, the accuracy dots, f = F + fact z, u = M(x) f, x_mid = x + dt/2
- * u, overlap count, u = M(x_mid) f, x_new = x + dt u (and the quaternions
- * q_new = q + dt dQ/dt(q, O) for FT / FTS), overlap count, advance the draw
+ * u, overlap count, u = M(x_mid) f, x_new = x + dt u and the quaternions
+ * q_new = q + dt dQ/dt(q, O) for FT / FTS (the mid-point scheme;
+ * BD_evolve_BB03, 2579-2920, and BD_evolve_BM97, 2970-3190, change the
+ * intermediate step and the velocity combination), overlap count, advance the draw
  * counter, copy the status to pinned host memory.  M is the mobility of
  * the forces (F) or of the forces and torques (FT, FTS: particle-wise (F,
  * T) -> (U, O)); for FTS every product is the mobility problem with E = 0
@@ -45,6 +47,8 @@ struct stokes_hip_bd {
   int ncheb;
   unsigned long long seed;
   int fact_fix;
+  int scheme;                  /* 0 mid-point, 1 BB03, 2 BM97 */
+  double bb_n;                 /* BB03 step parameter */
   int real_cache;              /* dense real-space matrix at x (32 products) */
   int have_bounds;
   int fits;                    /* Lanczos fits so far */
@@ -52,7 +56,7 @@ struct stokes_hip_bd {
   double *acoef;               /* host [ncheb] */
   size_t n;                    /* operator dimension: 3 np (F), 6 np */
   double *buf;                 /* device: y z w f u work[4n] xm xn qn */
-  double *y, *z, *w, *f, *u, *xm, *xn, *qn, *work;
+  double *y, *z, *w, *f, *u, *xm, *xn, *qn, *work, *u0, *f2;
   /* FTS: GMRES of the noise products (brownian.c:597-600: restart 20,
    * max 2000, eps 1e-6) and work space */
   int fts_m, fts_max;
@@ -173,7 +177,7 @@ stokes_hip_bd_create (struct stokes *sys, const double *F_dev, double peclet,
   bd->sc_fact = bd->sc_dt = -1.0;
   bd->acoef = calloc ((size_t) n_cheb, sizeof (double));
   const size_t n = bd->n, np = (size_t) sys->np;
-  const size_t nbuf = 9 * n + 10 * np;
+  const size_t nbuf = 11 * n + 10 * np;
   int rc = bd->acoef ? 0 : SHK_ENOMEM;
   if (!rc) rc = shk_dev_malloc (c->dev, (void **) &bd->buf, sizeof (double) * nbuf);
   if (!rc && bd->ver == 2)
@@ -196,6 +200,9 @@ stokes_hip_bd_create (struct stokes *sys, const double *F_dev, double peclet,
   bd->xm = bd->work + 4 * n;
   bd->xn = bd->xm + 3 * np;
   bd->qn = bd->xn + 3 * np;
+  bd->u0 = bd->qn + 4 * np;
+  bd->f2 = bd->u0 + n;
+  bd->bb_n = 1.0;
   if (bd->fts_buf)
     {
       bd->fts_b = bd->fts_buf;
@@ -265,6 +272,22 @@ stokes_hip_bd_set_real_cache (stokes_hip_bd *bd, int enable)
       return STOKES_HIP_EARG;
     }
   bd->real_cache = enable != 0;
+  return 0;
+}
+
+int
+stokes_hip_bd_set_scheme (stokes_hip_bd *bd, int scheme, double bb_n)
+{
+  if (!bd || scheme < 0 || scheme > 2 || (scheme == 1 && !(bb_n > 0.0)))
+    {
+      stokes_hip_set_error ("bd_set_scheme: scheme 0 (mid-point), 1 (BB03,"
+                            " bb_n > 0) or 2 (BM97)");
+      return STOKES_HIP_EARG;
+    }
+  bd->scheme = scheme;
+  if (scheme == 1) bd->bb_n = bb_n;
+  shk_dev_graph_destroy (bd->graph);   /* the scheme is baked in */
+  bd->graph = NULL;
   return 0;
 }
 
@@ -380,25 +403,45 @@ bd_record (stokes_hip_bd *bd, shk_dev *d, int prec, int check_ol, double *x,
   if (!rc) rc = bd_cheb (bd, d, prec, bd->y, bd->z);
   if (!rc) rc = bd_op (bd, d, prec, bd->z, bd->w, 1);
   if (!rc) rc = shk_dev_bd_dots (d, bd->y, bd->z, bd->w, n, bd->st_dev);
-  /* step 1: drift at x, half step */
+  /* stage 1: drift at x (BB03: the Brownian force alone), Euler step to
+   * the intermediate configuration: dt / 2 (mid-point), dt / n (BB03), dt
+   * (BM97); the intermediate orientation is not formed, no force here
+   * depends on it */
+  const int sch = bd->scheme;
+  const double h1 = sch == 0 ? 0.5 : (sch == 1 ? 1.0 / bd->bb_n : 1.0);
+  const double *Fx = sch == 1 ? NULL : bd->F, *Tx = sch == 1 ? NULL : bd->T;
   if (!rc)
     rc = bd->ver == 0
-      ? shk_dev_bd_force (d, bd->f, bd->F, bd->z, bd->sc_dev, n)
-      : shk_dev_bd_force_ft (d, bd->f, bd->F, bd->T, bd->z, bd->sc_dev, bd->np);
-  if (!rc) rc = bd_op (bd, d, prec, bd->f, bd->u, 0);
-  /* the half-step orientation is not needed: no force here depends on it */
+      ? shk_dev_bd_force (d, bd->f, Fx, bd->z, bd->sc_dev, n)
+      : shk_dev_bd_force_ft (d, bd->f, Fx, Tx, bd->z, bd->sc_dev, bd->np);
+  if (!rc) rc = bd_op (bd, d, prec, bd->f, bd->u0, 0);
   if (!rc)
     rc = bd->ver == 0
-      ? shk_dev_bd_euler (d, bd->xm, x, bd->u, bd->sc_dev, 0.5, n3)
-      : shk_dev_bd_euler_ft (d, bd->xm, x, NULL, NULL, bd->u, bd->sc_dev, 0.5,
+      ? shk_dev_bd_euler (d, bd->xm, x, bd->u0, bd->sc_dev, h1, n3)
+      : shk_dev_bd_euler_ft (d, bd->xm, x, NULL, NULL, bd->u0, bd->sc_dev, h1,
                              bd->np);
   if (!rc && check_ol)
     rc = shk_dev_bd_overlap (d, bd->xm, NULL, s->np, s->periodic, s->lx, s->ly,
                              s->lz, 0, 0.0, bd->rmin_ol, &bd->st_dev->overlap[0]);
-  /* step 2: drift at x_mid with the same forces, full step from x */
+  /* stage 2: drift at the intermediate configuration with the same forces;
+   * mid-point: u = M(x1) f; BM97: u = (u0 + M(x1) f) / 2; BB03: u =
+   * M(x1) F + u0 + n/2 (M(x1) fact z - u0), its deterministic product at
+   * x1 as in the reference (brownian.c:2770-2816); then the full step from
+   * x */
   if (!rc && bd->real_cache) rc = shk_dev_real_cache (d, 0, prec);
   if (!rc) rc = shk_dev_set_pos (d, bd->xm, 1);
   if (!rc) rc = bd_op (bd, d, prec, bd->f, bd->u, 0);
+  if (!rc && sch == 2) rc = shk_dev_axpby (d, bd->u, bd->u0, 0.5, 0.5, n);
+  if (!rc && sch == 1)
+    {
+      rc = shk_dev_axpby (d, bd->u0, bd->u, 0.5 * bd->bb_n, 1.0 - 0.5 * bd->bb_n, n);
+      if (!rc)
+        rc = bd->ver == 0
+          ? shk_dev_bd_force (d, bd->f2, bd->F, NULL, bd->sc_dev, n)
+          : shk_dev_bd_force_ft (d, bd->f2, bd->F, bd->T, NULL, bd->sc_dev, bd->np);
+      if (!rc) rc = bd_op (bd, d, prec, bd->f2, bd->u, 0);
+      if (!rc) rc = shk_dev_axpby (d, bd->u, bd->u0, 1.0, 1.0, n);
+    }
   if (!rc)
     rc = bd->ver == 0
       ? shk_dev_bd_euler (d, bd->xn, x, bd->u, bd->sc_dev, 1.0, n3)

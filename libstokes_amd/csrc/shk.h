@@ -189,14 +189,14 @@ int shk_dev_gauss (shk_dev *d, unsigned long long seed,
 int shk_dev_counter_add (shk_dev *d, unsigned long long *counter_dev,
                          unsigned long long inc);
 int shk_dev_bd_status_clear (shk_dev *d, shk_bd_status *st);
-/* f = F + fact z (F may be NULL) */
+/* f = F + fact z (F or z may be NULL) */
 int shk_dev_bd_force (shk_dev *d, double *f, const double *F, const double *z,
                       const shk_bd_scalars *sc, size_t n);
 /* xo = xi + frac dt u */
 int shk_dev_bd_euler (shk_dev *d, double *xo, const double *xi, const double *u,
                       const shk_bd_scalars *sc, double frac, size_t n);
 /* FT / FTS, np particles: f[6 i + q] = (F | T)[3 i + q % 3] + fact z[6 i + q]
- * (F, T may be NULL) */
+ * (F, T, z may be NULL) */
 int shk_dev_bd_force_ft (shk_dev *d, double *f, const double *F, const double *T,
                          const double *z, const shk_bd_scalars *sc, int np);
 /* FT / FTS: xo = xi + frac dt U, and qo = qi + frac dt dQ/dt(qi, O) when qi

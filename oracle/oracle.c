@@ -1155,15 +1155,25 @@ orc_bd_op_cb (int n, const double *x, double *y, void *ud)
   else orc_atimes_3all (s, x, y);
 }
 
-/* One mid-point step, every particle mobile, no imposed flow, no
- * interaction forces: brownian.c:2352-2560 with BD_calc_FB's atimes route
- * (1545-1874, spectral bounds and coefficients given, so the refit of
- * 1707-1727 does not happen), BD_calc_forces (2273-2311) and
- * evolve_Euler_3all (1889-1930).  The mobility is orc_bd_op_cb: one
- * atimes_3all for F (brownian.c:512-520) and FT (559-587), the E = 0
- * mobility problem for FTS (597-646).  For FT / FTS the noise is
- * particle-wise (F, T) (the reference reorders it to [F; T] and back,
- * 1851-1871 and solve_mix_3all), f = F + fact z_F, t = T + fact z_T.
+/* One BD step, every particle mobile, no imposed flow, no interaction
+ * forces: scheme 0 the mid-point algorithm (BD_evolve_mid,
+ * brownian.c:2352-2560), 1 Banchio-Brady 2003 with step parameter bb_n
+ * (BD_evolve_BB03, 2579-2920), 2 Ball-Melrose 1997 (BD_evolve_BM97,
+ * 2970-3190), with BD_calc_FB's atimes route (1545-1874, spectral bounds
+ * and coefficients given, so the refit of 1707-1727 does not happen),
+ * BD_calc_forces (2273-2311) and evolve_Euler_3all (1889-1930).  The
+ * mobility is orc_bd_op_cb: one atimes_3all for F (brownian.c:512-520) and
+ * FT (559-587), the E = 0 mobility problem for FTS (597-646).  For FT /
+ * FTS the noise is particle-wise (F, T) (the reference reorders it to
+ * [F; T] and back, 1851-1871 and solve_mix_3all), f = F + fact z_F, t = T
+ * + fact z_T.
+ *   mid:  u0 = M(x) f, x1 = x + dt/2 u0 | u = M(x1) f, x' = x + dt u
+ *   BM97: u0 = M(x) f, x1 = x + dt u0   | u = (u0 + M(x1) f) / 2, x' = x + dt u
+ *   BB03: u0 = M(x) fact z, x1 = x + dt/n u0 | uB = u0 + n/2 (M(x1) fact z
+ *         - u0), u = M(x1) F + uB (the deterministic product at x1: the
+ *         reference does not reset the positions before it,
+ *         brownian.c:2770-2816), x' = x + dt u
+ * with the overlap checks after x1 (stage 1) and x' (stage 2).
  *
  * draws[ndraw][n] (n = 3 np or 6 np) are the Gaussian vectors BD_calc_FB
  * consumes, one per (re)start of the noise.  fix_fact != 0 recomputes fact
@@ -1171,52 +1181,76 @@ orc_bd_op_cb (int n, const double *x, double *y, void *ud)
  * (and q [4 np] for FT / FTS when not NULL) are updated in place on
  * success; returns the dt taken, or -1 if the draws ran out. */
 double
-orc_bd_mid (orc_sys *s, double *x, double *q, const double *F,
-            const double *T, double peclet, double dt, double dt_lim,
-            double rmin_ol, int ncheb, const double *acoef, double e0,
-            double e1, int ndraw, const double *draws, int fix_fact,
-            int *used, int *halvings, double *err)
+orc_bd_step (orc_sys *s, int scheme, double bb_n, double *x, double *q,
+             const double *F, const double *T, double peclet, double dt,
+             double dt_lim, double rmin_ol, int ncheb, const double *acoef,
+             double e0, double e1, int ndraw, const double *draws,
+             int fix_fact, int *used, int *halvings, double *err)
 {
   const int np = s->np, ver = s->version;
   const int n = np * (ver == 0 ? 3 : 6), n3 = 3 * np;
   double *z = malloc (sizeof (double) * n);
   double *f = malloc (sizeof (double) * n);
   double *u = malloc (sizeof (double) * n);
+  double *u0 = malloc (sizeof (double) * n);
   double *xmid = malloc (sizeof (double) * n3);
   double dt_local = dt;
   double fact = orc_bd_fact (peclet, dt_local);
   int k = 0, nh = 0, stage;
   double ret = -1.0;
   const double *pos0 = s->pos;
+#define UIDX(i) (ver == 0 ? (i) : 6 * ((i) / 3) + ((i) % 3))
+  /* f = (with_ext ? F, T : 0) + zfac * fact z */
+#define FORCES(with_ext, zfac)                                              \
+  for (int i = 0; i < n; i++)                                               \
+    {                                                                       \
+      const double *ext = ver == 0 ? F : ((i % 6) < 3 ? F : T);             \
+      int ie = ver == 0 ? i : 3 * (i / 6) + (i % 3);                        \
+      f[i] = ((with_ext) && ext ? ext[ie] : 0.0) + (zfac) * fact * z[i];    \
+    }
 
-redo:                              /* BD_evolve_mid_REDO */
+redo:                              /* REDO */
   s->pos = x;
-redo_fb:                           /* BD_evolve_mid_REDO_FB */
+redo_fb:                           /* REDO_FB */
   if (k >= ndraw) goto done;
   orc_cheb_eval_atimes (ncheb, acoef, n, draws + (size_t) k * n, z, e0, e1,
                         orc_bd_op_cb, (void *) s);
   *err = orc_cheb_error_minvsqrt (n, draws + (size_t) k * n, z,
                                   orc_bd_op_cb, (void *) s);
   k++;
-redo_scale:                        /* BD_evolve_mid_REDO_scale */
-  for (int i = 0; i < n; i++)
-    {
-      const double *ext = ver == 0 ? F : ((i % 6) < 3 ? F : T);
-      int ie = ver == 0 ? i : 3 * (i / 6) + (i % 3);
-      f[i] = (ext ? ext[ie] : 0.0) + fact * z[i];
-    }
+redo_scale:                        /* REDO_scale */
   for (int i = 0; i < n3; i++) xmid[i] = x[i];
   s->pos = x;
-  orc_bd_op_cb (n, f, u, (void *) s);
-  for (int i = 0; i < n3; i++)
-    xmid[i] += dt_local * 0.5 * u[ver == 0 ? i : 6 * (i / 3) + (i % 3)];
+  if (scheme == 1)
+    {
+      FORCES (0, 1.0);
+    }
+  else
+    {
+      FORCES (1, 1.0);
+    }
+  orc_bd_op_cb (n, f, u0, (void *) s);
+  {
+    const double h = scheme == 0 ? dt_local * 0.5
+      : (scheme == 1 ? dt_local / bb_n : dt_local);
+    for (int i = 0; i < n3; i++) xmid[i] += h * u0[UIDX (i)];
+  }
   stage = 1;
   if (s->rmin == 0.0 && orc_check_overlap (s, xmid, rmin_ol) > 0)
     goto overlap;
   s->pos = xmid;
   orc_bd_op_cb (n, f, u, (void *) s);
-  for (int i = 0; i < n3; i++)
-    xmid[i] = x[i] + dt_local * u[ver == 0 ? i : 6 * (i / 3) + (i % 3)];
+  if (scheme == 2)
+    for (int i = 0; i < n; i++) u[i] = 0.5 * (u0[i] + u[i]);
+  else if (scheme == 1)
+    {
+      const double bbf = 0.5 * bb_n;
+      for (int i = 0; i < n; i++) u0[i] = u0[i] + bbf * (u[i] - u0[i]);
+      FORCES (1, 0.0);
+      orc_bd_op_cb (n, f, u, (void *) s);     /* still at xmid */
+      for (int i = 0; i < n; i++) u[i] += u0[i];
+    }
+  for (int i = 0; i < n3; i++) xmid[i] = x[i] + dt_local * u[UIDX (i)];
   stage = 2;
   if (s->rmin == 0.0 && orc_check_overlap (s, xmid, rmin_ol) > 0)
     goto overlap;
@@ -1243,11 +1277,26 @@ overlap:
   fact = orc_bd_fact (peclet, dt_local);
   goto redo_scale;
 done:
+#undef UIDX
+#undef FORCES
   s->pos = pos0;
   *used = k;
   *halvings = nh;
-  free (z); free (f); free (u); free (xmid);
+  free (z); free (f); free (u); free (u0); free (xmid);
   return ret;
+}
+
+/* the mid-point step (orc_bd_step, scheme 0) */
+double
+orc_bd_mid (orc_sys *s, double *x, double *q, const double *F,
+            const double *T, double peclet, double dt, double dt_lim,
+            double rmin_ol, int ncheb, const double *acoef, double e0,
+            double e1, int ndraw, const double *draws, int fix_fact,
+            int *used, int *halvings, double *err)
+{
+  return orc_bd_step (s, 0, 0.0, x, q, F, T, peclet, dt, dt_lim, rmin_ol,
+                      ncheb, acoef, e0, e1, ndraw, draws, fix_fact, used,
+                      halvings, err);
 }
 
 /* the F-version step (orc_bd_mid with no torques or orientations) */

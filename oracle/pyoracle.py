@@ -70,6 +70,7 @@ def lib():
         L.orc_rng_gaussian.restype = C.c_double
         L.orc_bd_mid_F.restype = C.c_double
         L.orc_bd_mid.restype = C.c_double
+        L.orc_bd_step.restype = C.c_double
         L.orc_check_overlap.restype = C.c_int
         L.orc_xi_by_tratio.argtypes = [C.POINTER(OrcSys), C.c_double]
         L.orc_set_xi.argtypes = [C.POINTER(OrcSys), C.c_double, C.c_double]
@@ -212,10 +213,12 @@ class System:
         return x, dtt, used.value, nh.value, err.value
 
     def bd_mid(self, x, draws, peclet, dt, ncheb, acoef, e0, e1, q=None,
-               F=None, T=None, dt_lim=1e-6, rmin_ol=1.0, fix_fact=False):
-        """one mid-point BD step (brownian.c:2352-2560) of the system's
-        version (F, FT, FTS); draws[k] has 3 np (F) or 6 np values; returns
-        (x_new, q_new, dt_taken, draws_used, halvings, err_cheb)"""
+               F=None, T=None, dt_lim=1e-6, rmin_ol=1.0, fix_fact=False,
+               scheme=0, bb_n=1.0):
+        """one BD step (scheme 0 mid-point brownian.c:2352-2560, 1 BB03,
+        2 BM97) of the system's version (F, FT, FTS); draws[k] has 3 np (F)
+        or 6 np values; returns (x_new, q_new, dt_taken, draws_used,
+        halvings, err_cheb)"""
         x = np.array(x, dtype=np.float64).reshape(-1)
         nop = x.size * (1 if self.version == 0 else 2)
         draws = np.ascontiguousarray(draws, np.float64).reshape(-1)
@@ -228,8 +231,9 @@ class System:
             return None if v is None else _p(np.ascontiguousarray(v, np.float64))
         Fa = None if F is None else np.ascontiguousarray(F, np.float64)
         Ta = None if T is None else np.ascontiguousarray(T, np.float64)
-        dtt = lib().orc_bd_mid(
-            C.byref(self.s), _p(x), None if qq is None else _p(qq), opt(Fa),
+        dtt = lib().orc_bd_step(
+            C.byref(self.s), C.c_int(scheme), C.c_double(bb_n), _p(x),
+            None if qq is None else _p(qq), opt(Fa),
             opt(Ta), C.c_double(peclet), C.c_double(dt), C.c_double(dt_lim),
             C.c_double(rmin_ol), C.c_int(ncheb), _p(a), C.c_double(e0),
             C.c_double(e1), C.c_int(nd), _p(draws),

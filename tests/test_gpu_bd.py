@@ -419,3 +419,61 @@ def test_bd_fts_reference_iteration_parameters():
     _say(f"  |x_gpu - x_oracle| / disp = {e:.2e}")
     assert e < 1e-4
     bd.close()
+
+
+@pytest.mark.parametrize("version,n,scheme,bb_n", [(0, 27, 1, 2.0), (0, 27, 2, 1.0),
+                                                   (1, 27, 1, 4.0), (1, 27, 2, 1.0),
+                                                   (2, 12, 1, 2.0)])
+def test_bd_schemes_match_oracle(version, n, scheme, bb_n):
+    """Banchio-Brady 2003 and Ball-Melrose 1997 steps (positions and, for
+    FT / FTS, quaternions) against the oracle's step fed with the device's
+    draw"""
+    phi = 0.1
+    _say(f"bd scheme {scheme} (n = {bb_n}) version={version} N={n}")
+    pos, L = workloads.configuration(n, phi, seed=7)
+    s = ls.Stokes(version, pos, box=(L, L, L), tr=1.0)
+    o = po.System(version, pos, box=(L, L, L), tr=1.0)
+    G = (o.matrix() if version < 2 else po.ft_mobility_dense(o))
+    w = np.linalg.eigvalsh(0.5 * (G + G.T))
+    e0, e1 = 0.95 * w[0], 1.05 * w[-1]
+    nch, pe, dt = 30, 1.0, 1e-3
+    a = po.cheb_coef(nch, 0, e0, e1)
+    rng = np.random.default_rng(11 + scheme)
+    d3 = 3 * n
+    F = rng.standard_normal(d3)
+    T = rng.standard_normal(d3) if version > 0 else None
+    q = None
+    if version > 0:
+        q = rng.standard_normal((n, 4))
+        q = (q / np.linalg.norm(q, axis=1)[:, None]).reshape(-1)
+    bd = ls.BD(s, peclet=pe, n_cheb=nch, eps=1e-6, dt_lim=1e-6, rmin_ol=1.0,
+               seed=5, force_dev=ls.DeviceArray.from_host(F), bounds=(e0, e1))
+    bd.set_scheme(scheme, bb_n)
+    if version > 0:
+        bd.set_torque(ls.DeviceArray.from_host(T))
+    if version == 2:
+        s.set_iter(eps=1e-13)
+        bd.set_fts_iter(30, 2000, 1e-13)
+    xd = ls.DeviceArray.from_host(pos.reshape(-1))
+    qd = ls.DeviceArray.from_host(q) if q is not None else None
+    x = pos.reshape(-1).copy()
+    for step in range(2):
+        info = bd.step(xd, dt, qd)
+        assert info.attempts == 1
+        y, _ = bd.last_noise()
+        o.pos[:] = x
+        xo, qo, dto, used, nh, err = o.bd_mid(x, y[None], pe, dt, nch, a, e0,
+                                              e1, q=q, F=F, T=T, scheme=scheme,
+                                              bb_n=bb_n)
+        assert used == 1 and nh == 0 and dto == info.dt
+        disp = np.abs(xo - x).max()
+        ex = np.abs(xd.to_host() - xo).max() / disp
+        msg = f"  step {step}: x err / disp {ex:.2e}"
+        assert ex < (1e-8 if version == 2 else 1e-10)
+        if q is not None:
+            eq = np.abs(qd.to_host() - qo).max() / np.abs(qo - q).max()
+            msg += f", q err / dq {eq:.2e}"
+            assert eq < (1e-8 if version == 2 else 1e-10)
+        _say(msg)
+        x, q = xo, qo
+    bd.close()

@@ -335,3 +335,56 @@ def test_bd_step_ft_fts_composition(version):
                                           for i in range(8)])
     assert np.abs(qn - qw.reshape(-1)).max() < 1e-13 * np.abs(q).max()
     assert np.abs(qn - q).max() > 1e-6
+
+
+@pytest.mark.parametrize("scheme,bb_n", [(1, 2.0), (1, 4.0), (2, 1.0)])
+def test_bd_bb03_bm97_vs_reference_composition(scheme, bb_n):
+    """BD_evolve_BB03 (brownian.c:2579-2920) and BD_evolve_BM97
+    (2970-3190), F version, composed from the compiled reference's own
+    chebyshev_eval_atimes and atimes_3all (no overlap in this step); BB03
+    takes its deterministic product at the intermediate configuration, as
+    the reference does (2770-2816)"""
+    import ctypes as C
+    L = 8.0
+    pos = _eight(L)
+    s = po.System(0, pos, box=(L, L, L), tr=1.0)
+    n = pos.size
+    rng = np.random.default_rng(30 + scheme)
+    y = rng.standard_normal(n)
+    F = rng.standard_normal(n)
+    pe, dt, nch = 1.0, 1e-3, 30
+    M = s.matrix()
+    w = np.linalg.eigvalsh(0.5 * (M + M.T))
+    e0, e1 = 0.95 * w[0], 1.05 * w[-1]
+    a = po.cheb_coef(nch, 0, e0, e1)
+    xn, _, dtt, used, nh, err = s.bd_mid(pos, y[None], pe, dt, nch, a, e0, e1,
+                                         F=F, scheme=scheme, bb_n=bb_n)
+    assert used == 1 and nh == 0 and dtt == dt
+    R = po.ref()
+
+    def ref_sys(p):
+        t = po.System(0, p, box=(L, L, L), tr=1.0)
+        return t, R.ref_sys_new(C.byref(t.s))
+
+    def ref_op(h, f):
+        u = np.zeros(n)
+        R.ref_atimes(h, n, po._p(np.ascontiguousarray(f)), po._p(u))
+        return u
+    t0, h0 = ref_sys(pos)
+    z = np.zeros(n)
+    R.ref_cheb_eval_sys(nch, po._p(a), h0, po._p(y), po._p(z), e0, e1)
+    fact = math.sqrt(2 / (pe * dt))
+    x0 = pos.reshape(-1)
+    if scheme == 2:
+        u0 = ref_op(h0, F + fact * z)
+        t1, h1 = ref_sys(x0 + dt * u0)
+        u = 0.5 * (u0 + ref_op(h1, F + fact * z))
+    else:
+        u0 = ref_op(h0, fact * z)
+        t1, h1 = ref_sys(x0 + dt / bb_n * u0)
+        ub = u0 + 0.5 * bb_n * (ref_op(h1, fact * z) - u0)
+        u = ref_op(h1, F) + ub
+    R.ref_sys_free(h0)
+    R.ref_sys_free(h1)
+    want = x0 + dt * u
+    assert np.abs(xn - want).max() < 1e-12 * np.abs(want - x0).max()
