@@ -478,7 +478,7 @@ def main():
                     help="Ewald splitting for the tuned runs (default: tune)")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--batch", type=int, default=32,
+    ap.add_argument("--batch", type=int, default=128,
                     help="right-hand sides per launch for the batched figures")
     ap.add_argument("--ens-boxes", type=int, default=256,
                     help="boxes of the batched-ensemble figure (0: skip)")
@@ -601,6 +601,50 @@ def main():
                 flop_per_pair_image=FLOP_PER_IMAGE_FTS,
                 flop_per_pair_per_column=242.0)
 
+    # ---- the same batched MIXED product with the real-space blocks written
+    # as a dense FP32 matrix once per call (stokes_hip_set_dense_blocks;
+    # rebuilt from the positions on every call) times the block on the
+    # matrix cores: not the matrix-free headline, reported beside it
+    nd_cols = 2 * nb
+    X2 = np.concatenate([X, np.random.default_rng(5).standard_normal(
+        nb * 11 * n)])
+    xb2 = ls.DeviceArray.from_host(X2)
+    yb2 = ls.DeviceArray(nd_cols * 11 * n)
+    bestd = None
+    for tr in ([args.tr] if args.tr else [30.0, 100.0, 300.0]):
+        s = make(tr, ls.MIXED)
+        s.set_dense_blocks(True)
+        t = time_matvec(s, xb2, yb2, 3, 1, nrhs=nd_cols)
+        log(f"dense-block {nd_cols} mixed tr={tr:g}: {t * 1e3:.3f} ms")
+        if bestd is None or t < bestd[1]:
+            bestd = (tr, t)
+        s.close()
+    sd = make(bestd[0], ls.MIXED)
+    sd.set_dense_blocks(True)
+    td = time_matvec(sd, xb2, yb2, args.steps, args.warmup, nrhs=nd_cols)
+    Yd = yb2.to_host().reshape(nd_cols, -1)
+    errs_d = [float(np.linalg.norm(Yd[c] - Yb_64[c]) / np.linalg.norm(Yb_64[c]))
+              for c in range(nb)]
+    # the GEMM of one 128-column pass (HIP events around the real-space
+    # part: the block's FP32 rows and the split-K GEMM)
+    gemm_ms, _ = kernel_times(sd, xb2, yb2, 5, nrhs=nd_cols)
+    sd.close()
+    ndim = 11 * n
+    gflop = 2.0 * ndim * ndim * min(128, nd_cols)
+    dense = {"nrhs": nd_cols, "tr": bestd[0], "ms_per_call": td * 1e3,
+             "matvecs_per_s": nd_cols / td,
+             "rel_err_vs_fp64_max_column": max(errs_d),
+             "roofline": dict(bound="mfma",
+                              achieved=gflop / (gemm_ms * 1e-3) / 1e12,
+                              peak=PEAK_FP32, unit="TFLOP/s",
+                              frac=gflop / (gemm_ms * 1e-3) / 1e12 / PEAK_FP32,
+                              traffic=None,
+                              kernel="k_cache_gemm<8> (one 128-column pass: "
+                                     "2 (11N)^2 128 flop)",
+                              kernel_ms=gemm_ms)}
+    log(f"dense-block {nd_cols}: {td * 1e3:.3f} ms (tr={bestd[0]:g}), "
+        f"{nd_cols / td:.0f} matvecs/s, max col err {max(errs_d):.2e}")
+
     out = {
         "metric": METRIC,
         "value": nb / tb_mix,
@@ -630,6 +674,7 @@ def main():
                    "rel_err_fp64_tuned_vs_tr1": err_ref_split,
                    "rel_err_mixed_vs_fp64_tr1": err_mix_tr1},
         "roofline": roof,
+        "dense_real_blocks": dense,
     }
     if args.bd_steps > 0:
         out["bd"] = bench_bd(args, log)

@@ -67,6 +67,13 @@ int stokes_hip_set_split (struct stokes *sys, double tratio);
  * configuration and the pair sum costs more than reading the matrix (the
  * F version: 9 values per pair).  No counterpart in the reference. */
 int stokes_hip_set_real_cache (struct stokes *sys, int enable);
+/* enable: MIXED products of more columns than one wave covers (32 FTS, 64
+ * FT, 128 F) write the real-space blocks once per call as a dense FP32
+ * matrix ((D np)^2 values, at most 4 GiB, rebuilt from the positions on
+ * every call) and multiply it with the block on the matrix cores, instead
+ * of evaluating every pair once per 32-column group matrix-free.  Off by
+ * default (matrix-free).  No counterpart in the reference. */
+int stokes_hip_set_dense_blocks (struct stokes *sys, int enable);
 /* time reps operator calls of nrhs columns at each candidate tratio (NULL:
  * 0 (= sys->xi), 10, 100, 300, 1000, 3000, 1e4) and keep the fastest as
  * the splitting of sys */

@@ -70,7 +70,7 @@ EXPORTS = [
     "stokes_hip_bd_last_noise", "stokes_hip_bd_destroy",
     "stokes_hip_bd_step_q", "stokes_hip_bd_set_torque",
     "stokes_hip_bd_set_fts_iter", "stokes_hip_ensemble_autotune",
-    "stokes_hip_bd_set_scheme",
+    "stokes_hip_bd_set_scheme", "stokes_hip_set_dense_blocks",
 ]
 
 
@@ -163,6 +163,7 @@ def lib():
         getattr(L, f).argtypes = [sp, C.c_int]
     L.stokes_hip_set_split.argtypes = [sp, C.c_double]
     L.stokes_hip_set_real_cache.argtypes = [sp, C.c_int]
+    L.stokes_hip_set_dense_blocks.argtypes = [sp, C.c_int]
     L.stokes_hip_solve_mix_dev.argtypes = [sp, _vp, _vp]
     L.stokes_hip_solve_mob_3fts_dev.argtypes = [sp, _vp, _vp,
                                                 C.POINTER(C.c_int), _dp]
@@ -407,6 +408,11 @@ class Stokes:
         following products (keep the positions fixed)"""
         check(self.L.stokes_hip_set_real_cache(self.sys, int(bool(on))),
               "set_real_cache")
+
+    def set_dense_blocks(self, on=True):
+        """per-call dense real-space matrix for wide MIXED blocks"""
+        check(self.L.stokes_hip_set_dense_blocks(self.sys, int(bool(on))),
+              "set_dense_blocks")
 
     def set_iter(self, solver="gmres", max_it=2000, restart=30, eps=1e-12):
         self.L.stokes_set_iter(self.sys, solver.encode(), max_it, restart,

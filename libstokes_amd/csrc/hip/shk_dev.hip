@@ -64,6 +64,9 @@ struct shk_dev {
   DevBuf adjfg, adjcs, adjpart;
   DevBuf km, ptab;                // k lattice indices, per-particle phase tables
   unsigned long pos_gen = 0;      // positions uploads so far
+  unsigned long dense_gen = 0;    // positions of the per-call dense matrix
+  int dense_ok = 0;               // rcache holds that matrix (FP32)
+  int dense_blocks = 0;           // allow the per-call dense path
   unsigned long ptab_gen = 0;     // positions the single-column table was built
   int ptab_kind = 0;              // for (1 FP32, 2 FP64; 0 none)
   DevBuf rcache, rz;              // dense real-space matrix, its vector(s)
@@ -308,6 +311,7 @@ extern "C" int shk_dev_set_system (shk_dev *d, const shk_params *p, const double
   d->prm = *p;
   d->rc_use = 0;                  // a cached real-space matrix is stale now
   d->ptab_kind = 0;               // so are the phase tables (k basis)
+  d->dense_ok = 0;
   int rc = upload_pos (d, pos, a, 0, 0);
   if (rc) return rc;
   if (p->periodic)
@@ -746,7 +750,7 @@ static int recip_block_launch_mfma (shk_dev *d, const float *Zt, int zs, int npa
 }
 
 template <int V, typename T>
-static int atimes_block_vt (shk_dev *d, const double *x, double *y, int B)
+static int atimes_block_vt (shk_dev *d, const double *x, double *y, int B, bool dense)
 {
   constexpr int D = Dim<V>::D;
   constexpr int JW = 256;
@@ -802,8 +806,7 @@ static int atimes_block_vt (shk_dev *d, const double *x, double *y, int B)
   const void *cand = std::is_same<T, double>::value ? d->cand.p : d->candf.p;
   // timing: ev[0] -> ev[1] brackets the real-space kernel alone
   if (d->timing) HIPCHK (hipEventRecord (d->ev[0], d->stream));
-  const int want_prec = MF ? SHK_MIXED : SHK_FP64;
-  if (d->rc_use && d->rc_prec == want_prec)
+  if (dense)
     {
       // real space from the cached dense matrix: Y (ND x B, row-major) =
       // Mr (ND x ND, row-major) Z (ND x B, row-major).  MIXED: split-K
@@ -939,11 +942,35 @@ static int atimes_block_vt (shk_dev *d, const double *x, double *y, int B)
   return SHK_OK;
 }
 
+template <int V, typename T> static int real_cache_build (shk_dev *d);
+
+// MIXED blocks wider than one wave's column tiles (32 / 64 / 128 columns for
+// FTS / FT / F) evaluate every pair's scalars once per column group in the
+// matrix-free kernel.  With dense blocks allowed (shk_dev_set_dense_blocks)
+// the real-space blocks are instead written once as a dense FP32 matrix for
+// this call (rebuilt after every positions upload, reused by the further
+// 128-column passes of the call) and multiplied by the split-K GEMM, when
+// the matrix stays within 4 GiB
 template <int V>
 static int atimes_block_v (shk_dev *d, const double *x, double *y, int B, int prec)
 {
-  return prec == SHK_MIXED ? atimes_block_vt<V, float> (d, x, y, B)
-                           : atimes_block_vt<V, double> (d, x, y, B);
+  constexpr int D = Dim<V>::D;
+  constexpr int CTMAX = V == 2 ? 2 : (V == 1 ? 4 : 8);
+  const shk_params &p = d->prm;
+  const bool explicit_cache = d->rc_use && d->rc_prec == prec;
+  const double nd = (double) p.np * D;
+  const bool auto_dense = d->dense_blocks && !d->rc_use && prec == SHK_MIXED && p.periodic
+                          && B > 16 * CTMAX && nd * nd * 4.0 <= 4.0 * 1024.0 * 1024.0 * 1024.0;
+  if (auto_dense && !(d->dense_ok && d->dense_gen == d->pos_gen))
+    {
+      int rc = real_cache_build<V, float> (d);
+      if (rc) return rc;
+      d->dense_ok = 1;
+      d->dense_gen = d->pos_gen;
+    }
+  const bool dense = explicit_cache || auto_dense;
+  return prec == SHK_MIXED ? atimes_block_vt<V, float> (d, x, y, B, dense)
+                           : atimes_block_vt<V, double> (d, x, y, B, dense);
 }
 
 template <int V, typename T>
@@ -969,6 +996,13 @@ static int real_cache_build (shk_dev *d)
   return SHK_OK;
 }
 
+extern "C" int shk_dev_set_dense_blocks (shk_dev *d, int on)
+{
+  if (!d) { set_err ("shk_dev_set_dense_blocks: no device context"); return SHK_EARG; }
+  d->dense_blocks = on != 0;
+  return SHK_OK;
+}
+
 extern "C" int shk_dev_real_cache (shk_dev *d, int on, int prec)
 {
   if (d && !on)
@@ -979,6 +1013,7 @@ extern "C" int shk_dev_real_cache (shk_dev *d, int on, int prec)
   if (!d || !d->have_sys) { set_err ("shk_dev_real_cache: no system"); return SHK_EARG; }
   const shk_params &p = d->prm;
   if (!p.periodic) { set_err ("shk_dev_real_cache: periodic systems only"); return SHK_EARG; }
+  d->dense_ok = 0;                // the buffer now holds the explicit cache
   int rc;
   if (p.version == 0)
     rc = prec == SHK_MIXED ? real_cache_build<0, float> (d) : real_cache_build<0, double> (d);

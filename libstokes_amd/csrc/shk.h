@@ -138,6 +138,9 @@ int shk_dev_adjoint_x (shk_dev *d, const double *f, const double *g,
  * precision prec (periodic systems) and use it for the real-space part of
  * every single-rhs product until turned off; off: back to the pair kernels.
  * The caller keeps the positions fixed while it is on. */
+/* allow (on != 0) the per-call dense FP32 real-space matrix for MIXED
+ * blocks wider than one wave's column tiles */
+int shk_dev_set_dense_blocks (shk_dev *d, int on);
 int shk_dev_real_cache (shk_dev *d, int on, int prec);
 
 /* y = a x + b y (len doubles) */

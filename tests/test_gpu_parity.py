@@ -201,16 +201,21 @@ def test_block_rhs(version, poly, nrhs):
     s.close()
 
 
-@pytest.mark.parametrize("tr,B", [(3000.0, 32), (300.0, 7)])
-def test_block_mixed_full_size(tr, B):
+@pytest.mark.parametrize("tr,B,dense", [(3000.0, 32, False), (300.0, 7, False),
+                                        (300.0, 128, False), (300.0, 128, True),
+                                        (1000.0, 200, True)])
+def test_block_mixed_full_size(tr, B, dense):
     """N = 1024, phi = 0.3 FTS: the MIXED block path (FP32 matrix-core
-    reciprocal sums) against the FP64 block path column by column, 1e-6;
-    bitwise repeatable"""
+    sums; with dense blocks allowed, beyond 32 columns the real-space part
+    as a per-call dense FP32 matrix times the block, 200 columns: two
+    passes on one matrix) against the FP64 block path column by column,
+    1e-6; bitwise repeatable"""
     n = 1024
     pos, L = workloads.configuration(n, 0.3, seed=1)
     X = np.random.default_rng(3).standard_normal(B * 11 * n)
     s = ls.Stokes(2, pos, box=(L,) * 3, tr=1.0)
     s.set_split(tr)
+    s.set_dense_blocks(dense)
     xd = ls.DeviceArray.from_host(X)
     yd = ls.DeviceArray(X.size)
     s.atimes_dev(xd, yd, nrhs=B)
@@ -352,3 +357,31 @@ def test_full_size_row_sample_and_properties(n, phi, version):
     assert e_mix <= TOLMIX
     assert e_tr <= 1e-11
     assert np.array_equal(y64, y64b)
+
+
+def test_block_mixed_dense_unequal_spheres():
+    """unequal spheres, FTS, 48 columns: the per-call dense real-space path
+    of the MIXED block against the FP64 block path (1e-6) and against the
+    oracle on two columns (1e-6)"""
+    n = 120
+    pos, a, L = workloads.configuration_poly(n, 0.2, seed=21)
+    B = 48
+    X = np.random.default_rng(22).standard_normal(B * 11 * n)
+    s = ls.Stokes(2, pos, box=(L,) * 3, tr=1.0, a=a)
+    s.set_split(100.0)
+    s.set_dense_blocks(True)
+    xd = ls.DeviceArray.from_host(X)
+    yd = ls.DeviceArray(X.size)
+    s.atimes_dev(xd, yd, nrhs=B)
+    Y = yd.to_host().reshape(B, -1)
+    s.set_precision(ls.MIXED)
+    s.atimes_dev(xd, yd, nrhs=B)
+    Ym = yd.to_host().reshape(B, -1)
+    errs = [relerr(Ym[c], Y[c]) for c in range(B)]
+    _say(f"  unequal spheres N={n} B={B}: MIXED dense block vs FP64 block "
+         f"max {max(errs):.2e}")
+    assert max(errs) <= TOLMIX
+    o = po.System(2, pos, box=(L, L, L), tr=1.0, a=a)
+    for c in (0, B - 1):
+        assert relerr(Ym[c], o.atimes(X[c * 11 * n:(c + 1) * 11 * n])) <= TOLMIX
+    s.close()
