@@ -250,7 +250,9 @@ typedef struct stokes_hip_bd_info {
 int stokes_hip_bd_set_real_cache (stokes_hip_bd *bd, int enable);
 
 /* sys: version 0 (F), 1 (FT) or 2 (FTS), equal radii (a == NULL), no
- * shear, set up as for atimes_3all.  F_dev: constant force [3 np] (BD->F)
+ * shear, set up as for atimes_3all; particles nm..np-1 (stokes_set_np) are
+ * fixed: the step moves x [3 nm] (and q [4 nm]), the fixed positions are
+ * read from sys->pos at each step.  F_dev: constant force [3 np] (BD->F)
  * or NULL; it is read on
  * every step and must stay allocated until stokes_hip_bd_destroy.  peclet:
  * fact = sqrt(2 / (Pe dt)), 0 when Pe <= 0 (brownian.c:2177-2187).
@@ -268,7 +270,7 @@ int stokes_hip_bd_set_bounds (stokes_hip_bd *bd, double lmin, double lmax);
 /* enable != 0: recompute fact when a redraw restores dt (the reference
  * keeps the value of the halved dt, brownian.c:2446-2449, 2512-2515) */
 int stokes_hip_bd_set_fact_fix (stokes_hip_bd *bd, int enable);
-/* one step from the positions x_dev [3 np] (updated in place), step dt */
+/* one step from the positions x_dev [3 nm] (updated in place), step dt */
 int stokes_hip_bd_step (stokes_hip_bd *bd, double *x_dev, double dt,
                         stokes_hip_bd_info *info);
 /* the same with the orientations (BD_evolve_mid's q, brownian.c:2352):

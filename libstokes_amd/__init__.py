@@ -295,16 +295,17 @@ class Stokes:
 
     def __init__(self, version, pos, box=None, tr=None, xi=None, eps=1e-12,
                  a=None, rmin=0.0, shear_mode=0, shear_shift=0.0,
-                 precision=FP64):
+                 precision=FP64, nm=None):
         L = lib()
         self.L = L
         self.sys = L.stokes_init()
         s = self.sys.contents
         pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1)
         self.np = pos.size // 3
+        self.nm = self.np if nm is None else int(nm)
         s.version = version
         self.version = version
-        L.stokes_set_np(self.sys, self.np, self.np)
+        L.stokes_set_np(self.sys, self.np, self.nm)
         L.stokes_set_pos(self.sys, _p(pos))
         if a is not None:
             a = np.ascontiguousarray(a, dtype=np.float64)
@@ -602,7 +603,7 @@ class BD:
         return info
 
     def last_noise(self):
-        n = (3 if self.stokes.version == 0 else 6) * self.stokes.np
+        n = (3 if self.stokes.version == 0 else 6) * self.stokes.s.nm
         y = DeviceArray(n)
         z = DeviceArray(n)
         check(self.L.stokes_hip_bd_last_noise(self.h, y.ptr, z.ptr),

@@ -42,7 +42,10 @@ struct stokes_hip_bd {
   stokes_ctx *c;
   const double *F, *T;         /* device [3 np] or NULL */
   int ver;                     /* sys->version */
-  int np;
+  int np, nm;                  /* particles, mobile particles (0..nm-1) */
+  double *posfull;             /* device [3 np]: mobile from x, fixed from
+                                  sys->pos (nm < np only) */
+  double *fpad, *upad;         /* device [D np]: zero-padded F / FT products */
   double peclet, eps, dt_lim, rmin_ol;
   int ncheb;
   unsigned long long seed;
@@ -135,10 +138,11 @@ stokes_hip_bd_create (struct stokes *sys, const double *F_dev, double peclet,
       stokes_hip_set_error ("bd_create: version %d", sys->version);
       return STOKES_HIP_EARG;
     }
-  if (sys->a != NULL || sys->shear_mode != 0 || sys->nm != sys->np)
+  if (sys->a != NULL || sys->shear_mode != 0 || sys->nm < 1
+      || sys->nm > sys->np)
     {
-      stokes_hip_set_error ("bd_create: needs equal radii, no shear and every"
-                            " particle mobile");
+      stokes_hip_set_error ("bd_create: needs equal radii, no shear and"
+                            " 1 <= nm <= np");
       return STOKES_HIP_EUNSUP;
     }
   if (n_cheb < 2 || !(eps > 0.0) || dt_lim < 0.0 || sys->np <= 0)
@@ -165,7 +169,8 @@ stokes_hip_bd_create (struct stokes *sys, const double *F_dev, double peclet,
   bd->seed = seed;
   bd->ver = sys->version;
   bd->np = sys->np;
-  bd->n = (size_t) sys->np * (sys->version == 0 ? 3 : 6);
+  bd->nm = sys->nm;
+  bd->n = (size_t) sys->nm * (sys->version == 0 ? 3 : 6);
   bd->fts_m = 20;
   bd->fts_max = 2000;
   bd->fts_eps = 1e-6;
@@ -177,7 +182,8 @@ stokes_hip_bd_create (struct stokes *sys, const double *F_dev, double peclet,
   bd->sc_fact = bd->sc_dt = -1.0;
   bd->acoef = calloc ((size_t) n_cheb, sizeof (double));
   const size_t n = bd->n, np = (size_t) sys->np;
-  const size_t nbuf = 11 * n + 10 * np;
+  const size_t dfull = (size_t) sys->np * (sys->version == 0 ? 3 : 6);
+  const size_t nbuf = 11 * n + 10 * np + 3 * np + 2 * dfull;
   int rc = bd->acoef ? 0 : SHK_ENOMEM;
   if (!rc) rc = shk_dev_malloc (c->dev, (void **) &bd->buf, sizeof (double) * nbuf);
   if (!rc && bd->ver == 2)
@@ -202,6 +208,9 @@ stokes_hip_bd_create (struct stokes *sys, const double *F_dev, double peclet,
   bd->qn = bd->xn + 3 * np;
   bd->u0 = bd->qn + 4 * np;
   bd->f2 = bd->u0 + n;
+  bd->posfull = bd->f2 + n;
+  bd->fpad = bd->posfull + 3 * np;
+  bd->upad = bd->fpad + dfull;
   bd->bb_n = 1.0;
   if (bd->fts_buf)
     {
@@ -219,6 +228,8 @@ stokes_hip_bd_create (struct stokes *sys, const double *F_dev, double peclet,
   unsigned long long zero = 0;
   rc = shk_dev_h2d (c->dev, bd->counter, &zero, sizeof (zero));
   if (!rc) rc = shk_dev_scal (c->dev, bd->buf, NULL, 0.0, nbuf);
+  if (!rc && bd->fts_buf)
+    rc = shk_dev_scal (c->dev, bd->fts_buf, NULL, 0.0, 22 * np);
   if (!rc) rc = shk_dev_sync (c->dev);
   if (rc)
     {
@@ -349,16 +360,38 @@ static int
 bd_op (stokes_hip_bd *bd, shk_dev *d, int prec, const double *x, double *y,
        int noise)
 {
-  if (bd->ver != 2) return shk_dev_atimes (d, x, y, 1, prec);
+  if (bd->ver != 2)
+    {
+      if (bd->nm == bd->np) return shk_dev_atimes (d, x, y, 1, prec);
+      /* fixed particles: zero forces there, the mobile rows kept
+       * (brownian.c:521-547, 562-588); the tail of fpad stays zero */
+      int rc = shk_dev_scal (d, bd->fpad, x, 1.0, bd->n);
+      if (!rc) rc = shk_dev_atimes (d, bd->fpad, bd->upad, 1, prec);
+      if (!rc) rc = shk_dev_scal (d, y, bd->upad, 1.0, bd->n);
+      return rc;
+    }
+  /* FTS: the mixed problem with the fixed particles at rest (U = O = E = 0;
+   * the tail of fts_b stays zero); nm = np is the mobility problem */
   int m = bd->fts_m, maxit = bd->fts_max, it = 0;
   double eps = bd->fts_eps, rr = 0.0;
   if (!noise) stokes_hip_iter_params (bd->sys, &m, &maxit, &eps);
-  int rc = shk_dev_ft_fts (d, x, bd->np, 0, bd->fts_b);
-  if (!rc) rc = stokes_hip_fts_solve (bd->c, 1, bd->np, bd->fts_b, bd->fts_x,
+  int rc = shk_dev_ft_fts (d, x, bd->nm, 0, bd->fts_b);
+  if (!rc) rc = stokes_hip_fts_solve (bd->c, 1, bd->nm, bd->fts_b, bd->fts_x,
                                       m, maxit, eps, bd->fts_ws, &it, &rr);
   bd->fts_iters += it;
-  if (!rc) rc = shk_dev_ft_fts (d, bd->fts_x, bd->np, 1, y);
+  if (!rc) rc = shk_dev_ft_fts (d, bd->fts_x, bd->nm, 1, y);
   return rc;
+}
+
+/* positions of all particles for the operator: x [3 nm] itself when every
+ * particle is mobile, else x copied into the mobile part of posfull (the
+ * fixed part uploaded by bd_step_q) */
+static const double *
+bd_pos (stokes_hip_bd *bd, shk_dev *d, const double *x, int *rc)
+{
+  if (bd->nm == bd->np) return x;
+  *rc = shk_dev_scal (d, bd->posfull, x, 1.0, 3 * (size_t) bd->nm);
+  return bd->posfull;
 }
 
 /* Clenshaw recursion of chebyshev.c:163-228 over bd_op: z = sum_j a_j
@@ -393,11 +426,12 @@ bd_record (stokes_hip_bd *bd, shk_dev *d, int prec, int check_ol, double *x,
 {
   const struct stokes *s = bd->sys;
   const size_t n = bd->n;
-  const size_t n3 = (size_t) bd->np * 3;
+  const size_t n3 = (size_t) bd->nm * 3;
   int rc = shk_dev_bd_status_clear (d, bd->st_dev);
   if (!rc) rc = shk_dev_gauss (d, bd->seed, bd->counter, n, bd->y);
   /* BD_calc_FB at x: z = M^{-1/2} y and its accuracy (chebyshev.c:241-269) */
-  if (!rc) rc = shk_dev_set_pos (d, x, 1);
+  const double *px = rc ? NULL : bd_pos (bd, d, x, &rc);
+  if (!rc) rc = shk_dev_set_pos (d, px, 1);
   /* the products at x share one dense real-space matrix */
   if (!rc && bd->real_cache) rc = shk_dev_real_cache (d, 1, prec);
   if (!rc) rc = bd_cheb (bd, d, prec, bd->y, bd->z);
@@ -413,15 +447,16 @@ bd_record (stokes_hip_bd *bd, shk_dev *d, int prec, int check_ol, double *x,
   if (!rc)
     rc = bd->ver == 0
       ? shk_dev_bd_force (d, bd->f, Fx, bd->z, bd->sc_dev, n)
-      : shk_dev_bd_force_ft (d, bd->f, Fx, Tx, bd->z, bd->sc_dev, bd->np);
+      : shk_dev_bd_force_ft (d, bd->f, Fx, Tx, bd->z, bd->sc_dev, bd->nm);
   if (!rc) rc = bd_op (bd, d, prec, bd->f, bd->u0, 0);
   if (!rc)
     rc = bd->ver == 0
       ? shk_dev_bd_euler (d, bd->xm, x, bd->u0, bd->sc_dev, h1, n3)
       : shk_dev_bd_euler_ft (d, bd->xm, x, NULL, NULL, bd->u0, bd->sc_dev, h1,
-                             bd->np);
+                             bd->nm);
+  /* the mobile particles only (check_overlap, brownian.c:305-310) */
   if (!rc && check_ol)
-    rc = shk_dev_bd_overlap (d, bd->xm, NULL, s->np, s->periodic, s->lx, s->ly,
+    rc = shk_dev_bd_overlap (d, bd->xm, NULL, bd->nm, s->periodic, s->lx, s->ly,
                              s->lz, 0, 0.0, bd->rmin_ol, &bd->st_dev->overlap[0]);
   /* stage 2: drift at the intermediate configuration with the same forces;
    * mid-point: u = M(x1) f; BM97: u = (u0 + M(x1) f) / 2; BB03: u =
@@ -429,7 +464,8 @@ bd_record (stokes_hip_bd *bd, shk_dev *d, int prec, int check_ol, double *x,
    * x1 as in the reference (brownian.c:2770-2816); then the full step from
    * x */
   if (!rc && bd->real_cache) rc = shk_dev_real_cache (d, 0, prec);
-  if (!rc) rc = shk_dev_set_pos (d, bd->xm, 1);
+  const double *pm = rc ? NULL : bd_pos (bd, d, bd->xm, &rc);
+  if (!rc) rc = shk_dev_set_pos (d, pm, 1);
   if (!rc) rc = bd_op (bd, d, prec, bd->f, bd->u, 0);
   if (!rc && sch == 2) rc = shk_dev_axpby (d, bd->u, bd->u0, 0.5, 0.5, n);
   if (!rc && sch == 1)
@@ -438,7 +474,7 @@ bd_record (stokes_hip_bd *bd, shk_dev *d, int prec, int check_ol, double *x,
       if (!rc)
         rc = bd->ver == 0
           ? shk_dev_bd_force (d, bd->f2, bd->F, NULL, bd->sc_dev, n)
-          : shk_dev_bd_force_ft (d, bd->f2, bd->F, bd->T, NULL, bd->sc_dev, bd->np);
+          : shk_dev_bd_force_ft (d, bd->f2, bd->F, bd->T, NULL, bd->sc_dev, bd->nm);
       if (!rc) rc = bd_op (bd, d, prec, bd->f2, bd->u, 0);
       if (!rc) rc = shk_dev_axpby (d, bd->u, bd->u0, 1.0, 1.0, n);
     }
@@ -446,9 +482,9 @@ bd_record (stokes_hip_bd *bd, shk_dev *d, int prec, int check_ol, double *x,
     rc = bd->ver == 0
       ? shk_dev_bd_euler (d, bd->xn, x, bd->u, bd->sc_dev, 1.0, n3)
       : shk_dev_bd_euler_ft (d, bd->xn, x, q ? bd->qn : NULL, q, bd->u,
-                             bd->sc_dev, 1.0, bd->np);
+                             bd->sc_dev, 1.0, bd->nm);
   if (!rc && check_ol)
-    rc = shk_dev_bd_overlap (d, bd->xn, NULL, s->np, s->periodic, s->lx, s->ly,
+    rc = shk_dev_bd_overlap (d, bd->xn, NULL, bd->nm, s->periodic, s->lx, s->ly,
                              s->lz, 0, 0.0, bd->rmin_ol, &bd->st_dev->overlap[1]);
   if (!rc) rc = shk_dev_counter_add (d, bd->counter, 1);
   if (!rc) rc = shk_dev_d2h_async (d, bd->st_host, bd->st_dev, sizeof (shk_bd_status));
@@ -465,12 +501,14 @@ bd_capture (stokes_hip_bd *bd, stokes_ctx *c, double *x, double *q)
   /* every work buffer the operator uses must exist before the capture:
    * one eager product allocates them */
   shk_dev_timing (d, 0);
-  int rc = shk_dev_set_pos (d, x, 1);
-  if (!rc) rc = shk_dev_atimes (d, bd->z, bd->w, 1, c->prec);
+  int rc = 0;
+  const double *px = bd_pos (bd, d, x, &rc);
+  if (!rc) rc = shk_dev_set_pos (d, px, 1);
+  if (!rc) rc = bd_op (bd, d, c->prec, bd->z, bd->w, 1);
   if (!rc && bd->real_cache)
     {
       rc = shk_dev_real_cache (d, 1, c->prec);
-      if (!rc) rc = shk_dev_atimes (d, bd->z, bd->w, 1, c->prec);
+      if (!rc) rc = bd_op (bd, d, c->prec, bd->z, bd->w, 1);
       int rc2 = shk_dev_real_cache (d, 0, c->prec);
       if (!rc) rc = rc2;
     }
@@ -526,9 +564,10 @@ bd_fit_bounds (stokes_hip_bd *bd, stokes_ctx *c, double *x)
 {
   double l0, l1;
   int level = bd->fits < 3 ? bd->fits : 3;
-  c->pos_dev = x;
   /* the products are at x (an attempt leaves x_mid on the device) */
-  int rc = stokes_hip_prepare (c, 1);
+  int rc = 0;
+  c->pos_dev = bd_pos (bd, c->dev, x, &rc);
+  if (!rc) rc = stokes_hip_prepare (c, 1);
   if (rc) return rc;
   if (bd->real_cache)
     {
@@ -570,7 +609,16 @@ stokes_hip_bd_step_q (stokes_hip_bd *bd, double *x_dev, double *q_dev,
   stokes_hip_bd_info inf;
   memset (&inf, 0, sizeof (inf));
   int rc = 0;
-  c->pos_dev = x_dev;
+  if (bd->nm < bd->np)
+    {
+      /* the fixed particles where the system holds them now */
+      rc = shk_dev_h2d (d, bd->posfull + 3 * (size_t) bd->nm,
+                        bd->sys->pos + 3 * bd->nm,
+                        sizeof (double) * 3 * (size_t) (bd->np - bd->nm));
+      if (rc) { rc = bd_fail (rc, "bd_step"); goto out; }
+    }
+  c->pos_dev = bd_pos (bd, d, x_dev, &rc);
+  if (rc) { rc = bd_fail (rc, "bd_step"); goto out; }
   /* parameters of sys current on the device (positions from x_dev) */
   rc = stokes_hip_prepare (c, 1);
   if (rc) goto out;
@@ -671,9 +719,9 @@ stokes_hip_bd_step_q (stokes_hip_bd *bd, double *x_dev, double *q_dev,
         }
     }
   /* accepted: x <- x_new */
-  rc = shk_dev_d2d (d, x_dev, bd->xn, sizeof (double) * 3 * bd->np);
+  rc = shk_dev_d2d (d, x_dev, bd->xn, sizeof (double) * 3 * bd->nm);
   if (!rc && q_dev)
-    rc = shk_dev_d2d (d, q_dev, bd->qn, sizeof (double) * 4 * bd->np);
+    rc = shk_dev_d2d (d, q_dev, bd->qn, sizeof (double) * 4 * bd->nm);
   if (rc) { rc = bd_fail (rc, "bd_step"); goto out; }
   inf.dt = dt_local;
 out:

@@ -1011,9 +1011,17 @@ orc_bd_noise_F (const orc_sys *s, int ncheb, const double *acoef,
 int
 orc_check_overlap (const orc_sys *s, const double *pos, double rmin)
 {
+  return orc_check_overlap_n (s, pos, s->np, rmin);
+}
+
+/* the same over the first n particles (check_overlap loops over the
+ * mobile particles only, brownian.c:305-310) */
+int
+orc_check_overlap_n (const orc_sys *s, const double *pos, int n, double rmin)
+{
   int flag = 0;
-  for (int i = 0; i < s->np; i++)
-    for (int j = i; j < s->np; j++)
+  for (int i = 0; i < n; i++)
+    for (int j = i; j < n; j++)
       {
         double x = pos[3 * i] - pos[3 * j];
         double y = pos[3 * i + 1] - pos[3 * j + 1];
@@ -1060,29 +1068,44 @@ orc_quat_dqdt (const double *Q, const double *O, double *dQdt)
     }
 }
 
-/* The FTS mobility problem with E = 0 restricted to (F, T) -> (U, O)
- * particle-wise (BD_atimes_mob_FU, brownian.c:597-646, which solves it with
- * GMRES through solve_mob_3fts_0): the true operator M (11 np)^2 is the
- * extracted-form matrix (orc_matrix_ewald / _nonewald) times the S
- * transform of fts.c:660-670 (columns 6 and 10 of a particle: 2 m_6 +
- * m_10 and 2 m_10 + m_6, columns 7-9: 2 m_c), S = -M_ES^{-1} M_EF (F, T)
- * by Gaussian elimination with partial pivoting, (U, O) = M_UF (F, T) +
- * M_US S.  Exact to rounding: the limit of the reference's iteration.
- * O(np^3); small systems only. */
+/* The FTS mobility problem with E = 0 restricted to (F, T) -> (U, O) of
+ * the nm mobile particles, the np - nm fixed ones at rest (U = O = E = 0):
+ * BD_atimes_mob_FU, brownian.c:597-729, which solves it with GMRES through
+ * solve_mob_3fts_0 (nm = np) or solve_mix_3fts_0.  The true operator M
+ * (11 np)^2 is the extracted-form matrix (orc_matrix_ewald / _nonewald)
+ * times the S transform of fts.c:660-670 (columns 6 and 10 of a particle:
+ * 2 m_6 + m_10 and 2 m_10 + m_6, columns 7-9: 2 m_c); with K the known
+ * forces (F, T of the mobile particles), X the unknowns (S of the mobile,
+ * all 11 of the fixed particles) and R the rows with known zero (E of the
+ * mobile, all 11 of the fixed particles): M[R, X] xi = -M[R, K] ft by
+ * Gaussian elimination with partial pivoting, (U, O) = M[UO, K] ft +
+ * M[UO, X] xi.  Exact to rounding: the limit of the reference's
+ * iteration.  O(np^3); small systems only. */
 void
-orc_fts_mob_ft (const orc_sys *s, const double *ft, double *uo)
+orc_fts_mix_ft (const orc_sys *s, int nm, const double *ft, double *uo)
 {
-  const int np = s->np, n = 11 * np, ne = 5 * np;
+  const int np = s->np, n = 11 * np;
+  const int nk = 6 * nm, nx = 5 * nm + 11 * (np - nm);
+  int *K = malloc (sizeof (int) * (nk > 0 ? nk : 1));
+  int *X = malloc (sizeof (int) * (nx > 0 ? nx : 1));
+  for (int i = 0, a = 0, b = 0; i < np; i++)
+    for (int q = 0; q < 11; q++)
+      {
+        if (i < nm && q < 6) K[a++] = 11 * i + q;
+        else X[b++] = 11 * i + q;
+      }
+  /* the rows of known zero velocity are the unknowns' indices */
+  const int *R = X;
   double *M = malloc (sizeof (double) * (size_t) n * n);
-  double *A = malloc (sizeof (double) * (size_t) ne * ne);
-  double *b = malloc (sizeof (double) * ne);
-  double *S = malloc (sizeof (double) * ne);
+  double *A = malloc (sizeof (double) * (size_t) nx * nx);
+  double *b = malloc (sizeof (double) * nx);
+  double *S = malloc (sizeof (double) * nx);
   if (s->periodic) orc_matrix_ewald (s, M);
   else orc_matrix_nonewald (s, M);
   for (int r = 0; r < n; r++)
-    for (int i = 0; i < np; i++)
+    for (int p = 0; p < np; p++)
       {
-        double *m = M + (size_t) r * n + 11 * i + 6;
+        double *m = M + (size_t) r * n + 11 * p + 6;
         double m6 = m[0], m10 = m[4];
         m[0] = 2.0 * m6 + m10;
         m[1] *= 2.0;
@@ -1090,58 +1113,55 @@ orc_fts_mob_ft (const orc_sys *s, const double *ft, double *uo)
         m[3] *= 2.0;
         m[4] = 2.0 * m10 + m6;
       }
-  /* rows / columns of E, S: 11 i + 6 + k */
-#define ES(i, k) (11 * (i) + 6 + (k))
-#define FT(i, k) (11 * (i) + (k))
-  for (int a = 0; a < ne; a++)
+  for (int a = 0; a < nx; a++)
     {
-      int ra = ES (a / 5, a % 5);
       double acc = 0.0;
-      for (int c = 0; c < 6 * np; c++)
-        acc += M[(size_t) ra * n + FT (c / 6, c % 6)] * ft[c];
+      for (int c = 0; c < nk; c++) acc += M[(size_t) R[a] * n + K[c]] * ft[c];
       b[a] = -acc;
-      for (int c = 0; c < ne; c++)
-        A[(size_t) a * ne + c] = M[(size_t) ra * n + ES (c / 5, c % 5)];
+      for (int c = 0; c < nx; c++) A[(size_t) a * nx + c] = M[(size_t) R[a] * n + X[c]];
     }
-  for (int k = 0; k < ne; k++)
+  for (int k = 0; k < nx; k++)
     {
       int p = k;
-      for (int r = k + 1; r < ne; r++)
-        if (fabs (A[(size_t) r * ne + k]) > fabs (A[(size_t) p * ne + k])) p = r;
+      for (int r = k + 1; r < nx; r++)
+        if (fabs (A[(size_t) r * nx + k]) > fabs (A[(size_t) p * nx + k])) p = r;
       if (p != k)
         {
-          for (int c = 0; c < ne; c++)
+          for (int c = 0; c < nx; c++)
             {
-              double t = A[(size_t) k * ne + c];
-              A[(size_t) k * ne + c] = A[(size_t) p * ne + c];
-              A[(size_t) p * ne + c] = t;
+              double t = A[(size_t) k * nx + c];
+              A[(size_t) k * nx + c] = A[(size_t) p * nx + c];
+              A[(size_t) p * nx + c] = t;
             }
           double t = b[k]; b[k] = b[p]; b[p] = t;
         }
-      for (int r = k + 1; r < ne; r++)
+      for (int r = k + 1; r < nx; r++)
         {
-          double f = A[(size_t) r * ne + k] / A[(size_t) k * ne + k];
-          for (int c = k; c < ne; c++) A[(size_t) r * ne + c] -= f * A[(size_t) k * ne + c];
+          double f = A[(size_t) r * nx + k] / A[(size_t) k * nx + k];
+          for (int c = k; c < nx; c++) A[(size_t) r * nx + c] -= f * A[(size_t) k * nx + c];
           b[r] -= f * b[k];
         }
     }
-  for (int k = ne - 1; k >= 0; k--)
+  for (int k = nx - 1; k >= 0; k--)
     {
       double acc = b[k];
-      for (int c = k + 1; c < ne; c++) acc -= A[(size_t) k * ne + c] * S[c];
-      S[k] = acc / A[(size_t) k * ne + k];
+      for (int c = k + 1; c < nx; c++) acc -= A[(size_t) k * nx + c] * S[c];
+      S[k] = acc / A[(size_t) k * nx + k];
     }
-  for (int r = 0; r < 6 * np; r++)
+  for (int r = 0; r < nk; r++)
     {
-      int rr = FT (r / 6, r % 6);
       double acc = 0.0;
-      for (int c = 0; c < 6 * np; c++) acc += M[(size_t) rr * n + FT (c / 6, c % 6)] * ft[c];
-      for (int c = 0; c < ne; c++) acc += M[(size_t) rr * n + ES (c / 5, c % 5)] * S[c];
+      for (int c = 0; c < nk; c++) acc += M[(size_t) K[r] * n + K[c]] * ft[c];
+      for (int c = 0; c < nx; c++) acc += M[(size_t) K[r] * n + X[c]] * S[c];
       uo[r] = acc;
     }
-#undef ES
-#undef FT
-  free (M); free (A); free (b); free (S);
+  free (K); free (X); free (M); free (A); free (b); free (S);
+}
+
+void
+orc_fts_mob_ft (const orc_sys *s, const double *ft, double *uo)
+{
+  orc_fts_mix_ft (s, s->np, ft, uo);
 }
 
 /* the mobility of BD_atimes_mob_FU (brownian.c:494-739), every particle
@@ -1153,6 +1173,27 @@ orc_bd_op_cb (int n, const double *x, double *y, void *ud)
   const orc_sys *s = (const orc_sys *) ud;
   if (s->version == 2) orc_fts_mob_ft (s, x, y);
   else orc_atimes_3all (s, x, y);
+}
+
+/* the same with the particles nm..np-1 fixed: F / FT zero-pad the forces
+ * of the fixed particles and keep the mobile rows (brownian.c:521-547,
+ * 562-588), FTS the mixed problem with those particles at rest (650-729) */
+typedef struct { const orc_sys *s; int nm; } orc_bd_ctx;
+
+static void
+orc_bd_op_nm_cb (int n, const double *x, double *y, void *ud)
+{
+  const orc_bd_ctx *c = (const orc_bd_ctx *) ud;
+  const orc_sys *s = c->s;
+  if (c->nm == s->np) { orc_bd_op_cb (n, x, y, (void *) s); return; }
+  if (s->version == 2) { orc_fts_mix_ft (s, c->nm, x, y); return; }
+  const int d = s->version == 0 ? 3 : 6, nf = d * s->np, nmd = d * c->nm;
+  double *f = calloc ((size_t) nf, sizeof (double));
+  double *u = malloc (sizeof (double) * nf);
+  for (int i = 0; i < nmd; i++) f[i] = x[i];
+  orc_atimes_3all (s, f, u);
+  for (int i = 0; i < nmd; i++) y[i] = u[i];
+  free (f); free (u);
 }
 
 /* One BD step, every particle mobile, no imposed flow, no interaction
@@ -1175,20 +1216,29 @@ orc_bd_op_cb (int n, const double *x, double *y, void *ud)
  *         brownian.c:2770-2816), x' = x + dt u
  * with the overlap checks after x1 (stage 1) and x' (stage 2).
  *
- * draws[ndraw][n] (n = 3 np or 6 np) are the Gaussian vectors BD_calc_FB
+ * Particles nm..np-1 are fixed (BD_atimes_mob_FU's zero padding for F /
+ * FT, the mixed problem for FTS; the overlap check over the mobile ones).
+ * draws[ndraw][n] (n = 3 nm or 6 nm) are the Gaussian vectors BD_calc_FB
  * consumes, one per (re)start of the noise.  fix_fact != 0 recomputes fact
  * after a redraw resets dt (the reference keeps the halved-dt value).  x
- * (and q [4 np] for FT / FTS when not NULL) are updated in place on
+ * [3 nm] (and q [4 nm] for FT / FTS when not NULL) are updated in place on
  * success; returns the dt taken, or -1 if the draws ran out. */
 double
-orc_bd_step (orc_sys *s, int scheme, double bb_n, double *x, double *q,
-             const double *F, const double *T, double peclet, double dt,
-             double dt_lim, double rmin_ol, int ncheb, const double *acoef,
-             double e0, double e1, int ndraw, const double *draws,
-             int fix_fact, int *used, int *halvings, double *err)
+orc_bd_step_nm (orc_sys *s, int nm, int scheme, double bb_n, double *x,
+                double *q, const double *F, const double *T, double peclet,
+                double dt, double dt_lim, double rmin_ol, int ncheb,
+                const double *acoef, double e0, double e1, int ndraw,
+                const double *draws, int fix_fact, int *used, int *halvings,
+                double *err)
 {
-  const int np = s->np, ver = s->version;
+  const int np = nm, ver = s->version;
   const int n = np * (ver == 0 ? 3 : 6), n3 = 3 * np;
+  orc_bd_ctx ctx = {s, nm};
+  /* positions of all s->np particles: the mobile ones from x or x_mid,
+   * the fixed ones as the system holds them (stokes_set_pos_mobile) */
+  double *pfull = malloc (sizeof (double) * 3 * (size_t) s->np);
+  for (int i = 3 * nm; i < 3 * s->np; i++) pfull[i] = s->pos[i];
+#define SETPOS(src) do { for (int i_ = 0; i_ < n3; i_++) pfull[i_] = (src)[i_]; s->pos = pfull; } while (0)
   double *z = malloc (sizeof (double) * n);
   double *f = malloc (sizeof (double) * n);
   double *u = malloc (sizeof (double) * n);
@@ -1210,17 +1260,17 @@ orc_bd_step (orc_sys *s, int scheme, double bb_n, double *x, double *q,
     }
 
 redo:                              /* REDO */
-  s->pos = x;
+  SETPOS (x);
 redo_fb:                           /* REDO_FB */
   if (k >= ndraw) goto done;
   orc_cheb_eval_atimes (ncheb, acoef, n, draws + (size_t) k * n, z, e0, e1,
-                        orc_bd_op_cb, (void *) s);
+                        orc_bd_op_nm_cb, (void *) &ctx);
   *err = orc_cheb_error_minvsqrt (n, draws + (size_t) k * n, z,
-                                  orc_bd_op_cb, (void *) s);
+                                  orc_bd_op_nm_cb, (void *) &ctx);
   k++;
 redo_scale:                        /* REDO_scale */
   for (int i = 0; i < n3; i++) xmid[i] = x[i];
-  s->pos = x;
+  SETPOS (x);
   if (scheme == 1)
     {
       FORCES (0, 1.0);
@@ -1229,17 +1279,17 @@ redo_scale:                        /* REDO_scale */
     {
       FORCES (1, 1.0);
     }
-  orc_bd_op_cb (n, f, u0, (void *) s);
+  orc_bd_op_nm_cb (n, f, u0, (void *) &ctx);
   {
     const double h = scheme == 0 ? dt_local * 0.5
       : (scheme == 1 ? dt_local / bb_n : dt_local);
     for (int i = 0; i < n3; i++) xmid[i] += h * u0[UIDX (i)];
   }
   stage = 1;
-  if (s->rmin == 0.0 && orc_check_overlap (s, xmid, rmin_ol) > 0)
+  if (s->rmin == 0.0 && orc_check_overlap_n (s, xmid, nm, rmin_ol) > 0)
     goto overlap;
-  s->pos = xmid;
-  orc_bd_op_cb (n, f, u, (void *) s);
+  SETPOS (xmid);
+  orc_bd_op_nm_cb (n, f, u, (void *) &ctx);
   if (scheme == 2)
     for (int i = 0; i < n; i++) u[i] = 0.5 * (u0[i] + u[i]);
   else if (scheme == 1)
@@ -1247,12 +1297,12 @@ redo_scale:                        /* REDO_scale */
       const double bbf = 0.5 * bb_n;
       for (int i = 0; i < n; i++) u0[i] = u0[i] + bbf * (u[i] - u0[i]);
       FORCES (1, 0.0);
-      orc_bd_op_cb (n, f, u, (void *) s);     /* still at xmid */
+      orc_bd_op_nm_cb (n, f, u, (void *) &ctx);   /* still at xmid */
       for (int i = 0; i < n; i++) u[i] += u0[i];
     }
   for (int i = 0; i < n3; i++) xmid[i] = x[i] + dt_local * u[UIDX (i)];
   stage = 2;
-  if (s->rmin == 0.0 && orc_check_overlap (s, xmid, rmin_ol) > 0)
+  if (s->rmin == 0.0 && orc_check_overlap_n (s, xmid, nm, rmin_ol) > 0)
     goto overlap;
   for (int i = 0; i < n3; i++) x[i] = xmid[i];
   if (ver > 0 && q)
@@ -1279,11 +1329,26 @@ overlap:
 done:
 #undef UIDX
 #undef FORCES
+#undef SETPOS
   s->pos = pos0;
+  free (pfull);
   *used = k;
   *halvings = nh;
   free (z); free (f); free (u); free (u0); free (xmid);
   return ret;
+}
+
+/* every particle mobile */
+double
+orc_bd_step (orc_sys *s, int scheme, double bb_n, double *x, double *q,
+             const double *F, const double *T, double peclet, double dt,
+             double dt_lim, double rmin_ol, int ncheb, const double *acoef,
+             double e0, double e1, int ndraw, const double *draws,
+             int fix_fact, int *used, int *halvings, double *err)
+{
+  return orc_bd_step_nm (s, s->np, scheme, bb_n, x, q, F, T, peclet, dt,
+                         dt_lim, rmin_ol, ncheb, acoef, e0, e1, ndraw, draws,
+                         fix_fact, used, halvings, err);
 }
 
 /* the mid-point step (orc_bd_step, scheme 0) */

@@ -126,6 +126,19 @@ void orc_quat_dqdt (const double *Q, const double *O, double *dQdt);
 void orc_fts_mob_ft (const orc_sys *s, const double *ft, double *uo);
 /* the BD mobility of the system's version (BD_atimes_mob_FU) */
 void orc_bd_op_cb (int n, const double *x, double *y, void *ud);
+/* overlap count over the first n particles */
+int orc_check_overlap_n (const orc_sys *s, const double *pos, int n,
+                         double rmin);
+/* FTS mobility problem with E = 0, (F, T) -> (U, O) of the nm mobile
+ * particles, particles nm..np-1 at rest */
+void orc_fts_mix_ft (const orc_sys *s, int nm, const double *ft, double *uo);
+/* orc_bd_step with particles nm..np-1 fixed */
+double orc_bd_step_nm (orc_sys *s, int nm, int scheme, double bb_n, double *x,
+                       double *q, const double *F, const double *T,
+                       double peclet, double dt, double dt_lim, double rmin_ol,
+                       int ncheb, const double *acoef, double e0, double e1,
+                       int ndraw, const double *draws, int fix_fact, int *used,
+                       int *halvings, double *err);
 /* one BD step of scheme 0 (mid-point), 1 (BB03, step parameter bb_n) or 2
  * (BM97), F / FT / FTS (brownian.c:2352-3200); see oracle.c */
 double orc_bd_step (orc_sys *s, int scheme, double bb_n, double *x, double *q,

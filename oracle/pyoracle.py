@@ -71,6 +71,7 @@ def lib():
         L.orc_bd_mid_F.restype = C.c_double
         L.orc_bd_mid.restype = C.c_double
         L.orc_bd_step.restype = C.c_double
+        L.orc_bd_step_nm.restype = C.c_double
         L.orc_check_overlap.restype = C.c_int
         L.orc_xi_by_tratio.argtypes = [C.POINTER(OrcSys), C.c_double]
         L.orc_set_xi.argtypes = [C.POINTER(OrcSys), C.c_double, C.c_double]
@@ -214,12 +215,14 @@ class System:
 
     def bd_mid(self, x, draws, peclet, dt, ncheb, acoef, e0, e1, q=None,
                F=None, T=None, dt_lim=1e-6, rmin_ol=1.0, fix_fact=False,
-               scheme=0, bb_n=1.0):
+               scheme=0, bb_n=1.0, nm=None):
         """one BD step (scheme 0 mid-point brownian.c:2352-2560, 1 BB03,
         2 BM97) of the system's version (F, FT, FTS); draws[k] has 3 np (F)
         or 6 np values; returns (x_new, q_new, dt_taken, draws_used,
         halvings, err_cheb)"""
         x = np.array(x, dtype=np.float64).reshape(-1)
+        nm = self.np if nm is None else int(nm)
+        x = x[:3 * nm].copy()
         nop = x.size * (1 if self.version == 0 else 2)
         draws = np.ascontiguousarray(draws, np.float64).reshape(-1)
         nd = draws.size // nop
@@ -231,8 +234,9 @@ class System:
             return None if v is None else _p(np.ascontiguousarray(v, np.float64))
         Fa = None if F is None else np.ascontiguousarray(F, np.float64)
         Ta = None if T is None else np.ascontiguousarray(T, np.float64)
-        dtt = lib().orc_bd_step(
-            C.byref(self.s), C.c_int(scheme), C.c_double(bb_n), _p(x),
+        dtt = lib().orc_bd_step_nm(
+            C.byref(self.s), C.c_int(nm), C.c_int(scheme), C.c_double(bb_n),
+            _p(x),
             None if qq is None else _p(qq), opt(Fa),
             opt(Ta), C.c_double(peclet), C.c_double(dt), C.c_double(dt_lim),
             C.c_double(rmin_ol), C.c_int(ncheb), _p(a), C.c_double(e0),
@@ -240,6 +244,14 @@ class System:
             C.c_int(1 if fix_fact else 0), C.byref(used), C.byref(nh),
             C.byref(err))
         return x, qq, dtt, used.value, nh.value, err.value
+
+    def fts_mix_ft(self, nm, ft):
+        """(F, T) -> (U, O) of the nm mobile particles, the rest at rest,
+        FTS (BD_atimes_mob_FU with fixed particles)"""
+        ft = np.ascontiguousarray(ft, np.float64).reshape(-1)
+        uo = np.zeros_like(ft)
+        lib().orc_fts_mix_ft(C.byref(self.s), C.c_int(nm), _p(ft), _p(uo))
+        return uo
 
     def bd_op(self, x):
         """the BD mobility of the system's version (BD_atimes_mob_FU): F ->

@@ -477,3 +477,55 @@ def test_bd_schemes_match_oracle(version, n, scheme, bb_n):
         _say(msg)
         x, q = xo, qo
     bd.close()
+
+
+@pytest.mark.parametrize("version,n,nm", [(0, 27, 20), (1, 27, 20), (2, 12, 8)])
+def test_bd_fixed_particles_match_oracle(version, n, nm):
+    """mid-point steps with particles nm..n-1 fixed (BD_atimes_mob_FU's
+    zero padding for F / FT, the mixed problem for FTS; overlap check over
+    the mobile particles) against the oracle's step fed with the device's
+    draw; the fixed particles do not move"""
+    phi = 0.1
+    _say(f"bd fixed particles version={version} N={n} nm={nm}")
+    pos, L = workloads.configuration(n, phi, seed=9)
+    s = ls.Stokes(version, pos, box=(L, L, L), tr=1.0, nm=nm)
+    o = po.System(version, pos, box=(L, L, L), tr=1.0)
+    d = 3 if version == 0 else 6
+    if version < 2:
+        G = o.matrix()[:d * nm, :d * nm]
+    else:
+        G = np.stack([o.fts_mix_ft(nm, np.eye(6 * nm)[c])
+                      for c in range(6 * nm)], axis=1)
+    w = np.linalg.eigvalsh(0.5 * (G + G.T))
+    e0, e1 = 0.95 * w[0], 1.05 * w[-1]
+    nch, pe, dt = 30, 1.0, 1e-3
+    a = po.cheb_coef(nch, 0, e0, e1)
+    rng = np.random.default_rng(21 + version)
+    F = rng.standard_normal(3 * nm)
+    T = rng.standard_normal(3 * nm) if version > 0 else None
+    bd = ls.BD(s, peclet=pe, n_cheb=nch, eps=1e-6, dt_lim=1e-6, rmin_ol=1.0,
+               seed=3, force_dev=ls.DeviceArray.from_host(F), bounds=(e0, e1))
+    if version > 0:
+        bd.set_torque(ls.DeviceArray.from_host(T))
+    if version == 2:
+        s.set_iter(eps=1e-13)
+        bd.set_fts_iter(30, 2000, 1e-13)
+    x = pos.reshape(-1)[:3 * nm].copy()
+    xd = ls.DeviceArray.from_host(x)
+    for step in range(2):
+        info = bd.step(xd, dt)
+        assert info.attempts == 1
+        y, _ = bd.last_noise()
+        assert y.size == d * nm
+        full = pos.reshape(-1).copy()
+        full[:3 * nm] = x
+        o.pos[:] = full
+        xo, _, dto, used, nh, err = o.bd_mid(full, y[None], pe, dt, nch, a, e0,
+                                             e1, F=F, T=T, nm=nm)
+        assert used == 1 and nh == 0 and dto == info.dt
+        disp = np.abs(xo - x).max()
+        ex = np.abs(xd.to_host() - xo).max() / disp
+        _say(f"  step {step}: x err / disp {ex:.2e}")
+        assert ex < (1e-8 if version == 2 else 1e-10)
+        x = xo
+    bd.close()

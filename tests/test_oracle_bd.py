@@ -388,3 +388,69 @@ def test_bd_bb03_bm97_vs_reference_composition(scheme, bb_n):
     R.ref_sys_free(h1)
     want = x0 + dt * u
     assert np.abs(xn - want).max() < 1e-12 * np.abs(want - x0).max()
+
+
+def test_fts_mobility_with_fixed_particles():
+    """BD_atimes_mob_FU with particles nm..np-1 fixed (brownian.c:650-729):
+    the oracle's (F, T) -> (U, O) equals the restated dense mixed solve
+    (ewald-3fts-matrix.c:1192-1290) with the fixed particles at rest"""
+    L = 8.0
+    s = po.System(2, _eight(L), box=(L, L, L), tr=1.0)
+    nm = 5
+    rng = np.random.default_rng(8)
+    ft = rng.standard_normal(6 * nm)
+    uo = s.fts_mix_ft(nm, ft)
+    b = np.zeros(88)
+    b.reshape(8, 11)[:nm, :6] = ft.reshape(nm, 6)
+    x = po.solve_mix_dense(s, nm, b)
+    want = x.reshape(8, 11)[:nm, :6].reshape(-1)
+    assert np.abs(uo - want).max() < 1e-12 * np.abs(want).max()
+
+
+@pytest.mark.parametrize("version", [0, 1])
+def test_bd_step_fixed_particles_composition(version):
+    """a mid-point step with particles 5..7 fixed: the noise and drift use
+    the zero-padded product of the compiled reference's atimes_3all on the
+    mobile rows (brownian.c:521-547, 562-588); the fixed particles stay"""
+    import ctypes as C
+    L = 8.0
+    pos = _eight(L)
+    s = po.System(version, pos, box=(L, L, L), tr=1.0)
+    nm, d = 5, (3 if version == 0 else 6)
+    n = d * nm
+    rng = np.random.default_rng(40 + version)
+    y = rng.standard_normal(n)
+    F = rng.standard_normal(3 * nm)
+    T = rng.standard_normal(3 * nm) if version else None
+    pe, dt, nch = 1.0, 1e-3, 30
+    G = s.matrix()[:n, :n]
+    w = np.linalg.eigvalsh(0.5 * (G + G.T))
+    e0, e1 = 0.95 * w[0], 1.05 * w[-1]
+    a = po.cheb_coef(nch, 0, e0, e1)
+    xn, _, dtt, used, nh, err = s.bd_mid(pos, y[None], pe, dt, nch, a, e0, e1,
+                                         F=F, T=T, nm=nm)
+    assert used == 1 and nh == 0 and dtt == dt and xn.size == 3 * nm
+    R = po.ref()
+
+    def ref_op(p, f):
+        t = po.System(version, p, box=(L, L, L), tr=1.0)
+        h = R.ref_sys_new(C.byref(t.s))
+        full = np.zeros(d * 8)
+        full[:n] = f
+        u = np.zeros(d * 8)
+        R.ref_atimes(h, d * 8, po._p(full), po._p(u))
+        R.ref_sys_free(h)
+        return u[:n]
+    z = po.cheb_eval_dense(a, G, y, e0, e1)
+    fact = math.sqrt(2 / (pe * dt))
+    f = F + fact * z if version == 0 else np.concatenate(
+        [F.reshape(nm, 3), T.reshape(nm, 3)], axis=1).reshape(-1) + fact * z
+    x0 = pos.reshape(-1)
+    u0 = ref_op(pos, f)
+    U0 = u0 if version == 0 else u0.reshape(nm, 6)[:, :3].reshape(-1)
+    pm = pos.copy()
+    pm[:nm] = (x0[:3 * nm] + 0.5 * dt * U0).reshape(nm, 3)
+    u1 = ref_op(pm, f)
+    U1 = u1 if version == 0 else u1.reshape(nm, 6)[:, :3].reshape(-1)
+    want = x0[:3 * nm] + dt * U1
+    assert np.abs(xn - want).max() < 1e-12 * np.abs(want - x0[:3 * nm]).max()
