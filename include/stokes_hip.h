@@ -252,7 +252,7 @@ int stokes_hip_bd_set_real_cache (stokes_hip_bd *bd, int enable);
 /* sys: version 0 (F), 1 (FT) or 2 (FTS), equal radii (a == NULL), no
  * shear, set up as for atimes_3all; particles nm..np-1 (stokes_set_np) are
  * fixed: the step moves x [3 nm] (and q [4 nm]), the fixed positions are
- * read from sys->pos at each step.  F_dev: constant force [3 np] (BD->F)
+ * read from sys->pos at each step.  F_dev: constant force [3 nm] (BD->F)
  * or NULL; it is read on
  * every step and must stay allocated until stokes_hip_bd_destroy.  peclet:
  * fact = sqrt(2 / (Pe dt)), 0 when Pe <= 0 (brownian.c:2177-2187).
@@ -287,7 +287,7 @@ int stokes_hip_bd_step_q (stokes_hip_bd *bd, double *x_dev, double *q_dev,
  * evaluated at the intermediate configuration, as the reference does,
  * 2770-2816), 2 Ball-Melrose 1997 (BD_evolve_BM97, 2970-3190) */
 int stokes_hip_bd_set_scheme (stokes_hip_bd *bd, int scheme, double bb_n);
-/* FT / FTS: constant torques [3 np] (BD->T, brownian.c:2296-2299) or NULL,
+/* FT / FTS: constant torques [3 nm] (BD->T, brownian.c:2296-2299) or NULL,
  * read on every step */
 int stokes_hip_bd_set_torque (stokes_hip_bd *bd, const double *T_dev);
 /* FTS: GMRES parameters of the noise products and of the Lanczos bounds
