@@ -249,10 +249,12 @@ typedef struct stokes_hip_bd_info {
  * (its products are matrix-free). */
 int stokes_hip_bd_set_real_cache (stokes_hip_bd *bd, int enable);
 
-/* sys: version 0 (F), 1 (FT) or 2 (FTS), equal radii (a == NULL), no
- * shear, set up as for atimes_3all; particles nm..np-1 (stokes_set_np) are
- * fixed: the step moves x [3 nm] (and q [4 nm]), the fixed positions are
- * read from sys->pos at each step.  F_dev: constant force [3 nm] (BD->F)
+/* sys: version 0 (F), 1 (FT) or 2 (FTS), no shear, set up as for
+ * atimes_3all; particles nm..np-1 (stokes_set_np) are fixed: the step
+ * moves x [3 nm] (and q [4 nm]), the fixed positions are read from sys->pos
+ * at each step.  Unequal radii (sys->a, read at creation): the noise
+ * operator carries BD_atimes_mob_FU's Global scaling (U / a, O / a^3,
+ * brownian.c:430-480) and the overlap test a_i + a_j.  F_dev: constant force [3 nm] (BD->F)
  * or NULL; it is read on
  * every step and must stay allocated until stokes_hip_bd_destroy.  peclet:
  * fact = sqrt(2 / (Pe dt)), 0 when Pe <= 0 (brownian.c:2177-2187).

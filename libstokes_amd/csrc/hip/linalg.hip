@@ -86,6 +86,13 @@ __global__ void k_ft_fts (const double *__restrict__ x, int np, int mode,
   else if (q < 6) y[i * 6 + q] = x[t];
 }
 
+// y[k] *= s[k]
+__global__ void k_dmul (double *__restrict__ y, const double *__restrict__ sv, size_t n)
+{
+  const size_t k = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) y[k] *= sv[k];
+}
+
 }  // namespace
 
 #define CHK(call)                                                           \
@@ -158,6 +165,15 @@ extern "C" int shk_dev_ft_fts (shk_dev *d, const double *x, int np, int mode, do
   const size_t n = (size_t) np * 11;
   hipLaunchKernelGGL (k_ft_fts, dim3 ((unsigned) ((n + 255) / 256)), dim3 (256), 0, s, x, np,
                       mode, y);
+  CHK (hipGetLastError ());
+  return SHK_OK;
+}
+
+extern "C" int shk_dev_dmul (shk_dev *d, double *y, const double *sv, size_t n)
+{
+  if (n == 0) return SHK_OK;
+  hipStream_t s = (hipStream_t) shk_dev_stream (d);
+  hipLaunchKernelGGL (k_dmul, dim3 ((unsigned) ((n + 255) / 256)), dim3 (256), 0, s, y, sv, n);
   CHK (hipGetLastError ());
   return SHK_OK;
 }

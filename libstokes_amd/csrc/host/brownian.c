@@ -46,6 +46,10 @@ struct stokes_hip_bd {
   double *posfull;             /* device [3 np]: mobile from x, fixed from
                                   sys->pos (nm < np only) */
   double *fpad, *upad;         /* device [D np]: zero-padded F / FT products */
+  double *sv, *sqsv;           /* device [n]: the Global scaling of the noise
+                                  operator (1/a, 1/a^3) and its square root;
+                                  NULL for equal radii */
+  double *a_dev;               /* device [np] radii, NULL for equal radii */
   double peclet, eps, dt_lim, rmin_ol;
   int ncheb;
   unsigned long long seed;
@@ -138,11 +142,9 @@ stokes_hip_bd_create (struct stokes *sys, const double *F_dev, double peclet,
       stokes_hip_set_error ("bd_create: version %d", sys->version);
       return STOKES_HIP_EARG;
     }
-  if (sys->a != NULL || sys->shear_mode != 0 || sys->nm < 1
-      || sys->nm > sys->np)
+  if (sys->shear_mode != 0 || sys->nm < 1 || sys->nm > sys->np)
     {
-      stokes_hip_set_error ("bd_create: needs equal radii, no shear and"
-                            " 1 <= nm <= np");
+      stokes_hip_set_error ("bd_create: needs no shear and 1 <= nm <= np");
       return STOKES_HIP_EUNSUP;
     }
   if (n_cheb < 2 || !(eps > 0.0) || dt_lim < 0.0 || sys->np <= 0)
@@ -183,7 +185,8 @@ stokes_hip_bd_create (struct stokes *sys, const double *F_dev, double peclet,
   bd->acoef = calloc ((size_t) n_cheb, sizeof (double));
   const size_t n = bd->n, np = (size_t) sys->np;
   const size_t dfull = (size_t) sys->np * (sys->version == 0 ? 3 : 6);
-  const size_t nbuf = 11 * n + 10 * np + 3 * np + 2 * dfull;
+  const size_t nbuf = 11 * n + 10 * np + 3 * np + 2 * dfull
+    + (sys->a ? 2 * n + np : 0);
   int rc = bd->acoef ? 0 : SHK_ENOMEM;
   if (!rc) rc = shk_dev_malloc (c->dev, (void **) &bd->buf, sizeof (double) * nbuf);
   if (!rc && bd->ver == 2)
@@ -211,6 +214,12 @@ stokes_hip_bd_create (struct stokes *sys, const double *F_dev, double peclet,
   bd->posfull = bd->f2 + n;
   bd->fpad = bd->posfull + 3 * np;
   bd->upad = bd->fpad + dfull;
+  if (sys->a)
+    {
+      bd->sv = bd->upad + dfull;
+      bd->sqsv = bd->sv + n;
+      bd->a_dev = bd->sqsv + n;
+    }
   bd->bb_n = 1.0;
   if (bd->fts_buf)
     {
@@ -230,6 +239,27 @@ stokes_hip_bd_create (struct stokes *sys, const double *F_dev, double peclet,
   if (!rc) rc = shk_dev_scal (c->dev, bd->buf, NULL, 0.0, nbuf);
   if (!rc && bd->fts_buf)
     rc = shk_dev_scal (c->dev, bd->fts_buf, NULL, 0.0, 22 * np);
+  if (!rc && sys->a)
+    {
+      /* BD_scale_P_to_G_uo (brownian.c:430-480): U / a, O / a^3 */
+      const int dd = sys->version == 0 ? 3 : 6;
+      double *h = malloc (sizeof (double) * (2 * n + np));
+      if (!h) rc = SHK_ENOMEM;
+      else
+        {
+          for (int i = 0; i < sys->nm; i++)
+            for (int q = 0; q < dd; q++)
+              {
+                const double a = sys->a[i];
+                const double v = q < 3 ? 1.0 / a : 1.0 / (a * a * a);
+                h[dd * i + q] = v;
+                h[n + dd * i + q] = sqrt (v);
+              }
+          for (size_t i = 0; i < np; i++) h[2 * n + i] = sys->a[i];
+          rc = shk_dev_h2d (c->dev, bd->sv, h, sizeof (double) * (2 * n + np));
+          free (h);
+        }
+    }
   if (!rc) rc = shk_dev_sync (c->dev);
   if (rc)
     {
@@ -356,9 +386,23 @@ stokes_hip_bd_last_noise (stokes_hip_bd *bd, double *y_dev, double *z_dev)
  * (U, O) by GMRES (BD_atimes_mob_FU, brownian.c:597-634), with the noise
  * parameters (noise != 0) or those of stokes_set_iter (the step's solves,
  * solve_mix_3all) */
+static int bd_mob (stokes_hip_bd *bd, shk_dev *d, int prec, const double *x,
+                   double *y, int noise);
+
+/* scale != 0 applies the Global scaling of BD_atimes_mob_FU (unequal radii,
+ * brownian.c:736): the noise operator; the drift products do not carry it */
 static int
 bd_op (stokes_hip_bd *bd, shk_dev *d, int prec, const double *x, double *y,
-       int noise)
+       int noise, int scale)
+{
+  int rc = bd_mob (bd, d, prec, x, y, noise);
+  if (!rc && scale && bd->sv) rc = shk_dev_dmul (d, y, bd->sv, bd->n);
+  return rc;
+}
+
+static int
+bd_mob (stokes_hip_bd *bd, shk_dev *d, int prec, const double *x, double *y,
+        int noise)
 {
   if (bd->ver != 2)
     {
@@ -409,12 +453,12 @@ bd_cheb (stokes_hip_bd *bd, shk_dev *d, int prec, const double *y, double *z)
   if (!rc) rc = shk_dev_scal (d, b1, y, a[ncheb - 1], len);
   for (int j = ncheb - 2; j >= 1 && !rc; j--)
     {
-      rc = bd_op (bd, d, prec, b1, x, 1);
+      rc = bd_op (bd, d, prec, b1, x, 1, 1);
       if (!rc) rc = shk_dev_axpby4 (d, b2, x, b1, b0, y, 2.0 * da, 2.0 * db,
                                     a[j], len);
       double *t = b0; b0 = b1; b1 = b2; b2 = t;
     }
-  if (!rc) rc = bd_op (bd, d, prec, b1, x, 1);
+  if (!rc) rc = bd_op (bd, d, prec, b1, x, 1, 1);
   if (!rc) rc = shk_dev_axpby4 (d, z, x, b1, b0, y, da, db, a[0], len);
   return rc;
 }
@@ -435,7 +479,7 @@ bd_record (stokes_hip_bd *bd, shk_dev *d, int prec, int check_ol, double *x,
   /* the products at x share one dense real-space matrix */
   if (!rc && bd->real_cache) rc = shk_dev_real_cache (d, 1, prec);
   if (!rc) rc = bd_cheb (bd, d, prec, bd->y, bd->z);
-  if (!rc) rc = bd_op (bd, d, prec, bd->z, bd->w, 1);
+  if (!rc) rc = bd_op (bd, d, prec, bd->z, bd->w, 1, 1);
   if (!rc) rc = shk_dev_bd_dots (d, bd->y, bd->z, bd->w, n, bd->st_dev);
   /* stage 1: drift at x (BB03: the Brownian force alone), Euler step to
    * the intermediate configuration: dt / 2 (mid-point), dt / n (BB03), dt
@@ -448,7 +492,7 @@ bd_record (stokes_hip_bd *bd, shk_dev *d, int prec, int check_ol, double *x,
     rc = bd->ver == 0
       ? shk_dev_bd_force (d, bd->f, Fx, bd->z, bd->sc_dev, n)
       : shk_dev_bd_force_ft (d, bd->f, Fx, Tx, bd->z, bd->sc_dev, bd->nm);
-  if (!rc) rc = bd_op (bd, d, prec, bd->f, bd->u0, 0);
+  if (!rc) rc = bd_op (bd, d, prec, bd->f, bd->u0, 0, 0);
   if (!rc)
     rc = bd->ver == 0
       ? shk_dev_bd_euler (d, bd->xm, x, bd->u0, bd->sc_dev, h1, n3)
@@ -456,7 +500,7 @@ bd_record (stokes_hip_bd *bd, shk_dev *d, int prec, int check_ol, double *x,
                              bd->nm);
   /* the mobile particles only (check_overlap, brownian.c:305-310) */
   if (!rc && check_ol)
-    rc = shk_dev_bd_overlap (d, bd->xm, NULL, bd->nm, s->periodic, s->lx, s->ly,
+    rc = shk_dev_bd_overlap (d, bd->xm, bd->a_dev, bd->nm, s->periodic, s->lx, s->ly,
                              s->lz, 0, 0.0, bd->rmin_ol, &bd->st_dev->overlap[0]);
   /* stage 2: drift at the intermediate configuration with the same forces;
    * mid-point: u = M(x1) f; BM97: u = (u0 + M(x1) f) / 2; BB03: u =
@@ -466,7 +510,7 @@ bd_record (stokes_hip_bd *bd, shk_dev *d, int prec, int check_ol, double *x,
   if (!rc && bd->real_cache) rc = shk_dev_real_cache (d, 0, prec);
   const double *pm = rc ? NULL : bd_pos (bd, d, bd->xm, &rc);
   if (!rc) rc = shk_dev_set_pos (d, pm, 1);
-  if (!rc) rc = bd_op (bd, d, prec, bd->f, bd->u, 0);
+  if (!rc) rc = bd_op (bd, d, prec, bd->f, bd->u, 0, 0);
   if (!rc && sch == 2) rc = shk_dev_axpby (d, bd->u, bd->u0, 0.5, 0.5, n);
   if (!rc && sch == 1)
     {
@@ -475,7 +519,7 @@ bd_record (stokes_hip_bd *bd, shk_dev *d, int prec, int check_ol, double *x,
         rc = bd->ver == 0
           ? shk_dev_bd_force (d, bd->f2, bd->F, NULL, bd->sc_dev, n)
           : shk_dev_bd_force_ft (d, bd->f2, bd->F, bd->T, NULL, bd->sc_dev, bd->nm);
-      if (!rc) rc = bd_op (bd, d, prec, bd->f2, bd->u, 0);
+      if (!rc) rc = bd_op (bd, d, prec, bd->f2, bd->u, 0, 0);
       if (!rc) rc = shk_dev_axpby (d, bd->u, bd->u0, 1.0, 1.0, n);
     }
   if (!rc)
@@ -484,7 +528,7 @@ bd_record (stokes_hip_bd *bd, shk_dev *d, int prec, int check_ol, double *x,
       : shk_dev_bd_euler_ft (d, bd->xn, x, q ? bd->qn : NULL, q, bd->u,
                              bd->sc_dev, 1.0, bd->nm);
   if (!rc && check_ol)
-    rc = shk_dev_bd_overlap (d, bd->xn, NULL, bd->nm, s->periodic, s->lx, s->ly,
+    rc = shk_dev_bd_overlap (d, bd->xn, bd->a_dev, bd->nm, s->periodic, s->lx, s->ly,
                              s->lz, 0, 0.0, bd->rmin_ol, &bd->st_dev->overlap[1]);
   if (!rc) rc = shk_dev_counter_add (d, bd->counter, 1);
   if (!rc) rc = shk_dev_d2h_async (d, bd->st_host, bd->st_dev, sizeof (shk_bd_status));
@@ -504,11 +548,11 @@ bd_capture (stokes_hip_bd *bd, stokes_ctx *c, double *x, double *q)
   int rc = 0;
   const double *px = bd_pos (bd, d, x, &rc);
   if (!rc) rc = shk_dev_set_pos (d, px, 1);
-  if (!rc) rc = bd_op (bd, d, c->prec, bd->z, bd->w, 1);
+  if (!rc) rc = bd_op (bd, d, c->prec, bd->z, bd->w, 1, 1);
   if (!rc && bd->real_cache)
     {
       rc = shk_dev_real_cache (d, 1, c->prec);
-      if (!rc) rc = bd_op (bd, d, c->prec, bd->z, bd->w, 1);
+      if (!rc) rc = bd_op (bd, d, c->prec, bd->z, bd->w, 1, 1);
       int rc2 = shk_dev_real_cache (d, 0, c->prec);
       if (!rc) rc = rc2;
     }
@@ -553,8 +597,12 @@ bd_host_op (void *ud, const double *x, double *y)
   shk_dev *d = bd->c->dev;
   const size_t bytes = sizeof (double) * bd->n;
   double *xd = bd->work, *yd = bd->work + bd->n;
+  /* unequal radii: the noise operator S M (S the Global scaling) has the
+   * spectrum of the symmetric S^1/2 M S^1/2, which Lanczos takes */
   int rc = shk_dev_h2d (d, xd, x, bytes);
-  if (!rc) rc = bd_op (bd, d, bd->c->prec, xd, yd, 1);
+  if (!rc && bd->sqsv) rc = shk_dev_dmul (d, xd, bd->sqsv, bd->n);
+  if (!rc) rc = bd_op (bd, d, bd->c->prec, xd, yd, 1, 0);
+  if (!rc && bd->sqsv) rc = shk_dev_dmul (d, yd, bd->sqsv, bd->n);
   if (!rc) rc = shk_dev_d2h (d, y, yd, bytes);
   return rc ? bd_fail (rc, "bd_fit_bounds") : 0;
 }

@@ -207,6 +207,8 @@ int shk_dev_bd_force_ft (shk_dev *d, double *f, const double *F, const double *T
 int shk_dev_bd_euler_ft (shk_dev *d, double *xo, const double *xi, double *qo,
                          const double *qi, const double *u,
                          const shk_bd_scalars *sc, double frac, int np);
+/* y[k] *= s[k] */
+int shk_dev_dmul (shk_dev *d, double *y, const double *s, size_t n);
 /* (F, T) of 6 per particle -> (F, T, 0) of 11 (mode 0), or the first 6 of
  * 11 back (mode 1) */
 int shk_dev_ft_fts (shk_dev *d, const double *x, int np, int mode, double *y);

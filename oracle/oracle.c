@@ -1180,20 +1180,47 @@ orc_bd_op_cb (int n, const double *x, double *y, void *ud)
  * 562-588), FTS the mixed problem with those particles at rest (650-729) */
 typedef struct { const orc_sys *s; int nm; } orc_bd_ctx;
 
+static void orc_bd_mob (int n, const double *x, double *y, void *ud,
+                        int scaled);
+
+/* the noise operator (BD_atimes_mob_FU, with its Global scaling) */
 static void
 orc_bd_op_nm_cb (int n, const double *x, double *y, void *ud)
 {
+  orc_bd_mob (n, x, y, ud, 1);
+}
+
+/* the drift product (solve_mix_3all: no scaling) */
+static void
+orc_bd_drift (int n, const double *x, double *y, void *ud)
+{
+  orc_bd_mob (n, x, y, ud, 0);
+}
+
+static void
+orc_bd_mob (int n, const double *x, double *y, void *ud, int scaled)
+{
   const orc_bd_ctx *c = (const orc_bd_ctx *) ud;
   const orc_sys *s = c->s;
-  if (c->nm == s->np) { orc_bd_op_cb (n, x, y, (void *) s); return; }
-  if (s->version == 2) { orc_fts_mix_ft (s, c->nm, x, y); return; }
-  const int d = s->version == 0 ? 3 : 6, nf = d * s->np, nmd = d * c->nm;
-  double *f = calloc ((size_t) nf, sizeof (double));
-  double *u = malloc (sizeof (double) * nf);
-  for (int i = 0; i < nmd; i++) f[i] = x[i];
-  orc_atimes_3all (s, f, u);
-  for (int i = 0; i < nmd; i++) y[i] = u[i];
-  free (f); free (u);
+  const int d = s->version == 0 ? 3 : 6;
+  if (c->nm == s->np) orc_bd_op_cb (n, x, y, (void *) s);
+  else if (s->version == 2) orc_fts_mix_ft (s, c->nm, x, y);
+  else
+    {
+      const int nf = d * s->np, nmd = d * c->nm;
+      double *f = calloc ((size_t) nf, sizeof (double));
+      double *u = malloc (sizeof (double) * nf);
+      for (int i = 0; i < nmd; i++) f[i] = x[i];
+      orc_atimes_3all (s, f, u);
+      for (int i = 0; i < nmd; i++) y[i] = u[i];
+      free (f); free (u);
+    }
+  /* Particle-wise to Global scaling (BD_scale_P_to_G_uo, brownian.c:430-480):
+   * U / a, O / a^3 */
+  if (scaled && s->a)
+    for (int i = 0; i < c->nm; i++)
+      for (int q = 0; q < d; q++)
+        y[d * i + q] /= q < 3 ? s->a[i] : s->a[i] * s->a[i] * s->a[i];
 }
 
 /* One BD step, every particle mobile, no imposed flow, no interaction
@@ -1218,6 +1245,8 @@ orc_bd_op_nm_cb (int n, const double *x, double *y, void *ud)
  *
  * Particles nm..np-1 are fixed (BD_atimes_mob_FU's zero padding for F /
  * FT, the mixed problem for FTS; the overlap check over the mobile ones).
+ * Unequal radii: the noise operator carries BD_atimes_mob_FU's Global
+ * scaling (U / a, O / a^3); the drift products do not.
  * draws[ndraw][n] (n = 3 nm or 6 nm) are the Gaussian vectors BD_calc_FB
  * consumes, one per (re)start of the noise.  fix_fact != 0 recomputes fact
  * after a redraw resets dt (the reference keeps the halved-dt value).  x
@@ -1279,7 +1308,7 @@ redo_scale:                        /* REDO_scale */
     {
       FORCES (1, 1.0);
     }
-  orc_bd_op_nm_cb (n, f, u0, (void *) &ctx);
+  orc_bd_drift (n, f, u0, (void *) &ctx);
   {
     const double h = scheme == 0 ? dt_local * 0.5
       : (scheme == 1 ? dt_local / bb_n : dt_local);
@@ -1289,7 +1318,7 @@ redo_scale:                        /* REDO_scale */
   if (s->rmin == 0.0 && orc_check_overlap_n (s, xmid, nm, rmin_ol) > 0)
     goto overlap;
   SETPOS (xmid);
-  orc_bd_op_nm_cb (n, f, u, (void *) &ctx);
+  orc_bd_drift (n, f, u, (void *) &ctx);
   if (scheme == 2)
     for (int i = 0; i < n; i++) u[i] = 0.5 * (u0[i] + u[i]);
   else if (scheme == 1)
@@ -1297,7 +1326,7 @@ redo_scale:                        /* REDO_scale */
       const double bbf = 0.5 * bb_n;
       for (int i = 0; i < n; i++) u0[i] = u0[i] + bbf * (u[i] - u0[i]);
       FORCES (1, 0.0);
-      orc_bd_op_nm_cb (n, f, u, (void *) &ctx);   /* still at xmid */
+      orc_bd_drift (n, f, u, (void *) &ctx);   /* still at xmid */
       for (int i = 0; i < n; i++) u[i] += u0[i];
     }
   for (int i = 0; i < n3; i++) xmid[i] = x[i] + dt_local * u[UIDX (i)];

@@ -529,3 +529,59 @@ def test_bd_fixed_particles_match_oracle(version, n, nm):
         assert ex < (1e-8 if version == 2 else 1e-10)
         x = xo
     bd.close()
+
+
+@pytest.mark.parametrize("version,n", [(0, 27), (1, 27), (2, 12)])
+def test_bd_unequal_radii_match_oracle(version, n):
+    """mid-point steps of spheres with radii log-uniform on [0.5, 2]: the
+    noise operator carries BD_atimes_mob_FU's Global scaling (U / a,
+    O / a^3, brownian.c:430-480, 736), the drift does not, the overlap
+    check uses a_i + a_j; against the oracle's step fed with the device's
+    draw.  F: the Lanczos bounds of the integrator enclose the spectrum of
+    the scaled operator"""
+    phi = 0.1
+    _say(f"bd unequal radii version={version} N={n}")
+    pos, a, L = workloads.configuration_poly(n, phi, seed=31)
+    s = ls.Stokes(version, pos, box=(L, L, L), tr=1.0, a=a)
+    o = po.System(version, pos, box=(L, L, L), tr=1.0, a=a)
+    d = 3 if version == 0 else 6
+    if version < 2:
+        G = o.matrix()
+    else:
+        G = po.ft_mobility_dense(o)
+    sv = np.array([1.0 / a[i] if q < 3 else a[i] ** -3.0
+                   for i in range(n) for q in range(d)])
+    Gs = np.sqrt(sv)[:, None] * G * np.sqrt(sv)[None, :]
+    w = np.linalg.eigvalsh(0.5 * (Gs + Gs.T))
+    nch, pe, dt = 30, 1.0, 1e-3
+    rng = np.random.default_rng(41 + version)
+    F = rng.standard_normal(3 * n)
+    T = rng.standard_normal(3 * n) if version > 0 else None
+    kw = {} if version == 0 else {"bounds": (0.95 * w[0], 1.05 * w[-1])}
+    bd = ls.BD(s, peclet=pe, n_cheb=nch, eps=1e-2, dt_lim=1e-6, rmin_ol=1.0,
+               seed=6, force_dev=ls.DeviceArray.from_host(F), **kw)
+    if version > 0:
+        bd.set_torque(ls.DeviceArray.from_host(T))
+    if version == 2:
+        s.set_iter(eps=1e-13)
+        bd.set_fts_iter(30, 2000, 1e-13)
+    x = pos.reshape(-1).copy()
+    xd = ls.DeviceArray.from_host(x)
+    for step in range(2):
+        info = bd.step(xd, dt)
+        e0, e1 = info.eig
+        if version == 0 and step == 0:
+            _say(f"  bounds [{e0:.4g}, {e1:.4g}], spectrum [{w[0]:.4g}, {w[-1]:.4g}]")
+            assert e0 < w[0] and e1 > w[-1]
+        y, _ = bd.last_noise()
+        o.pos[:] = x
+        ac = po.cheb_coef(nch, 0, e0, e1)
+        xo, _, dto, used, nh, err = o.bd_mid(x, y[None], pe, dt, nch, ac, e0,
+                                             e1, F=F, T=T)
+        assert used == 1 and nh == 0 and dto == info.dt
+        disp = np.abs(xo - x).max()
+        ex = np.abs(xd.to_host() - xo).max() / disp
+        _say(f"  step {step}: x err / disp {ex:.2e}")
+        assert ex < (1e-8 if version == 2 else 1e-10)
+        x = xo
+    bd.close()
