@@ -282,6 +282,13 @@ int stokes_hip_bd_step (stokes_hip_bd *bd, double *x_dev, double dt,
  * tracks none */
 int stokes_hip_bd_step_q (stokes_hip_bd *bd, double *x_dev, double *q_dev,
                           double dt, stokes_hip_bd_info *info);
+/* draw the noise from the reference's own stream instead of the device
+ * generator: MT19937 seeded as KIrand_init_genrand (KIrand.c:137-157) with
+ * the polar method of KIrand_Gaussian (KIrand.c:108-135), the n = 3 nm or
+ * 6 nm draws of an attempt made on the host in BD_calc_FB's order and
+ * copied to the device by the attempt; trajectories then follow the
+ * reference's random stream */
+int stokes_hip_bd_set_rng_mt19937 (stokes_hip_bd *bd, unsigned long seed);
 /* the time-stepping scheme of BD->scheme (brownian.h:77-85, the dispatch
  * of brownian.c:2108-2125): 0 the mid-point algorithm (BD_evolve_mid,
  * the default), 1 Banchio-Brady 2003 with step parameter bb_n = BD->BB_n

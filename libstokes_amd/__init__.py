@@ -71,6 +71,7 @@ EXPORTS = [
     "stokes_hip_bd_step_q", "stokes_hip_bd_set_torque",
     "stokes_hip_bd_set_fts_iter", "stokes_hip_ensemble_autotune",
     "stokes_hip_bd_set_scheme", "stokes_hip_set_dense_blocks",
+    "stokes_hip_bd_set_rng_mt19937",
 ]
 
 
@@ -231,6 +232,7 @@ def lib():
                                        C.POINTER(BDInfo)]
     L.stokes_hip_bd_set_torque.argtypes = [_vp, _vp]
     L.stokes_hip_bd_set_scheme.argtypes = [_vp, C.c_int, C.c_double]
+    L.stokes_hip_bd_set_rng_mt19937.argtypes = [_vp, C.c_ulong]
     L.stokes_hip_bd_set_fts_iter.argtypes = [_vp, C.c_int, C.c_int,
                                              C.c_double]
     L.stokes_hip_bd_last_noise.argtypes = [_vp, _vp, _vp]
@@ -577,6 +579,11 @@ class BD:
         check(self.L.stokes_hip_bd_set_torque(
             self.h, torque_dev.ptr if torque_dev is not None else None),
             "bd_set_torque")
+
+    def set_rng_mt19937(self, seed):
+        """draw the noise from the reference's MT19937 + polar stream"""
+        check(self.L.stokes_hip_bd_set_rng_mt19937(self.h, int(seed)),
+              "bd_set_rng_mt19937")
 
     def set_scheme(self, scheme, bb_n=1.0):
         """0 mid-point, 1 Banchio-Brady 2003 (step parameter bb_n), 2

@@ -50,6 +50,10 @@ struct stokes_hip_bd {
                                   operator (1/a, 1/a^3) and its square root;
                                   NULL for equal radii */
   double *a_dev;               /* device [np] radii, NULL for equal radii */
+  /* the reference's random stream (MT19937 + polar, host side): draws
+   * written to pinned memory before an attempt, copied by the attempt */
+  stokes_hip_mt *mt;
+  double *y_host;
   double peclet, eps, dt_lim, rmin_ol;
   int ncheb;
   unsigned long long seed;
@@ -120,6 +124,8 @@ stokes_hip_bd_destroy (stokes_hip_bd *bd)
   shk_dev_graph_destroy (bd->graph);
   if (bd->buf) shk_dev_free (NULL, bd->buf);
   if (bd->fts_buf) shk_dev_free (NULL, bd->fts_buf);
+  shk_host_free (bd->y_host);
+  free (bd->mt);
   if (bd->small) shk_dev_free (NULL, bd->small);
   shk_host_free (bd->pinned);
   free (bd->acoef);
@@ -317,6 +323,33 @@ stokes_hip_bd_set_real_cache (stokes_hip_bd *bd, int enable)
 }
 
 int
+stokes_hip_bd_set_rng_mt19937 (stokes_hip_bd *bd, unsigned long seed)
+{
+  if (!bd)
+    {
+      stokes_hip_set_error ("bd_set_rng_mt19937: NULL handle");
+      return STOKES_HIP_EARG;
+    }
+  if (!bd->mt)
+    {
+      bd->mt = malloc (sizeof (stokes_hip_mt));
+      void *h = NULL;
+      if (!bd->mt || shk_host_alloc (&h, sizeof (double) * bd->n))
+        {
+          free (bd->mt);
+          bd->mt = NULL;
+          stokes_hip_set_error ("bd_set_rng_mt19937: allocation failed");
+          return STOKES_HIP_ENOMEM;
+        }
+      bd->y_host = h;
+    }
+  stokes_hip_mt_seed (bd->mt, seed);
+  shk_dev_graph_destroy (bd->graph);   /* the draw source is baked in */
+  bd->graph = NULL;
+  return 0;
+}
+
+int
 stokes_hip_bd_set_scheme (stokes_hip_bd *bd, int scheme, double bb_n)
 {
   if (!bd || scheme < 0 || scheme > 2 || (scheme == 1 && !(bb_n > 0.0)))
@@ -472,7 +505,9 @@ bd_record (stokes_hip_bd *bd, shk_dev *d, int prec, int check_ol, double *x,
   const size_t n = bd->n;
   const size_t n3 = (size_t) bd->nm * 3;
   int rc = shk_dev_bd_status_clear (d, bd->st_dev);
-  if (!rc) rc = shk_dev_gauss (d, bd->seed, bd->counter, n, bd->y);
+  if (!rc)
+    rc = bd->mt ? shk_dev_h2d_async (d, bd->y, bd->y_host, sizeof (double) * n)
+                : shk_dev_gauss (d, bd->seed, bd->counter, n, bd->y);
   /* BD_calc_FB at x: z = M^{-1/2} y and its accuracy (chebyshev.c:241-269) */
   const double *px = rc ? NULL : bd_pos (bd, d, x, &rc);
   if (!rc) rc = shk_dev_set_pos (d, px, 1);
@@ -711,10 +746,14 @@ stokes_hip_bd_step_q (stokes_hip_bd *bd, double *x_dev, double *q_dev,
         }
       if (!fresh)
         {
-          /* replay the previous draw */
+          /* replay the previous draw (the host stream keeps it in y_host) */
           rc = shk_dev_counter_add (d, bd->counter, (unsigned long long) -1);
           if (rc) { rc = bd_fail (rc, "bd_step"); goto out; }
         }
+      else if (bd->mt)
+        /* BD_calc_FB's n draws in order (brownian.c:1604-1609) */
+        for (size_t k = 0; k < bd->n; k++)
+          bd->y_host[k] = stokes_hip_mt_gaussian (bd->mt);
       if (bd->ver == 2)
         {
           rc = bd_record (bd, d, c->prec, bd->sys->rmin == 0.0, x_dev, q_dev);

@@ -94,4 +94,14 @@ int stokes_hip_lanczos_op (int n, int niter, unsigned long seed, double margin,
                            stokes_hip_host_op op, void *ud, double *lmin,
                            double *lmax);
 
+/* MT19937 + polar method, the reference's Brownian random stream */
+typedef struct stokes_hip_mt {
+  unsigned long mt[624];
+  int idx;
+  int has_saved;
+  double saved;
+} stokes_hip_mt;
+void stokes_hip_mt_seed (stokes_hip_mt *r, unsigned long seed);
+double stokes_hip_mt_gaussian (stokes_hip_mt *r);
+
 #endif

@@ -585,3 +585,36 @@ def test_bd_unequal_radii_match_oracle(version, n):
         assert ex < (1e-8 if version == 2 else 1e-10)
         x = xo
     bd.close()
+
+
+def test_bd_reference_random_stream():
+    """with stokes_hip_bd_set_rng_mt19937 the draws are the reference's own
+    stream (MT19937 + polar method, KIrand.c:108-157), bitwise equal to the
+    oracle's generator for the same seed across steps, and the steps equal
+    the oracle's fed with that stream; a dt halving replays the draw"""
+    n, phi = 27, 0.1
+    pos, L, s, o = _setup(n, phi, seed=8)
+    w, V = _spectrum(o)
+    e0, e1 = 0.95 * w[0], 1.05 * w[-1]
+    nch, pe, dt = 30, 1.0, 1e-3
+    a = po.cheb_coef(nch, 0, e0, e1)
+    bd = ls.BD(s, peclet=pe, n_cheb=nch, eps=1e-6, dt_lim=1e-6, rmin_ol=1.0,
+               seed=1, bounds=(e0, e1))
+    bd.set_rng_mt19937(4357)
+    rng = po.Rng(4357)
+    xd = ls.DeviceArray.from_host(pos.reshape(-1))
+    x = pos.reshape(-1).copy()
+    for step in range(3):
+        info = bd.step(xd, dt)
+        assert info.attempts == 1
+        y, _ = bd.last_noise()
+        yo = rng.gaussian(3 * n)
+        assert np.array_equal(y, yo)
+        o.pos[:] = x
+        xo, _, _, used, nh, _ = o.bd_mid(x, yo[None], pe, dt, nch, a, e0, e1)
+        disp = np.abs(xo - x).max()
+        e = np.abs(xd.to_host() - xo).max() / disp
+        _say(f"  step {step}: draws equal the MT19937 stream, x err / disp {e:.2e}")
+        assert e < 1e-10
+        x = xo
+    bd.close()
