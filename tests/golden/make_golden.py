@@ -120,7 +120,7 @@ def cases():
         out.append(dict(name=f"rpy_fix9_poly_{vn[v]}", version=v, pos=FIX9,
                         box=None, a=radii,
                         x=rng.standard_normal(9 * po.dim(v)), matrix=True))
-    # N=64, phi=0.1, RSA with gap 2.1, F/T/S ~ N(0,1)
+    # the N = 64 workload: phi = 0.1, RSA with gap 2.1, F/T/S ~ N(0,1)
     L = (64 * 4.0 * math.pi / 3.0 / 0.1) ** (1.0 / 3.0)
     pos64 = rsa(64, L, 2.1, rng)
     for v in (0, 2):

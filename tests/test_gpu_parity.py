@@ -333,8 +333,9 @@ print("legacy ok")
 
 @pytest.mark.parametrize("n,phi,version", [(1024, 0.3, 2), (2048, 0.2, 0)])
 def test_full_size_row_sample_and_properties(n, phi, version):
-    """BASELINE sizes: FP64 rows against the oracle's rows, MIXED against
-    FP64, invariance under a common translation, determinism."""
+    """full workload sizes (N = 1024 FTS phi 0.3, N = 2048 F phi 0.2): FP64
+    rows against the oracle's rows, MIXED against FP64, invariance under a
+    common translation, determinism."""
     pos, L = workloads.configuration(n, phi, seed=21)
     d = ls.dim(version)
     x = np.random.default_rng(22).standard_normal(n * d)
