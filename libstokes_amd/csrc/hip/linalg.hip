@@ -45,31 +45,33 @@ __global__ void k_axpby (double *__restrict__ y, const double *__restrict__ x, d
   if (k < n) y[k] = a * x[k] + b * y[k];
 }
 
-// per particle of 11, mobile when i < nm: lo = (x_0..5, 0) for mobile and
-// 0 for fixed particles, hi = (0, x_6..10) for mobile and x for fixed
-// particles; either may be NULL
-__global__ void k_fts_split (const double *__restrict__ x, int np, int nm,
+// per particle of D values, mobile when i < nm; the first qs values of a
+// mobile particle are its known forces (F; F, T), the rest of them and
+// every value of a fixed particle its known velocities: lo = the known
+// forces (0 elsewhere), hi = the rest (0 at the known forces); either may
+// be NULL
+__global__ void k_mix_split (const double *__restrict__ x, int np, int nm, int D, int qs,
                              double *__restrict__ lo, double *__restrict__ hi)
 {
   const size_t t = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (size_t) np * 11) return;
-  const int q = (int) (t % 11);
-  const bool mob = (int) (t / 11) < nm;
+  if (t >= (size_t) np * D) return;
+  const int q = (int) (t % D);
+  const bool kf = (int) (t / D) < nm && q < qs;
   const double v = x[t];
-  if (lo) lo[t] = mob && q < 6 ? v : 0.0;
-  if (hi) hi[t] = mob && q < 6 ? 0.0 : v;
+  if (lo) lo[t] = kf ? v : 0.0;
+  if (hi) hi[t] = kf ? 0.0 : v;
 }
 
-// y = lo(a) - z (mode 0) or y = z - hi(a) (mode 1), per particle of 11
-__global__ void k_fts_combine (const double *__restrict__ a, const double *__restrict__ z,
-                               int np, int nm, int mode, double *__restrict__ y)
+// y = lo(a) - z (mode 0) or y = z - hi(a) (mode 1), per particle of D
+__global__ void k_mix_combine (const double *__restrict__ a, const double *__restrict__ z,
+                               int np, int nm, int D, int qs, int mode, double *__restrict__ y)
 {
   const size_t t = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (size_t) np * 11) return;
-  const int q = (int) (t % 11);
-  const bool mob = (int) (t / 11) < nm;
-  if (mode == 0) y[t] = (mob && q < 6 ? a[t] : 0.0) - z[t];
-  else y[t] = z[t] - (mob && q < 6 ? 0.0 : a[t]);
+  if (t >= (size_t) np * D) return;
+  const int q = (int) (t % D);
+  const bool kf = (int) (t / D) < nm && q < qs;
+  if (mode == 0) y[t] = (kf ? a[t] : 0.0) - z[t];
+  else y[t] = z[t] - (kf ? 0.0 : a[t]);
 }
 
 // (F, T) per particle of 6 -> (F, T, 0) per particle of 11 (mode 0), or
@@ -137,26 +139,38 @@ extern "C" int shk_dev_vh (shk_dev *d, const double *V, const double *h_dev, int
   return SHK_OK;
 }
 
+extern "C" int shk_dev_mix_split (shk_dev *d, const double *x, int np, int nm, int D, int qs,
+                                  double *lo, double *hi)
+{
+  hipStream_t s = (hipStream_t) shk_dev_stream (d);
+  const size_t n = (size_t) np * D;
+  hipLaunchKernelGGL (k_mix_split, dim3 ((unsigned) ((n + 255) / 256)), dim3 (256), 0, s, x, np,
+                      nm, D, qs, lo, hi);
+  CHK (hipGetLastError ());
+  return SHK_OK;
+}
+
+extern "C" int shk_dev_mix_combine (shk_dev *d, const double *a, const double *z, int np,
+                                    int nm, int D, int qs, int mode, double *y)
+{
+  hipStream_t s = (hipStream_t) shk_dev_stream (d);
+  const size_t n = (size_t) np * D;
+  hipLaunchKernelGGL (k_mix_combine, dim3 ((unsigned) ((n + 255) / 256)), dim3 (256), 0, s, a,
+                      z, np, nm, D, qs, mode, y);
+  CHK (hipGetLastError ());
+  return SHK_OK;
+}
+
 extern "C" int shk_dev_fts_split (shk_dev *d, const double *x, int np, int nm, double *lo,
                                   double *hi)
 {
-  hipStream_t s = (hipStream_t) shk_dev_stream (d);
-  const size_t n = (size_t) np * 11;
-  hipLaunchKernelGGL (k_fts_split, dim3 ((unsigned) ((n + 255) / 256)), dim3 (256), 0, s, x, np,
-                      nm, lo, hi);
-  CHK (hipGetLastError ());
-  return SHK_OK;
+  return shk_dev_mix_split (d, x, np, nm, 11, 6, lo, hi);
 }
 
 extern "C" int shk_dev_fts_combine (shk_dev *d, const double *a, const double *z, int np,
                                     int nm, int mode, double *y)
 {
-  hipStream_t s = (hipStream_t) shk_dev_stream (d);
-  const size_t n = (size_t) np * 11;
-  hipLaunchKernelGGL (k_fts_combine, dim3 ((unsigned) ((n + 255) / 256)), dim3 (256), 0, s, a,
-                      z, np, nm, mode, y);
-  CHK (hipGetLastError ());
-  return SHK_OK;
+  return shk_dev_mix_combine (d, a, z, np, nm, 11, 6, mode, y);
 }
 
 extern "C" int shk_dev_ft_fts (shk_dev *d, const double *x, int np, int mode, double *y)

@@ -195,9 +195,12 @@ stokes_hip_bd_create (struct stokes *sys, const double *F_dev, double peclet,
     + (sys->a ? 2 * n + np : 0);
   int rc = bd->acoef ? 0 : SHK_ENOMEM;
   if (!rc) rc = shk_dev_malloc (c->dev, (void **) &bd->buf, sizeof (double) * nbuf);
-  if (!rc && bd->ver == 2)
+  /* GMRES work space: the FTS products, and the F / FT drift with fixed
+   * particles (a mixed problem) */
+  const int dmix = sys->version == 2 ? 11 : (sys->version == 0 ? 3 : 6);
+  if (!rc && (bd->ver == 2 || sys->nm < sys->np))
     rc = shk_dev_malloc (c->dev, (void **) &bd->fts_buf, sizeof (double)
-                         * (22 * np + stokes_hip_fts_ws (sys->np, 64)));
+                         * (2 * dmix * np + stokes_hip_mix_ws (sys->np, dmix, 64)));
   if (!rc) rc = shk_dev_malloc (c->dev, &bd->small, 256);
   if (!rc) rc = shk_host_alloc (&bd->pinned, 256);
   if (rc)
@@ -230,8 +233,8 @@ stokes_hip_bd_create (struct stokes *sys, const double *F_dev, double peclet,
   if (bd->fts_buf)
     {
       bd->fts_b = bd->fts_buf;
-      bd->fts_x = bd->fts_b + 11 * np;
-      bd->fts_ws = bd->fts_x + 11 * np;
+      bd->fts_x = bd->fts_b + dmix * np;
+      bd->fts_ws = bd->fts_x + dmix * np;
     }
   bd->counter = (unsigned long long *) bd->small;
   bd->sc_dev = (shk_bd_scalars *) ((char *) bd->small + 64);
@@ -244,7 +247,7 @@ stokes_hip_bd_create (struct stokes *sys, const double *F_dev, double peclet,
   rc = shk_dev_h2d (c->dev, bd->counter, &zero, sizeof (zero));
   if (!rc) rc = shk_dev_scal (c->dev, bd->buf, NULL, 0.0, nbuf);
   if (!rc && bd->fts_buf)
-    rc = shk_dev_scal (c->dev, bd->fts_buf, NULL, 0.0, 22 * np);
+    rc = shk_dev_scal (c->dev, bd->fts_buf, NULL, 0.0, 2 * dmix * np);
   if (!rc && sys->a)
     {
       /* BD_scale_P_to_G_uo (brownian.c:430-480): U / a, O / a^3 */
@@ -440,11 +443,30 @@ bd_mob (stokes_hip_bd *bd, shk_dev *d, int prec, const double *x, double *y,
   if (bd->ver != 2)
     {
       if (bd->nm == bd->np) return shk_dev_atimes (d, x, y, 1, prec);
-      /* fixed particles: zero forces there, the mobile rows kept
-       * (brownian.c:521-547, 562-588); the tail of fpad stays zero */
-      int rc = shk_dev_scal (d, bd->fpad, x, 1.0, bd->n);
-      if (!rc) rc = shk_dev_atimes (d, bd->fpad, bd->upad, 1, prec);
-      if (!rc) rc = shk_dev_scal (d, y, bd->upad, 1.0, bd->n);
+      if (noise)
+        {
+          /* the noise operator with fixed particles: zero forces there, the
+           * mobile rows kept (brownian.c:521-547, 562-588); the tail of
+           * fpad stays zero */
+          int rc = shk_dev_scal (d, bd->fpad, x, 1.0, bd->n);
+          if (!rc) rc = shk_dev_atimes (d, bd->fpad, bd->upad, 1, prec);
+          if (!rc) rc = shk_dev_scal (d, y, bd->upad, 1.0, bd->n);
+          return rc;
+        }
+      /* the drift (solve_mix_3all -> solve_mix_3f / 3ft, ewald-3f.c:269-330):
+       * the fixed particles at rest (their velocities 0, forces unknown),
+       * a mixed problem by GMRES with stokes_set_iter's parameters; the
+       * tail of fts_b (the fixed velocities) stays zero */
+      const int D = bd->ver == 0 ? 3 : 6;
+      int m, maxit, it = 0;
+      double eps, rr = 0.0;
+      stokes_hip_iter_params (bd->sys, &m, &maxit, &eps);
+      int rc = shk_dev_scal (d, bd->fts_b, x, 1.0, bd->n);
+      if (!rc) rc = stokes_hip_mix_solve (bd->c, D, D, 1, bd->nm, bd->fts_b,
+                                          bd->fts_x, m, maxit, eps, bd->fts_ws,
+                                          &it, &rr);
+      bd->fts_iters += it;
+      if (!rc) rc = shk_dev_scal (d, y, bd->fts_x, 1.0, bd->n);
       return rc;
     }
   /* FTS: the mixed problem with the fixed particles at rest (U = O = E = 0;
@@ -713,6 +735,9 @@ stokes_hip_bd_step_q (stokes_hip_bd *bd, double *x_dev, double *q_dev,
 
   double dt_local = dt;
   double fact = bd_fact (bd->peclet, dt_local);
+  /* an attempt that steers GMRES from the host runs eagerly (FTS, and the
+   * F / FT drift with fixed particles); the others replay a graph */
+  const int eager = bd->ver == 2 || bd->nm < bd->np;
   int fresh = 1;           /* new Gaussian draw for this attempt */
   int refit_done = 0;      /* BD_calc_FB refits at most once per draw */
   for (;;)
@@ -724,7 +749,7 @@ stokes_hip_bd_step_q (stokes_hip_bd *bd, double *x_dev, double *q_dev,
           rc = STOKES_HIP_EARG;
           goto out;
         }
-      if (bd->ver != 2
+      if (!eager
           && (!bd->graph || bd->graph_c != c || bd->graph_x != x_dev
               || bd->graph_q != q_dev
               || bd->graph_gen != shk_dev_generation (d)
@@ -754,7 +779,7 @@ stokes_hip_bd_step_q (stokes_hip_bd *bd, double *x_dev, double *q_dev,
         /* BD_calc_FB's n draws in order (brownian.c:1604-1609) */
         for (size_t k = 0; k < bd->n; k++)
           bd->y_host[k] = stokes_hip_mt_gaussian (bd->mt);
-      if (bd->ver == 2)
+      if (eager)
         {
           rc = bd_record (bd, d, c->prec, bd->sys->rmin == 0.0, x_dev, q_dev);
           if (bd->real_cache && rc)

@@ -138,18 +138,21 @@ gmres_dev (shk_dev *d, size_t n, dev_op op, void *ud, const double *b,
 typedef struct {
   shk_dev *d;
   int np, nm, prec;
+  int D, qs;                    /* values per particle, known forces of a
+                                   mobile particle (11, 6 for FTS) */
   double *t1, *t2;
 } mix_op;
 
 /* A x = [(U, O, 0)_m, 0_f] - M [(0, 0, S)_m, (F, T, S)_f]
- * (atimes_mix_3fts, ewald-3fts.c:440-525; atimes_mob_3fts for nm = np) */
+ * (atimes_mix_3fts, ewald-3fts.c:440-525; atimes_mob_3fts for nm = np);
+ * F / FT: A x = [U_m, 0_f] - M [0_m, F_f] (atimes_mix_3f, ewald-3f.c) */
 static int
 mix_apply (void *ud, const double *x, double *y)
 {
   mix_op *m = ud;
-  int rc = shk_dev_fts_split (m->d, x, m->np, m->nm, NULL, m->t1);
+  int rc = shk_dev_mix_split (m->d, x, m->np, m->nm, m->D, m->qs, NULL, m->t1);
   if (!rc) rc = shk_dev_atimes (m->d, m->t1, m->t2, 1, m->prec);
-  if (!rc) rc = shk_dev_fts_combine (m->d, x, m->t2, m->np, m->nm, 0, y);
+  if (!rc) rc = shk_dev_mix_combine (m->d, x, m->t2, m->np, m->nm, m->D, m->qs, 0, y);
   return rc;
 }
 
@@ -191,22 +194,40 @@ stokes_hip_fts_solve (stokes_ctx *c, int kind, int nm, const double *b_dev,
                       double *x_dev, int m, int maxit, double eps, double *ws,
                       int *iters, double *relres)
 {
+  return stokes_hip_mix_solve (c, 11, 6, kind, nm, b_dev, x_dev, m, maxit, eps,
+                               ws, iters, relres);
+}
+
+size_t
+stokes_hip_mix_ws (int np, int D, int m)
+{
+  const size_t n = (size_t) np * D;
+  if (m > GMRES_MAX_RESTART) m = GMRES_MAX_RESTART;
+  return n * (size_t) (m + 1) + 4 * n + (size_t) (m + 2);
+}
+
+int
+stokes_hip_mix_solve (stokes_ctx *c, int D, int qs, int kind, int nm,
+                      const double *b_dev, double *x_dev, int m, int maxit,
+                      double eps, double *ws, int *iters, double *relres)
+{
   const int np = c->sys->np;
-  const size_t n = (size_t) np * 11;
+  const size_t n = (size_t) np * D;
   if (m > GMRES_MAX_RESTART) m = GMRES_MAX_RESTART;
   double *V = ws, *w = V + n * (size_t) (m + 1), *b = w + n, *t1 = b + n,
     *t2 = t1 + n, *hd = t2 + n;
-  mix_op op = {c->dev, np, nm, c->prec, t1, t2};
+  mix_op op = {c->dev, np, nm, c->prec, D, qs, t1, t2};
   int rc;
   if (kind == 0)
     rc = shk_dev_scal (c->dev, b, b_dev, 1.0, n);
   else
     {
       /* b = M [(F, T, 0)_m, 0_f] - [(0, 0, E)_m, (U, O, E)_f]
-       * (calc_b_mix_3fts, ewald-3fts.c:363-438) */
-      rc = shk_dev_fts_split (c->dev, b_dev, np, nm, t1, NULL);
+       * (calc_b_mix_3fts, ewald-3fts.c:363-438; calc_b_mix_3f: b = M
+       * [F_m, 0_f] - [0_m, U_f]) */
+      rc = shk_dev_mix_split (c->dev, b_dev, np, nm, D, qs, t1, NULL);
       if (!rc) rc = shk_dev_atimes (c->dev, t1, t2, 1, c->prec);
-      if (!rc) rc = shk_dev_fts_combine (c->dev, b_dev, t2, np, nm, 1, b);
+      if (!rc) rc = shk_dev_mix_combine (c->dev, b_dev, t2, np, nm, D, qs, 1, b);
     }
   int it = 0;
   double rr = 0.0;

@@ -80,6 +80,12 @@ int stokes_hip_cheb_launch (shk_dev *dev, int prec, int ncheb, const double *a,
  * stokes_hip_fts_ws (np, m) doubles of device memory.  Returns an SHK_E*
  * code. */
 size_t stokes_hip_fts_ws (int np, int m);
+/* the same for D values per particle, the first qs of a mobile particle
+ * its known forces (F: 3, 3; FT: 6, 6; FTS: 11, 6) */
+size_t stokes_hip_mix_ws (int np, int D, int m);
+int stokes_hip_mix_solve (stokes_ctx *c, int D, int qs, int kind, int nm,
+                          const double *b_dev, double *x_dev, int m, int maxit,
+                          double eps, double *ws, int *iters, double *relres);
 /* GMRES parameters of stokes_set_iter (restart, max, eps; defaults 30,
  * 2000, 1e-12) */
 void stokes_hip_iter_params (const struct stokes *sys, int *m, int *maxit,

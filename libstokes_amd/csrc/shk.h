@@ -161,6 +161,12 @@ int shk_dev_vh (shk_dev *d, const double *V, const double *h_dev, int m,
  * 0_fixed] - z; mode 1: y = z - [(0, a_6..10)_mobile, a_fixed] */
 int shk_dev_fts_split (shk_dev *d, const double *x, int np, int nm, double *lo,
                        double *hi);
+/* the same for D values per particle whose first qs are a mobile
+ * particle's known forces (FTS: D = 11, qs = 6; F, FT: qs = D) */
+int shk_dev_mix_split (shk_dev *d, const double *x, int np, int nm, int D, int qs,
+                       double *lo, double *hi);
+int shk_dev_mix_combine (shk_dev *d, const double *a, const double *z, int np,
+                         int nm, int D, int qs, int mode, double *y);
 int shk_dev_fts_combine (shk_dev *d, const double *a, const double *z, int np,
                          int nm, int mode, double *y);
 

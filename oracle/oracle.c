@@ -1081,18 +1081,34 @@ orc_quat_dqdt (const double *Q, const double *O, double *dQdt)
  * Gaussian elimination with partial pivoting, (U, O) = M[UO, K] ft +
  * M[UO, X] xi.  Exact to rounding: the limit of the reference's
  * iteration.  O(np^3); small systems only. */
+static void orc_mix_known (const orc_sys *s, int nm, int D, int qs,
+                           const double *ft, double *uo);
+
 void
 orc_fts_mix_ft (const orc_sys *s, int nm, const double *ft, double *uo)
 {
-  const int np = s->np, n = 11 * np;
-  const int nk = 6 * nm, nx = 5 * nm + 11 * (np - nm);
+  orc_mix_known (s, nm, 11, 6, ft, uo);
+}
+
+/* The same elimination for D values per particle (3 F, 6 FT, 11 FTS), the
+ * first qs of a mobile particle known (forces), the rest of it and all D
+ * of a fixed particle unknown, with zero velocity on the unknowns' rows.
+ * F / FT (D = qs): the mixed problem of solve_mix_3f / solve_mix_3ft
+ * (ewald-3f.c:269-330, ewald-3ft.c) with the fixed particles at rest, U_m
+ * out; the extracted form is the true operator there (no S transform). */
+static void
+orc_mix_known (const orc_sys *s, int nm, int D, int qs, const double *ft,
+               double *uo)
+{
+  const int np = s->np, n = D * np;
+  const int nk = qs * nm, nx = n - nk;
   int *K = malloc (sizeof (int) * (nk > 0 ? nk : 1));
   int *X = malloc (sizeof (int) * (nx > 0 ? nx : 1));
   for (int i = 0, a = 0, b = 0; i < np; i++)
-    for (int q = 0; q < 11; q++)
+    for (int q = 0; q < D; q++)
       {
-        if (i < nm && q < 6) K[a++] = 11 * i + q;
-        else X[b++] = 11 * i + q;
+        if (i < nm && q < qs) K[a++] = D * i + q;
+        else X[b++] = D * i + q;
       }
   /* the rows of known zero velocity are the unknowns' indices */
   const int *R = X;
@@ -1102,7 +1118,7 @@ orc_fts_mix_ft (const orc_sys *s, int nm, const double *ft, double *uo)
   double *S = malloc (sizeof (double) * nx);
   if (s->periodic) orc_matrix_ewald (s, M);
   else orc_matrix_nonewald (s, M);
-  for (int r = 0; r < n; r++)
+  for (int r = 0; r < n && D == 11; r++)
     for (int p = 0; p < np; p++)
       {
         double *m = M + (size_t) r * n + 11 * p + 6;
@@ -1175,9 +1191,11 @@ orc_bd_op_cb (int n, const double *x, double *y, void *ud)
   else orc_atimes_3all (s, x, y);
 }
 
-/* the same with the particles nm..np-1 fixed: F / FT zero-pad the forces
- * of the fixed particles and keep the mobile rows (brownian.c:521-547,
- * 562-588), FTS the mixed problem with those particles at rest (650-729) */
+/* the same with the particles nm..np-1 fixed: the noise operator of F / FT
+ * zero-pads the forces of the fixed particles and keeps the mobile rows
+ * (brownian.c:521-547, 562-588), that of FTS is the mixed problem with
+ * those particles at rest (650-729); the drift (solve_mix_3all) holds them
+ * at rest in every version */
 typedef struct { const orc_sys *s; int nm; } orc_bd_ctx;
 
 static void orc_bd_mob (int n, const double *x, double *y, void *ud,
@@ -1205,6 +1223,9 @@ orc_bd_mob (int n, const double *x, double *y, void *ud, int scaled)
   const int d = s->version == 0 ? 3 : 6;
   if (c->nm == s->np) orc_bd_op_cb (n, x, y, (void *) s);
   else if (s->version == 2) orc_fts_mix_ft (s, c->nm, x, y);
+  else if (!scaled)
+    /* the drift (solve_mix_3f / 3ft): fixed particles at rest */
+    orc_mix_known (s, c->nm, d, d, x, y);
   else
     {
       const int nf = d * s->np, nmd = d * c->nm;

@@ -481,10 +481,11 @@ def test_bd_schemes_match_oracle(version, n, scheme, bb_n):
 
 @pytest.mark.parametrize("version,n,nm", [(0, 27, 20), (1, 27, 20), (2, 12, 8)])
 def test_bd_fixed_particles_match_oracle(version, n, nm):
-    """mid-point steps with particles nm..n-1 fixed (BD_atimes_mob_FU's
-    zero padding for F / FT, the mixed problem for FTS; overlap check over
-    the mobile particles) against the oracle's step fed with the device's
-    draw; the fixed particles do not move"""
+    """mid-point steps with particles nm..n-1 fixed (the noise by
+    BD_atimes_mob_FU's zero padding for F / FT, the mixed problem for FTS;
+    the drift by the mixed problem of solve_mix_3all, GMRES on the device;
+    overlap check over the mobile particles) against the oracle's step fed
+    with the device's draw; the fixed particles do not move"""
     phi = 0.1
     _say(f"bd fixed particles version={version} N={n} nm={nm}")
     pos, L = workloads.configuration(n, phi, seed=9)
@@ -507,8 +508,8 @@ def test_bd_fixed_particles_match_oracle(version, n, nm):
                seed=3, force_dev=ls.DeviceArray.from_host(F), bounds=(e0, e1))
     if version > 0:
         bd.set_torque(ls.DeviceArray.from_host(T))
+    s.set_iter(eps=1e-13)
     if version == 2:
-        s.set_iter(eps=1e-13)
         bd.set_fts_iter(30, 2000, 1e-13)
     x = pos.reshape(-1)[:3 * nm].copy()
     xd = ls.DeviceArray.from_host(x)

@@ -409,9 +409,12 @@ def test_fts_mobility_with_fixed_particles():
 
 @pytest.mark.parametrize("version", [0, 1])
 def test_bd_step_fixed_particles_composition(version):
-    """a mid-point step with particles 5..7 fixed: the noise and drift use
-    the zero-padded product of the compiled reference's atimes_3all on the
-    mobile rows (brownian.c:521-547, 562-588); the fixed particles stay"""
+    """a mid-point step with particles 5..7 fixed: the noise uses the
+    zero-padded product of the compiled reference's atimes_3all on the
+    mobile rows (brownian.c:521-547, 562-588); the drift is the mixed
+    problem of solve_mix_3f / 3ft (ewald-3f.c:269-330), the fixed particles
+    at rest with unknown forces, here by the Schur complement of the
+    reference operator's columns; the fixed particles stay"""
     import ctypes as C
     L = 8.0
     pos = _eight(L)
@@ -435,12 +438,17 @@ def test_bd_step_fixed_particles_composition(version):
     def ref_op(p, f):
         t = po.System(version, p, box=(L, L, L), tr=1.0)
         h = R.ref_sys_new(C.byref(t.s))
-        full = np.zeros(d * 8)
-        full[:n] = f
-        u = np.zeros(d * 8)
-        R.ref_atimes(h, d * 8, po._p(full), po._p(u))
+        M = np.zeros((d * 8, d * 8))
+        for k in range(d * 8):
+            e = np.zeros(d * 8)
+            e[k] = 1.0
+            u = np.zeros(d * 8)
+            R.ref_atimes(h, d * 8, po._p(e), po._p(u))
+            M[:, k] = u
         R.ref_sys_free(h)
-        return u[:n]
+        # U_f = 0: M_fm f + M_ff F_f = 0, U_m = M_mm f + M_mf F_f
+        Ff = np.linalg.solve(M[n:, n:], -M[n:, :n] @ f)
+        return M[:n, :n] @ f + M[:n, n:] @ Ff
     z = po.cheb_eval_dense(a, G, y, e0, e1)
     fact = math.sqrt(2 / (pe * dt))
     f = F + fact * z if version == 0 else np.concatenate(
