@@ -59,6 +59,8 @@ EXPORTS = [
     "solve_mob_3fts_0", "stokes_hip_solve_mix_3fts_dev",
     "stokes_hip_solve_res_3fts_dev", "solve_res_3fts_0", "solve_res_3fts",
     "solve_mob_3fts", "solve_mix_3fts_0", "solve_mix_3fts",
+    "solve_res_3f", "solve_res_3f_0", "solve_mob_3f", "solve_mix_3f",
+    "solve_res_3ft", "solve_res_3ft_0", "solve_mob_3ft", "solve_mix_3ft",
     "shift_labo_to_rest_U", "shift_rest_to_labo_U", "shift_labo_to_rest_O",
     "shift_rest_to_labo_O", "shift_labo_to_rest_E",
     "stokes_hip_ensemble_create", "stokes_hip_ensemble_atimes",
@@ -176,7 +178,15 @@ def lib():
                  ("solve_res_3fts", [sp] + [_dp] * 6),
                  ("solve_mob_3fts", [sp] + [_dp] * 6),
                  ("solve_mix_3fts_0", [sp, _vp] + [_dp] * 12),
-                 ("solve_mix_3fts", [sp] + [_dp] * 12)):
+                 ("solve_mix_3fts", [sp] + [_dp] * 12),
+                 ("solve_res_3f", [sp] + [_dp] * 2),
+                 ("solve_res_3f_0", [sp] + [_dp] * 2),
+                 ("solve_mob_3f", [sp] + [_dp] * 2),
+                 ("solve_mix_3f", [sp] + [_dp] * 4),
+                 ("solve_res_3ft", [sp] + [_dp] * 4),
+                 ("solve_res_3ft_0", [sp] + [_dp] * 4),
+                 ("solve_mob_3ft", [sp] + [_dp] * 4),
+                 ("solve_mix_3ft", [sp] + [_dp] * 8)):
         getattr(L, f).argtypes = a
         getattr(L, f).restype = None
     L.stokes_set_iter.argtypes = [sp, C.c_char_p, C.c_int, C.c_int,

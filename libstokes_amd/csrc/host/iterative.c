@@ -18,7 +18,10 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include "ewald-3f.h"
+#include "ewald-3ft.h"
 #include "ewald-3fts.h"
+#include "ewald.h"
 #include "stokes_hip.h"
 #include "stokes_internal.h"
 
@@ -246,22 +249,39 @@ stokes_hip_iter_params (const struct stokes *sys, int *m, int *maxit,
   iter_params (sys, NULL, m, maxit, eps);
 }
 
+/* values per particle of a version (F 3, FT 6, FTS 11) and the known
+ * forces of a mobile particle in the mixed problem (3, 6, 6) */
+static int
+ver_dim (int version)
+{
+  return version == 0 ? 3 : (version == 1 ? 6 : 11);
+}
+
+static int
+ver_qs (int version)
+{
+  return version == 0 ? 3 : 6;
+}
+
 /* kind 0: resistance, b_dev = (U, O, E) per particle -> x = (F, T, S);
  * kind 1: mixed with nm mobile particles, b_dev = [(F, T, E)_m,
- * (U, O, E)_f] -> x = [(U, O, S)_m, (F, T, S)_f] */
+ * (U, O, E)_f] -> x = [(U, O, S)_m, (F, T, S)_f]; the F and FT versions
+ * the same with (U), (F) or (U, O), (F, T) per particle.  version: the
+ * version the caller requires of sys */
 static int
-solve_fts_dev (struct stokes *sys, int kind, int nm, const double *b_dev,
-               double *x_dev, int *iters, double *relres, const char *what)
+solve_dev (struct stokes *sys, int version, int kind, int nm,
+           const double *b_dev, double *x_dev, int *iters, double *relres,
+           const char *what)
 {
   if (sys == NULL || b_dev == NULL || x_dev == NULL)
     {
       stokes_hip_set_error ("%s: NULL argument", what);
       return STOKES_HIP_EARG;
     }
-  if (sys->version != 2)
+  if (sys->version != version)
     {
-      stokes_hip_set_error ("%s: FTS version only (version %d)", what,
-                            sys->version);
+      stokes_hip_set_error ("%s: version %d required (version %d)", what,
+                            version, sys->version);
       return STOKES_HIP_EARG;
     }
   if (nm < 0 || nm > sys->np)
@@ -276,9 +296,10 @@ solve_fts_dev (struct stokes *sys, int kind, int nm, const double *b_dev,
   int m, maxit;
   double eps;
   iter_params (sys, NULL, &m, &maxit, &eps);
+  const int D = ver_dim (version);
   double *buf = NULL;
   rc = shk_dev_malloc (c->dev, (void **) &buf,
-                       sizeof (double) * stokes_hip_fts_ws (sys->np, m));
+                       sizeof (double) * stokes_hip_mix_ws (sys->np, D, m));
   if (rc)
     {
       stokes_hip_set_error ("%s: %s", what, shk_last_error ());
@@ -286,8 +307,8 @@ solve_fts_dev (struct stokes *sys, int kind, int nm, const double *b_dev,
     }
   int it = 0;
   double rr = 0.0;
-  rc = stokes_hip_fts_solve (c, kind, nm, b_dev, x_dev, m, maxit, eps, buf,
-                             &it, &rr);
+  rc = stokes_hip_mix_solve (c, D, ver_qs (version), kind, nm, b_dev, x_dev,
+                             m, maxit, eps, buf, &it, &rr);
   shk_dev_free (c->dev, buf);
   if (rc)
     {
@@ -303,7 +324,7 @@ int
 stokes_hip_solve_mob_3fts_dev (struct stokes *sys, const double *fte_dev,
                                double *uos_dev, int *iters, double *relres)
 {
-  return solve_fts_dev (sys, 1, sys ? sys->np : 0, fte_dev, uos_dev, iters,
+  return solve_dev (sys, 2, 1, sys ? sys->np : 0, fte_dev, uos_dev, iters,
                         relres, "solve_mob_3fts");
 }
 
@@ -311,7 +332,7 @@ int
 stokes_hip_solve_mix_3fts_dev (struct stokes *sys, const double *b_dev,
                                double *x_dev, int *iters, double *relres)
 {
-  return solve_fts_dev (sys, 1, sys ? sys->nm : 0, b_dev, x_dev, iters,
+  return solve_dev (sys, 2, 1, sys ? sys->nm : 0, b_dev, x_dev, iters,
                         relres, "solve_mix_3fts");
 }
 
@@ -319,21 +340,30 @@ int
 stokes_hip_solve_res_3fts_dev (struct stokes *sys, const double *uoe_dev,
                                double *fts_dev, int *iters, double *relres)
 {
-  return solve_fts_dev (sys, 0, sys ? sys->np : 0, uoe_dev, fts_dev, iters,
+  return solve_dev (sys, 2, 0, sys ? sys->np : 0, uoe_dev, fts_dev, iters,
                         relres, "solve_res_3fts");
 }
 
 /* ---- reference entry points (host arrays; void, exit(1) on failure) ---- */
 
+/* as the reference's solve_* do: a wrong version is reset with a message
+ * (ewald-3f.c:50-55, ewald-3ft.c:53-58, ewald-3fts.c:54-59) */
+static void
+check_version (struct stokes *sys, int version, const char *name)
+{
+  static const char *vn[] = {"F", "FT", "FTS"};
+  if (sys->version != version)
+    {
+      fprintf (stderr, "libstokes %s : the version is wrong. reset to %s\n",
+               name, vn[version]);
+      sys->version = version;
+    }
+}
+
 static void
 check_fts (struct stokes *sys, const char *name)
 {
-  if (sys->version != 2)
-    {
-      fprintf (stderr, "libstokes %s : the version is wrong. reset to FTS\n",
-               name);
-      sys->version = 2;
-    }
+  check_version (sys, 2, name);
 }
 
 /* pack per-particle blocks of 3, 3, 5 from three arrays into h[11 n] */
@@ -359,20 +389,21 @@ unpack_335 (int n, const double *h, double *a, double *b, double *c)
     }
 }
 
-/* solve on host vectors h[11 np] (in: b, out: x) with sys->it replaced by
- * iter when not NULL */
+/* solve on host vectors h[D np] of sys->version (in: b, out: x) with
+ * sys->it replaced by iter when not NULL */
 static void
 solve_host (struct stokes *sys, struct iter *iter, int kind, int nm,
             double *h, const char *name)
 {
   struct iter *keep = sys->it;
   if (iter != NULL) sys->it = iter;
-  size_t n = (size_t) sys->np * 11;
+  size_t n = (size_t) sys->np * ver_dim (sys->version);
   double *bd = NULL, *xd = NULL;
   int rc = stokes_hip_malloc ((void **) &bd, sizeof (double) * n);
   if (!rc) rc = stokes_hip_malloc ((void **) &xd, sizeof (double) * n);
   if (!rc) rc = stokes_hip_memcpy_h2d (bd, h, sizeof (double) * n);
-  if (!rc) rc = solve_fts_dev (sys, kind, nm, bd, xd, NULL, NULL, name);
+  if (!rc) rc = solve_dev (sys, sys->version, kind, nm, bd, xd, NULL, NULL,
+                           name);
   if (!rc) rc = stokes_hip_memcpy_d2h (h, xd, sizeof (double) * n);
   if (bd) stokes_hip_free (bd);
   if (xd) stokes_hip_free (xd);
@@ -442,6 +473,151 @@ solve_mix_3fts_0 (struct stokes *sys, struct iter *iter, const double *f,
   unpack_335 (nm, h, u, o, s);
   unpack_335 (nf, h + 11 * (size_t) nm, ff, tf, sf);
   free (h);
+}
+
+/* ---- F and FT versions (ewald-3f.c, ewald-3ft.c) ---- */
+
+/* h[6 n] particle-wise (a_i, b_i) from two arrays of 3 per particle and
+ * back (set_ft_by_FT / set_FT_by_ft) */
+static void
+pack_33 (int n, double *h, const double *a, const double *b)
+{
+  for (int i = 0; i < n; i++)
+    {
+      memcpy (h + 6 * i, a + 3 * i, sizeof (double) * 3);
+      memcpy (h + 6 * i + 3, b + 3 * i, sizeof (double) * 3);
+    }
+}
+
+static void
+unpack_33 (int n, const double *h, double *a, double *b)
+{
+  for (int i = 0; i < n; i++)
+    {
+      memcpy (a + 3 * i, h + 6 * i, sizeof (double) * 3);
+      memcpy (b + 3 * i, h + 6 * i + 3, sizeof (double) * 3);
+    }
+}
+
+void
+solve_res_3f_0 (struct stokes *sys, const double *u, double *f)
+{
+  check_version (sys, 0, "solve_res_3f_0");
+  const size_t n = (size_t) sys->np * 3;
+  double *h = alloc_h (n, "solve_res_3f_0");
+  memcpy (h, u, sizeof (double) * n);
+  solve_host (sys, NULL, 0, sys->np, h, "solve_res_3f_0");
+  memcpy (f, h, sizeof (double) * n);
+  free (h);
+}
+
+void
+solve_res_3f (struct stokes *sys, const double *u, double *f)
+{
+  check_version (sys, 0, "solve_res_3f");
+  double *u0 = alloc_h ((size_t) sys->np * 3, "solve_res_3f");
+  shift_labo_to_rest_U (sys, sys->np, u, u0);
+  solve_res_3f_0 (sys, u0, f);
+  free (u0);
+}
+
+void
+solve_mob_3f (struct stokes *sys, const double *f, double *u)
+{
+  check_version (sys, 0, "solve_mob_3f");
+  atimes_3all (sys->np * 3, f, u, sys);
+  shift_rest_to_labo_U (sys, sys->np, u);
+}
+
+void
+solve_mix_3f (struct stokes *sys, const double *f, const double *uf,
+              double *u, double *ff)
+{
+  check_version (sys, 0, "solve_mix_3f");
+  const int np = sys->np, nm = sys->nm, nf = np - nm;
+  if (nm == np)
+    {
+      solve_mob_3f (sys, f, u);
+      return;
+    }
+  double *h = alloc_h ((size_t) np * 3, "solve_mix_3f");
+  memcpy (h, f, sizeof (double) * 3 * (size_t) nm);
+  /* the shift takes the positions of particles 0..nf-1, as the reference
+   * does (ewald-3f.c:301) */
+  shift_labo_to_rest_U (sys, nf, uf, h + 3 * (size_t) nm);
+  solve_host (sys, NULL, 1, nm, h, "solve_mix_3f");
+  memcpy (u, h, sizeof (double) * 3 * (size_t) nm);
+  memcpy (ff, h + 3 * (size_t) nm, sizeof (double) * 3 * (size_t) nf);
+  free (h);
+  shift_rest_to_labo_U (sys, nm, u);
+}
+
+void
+solve_res_3ft_0 (struct stokes *sys, const double *u, const double *o,
+                 double *f, double *t)
+{
+  check_version (sys, 1, "solve_res_3ft_0");
+  double *h = alloc_h ((size_t) sys->np * 6, "solve_res_3ft_0");
+  pack_33 (sys->np, h, u, o);
+  solve_host (sys, NULL, 0, sys->np, h, "solve_res_3ft_0");
+  unpack_33 (sys->np, h, f, t);
+  free (h);
+}
+
+void
+solve_res_3ft (struct stokes *sys, const double *u, const double *o,
+               double *f, double *t)
+{
+  check_version (sys, 1, "solve_res_3ft");
+  const size_t n3 = (size_t) sys->np * 3;
+  double *u0 = alloc_h (2 * n3, "solve_res_3ft"), *o0 = u0 + n3;
+  shift_labo_to_rest_U (sys, sys->np, u, u0);
+  shift_labo_to_rest_O (sys, sys->np, o, o0);
+  solve_res_3ft_0 (sys, u0, o0, f, t);
+  free (u0);
+}
+
+void
+solve_mob_3ft (struct stokes *sys, const double *f, const double *t,
+               double *u, double *o)
+{
+  check_version (sys, 1, "solve_mob_3ft");
+  const size_t n6 = (size_t) sys->np * 6;
+  double *h = alloc_h (2 * n6, "solve_mob_3ft"), *y = h + n6;
+  pack_33 (sys->np, h, f, t);
+  atimes_3all (sys->np * 6, h, y, sys);
+  unpack_33 (sys->np, y, u, o);
+  free (h);
+  shift_rest_to_labo_U (sys, sys->np, u);
+  shift_rest_to_labo_O (sys, sys->np, o);
+}
+
+void
+solve_mix_3ft (struct stokes *sys, const double *f, const double *t,
+               const double *uf, const double *of, double *u, double *o,
+               double *ff, double *tf)
+{
+  check_version (sys, 1, "solve_mix_3ft");
+  const int np = sys->np, nm = sys->nm, nf = np - nm;
+  if (nm == np)
+    {
+      solve_mob_3ft (sys, f, t, u, o);
+      return;
+    }
+  double *h = alloc_h ((size_t) np * 6 + 6 * (size_t) nf, "solve_mix_3ft");
+  double *uf0 = h + (size_t) np * 6, *of0 = uf0 + 3 * (size_t) nf;
+  /* the shifts take the positions of particles 0..nf-1, as the reference
+   * does (ewald-3ft.c:359-360) */
+  shift_labo_to_rest_U (sys, nf, uf, uf0);
+  shift_labo_to_rest_O (sys, nf, of, of0);
+  pack_33 (nm, h, f, t);
+  pack_33 (nf, h + 6 * (size_t) nm, uf0, of0);
+  solve_host (sys, NULL, 1, nm, h, "solve_mix_3ft");
+  unpack_33 (nm, h, u, o);
+  unpack_33 (nf, h + 6 * (size_t) nm, ff, tf);
+  free (h);
+  shift_rest_to_labo_U (sys, nm, u);
+  shift_rest_to_labo_O (sys, nm, o);
 }
 
 /* ---- imposed flow: labo frame <-> fluid-rest frame ---- */

@@ -1,8 +1,11 @@
-"""FTS problems by GMRES on the device (ewald-3fts.c): the natural
-mobility problem (stokes_hip_solve_mob_3fts_dev, solve_mob_3fts_0), the
-mixed problem with fixed particles (stokes_hip_solve_mix_3fts_dev,
-solve_mix_3fts_0), the resistance problem (stokes_hip_solve_res_3fts_dev,
-solve_res_3fts_0) and the labo-frame wrappers.
+"""F, FT and FTS problems by GMRES on the device (ewald-3f.c, ewald-3ft.c,
+ewald-3fts.c): for FTS the natural mobility problem
+(stokes_hip_solve_mob_3fts_dev, solve_mob_3fts_0), the mixed problem with
+fixed particles (stokes_hip_solve_mix_3fts_dev, solve_mix_3fts_0), the
+resistance problem (stokes_hip_solve_res_3fts_dev, solve_res_3fts_0) and the
+labo-frame wrappers; for F and FT the reference entry points solve_res_3f(t)
+(_0), solve_mob_3f(t) and solve_mix_3f(t) against numpy's dense solves of the
+oracle's operator matrix.
 
 Mobility / mixed: the same problems the dense path (solve_mix) solves by LU
 and the numpy restatement of ewald-3fts-matrix.c by explicit inversion.
@@ -250,4 +253,82 @@ def test_labo_frame_wrappers():
     err = relerr(got, want)
     _say(f"  labo-frame wrappers: mixed {err:.2e}")
     assert err < 1e-12
+    s.close()
+
+
+@pytest.mark.parametrize("version", [0, 1])
+def test_f_ft_entry_points(version):
+    """solve_res_3f(t)(_0), solve_mob_3f(t) and solve_mix_3f(t) (ewald-3f.c,
+    ewald-3ft.c) against numpy's dense solves of the oracle's operator
+    matrix, in the labo frame with an imposed flow; the mixed problem holds
+    the fixed particles' velocities and shifts them with the positions of
+    particles 0..nf-1 (ewald-3f.c:301, ewald-3ft.c:359-360)"""
+    n, nm = 32, 24
+    nf, d = n - nm, (3 if version == 0 else 6)
+    _say(f"F / FT entry points version={version} N={n} nm={nm}")
+    pos, L = workloads.configuration(n, 0.1, seed=35 + version)
+    o = po.System(version, pos, box=(L, L, L), tr=1.0)
+    M = o.matrix()
+    s = ls.Stokes(version, pos, box=(L, L, L), tr=1.0)
+    s.set_iter(max_it=1000, restart=30, eps=1e-13)
+    Ui, Oi = np.array([0.3, -0.2, 0.1]), np.array([0.05, 0.1, -0.07])
+    Ei = np.array([0.2, 0.05, -0.1, 0.03, -0.15])
+    s.L.stokes_set_Ui(s.sys, *Ui)
+    s.L.stokes_set_Oi(s.sys, *Oi)
+    s.L.stokes_set_Ei(s.sys, *Ei)
+    rng = np.random.default_rng(10 + version)
+    P = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))
+
+    def pk(a, b, k):        # (a_i, b_i) per particle, or a alone for F
+        if version == 0:
+            return a.copy()
+        return np.concatenate([a.reshape(k, 3), b.reshape(k, 3)], 1).reshape(-1)
+
+    def imp(k):             # imposed (U, O) of particles 0..k-1
+        return pk(_imposed_u(pos[:k], Ui, Oi, Ei).reshape(-1), np.tile(Oi, k), k)
+
+    u, om = rng.standard_normal(3 * n), rng.standard_normal(3 * n)
+    # resistance: fluid-rest frame and labo frame
+    f, t = np.zeros(3 * n), np.zeros(3 * n)
+    if version == 0:
+        s.L.solve_res_3f_0(s.sys, P(u), P(f))
+    else:
+        s.L.solve_res_3ft_0(s.sys, P(u), P(om), P(f), P(t))
+    want = np.linalg.solve(M, pk(u, om, n))
+    e0 = relerr(pk(f, t, n), want)
+    if version == 0:
+        s.L.solve_res_3f(s.sys, P(u), P(f))
+    else:
+        s.L.solve_res_3ft(s.sys, P(u), P(om), P(f), P(t))
+    want = np.linalg.solve(M, pk(u, om, n) - imp(n))
+    e1 = relerr(pk(f, t, n), want)
+    # mobility, labo frame
+    f, t = rng.standard_normal(3 * n), rng.standard_normal(3 * n)
+    uu, oo = np.zeros(3 * n), np.zeros(3 * n)
+    if version == 0:
+        s.L.solve_mob_3f(s.sys, P(f), P(uu))
+    else:
+        s.L.solve_mob_3ft(s.sys, P(f), P(t), P(uu), P(oo))
+    e2 = relerr(pk(uu, oo, n), M @ pk(f, t, n) + imp(n))
+    # mixed, labo frame
+    s.s.nm = nm
+    f, t = rng.standard_normal(3 * nm), rng.standard_normal(3 * nm)
+    uf, of = rng.standard_normal(3 * nf), rng.standard_normal(3 * nf)
+    uu, oo = np.zeros(3 * nm), np.zeros(3 * nm)
+    ff, tf = np.zeros(3 * nf), np.zeros(3 * nf)
+    if version == 0:
+        s.L.solve_mix_3f(s.sys, P(f), P(uf), P(uu), P(ff))
+    else:
+        s.L.solve_mix_3ft(s.sys, P(f), P(t), P(uf), P(of), P(uu), P(oo),
+                          P(ff), P(tf))
+    m = d * nm
+    fm = pk(f, t, nm)
+    uf0 = pk(uf, of, nf) - imp(nf)
+    Ff = np.linalg.solve(M[m:, m:], uf0 - M[m:, :m] @ fm)
+    um = M[:m, :m] @ fm + M[:m, m:] @ Ff + imp(nm)
+    e3 = relerr(np.concatenate([pk(uu, oo, nm), pk(ff, tf, nf)]),
+                np.concatenate([um, Ff]))
+    _say(f"  res_0 {e0:.2e}, res {e1:.2e}, mob {e2:.2e}, mix {e3:.2e}")
+    assert max(e0, e1, e3) < 1e-10
+    assert e2 < 1e-12
     s.close()
