@@ -68,6 +68,9 @@ struct stokes_hip_bd {
   size_t n;                    /* operator dimension: 3 np (F), 6 np */
   double *buf;                 /* device: y z w f u work[4n] xm xn qn */
   double *y, *z, *w, *f, *u, *xm, *xn, *qn, *work, *u0, *f2;
+  double *xs;                  /* device [3 nm]: the intermediate
+                                  configuration a failed attempt leaves in
+                                  the system (the retry's first stage) */
   /* FTS: GMRES of the noise products (brownian.c:597-600: restart 20,
    * max 2000, eps 1e-6) and work space */
   int fts_m, fts_max;
@@ -192,7 +195,7 @@ stokes_hip_bd_create (struct stokes *sys, const double *F_dev, double peclet,
   const size_t n = bd->n, np = (size_t) sys->np;
   const size_t dfull = (size_t) sys->np * (sys->version == 0 ? 3 : 6);
   const size_t nbuf = 11 * n + 10 * np + 3 * np + 2 * dfull
-    + (sys->a ? 2 * n + np : 0);
+    + (sys->a ? 2 * n + np : 0) + 3 * np;
   int rc = bd->acoef ? 0 : SHK_ENOMEM;
   if (!rc) rc = shk_dev_malloc (c->dev, (void **) &bd->buf, sizeof (double) * nbuf);
   /* GMRES work space: the FTS products, and the F / FT drift with fixed
@@ -229,6 +232,7 @@ stokes_hip_bd_create (struct stokes *sys, const double *F_dev, double peclet,
       bd->sqsv = bd->sv + n;
       bd->a_dev = bd->sqsv + n;
     }
+  bd->xs = bd->buf + nbuf - 3 * np;
   bd->bb_n = 1.0;
   if (bd->fts_buf)
     {
@@ -521,23 +525,27 @@ bd_cheb (stokes_hip_bd *bd, shk_dev *d, int prec, const double *y, double *z)
 /* the kernels of one attempt, in stream order (captured for F and FT) */
 static int
 bd_record (stokes_hip_bd *bd, shk_dev *d, int prec, int check_ol, double *x,
-           double *q)
+           double *q, const double *p1, int noise)
 {
   const struct stokes *s = bd->sys;
   const size_t n = bd->n;
   const size_t n3 = (size_t) bd->nm * 3;
+  /* p1: the positions the system holds for the noise and the first stage
+   * (x, or the intermediate configuration of a failed attempt); noise = 0
+   * keeps z of the previous attempt (REDO_scale) */
+  const int cache = bd->real_cache && noise;
   int rc = shk_dev_bd_status_clear (d, bd->st_dev);
-  if (!rc)
+  if (!rc && noise)
     rc = bd->mt ? shk_dev_h2d_async (d, bd->y, bd->y_host, sizeof (double) * n)
                 : shk_dev_gauss (d, bd->seed, bd->counter, n, bd->y);
-  /* BD_calc_FB at x: z = M^{-1/2} y and its accuracy (chebyshev.c:241-269) */
-  const double *px = rc ? NULL : bd_pos (bd, d, x, &rc);
+  /* BD_calc_FB: z = M^{-1/2} y and its accuracy (chebyshev.c:241-269) */
+  const double *px = rc ? NULL : bd_pos (bd, d, p1, &rc);
   if (!rc) rc = shk_dev_set_pos (d, px, 1);
-  /* the products at x share one dense real-space matrix */
-  if (!rc && bd->real_cache) rc = shk_dev_real_cache (d, 1, prec);
-  if (!rc) rc = bd_cheb (bd, d, prec, bd->y, bd->z);
-  if (!rc) rc = bd_op (bd, d, prec, bd->z, bd->w, 1, 1);
-  if (!rc) rc = shk_dev_bd_dots (d, bd->y, bd->z, bd->w, n, bd->st_dev);
+  /* the products at p1 share one dense real-space matrix */
+  if (!rc && cache) rc = shk_dev_real_cache (d, 1, prec);
+  if (!rc && noise) rc = bd_cheb (bd, d, prec, bd->y, bd->z);
+  if (!rc && noise) rc = bd_op (bd, d, prec, bd->z, bd->w, 1, 1);
+  if (!rc && noise) rc = shk_dev_bd_dots (d, bd->y, bd->z, bd->w, n, bd->st_dev);
   /* stage 1: drift at x (BB03: the Brownian force alone), Euler step to
    * the intermediate configuration: dt / 2 (mid-point), dt / n (BB03), dt
    * (BM97); the intermediate orientation is not formed, no force here
@@ -564,7 +572,7 @@ bd_record (stokes_hip_bd *bd, shk_dev *d, int prec, int check_ol, double *x,
    * M(x1) F + u0 + n/2 (M(x1) fact z - u0), its deterministic product at
    * x1 as in the reference (brownian.c:2770-2816); then the full step from
    * x */
-  if (!rc && bd->real_cache) rc = shk_dev_real_cache (d, 0, prec);
+  if (!rc && cache) rc = shk_dev_real_cache (d, 0, prec);
   const double *pm = rc ? NULL : bd_pos (bd, d, bd->xm, &rc);
   if (!rc) rc = shk_dev_set_pos (d, pm, 1);
   if (!rc) rc = bd_op (bd, d, prec, bd->f, bd->u, 0, 0);
@@ -587,7 +595,7 @@ bd_record (stokes_hip_bd *bd, shk_dev *d, int prec, int check_ol, double *x,
   if (!rc && check_ol)
     rc = shk_dev_bd_overlap (d, bd->xn, bd->a_dev, bd->nm, s->periodic, s->lx, s->ly,
                              s->lz, 0, 0.0, bd->rmin_ol, &bd->st_dev->overlap[1]);
-  if (!rc) rc = shk_dev_counter_add (d, bd->counter, 1);
+  if (!rc && noise) rc = shk_dev_counter_add (d, bd->counter, 1);
   if (!rc) rc = shk_dev_d2h_async (d, bd->st_host, bd->st_dev, sizeof (shk_bd_status));
   return rc;
 }
@@ -625,7 +633,7 @@ bd_capture (stokes_hip_bd *bd, stokes_ctx *c, double *x, double *q)
       shk_dev_timing (d, c->timing);
       return bd_fail (rc, "bd_step (capture)");
     }
-  rc = bd_record (bd, d, c->prec, check_ol, x, q);
+  rc = bd_record (bd, d, c->prec, check_ol, x, q, x, 1);
   if (rc)
     {
       shk_dev_capture_abort (d);
@@ -740,8 +748,16 @@ stokes_hip_bd_step_q (stokes_hip_bd *bd, double *x_dev, double *q_dev,
   const int eager = bd->ver == 2 || bd->nm < bd->np;
   int fresh = 1;           /* new Gaussian draw for this attempt */
   int refit_done = 0;      /* BD_calc_FB refits at most once per draw */
+  /* stale: the system holds the intermediate configuration of a failed
+   * attempt (bd->xs), as the reference's REDO_scale leaves it after an
+   * overlap in the second stage (brownian.c:2523, 2881, 3162); such an
+   * attempt runs eagerly.  keep: the attempt keeps the noise of the
+   * previous one (REDO_scale); the graph's replays at x recompute it from
+   * the same draw, which gives the same values */
+  int stale = 0, keep = 0;
   for (;;)
     {
+      const int run_eager = eager || stale;
       if (inf.attempts >= BD_MAX_ATTEMPTS)
         {
           stokes_hip_set_error ("bd_step: no acceptable step after %d attempts",
@@ -749,7 +765,7 @@ stokes_hip_bd_step_q (stokes_hip_bd *bd, double *x_dev, double *q_dev,
           rc = STOKES_HIP_EARG;
           goto out;
         }
-      if (!eager
+      if (!run_eager
           && (!bd->graph || bd->graph_c != c || bd->graph_x != x_dev
               || bd->graph_q != q_dev
               || bd->graph_gen != shk_dev_generation (d)
@@ -769,7 +785,9 @@ stokes_hip_bd_step_q (stokes_hip_bd *bd, double *x_dev, double *q_dev,
           bd->sc_fact = fact;
           bd->sc_dt = dt_local;
         }
-      if (!fresh)
+      if (keep)
+        ;
+      else if (!fresh)
         {
           /* replay the previous draw (the host stream keeps it in y_host) */
           rc = shk_dev_counter_add (d, bd->counter, (unsigned long long) -1);
@@ -779,9 +797,10 @@ stokes_hip_bd_step_q (stokes_hip_bd *bd, double *x_dev, double *q_dev,
         /* BD_calc_FB's n draws in order (brownian.c:1604-1609) */
         for (size_t k = 0; k < bd->n; k++)
           bd->y_host[k] = stokes_hip_mt_gaussian (bd->mt);
-      if (eager)
+      if (run_eager)
         {
-          rc = bd_record (bd, d, c->prec, bd->sys->rmin == 0.0, x_dev, q_dev);
+          rc = bd_record (bd, d, c->prec, bd->sys->rmin == 0.0, x_dev, q_dev,
+                          stale ? bd->xs : x_dev, !keep);
           if (bd->real_cache && rc)
             shk_dev_real_cache (d, 0, c->prec);
         }
@@ -793,17 +812,18 @@ stokes_hip_bd_step_q (stokes_hip_bd *bd, double *x_dev, double *q_dev,
       const shk_bd_status st = *bd->st_host;
       inf.overlaps[0] = st.overlap[0];
       inf.overlaps[1] = st.overlap[1];
-      if (!(st.zmz > 0.0))
+      if (!keep && !(st.zmz > 0.0))
         {
           stokes_hip_set_error ("bd_step: z.Mz = %g is not positive", st.zmz);
           rc = STOKES_HIP_EARG;
           goto out;
         }
-      inf.err_cheb = fabs (st.zmz - st.yy) / st.zmz;
-      if (inf.err_cheb > bd->eps && !refit_done)
+      if (!keep) inf.err_cheb = fabs (st.zmz - st.yy) / st.zmz;
+      if (!keep && inf.err_cheb > bd->eps && !refit_done)
         {
-          /* brownian.c:1707-1727: new bounds at x, same draw */
-          rc = bd_fit_bounds (bd, c, x_dev);
+          /* brownian.c:1707-1727: new bounds where the noise is taken,
+           * same draw */
+          rc = bd_fit_bounds (bd, c, stale ? bd->xs : x_dev);
           if (rc) goto out;
           inf.refits++;
           refit_done = 1;
@@ -818,16 +838,28 @@ stokes_hip_bd_step_q (stokes_hip_bd *bd, double *x_dev, double *q_dev,
       inf.halvings++;
       if (dt_local < bd->dt_lim)
         {
+          /* REDO_FB (stage 1: the positions stay) or REDO (stage 2: x) */
           dt_local = dt;
           if (bd->fact_fix) fact = bd_fact (bd->peclet, dt_local);
           inf.redraws++;
           fresh = 1;
           refit_done = 0;
+          keep = 0;
+          if (stage == 2) stale = 0;
         }
       else
         {
+          /* REDO_scale: the noise kept; after the second stage the system
+           * holds this attempt's intermediate configuration */
           fact = bd_fact (bd->peclet, dt_local);
           fresh = 0;
+          if (stage == 2)
+            {
+              rc = shk_dev_d2d (d, bd->xs, bd->xm, sizeof (double) * 3 * bd->nm);
+              if (rc) { rc = bd_fail (rc, "bd_step"); goto out; }
+              stale = 1;
+            }
+          keep = stale;
         }
     }
   /* accepted: x <- x_new */

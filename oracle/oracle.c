@@ -1319,8 +1319,13 @@ redo_fb:                           /* REDO_FB */
                                   orc_bd_op_nm_cb, (void *) &ctx);
   k++;
 redo_scale:                        /* REDO_scale */
+  /* the positions are not set here (REDO_scale, brownian.c:2400, 2637,
+   * 3025): after an overlap in the second stage with dt / 2 still above
+   * dt_lim (brownian.c:2523, 2881, 3162) the system keeps the intermediate
+   * configuration of the failed attempt, and the first-stage product of
+   * the retry is taken there; the noise is kept (computed where it was
+   * drawn) */
   for (int i = 0; i < n3; i++) xmid[i] = x[i];
-  SETPOS (x);
   if (scheme == 1)
     {
       FORCES (0, 1.0);

@@ -619,3 +619,58 @@ def test_bd_reference_random_stream():
         assert e < 1e-10
         x = xo
     bd.close()
+
+
+@pytest.mark.parametrize("version", [0, 1, 2])
+def test_bd_retry_after_second_stage_overlap(version):
+    """eight spheres, two of them pushed together so that the second stage
+    of the first attempt overlaps while dt / 2 stays above dt_lim: the retry
+    keeps the noise and takes its first-stage product at the intermediate
+    configuration of the failed attempt (REDO_scale, brownian.c:2400,
+    2523), an eager attempt on the device; against the oracle's step fed
+    with the device's draw"""
+    L, gap = 8.0, 0.05
+    pos = np.array([[1, 1, 1], [4, 1.2, 0.8], [1.1, 4.2, 1], [4, 4, 1.3],
+                    [1, 1.1, 4], [4.2, 1, 4], [1, 4, 4.1], [4, 4.1, 4.2]],
+                   dtype=np.float64) + 0.3
+    pos[1] = pos[0] + np.array([2.0 + gap, 0.0, 0.0])
+    _say(f"bd second-stage retry version={version}")
+    s = ls.Stokes(version, pos, box=(L, L, L), tr=1.0)
+    o = po.System(version, pos, box=(L, L, L), tr=1.0)
+    d = 3 if version == 0 else 6
+    n = 8
+    if version < 2:
+        G = o.matrix()
+    else:
+        G = np.stack([o.fts_mix_ft(n, np.eye(6 * n)[c]) for c in range(6 * n)],
+                     axis=1)
+    w = np.linalg.eigvalsh(0.5 * (G + G.T))
+    e0, e1 = 0.95 * w[0], 1.05 * w[-1]
+    nch, pe, dt = 30, 1e6, 1e-3
+    a = po.cheb_coef(nch, 0, e0, e1)
+    F = np.zeros(3 * n)
+    F[0], F[3] = 100.0, -100.0
+    T = np.zeros(3 * n) if version > 0 else None
+    bd = ls.BD(s, peclet=pe, n_cheb=nch, eps=1.0, dt_lim=1e-6, rmin_ol=1.0,
+               seed=5, force_dev=ls.DeviceArray.from_host(F), bounds=(e0, e1))
+    if version > 0:
+        bd.set_torque(ls.DeviceArray.from_host(T))
+    if version == 2:
+        s.set_iter(eps=1e-13)
+        bd.set_fts_iter(30, 2000, 1e-13)
+    xd = ls.DeviceArray.from_host(pos.reshape(-1))
+    info = bd.step(xd, dt)
+    y, _ = bd.last_noise()
+    assert y.size == d * n
+    xo, _, dto, used, nh, _ = o.bd_mid(pos.reshape(-1), y[None], pe, dt, nch,
+                                       a, e0, e1, F=F, T=T)
+    _say(f"  halvings gpu {info.halvings} oracle {nh}, dt {info.dt} {dto}, "
+         f"overlaps of the last attempt {info.overlaps[0]} {info.overlaps[1]}")
+    assert info.halvings == nh == 1 and info.redraws == 0 and used == 1
+    assert info.dt == dto == dt / 2 and info.attempts == 2
+    xg = xd.to_host()
+    disp = np.abs(xo - pos.reshape(-1)).max()
+    ex = np.abs(xg - xo).max() / disp
+    _say(f"  x err / disp {ex:.2e}")
+    assert ex < (1e-8 if version == 2 else 1e-10)
+    bd.close()

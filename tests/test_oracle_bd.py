@@ -136,7 +136,11 @@ def _two_spheres(gap, L=10.0):
 
 def test_bd_overlap_halving_keeps_draw():
     """a large first step that overlaps: dt is halved (fact recomputed) and
-    the same draw is reused until the step is accepted"""
+    the same draw is reused until the step is accepted.  Here the first two
+    attempts overlap in the first stage, the third in the second, so the
+    accepted attempt takes its first-stage product at the third attempt's
+    intermediate configuration (REDO_scale sets no positions,
+    brownian.c:2400, 2523)"""
     s, pos = _two_spheres(2.05)
     y = np.zeros(6)
     F = np.array([50.0, 0, 0, -50.0, 0, 0])    # pushes the pair together
@@ -144,14 +148,37 @@ def test_bd_overlap_halving_keeps_draw():
     a = po.cheb_coef(nch, 0, 0.3, 1.5)
     xn, dtt, used, nh, err = s.bd_mid_F(pos, y[None], 1.0, 0.01, nch, a,
                                         0.3, 1.5, F=F, dt_lim=1e-8)
-    assert used == 1 and nh >= 1
+    assert used == 1 and nh == 3
     assert dtt == pytest.approx(0.01 / 2**nh)
     assert s.check_overlap(xn, 1.0) == 0
-    # the accepted step is the plain mid-point step at the final dt
-    x2, dt2, _, nh2, _ = s.bd_mid_F(pos, y[None], 1.0, dtt, nch, a, 0.3, 1.5,
+    # the same control flow written out with the operator (no noise)
+    def at(p, f):
+        keep = s.pos.copy()
+        s.pos[:] = p.reshape(s.pos.shape)
+        u = s.atimes(f)
+        s.pos[:] = keep
+        return u
+
+    x, dt, P, stages = pos.reshape(-1), 0.01, pos.reshape(-1), []
+    for _ in range(8):
+        x1 = x + 0.5 * dt * at(P, F)
+        if s.check_overlap(x1, 1.0):
+            stages.append(1)
+            dt /= 2
+            continue
+        P = x1
+        x2 = x + dt * at(x1, F)
+        if s.check_overlap(x2, 1.0):
+            stages.append(2)
+            dt /= 2
+            continue
+        break
+    assert stages == [1, 1, 2] and dt == dtt
+    assert np.abs(xn - x2).max() < 1e-14 * np.abs(x2 - x).max()
+    # the plain mid-point step at the final dt differs
+    x3, dt3, _, nh3, _ = s.bd_mid_F(pos, y[None], 1.0, dtt, nch, a, 0.3, 1.5,
                                     F=F, dt_lim=1e-8)
-    assert nh2 == 0 and dt2 == dtt
-    assert np.array_equal(xn, x2)
+    assert nh3 == 0 and not np.allclose(xn, x3, rtol=0, atol=1e-9)
 
 
 def test_bd_overlap_redraw_below_dt_lim():
@@ -462,3 +489,64 @@ def test_bd_step_fixed_particles_composition(version):
     U1 = u1 if version == 0 else u1.reshape(nm, 6)[:, :3].reshape(-1)
     want = x0[:3 * nm] + dt * U1
     assert np.abs(xn - want).max() < 1e-12 * np.abs(want - x0[:3 * nm]).max()
+
+
+def _approaching_pair(L, gap):
+    """_eight with particles 0 and 1 at centre distance 2 + gap along x"""
+    pos = _eight(L)
+    pos[1] = pos[0] + np.array([2.0 + gap, 0.0, 0.0])
+    return pos
+
+
+def test_bd_retry_after_second_stage_overlap_keeps_positions():
+    """a mid-point step whose second stage overlaps while dt / 2 stays above
+    dt_lim: the retry (REDO_scale, brownian.c:2400, 2523) keeps the noise
+    and takes its first-stage product at the intermediate configuration of
+    the failed attempt, which the system still holds; composed here from the
+    compiled reference's atimes_3all, and unlike the composition that takes
+    it at x"""
+    import ctypes as C
+    L, gap = 8.0, 0.05
+    pos = _approaching_pair(L, gap)
+    s = po.System(0, pos, box=(L, L, L), tr=1.0)
+    n = 24
+    y = np.random.default_rng(44).standard_normal(n)
+    F = np.zeros(n)
+    F[0], F[3] = 100.0, -100.0
+    pe, dt, nch = 1e6, 1e-3, 30
+    G = s.matrix()
+    w = np.linalg.eigvalsh(0.5 * (G + G.T))
+    e0, e1 = 0.95 * w[0], 1.05 * w[-1]
+    a = po.cheb_coef(nch, 0, e0, e1)
+    xn, _, dtt, used, nh, err = s.bd_mid(pos, y[None], pe, dt, nch, a, e0,
+                                         e1, F=F)
+    assert used == 1 and nh == 1 and dtt == dt / 2
+    R = po.ref()
+
+    def M(p, f):
+        t = po.System(0, p.reshape(-1, 3), box=(L, L, L), tr=1.0)
+        h = R.ref_sys_new(C.byref(t.s))
+        u = np.zeros(n)
+        R.ref_atimes(h, n, po._p(np.ascontiguousarray(f)), po._p(u))
+        R.ref_sys_free(h)
+        return u
+
+    def contact(p):
+        d = p.reshape(-1, 3)
+        return np.linalg.norm(d[1] - d[0]) <= 2.0
+
+    x = pos.reshape(-1)
+    z = po.cheb_eval_dense(a, G, y, e0, e1)
+    f = F + math.sqrt(2 / (pe * dt)) * z
+    x1 = x + 0.5 * dt * M(x, f)
+    assert not contact(x1)
+    assert contact(x + dt * M(x1, f))            # the second stage overlaps
+    h = dt / 2
+    f = F + math.sqrt(2 / (pe * h)) * z
+    xm = x + 0.5 * h * M(x1, f)                   # first stage at x1
+    want = x + h * M(xm, f)
+    xm_x = x + 0.5 * h * M(x, f)                  # the same taken at x
+    other = x + h * M(xm_x, f)
+    disp = np.abs(want - x).max()
+    assert np.abs(xn - want).max() < 1e-12 * disp
+    assert np.abs(other - want).max() > 1e-6 * disp
