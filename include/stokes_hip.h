@@ -206,21 +206,25 @@ void stokes_hip_ensemble_destroy (stokes_hip_ensemble *e);
 
 /* ---- Brownian dynamics ------------------------------------------------
  * The mid-point step of BD_evolve_mid (brownian.c:2352-2560) for the F, FT
- * and FTS versions with every particle mobile (the BD_params fields it
- * reads are the arguments of stokes_hip_bd_create).  M is the mobility of
+ * and FTS versions (the BD_params fields it reads are the arguments of
+ * stokes_hip_bd_create), in a fluid at rest.  M is the mobility of
  * BD_atimes_mob_FU (brownian.c:494-739): F -> U (F), (F, T) -> (U, O)
  * particle-wise (FT), and for FTS the mobility problem with E = 0 solved
- * by GMRES for each product.  For F and FT an attempt runs on the GPU as
- * one captured graph (FTS: the same launches, eagerly, the GMRES steered
- * from the host): Gaussian draw y, z = M^{-1/2} y by the Chebyshev
- * recursion (BD_calc_FB, brownian.c:1545-1874), its accuracy
- * |z.Mz - y.y| / z.Mz, drift at x, Euler half step, overlap count, drift
- * at the mid point, Euler full step, overlap count.  The host then takes
- * the reference's decisions: refit the spectral bounds once when the
- * accuracy exceeds eps (brownian.c:1707-1727, FP64 Lanczos in place of
- * ARPACK), halve dt on overlap and retry with the same noise, or redraw
- * the noise and restore dt once dt falls below dt_lim
- * (brownian.c:2435-2459, 2501-2525).
+ * by GMRES for each product; the drift is that of solve_mix_3all, which
+ * with fixed particles holds them at rest.  For F and FT with every
+ * particle mobile an attempt runs on the GPU as one captured graph (FTS,
+ * and fixed particles: the same launches, eagerly, the GMRES steered from
+ * the host): Gaussian draw y, z = M^{-1/2} y by the Chebyshev recursion
+ * (BD_calc_FB, brownian.c:1545-1874), its accuracy |z.Mz - y.y| / z.Mz,
+ * drift at x, Euler half step, overlap count, drift at the mid point,
+ * Euler full step, overlap count.  The host then takes the reference's
+ * decisions: refit the spectral bounds once when the accuracy exceeds eps
+ * (brownian.c:1707-1727, FP64 Lanczos in place of ARPACK), halve dt on
+ * overlap and retry with the same noise, or redraw the noise and restore
+ * dt once dt falls below dt_lim (brownian.c:2435-2459, 2501-2525).  As in
+ * the reference, a retry after an overlap in the second stage takes its
+ * first-stage product at the failed attempt's intermediate configuration
+ * (REDO_scale sets no positions, brownian.c:2400, 2523).
  *
  * Noise: Philox4x32-10 with Box-Muller (the reference draws from MT19937
  * with the polar method, KIrand.c:108-135), so trajectories follow the
@@ -249,7 +253,8 @@ typedef struct stokes_hip_bd_info {
  * (its products are matrix-free). */
 int stokes_hip_bd_set_real_cache (stokes_hip_bd *bd, int enable);
 
-/* sys: version 0 (F), 1 (FT) or 2 (FTS), no shear, set up as for
+/* sys: version 0 (F), 1 (FT) or 2 (FTS), no shear and no imposed flow
+ * (Ui, Oi, Ei zero: refused here and at each step), set up as for
  * atimes_3all; particles nm..np-1 (stokes_set_np) are fixed: the step
  * moves x [3 nm] (and q [4 nm]), the fixed positions are read from sys->pos
  * at each step.  Unequal radii (sys->a, read at creation): the noise

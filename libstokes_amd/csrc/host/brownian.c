@@ -135,6 +135,20 @@ stokes_hip_bd_destroy (stokes_hip_bd *bd)
   free (bd);
 }
 
+/* the step's drift is solve_mix_3all in the labo frame (ewald-3f.c:
+ * 269-321 and its FT / FTS forms), which adds the imposed flow Ui + Oi x
+ * x + Ei . x (and Oi, and shifts E by Ei for FTS); the integrator here
+ * takes the fluid at rest, so a system with an imposed flow is refused */
+static int
+bd_fluid_at_rest (const struct stokes *sys)
+{
+  for (int k = 0; k < 3; k++)
+    if (sys->Ui[k] != 0.0 || sys->Oi[k] != 0.0) return 0;
+  for (int k = 0; k < 5; k++)
+    if (sys->Ei[k] != 0.0) return 0;
+  return 1;
+}
+
 int
 stokes_hip_bd_create (struct stokes *sys, const double *F_dev, double peclet,
                       int n_cheb, double eps, double dt_lim, double rmin_ol,
@@ -154,6 +168,12 @@ stokes_hip_bd_create (struct stokes *sys, const double *F_dev, double peclet,
   if (sys->shear_mode != 0 || sys->nm < 1 || sys->nm > sys->np)
     {
       stokes_hip_set_error ("bd_create: needs no shear and 1 <= nm <= np");
+      return STOKES_HIP_EUNSUP;
+    }
+  if (!bd_fluid_at_rest (sys))
+    {
+      stokes_hip_set_error ("bd_create: the integrator takes the fluid at rest"
+                            " (Ui, Oi, Ei all zero)");
       return STOKES_HIP_EUNSUP;
     }
   if (n_cheb < 2 || !(eps > 0.0) || dt_lim < 0.0 || sys->np <= 0)
@@ -715,6 +735,12 @@ stokes_hip_bd_step_q (stokes_hip_bd *bd, double *x_dev, double *q_dev,
       return STOKES_HIP_EARG;
     }
   if (bd->ver == 0) q_dev = NULL;      /* evolve_Euler_3all ignores q for F */
+  if (bd->sys->shear_mode != 0 || !bd_fluid_at_rest (bd->sys))
+    {
+      stokes_hip_set_error ("bd_step: the integrator takes the fluid at rest"
+                            " (no shear, Ui, Oi, Ei all zero)");
+      return STOKES_HIP_EUNSUP;
+    }
   stokes_ctx *c = stokes_hip_ctx (bd->sys);
   if (!c) return STOKES_HIP_ENODEV;
   shk_dev *d = c->dev;
