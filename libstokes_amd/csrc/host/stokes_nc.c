@@ -1,0 +1,576 @@
+/* stokes_nc.c -- libstokes trajectory files in the netCDF classic format
+ * with 64-bit offsets (CDF-2), written without the netCDF library.
+ *
+ * The file layout (dimensions, variables, their long_name attributes, the
+ * flag table of stokes_nc_init and the index variables filled at the end
+ * of the definitions) is that of the reference's stokes-nc.c:716-1366 and
+ * 1445-1598, so the files read the same in tools that read the reference's
+ * files.  The encoding is the published netCDF classic format: a header
+ * ("CDF" 2, the record count, the dimension list, an absent global
+ * attribute list, the variable list with each variable's dimension ids,
+ * attributes, type, padded size vsize and 8-byte begin offset), then the
+ * fixed-size variables one after the other, then the records, each record
+ * holding every record variable's slab in definition order; all values
+ * big-endian.  As the netCDF library does in its default fill mode,
+ * fixed-size variables and new records are filled with the type's fill
+ * value before data are written.  Errors print a message and exit, as
+ * stokes_nc_error does (stokes-nc.c:33-48).
+ */
+#define _POSIX_C_SOURCE 200809L
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include "stokes-nc.h"
+
+enum { NCT_CHAR = 2, NCT_INT = 4, NCT_DOUBLE = 6 };
+enum { NC_TAG_DIM = 0x0A, NC_TAG_VAR = 0x0B, NC_TAG_ATT = 0x0C };
+#define NC_MAXDIM 8
+#define NC_MAXVAR 72
+/* default fill values of the classic format */
+#define FILL_DOUBLE 9.9692099683868690e+36
+#define FILL_INT (-2147483647)
+
+typedef struct {
+  char name[16];
+  int len;                      /* 0: the record dimension */
+} nc_dim;
+
+typedef struct {
+  char name[16];
+  int type;
+  int nd;
+  int dim[3];
+  const char *longname;         /* NULL: no attribute */
+  int rec;                      /* first dimension is the record one */
+  size_t count;                 /* values (per record for record vars) */
+  size_t vsize;                 /* bytes, padded to 4 */
+  long long begin;              /* file offset (record vars: of record 0) */
+} nc_var;
+
+struct stokes_nc {
+  FILE *fp;
+  const char *fname;
+  int np, npf;
+  int ndims, nvars;
+  nc_dim dims[NC_MAXDIM];
+  nc_var vars[NC_MAXVAR];
+  int time_dim;
+  int numrecs;
+  long long recsize;
+};
+
+static void
+nc_die (const char *fname, const char *what, const char *name)
+{
+  fprintf (stderr, "libstokes-hip stokes_nc (%s): %s%s%s\n",
+           fname ? fname : "?", what, name ? " for " : "", name ? name : "");
+  exit (1);
+}
+
+/* ---- big-endian encoding ---- */
+
+static void
+be32 (unsigned char *p, uint32_t v)
+{
+  p[0] = (unsigned char) (v >> 24);
+  p[1] = (unsigned char) (v >> 16);
+  p[2] = (unsigned char) (v >> 8);
+  p[3] = (unsigned char) v;
+}
+
+static void
+be64 (unsigned char *p, uint64_t v)
+{
+  be32 (p, (uint32_t) (v >> 32));
+  be32 (p + 4, (uint32_t) v);
+}
+
+/* a growing byte buffer for the header */
+typedef struct {
+  unsigned char *b;
+  size_t n, cap;
+} nc_buf;
+
+static void
+buf_put (nc_buf *h, const void *p, size_t n)
+{
+  if (h->n + n > h->cap)
+    {
+      size_t cap = h->cap ? 2 * h->cap : 1024;
+      while (cap < h->n + n) cap *= 2;
+      unsigned char *b = realloc (h->b, cap);
+      if (!b) nc_die (NULL, "header allocation failed", NULL);
+      h->b = b;
+      h->cap = cap;
+    }
+  memcpy (h->b + h->n, p, n);
+  h->n += n;
+}
+
+static void
+buf_u32 (nc_buf *h, uint32_t v)
+{
+  unsigned char t[4];
+  be32 (t, v);
+  buf_put (h, t, 4);
+}
+
+static void
+buf_u64 (nc_buf *h, uint64_t v)
+{
+  unsigned char t[8];
+  be64 (t, v);
+  buf_put (h, t, 8);
+}
+
+/* name or text: length, bytes, zero padding to 4 */
+static void
+buf_text (nc_buf *h, const char *s)
+{
+  size_t n = strlen (s);
+  static const unsigned char zero[4] = {0, 0, 0, 0};
+  buf_u32 (h, (uint32_t) n);
+  buf_put (h, s, n);
+  if (n % 4) buf_put (h, zero, 4 - n % 4);
+}
+
+/* ---- define mode ---- */
+
+static int
+def_dim (struct stokes_nc *nc, const char *name, int len)
+{
+  if (nc->ndims >= NC_MAXDIM) nc_die (nc->fname, "too many dimensions", name);
+  nc_dim *d = &nc->dims[nc->ndims];
+  snprintf (d->name, sizeof (d->name), "%s", name);
+  d->len = len;
+  return nc->ndims++;
+}
+
+static void
+def_var (struct stokes_nc *nc, const char *name, int type, int nd,
+         const int *dims, const char *longname)
+{
+  if (nc->nvars >= NC_MAXVAR) nc_die (nc->fname, "too many variables", name);
+  nc_var *v = &nc->vars[nc->nvars++];
+  memset (v, 0, sizeof (*v));
+  snprintf (v->name, sizeof (v->name), "%s", name);
+  v->type = type;
+  v->nd = nd;
+  v->count = 1;
+  for (int k = 0; k < nd; k++)
+    {
+      v->dim[k] = dims[k];
+      if (nc->dims[dims[k]].len == 0)
+        v->rec = k == 0;
+      else
+        v->count *= (size_t) nc->dims[dims[k]].len;
+    }
+  v->longname = longname;
+  const size_t bytes = v->count * (type == NCT_INT ? 4 : 8);
+  v->vsize = (bytes + 3) / 4 * 4;
+}
+
+static nc_var *
+var_of (struct stokes_nc *nc, const char *name)
+{
+  for (int k = 0; k < nc->nvars; k++)
+    if (strcmp (nc->vars[k].name, name) == 0) return &nc->vars[k];
+  nc_die (nc->fname, "variable not defined in this file", name);
+  return NULL;
+}
+
+/* write count values of v (doubles, or ints when v is NC_INT) at offset */
+static void
+write_values (struct stokes_nc *nc, const nc_var *v, long long off,
+              const double *d, const int *iv)
+{
+  unsigned char *b = malloc (v->vsize ? v->vsize : 4);
+  if (!b) nc_die (nc->fname, "allocation failed", v->name);
+  memset (b, 0, v->vsize);
+  for (size_t k = 0; k < v->count; k++)
+    {
+      if (v->type == NCT_INT)
+        be32 (b + 4 * k, (uint32_t) (iv ? iv[k] : FILL_INT));
+      else
+        {
+          double x = d ? d[k] : FILL_DOUBLE;
+          uint64_t u;
+          memcpy (&u, &x, 8);
+          be64 (b + 8 * k, u);
+        }
+    }
+  if (fseeko (nc->fp, (off_t) off, SEEK_SET) != 0
+      || fwrite (b, 1, v->vsize, nc->fp) != v->vsize)
+    nc_die (nc->fname, "write failed", v->name);
+  free (b);
+}
+
+/* leave define mode: header, offsets, fill of the fixed-size variables */
+static void
+end_def (struct stokes_nc *nc)
+{
+  /* the header size is independent of the begin values (8 bytes each) */
+  nc_buf h = {0};
+  for (int pass = 0; pass < 2; pass++)
+    {
+      h.n = 0;
+      buf_put (&h, "CDF\002", 4);
+      buf_u32 (&h, 0);                         /* numrecs */
+      buf_u32 (&h, NC_TAG_DIM);
+      buf_u32 (&h, (uint32_t) nc->ndims);
+      for (int k = 0; k < nc->ndims; k++)
+        {
+          buf_text (&h, nc->dims[k].name);
+          buf_u32 (&h, (uint32_t) nc->dims[k].len);
+        }
+      buf_u32 (&h, 0);                         /* no global attributes */
+      buf_u32 (&h, 0);
+      buf_u32 (&h, NC_TAG_VAR);
+      buf_u32 (&h, (uint32_t) nc->nvars);
+      for (int k = 0; k < nc->nvars; k++)
+        {
+          const nc_var *v = &nc->vars[k];
+          buf_text (&h, v->name);
+          buf_u32 (&h, (uint32_t) v->nd);
+          for (int j = 0; j < v->nd; j++) buf_u32 (&h, (uint32_t) v->dim[j]);
+          if (v->longname)
+            {
+              buf_u32 (&h, NC_TAG_ATT);
+              buf_u32 (&h, 1);
+              buf_text (&h, "long_name");
+              buf_u32 (&h, NCT_CHAR);
+              buf_text (&h, v->longname);
+            }
+          else
+            {
+              buf_u32 (&h, 0);
+              buf_u32 (&h, 0);
+            }
+          buf_u32 (&h, (uint32_t) v->type);
+          buf_u32 (&h, (uint32_t) v->vsize);
+          buf_u64 (&h, (uint64_t) v->begin);
+        }
+      if (pass == 1) break;
+      /* offsets: the fixed-size variables after the header, then the
+       * records */
+      long long off = (long long) h.n;
+      for (int k = 0; k < nc->nvars; k++)
+        if (!nc->vars[k].rec)
+          {
+            nc->vars[k].begin = off;
+            off += (long long) nc->vars[k].vsize;
+          }
+      nc->recsize = 0;
+      for (int k = 0; k < nc->nvars; k++)
+        if (nc->vars[k].rec)
+          {
+            nc->vars[k].begin = off + nc->recsize;
+            nc->recsize += (long long) nc->vars[k].vsize;
+          }
+    }
+  if (fwrite (h.b, 1, h.n, nc->fp) != h.n)
+    nc_die (nc->fname, "header write failed", NULL);
+  free (h.b);
+  for (int k = 0; k < nc->nvars; k++)
+    if (!nc->vars[k].rec)
+      write_values (nc, &nc->vars[k], nc->vars[k].begin, NULL, NULL);
+}
+
+/* ---- data ---- */
+
+static void
+put_fixed (struct stokes_nc *nc, const char *name, const double *d,
+           const int *iv)
+{
+  const nc_var *v = var_of (nc, name);
+  if (v->rec) nc_die (nc->fname, "a record variable needs a step", name);
+  write_values (nc, v, v->begin, d, iv);
+}
+
+static void
+put_rec (struct stokes_nc *nc, const char *name, int step, const double *d,
+         const int *iv)
+{
+  const nc_var *v = var_of (nc, name);
+  if (!v->rec) nc_die (nc->fname, "not a record variable", name);
+  if (step < 0) nc_die (nc->fname, "negative step", name);
+  if (step >= nc->numrecs)
+    {
+      /* new records: every record variable filled first */
+      for (int r = nc->numrecs; r <= step; r++)
+        for (int k = 0; k < nc->nvars; k++)
+          if (nc->vars[k].rec)
+            write_values (nc, &nc->vars[k],
+                          nc->vars[k].begin + (long long) r * nc->recsize,
+                          NULL, NULL);
+      nc->numrecs = step + 1;
+      unsigned char t[4];
+      be32 (t, (uint32_t) nc->numrecs);
+      if (fseeko (nc->fp, 4, SEEK_SET) != 0 || fwrite (t, 1, 4, nc->fp) != 4)
+        nc_die (nc->fname, "record count write failed", name);
+    }
+  write_values (nc, v, v->begin + (long long) step * nc->recsize, d, iv);
+}
+
+/* ---- stokes_nc_init_ (stokes-nc.c:716-1366) ---- */
+
+typedef struct {
+  int ui0, oi0, ei0, ui, oi, ei;
+  int x0, u0, o0, e0, f0, t0, s0;
+  int xf0, uf0, of0, ef0, ff0, tf0, sf0;
+  int x, u, o, e, f, t, s, q;
+  int xf, uf, of, ef, ff, tf, sf;
+  int a, af;
+} nc_flags;
+
+static struct stokes_nc *
+nc_create (const char *filename, int nm, int nf, const nc_flags *fl,
+           int shear_mode, int flag_rng)
+{
+  struct stokes_nc *nc = calloc (1, sizeof (*nc));
+  if (!nc) nc_die (filename, "allocation failed", NULL);
+  nc->fname = filename;
+  nc->np = nm;
+  nc->npf = nf;
+  /* NC_NOCLOBBER: an existing file is an error */
+  nc->fp = fopen (filename, "wxb");
+  if (!nc->fp) nc_die (filename, "cannot create (exists, or no access)", NULL);
+  int dv[3];
+  const int p_dim = def_dim (nc, "p", nm);
+  dv[0] = p_dim;
+  def_var (nc, "p", NCT_INT, 1, dv, NULL);
+  int pf_dim = -1;
+  if (nf > 0)
+    {
+      pf_dim = def_dim (nc, "pf", nf);
+      dv[0] = pf_dim;
+      def_var (nc, "pf", NCT_INT, 1, dv, NULL);
+    }
+  const int vec_dim = def_dim (nc, "vec", 3);
+  dv[0] = vec_dim;
+  def_var (nc, "vec", NCT_INT, 1, dv, NULL);
+  const int stt_dim = def_dim (nc, "stt", 5);
+  dv[0] = stt_dim;
+  def_var (nc, "stt", NCT_INT, 1, dv, NULL);
+  const int quat_dim = def_dim (nc, "quat", 4);
+  dv[0] = quat_dim;
+  def_var (nc, "quat", NCT_INT, 1, dv, NULL);
+  int rng_dim = -1;
+  if (flag_rng)
+    {
+      rng_dim = def_dim (nc, "rng", 624);      /* MT19937 state words */
+      dv[0] = rng_dim;
+      def_var (nc, "rng", NCT_INT, 1, dv, NULL);
+    }
+  const int t = def_dim (nc, "time", 0);
+  nc->time_dim = t;
+  dv[0] = t;
+  def_var (nc, "time", NCT_DOUBLE, 1, dv, NULL);
+
+#define V1(n, a, ln) do { int d_[1] = {a}; def_var (nc, n, NCT_DOUBLE, 1, d_, ln); } while (0)
+#define V2(n, a, b, ln) do { int d_[2] = {a, b}; def_var (nc, n, NCT_DOUBLE, 2, d_, ln); } while (0)
+#define V3(n, a, b, c, ln) do { int d_[3] = {a, b, c}; def_var (nc, n, NCT_DOUBLE, 3, d_, ln); } while (0)
+  V1 ("l", vec_dim, "lattice vector");
+  def_var (nc, "shear_mode", NCT_INT, 0, dv, "shear mode");
+  if (shear_mode != 0)
+    {
+      def_var (nc, "shear_rate", NCT_DOUBLE, 0, dv, "shear rate");
+      V1 ("shear_shift", t, "cell shift for shear boundary condition");
+    }
+  if (fl->ui0) V1 ("Ui0", vec_dim, "constant imposed translational velocity");
+  if (fl->oi0) V1 ("Oi0", vec_dim, "constant imposed angular velocity");
+  if (fl->ei0) V1 ("Ei0", stt_dim, "constant imposed strain");
+  if (fl->ui) V2 ("Ui", t, vec_dim, "(changing) imposed translational velocity");
+  if (fl->oi) V2 ("Oi", t, vec_dim, "(changing) imposed angular velocity");
+  if (fl->ei) V2 ("Ei", t, stt_dim, "(changing) imposed strain");
+  if (fl->x0) V2 ("x0", p_dim, vec_dim, "positions of particle center at t0");
+  if (fl->u0) V2 ("U0", p_dim, vec_dim, "translational velocities at t0");
+  if (fl->o0) V2 ("O0", p_dim, vec_dim, "angular velocities at t0");
+  if (fl->e0) V2 ("E0", p_dim, stt_dim, "rate of strain at t0");
+  if (fl->f0) V2 ("F0", p_dim, vec_dim, "forces at t0");
+  if (fl->t0) V2 ("T0", p_dim, vec_dim, "torques at t0");
+  if (fl->s0) V2 ("S0", p_dim, stt_dim, "stresslets at t0");
+  if (fl->x) V3 ("x", t, p_dim, vec_dim, "positions of particle center");
+  if (fl->q) V3 ("q", t, p_dim, quat_dim, "quaternion of particle direction");
+  if (fl->a) V1 ("a", p_dim, "radii of particle");
+  if (fl->u) V3 ("U", t, p_dim, vec_dim, "translational velocities");
+  if (fl->o) V3 ("O", t, p_dim, vec_dim, "angular velocities");
+  if (fl->e) V3 ("E", t, p_dim, stt_dim, "rate of strain");
+  if (fl->f) V3 ("F", t, p_dim, vec_dim, "forces");
+  if (fl->t) V3 ("T", t, p_dim, vec_dim, "torques");
+  if (fl->s) V3 ("S", t, p_dim, stt_dim, "stresslets");
+  if (nf > 0)
+    {
+      const int q = pf_dim;
+      if (fl->xf0) V2 ("xf0", q, vec_dim, "positions of fixed particle center at t0");
+      if (fl->af) V1 ("af", q, "radii of particle");
+      if (fl->uf0) V2 ("Uf0", q, vec_dim, "translational velocities for fixed particles at t0");
+      if (fl->of0) V2 ("Of0", q, vec_dim, "angular velocities for fixed particles at t0");
+      if (fl->ef0) V2 ("Ef0", q, stt_dim, "rate of strain for fixed particles at t0");
+      if (fl->ff0) V2 ("Ff0", q, vec_dim, "forces for fixed particles at t0");
+      if (fl->tf0) V2 ("Tf0", q, vec_dim, "torques for fixed particles at t0");
+      if (fl->sf0) V2 ("Sf0", q, stt_dim, "stresslets for fixed particles at t0");
+      if (fl->xf) V3 ("xf", t, q, vec_dim, "positions of fixed particle center");
+      if (fl->uf) V3 ("Uf", t, q, vec_dim, "translational velocities for fixed particles");
+      if (fl->of) V3 ("Of", t, q, vec_dim, "angular velocities for fixed particles");
+      if (fl->ef) V3 ("Ef", t, q, stt_dim, "rate of strain for fixed particles");
+      if (fl->ff) V3 ("Ff", t, q, vec_dim, "forces for fixed particles");
+      if (fl->tf) V3 ("Tf", t, q, vec_dim, "torques for fixed particles");
+      if (fl->sf) V3 ("Sf", t, q, stt_dim, "stresslets for fixed particles");
+    }
+  if (flag_rng)
+    {
+      int d2[2] = {t, rng_dim};
+      def_var (nc, "mt", NCT_INT, 2, d2, "random number generator table");
+      def_var (nc, "mti", NCT_INT, 1, &t, "random number generator index");
+      def_var (nc, "mt_Ghs", NCT_INT, 1, &t, "Gaussian random number generator flag");
+      V1 ("mt_Gs", t, "Gaussian random number generator saved value");
+    }
+#undef V1
+#undef V2
+#undef V3
+  end_def (nc);
+
+  /* (0, 0, 0): a non-periodic system until stokes_nc_set_l */
+  const double zero[3] = {0.0, 0.0, 0.0};
+  put_fixed (nc, "l", zero, NULL);
+  const int nidx = nm > 624 ? nm : 624;
+  const int nmax = nidx > nf ? nidx : nf;
+  int *idx = malloc (sizeof (int) * (size_t) nmax);
+  if (!idx) nc_die (filename, "allocation failed", NULL);
+  for (int i = 0; i < nmax; i++) idx[i] = i;
+  put_fixed (nc, "p", NULL, idx);
+  put_fixed (nc, "vec", NULL, idx);
+  put_fixed (nc, "stt", NULL, idx);
+  put_fixed (nc, "quat", NULL, idx);
+  if (nf > 0) put_fixed (nc, "pf", NULL, idx);
+  if (flag_rng) put_fixed (nc, "rng", NULL, idx);
+  free (idx);
+  put_fixed (nc, "shear_mode", NULL, &shear_mode);
+  return nc;
+}
+
+struct stokes_nc *
+stokes_nc_x_init (const char *filename, int np)
+{
+  nc_flags fl;
+  memset (&fl, 0, sizeof (fl));
+  fl.x = 1;
+  return nc_create (filename, np, 0, &fl, 0, 0);
+}
+
+/* the flag table of stokes-nc.c:1445-1598 */
+struct stokes_nc *
+stokes_nc_init (const char *filename, int np, int nf, int version,
+                int flag_poly, int flag_Q, int flag_it, int shear_mode,
+                int flag_rng)
+{
+  nc_flags fl;
+  memset (&fl, 0, sizeof (fl));
+  if (flag_it == 0) fl.ui0 = fl.oi0 = fl.ei0 = 1;
+  else fl.ui = fl.oi = fl.ei = 1;
+  fl.q = flag_Q;
+  fl.a = flag_poly;
+  fl.f0 = fl.x = fl.u = 1;
+  if (version >= 1) fl.t0 = fl.o = 1;
+  if (version == 2) fl.e0 = fl.s = 1;
+  if (nf > 0)
+    {
+      fl.af = flag_poly;
+      fl.xf0 = fl.uf0 = fl.ff = 1;
+      if (version >= 1) fl.of0 = fl.tf = 1;
+      if (version == 2) fl.ef0 = fl.sf = 1;
+    }
+  return nc_create (filename, np, nf, &fl, shear_mode, flag_rng);
+}
+
+void
+stokes_nc_free (struct stokes_nc *nc)
+{
+  if (!nc) return;
+  if (fclose (nc->fp) != 0) nc_die (nc->fname, "close failed", NULL);
+  free (nc);
+}
+
+/* ---- setters (stokes-nc.c:1625-2645) ---- */
+
+void
+stokes_nc_set_shear_rate (struct stokes_nc *nc, double shear_rate)
+{
+  put_fixed (nc, "shear_rate", &shear_rate, NULL);
+}
+
+#define FIXED(fn, var)                                                     \
+  void fn (struct stokes_nc *nc, const double *v) { put_fixed (nc, var, v, NULL); }
+#define RECORD(fn, var)                                                    \
+  void fn (struct stokes_nc *nc, int step, const double *v)                \
+  { put_rec (nc, var, step, v, NULL); }
+
+FIXED (stokes_nc_set_l, "l")
+FIXED (stokes_nc_set_ui0, "Ui0")
+FIXED (stokes_nc_set_oi0, "Oi0")
+FIXED (stokes_nc_set_ei0, "Ei0")
+FIXED (stokes_nc_set_x0, "x0")
+FIXED (stokes_nc_set_u0, "U0")
+FIXED (stokes_nc_set_o0, "O0")
+FIXED (stokes_nc_set_e0, "E0")
+FIXED (stokes_nc_set_f0, "F0")
+FIXED (stokes_nc_set_t0, "T0")
+FIXED (stokes_nc_set_s0, "S0")
+FIXED (stokes_nc_set_xf0, "xf0")
+FIXED (stokes_nc_set_uf0, "Uf0")
+FIXED (stokes_nc_set_of0, "Of0")
+FIXED (stokes_nc_set_ef0, "Ef0")
+FIXED (stokes_nc_set_ff0, "Ff0")
+FIXED (stokes_nc_set_tf0, "Tf0")
+FIXED (stokes_nc_set_sf0, "Sf0")
+FIXED (stokes_nc_set_a, "a")
+FIXED (stokes_nc_set_af, "af")
+RECORD (stokes_nc_set_ui, "Ui")
+RECORD (stokes_nc_set_oi, "Oi")
+RECORD (stokes_nc_set_ei, "Ei")
+RECORD (stokes_nc_set_x, "x")
+RECORD (stokes_nc_set_q, "q")
+RECORD (stokes_nc_set_u, "U")
+RECORD (stokes_nc_set_o, "O")
+RECORD (stokes_nc_set_e, "E")
+RECORD (stokes_nc_set_f, "F")
+RECORD (stokes_nc_set_t, "T")
+RECORD (stokes_nc_set_s, "S")
+RECORD (stokes_nc_set_xf, "xf")
+RECORD (stokes_nc_set_uf, "Uf")
+RECORD (stokes_nc_set_of, "Of")
+RECORD (stokes_nc_set_ef, "Ef")
+RECORD (stokes_nc_set_ff, "Ff")
+RECORD (stokes_nc_set_tf, "Tf")
+RECORD (stokes_nc_set_sf, "Sf")
+
+void
+stokes_nc_set_time (struct stokes_nc *nc, int step, double time)
+{
+  put_rec (nc, "time", step, &time, NULL);
+}
+
+void
+stokes_nc_set_shear_shift (struct stokes_nc *nc, int step, double shear_shift)
+{
+  put_rec (nc, "shear_shift", step, &shear_shift, NULL);
+}
+
+/* the whole MT19937 state (the reference stores half of it on LP64,
+ * stokes-nc.c:2667-2671): the 624 words as their 32-bit patterns, mti, and
+ * the polar method's saved-value flag and value */
+void
+stokes_nc_set_rng_state (struct stokes_nc *nc, int step, const uint32_t *mt,
+                         int mti, int gauss_has_saved, double gauss_saved)
+{
+  int *w = malloc (sizeof (int) * 624);
+  if (!w) nc_die (nc->fname, "allocation failed", "mt");
+  for (int k = 0; k < 624; k++) memcpy (&w[k], &mt[k], 4);
+  put_rec (nc, "mt", step, NULL, w);
+  free (w);
+  put_rec (nc, "mti", step, NULL, &mti);
+  put_rec (nc, "mt_Ghs", step, NULL, &gauss_has_saved);
+  put_rec (nc, "mt_Gs", step, &gauss_saved, NULL);
+}
