@@ -2,13 +2,14 @@
  * of the reference's stokes-nc.h).  The files are netCDF classic with
  * 64-bit offsets, written by the library itself (no netCDF library), with
  * the dimensions, variables and long_name attributes the reference's
- * stokes_nc_init defines.  struct stokes_nc is opaque here (the reference
- * exposes its netCDF ids).  Errors print a message and exit, as the
- * reference's stokes_nc_error does.  Not provided: the reading half
- * (stokes-nc-read.c), stokes_nc_print_actives, the BD parameter files
- * (stokes_nc_set_by_params and its readers) and stokes_nc_set_rng (which
- * takes the reference's struct KIrand; stokes_nc_set_rng_state writes the
- * same variables). */
+ * stokes_nc_init defines, and read back by the functions of the
+ * reference's stokes-nc-read.h (any classic or 64-bit-offset file).
+ * Errors print a message and exit, as the reference's stokes_nc_error
+ * does.  Not provided: stokes_nc_print_actives, the BD parameter files
+ * (stokes_nc_set_by_params and its readers), and stokes_nc_set_rng /
+ * stokes_nc_get_rng, which take the reference's struct KIrand
+ * (stokes_nc_set_rng_state / stokes_nc_get_rng_state use the same
+ * variables with a raw MT19937 state). */
 #ifndef LIBSTOKES_HIP_STOKES_NC_H
 #define LIBSTOKES_HIP_STOKES_NC_H
 
@@ -18,7 +19,27 @@
 extern "C" {
 #endif
 
-struct stokes_nc;
+/* the reference's struct stokes_nc (stokes-nc.h:27-158): the same int
+ * fields in the same order, so code that reads nc->np, nc->ntime or the
+ * flags compiles and reads them unchanged; the *_id fields hold this
+ * library's dimension and variable indices (no netCDF ids), id the file
+ * descriptor.  priv: the library's own state */
+struct stokes_nc {
+  int id, np, p_dim, p_id, npf, pf_dim, pf_id, nvec, vec_dim, vec_id, nstt,
+      stt_dim, stt_id, nquat, quat_dim, quat_id, nrng, rng_dim, rng_id,
+      flag_rng, ntime, time_dim, time_id, l_id, ui0_id, oi0_id, ei0_id,
+      ui_id, oi_id, ei_id, x0_id, u0_id, o0_id, e0_id, f0_id, t0_id, s0_id,
+      xf0_id, uf0_id, of0_id, ef0_id, ff0_id, tf0_id, sf0_id, x_id, u_id,
+      o_id, e_id, f_id, t_id, s_id, q_id, xf_id, uf_id, of_id, ef_id, ff_id,
+      tf_id, sf_id, a_id, af_id, shear_mode_id, shear_rate_id,
+      shear_shift_id, mt_id, mti_id, mt_Ghs_id, mt_Gs_id, flag_ui0,
+      flag_oi0, flag_ei0, flag_ui, flag_oi, flag_ei, flag_x0, flag_u0,
+      flag_o0, flag_e0, flag_f0, flag_t0, flag_s0, flag_xf0, flag_uf0,
+      flag_of0, flag_ef0, flag_ff0, flag_tf0, flag_sf0, flag_x, flag_u,
+      flag_o, flag_e, flag_f, flag_t, flag_s, flag_q, flag_xf, flag_uf,
+      flag_of, flag_ef, flag_ff, flag_tf, flag_sf, flag_a, flag_af;
+  void *priv;
+};
 
 /* replaces stokes-nc.c:1367-1443: a file with x[time][p][vec] only
  * (np mobile particles); an existing file is an error (NC_NOCLOBBER) */
@@ -93,6 +114,45 @@ void stokes_nc_set_sf (struct stokes_nc *nc, int step, const double *sf);
 void stokes_nc_set_rng_state (struct stokes_nc *nc, int step,
                               const uint32_t *mt, int mti,
                               int gauss_has_saved, double gauss_saved);
+
+/* ---- reading (the reference's stokes-nc-read.h) ---- */
+
+/* replaces stokes-nc-read.c:741-745: open a file for reading; np, npf,
+ * ntime and the flag_* fields describe it */
+struct stokes_nc *stokes_nc_open (const char *filename);
+
+/* replaces stokes-nc-read.c:750-754: open for appending records (the
+ * setters then write past ntime) */
+struct stokes_nc *stokes_nc_reopen (const char *filename);
+
+/* replaces stokes-nc-read.c:763-898: Ui0, Oi0, Ei0, a, af or l (Ui, Oi, Ei:
+ * their first record); another name prints a message and reads nothing */
+void stokes_nc_get_array1d (const struct stokes_nc *nc, const char *name,
+                            double *x);
+
+/* replaces stokes-nc-read.c:902-1118: a time-independent [np or npf][3 or
+ * 5] entry (x0, U0, ..., Sf0) */
+void stokes_nc_get_data0 (const struct stokes_nc *nc, const char *name,
+                          double *x);
+
+/* replaces stokes-nc-read.c:1122-1356: entry name (x, q, U, ..., Sf) of
+ * record step */
+void stokes_nc_get_data (const struct stokes_nc *nc, const char *name,
+                         int step, double *x);
+
+/* replaces stokes-nc-read.c:1364-1385: time[ntime] */
+void stokes_nc_get_time (const struct stokes_nc *nc, double *time);
+
+/* replaces stokes-nc-read.c:1393-1413: time[step] */
+double stokes_nc_get_time_step (const struct stokes_nc *nc, int step);
+
+/* the random-generator state of record step as stokes_nc_set_rng_state
+ * writes it (mt[624] as 32-bit words); no counterpart in the reference
+ * under this name (stokes_nc_get_rng, stokes-nc-read.c:1419-1482, fills a
+ * struct KIrand) */
+void stokes_nc_get_rng_state (const struct stokes_nc *nc, int step,
+                              uint32_t *mt, int *mti, int *gauss_has_saved,
+                              double *gauss_saved);
 
 #ifdef __cplusplus
 }

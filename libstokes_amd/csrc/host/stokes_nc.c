@@ -17,6 +17,7 @@
  * stokes_nc_error does (stokes-nc.c:33-48).
  */
 #define _POSIX_C_SOURCE 200809L
+#include <stddef.h>
 #include <stdint.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -48,17 +49,19 @@ typedef struct {
   long long begin;              /* file offset (record vars: of record 0) */
 } nc_var;
 
-struct stokes_nc {
+/* the file: dimensions, variables, record bookkeeping */
+typedef struct {
   FILE *fp;
-  const char *fname;
-  int np, npf;
+  char fname[256];
   int ndims, nvars;
   nc_dim dims[NC_MAXDIM];
   nc_var vars[NC_MAXVAR];
-  int time_dim;
   int numrecs;
   long long recsize;
-};
+  int *ntime;                   /* the public struct's ntime */
+} nc_file;
+
+#define NCF(nc) ((nc_file *) (nc)->priv)
 
 static void
 nc_die (const char *fname, const char *what, const char *name)
@@ -138,7 +141,7 @@ buf_text (nc_buf *h, const char *s)
 /* ---- define mode ---- */
 
 static int
-def_dim (struct stokes_nc *nc, const char *name, int len)
+def_dim (nc_file *nc, const char *name, int len)
 {
   if (nc->ndims >= NC_MAXDIM) nc_die (nc->fname, "too many dimensions", name);
   nc_dim *d = &nc->dims[nc->ndims];
@@ -148,7 +151,7 @@ def_dim (struct stokes_nc *nc, const char *name, int len)
 }
 
 static void
-def_var (struct stokes_nc *nc, const char *name, int type, int nd,
+def_var (nc_file *nc, const char *name, int type, int nd,
          const int *dims, const char *longname)
 {
   if (nc->nvars >= NC_MAXVAR) nc_die (nc->fname, "too many variables", name);
@@ -172,7 +175,7 @@ def_var (struct stokes_nc *nc, const char *name, int type, int nd,
 }
 
 static nc_var *
-var_of (struct stokes_nc *nc, const char *name)
+var_of (nc_file *nc, const char *name)
 {
   for (int k = 0; k < nc->nvars; k++)
     if (strcmp (nc->vars[k].name, name) == 0) return &nc->vars[k];
@@ -182,7 +185,7 @@ var_of (struct stokes_nc *nc, const char *name)
 
 /* write count values of v (doubles, or ints when v is NC_INT) at offset */
 static void
-write_values (struct stokes_nc *nc, const nc_var *v, long long off,
+write_values (nc_file *nc, const nc_var *v, long long off,
               const double *d, const int *iv)
 {
   unsigned char *b = malloc (v->vsize ? v->vsize : 4);
@@ -208,7 +211,7 @@ write_values (struct stokes_nc *nc, const nc_var *v, long long off,
 
 /* leave define mode: header, offsets, fill of the fixed-size variables */
 static void
-end_def (struct stokes_nc *nc)
+end_def (nc_file *nc)
 {
   /* the header size is independent of the begin values (8 bytes each) */
   nc_buf h = {0};
@@ -280,7 +283,7 @@ end_def (struct stokes_nc *nc)
 /* ---- data ---- */
 
 static void
-put_fixed (struct stokes_nc *nc, const char *name, const double *d,
+put_fixed (nc_file *nc, const char *name, const double *d,
            const int *iv)
 {
   const nc_var *v = var_of (nc, name);
@@ -289,7 +292,7 @@ put_fixed (struct stokes_nc *nc, const char *name, const double *d,
 }
 
 static void
-put_rec (struct stokes_nc *nc, const char *name, int step, const double *d,
+put_rec (nc_file *nc, const char *name, int step, const double *d,
          const int *iv)
 {
   const nc_var *v = var_of (nc, name);
@@ -305,6 +308,7 @@ put_rec (struct stokes_nc *nc, const char *name, int step, const double *d,
                           nc->vars[k].begin + (long long) r * nc->recsize,
                           NULL, NULL);
       nc->numrecs = step + 1;
+      if (nc->ntime) *nc->ntime = nc->numrecs;
       unsigned char t[4];
       be32 (t, (uint32_t) nc->numrecs);
       if (fseeko (nc->fp, 4, SEEK_SET) != 0 || fwrite (t, 1, 4, nc->fp) != 4)
@@ -324,15 +328,15 @@ typedef struct {
   int a, af;
 } nc_flags;
 
+static struct stokes_nc *nc_public (nc_file *f);
+
 static struct stokes_nc *
 nc_create (const char *filename, int nm, int nf, const nc_flags *fl,
            int shear_mode, int flag_rng)
 {
-  struct stokes_nc *nc = calloc (1, sizeof (*nc));
+  nc_file *nc = calloc (1, sizeof (*nc));
   if (!nc) nc_die (filename, "allocation failed", NULL);
-  nc->fname = filename;
-  nc->np = nm;
-  nc->npf = nf;
+  snprintf (nc->fname, sizeof (nc->fname), "%s", filename);
   /* NC_NOCLOBBER: an existing file is an error */
   nc->fp = fopen (filename, "wxb");
   if (!nc->fp) nc_die (filename, "cannot create (exists, or no access)", NULL);
@@ -364,7 +368,6 @@ nc_create (const char *filename, int nm, int nf, const nc_flags *fl,
       def_var (nc, "rng", NCT_INT, 1, dv, NULL);
     }
   const int t = def_dim (nc, "time", 0);
-  nc->time_dim = t;
   dv[0] = t;
   def_var (nc, "time", NCT_DOUBLE, 1, dv, NULL);
 
@@ -448,6 +451,86 @@ nc_create (const char *filename, int nm, int nf, const nc_flags *fl,
   if (flag_rng) put_fixed (nc, "rng", NULL, idx);
   free (idx);
   put_fixed (nc, "shear_mode", NULL, &shear_mode);
+  return nc_public (nc);
+}
+
+/* the public struct: sizes, this file's dimension and variable indices in
+ * the reference's *_dim / *_id fields, and a flag per data variable */
+typedef struct {
+  const char *name;
+  size_t id;                    /* offset of the *_id field */
+  long flag;                    /* offset of the flag_* field, or -1 */
+} nc_field;
+
+#define FD(nm, f) {nm, offsetof (struct stokes_nc, f##_id), \
+                   (long) offsetof (struct stokes_nc, flag_##f)}
+#define FA(nm, f) {nm, offsetof (struct stokes_nc, f##_id), -1}
+static const nc_field nc_fields[] = {
+  FA ("p", p), FA ("pf", pf), FA ("vec", vec), FA ("stt", stt),
+  FA ("quat", quat), FA ("rng", rng), FA ("time", time), FA ("l", l),
+  FA ("shear_mode", shear_mode), FA ("shear_rate", shear_rate),
+  FA ("shear_shift", shear_shift), FA ("mt", mt), FA ("mti", mti),
+  FA ("mt_Ghs", mt_Ghs), FA ("mt_Gs", mt_Gs),
+  FD ("Ui0", ui0), FD ("Oi0", oi0), FD ("Ei0", ei0), FD ("Ui", ui),
+  FD ("Oi", oi), FD ("Ei", ei), FD ("x0", x0), FD ("U0", u0), FD ("O0", o0),
+  FD ("E0", e0), FD ("F0", f0), FD ("T0", t0), FD ("S0", s0),
+  FD ("xf0", xf0), FD ("Uf0", uf0), FD ("Of0", of0), FD ("Ef0", ef0),
+  FD ("Ff0", ff0), FD ("Tf0", tf0), FD ("Sf0", sf0), FD ("x", x),
+  FD ("U", u), FD ("O", o), FD ("E", e), FD ("F", f), FD ("T", t),
+  FD ("S", s), FD ("q", q), FD ("xf", xf), FD ("Uf", uf), FD ("Of", of),
+  FD ("Ef", ef), FD ("Ff", ff), FD ("Tf", tf), FD ("Sf", sf), FD ("a", a),
+  FD ("af", af),
+};
+#undef FD
+#undef FA
+
+static int
+dim_of (const nc_file *f, const char *name)
+{
+  for (int k = 0; k < f->ndims; k++)
+    if (strcmp (f->dims[k].name, name) == 0) return k;
+  return -1;
+}
+
+static int
+var_index (const nc_file *f, const char *name)
+{
+  for (int k = 0; k < f->nvars; k++)
+    if (strcmp (f->vars[k].name, name) == 0) return k;
+  return -1;
+}
+
+static struct stokes_nc *
+nc_public (nc_file *f)
+{
+  struct stokes_nc *nc = calloc (1, sizeof (*nc));
+  if (!nc) nc_die (f->fname, "allocation failed", NULL);
+  nc->priv = f;
+  nc->id = fileno (f->fp);
+  const int pd = dim_of (f, "p"), qd = dim_of (f, "pf"), rd = dim_of (f, "rng");
+  nc->p_dim = pd;
+  nc->pf_dim = qd;
+  nc->vec_dim = dim_of (f, "vec");
+  nc->stt_dim = dim_of (f, "stt");
+  nc->quat_dim = dim_of (f, "quat");
+  nc->rng_dim = rd;
+  nc->time_dim = dim_of (f, "time");
+  nc->np = pd >= 0 ? f->dims[pd].len : 0;
+  nc->npf = qd >= 0 ? f->dims[qd].len : 0;
+  nc->nvec = 3;
+  nc->nstt = 5;
+  nc->nquat = 4;
+  nc->nrng = rd >= 0 ? f->dims[rd].len : 0;
+  nc->flag_rng = rd >= 0;
+  nc->ntime = f->numrecs;
+  f->ntime = &nc->ntime;
+  for (size_t k = 0; k < sizeof (nc_fields) / sizeof (nc_fields[0]); k++)
+    {
+      const int v = var_index (f, nc_fields[k].name);
+      *(int *) ((char *) nc + nc_fields[k].id) = v;
+      if (nc_fields[k].flag >= 0)
+        *(int *) ((char *) nc + nc_fields[k].flag) = v >= 0;
+    }
   return nc;
 }
 
@@ -489,7 +572,9 @@ void
 stokes_nc_free (struct stokes_nc *nc)
 {
   if (!nc) return;
-  if (fclose (nc->fp) != 0) nc_die (nc->fname, "close failed", NULL);
+  nc_file *f = NCF (nc);
+  if (fclose (f->fp) != 0) nc_die (f->fname, "close failed", NULL);
+  free (f);
   free (nc);
 }
 
@@ -498,14 +583,15 @@ stokes_nc_free (struct stokes_nc *nc)
 void
 stokes_nc_set_shear_rate (struct stokes_nc *nc, double shear_rate)
 {
-  put_fixed (nc, "shear_rate", &shear_rate, NULL);
+  put_fixed (NCF (nc), "shear_rate", &shear_rate, NULL);
 }
 
 #define FIXED(fn, var)                                                     \
-  void fn (struct stokes_nc *nc, const double *v) { put_fixed (nc, var, v, NULL); }
+  void fn (struct stokes_nc *nc, const double *v)                         \
+  { put_fixed (NCF (nc), var, v, NULL); }
 #define RECORD(fn, var)                                                    \
   void fn (struct stokes_nc *nc, int step, const double *v)                \
-  { put_rec (nc, var, step, v, NULL); }
+  { put_rec (NCF (nc), var, step, v, NULL); }
 
 FIXED (stokes_nc_set_l, "l")
 FIXED (stokes_nc_set_ui0, "Ui0")
@@ -549,22 +635,23 @@ RECORD (stokes_nc_set_sf, "Sf")
 void
 stokes_nc_set_time (struct stokes_nc *nc, int step, double time)
 {
-  put_rec (nc, "time", step, &time, NULL);
+  put_rec (NCF (nc), "time", step, &time, NULL);
 }
 
 void
 stokes_nc_set_shear_shift (struct stokes_nc *nc, int step, double shear_shift)
 {
-  put_rec (nc, "shear_shift", step, &shear_shift, NULL);
+  put_rec (NCF (nc), "shear_shift", step, &shear_shift, NULL);
 }
 
 /* the whole MT19937 state (the reference stores half of it on LP64,
  * stokes-nc.c:2667-2671): the 624 words as their 32-bit patterns, mti, and
  * the polar method's saved-value flag and value */
 void
-stokes_nc_set_rng_state (struct stokes_nc *nc, int step, const uint32_t *mt,
+stokes_nc_set_rng_state (struct stokes_nc *pub, int step, const uint32_t *mt,
                          int mti, int gauss_has_saved, double gauss_saved)
 {
+  nc_file *nc = NCF (pub);
   int *w = malloc (sizeof (int) * 624);
   if (!w) nc_die (nc->fname, "allocation failed", "mt");
   for (int k = 0; k < 624; k++) memcpy (&w[k], &mt[k], 4);
@@ -573,4 +660,260 @@ stokes_nc_set_rng_state (struct stokes_nc *nc, int step, const uint32_t *mt,
   put_rec (nc, "mti", step, NULL, &mti);
   put_rec (nc, "mt_Ghs", step, NULL, &gauss_has_saved);
   put_rec (nc, "mt_Gs", step, &gauss_saved, NULL);
+}
+
+/* ---- reading (stokes-nc-read.c) ---- */
+
+static uint32_t
+rd32 (nc_file *f)
+{
+  unsigned char t[4];
+  if (fread (t, 1, 4, f->fp) != 4) nc_die (f->fname, "truncated header", NULL);
+  return ((uint32_t) t[0] << 24) | ((uint32_t) t[1] << 16)
+    | ((uint32_t) t[2] << 8) | t[3];
+}
+
+static void
+rdname (nc_file *f, char *out, size_t cap)
+{
+  uint32_t n = rd32 (f);
+  size_t pad = (4 - n % 4) % 4;
+  char *b = malloc (n + pad + 1);
+  if (!b || fread (b, 1, n + pad, f->fp) != n + pad)
+    nc_die (f->fname, "truncated header", NULL);
+  b[n] = 0;
+  snprintf (out, cap, "%s", b);
+  free (b);
+}
+
+static size_t
+type_size (int type)
+{
+  return type == 1 || type == 2 ? 1 : (type == 3 ? 2 : (type == 6 ? 8 : 4));
+}
+
+/* an attribute list: skipped (names and values are not needed here) */
+static void
+skip_atts (nc_file *f)
+{
+  uint32_t tag = rd32 (f), n = rd32 (f);
+  if (tag == 0 && n == 0) return;
+  if (tag != NC_TAG_ATT) nc_die (f->fname, "bad attribute list", NULL);
+  for (uint32_t k = 0; k < n; k++)
+    {
+      char nm[256];
+      rdname (f, nm, sizeof (nm));
+      const int type = (int) rd32 (f);
+      const size_t bytes = rd32 (f) * type_size (type);
+      if (fseeko (f->fp, (off_t) ((bytes + 3) / 4 * 4), SEEK_CUR) != 0)
+        nc_die (f->fname, "truncated header", nm);
+    }
+}
+
+static struct stokes_nc *
+nc_open (const char *filename, int writable)
+{
+  nc_file *f = calloc (1, sizeof (*f));
+  if (!f) nc_die (filename, "allocation failed", NULL);
+  snprintf (f->fname, sizeof (f->fname), "%s", filename);
+  f->fp = fopen (filename, writable ? "r+b" : "rb");
+  if (!f->fp) nc_die (filename, "cannot open", NULL);
+  unsigned char magic[4];
+  if (fread (magic, 1, 4, f->fp) != 4 || memcmp (magic, "CDF", 3) != 0
+      || (magic[3] != 1 && magic[3] != 2))
+    nc_die (filename, "not a netCDF classic or 64-bit-offset file", NULL);
+  const int off64 = magic[3] == 2;
+  f->numrecs = (int) rd32 (f);
+  uint32_t tag = rd32 (f), n = rd32 (f);
+  if (tag == NC_TAG_DIM)
+    for (uint32_t k = 0; k < n; k++)
+      {
+        if (f->ndims >= NC_MAXDIM) nc_die (filename, "too many dimensions", NULL);
+        nc_dim *d = &f->dims[f->ndims++];
+        rdname (f, d->name, sizeof (d->name));
+        d->len = (int) rd32 (f);
+      }
+  skip_atts (f);                                /* global attributes */
+  tag = rd32 (f);
+  n = rd32 (f);
+  if (tag == NC_TAG_VAR)
+    for (uint32_t k = 0; k < n; k++)
+      {
+        if (f->nvars >= NC_MAXVAR) nc_die (filename, "too many variables", NULL);
+        nc_var *v = &f->vars[f->nvars++];
+        memset (v, 0, sizeof (*v));
+        rdname (f, v->name, sizeof (v->name));
+        v->nd = (int) rd32 (f);
+        if (v->nd > 3) nc_die (filename, "variable of more than 3 dimensions", v->name);
+        v->count = 1;
+        for (int j = 0; j < v->nd; j++)
+          {
+            v->dim[j] = (int) rd32 (f);
+            if (v->dim[j] < 0 || v->dim[j] >= f->ndims)
+              nc_die (filename, "bad dimension id", v->name);
+            if (f->dims[v->dim[j]].len == 0) v->rec = j == 0;
+            else v->count *= (size_t) f->dims[v->dim[j]].len;
+          }
+        skip_atts (f);
+        v->type = (int) rd32 (f);
+        v->vsize = rd32 (f);
+        if (off64)
+          {
+            uint64_t hi = rd32 (f), lo = rd32 (f);
+            v->begin = (long long) ((hi << 32) | lo);
+          }
+        else
+          v->begin = (long long) rd32 (f);
+      }
+  for (int k = 0; k < f->nvars; k++)
+    if (f->vars[k].rec) f->recsize += (long long) f->vars[k].vsize;
+  return nc_public (f);
+}
+
+struct stokes_nc *
+stokes_nc_open (const char *filename)
+{
+  return nc_open (filename, 0);
+}
+
+struct stokes_nc *
+stokes_nc_reopen (const char *filename)
+{
+  return nc_open (filename, 1);
+}
+
+/* count values of v at offset as doubles (NC_INT, NC_FLOAT, NC_DOUBLE) */
+static void
+read_values (nc_file *f, const nc_var *v, long long off, double *x, int *iv)
+{
+  const size_t ts = type_size (v->type), bytes = v->count * ts;
+  unsigned char *b = malloc (bytes ? bytes : 1);
+  if (!b) nc_die (f->fname, "allocation failed", v->name);
+  if (fseeko (f->fp, (off_t) off, SEEK_SET) != 0
+      || fread (b, 1, bytes, f->fp) != bytes)
+    nc_die (f->fname, "read failed", v->name);
+  for (size_t k = 0; k < v->count; k++)
+    {
+      const unsigned char *q = b + ts * k;
+      uint32_t w = ((uint32_t) q[0] << 24) | ((uint32_t) q[1] << 16)
+        | ((uint32_t) q[2] << 8) | q[3];
+      double val;
+      if (v->type == NCT_DOUBLE)
+        {
+          uint64_t u = ((uint64_t) w << 32)
+            | ((uint32_t) q[4] << 24) | ((uint32_t) q[5] << 16)
+            | ((uint32_t) q[6] << 8) | q[7];
+          memcpy (&val, &u, 8);
+        }
+      else if (v->type == 5)
+        {
+          float fl;
+          memcpy (&fl, &w, 4);
+          val = fl;
+        }
+      else if (v->type == NCT_INT)
+        {
+          int32_t i;
+          memcpy (&i, &w, 4);
+          val = i;
+          if (iv) iv[k] = i;
+        }
+      else
+        nc_die (f->fname, "unsupported variable type", v->name);
+      if (x) x[k] = val;
+    }
+  free (b);
+}
+
+/* a variable of the reference's reading functions by name, with the
+ * shape the function takes (rec: a record variable; nd: its dimensions) */
+static const nc_var *
+want_var (nc_file *f, const char *name, int rec, int nd, const char *who)
+{
+  const int k = var_index (f, name);
+  if (k < 0) nc_die (f->fname, "variable not in this file", name);
+  const nc_var *v = &f->vars[k];
+  if (v->rec != rec || v->nd != nd)
+    {
+      fprintf (stderr, "invalid name %s for %s()\n", name, who);
+      return NULL;
+    }
+  return v;
+}
+
+void
+stokes_nc_get_array1d (const struct stokes_nc *nc, const char *name, double *x)
+{
+  nc_file *f = NCF (nc);
+  static const char *ok[] = {"Ui0", "Oi0", "Ei0", "Ui", "Oi", "Ei", "a", "af", "l"};
+  int found = 0;
+  for (size_t k = 0; k < sizeof (ok) / sizeof (ok[0]); k++)
+    found |= strcmp (ok[k], name) == 0;
+  if (!found)
+    {
+      fprintf (stderr, "invalid name for stokes_nc_get_array1d()\n"
+               "which is for Ui0, Oi0, Ei0, Ui, Oi, Ei, a, af, or l.\n");
+      return;
+    }
+  const int k = var_index (f, name);
+  if (k < 0) nc_die (f->fname, "variable not in this file", name);
+  /* Ui, Oi, Ei: their first record */
+  read_values (f, &f->vars[k], f->vars[k].begin, x, NULL);
+}
+
+void
+stokes_nc_get_data0 (const struct stokes_nc *nc, const char *name, double *x)
+{
+  nc_file *f = NCF (nc);
+  const nc_var *v = want_var (f, name, 0, 2, "stokes_nc_get_data0");
+  if (v) read_values (f, v, v->begin, x, NULL);
+}
+
+void
+stokes_nc_get_data (const struct stokes_nc *nc, const char *name, int step,
+                    double *x)
+{
+  nc_file *f = NCF (nc);
+  const nc_var *v = want_var (f, name, 1, 3, "stokes_nc_get_data");
+  if (!v) return;
+  if (step < 0 || step >= f->numrecs) nc_die (f->fname, "step out of range", name);
+  read_values (f, v, v->begin + (long long) step * f->recsize, x, NULL);
+}
+
+void
+stokes_nc_get_time (const struct stokes_nc *nc, double *time)
+{
+  for (int s = 0; s < nc->ntime; s++) time[s] = stokes_nc_get_time_step (nc, s);
+}
+
+double
+stokes_nc_get_time_step (const struct stokes_nc *nc, int step)
+{
+  nc_file *f = NCF (nc);
+  const nc_var *v = want_var (f, "time", 1, 1, "stokes_nc_get_time_step");
+  if (step < 0 || step >= f->numrecs) nc_die (f->fname, "step out of range", "time");
+  double t = 0.0;
+  read_values (f, v, v->begin + (long long) step * f->recsize, &t, NULL);
+  return t;
+}
+
+void
+stokes_nc_get_rng_state (const struct stokes_nc *nc, int step, uint32_t *mt,
+                         int *mti, int *gauss_has_saved, double *gauss_saved)
+{
+  nc_file *f = NCF (nc);
+  if (step < 0 || step >= f->numrecs) nc_die (f->fname, "step out of range", "mt");
+  const long long r = (long long) step * f->recsize;
+  const nc_var *v = want_var (f, "mt", 1, 2, "stokes_nc_get_rng_state");
+  int *w = malloc (sizeof (int) * v->count);
+  if (!w) nc_die (f->fname, "allocation failed", "mt");
+  read_values (f, v, v->begin + r, NULL, w);
+  for (size_t k = 0; k < v->count; k++) memcpy (&mt[k], &w[k], 4);
+  free (w);
+  v = want_var (f, "mti", 1, 1, "stokes_nc_get_rng_state");
+  read_values (f, v, v->begin + r, NULL, mti);
+  v = want_var (f, "mt_Ghs", 1, 1, "stokes_nc_get_rng_state");
+  read_values (f, v, v->begin + r, NULL, gauss_has_saved);
+  v = want_var (f, "mt_Gs", 1, 1, "stokes_nc_get_rng_state");
+  read_values (f, v, v->begin + r, gauss_saved, NULL);
 }

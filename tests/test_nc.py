@@ -215,3 +215,117 @@ def test_x_only_file_and_no_clobber(tmp_path):
                            os.path.abspath(__file__))))
     assert r.returncode == 1 and b"cannot create" in r.stderr
     assert os.path.getsize(fn) > 0
+
+
+class _NC(C.Structure):
+    """the head of struct stokes_nc: the reference's first int fields"""
+    _fields_ = [(n, C.c_int) for n in (
+        "id", "np", "p_dim", "p_id", "npf", "pf_dim", "pf_id", "nvec",
+        "vec_dim", "vec_id", "nstt", "stt_dim", "stt_id", "nquat",
+        "quat_dim", "quat_id", "nrng", "rng_dim", "rng_id", "flag_rng",
+        "ntime")]
+
+
+def _rlib():
+    L = _lib()
+    P = C.POINTER(C.c_double)
+    for f in ("stokes_nc_open", "stokes_nc_reopen"):
+        getattr(L, f).restype = C.POINTER(_NC)
+        getattr(L, f).argtypes = [C.c_char_p]
+    for f in ("stokes_nc_get_array1d", "stokes_nc_get_data0"):
+        getattr(L, f).argtypes = [C.c_void_p, C.c_char_p, P]
+    L.stokes_nc_get_data.argtypes = [C.c_void_p, C.c_char_p, C.c_int, P]
+    L.stokes_nc_get_time.argtypes = [C.c_void_p, P]
+    L.stokes_nc_get_time_step.argtypes = [C.c_void_p, C.c_int]
+    L.stokes_nc_get_time_step.restype = C.c_double
+    L.stokes_nc_get_rng_state.argtypes = [
+        C.c_void_p, C.c_int, C.POINTER(C.c_uint32), C.POINTER(C.c_int),
+        C.POINTER(C.c_int), P]
+    return L
+
+
+def test_reader_and_append(tmp_path):
+    """stokes_nc_open reads what the writer wrote and a file scipy wrote;
+    stokes_nc_reopen appends records that scipy then reads"""
+    L = _rlib()
+    fn = str(tmp_path / "r.nc")
+    np_, nf = 4, 2
+    nc = L.stokes_nc_init(fn.encode(), np_, nf, 1, 1, 1, 0, 0, 1)
+    rng = np.random.default_rng(5)
+    a, uf0, f0 = (rng.standard_normal(k) for k in (np_, 3 * nf, 3 * np_))
+    L.stokes_nc_set_a(nc, _p(a))
+    L.stokes_nc_set_uf0(nc, _p(uf0))
+    L.stokes_nc_set_f0(nc, _p(f0))
+    xs = [rng.standard_normal(3 * np_) for _ in range(3)]
+    qs = [rng.standard_normal(4 * np_) for _ in range(3)]
+    mt = rng.integers(0, 2**32, 624, dtype=np.uint32)
+    for s in range(3):
+        L.stokes_nc_set_time(nc, s, 0.5 * s)
+        L.stokes_nc_set_x(nc, s, _p(xs[s]))
+        L.stokes_nc_set_q(nc, s, _p(qs[s]))
+    L.stokes_nc_set_rng_state(nc, 1, mt.ctypes.data_as(C.POINTER(C.c_uint32)),
+                              7, 1, -0.25)
+    L.stokes_nc_free(nc)
+
+    r = L.stokes_nc_open(fn.encode())
+    h = r.contents
+    assert (h.np, h.npf, h.nvec, h.nstt, h.nquat, h.ntime) == (np_, nf, 3, 5,
+                                                             4, 3)
+    assert h.flag_rng == 1 and h.nrng == 624
+    out = np.zeros(3 * np_)
+    L.stokes_nc_get_array1d(r, b"a", _p(out))
+    assert np.array_equal(out[:np_], a)
+    L.stokes_nc_get_data0(r, b"Uf0", _p(out))
+    assert np.array_equal(out[:3 * nf], uf0)
+    L.stokes_nc_get_data0(r, b"F0", _p(out))
+    assert np.array_equal(out, f0)
+    q = np.zeros(4 * np_)
+    for s in range(3):
+        L.stokes_nc_get_data(r, b"x", s, _p(out))
+        assert np.array_equal(out, xs[s])
+        L.stokes_nc_get_data(r, b"q", s, _p(q))
+        assert np.array_equal(q, qs[s])
+    t = np.zeros(3)
+    L.stokes_nc_get_time(r, _p(t))
+    assert np.array_equal(t, [0.0, 0.5, 1.0])
+    assert L.stokes_nc_get_time_step(r, 2) == 1.0
+    mt2 = np.zeros(624, np.uint32)
+    mti, ghs, gs = C.c_int(), C.c_int(), C.c_double()
+    L.stokes_nc_get_rng_state(r, 1, mt2.ctypes.data_as(C.POINTER(C.c_uint32)),
+                              C.byref(mti), C.byref(ghs), C.byref(gs))
+    assert np.array_equal(mt2, mt) and (mti.value, ghs.value, gs.value) == (
+        7, 1, -0.25)
+    L.stokes_nc_free(r)
+
+    # append a record to the same file, read it back with scipy
+    r = L.stokes_nc_reopen(fn.encode())
+    x3 = rng.standard_normal(3 * np_)
+    L.stokes_nc_set_time(r, 3, 1.5)
+    L.stokes_nc_set_x(r, 3, _p(x3))
+    assert r.contents.ntime == 4
+    L.stokes_nc_free(r)
+    with netcdf_file(fn, "r", mmap=False) as f:
+        assert f.variables["time"].shape == (4,)
+        assert np.array_equal(f.variables["x"][3].reshape(-1), x3)
+        assert np.array_equal(f.variables["x"][1].reshape(-1), xs[1])
+
+    # a CDF-1 file written by scipy (32-bit offsets, other attributes)
+    fn2 = str(tmp_path / "s.nc")
+    with netcdf_file(fn2, "w", version=1) as f:
+        f.history = "written by scipy"
+        f.createDimension("time", None)        # scipy: the record one first
+        for d, n in (("p", 2), ("vec", 3), ("stt", 5), ("quat", 4)):
+            f.createDimension(d, n)
+        v = f.createVariable("time", "d", ("time",))
+        x = f.createVariable("x", "d", ("time", "p", "vec"))
+        x.units = "a"
+        for s in range(2):
+            v[s] = 2.0 * s
+            x[s] = np.arange(6.0).reshape(2, 3) + 10 * s
+    r = L.stokes_nc_open(fn2.encode())
+    assert r.contents.np == 2 and r.contents.ntime == 2
+    out = np.zeros(6)
+    L.stokes_nc_get_data(r, b"x", 1, _p(out))
+    assert np.array_equal(out, np.arange(6.0) + 10)
+    assert L.stokes_nc_get_time_step(r, 1) == 2.0
+    L.stokes_nc_free(r)
