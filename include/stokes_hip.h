@@ -207,11 +207,14 @@ void stokes_hip_ensemble_destroy (stokes_hip_ensemble *e);
 /* ---- Brownian dynamics ------------------------------------------------
  * The mid-point step of BD_evolve_mid (brownian.c:2352-2560) for the F, FT
  * and FTS versions (the BD_params fields it reads are the arguments of
- * stokes_hip_bd_create), in a fluid at rest.  M is the mobility of
+ * stokes_hip_bd_create).  M is the mobility of
  * BD_atimes_mob_FU (brownian.c:494-739): F -> U (F), (F, T) -> (U, O)
  * particle-wise (FT), and for FTS the mobility problem with E = 0 solved
- * by GMRES for each product; the drift is that of solve_mix_3all, which
- * with fixed particles holds them at rest.  For F and FT with every
+ * by GMRES for each product; the drift is that of solve_mix_3all in the
+ * labo frame: with fixed particles it holds them at rest, and with an
+ * imposed flow (sys->Ui, Oi, Ei, read at each step) it adds the flow at the
+ * positions of each drift product (FTS: E = -Ei in the rest frame), the
+ * noise staying in the rest frame.  For F and FT with every
  * particle mobile an attempt runs on the GPU as one captured graph (FTS,
  * and fixed particles: the same launches, eagerly, the GMRES steered from
  * the host): Gaussian draw y, z = M^{-1/2} y by the Chebyshev recursion
@@ -253,9 +256,8 @@ typedef struct stokes_hip_bd_info {
  * (its products are matrix-free). */
 int stokes_hip_bd_set_real_cache (stokes_hip_bd *bd, int enable);
 
-/* sys: version 0 (F), 1 (FT) or 2 (FTS), no shear and no imposed flow
- * (Ui, Oi, Ei zero: refused here and at each step), set up as for
- * atimes_3all; particles nm..np-1 (stokes_set_np) are fixed: the step
+/* sys: version 0 (F), 1 (FT) or 2 (FTS), no shear (refused here and at
+ * each step), set up as for atimes_3all; particles nm..np-1 (stokes_set_np) are fixed: the step
  * moves x [3 nm] (and q [4 nm]), the fixed positions are read from sys->pos
  * at each step.  Unequal radii (sys->a, read at creation): the noise
  * operator carries BD_atimes_mob_FU's Global scaling (U / a, O / a^3,

@@ -291,6 +291,39 @@ extern "C" int shk_dev_bd_force_ft (shk_dev *d, double *f, const double *F, cons
   return SHK_OK;
 }
 
+// the imposed flow at particle i (shift_rest_to_labo_U / _O, f.c:621-660,
+// ft.c:935-955; the strain part of shift_labo_to_rest_E, fts.c:2303-2320),
+// added with sign to v[D i + q] for q in [qlo, qhi)
+__global__ void k_bd_imposed (double *__restrict__ v, const double *__restrict__ p, int n,
+                              int D, int qlo, int qhi, shk_flow fl, double sign)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double x = p[3 * i], y = p[3 * i + 1], z = p[3 * i + 2];
+  const double *E = fl.E;
+  const double Ezz = -E[0] - E[4];
+  double w[11];
+  w[0] = fl.U[0] + fl.O[1] * z - fl.O[2] * y + E[0] * x + E[1] * y + E[2] * z;
+  w[1] = fl.U[1] + fl.O[2] * x - fl.O[0] * z + E[1] * x + E[4] * y + E[3] * z;
+  w[2] = fl.U[2] + fl.O[0] * y - fl.O[1] * x + E[2] * x + E[3] * y + Ezz * z;
+  for (int q = 0; q < 3; q++) w[3 + q] = fl.O[q];
+  for (int q = 0; q < 5; q++) w[6 + q] = E[q];
+  double *vi = v + (size_t) i * D;
+  for (int q = qlo; q < qhi; q++) vi[q] += sign * w[q];
+}
+
+extern "C" int shk_dev_bd_imposed (shk_dev *d, double *v, const double *p, int n, int D,
+                                   int qlo, int qhi, const shk_flow *fl, double sign)
+{
+  if (n <= 0) return SHK_OK;
+  if (qlo < 0 || qhi > D || qhi > 11) return SHK_EARG;
+  hipStream_t s = (hipStream_t) shk_dev_stream (d);
+  hipLaunchKernelGGL (k_bd_imposed, dim3 (blocks_for ((size_t) n, 256)), dim3 (256), 0, s, v, p,
+                      n, D, qlo, qhi, *fl, sign);
+  CHK (hipGetLastError ());
+  return SHK_OK;
+}
+
 extern "C" int shk_dev_bd_euler_ft (shk_dev *d, double *xo, const double *xi, double *qo,
                                     const double *qi, const double *u, const shk_bd_scalars *sc,
                                     double frac, int np)

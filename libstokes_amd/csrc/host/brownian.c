@@ -68,6 +68,10 @@ struct stokes_hip_bd {
   size_t n;                    /* operator dimension: 3 np (F), 6 np */
   double *buf;                 /* device: y z w f u work[4n] xm xn qn */
   double *y, *z, *w, *f, *u, *xm, *xn, *qn, *work, *u0, *f2;
+  const double *pcur;          /* device: positions of the products now */
+  shk_flow flow;               /* the system's imposed flow (Ui, Oi, Ei) */
+  int has_flow;
+  shk_flow graph_flow;         /* the flow the graph was captured with */
   double *xs;                  /* device [3 nm]: the intermediate
                                   configuration a failed attempt leaves in
                                   the system (the retry's first stage) */
@@ -135,10 +139,7 @@ stokes_hip_bd_destroy (stokes_hip_bd *bd)
   free (bd);
 }
 
-/* the step's drift is solve_mix_3all in the labo frame (ewald-3f.c:
- * 269-321 and its FT / FTS forms), which adds the imposed flow Ui + Oi x
- * x + Ei . x (and Oi, and shifts E by Ei for FTS); the integrator here
- * takes the fluid at rest, so a system with an imposed flow is refused */
+/* no imposed flow: Ui, Oi, Ei all zero */
 static int
 bd_fluid_at_rest (const struct stokes *sys)
 {
@@ -167,13 +168,8 @@ stokes_hip_bd_create (struct stokes *sys, const double *F_dev, double peclet,
     }
   if (sys->shear_mode != 0 || sys->nm < 1 || sys->nm > sys->np)
     {
-      stokes_hip_set_error ("bd_create: needs no shear and 1 <= nm <= np");
-      return STOKES_HIP_EUNSUP;
-    }
-  if (!bd_fluid_at_rest (sys))
-    {
-      stokes_hip_set_error ("bd_create: the integrator takes the fluid at rest"
-                            " (Ui, Oi, Ei all zero)");
+      stokes_hip_set_error ("bd_create: needs no shear and 1 <= nm <= np"
+                            " (shear is not supported)");
       return STOKES_HIP_EUNSUP;
     }
   if (n_cheb < 2 || !(eps > 0.0) || dt_lim < 0.0 || sys->np <= 0)
@@ -464,9 +460,23 @@ static int
 bd_mob (stokes_hip_bd *bd, shk_dev *d, int prec, const double *x, double *y,
         int noise)
 {
+  /* the drift is solve_mix_3all in the labo frame: with an imposed flow
+   * the known velocities and strains go to the fluid-rest frame first and
+   * the velocities come back with the flow at the positions of the product
+   * (shift_rest_to_labo_U / _O); the noise operator stays in the rest
+   * frame */
+  const int flow = !noise && bd->has_flow;
+  const int nf = bd->np - bd->nm;
   if (bd->ver != 2)
     {
-      if (bd->nm == bd->np) return shk_dev_atimes (d, x, y, 1, prec);
+      const int D = bd->ver == 0 ? 3 : 6;
+      if (bd->nm == bd->np)
+        {
+          int rc = shk_dev_atimes (d, x, y, 1, prec);
+          if (!rc && flow)
+            rc = shk_dev_bd_imposed (d, y, bd->pcur, bd->nm, D, 0, D, &bd->flow, 1.0);
+          return rc;
+        }
       if (noise)
         {
           /* the noise operator with fixed particles: zero forces there, the
@@ -480,29 +490,46 @@ bd_mob (stokes_hip_bd *bd, shk_dev *d, int prec, const double *x, double *y,
       /* the drift (solve_mix_3all -> solve_mix_3f / 3ft, ewald-3f.c:269-330):
        * the fixed particles at rest (their velocities 0, forces unknown),
        * a mixed problem by GMRES with stokes_set_iter's parameters; the
-       * tail of fts_b (the fixed velocities) stays zero */
-      const int D = bd->ver == 0 ? 3 : 6;
+       * tail of fts_b holds the fixed particles' rest-frame velocities:
+       * zero, or minus the imposed flow at the positions of particles
+       * 0..nf-1, as the shift of ewald-3f.c:301 takes them */
       int m, maxit, it = 0;
       double eps, rr = 0.0;
       stokes_hip_iter_params (bd->sys, &m, &maxit, &eps);
       int rc = shk_dev_scal (d, bd->fts_b, x, 1.0, bd->n);
+      if (!rc) rc = shk_dev_scal (d, bd->fts_b + bd->n, NULL, 0.0, (size_t) D * nf);
+      if (!rc && flow)
+        rc = shk_dev_bd_imposed (d, bd->fts_b + bd->n, bd->pcur, nf, D, 0, D, &bd->flow, -1.0);
       if (!rc) rc = stokes_hip_mix_solve (bd->c, D, D, 1, bd->nm, bd->fts_b,
                                           bd->fts_x, m, maxit, eps, bd->fts_ws,
                                           &it, &rr);
       bd->fts_iters += it;
       if (!rc) rc = shk_dev_scal (d, y, bd->fts_x, 1.0, bd->n);
+      if (!rc && flow)
+        rc = shk_dev_bd_imposed (d, y, bd->pcur, bd->nm, D, 0, D, &bd->flow, 1.0);
       return rc;
     }
-  /* FTS: the mixed problem with the fixed particles at rest (U = O = E = 0;
-   * the tail of fts_b stays zero); nm = np is the mobility problem */
+  /* FTS: the mixed problem with the fixed particles at rest (U = O = E = 0
+   * in the rest frame; with an imposed flow the drift's known values are
+   * E = -Ei for the mobile particles and -(u_inf, Oi, Ei) for the fixed
+   * ones, ewald-3fts.c:603-667); nm = np is the mobility problem */
   int m = bd->fts_m, maxit = bd->fts_max, it = 0;
   double eps = bd->fts_eps, rr = 0.0;
   if (!noise) stokes_hip_iter_params (bd->sys, &m, &maxit, &eps);
   int rc = shk_dev_ft_fts (d, x, bd->nm, 0, bd->fts_b);
+  if (!rc && nf > 0)
+    rc = shk_dev_scal (d, bd->fts_b + (size_t) 11 * bd->nm, NULL, 0.0, (size_t) 11 * nf);
+  if (!rc && flow)
+    rc = shk_dev_bd_imposed (d, bd->fts_b, bd->pcur, bd->nm, 11, 6, 11, &bd->flow, -1.0);
+  if (!rc && flow)
+    rc = shk_dev_bd_imposed (d, bd->fts_b + (size_t) 11 * bd->nm, bd->pcur, nf, 11, 0, 11,
+                             &bd->flow, -1.0);
   if (!rc) rc = stokes_hip_fts_solve (bd->c, 1, bd->nm, bd->fts_b, bd->fts_x,
                                       m, maxit, eps, bd->fts_ws, &it, &rr);
   bd->fts_iters += it;
   if (!rc) rc = shk_dev_ft_fts (d, bd->fts_x, bd->nm, 1, y);
+  if (!rc && flow)
+    rc = shk_dev_bd_imposed (d, y, bd->pcur, bd->nm, 6, 0, 6, &bd->flow, 1.0);
   return rc;
 }
 
@@ -561,6 +588,7 @@ bd_record (stokes_hip_bd *bd, shk_dev *d, int prec, int check_ol, double *x,
   /* BD_calc_FB: z = M^{-1/2} y and its accuracy (chebyshev.c:241-269) */
   const double *px = rc ? NULL : bd_pos (bd, d, p1, &rc);
   if (!rc) rc = shk_dev_set_pos (d, px, 1);
+  bd->pcur = px;
   /* the products at p1 share one dense real-space matrix */
   if (!rc && cache) rc = shk_dev_real_cache (d, 1, prec);
   if (!rc && noise) rc = bd_cheb (bd, d, prec, bd->y, bd->z);
@@ -595,6 +623,7 @@ bd_record (stokes_hip_bd *bd, shk_dev *d, int prec, int check_ol, double *x,
   if (!rc && cache) rc = shk_dev_real_cache (d, 0, prec);
   const double *pm = rc ? NULL : bd_pos (bd, d, bd->xm, &rc);
   if (!rc) rc = shk_dev_set_pos (d, pm, 1);
+  bd->pcur = pm;
   if (!rc) rc = bd_op (bd, d, prec, bd->f, bd->u, 0, 0);
   if (!rc && sch == 2) rc = shk_dev_axpby (d, bd->u, bd->u0, 0.5, 0.5, n);
   if (!rc && sch == 1)
@@ -633,6 +662,7 @@ bd_capture (stokes_hip_bd *bd, stokes_ctx *c, double *x, double *q)
   int rc = 0;
   const double *px = bd_pos (bd, d, x, &rc);
   if (!rc) rc = shk_dev_set_pos (d, px, 1);
+  bd->pcur = px;
   if (!rc) rc = bd_op (bd, d, c->prec, bd->z, bd->w, 1, 1);
   if (!rc && bd->real_cache)
     {
@@ -671,6 +701,7 @@ bd_capture (stokes_hip_bd *bd, stokes_ctx *c, double *x, double *q)
   bd->graph_prec = c->prec;
   bd->graph_ol = check_ol;
   bd->graph_rc = bd->real_cache;
+  bd->graph_flow = bd->flow;
   return 0;
 }
 
@@ -700,6 +731,7 @@ bd_fit_bounds (stokes_hip_bd *bd, stokes_ctx *c, double *x)
   /* the products are at x (an attempt leaves x_mid on the device) */
   int rc = 0;
   c->pos_dev = bd_pos (bd, c->dev, x, &rc);
+  bd->pcur = c->pos_dev;
   if (!rc) rc = stokes_hip_prepare (c, 1);
   if (rc) return rc;
   if (bd->real_cache)
@@ -735,12 +767,19 @@ stokes_hip_bd_step_q (stokes_hip_bd *bd, double *x_dev, double *q_dev,
       return STOKES_HIP_EARG;
     }
   if (bd->ver == 0) q_dev = NULL;      /* evolve_Euler_3all ignores q for F */
-  if (bd->sys->shear_mode != 0 || !bd_fluid_at_rest (bd->sys))
+  if (bd->sys->shear_mode != 0)
     {
-      stokes_hip_set_error ("bd_step: the integrator takes the fluid at rest"
-                            " (no shear, Ui, Oi, Ei all zero)");
+      stokes_hip_set_error ("bd_step: shear is not supported");
       return STOKES_HIP_EUNSUP;
     }
+  /* the imposed flow of the system now (solve_mix_3all's labo frame) */
+  for (int k = 0; k < 3; k++)
+    {
+      bd->flow.U[k] = bd->sys->Ui[k];
+      bd->flow.O[k] = bd->sys->Oi[k];
+    }
+  for (int k = 0; k < 5; k++) bd->flow.E[k] = bd->sys->Ei[k];
+  bd->has_flow = !bd_fluid_at_rest (bd->sys);
   stokes_ctx *c = stokes_hip_ctx (bd->sys);
   if (!c) return STOKES_HIP_ENODEV;
   shk_dev *d = c->dev;
@@ -797,7 +836,8 @@ stokes_hip_bd_step_q (stokes_hip_bd *bd, double *x_dev, double *q_dev,
               || bd->graph_gen != shk_dev_generation (d)
               || bd->graph_uploads != c->uploads || bd->graph_prec != c->prec
               || bd->graph_ol != (bd->sys->rmin == 0.0)
-              || bd->graph_rc != bd->real_cache))
+              || bd->graph_rc != bd->real_cache
+              || memcmp (&bd->graph_flow, &bd->flow, sizeof (shk_flow)) != 0))
         {
           rc = bd_capture (bd, c, x_dev, q_dev);
           if (rc) goto out;

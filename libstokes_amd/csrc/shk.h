@@ -215,6 +215,15 @@ int shk_dev_bd_euler_ft (shk_dev *d, double *xo, const double *xi, double *qo,
                          const shk_bd_scalars *sc, double frac, int np);
 /* y[k] *= s[k] */
 int shk_dev_dmul (shk_dev *d, double *y, const double *s, size_t n);
+/* an imposed flow: Ui, Oi and Ei in the extracted form (xx xy xz yz yy) */
+typedef struct shk_flow {
+  double U[3], O[3], E[5];
+} shk_flow;
+/* v[D i + q] += sign * (the flow at particle i, positions p[3 i]) for q in
+ * [qlo, qhi), i < n: q < 3 the velocity Ui + Oi x p + Ei . p, q < 6 Oi,
+ * q < 11 Ei */
+int shk_dev_bd_imposed (shk_dev *d, double *v, const double *p, int n, int D,
+                        int qlo, int qhi, const shk_flow *fl, double sign);
 /* (F, T) of 6 per particle -> (F, T, 0) of 11 (mode 0), or the first 6 of
  * 11 back (mode 1) */
 int shk_dev_ft_fts (shk_dev *d, const double *x, int np, int mode, double *y);

@@ -1082,12 +1082,12 @@ orc_quat_dqdt (const double *Q, const double *O, double *dQdt)
  * M[UO, X] xi.  Exact to rounding: the limit of the reference's
  * iteration.  O(np^3); small systems only. */
 static void orc_mix_known (const orc_sys *s, int nm, int D, int qs,
-                           const double *ft, double *uo);
+                           const double *ft, const double *vr, double *uo);
 
 void
 orc_fts_mix_ft (const orc_sys *s, int nm, const double *ft, double *uo)
 {
-  orc_mix_known (s, nm, 11, 6, ft, uo);
+  orc_mix_known (s, nm, 11, 6, ft, NULL, uo);
 }
 
 /* The same elimination for D values per particle (3 F, 6 FT, 11 FTS), the
@@ -1098,8 +1098,10 @@ orc_fts_mix_ft (const orc_sys *s, int nm, const double *ft, double *uo)
  * out; the extracted form is the true operator there (no S transform). */
 static void
 orc_mix_known (const orc_sys *s, int nm, int D, int qs, const double *ft,
-               double *uo)
+               const double *vr, double *uo)
 {
+  /* vr: the known values on the unknowns' rows (velocities of the fixed
+   * particles, strains), in the order of the unknowns; NULL: zero */
   const int np = s->np, n = D * np;
   const int nk = qs * nm, nx = n - nk;
   int *K = malloc (sizeof (int) * (nk > 0 ? nk : 1));
@@ -1133,7 +1135,7 @@ orc_mix_known (const orc_sys *s, int nm, int D, int qs, const double *ft,
     {
       double acc = 0.0;
       for (int c = 0; c < nk; c++) acc += M[(size_t) R[a] * n + K[c]] * ft[c];
-      b[a] = -acc;
+      b[a] = (vr ? vr[a] : 0.0) - acc;
       for (int c = 0; c < nx; c++) A[(size_t) a * nx + c] = M[(size_t) R[a] * n + X[c]];
     }
   for (int k = 0; k < nx; k++)
@@ -1215,17 +1217,83 @@ orc_bd_drift (int n, const double *x, double *y, void *ud)
   orc_bd_mob (n, x, y, ud, 0);
 }
 
+/* the imposed flow at position p: w[0..2] = Ui + Oi x p + Ei . p, w[3..5]
+ * = Oi, w[6..10] = Ei (Ei extracted: xx xy xz yz yy; f.c:569-660,
+ * ft.c:901-955, fts.c:2303-2320) */
+static void
+orc_flow_at (const orc_sys *s, const double *p, double *w)
+{
+  const double *E = s->Ei, Ezz = -s->Ei[0] - s->Ei[4];
+  w[0] = s->Ui[0] + s->Oi[1] * p[2] - s->Oi[2] * p[1]
+    + E[0] * p[0] + E[1] * p[1] + E[2] * p[2];
+  w[1] = s->Ui[1] + s->Oi[2] * p[0] - s->Oi[0] * p[2]
+    + E[1] * p[0] + E[4] * p[1] + E[3] * p[2];
+  w[2] = s->Ui[2] + s->Oi[0] * p[1] - s->Oi[1] * p[0]
+    + E[2] * p[0] + E[3] * p[1] + Ezz * p[2];
+  for (int q = 0; q < 3; q++) w[3 + q] = s->Oi[q];
+  for (int q = 0; q < 5; q++) w[6 + q] = E[q];
+}
+
+static int
+orc_has_flow (const orc_sys *s)
+{
+  for (int k = 0; k < 3; k++)
+    if (s->Ui[k] != 0.0 || s->Oi[k] != 0.0) return 1;
+  for (int k = 0; k < 5; k++)
+    if (s->Ei[k] != 0.0) return 1;
+  return 0;
+}
+
+/* the drift in an imposed flow: solve_mix_3all in the labo frame
+ * (ewald-3f.c:269-321, ewald-3ft.c:323-382, ewald-3fts.c:603-667 with the
+ * BD's zero labo-frame E and fixed-particle velocities): known values to
+ * the rest frame (mobile E = -Ei; fixed particle k: -(u_inf, Oi, Ei) at the
+ * position of particle k, as the reference's shift takes it), the mixed
+ * problem by elimination, the mobile velocities back with the flow at
+ * their positions */
+static void
+orc_bd_drift_flow (const orc_sys *s, int nm, const double *x, double *y)
+{
+  const int D = s->version == 0 ? 3 : (s->version == 1 ? 6 : 11);
+  const int qs = s->version == 0 ? 3 : 6, np = s->np;
+  const int nx = D * np - qs * nm;
+  double *vr = malloc (sizeof (double) * (nx > 0 ? nx : 1));
+  double *uo = malloc (sizeof (double) * qs * nm);
+  for (int i = 0, a = 0; i < np; i++)
+    for (int q = 0; q < D; q++)
+      {
+        if (i < nm && q < qs) continue;
+        double w[11];
+        orc_flow_at (s, s->pos + 3 * (i < nm ? i : i - nm), w);
+        vr[a++] = -w[q];
+      }
+  orc_mix_known (s, nm, D, qs, x, vr, uo);
+  for (int i = 0; i < nm; i++)
+    {
+      double w[11];
+      orc_flow_at (s, s->pos + 3 * i, w);
+      for (int q = 0; q < qs; q++) y[qs * i + q] = uo[qs * i + q] + w[q];
+    }
+  free (vr);
+  free (uo);
+}
+
 static void
 orc_bd_mob (int n, const double *x, double *y, void *ud, int scaled)
 {
   const orc_bd_ctx *c = (const orc_bd_ctx *) ud;
   const orc_sys *s = c->s;
   const int d = s->version == 0 ? 3 : 6;
+  if (!scaled && orc_has_flow (s))
+    {
+      orc_bd_drift_flow (s, c->nm, x, y);
+      return;
+    }
   if (c->nm == s->np) orc_bd_op_cb (n, x, y, (void *) s);
   else if (s->version == 2) orc_fts_mix_ft (s, c->nm, x, y);
   else if (!scaled)
     /* the drift (solve_mix_3f / 3ft): fixed particles at rest */
-    orc_mix_known (s, c->nm, d, d, x, y);
+    orc_mix_known (s, c->nm, d, d, x, NULL, y);
   else
     {
       const int nf = d * s->np, nmd = d * c->nm;
@@ -1244,8 +1312,8 @@ orc_bd_mob (int n, const double *x, double *y, void *ud, int scaled)
         y[d * i + q] /= q < 3 ? s->a[i] : s->a[i] * s->a[i] * s->a[i];
 }
 
-/* One BD step, every particle mobile, no imposed flow, no interaction
- * forces: scheme 0 the mid-point algorithm (BD_evolve_mid,
+/* One BD step, every particle mobile, no interaction forces, the drift in
+ * the system's imposed flow (Ui, Oi, Ei; zero: a fluid at rest): scheme 0 the mid-point algorithm (BD_evolve_mid,
  * brownian.c:2352-2560), 1 Banchio-Brady 2003 with step parameter bb_n
  * (BD_evolve_BB03, 2579-2920), 2 Ball-Melrose 1997 (BD_evolve_BM97,
  * 2970-3190), with BD_calc_FB's atimes route (1545-1874, spectral bounds

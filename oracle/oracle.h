@@ -42,6 +42,8 @@ typedef struct orc_sys {
 
   int fix_recip_g;      /* 1: reciprocal G / G-tilde with the sign that makes
                            the FTS sum xi-independent (0: as the reference) */
+  double Ui[3], Oi[3], Ei[5];  /* imposed flow (stokes.h:116-118): read by
+                                  the BD step's drift only */
 } orc_sys;
 
 /* set-up (stokes.c) */

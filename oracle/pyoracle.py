@@ -36,6 +36,7 @@ class OrcSys(C.Structure):
         ("rmx", _ip), ("rmy", _ip), ("rmz", _ip),
         ("shear_mode", C.c_int), ("shear_shift", C.c_double),
         ("fix_recip_g", C.c_int),
+        ("Ui", C.c_double * 3), ("Oi", C.c_double * 3), ("Ei", C.c_double * 5),
     ]
 
 

@@ -674,3 +674,66 @@ def test_bd_retry_after_second_stage_overlap(version):
     _say(f"  x err / disp {ex:.2e}")
     assert ex < (1e-8 if version == 2 else 1e-10)
     bd.close()
+
+
+@pytest.mark.parametrize("version,n,nm,scheme", [
+    (0, 27, 27, 0), (0, 27, 27, 1), (1, 27, 20, 0), (2, 12, 12, 0),
+    (2, 12, 8, 1)])
+def test_bd_imposed_flow_matches_oracle(version, n, nm, scheme):
+    """steps in an imposed flow (Ui, Oi, Ei; no shear): every drift product
+    is solve_mix_3all in the labo frame (the flow added at the positions of
+    the product; fixed particles at rest in the labo frame; FTS with
+    E = -Ei), the noise in the rest frame; against the oracle's step fed
+    with the device's draw"""
+    phi = 0.1
+    _say(f"bd imposed flow version={version} N={n} nm={nm} scheme={scheme}")
+    pos, L = workloads.configuration(n, phi, seed=12)
+    s = ls.Stokes(version, pos, box=(L, L, L), tr=1.0, nm=nm)
+    o = po.System(version, pos, box=(L, L, L), tr=1.0)
+    Ui, Oi = np.array([0.2, -0.1, 0.3]), np.array([0.1, 0.2, -0.15])
+    Ei = np.array([0.05, 0.1, -0.07, 0.02, -0.04])
+    s.L.stokes_set_Ui(s.sys, *Ui)
+    s.L.stokes_set_Oi(s.sys, *Oi)
+    s.L.stokes_set_Ei(s.sys, *Ei)
+    o.s.Ui[:], o.s.Oi[:], o.s.Ei[:] = Ui, Oi, Ei
+    d = 3 if version == 0 else 6
+    if version < 2:
+        G = o.matrix()[:d * nm, :d * nm]
+    else:
+        G = np.stack([o.fts_mix_ft(nm, np.eye(6 * nm)[c])
+                      for c in range(6 * nm)], axis=1)
+    w = np.linalg.eigvalsh(0.5 * (G + G.T))
+    e0, e1 = 0.95 * w[0], 1.05 * w[-1]
+    nch, pe, dt = 30, 1.0, 1e-3
+    a = po.cheb_coef(nch, 0, e0, e1)
+    rng = np.random.default_rng(31 + version)
+    F = rng.standard_normal(3 * nm)
+    T = rng.standard_normal(3 * nm) if version > 0 else None
+    bd = ls.BD(s, peclet=pe, n_cheb=nch, eps=1e-6, dt_lim=1e-6, rmin_ol=1.0,
+               seed=4, force_dev=ls.DeviceArray.from_host(F), bounds=(e0, e1))
+    if version > 0:
+        bd.set_torque(ls.DeviceArray.from_host(T))
+    if scheme:
+        bd.set_scheme(1, 2.0)
+    s.set_iter(eps=1e-13)
+    if version == 2:
+        bd.set_fts_iter(30, 2000, 1e-13)
+    x = pos.reshape(-1)[:3 * nm].copy()
+    xd = ls.DeviceArray.from_host(x)
+    for step in range(2):
+        info = bd.step(xd, dt)
+        assert info.attempts == 1
+        y, _ = bd.last_noise()
+        full = pos.reshape(-1).copy()
+        full[:3 * nm] = x
+        o.pos[:] = full
+        xo, _, dto, used, nh, err = o.bd_mid(full, y[None], pe, dt, nch, a, e0,
+                                             e1, F=F, T=T, nm=nm,
+                                             scheme=scheme, bb_n=2.0)
+        assert used == 1 and nh == 0 and dto == info.dt
+        disp = np.abs(xo - x).max()
+        ex = np.abs(xd.to_host() - xo).max() / disp
+        _say(f"  step {step}: x err / disp {ex:.2e}")
+        assert ex < (1e-8 if version == 2 else 1e-10)
+        x = xo
+    bd.close()

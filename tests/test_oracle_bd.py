@@ -550,3 +550,77 @@ def test_bd_retry_after_second_stage_overlap_keeps_positions():
     disp = np.abs(want - x).max()
     assert np.abs(xn - want).max() < 1e-12 * disp
     assert np.abs(other - want).max() > 1e-6 * disp
+
+
+def _uinf(p, Ui, Oi, Ei):
+    E = np.array([[Ei[0], Ei[1], Ei[2]], [Ei[1], Ei[4], Ei[3]],
+                  [Ei[2], Ei[3], -Ei[0] - Ei[4]]])
+    p = p.reshape(-1, 3)
+    return (Ui + np.cross(Oi, p) + p @ E.T).reshape(-1)
+
+
+@pytest.mark.parametrize("scheme,nm", [(0, 8), (1, 8), (0, 5)])
+def test_bd_step_in_imposed_flow_composition(scheme, nm):
+    """F steps in an imposed flow (Ui, Oi, Ei, no shear): every drift is
+    solve_mix_3f in the labo frame, M f plus the flow at the positions of
+    the product (shift_rest_to_labo_U, f.c:621-660), so BB03's combination
+    of three drift products carries the flow three times, as the reference
+    computes it (brownian.c:2818-2829); with fixed particles their
+    rest-frame velocities are minus the flow at the positions of particles
+    0..nf-1 (ewald-3f.c:301); the noise stays in the rest frame.  Composed
+    from the compiled reference's atimes_3all"""
+    import ctypes as C
+    L = 8.0
+    pos = _eight(L)
+    s = po.System(0, pos, box=(L, L, L), tr=1.0)
+    Ui, Oi = np.array([0.2, -0.1, 0.3]), np.array([0.1, 0.2, -0.15])
+    Ei = np.array([0.05, 0.1, -0.07, 0.02, -0.04])
+    s.s.Ui[:], s.s.Oi[:], s.s.Ei[:] = Ui, Oi, Ei
+    n = 3 * nm
+    rng = np.random.default_rng(50 + scheme + nm)
+    y = rng.standard_normal(n)
+    F = rng.standard_normal(n)
+    pe, dt, nch, bbn = 1.0, 1e-3, 30, 2.0
+    G = s.matrix()[:n, :n]
+    w = np.linalg.eigvalsh(0.5 * (G + G.T))
+    e0, e1 = 0.95 * w[0], 1.05 * w[-1]
+    a = po.cheb_coef(nch, 0, e0, e1)
+    xn, _, dtt, used, nh, err = s.bd_mid(pos, y[None], pe, dt, nch, a, e0, e1,
+                                         F=F, scheme=scheme, bb_n=bbn, nm=nm)
+    assert used == 1 and nh == 0
+    R = po.ref()
+
+    def drift(p, f):
+        full = pos.copy()
+        full[:nm] = p.reshape(-1, 3)
+        t = po.System(0, full, box=(L, L, L), tr=1.0)
+        h = R.ref_sys_new(C.byref(t.s))
+        M = np.zeros((24, 24))
+        for k in range(24):
+            e = np.zeros(24)
+            e[k] = 1.0
+            u = np.zeros(24)
+            R.ref_atimes(h, 24, po._p(e), po._p(u))
+            M[:, k] = u
+        R.ref_sys_free(h)
+        if nm == 8:
+            um = M @ f
+        else:
+            uf0 = -_uinf(full[:8 - nm], Ui, Oi, Ei)
+            Ff = np.linalg.solve(M[n:, n:], uf0 - M[n:, :n] @ f)
+            um = M[:n, :n] @ f + M[:n, n:] @ Ff
+        return um + _uinf(full[:nm], Ui, Oi, Ei)
+
+    z = po.cheb_eval_dense(a, G, y, e0, e1)
+    fz = math.sqrt(2 / (pe * dt)) * z
+    x = pos[:nm].reshape(-1)
+    if scheme == 0:
+        x1 = x + 0.5 * dt * drift(x, F + fz)
+        want = x + dt * drift(x1, F + fz)
+    else:
+        u0 = drift(x, fz)
+        x1 = x + dt / bbn * u0
+        uB = u0 + 0.5 * bbn * (drift(x1, fz) - u0)
+        want = x + dt * (drift(x1, F) + uB)
+    disp = np.abs(want - x).max()
+    assert np.abs(xn - want).max() < 1e-12 * disp
