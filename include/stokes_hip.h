@@ -256,8 +256,8 @@ typedef struct stokes_hip_bd_info {
  * (its products are matrix-free). */
 int stokes_hip_bd_set_real_cache (stokes_hip_bd *bd, int enable);
 
-/* sys: version 0 (F), 1 (FT) or 2 (FTS), no shear (refused here and at
- * each step), set up as for atimes_3all; particles nm..np-1 (stokes_set_np) are fixed: the step
+/* sys: version 0 (F), 1 (FT) or 2 (FTS), set up as for atimes_3all (a
+ * sheared system must be periodic and stepped by stokes_hip_bd_step_t); particles nm..np-1 (stokes_set_np) are fixed: the step
  * moves x [3 nm] (and q [4 nm]), the fixed positions are read from sys->pos
  * at each step.  Unequal radii (sys->a, read at creation): the noise
  * operator carries BD_atimes_mob_FU's Global scaling (U / a, O / a^3,
@@ -289,6 +289,19 @@ int stokes_hip_bd_step (stokes_hip_bd *bd, double *x_dev, double dt,
  * tracks none */
 int stokes_hip_bd_step_q (stokes_hip_bd *bd, double *x_dev, double *q_dev,
                           double dt, stokes_hip_bd_info *info);
+/* the same at time t: a sheared system (sys->shear_mode 1 or 2, periodic)
+ * takes its cell shift from t as BD_evolve_mid / BB03 / BM97 do
+ * (stokes_set_shear_shift at t for the noise and the first stage, at t + h
+ * for the second, h = dt / 2, dt / n or dt; BB03's deterministic product
+ * at t with the intermediate positions, brownian.c:2393, 2468, 2741, 2805,
+ * 3092), with the reference of stokes_hip_bd_set_shear_ref; a sheared
+ * system must be stepped by this call.  The attempts of a sheared system
+ * run eagerly (the shift changes between the products) */
+int stokes_hip_bd_step_t (stokes_hip_bd *bd, double *x_dev, double *q_dev,
+                          double t, double dt, stokes_hip_bd_info *info);
+/* BD_set_shear_shift_ref (brownian.c:274-280): the cell shift s0 at time
+ * t0 (default 0, 0) */
+int stokes_hip_bd_set_shear_ref (stokes_hip_bd *bd, double t0, double s0);
 /* draw the noise from the reference's own stream instead of the device
  * generator: MT19937 seeded as KIrand_init_genrand (KIrand.c:137-157) with
  * the polar method of KIrand_Gaussian (KIrand.c:108-135), the n = 3 nm or

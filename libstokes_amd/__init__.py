@@ -71,6 +71,7 @@ EXPORTS = [
     "stokes_hip_bd_set_real_cache", "stokes_hip_set_real_cache",
     "stokes_hip_bd_last_noise", "stokes_hip_bd_destroy",
     "stokes_hip_bd_step_q", "stokes_hip_bd_set_torque",
+    "stokes_hip_bd_step_t", "stokes_hip_bd_set_shear_ref",
     "stokes_hip_bd_set_fts_iter", "stokes_hip_ensemble_autotune",
     "stokes_hip_bd_set_scheme", "stokes_hip_set_dense_blocks",
     "stokes_hip_bd_set_rng_mt19937",
@@ -152,6 +153,7 @@ def lib():
     L.stokes_set_l.argtypes = [sp, C.c_double, C.c_double, C.c_double]
     L.stokes_set_xi.argtypes = [sp, C.c_double, C.c_double]
     L.stokes_set_Ui.argtypes = [sp] + [C.c_double] * 3
+    L.stokes_set_shear.argtypes = [sp, C.c_int, C.c_double]
     L.stokes_set_Oi.argtypes = [sp] + [C.c_double] * 3
     L.stokes_set_Ei.argtypes = [sp] + [C.c_double] * 5
     L.xi_by_tratio.restype = C.c_double
@@ -240,6 +242,9 @@ def lib():
                                      C.POINTER(BDInfo)]
     L.stokes_hip_bd_step_q.argtypes = [_vp, _vp, _vp, C.c_double,
                                        C.POINTER(BDInfo)]
+    L.stokes_hip_bd_step_t.argtypes = [_vp, _vp, _vp, C.c_double, C.c_double,
+                                       C.POINTER(BDInfo)]
+    L.stokes_hip_bd_set_shear_ref.argtypes = [_vp, C.c_double, C.c_double]
     L.stokes_hip_bd_set_torque.argtypes = [_vp, _vp]
     L.stokes_hip_bd_set_scheme.argtypes = [_vp, C.c_int, C.c_double]
     L.stokes_hip_bd_set_rng_mt19937.argtypes = [_vp, C.c_ulong]
@@ -607,17 +612,26 @@ class BD:
                                                 int(max_iter), float(eps)),
               "bd_set_fts_iter")
 
-    def step(self, x_dev, dt, q_dev=None):
+    def step(self, x_dev, dt, q_dev=None, t=None):
+        """one step; t: the time of the step (needed for a sheared system)"""
         info = BDInfo()
-        rc = self.L.stokes_hip_bd_step_q(
-            self.h, x_dev.ptr, q_dev.ptr if q_dev is not None else None,
-            float(dt), C.byref(info))
+        qp = q_dev.ptr if q_dev is not None else None
+        if t is None:
+            rc = self.L.stokes_hip_bd_step_q(self.h, x_dev.ptr, qp, float(dt),
+                                             C.byref(info))
+        else:
+            rc = self.L.stokes_hip_bd_step_t(self.h, x_dev.ptr, qp, float(t),
+                                             float(dt), C.byref(info))
         if rc != 0:
             msg = self.L.stokes_hip_last_error()
             raise StokesError(f"bd_step failed ({rc}): "
                               f"{msg.decode() if msg else ''} "
                               f"[{info.summary()}]")
         return info
+
+    def set_shear_ref(self, t0, s0):
+        check(self.L.stokes_hip_bd_set_shear_ref(self.h, float(t0), float(s0)),
+              "bd_set_shear_ref")
 
     def last_noise(self):
         n = (3 if self.stokes.version == 0 else 6) * self.stokes.s.nm

@@ -69,6 +69,11 @@ struct stokes_hip_bd {
   double *buf;                 /* device: y z w f u work[4n] xm xn qn */
   double *y, *z, *w, *f, *u, *xm, *xn, *qn, *work, *u0, *f2;
   const double *pcur;          /* device: positions of the products now */
+  int shear;                   /* sys->shear_mode of the step */
+  double t0, s0;               /* BD_set_shear_shift_ref (brownian.c:274) */
+  double sh_cur;               /* the cell shift the system holds */
+  double sh_after;             /* the shift the last attempt's second stage
+                                  set */
   shk_flow flow;               /* the system's imposed flow (Ui, Oi, Ei) */
   int has_flow;
   shk_flow graph_flow;         /* the flow the graph was captured with */
@@ -166,10 +171,11 @@ stokes_hip_bd_create (struct stokes *sys, const double *F_dev, double peclet,
       stokes_hip_set_error ("bd_create: version %d", sys->version);
       return STOKES_HIP_EARG;
     }
-  if (sys->shear_mode != 0 || sys->nm < 1 || sys->nm > sys->np)
+  if ((sys->shear_mode != 0 && !sys->periodic) || sys->nm < 1
+      || sys->nm > sys->np)
     {
-      stokes_hip_set_error ("bd_create: needs no shear and 1 <= nm <= np"
-                            " (shear is not supported)");
+      stokes_hip_set_error ("bd_create: needs 1 <= nm <= np (and a periodic"
+                            " system for shear)");
       return STOKES_HIP_EUNSUP;
     }
   if (n_cheb < 2 || !(eps > 0.0) || dt_lim < 0.0 || sys->np <= 0)
@@ -570,10 +576,24 @@ bd_cheb (stokes_hip_bd *bd, shk_dev *d, int prec, const double *y, double *z)
 }
 
 /* the kernels of one attempt, in stream order (captured for F and FT) */
+/* sheared systems: the cell shift of the products that follow (the
+ * system's parameters uploaded again, an eager attempt only) */
+static int
+bd_shift (stokes_hip_bd *bd, double shift)
+{
+  bd->sys->shear_shift = shift;
+  return stokes_hip_prepare (bd->c, 1);
+}
+
 static int
 bd_record (stokes_hip_bd *bd, shk_dev *d, int prec, int check_ol, double *x,
-           double *q, const double *p1, int noise)
+           double *q, const double *p1, int noise, const double *sh)
 {
+  /* sh (sheared systems): the cell shifts of the noise and first stage,
+   * of the second stage, and of BB03's deterministic product (which the
+   * reference takes at t with the intermediate positions, brownian.c:
+   * 2800-2816) */
+  const int shear = bd->shear && sh;
   const struct stokes *s = bd->sys;
   const size_t n = bd->n;
   const size_t n3 = (size_t) bd->nm * 3;
@@ -586,6 +606,7 @@ bd_record (stokes_hip_bd *bd, shk_dev *d, int prec, int check_ol, double *x,
     rc = bd->mt ? shk_dev_h2d_async (d, bd->y, bd->y_host, sizeof (double) * n)
                 : shk_dev_gauss (d, bd->seed, bd->counter, n, bd->y);
   /* BD_calc_FB: z = M^{-1/2} y and its accuracy (chebyshev.c:241-269) */
+  if (!rc && shear) rc = bd_shift (bd, sh[0]);
   const double *px = rc ? NULL : bd_pos (bd, d, p1, &rc);
   if (!rc) rc = shk_dev_set_pos (d, px, 1);
   bd->pcur = px;
@@ -614,13 +635,15 @@ bd_record (stokes_hip_bd *bd, shk_dev *d, int prec, int check_ol, double *x,
   /* the mobile particles only (check_overlap, brownian.c:305-310) */
   if (!rc && check_ol)
     rc = shk_dev_bd_overlap (d, bd->xm, bd->a_dev, bd->nm, s->periodic, s->lx, s->ly,
-                             s->lz, 0, 0.0, bd->rmin_ol, &bd->st_dev->overlap[0]);
+                             s->lz, shear ? s->shear_mode : 0, shear ? sh[0] : 0.0,
+                             bd->rmin_ol, &bd->st_dev->overlap[0]);
   /* stage 2: drift at the intermediate configuration with the same forces;
    * mid-point: u = M(x1) f; BM97: u = (u0 + M(x1) f) / 2; BB03: u =
    * M(x1) F + u0 + n/2 (M(x1) fact z - u0), its deterministic product at
    * x1 as in the reference (brownian.c:2770-2816); then the full step from
    * x */
   if (!rc && cache) rc = shk_dev_real_cache (d, 0, prec);
+  if (!rc && shear) rc = bd_shift (bd, sh[1]);
   const double *pm = rc ? NULL : bd_pos (bd, d, bd->xm, &rc);
   if (!rc) rc = shk_dev_set_pos (d, pm, 1);
   bd->pcur = pm;
@@ -633,6 +656,11 @@ bd_record (stokes_hip_bd *bd, shk_dev *d, int prec, int check_ol, double *x,
         rc = bd->ver == 0
           ? shk_dev_bd_force (d, bd->f2, bd->F, NULL, bd->sc_dev, n)
           : shk_dev_bd_force_ft (d, bd->f2, bd->F, bd->T, NULL, bd->sc_dev, bd->nm);
+      if (!rc && shear)
+        {
+          rc = bd_shift (bd, sh[2]);
+          if (!rc) rc = shk_dev_set_pos (d, pm, 1);
+        }
       if (!rc) rc = bd_op (bd, d, prec, bd->f2, bd->u, 0, 0);
       if (!rc) rc = shk_dev_axpby (d, bd->u, bd->u0, 1.0, 1.0, n);
     }
@@ -643,7 +671,9 @@ bd_record (stokes_hip_bd *bd, shk_dev *d, int prec, int check_ol, double *x,
                              bd->sc_dev, 1.0, bd->nm);
   if (!rc && check_ol)
     rc = shk_dev_bd_overlap (d, bd->xn, bd->a_dev, bd->nm, s->periodic, s->lx, s->ly,
-                             s->lz, 0, 0.0, bd->rmin_ol, &bd->st_dev->overlap[1]);
+                             s->lz, shear ? s->shear_mode : 0,
+                             shear ? sh[sch == 1 ? 2 : 1] : 0.0, bd->rmin_ol,
+                             &bd->st_dev->overlap[1]);
   if (!rc && noise) rc = shk_dev_counter_add (d, bd->counter, 1);
   if (!rc) rc = shk_dev_d2h_async (d, bd->st_host, bd->st_dev, sizeof (shk_bd_status));
   return rc;
@@ -683,7 +713,7 @@ bd_capture (stokes_hip_bd *bd, stokes_ctx *c, double *x, double *q)
       shk_dev_timing (d, c->timing);
       return bd_fail (rc, "bd_step (capture)");
     }
-  rc = bd_record (bd, d, c->prec, check_ol, x, q, x, 1);
+  rc = bd_record (bd, d, c->prec, check_ol, x, q, x, 1, NULL);
   if (rc)
     {
       shk_dev_capture_abort (d);
@@ -757,9 +787,39 @@ stokes_hip_bd_step (stokes_hip_bd *bd, double *x_dev, double dt,
   return stokes_hip_bd_step_q (bd, x_dev, NULL, dt, info);
 }
 
+static int bd_step (stokes_hip_bd *bd, double *x_dev, double *q_dev,
+                    int have_t, double t, double dt, stokes_hip_bd_info *info);
+
 int
 stokes_hip_bd_step_q (stokes_hip_bd *bd, double *x_dev, double *q_dev,
                       double dt, stokes_hip_bd_info *info)
+{
+  return bd_step (bd, x_dev, q_dev, 0, 0.0, dt, info);
+}
+
+int
+stokes_hip_bd_step_t (stokes_hip_bd *bd, double *x_dev, double *q_dev,
+                      double t, double dt, stokes_hip_bd_info *info)
+{
+  return bd_step (bd, x_dev, q_dev, 1, t, dt, info);
+}
+
+int
+stokes_hip_bd_set_shear_ref (stokes_hip_bd *bd, double t0, double s0)
+{
+  if (!bd)
+    {
+      stokes_hip_set_error ("bd_set_shear_ref: NULL handle");
+      return STOKES_HIP_EARG;
+    }
+  bd->t0 = t0;
+  bd->s0 = s0;
+  return 0;
+}
+
+static int
+bd_step (stokes_hip_bd *bd, double *x_dev, double *q_dev, int have_t,
+         double t, double dt, stokes_hip_bd_info *info)
 {
   if (!bd || !x_dev || !(dt > 0.0))
     {
@@ -767,11 +827,15 @@ stokes_hip_bd_step_q (stokes_hip_bd *bd, double *x_dev, double *q_dev,
       return STOKES_HIP_EARG;
     }
   if (bd->ver == 0) q_dev = NULL;      /* evolve_Euler_3all ignores q for F */
-  if (bd->sys->shear_mode != 0)
+  bd->shear = bd->sys->shear_mode != 0;
+  if (bd->shear && !have_t)
     {
-      stokes_hip_set_error ("bd_step: shear is not supported");
-      return STOKES_HIP_EUNSUP;
+      stokes_hip_set_error ("bd_step: a sheared system needs the time of the"
+                            " step (stokes_hip_bd_step_t)");
+      return STOKES_HIP_EARG;
     }
+  if (bd->shear)
+    bd->sys->shear_shift = stokes_get_shear_shift (bd->sys, t, bd->t0, bd->s0);
   /* the imposed flow of the system now (solve_mix_3all's labo frame) */
   for (int k = 0; k < 3; k++)
     {
@@ -810,7 +874,11 @@ stokes_hip_bd_step_q (stokes_hip_bd *bd, double *x_dev, double *q_dev,
   double fact = bd_fact (bd->peclet, dt_local);
   /* an attempt that steers GMRES from the host runs eagerly (FTS, and the
    * F / FT drift with fixed particles); the others replay a graph */
-  const int eager = bd->ver == 2 || bd->nm < bd->np;
+  const int eager = bd->ver == 2 || bd->nm < bd->np || bd->shear;
+  /* the cell shift at t (stokes_set_shear_shift at REDO, brownian.c:2393) */
+  const double sh_t = bd->shear
+    ? stokes_get_shear_shift (bd->sys, t, bd->t0, bd->s0) : 0.0;
+  bd->sh_cur = sh_t;
   int fresh = 1;           /* new Gaussian draw for this attempt */
   int refit_done = 0;      /* BD_calc_FB refits at most once per draw */
   /* stale: the system holds the intermediate configuration of a failed
@@ -865,8 +933,18 @@ stokes_hip_bd_step_q (stokes_hip_bd *bd, double *x_dev, double *q_dev,
           bd->y_host[k] = stokes_hip_mt_gaussian (bd->mt);
       if (run_eager)
         {
+          /* the shifts of this attempt: where the system is, at t + h
+           * (h = dt / 2, dt / n, dt), and at t */
+          const double h = bd->scheme == 0 ? 0.5 * dt_local
+            : (bd->scheme == 1 ? dt_local / bd->bb_n : dt_local);
+          const double sh[3] = {
+            bd->sh_cur,
+            bd->shear ? stokes_get_shear_shift (bd->sys, t + h, bd->t0, bd->s0) : 0.0,
+            sh_t};
           rc = bd_record (bd, d, c->prec, bd->sys->rmin == 0.0, x_dev, q_dev,
-                          stale ? bd->xs : x_dev, !keep);
+                          stale ? bd->xs : x_dev, !keep, sh);
+          if (!rc && bd->shear)
+            bd->sh_after = sh[bd->scheme == 1 ? 2 : 1];
           if (bd->real_cache && rc)
             shk_dev_real_cache (d, 0, c->prec);
         }
@@ -889,6 +967,7 @@ stokes_hip_bd_step_q (stokes_hip_bd *bd, double *x_dev, double *q_dev,
         {
           /* brownian.c:1707-1727: new bounds where the noise is taken,
            * same draw */
+          if (bd->shear) bd->sys->shear_shift = bd->sh_cur;
           rc = bd_fit_bounds (bd, c, stale ? bd->xs : x_dev);
           if (rc) goto out;
           inf.refits++;
@@ -911,7 +990,11 @@ stokes_hip_bd_step_q (stokes_hip_bd *bd, double *x_dev, double *q_dev,
           fresh = 1;
           refit_done = 0;
           keep = 0;
-          if (stage == 2) stale = 0;
+          if (stage == 2)
+            {
+              stale = 0;
+              bd->sh_cur = sh_t;
+            }
         }
       else
         {
@@ -924,6 +1007,8 @@ stokes_hip_bd_step_q (stokes_hip_bd *bd, double *x_dev, double *q_dev,
               rc = shk_dev_d2d (d, bd->xs, bd->xm, sizeof (double) * 3 * bd->nm);
               if (rc) { rc = bd_fail (rc, "bd_step"); goto out; }
               stale = 1;
+              /* and the shift the second stage left */
+              bd->sh_cur = bd->sh_after;
             }
           keep = stale;
         }

@@ -1377,8 +1377,19 @@ orc_bd_step_nm (orc_sys *s, int nm, int scheme, double bb_n, double *x,
       f[i] = ((with_ext) && ext ? ext[ie] : 0.0) + (zfac) * fact * z[i];    \
     }
 
+  /* sheared systems: the cell shift at time tt (stokes_get_shear_shift,
+   * stokes.c:554-587: s0 + L rate (tt - t0), L = ly or lz, brought into
+   * [-lx/2, lx/2)), set where the reference calls stokes_set_shear_shift
+   * (brownian.c:2393, 2468; 2630, 2741, 2805; 3017, 3092) */
+#define SHIFT(tt) do { if (s->shear_mode) {                                 \
+    double sh_ = s->bd_s0 + (s->shear_mode == 1 ? s->ly : s->lz)            \
+      * s->shear_rate * ((tt) - s->bd_t0);                                  \
+    while (sh_ < -0.5 * s->lx) sh_ += s->lx;                                \
+    while (sh_ >= 0.5 * s->lx) sh_ -= s->lx;                                \
+    s->shear_shift = sh_; } } while (0)
 redo:                              /* REDO */
   SETPOS (x);
+  SHIFT (s->bd_t);
 redo_fb:                           /* REDO_FB */
   if (k >= ndraw) goto done;
   orc_cheb_eval_atimes (ncheb, acoef, n, draws + (size_t) k * n, z, e0, e1,
@@ -1412,6 +1423,8 @@ redo_scale:                        /* REDO_scale */
   if (s->rmin == 0.0 && orc_check_overlap_n (s, xmid, nm, rmin_ol) > 0)
     goto overlap;
   SETPOS (xmid);
+  SHIFT (s->bd_t + (scheme == 0 ? dt_local * 0.5
+                    : (scheme == 1 ? dt_local / bb_n : dt_local)));
   orc_bd_drift (n, f, u, (void *) &ctx);
   if (scheme == 2)
     for (int i = 0; i < n; i++) u[i] = 0.5 * (u0[i] + u[i]);
@@ -1420,6 +1433,7 @@ redo_scale:                        /* REDO_scale */
       const double bbf = 0.5 * bb_n;
       for (int i = 0; i < n; i++) u0[i] = u0[i] + bbf * (u[i] - u0[i]);
       FORCES (1, 0.0);
+      SHIFT (s->bd_t);                         /* brownian.c:2805 */
       orc_bd_drift (n, f, u, (void *) &ctx);   /* still at xmid */
       for (int i = 0; i < n; i++) u[i] += u0[i];
     }
@@ -1453,6 +1467,7 @@ done:
 #undef UIDX
 #undef FORCES
 #undef SETPOS
+#undef SHIFT
   s->pos = pos0;
   free (pfull);
   *used = k;

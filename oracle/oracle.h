@@ -44,6 +44,8 @@ typedef struct orc_sys {
                            the FTS sum xi-independent (0: as the reference) */
   double Ui[3], Oi[3], Ei[5];  /* imposed flow (stokes.h:116-118): read by
                                   the BD step's drift only */
+  double shear_rate;    /* sheared BD steps: the shift at time tt is */
+  double bd_t, bd_t0, bd_s0;   /* s0 + L rate (tt - t0), the step at bd_t */
 } orc_sys;
 
 /* set-up (stokes.c) */

@@ -37,6 +37,8 @@ class OrcSys(C.Structure):
         ("shear_mode", C.c_int), ("shear_shift", C.c_double),
         ("fix_recip_g", C.c_int),
         ("Ui", C.c_double * 3), ("Oi", C.c_double * 3), ("Ei", C.c_double * 5),
+        ("shear_rate", C.c_double), ("bd_t", C.c_double),
+        ("bd_t0", C.c_double), ("bd_s0", C.c_double),
     ]
 
 

@@ -168,14 +168,14 @@ def test_chebyshev_host_functions_match_oracle():
             po.cheb_eval(a, 0.1, 3.0, x), rel=1e-14)
 
 
-def test_bd_refuses_shear():
-    """BD under shear (a time-dependent cell shift) is not supported: such a
-    system is refused before any device work, not stepped unsheared"""
+def test_bd_refuses_shear_without_a_box():
+    """shear needs a periodic system: an open system with a shear mode is
+    refused before any device work"""
     pos = np.array([[1.0, 1.0, 1.0], [4.0, 1.0, 1.0]])
-    s = ls.Stokes(0, pos, box=(8.0, 8.0, 8.0), tr=1.0)
+    s = ls.Stokes(0, pos, tr=1.0)
     out = C.c_void_p()
     s.L.stokes_set_shear(s.sys, 1, 0.5)
     rc = s.L.stokes_hip_bd_create(s.sys, None, 1.0, 20, 1e-3, 1e-6, 1.0, 1,
                                   C.byref(out))
-    assert rc == -4 and not out.value
-    assert b"shear" in s.L.stokes_hip_last_error()
+    assert rc == -4 and not out.value                # STOKES_HIP_EUNSUP
+    assert b"periodic" in s.L.stokes_hip_last_error()

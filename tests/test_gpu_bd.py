@@ -737,3 +737,75 @@ def test_bd_imposed_flow_matches_oracle(version, n, nm, scheme):
         assert ex < (1e-8 if version == 2 else 1e-10)
         x = xo
     bd.close()
+
+
+@pytest.mark.parametrize("version,n,scheme", [(0, 27, 0), (0, 27, 1),
+                                              (1, 27, 2), (2, 12, 0)])
+def test_bd_sheared_steps_match_oracle(version, n, scheme):
+    """steps of a sheared box (shear_mode 1, stokes_set_shear's imposed
+    flow, the cell shift from the step's time: t for the noise and the
+    first stage, t + h for the second, BB03's deterministic product at t,
+    brownian.c:2393-2468, 2630-2805, 3017-3092) against the oracle's step
+    fed with the device's draw"""
+    L0, phi, rate, t0, s0 = None, 0.1, 0.8, 1.0, 0.4
+    _say(f"bd sheared version={version} N={n} scheme={scheme}")
+    pos, L = workloads.configuration(n, phi, seed=14)
+    s = ls.Stokes(version, pos, box=(L, L, L), tr=1.0)
+    s.L.stokes_set_shear(s.sys, 1, rate)
+    o = po.System(version, pos, box=(L, L, L), tr=1.0, shear_mode=1)
+    o.s.Oi[:] = [0.0, 0.0, -0.5 * rate]
+    o.s.Ei[:] = [0.0, 0.5 * rate, 0.0, 0.0, 0.0]
+    o.s.shear_rate, o.s.bd_t0, o.s.bd_s0 = rate, t0, s0
+
+    def shift(tt):
+        v = s0 + L * rate * (tt - t0)
+        while v < -L / 2:
+            v += L
+        while v >= L / 2:
+            v -= L
+        return v
+
+    t = 2.3
+    o.s.shear_shift = shift(t)
+    d = 3 if version == 0 else 6
+    if version < 2:
+        G = o.matrix()
+    else:
+        G = np.stack([o.fts_mix_ft(n, np.eye(6 * n)[c]) for c in range(6 * n)],
+                     axis=1)
+    w = np.linalg.eigvalsh(0.5 * (G + G.T))
+    e0, e1 = 0.9 * w[0], 1.1 * w[-1]
+    nch, pe, dt = 30, 1.0, 1e-3
+    a = po.cheb_coef(nch, 0, e0, e1)
+    rng = np.random.default_rng(41 + version)
+    F = rng.standard_normal(3 * n)
+    T = rng.standard_normal(3 * n) if version > 0 else None
+    bd = ls.BD(s, peclet=pe, n_cheb=nch, eps=1.0, dt_lim=1e-6, rmin_ol=1.0,
+               seed=6, force_dev=ls.DeviceArray.from_host(F), bounds=(e0, e1))
+    bd.set_shear_ref(t0, s0)
+    if version > 0:
+        bd.set_torque(ls.DeviceArray.from_host(T))
+    if scheme:
+        bd.set_scheme(scheme, 2.0)
+    s.set_iter(eps=1e-13)
+    if version == 2:
+        bd.set_fts_iter(30, 2000, 1e-13)
+    x = pos.reshape(-1).copy()
+    xd = ls.DeviceArray.from_host(x)
+    for step in range(2):
+        info = bd.step(xd, dt, t=t)
+        assert info.attempts == 1
+        y, _ = bd.last_noise()
+        o.pos[:] = x
+        o.s.bd_t = t
+        xo, _, dto, used, nh, err = o.bd_mid(x, y[None], pe, dt, nch, a, e0,
+                                             e1, F=F, T=T, scheme=scheme,
+                                             bb_n=2.0)
+        assert used == 1 and nh == 0 and dto == info.dt
+        disp = np.abs(xo - x).max()
+        ex = np.abs(xd.to_host() - xo).max() / disp
+        _say(f"  step {step} (t = {t}): x err / disp {ex:.2e}")
+        assert ex < (1e-8 if version == 2 else 1e-10)
+        x = xo
+        t += dt
+    bd.close()

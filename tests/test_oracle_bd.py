@@ -624,3 +624,72 @@ def test_bd_step_in_imposed_flow_composition(scheme, nm):
         want = x + dt * (drift(x1, F) + uB)
     disp = np.abs(want - x).max()
     assert np.abs(xn - want).max() < 1e-12 * disp
+
+
+@pytest.mark.parametrize("scheme", [0, 1])
+def test_bd_sheared_step_composition(scheme):
+    """an F step of a sheared box (shear_mode 1, rate 0.8, reference shift
+    s0 at t0): the noise and first stage at the cell shift of t, the second
+    at t + h (mid-point h = dt/2), BB03's deterministic product back at t
+    with the intermediate positions (brownian.c:2393, 2468, 2741, 2805);
+    the drift carries the shear flow of stokes_set_shear (Oi_z = -rate/2,
+    Ei_xy = rate/2).  Composed from the compiled reference's atimes_3all at
+    those shifts"""
+    import ctypes as C
+    L, rate, t, t0, s0, bbn = 8.0, 0.8, 3.7, 1.0, 0.4, 2.0
+    pos = _eight(L)
+    s = po.System(0, pos, box=(L, L, L), tr=1.0, shear_mode=1)
+    Ui, Oi = np.zeros(3), np.array([0.0, 0.0, -0.5 * rate])
+    Ei = np.array([0.0, 0.5 * rate, 0.0, 0.0, 0.0])
+    s.s.Oi[:], s.s.Ei[:] = Oi, Ei
+    s.s.shear_rate, s.s.bd_t, s.s.bd_t0, s.s.bd_s0 = rate, t, t0, s0
+
+    def shift(tt):
+        v = s0 + L * rate * (tt - t0)
+        while v < -L / 2:
+            v += L
+        while v >= L / 2:
+            v -= L
+        return v
+
+    n = 24
+    rng = np.random.default_rng(60 + scheme)
+    y = rng.standard_normal(n)
+    F = rng.standard_normal(n)
+    pe, dt, nch = 1.0, 1e-3, 30
+    s.s.shear_shift = shift(t)
+    G = s.matrix()
+    w = np.linalg.eigvalsh(0.5 * (G + G.T))
+    e0, e1 = 0.95 * w[0], 1.05 * w[-1]
+    a = po.cheb_coef(nch, 0, e0, e1)
+    xn, _, dtt, used, nh, err = s.bd_mid(pos, y[None], pe, dt, nch, a, e0, e1,
+                                         F=F, scheme=scheme, bb_n=bbn)
+    assert used == 1 and nh == 0
+    R = po.ref()
+
+    def drift(p, f, sh):
+        tt = po.System(0, p.reshape(-1, 3), box=(L, L, L), tr=1.0,
+                       shear_mode=1, shear_shift=sh)
+        h = R.ref_sys_new(C.byref(tt.s))
+        u = np.zeros(n)
+        R.ref_atimes(h, n, po._p(np.ascontiguousarray(f)), po._p(u))
+        R.ref_sys_free(h)
+        return u + _uinf(p, Ui, Oi, Ei)
+
+    z = po.cheb_eval_dense(a, G, y, e0, e1)
+    fz = math.sqrt(2 / (pe * dt)) * z
+    x = pos.reshape(-1)
+    if scheme == 0:
+        x1 = x + 0.5 * dt * drift(x, F + fz, shift(t))
+        want = x + dt * drift(x1, F + fz, shift(t + 0.5 * dt))
+    else:
+        u0 = drift(x, fz, shift(t))
+        x1 = x + dt / bbn * u0
+        uB = u0 + 0.5 * bbn * (drift(x1, fz, shift(t + dt / bbn)) - u0)
+        want = x + dt * (drift(x1, F, shift(t)) + uB)
+    disp = np.abs(want - x).max()
+    assert np.abs(xn - want).max() < 1e-12 * disp
+    # the shifts matter: the same without the change at t + h differs
+    if scheme == 0:
+        other = x + dt * drift(x1, F + fz, shift(t))
+        assert np.abs(other - want).max() > 1e-9 * disp
