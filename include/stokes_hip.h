@@ -268,7 +268,11 @@ int stokes_hip_bd_set_real_cache (stokes_hip_bd *bd, int enable);
  * n_cheb: order of the 1/sqrt(x) fit (BD->n_minv > 0).  eps: refit
  * threshold (BD->eps).  dt_lim, rmin_ol: BD->dt_lim and BD->rmin (pairs
  * with r^2 < rmin_ol (a_i+a_j)^2 overlap; checked only when sys->rmin is
- * 0, as in the reference).  seed: key of the noise generator. */
+ * 0, as in the reference).  seed: key of the noise generator.
+ * FTS: the drift's known strain is the imposed flow's alone (E = -Ei in the
+ * rest frame), i.e. the reference's BD->E (brownian.c:146, 2304, 2795) is
+ * taken as zero; there is no entry point for a nonzero BD->E, so a caller
+ * that needs one cannot use this integrator. */
 int stokes_hip_bd_create (struct stokes *sys, const double *F_dev,
                           double peclet, int n_cheb, double eps,
                           double dt_lim, double rmin_ol,

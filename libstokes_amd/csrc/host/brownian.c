@@ -591,8 +591,8 @@ bd_record (stokes_hip_bd *bd, shk_dev *d, int prec, int check_ol, double *x,
 {
   /* sh (sheared systems): the cell shifts of the noise and first stage,
    * of the second stage, and of BB03's deterministic product (which the
-   * reference takes at t with the intermediate positions, brownian.c:
-   * 2800-2816) */
+   * reference takes at t and at x: BD_add_FP (BD, x, f) moves the system
+   * back there, brownian.c:2209-2212, 2800-2816) */
   const int shear = bd->shear && sh;
   const struct stokes *s = bd->sys;
   const size_t n = bd->n;
@@ -639,9 +639,8 @@ bd_record (stokes_hip_bd *bd, shk_dev *d, int prec, int check_ol, double *x,
                              bd->rmin_ol, &bd->st_dev->overlap[0]);
   /* stage 2: drift at the intermediate configuration with the same forces;
    * mid-point: u = M(x1) f; BM97: u = (u0 + M(x1) f) / 2; BB03: u =
-   * M(x1) F + u0 + n/2 (M(x1) fact z - u0), its deterministic product at
-   * x1 as in the reference (brownian.c:2770-2816); then the full step from
-   * x */
+   * M(x) F + u0 + n/2 (M(x1) fact z - u0), its deterministic product back
+   * at x (brownian.c:2800-2816); then the full step from x */
   if (!rc && cache) rc = shk_dev_real_cache (d, 0, prec);
   if (!rc && shear) rc = bd_shift (bd, sh[1]);
   const double *pm = rc ? NULL : bd_pos (bd, d, bd->xm, &rc);
@@ -656,11 +655,10 @@ bd_record (stokes_hip_bd *bd, shk_dev *d, int prec, int check_ol, double *x,
         rc = bd->ver == 0
           ? shk_dev_bd_force (d, bd->f2, bd->F, NULL, bd->sc_dev, n)
           : shk_dev_bd_force_ft (d, bd->f2, bd->F, bd->T, NULL, bd->sc_dev, bd->nm);
-      if (!rc && shear)
-        {
-          rc = bd_shift (bd, sh[2]);
-          if (!rc) rc = shk_dev_set_pos (d, pm, 1);
-        }
+      if (!rc && shear) rc = bd_shift (bd, sh[2]);
+      const double *p0 = rc ? NULL : bd_pos (bd, d, x, &rc);
+      if (!rc) rc = shk_dev_set_pos (d, p0, 1);
+      bd->pcur = p0;
       if (!rc) rc = bd_op (bd, d, prec, bd->f2, bd->u, 0, 0);
       if (!rc) rc = shk_dev_axpby (d, bd->u, bd->u0, 1.0, 1.0, n);
     }
@@ -883,8 +881,10 @@ bd_step (stokes_hip_bd *bd, double *x_dev, double *q_dev, int have_t,
   int refit_done = 0;      /* BD_calc_FB refits at most once per draw */
   /* stale: the system holds the intermediate configuration of a failed
    * attempt (bd->xs), as the reference's REDO_scale leaves it after an
-   * overlap in the second stage (brownian.c:2523, 2881, 3162); such an
-   * attempt runs eagerly.  keep: the attempt keeps the noise of the
+   * overlap in the second stage of the mid-point and BM97 schemes
+   * (brownian.c:2523, 3162; BB03 has moved back to x for its deterministic
+   * product, 2808, so its retries start from x); such an attempt runs
+   * eagerly.  keep: the attempt keeps the noise of the
    * previous one (REDO_scale); the graph's replays at x recompute it from
    * the same draw, which gives the same values */
   int stale = 0, keep = 0;
@@ -998,11 +998,12 @@ bd_step (stokes_hip_bd *bd, double *x_dev, double *q_dev, int have_t,
         }
       else
         {
-          /* REDO_scale: the noise kept; after the second stage the system
-           * holds this attempt's intermediate configuration */
+          /* REDO_scale: the noise kept; after the second stage of the
+           * mid-point and BM97 schemes the system holds this attempt's
+           * intermediate configuration (BB03's is back at x) */
           fact = bd_fact (bd->peclet, dt_local);
           fresh = 0;
-          if (stage == 2)
+          if (stage == 2 && bd->scheme != 1)
             {
               rc = shk_dev_d2d (d, bd->xs, bd->xm, sizeof (double) * 3 * bd->nm);
               if (rc) { rc = bd_fail (rc, "bd_step"); goto out; }

@@ -1327,9 +1327,9 @@ orc_bd_mob (int n, const double *x, double *y, void *ud, int scaled)
  *   mid:  u0 = M(x) f, x1 = x + dt/2 u0 | u = M(x1) f, x' = x + dt u
  *   BM97: u0 = M(x) f, x1 = x + dt u0   | u = (u0 + M(x1) f) / 2, x' = x + dt u
  *   BB03: u0 = M(x) fact z, x1 = x + dt/n u0 | uB = u0 + n/2 (M(x1) fact z
- *         - u0), u = M(x1) F + uB (the deterministic product at x1: the
- *         reference does not reset the positions before it,
- *         brownian.c:2770-2816), x' = x + dt u
+ *         - u0), u = M(x) F + uB (the deterministic product back at x and
+ *         the shift of t: BD_add_FP (BD, x, f) moves the system to x before
+ *         it, brownian.c:2209-2212, 2800-2816), x' = x + dt u
  * with the overlap checks after x1 (stage 1) and x' (stage 2).
  *
  * Particles nm..np-1 are fixed (BD_atimes_mob_FU's zero padding for F /
@@ -1400,10 +1400,11 @@ redo_fb:                           /* REDO_FB */
 redo_scale:                        /* REDO_scale */
   /* the positions are not set here (REDO_scale, brownian.c:2400, 2637,
    * 3025): after an overlap in the second stage with dt / 2 still above
-   * dt_lim (brownian.c:2523, 2881, 3162) the system keeps the intermediate
-   * configuration of the failed attempt, and the first-stage product of
-   * the retry is taken there; the noise is kept (computed where it was
-   * drawn) */
+   * dt_lim (brownian.c:2523, 3162) the mid-point and BM97 systems keep the
+   * intermediate configuration of the failed attempt, and the first-stage
+   * product of the retry is taken there; BB03 has moved back to x for its
+   * deterministic product (2808), so its retries start from x; the noise
+   * is kept (computed where it was drawn) */
   for (int i = 0; i < n3; i++) xmid[i] = x[i];
   if (scheme == 1)
     {
@@ -1434,7 +1435,8 @@ redo_scale:                        /* REDO_scale */
       for (int i = 0; i < n; i++) u0[i] = u0[i] + bbf * (u[i] - u0[i]);
       FORCES (1, 0.0);
       SHIFT (s->bd_t);                         /* brownian.c:2805 */
-      orc_bd_drift (n, f, u, (void *) &ctx);   /* still at xmid */
+      SETPOS (x);                              /* BD_add_FP, 2808 */
+      orc_bd_drift (n, f, u, (void *) &ctx);
       for (int i = 0; i < n; i++) u[i] += u0[i];
     }
   for (int i = 0; i < n3; i++) xmid[i] = x[i] + dt_local * u[UIDX (i)];

@@ -106,6 +106,7 @@ def ref():
         R.ref_cpu_split.argtypes = [C.c_void_p, C.c_int]
         R.ref_cheb_eval.restype = C.c_double
         R.ref_cheb_error_minvsqrt_dense.restype = C.c_double
+        R.ref_cheb_error_Rsqrt_dense.restype = C.c_double
         R.ref_rng_new.restype = C.c_void_p
         R.ref_rng_new.argtypes = [C.c_ulong]
         R.ref_rng_free.argtypes = [C.c_void_p]
@@ -408,6 +409,30 @@ def cheb_eval_dense(a, mat, y, e0, e1):
                            C.c_int(n), _p(y), _p(z), C.c_double(e0),
                            C.c_double(e1), cb, C.byref(d))
     return z
+
+
+def cheb_error_dense(kind, mat, y, z):
+    """the oracle's chebyshev_error_minvsqrt (kind 0, chebyshev.c:241-269)
+    or chebyshev_error_Rsqrt (kind 1, 282-310) with the dense operator mat"""
+    mat = np.ascontiguousarray(mat, np.float64)
+    y = np.ascontiguousarray(y, np.float64)
+    z = np.ascontiguousarray(z, np.float64)
+    d = _Dense(y.size, _p(mat))
+    L = lib()
+    cb = C.cast(L.orc_atimes_dense_cb, C.c_void_p)
+    f = L.orc_cheb_error_minvsqrt if kind == 0 else L.orc_cheb_error_Rsqrt
+    return f(C.c_int(y.size), _p(y), _p(z), cb, C.byref(d))
+
+
+def ref_cheb_error_dense(kind, mat, y, z):
+    """the compiled reference's chebyshev_error_minvsqrt / _Rsqrt"""
+    mat = np.ascontiguousarray(mat, np.float64)
+    y = np.ascontiguousarray(y, np.float64)
+    z = np.ascontiguousarray(z, np.float64)
+    R = ref()
+    f = (R.ref_cheb_error_minvsqrt_dense if kind == 0
+         else R.ref_cheb_error_Rsqrt_dense)
+    return f(C.c_int(y.size), _p(mat), _p(y), _p(z))
 
 
 def cheb_eval_sys(a, system, y, e0, e1):

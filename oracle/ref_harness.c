@@ -134,6 +134,14 @@ ref_cheb_error_minvsqrt_dense (int n, const double *m, const double *y,
   return chebyshev_error_minvsqrt (n, y, z, ref_dense_atimes, &d);
 }
 
+double
+ref_cheb_error_Rsqrt_dense (int n, const double *m, const double *y,
+                            const double *z)
+{
+  ref_dense d = {n, m};
+  return chebyshev_error_Rsqrt (n, y, z, ref_dense_atimes, &d);
+}
+
 void
 ref_cheb_eval_sys (int ncheb, const double *a, struct stokes *s,
                    const double *y, double *z, double e0, double e1)

@@ -211,6 +211,26 @@ def main():
         store[f"cheb{fid}/a"] = a
         store[f"cheb{fid}/z"] = z
         store[f"cheb{fid}/bounds"] = np.array([e0, e1])
+    # chebyshev_error_minvsqrt / _Rsqrt (chebyshev.c:241-310) on the same
+    # matrix: low-order fits (errors of order 1e-3) and unrelated vectors
+    rng = np.random.default_rng(77)
+    for fid, nch in ((0, 5), (1, 4)):
+        a = np.zeros(nch)
+        R.ref_cheb_coef(C.c_int(nch), C.c_int(fid), C.c_double(e0),
+                        C.c_double(e1), a.ctypes.data_as(C.POINTER(C.c_double)))
+        y = rng.standard_normal(n)
+        z = np.zeros(n)
+        R.ref_cheb_eval_dense(
+            C.c_int(nch), a.ctypes.data_as(C.POINTER(C.c_double)), C.c_int(n),
+            A.ctypes.data_as(C.POINTER(C.c_double)),
+            y.ctypes.data_as(C.POINTER(C.c_double)),
+            z.ctypes.data_as(C.POINTER(C.c_double)), C.c_double(e0),
+            C.c_double(e1))
+        zr = rng.standard_normal(n)
+        store[f"cheberr{fid}/y"] = y
+        store[f"cheberr{fid}/z"] = np.stack([z, zr])
+        store[f"cheberr{fid}/err"] = np.array(
+            [po.ref_cheb_error_dense(fid, A, y, zz) for zz in (z, zr)])
     # MT19937 + polar Gaussian (KIrand.c), seeds 0 and 5489
     for seed in (0, 5489):
         h = R.ref_rng_new(seed)

@@ -809,3 +809,63 @@ def test_bd_sheared_steps_match_oracle(version, n, scheme):
         x = xo
         t += dt
     bd.close()
+
+
+@pytest.mark.parametrize("scheme,halvings", [(0, 3), (1, 3)])
+def test_bd_sheared_retries_match_oracle(scheme, halvings):
+    """a sheared F step (shear_mode 1, rate 0.8) that overlaps in its first
+    stage and then in its second before it is accepted (the reference's
+    random stream, seed 18): the mid-point retry after the second-stage
+    overlap starts from the failed attempt's intermediate configuration at
+    the shift it left, BB03's from x at the shift of t (brownian.c:2400,
+    2523, 2637, 2808); against the oracle's step, itself pinned to the
+    reference's control flow composed from its compiled atimes_3all"""
+    L, gap, rate, t, t0, s0 = 8.0, 0.03, 0.8, 3.7, 1.0, 0.4
+    pos = np.array([[1, 1, 1], [4, 1.2, 0.8], [1.1, 4.2, 1], [4, 4, 1.3],
+                    [1, 1.1, 4], [4.2, 1, 4], [1, 4, 4.1], [4, 4.1, 4.2]],
+                   dtype=np.float64) + 0.3
+    pos[1] = pos[0] + np.array([2.0 + gap, 0.0, 0.0])
+    _say(f"bd sheared retries scheme={scheme}")
+    s = ls.Stokes(0, pos, box=(L, L, L), tr=1.0)
+    s.L.stokes_set_shear(s.sys, 1, rate)
+    o = po.System(0, pos, box=(L, L, L), tr=1.0, shear_mode=1)
+    o.s.Oi[:] = [0.0, 0.0, -0.5 * rate]
+    o.s.Ei[:] = [0.0, 0.5 * rate, 0.0, 0.0, 0.0]
+    o.s.shear_rate, o.s.bd_t0, o.s.bd_s0, o.s.bd_t = rate, t0, s0, t
+
+    def shift(tt):
+        v = s0 + L * rate * (tt - t0)
+        while v < -L / 2:
+            v += L
+        while v >= L / 2:
+            v -= L
+        return v
+
+    o.s.shear_shift = shift(t)
+    G = o.matrix()
+    w = np.linalg.eigvalsh(0.5 * (G + G.T))
+    e0, e1 = 0.95 * w[0], 1.05 * w[-1]
+    nch, pe, dt = 30, 1.0, 1e-3
+    a = po.cheb_coef(nch, 0, e0, e1)
+    F = np.zeros(24)
+    F[0], F[3] = 30.0, -30.0
+    bd = ls.BD(s, peclet=pe, n_cheb=nch, eps=1.0, dt_lim=1e-6, rmin_ol=1.0,
+               seed=5, force_dev=ls.DeviceArray.from_host(F), bounds=(e0, e1))
+    bd.set_rng_mt19937(18)
+    bd.set_shear_ref(t0, s0)
+    if scheme:
+        bd.set_scheme(1, 2.0)
+    s.set_iter(eps=1e-13)
+    xd = ls.DeviceArray.from_host(pos.reshape(-1))
+    info = bd.step(xd, dt, t=t)
+    y = po.Rng(18).gaussian(24)
+    xo, _, dto, used, nh, _ = o.bd_mid(pos.reshape(-1), y[None], pe, dt, nch,
+                                       a, e0, e1, F=F, scheme=scheme, bb_n=2.0)
+    _say(f"  halvings gpu {info.halvings} oracle {nh}, dt {info.dt} {dto}")
+    assert info.halvings == nh == halvings and info.redraws == 0 and used == 1
+    assert info.dt == dto
+    disp = np.abs(xo - pos.reshape(-1)).max()
+    ex = np.abs(xd.to_host() - xo).max() / disp
+    _say(f"  x err / disp {ex:.2e}")
+    assert ex < 1e-10
+    bd.close()

@@ -323,6 +323,27 @@ lib.chebyshev_eval_atimes(C.c_int(20), P(a), C.c_int(18), P(yy), P(zz), C.c_doub
                           C.c_double(1.1*ev[-1]), lib.atimes_3all, s0.sys)
 zo = po.cheb_eval_sys(a, o0, yy, 0.9*ev[0], 1.1*ev[-1])
 assert np.linalg.norm(zz - zo) <= 1e-12 * np.linalg.norm(zo)
+# chebyshev_error_minvsqrt / _Rsqrt (chebyshev.c:241-310) over atimes_3all,
+# for the order-20 fit and for an unrelated z, against the oracle's
+lib.chebyshev_error_minvsqrt.restype = C.c_double
+lib.chebyshev_error_Rsqrt.restype = C.c_double
+cb = C.cast(po.lib().orc_atimes_sys_cb, C.c_void_p)
+for kind, fn, of in ((0, lib.chebyshev_error_minvsqrt, po.lib().orc_cheb_error_minvsqrt),
+                     (1, lib.chebyshev_error_Rsqrt, po.lib().orc_cheb_error_Rsqrt)):
+    for z in (zz, np.random.default_rng(4 + kind).standard_normal(18)):
+        got = fn(C.c_int(18), P(yy), P(z), lib.atimes_3all, s0.sys)
+        want = of(C.c_int(18), P(yy), P(z), cb, C.byref(o0.s))
+        assert abs(got - want) <= 1e-10 * abs(want) + 1e-15, (kind, got, want)
+        print("cheb error", kind, got, want)
+# A24: with the timing split on, a legacy call writes cpu2 (real space),
+# cpu3 (reciprocal space) and cpu1 = cpu2 + cpu3 (ewald.c:647, 791-794,
+# 1077-1080)
+s.set_timing(True)
+s.s.cpu1 = s.s.cpu2 = s.s.cpu3 = -1.0
+lib.atimes_3all(C.c_int(n), P(x), P(y), s.sys)
+c1, c2, c3 = s.times()
+print("timing", c1, c2, c3)
+assert c2 > 0 and c3 > 0 and c1 == c2 + c3
 print("legacy ok")
 """ % (ROOT, os.path.join(ROOT, "oracle"))
     r = subprocess.run([sys.executable, "-c", code], capture_output=True,

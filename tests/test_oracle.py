@@ -19,6 +19,7 @@ import numpy as np
 import pytest
 from scipy.special import erfc
 
+import libstokes_amd as ls
 import pyoracle as po
 from conftest import relerr, maxrel
 
@@ -350,6 +351,48 @@ def test_chebyshev_golden(golden):
     z = po.cheb_eval_dense(po.cheb_coef(50, 0, e0, e1), A, np.ones(10), e0, e1)
     yAy = np.ones(10) @ np.linalg.solve(A, np.ones(10))
     assert z @ z == pytest.approx(yAy, rel=2e-10)
+
+
+def test_chebyshev_error_golden(golden):
+    """chebyshev_error_minvsqrt (chebyshev.c:241-269) and _Rsqrt (282-310)
+    of the compiled reference on the check-brownian.c matrix: for low-order
+    fits (errors 1e-4 ... 2e-3) and for unrelated z (errors of order 1, so
+    that a swapped denominator or operand shows); the oracle and the
+    library's exported symbols (host code, a dense callback) agree"""
+    import ctypes as C
+    data, _ = golden
+    A = data["cheb0/A"]
+    L = ls.lib()
+    cb_t = C.CFUNCTYPE(None, C.c_int, C.POINTER(C.c_double),
+                       C.POINTER(C.c_double), C.c_void_p)
+
+    def dense(n, x, y, ud):
+        xv = np.ctypeslib.as_array(x, (n,))
+        np.ctypeslib.as_array(y, (n,))[:] = A @ xv
+    cb = cb_t(dense)
+    for kind, fn in ((0, L.chebyshev_error_minvsqrt),
+                     (1, L.chebyshev_error_Rsqrt)):
+        fn.restype = C.c_double
+        fn.argtypes = [C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                       cb_t, C.c_void_p]
+        y = data[f"cheberr{kind}/y"]
+        for z, want in zip(data[f"cheberr{kind}/z"], data[f"cheberr{kind}/err"]):
+            z = np.ascontiguousarray(z)
+            assert po.cheb_error_dense(kind, A, y, z) == pytest.approx(
+                want, rel=1e-12)
+            got = fn(y.size, y.ctypes.data_as(C.POINTER(C.c_double)),
+                     z.ctypes.data_as(C.POINTER(C.c_double)), cb, None)
+            assert got == pytest.approx(want, rel=1e-12)
+        # the fit's error is the one of the closed form f(A) y
+        w, V = np.linalg.eigh(A)
+        f = 1 / np.sqrt(w) if kind == 0 else np.sqrt(w)
+        zf = V @ (f * (V.T @ y))
+        exact = po.cheb_error_dense(kind, A, y, zf)
+        assert exact < 1e-13
+    if po.ref_available():
+        z = data["cheberr1/z"][1]
+        assert po.ref_cheb_error_dense(1, A, data["cheberr1/y"], z) == \
+            data["cheberr1/err"][1]
 
 
 def test_chebyshev_scalar_fits():

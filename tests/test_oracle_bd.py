@@ -369,8 +369,8 @@ def test_bd_bb03_bm97_vs_reference_composition(scheme, bb_n):
     """BD_evolve_BB03 (brownian.c:2579-2920) and BD_evolve_BM97
     (2970-3190), F version, composed from the compiled reference's own
     chebyshev_eval_atimes and atimes_3all (no overlap in this step); BB03
-    takes its deterministic product at the intermediate configuration, as
-    the reference does (2770-2816)"""
+    takes its deterministic product back at x: BD_add_FP (BD, x, f) moves
+    the system there before it (2209-2212, 2800-2816)"""
     import ctypes as C
     L = 8.0
     pos = _eight(L)
@@ -410,7 +410,7 @@ def test_bd_bb03_bm97_vs_reference_composition(scheme, bb_n):
         u0 = ref_op(h0, fact * z)
         t1, h1 = ref_sys(x0 + dt / bb_n * u0)
         ub = u0 + 0.5 * bb_n * (ref_op(h1, fact * z) - u0)
-        u = ref_op(h1, F) + ub
+        u = ref_op(h0, F) + ub
     R.ref_sys_free(h0)
     R.ref_sys_free(h1)
     want = x0 + dt * u
@@ -621,7 +621,7 @@ def test_bd_step_in_imposed_flow_composition(scheme, nm):
         u0 = drift(x, fz)
         x1 = x + dt / bbn * u0
         uB = u0 + 0.5 * bbn * (drift(x1, fz) - u0)
-        want = x + dt * (drift(x1, F) + uB)
+        want = x + dt * (drift(x, F) + uB)
     disp = np.abs(want - x).max()
     assert np.abs(xn - want).max() < 1e-12 * disp
 
@@ -631,7 +631,7 @@ def test_bd_sheared_step_composition(scheme):
     """an F step of a sheared box (shear_mode 1, rate 0.8, reference shift
     s0 at t0): the noise and first stage at the cell shift of t, the second
     at t + h (mid-point h = dt/2), BB03's deterministic product back at t
-    with the intermediate positions (brownian.c:2393, 2468, 2741, 2805);
+    and at x (brownian.c:2393, 2468, 2741, 2805, 2808);
     the drift carries the shear flow of stokes_set_shear (Oi_z = -rate/2,
     Ei_xy = rate/2).  Composed from the compiled reference's atimes_3all at
     those shifts"""
@@ -686,10 +686,144 @@ def test_bd_sheared_step_composition(scheme):
         u0 = drift(x, fz, shift(t))
         x1 = x + dt / bbn * u0
         uB = u0 + 0.5 * bbn * (drift(x1, fz, shift(t + dt / bbn)) - u0)
-        want = x + dt * (drift(x1, F, shift(t)) + uB)
+        want = x + dt * (drift(x, F, shift(t)) + uB)
     disp = np.abs(want - x).max()
     assert np.abs(xn - want).max() < 1e-12 * disp
     # the shifts matter: the same without the change at t + h differs
     if scheme == 0:
         other = x + dt * drift(x1, F + fz, shift(t))
         assert np.abs(other - want).max() > 1e-9 * disp
+
+
+def _compose_ref_step(scheme, x, F, ys, pe, dt, dt_lim, bbn, t, shift, L,
+                      a, e0, e1, flow, shear_mode):
+    """the reference's F step, BD_evolve_mid (brownian.c:2352-2560) or
+    BD_evolve_BB03 (2579-2920), with its REDO / REDO_FB / REDO_scale control
+    flow: each product is the compiled reference's atimes_3all at the
+    positions and cell shift the reference's system holds at that point
+    (stokes_set_pos_mobile and stokes_set_shear_shift at 2390-2393, 2466-2468;
+    2627-2630, 2738-2741, 2802-2808 where BD_add_FP moves BB03 back to x),
+    the noise chebyshev_eval_atimes of the dense operator where it is drawn;
+    returns (x_new, dt_taken, path of overlap stages)"""
+    import ctypes as C
+    R = po.ref()
+    n = x.size
+    Ui, Oi, Ei = flow
+
+    def M(p, sh, f):
+        s = po.System(0, p.reshape(-1, 3), box=(L, L, L), tr=1.0,
+                      shear_mode=shear_mode, shear_shift=sh)
+        h = R.ref_sys_new(C.byref(s.s))
+        u = np.zeros(n)
+        R.ref_atimes(h, n, po._p(np.ascontiguousarray(f)), po._p(u))
+        R.ref_sys_free(h)
+        return u
+
+    def drift(p, sh, f):
+        return M(p, sh, f) + _uinf(p, Ui, Oi, Ei)
+
+    def overlap(p, sh):
+        s = po.System(0, p.reshape(-1, 3), box=(L, L, L), tr=1.0,
+                      shear_mode=shear_mode, shear_shift=sh)
+        return s.check_overlap(p, 1.0)
+
+    def noise(p, sh, y):
+        G = np.stack([M(p, sh, e) for e in np.eye(n)], axis=1)
+        return po.cheb_eval_dense(a, G, y, e0, e1)
+
+    path, k, dtl = [], 0, dt
+    fact = math.sqrt(2 / (pe * dtl))
+    label = "redo"
+    while True:
+        if label == "redo":
+            P, S = x.copy(), shift(t)
+        if label in ("redo", "redo_fb"):
+            z = noise(P, S, ys[k])
+            k += 1
+        if scheme == 0:
+            f = F + fact * z
+            u0 = drift(P, S, f)
+            x1 = x + 0.5 * dtl * u0
+        else:
+            u0 = drift(P, S, fact * z)
+            x1 = x + dtl / bbn * u0
+        if overlap(x1, S):
+            stage = 1
+        else:
+            P, S = x1, shift(t + (0.5 * dtl if scheme == 0 else dtl / bbn))
+            if scheme == 0:
+                u = drift(P, S, f)
+            else:
+                uB = u0 + 0.5 * bbn * (drift(P, S, fact * z) - u0)
+                P, S = x.copy(), shift(t)
+                u = drift(P, S, F) + uB
+            xn = x + dtl * u
+            if not overlap(xn, S):
+                return xn, dtl, path
+            stage = 2
+        path.append(stage)
+        dtl *= 0.5
+        if dtl < dt_lim:
+            dtl = dt
+            label = "redo_fb" if stage == 1 else "redo"
+        else:
+            fact = math.sqrt(2 / (pe * dtl))
+            label = "scale"
+
+
+@pytest.mark.parametrize("scheme,gap,F0,ys,pe,shear,want_path", [
+    (0, 0.02, 100.0, 0.0, 1e6, 1, [1, 2]),
+    (1, 0.06, 50.0, 3.0, 1.0, 1, [1, 2, 2]),
+    (1, 0.02, 100.0, 0.0, 1e6, 0, [2, 2]),
+])
+def test_bd_retries_match_reference_control_flow(scheme, gap, F0, ys, pe,
+                                                 shear, want_path):
+    """steps that halve after first- and second-stage overlaps, sheared
+    (shear_mode 1, rate 0.8) and not: the oracle's step equals the
+    reference's control flow composed from its compiled atimes_3all.  The
+    mid-point retry after a second-stage overlap starts from the failed
+    attempt's intermediate configuration at the shift it left; BB03's starts
+    from x at the shift of t (its deterministic product moved the system
+    back); a first-stage retry keeps x and the shift of t"""
+    L, rate, t, t0, s0, bbn = 8.0, 0.8, 3.7, 1.0, 0.4, 2.0
+    pos = _eight(L)
+    pos[1] = pos[0] + np.array([2.0 + gap, 0.0, 0.0])
+
+    def shift(tt):
+        if not shear:
+            return 0.0
+        v = s0 + L * rate * (tt - t0)
+        while v < -L / 2:
+            v += L
+        while v >= L / 2:
+            v -= L
+        return v
+
+    flow = ((np.zeros(3), np.array([0.0, 0.0, -0.5 * rate]),
+             np.array([0.0, 0.5 * rate, 0.0, 0.0, 0.0])) if shear
+            else (np.zeros(3), np.zeros(3), np.zeros(5)))
+    s = po.System(0, pos, box=(L, L, L), tr=1.0, shear_mode=shear)
+    s.s.Oi[:], s.s.Ei[:] = flow[1], flow[2]
+    s.s.shear_rate, s.s.bd_t, s.s.bd_t0, s.s.bd_s0 = rate, t, t0, s0
+    s.s.shear_shift = shift(t)
+    n = 24
+    y = np.random.default_rng(5).standard_normal(n) * 0.1
+    y[0] += ys
+    y[3] -= ys
+    F = np.zeros(n)
+    F[0], F[3] = F0, -F0
+    G = s.matrix()
+    w = np.linalg.eigvalsh(0.5 * (G + G.T))
+    e0, e1 = 0.95 * w[0], 1.05 * w[-1]
+    a = po.cheb_coef(30, 0, e0, e1)
+    dt = 1e-3
+    xn, _, dtt, used, nh, err = s.bd_mid(pos, np.stack([y] * 4), pe, dt, 30,
+                                         a, e0, e1, F=F, scheme=scheme,
+                                         bb_n=bbn)
+    want, dtw, path = _compose_ref_step(scheme, pos.reshape(-1), F, [y] * 4,
+                                        pe, dt, 1e-6, bbn, t, shift, L, a, e0,
+                                        e1, flow, shear)
+    assert path == want_path
+    assert nh == len(want_path) and dtt == dtw and used == 1
+    disp = np.abs(want - pos.reshape(-1)).max()
+    assert np.abs(xn - want).max() < 1e-12 * disp
