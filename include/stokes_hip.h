@@ -105,6 +105,18 @@ int stokes_hip_use_host_pos (struct stokes *sys);
 /* y = M x for nrhs vectors, device pointers */
 int stokes_hip_atimes_dev (struct stokes *sys, int nrhs, const double *x_dev,
                            double *y_dev);
+/* y = M x for one vector (device pointers) without allocating: no
+ * hipMalloc / hipFree or allocating library call.  No counterpart in the
+ * reference; for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops).  The product's scratch lives in one
+ * 64 MiB device array per GPU, held until the process ends; a system whose
+ * product needs more is refused (STOKES_HIP_ENOMEM).  Every such call on
+ * one GPU shares that array, so calls that overlap on one GPU race.  The
+ * system must have been set up on the device by an ordinary product (e.g.
+ * stokes_hip_atimes_dev) with its current parameters; positions are read
+ * as by stokes_hip_atimes_dev.  Equal spheres only (sys->a == NULL). */
+int stokes_hip_atimes_dev_noalloc (struct stokes *sys, const double *x_dev,
+                                   double *y_dev);
 /* int-returning form of atimes_3all (host pointers) */
 int stokes_hip_atimes (struct stokes *sys, int n, const double *x, double *y);
 

@@ -51,7 +51,8 @@ EXPORTS = [
     "stokes_hip_memcpy_h2d", "stokes_hip_memcpy_d2h", "stokes_hip_sync",
     "stokes_hip_copy_calibrate",
     "stokes_hip_set_pos_dev", "stokes_hip_use_host_pos",
-    "stokes_hip_atimes_dev", "stokes_hip_atimes", "stokes_hip_make_matrix_dev",
+    "stokes_hip_atimes_dev", "stokes_hip_atimes_dev_noalloc", "stokes_hip_atimes",
+    "stokes_hip_make_matrix_dev",
     "stokes_hip_make_matrix", "stokes_hip_extmat_dev",
     "stokes_hip_cheb_eval_dev", "stokes_hip_lanczos_bounds",
     "stokes_hip_adjoint_dev", "stokes_hip_scalars_real_f_deriv", "stokes_hip_solve_mix_dev", "solve_mob_3fts_matrix_0",

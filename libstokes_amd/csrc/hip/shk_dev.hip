@@ -7,6 +7,7 @@
 #include <cstring>
 #include <cstdarg>
 #include <vector>
+#include <mutex>
 #include <type_traits>
 #include "../shk.h"
 #include "ewald_kernels.h"
@@ -80,10 +81,46 @@ struct shk_dev {
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   double last_ms[3] = {0, 0, 0};
   unsigned long generation = 0;   // bumped whenever a work buffer moves
+  // allocation-free products: while carve_base is set, the scratch buffers
+  // below are carved from the per-GPU array instead of allocated, and any
+  // other buffer that would have to grow is refused
+  unsigned char *carve_base = nullptr;
+  size_t carve_cap = 0, carve_off = 0;
+  DevBuf carve_saved[6];
 };
+
+// the work buffers of the single-column product that hold nothing between
+// calls (every entry read is written earlier in the same call)
+static DevBuf *scratch_buf (shk_dev *d, int k)
+{
+  DevBuf *t[6] = {&d->partial, &d->rz, &d->kpart, &d->kpq, &d->kback, &d->ptab};
+  return t[k];
+}
 
 static int ensure (shk_dev *d, DevBuf &b, size_t bytes)
 {
+  if (d->carve_base)
+    {
+      for (int k = 0; k < 6; k++)
+        if (&b == scratch_buf (d, k))
+          {
+            const size_t off = (d->carve_off + 255) & ~(size_t) 255;
+            if (off + bytes > d->carve_cap)
+              {
+                set_err ("allocation-free product: this system needs more than the %zu bytes"
+                         " of scratch the library holds per GPU", d->carve_cap);
+                return SHK_ENOMEM;
+              }
+            b.p = d->carve_base + off;
+            b.bytes = bytes;
+            d->carve_off = off + bytes;
+            return SHK_OK;
+          }
+      if (b.bytes >= bytes && b.p) return SHK_OK;
+      set_err ("allocation-free product: a device buffer of this system is not set up"
+               " (run one ordinary product on it first)");
+      return SHK_EARG;
+    }
   if (b.bytes >= bytes && b.p) return SHK_OK;
   if (b.p) { HIPCHK (hipFree (b.p)); b.p = nullptr; b.bytes = 0; }
   if (bytes == 0) return SHK_OK;
@@ -1025,6 +1062,66 @@ extern "C" int shk_dev_real_cache (shk_dev *d, int on, int prec)
   d->rc_use = 1;
   d->rc_prec = prec;
   return SHK_OK;
+}
+
+// ---- allocation-free products -------------------------------------------
+// Scratch of the single-column product for callers that cannot allocate:
+// one array per GPU (each GPU that runs such a product holds its own copy
+// until the process ends).  Host code reaches it only through the address
+// hipGetSymbolAddress gives on each device, never through its name.  All
+// such products on one GPU share it: calls that overlap on one GPU race.
+static __device__ __align__ (256) unsigned char scratch[SHK_NOALLOC_BYTES];
+
+static std::mutex g_scratch_lock;
+static unsigned char *g_scratch_addr[SHK_MAX_DEVICES];
+
+static int scratch_address (unsigned char **out)
+{
+  int dev = -1;
+  HIPCHK (hipGetDevice (&dev));
+  if (dev < 0 || dev >= SHK_MAX_DEVICES)
+    {
+      set_err ("allocation-free product: device %d beyond the %d the library tracks", dev,
+               SHK_MAX_DEVICES);
+      return SHK_EARG;
+    }
+  std::lock_guard<std::mutex> g (g_scratch_lock);
+  if (!g_scratch_addr[dev])
+    {
+      void *p = nullptr;
+      hipError_t e = hipGetSymbolAddress (&p, HIP_SYMBOL (scratch));
+      if (e != hipSuccess || !p)
+        {
+          set_err ("hipGetSymbolAddress(scratch) on device %d: %s", dev, hipGetErrorString (e));
+          return SHK_EHIP;
+        }
+      g_scratch_addr[dev] = (unsigned char *) p;
+    }
+  *out = g_scratch_addr[dev];
+  return SHK_OK;
+}
+
+extern "C" int shk_dev_noalloc_begin (shk_dev *d)
+{
+  if (!d || !d->have_sys) { set_err ("allocation-free product: no system"); return SHK_EARG; }
+  unsigned char *base = nullptr;
+  int rc = scratch_address (&base);
+  if (rc) return rc;
+  for (int k = 0; k < 6; k++) d->carve_saved[k] = *scratch_buf (d, k);
+  d->carve_base = base;
+  d->carve_cap = SHK_NOALLOC_BYTES;
+  d->carve_off = 0;
+  d->ptab_kind = 0;          // the phase tables are rebuilt in the scratch
+  return SHK_OK;
+}
+
+extern "C" void shk_dev_noalloc_end (shk_dev *d)
+{
+  if (!d || !d->carve_base) return;
+  for (int k = 0; k < 6; k++) *scratch_buf (d, k) = d->carve_saved[k];
+  d->carve_base = nullptr;
+  d->carve_cap = d->carve_off = 0;
+  d->ptab_kind = 0;          // the owned table no longer matches the positions
 }
 
 extern "C" int shk_dev_atimes (shk_dev *d, const double *x, double *y, int nrhs, int prec)

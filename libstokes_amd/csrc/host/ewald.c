@@ -108,6 +108,18 @@ stokes_hip_ctx (struct stokes *sys)
   return c;
 }
 
+/* the context of sys if one exists (none is created) */
+static stokes_ctx *
+ctx_find (struct stokes *sys)
+{
+  stokes_ctx *c = NULL;
+  pthread_mutex_lock (&g_lock);
+  for (int i = 0; i < g_nctx; i++)
+    if (g_ctx[i]->sys == sys) { c = g_ctx[i]; break; }
+  pthread_mutex_unlock (&g_lock);
+  return c;
+}
+
 static void
 ctx_free (stokes_ctx *c)
 {
@@ -396,6 +408,35 @@ same_params (const shk_params *a, const shk_params *b)
   return memcmp (a, b, sizeof (shk_params)) == 0;
 }
 
+/* the parameters of sys as the device path takes them (pu), and whether
+ * the lattice tables have to be rebuilt and uploaded (the return value) */
+static int
+prepare_params (stokes_ctx *c, shk_params *pup)
+{
+  struct stokes *s = c->sys;
+  shk_params pu;
+  memset (&pu, 0, sizeof (pu));
+  pu.version = s->version;
+  pu.np = s->np;
+  pu.periodic = s->periodic ? 1 : 0;
+  pu.poly = s->a != NULL;
+  pu.lx = s->lx; pu.ly = s->ly; pu.lz = s->lz; pu.pivol = s->pivol;
+  pu.xi = s->xi; pu.xi2 = s->xi2; pu.xiaspi = s->xiaspi; pu.xia2 = s->xia2;
+  pu.rmax2 = s->rmax2; pu.rmin = s->rmin;
+  pu.self_a = s->self_a; pu.self_c = s->self_c; pu.self_m = s->self_m;
+  pu.nr = pu.periodic ? s->nr : 0;
+  pu.gsign = c->fix_recip_g ? -1.0 : 1.0;
+  const int split = c->split_tr > 0.0 && pu.periodic && s->shear_mode == 0;
+  int full = !c->valid || c->shear_mode != s->shear_mode
+    || c->shear_shift != s->shear_shift || pu.poly
+    || c->a_ptr != s->a || (split ? c->split_tr : 0.0) != c->split_used_tr;
+  /* nk is only known after enumeration; compare the rest */
+  pu.nk = c->prm_user.nk;
+  if (!full && !same_params (&pu, &c->prm_user)) full = 1;
+  *pup = pu;
+  return full;
+}
+
 int
 stokes_hip_prepare (stokes_ctx *c, int positions_on_device)
 {
@@ -416,27 +457,11 @@ stokes_hip_prepare (stokes_ctx *c, int positions_on_device)
       return STOKES_HIP_EUNSUP;
     }
   shk_params pu;
-  memset (&pu, 0, sizeof (pu));
-  pu.version = s->version;
-  pu.np = s->np;
-  pu.periodic = s->periodic ? 1 : 0;
-  pu.poly = s->a != NULL;
-  pu.lx = s->lx; pu.ly = s->ly; pu.lz = s->lz; pu.pivol = s->pivol;
-  pu.xi = s->xi; pu.xi2 = s->xi2; pu.xiaspi = s->xiaspi; pu.xia2 = s->xia2;
-  pu.rmax2 = s->rmax2; pu.rmin = s->rmin;
-  pu.self_a = s->self_a; pu.self_c = s->self_c; pu.self_m = s->self_m;
-  pu.nr = pu.periodic ? s->nr : 0;
-  pu.gsign = c->fix_recip_g ? -1.0 : 1.0;
+  const int full = prepare_params (c, &pu);
   /* another splitting computes the same sum (to the Ewald tolerance) except
    * for the reference-sign G term of FTS, which add_g_correction restores */
   const int gcorr = s->version == 2 && !c->fix_recip_g;
   const int split = c->split_tr > 0.0 && pu.periodic && s->shear_mode == 0;
-  int full = !c->valid || c->shear_mode != s->shear_mode
-    || c->shear_shift != s->shear_shift || pu.poly
-    || c->a_ptr != s->a || (split ? c->split_tr : 0.0) != c->split_used_tr;
-  /* nk is only known after enumeration; compare the rest */
-  pu.nk = c->prm_user.nk;
-  if (!full && !same_params (&pu, &c->prm_user)) full = 1;
   if (full)
     {
       double *img = NULL, *kv = NULL, *ks = NULL, *cand = NULL;
@@ -815,6 +840,48 @@ stokes_hip_atimes_dev (struct stokes *sys, int nrhs, const double *x_dev,
   if (rc) return rc;
   rc = shk_dev_atimes (c->dev, x_dev, y_dev, nrhs, c->prec);
   if (rc) return dev_fail (rc, "atimes");
+  record_times (c);
+  return 0;
+}
+
+int
+stokes_hip_atimes_dev_noalloc (struct stokes *sys, const double *x_dev,
+                               double *y_dev)
+{
+  stokes_ctx *c = sys ? ctx_find (sys) : NULL;
+  if (!c || !c->valid)
+    {
+      stokes_hip_set_error ("atimes_dev_noalloc: no device set-up for this"
+                            " system yet (one ordinary product, such as"
+                            " stokes_hip_atimes_dev, sets it up)");
+      return STOKES_HIP_EARG;
+    }
+  if (sys->a != NULL)
+    {
+      stokes_hip_set_error ("atimes_dev_noalloc: unequal radii re-upload the"
+                            " system's tables on every product; equal"
+                            " spheres only");
+      return STOKES_HIP_EUNSUP;
+    }
+  shk_params pu;
+  if (prepare_params (c, &pu))
+    {
+      stokes_hip_set_error ("atimes_dev_noalloc: the system's parameters"
+                            " changed since its device set-up (one ordinary"
+                            " product sets it up again)");
+      return STOKES_HIP_EARG;
+    }
+  int rc = shk_dev_noalloc_begin (c->dev);
+  if (rc) return dev_fail (rc, "atimes_dev_noalloc");
+  /* positions only: the tables are current */
+  rc = stokes_hip_prepare (c, c->pos_dev != NULL);
+  if (!rc)
+    {
+      int r2 = shk_dev_atimes (c->dev, x_dev, y_dev, 1, c->prec);
+      if (r2) rc = dev_fail (r2, "atimes_dev_noalloc");
+    }
+  shk_dev_noalloc_end (c->dev);
+  if (rc) return rc;
   record_times (c);
   return 0;
 }

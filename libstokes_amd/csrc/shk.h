@@ -80,6 +80,18 @@ int shk_dev_set_pos (shk_dev *d, const double *pos, int on_device);
 /* y = M x for nrhs right-hand sides (column-major n x nrhs), device
  * pointers.  prec: SHK_FP64 or SHK_MIXED. */
 int shk_dev_atimes (shk_dev *d, const double *x, double *y, int nrhs, int prec);
+
+/* Allocation-free single-column products: between begin and end the
+ * scratch of the product (chunk partials, reciprocal partials, P / Q,
+ * phase tables) is carved from a per-GPU device array of
+ * SHK_NOALLOC_BYTES (no hipMalloc / hipFree); a buffer of the system that
+ * would have to grow is refused with SHK_EARG, a scratch need beyond the
+ * array with SHK_ENOMEM.  end restores the context's own buffers. */
+#define SHK_NOALLOC_BYTES ((size_t) 64 << 20)
+#define SHK_MAX_DEVICES 64
+int  shk_dev_noalloc_begin (shk_dev *d);
+void shk_dev_noalloc_end (shk_dev *d);
+
 /* dense (n x n) row-major extracted-form matrix, device pointer */
 int shk_dev_matrix (shk_dev *d, double *mat);
 /* y = mat . T(x) (FTS, extracted form) or mat . x (F/FT), device pointers */
