@@ -83,7 +83,10 @@ def configuration_poly(n, phi, seed=0, a_lo=0.5, a_hi=2.0, gap=1.05,
     L = (np.sum(4.0 * math.pi / 3.0 * a ** 3) / phi) ** (1.0 / 3.0)
     order = np.argsort(-a)
     pos = np.empty((n, 3))
-    placed = []
+    # the placed spheres in placement order
+    pp = np.empty((n, 3))
+    pa = np.empty(n)
+    k = 0
     tries = 0
     for idx in order:
         while True:
@@ -91,14 +94,15 @@ def configuration_poly(n, phi, seed=0, a_lo=0.5, a_hi=2.0, gap=1.05,
             if tries > max_tries:
                 raise RuntimeError("random sequential addition did not converge")
             p = rng.uniform(0.0, L, 3)
-            if placed:
-                q = np.array(placed)
-                d = _min_image(pos[q] - p, L)
-                lim = gap * (a[q] + a[idx])
+            if k:
+                d = _min_image(pp[:k] - p, L)
+                lim = gap * (pa[:k] + a[idx])
                 if np.any(np.einsum("ij,ij->i", d, d) < lim * lim):
                     continue
             pos[idx] = p
-            placed.append(idx)
+            pp[k] = p
+            pa[k] = a[idx]
+            k += 1
             break
     return pos, a, L
 
