@@ -465,9 +465,19 @@ void
 orc_block_atimes (int version, const double *x, double *y,
                   double ex, double ey, double ez, const double *s)
 {
+  orc_block_atimes_sign (version, x, y, ex, ey, ez, s, 0);
+}
+
+/* the same with the i == j signs when self != 0 (matrix_fts_self_atimes,
+ * fts.c:1024-1400, and its F / FT forms) */
+void
+orc_block_atimes_sign (int version, const double *x, double *y,
+                       double ex, double ey, double ez, const double *s,
+                       int self)
+{
   int d = version == 0 ? 3 : (version == 1 ? 6 : 11);
   double B[11][11];
-  orc_block_dense (version, ex, ey, ez, s, 0, B);
+  orc_block_dense (version, ex, ey, ez, s, self, B);
   double z[11];
   for (int k = 0; k < d; k++) z[k] = x[k];
   if (version == 2)
@@ -504,10 +514,16 @@ orc_block_ij (int version, int i, int j, double ex, double ey, double ez,
 /* operators                                                            */
 /* ------------------------------------------------------------------ */
 
+int
+orc_dim_of (int version)
+{
+  return version == 0 ? 3 : (version == 1 ? 6 : 11);
+}
+
 static int
 orc_dim (int version)
 {
-  return version == 0 ? 3 : (version == 1 ? 6 : 11);
+  return orc_dim_of (version);
 }
 
 /* self scalars; ewald.c:540-643 (mono: stokes.c:714-716; poly 591-598) */

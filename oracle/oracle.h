@@ -69,6 +69,10 @@ void orc_scalars_nonewald_poly (int version, double r, double aa, double ab,
 /* block kernels (f.c, ft.c, fts.c) */
 void orc_block_atimes (int version, const double *x, double *y,
                        double ex, double ey, double ez, const double *s);
+void orc_block_atimes_sign (int version, const double *x, double *y,
+                            double ex, double ey, double ez, const double *s,
+                            int self);
+int  orc_dim_of (int version);
 void orc_block_ij (int version, int i, int j,
                    double ex, double ey, double ez, const double *s,
                    int n, double *mat);
@@ -163,6 +167,25 @@ double orc_bd_mid_F (orc_sys *s, double *x, const double *F, double peclet,
                      const double *acoef, double e0, double e1, int ndraw,
                      const double *draws, int fix_fact, int *used,
                      int *halvings, double *err);
+
+/* lubrication (orc_lub.c): two-body coefficients, resistance scalars,
+ * far-field inverse, calc_lub_3fts */
+int  orc_twobody_init (int nmax);
+void orc_twobody_table (int fam, int n, double l, double *f);
+void orc_twobody_far (int version, const double *f, int n, double l, double s,
+                      double *out);
+void orc_twobody_lub (int version, const double *f, int N1, int mmax,
+                      double l, double s, double yc12_g5, double *out);
+void orc_twobody_scale (int version, double *two, double a1, double l, int sd);
+void orc_twobody_res (int version, double r, double aa, double ab, int n,
+                      int flag_lub, int flag_scale, double *res);
+void orc_two_body_res (double s, double *res);
+void orc_minv_poly (int version, double r, double aa, double ab, double *out);
+void orc_minv_fts (double s, double *res);
+void orc_block_atimes_self (int version, const double *x, double *y,
+                            double ex, double ey, double ez, const double *s);
+void orc_lub_3fts (const orc_sys *sys, double lubmin2, double lubmax, int nmax,
+                   int flag_lub, const double *uoe, double *fts);
 
 #ifdef __cplusplus
 }

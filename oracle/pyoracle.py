@@ -67,6 +67,19 @@ def lib():
         L.orc_cheb_eval.restype = C.c_double
         L.orc_cheb_error_minvsqrt.restype = C.c_double
         L.orc_cheb_error_Rsqrt.restype = C.c_double
+        L.orc_twobody_table.argtypes = [C.c_int, C.c_int, C.c_double, _dp]
+        L.orc_twobody_res.argtypes = [C.c_int, C.c_double, C.c_double,
+                                      C.c_double, C.c_int, C.c_int, C.c_int,
+                                      _dp]
+        L.orc_two_body_res.argtypes = [C.c_double, _dp]
+        L.orc_minv_poly.argtypes = [C.c_int, C.c_double, C.c_double,
+                                    C.c_double, _dp]
+        L.orc_minv_fts.argtypes = [C.c_double, _dp]
+        L.orc_block_atimes_sign.argtypes = [C.c_int, _dp, _dp, C.c_double,
+                                            C.c_double, C.c_double, _dp,
+                                            C.c_int]
+        L.orc_lub_3fts.argtypes = [C.c_void_p, C.c_double, C.c_double,
+                                   C.c_int, C.c_int, _dp, _dp]
         L.orc_func.restype = C.c_double
         L.orc_rng_int32.restype = C.c_ulong
         L.orc_rng_real2.restype = C.c_double
@@ -116,6 +129,17 @@ def ref():
         R.ref_rng_gaussian.argtypes = [C.c_void_p]
         R.ref_cheb_eval_sys.argtypes = [C.c_int, _dp, C.c_void_p, _dp, _dp,
                                         C.c_double, C.c_double]
+        R.ref_twobody_table.argtypes = [C.c_int, C.c_int, C.c_double, _dp]
+        R.ref_twobody_res.argtypes = [C.c_int, C.c_double, C.c_double,
+                                      C.c_double, C.c_int, C.c_int, C.c_int,
+                                      _dp]
+        R.ref_two_body_res.argtypes = [C.c_double, _dp]
+        R.ref_minv_fts.argtypes = [C.c_double, _dp]
+        R.ref_fts_block.argtypes = [C.c_int, _dp, _dp, _dp, _dp]
+        R.ref_lub_3fts_mono.argtypes = [C.c_void_p, C.c_double, C.c_double,
+                                        _dp, _dp]
+        R.scalars_nonewald_poly_full.argtypes = [C.c_int, C.c_double,
+                                                 C.c_double, C.c_double, _dp]
     return _ref
 
 
@@ -274,6 +298,27 @@ class System:
         R.ref_atimes(h, self.n, _p(x), _p(y))
         R.ref_sys_free(h)
         return y
+
+    def lub(self, uoe, lubmin2=4.0 + 1e-12, lubmax=3.0, nmax=100,
+            flag_lub=1):
+        """calc_lub_3fts (lub.c:576-768) of the oracle: fts = L(uoe);
+        lubmin2 is r^2 (equal spheres) or s^2 (unequal), lubmax r or s"""
+        uoe = np.ascontiguousarray(uoe, np.float64)
+        fts = np.zeros_like(uoe)
+        lib().orc_lub_3fts(C.byref(self.s), lubmin2, lubmax, nmax, flag_lub,
+                           _p(uoe), _p(fts))
+        return fts
+
+    def ref_lub_mono(self, uoe, lubmin2=4.0 + 1e-12, lubmax=3.0):
+        """the compiled reference's calc_lub_fts_2b over lub.c's pair loop
+        (equal spheres)"""
+        R = ref()
+        h = R.ref_sys_new(C.byref(self.s))
+        uoe = np.ascontiguousarray(uoe, np.float64)
+        fts = np.zeros_like(uoe)
+        R.ref_lub_3fts_mono(h, lubmin2, lubmax, _p(uoe), _p(fts))
+        R.ref_sys_free(h)
+        return fts
 
     def ref_matrix(self):
         R = ref()
@@ -471,3 +516,69 @@ def quat_dqdt(q, o):
     d = np.zeros(4)
     lib().orc_quat_dqdt(_p(q), _p(o), _p(d))
     return d
+
+
+# ---- lubrication (orc_lub.c) ---------------------------------------------
+TWOBODY_FAMILIES = ("XA", "YA", "YB", "XC", "YC", "XG", "YG", "YH", "XM",
+                    "YM", "ZM")
+
+
+def twobody_table(fam, n, lam, reference=False):
+    """f_0..f_n of family fam (index into TWOBODY_FAMILIES) at lambda"""
+    f = np.zeros(n + 1)
+    (ref().ref_twobody_table if reference else lib().orc_twobody_table)(
+        fam, n, lam, _p(f))
+    return f
+
+
+def twobody_res(version, r, aa, ab, n=100, flag_lub=1, flag_scale=1,
+                reference=False):
+    """twobody_scalars_res (twobody.c:2786-2814): 22 scalars"""
+    out = np.zeros(22)
+    (ref().ref_twobody_res if reference else lib().orc_twobody_res)(
+        version, r, aa, ab, n, flag_lub, flag_scale, _p(out))
+    return out
+
+
+def two_body_res(s, reference=False):
+    """scalar_two_body_res (two-body-res.c:31): 22 scalars"""
+    out = np.zeros(22)
+    (ref().ref_two_body_res if reference else lib().orc_two_body_res)(
+        s, _p(out))
+    return out
+
+
+def minv_fts(s, reference=False):
+    """scalar_minv_fts (fts.c:1490): 22 scalars"""
+    out = np.zeros(22)
+    (ref().ref_minv_fts if reference else lib().orc_minv_fts)(s, _p(out))
+    return out
+
+
+def minv_poly(version, r, aa, ab):
+    """scalars_minv_*_poly (minv-poly.c): 44 scalars (11, 12, 21, 22)"""
+    out = np.zeros(44)
+    lib().orc_minv_poly(version, r, aa, ab, _p(out))
+    return out
+
+
+def nonewald_poly_full(version, r, aa, ab):
+    """the compiled reference's scalars_nonewald_poly_full (non-ewald.c:
+    230-318): 44 scalars"""
+    out = np.zeros(44)
+    ref().scalars_nonewald_poly_full(version, r, aa, ab, _p(out))
+    return out
+
+
+def fts_block(x, e, sc, self_, reference=False):
+    """y = B(e; sc) x, matrix_fts_atimes / matrix_fts_self_atimes"""
+    x = np.ascontiguousarray(x, np.float64)
+    e = np.ascontiguousarray(e, np.float64)
+    sc = np.ascontiguousarray(sc, np.float64)
+    y = np.zeros(11)
+    if reference:
+        ref().ref_fts_block(int(self_), _p(x), _p(y), _p(e), _p(sc))
+    else:
+        lib().orc_block_atimes_sign(2, _p(x), _p(y), e[0], e[1], e[2], _p(sc),
+                                    int(self_))
+    return y

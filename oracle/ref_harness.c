@@ -15,6 +15,10 @@
 #include "non-ewald.h"
 #include "chebyshev.h"
 #include "KIrand.h"
+#include "fts.h"
+#include "two-body-res.h"
+#include "twobody.h"
+#include "lub.h"
 #include "oracle.h"
 
 struct stokes *
@@ -162,3 +166,98 @@ ref_rng_new (unsigned long seed)
 void ref_rng_free (void *r) { KIrand_free (r); }
 unsigned long ref_rng_int32 (void *r) { return KIrand_genrand_int32 (r); }
 double ref_rng_gaussian (void *r) { return KIrand_Gaussian (r); }
+
+/* ---- lubrication ------------------------------------------------------ */
+
+/* the reference's two-body tables twobody_XA ... twobody_ZM (XA.c ... ZM.c),
+ * family k in that order */
+void
+ref_twobody_table (int k, int n, double l, double *f)
+{
+  void (*t[11]) (int, double, double *) = {
+    twobody_XA, twobody_YA, twobody_YB, twobody_XC, twobody_YC, twobody_XG,
+    twobody_YG, twobody_YH, twobody_XM, twobody_YM, twobody_ZM};
+  t[k] (n, l, f);
+}
+
+/* twobody_scalars_res (twobody.c:2786-2814) with the table of l = ab / aa */
+void
+ref_twobody_res (int version, double r, double aa, double ab, int n,
+                 int flag_lub, int flag_scale, double *res)
+{
+  struct twobody_f *f = twobody_f_init (n, ab / aa);
+  twobody_scalars_res (version, r, aa, ab, f, n, flag_lub, flag_scale, res);
+  twobody_f_free (f);
+}
+
+void ref_two_body_res (double s, double *res) { scalar_two_body_res (s, res); }
+void ref_minv_fts (double s, double *res) { scalar_minv_fts (s, res); }
+
+/* matrix_fts_self_atimes / matrix_fts_atimes (fts.c): y += B(e; sc) x with
+ * sc the 11 scalars xa ya yb xc yc xg yg yh xm ym zm */
+void
+ref_fts_block (int self, const double *x, double *y, const double *e,
+               const double *sc)
+{
+  if (self)
+    matrix_fts_self_atimes (x, y, e[0], e[1], e[2], sc[0], sc[1], sc[2],
+                            sc[3], sc[4], sc[5], sc[6], sc[7], sc[8], sc[9],
+                            sc[10]);
+  else
+    matrix_fts_atimes (x, y, e[0], e[1], e[2], sc[0], sc[1], sc[2], sc[3],
+                       sc[4], sc[5], sc[6], sc[7], sc[8], sc[9], sc[10]);
+}
+
+int
+ref_cond_lub (const double *x1, const double *x2, double a1, double a2,
+              double lubmax2, int poly)
+{
+  return poly ? cond_lub_poly (x1, x2, a1, a2, lubmax2)
+              : cond_lub (x1, x2, lubmax2);
+}
+
+/* calc_lub_3fts (lub.c:576-768) for equal spheres: the reference's pair
+ * loop (j >= i, image cells -imax..imax, cond_lub) over its compiled
+ * calc_lub_fts_2b (fts.c:1758-1869); lub.c itself also reaches the
+ * unequal-sphere path, which needs LAPACK (minv-poly.c), so the loop is
+ * driven here */
+void
+ref_lub_3fts_mono (struct stokes *sys, double lubmin2, double lubmax,
+                   const double *uoe, double *fts)
+{
+  int np = sys->np;
+  sys->lubmin2 = lubmin2;
+  sys->lubmax = lubmax;
+  for (int i = 0; i < np * 11; i++) fts[i] = 0.0;
+  double lubmax2 = lubmax * lubmax;
+  int ix0 = 0, iy0 = 0, iz0 = 0;
+  if (sys->periodic)
+    {
+      double lm = lubmax;
+      if (lm <= 0.0)
+        {
+          lm = sys->lx;
+          if (sys->ly > lm) lm = sys->ly;
+          if (sys->lz > lm) lm = sys->lz;
+        }
+      ix0 = (int) (lm / sys->lx) + 1;
+      iy0 = (int) (lm / sys->ly) + 1;
+      iz0 = (int) (lm / sys->lz) + 1;
+    }
+  for (int i = 0; i < np; i++)
+    for (int j = i; j < np; j++)
+      for (int ix = -ix0; ix <= ix0; ix++)
+        for (int iy = -iy0; iy <= iy0; iy++)
+          for (int iz = -iz0; iz <= iz0; iz++)
+            {
+              double p[3] = {sys->pos[3 * j] + sys->lx * ix,
+                             sys->pos[3 * j + 1] + sys->ly * iy,
+                             sys->pos[3 * j + 2] + sys->lz * iz};
+              if (sys->shear_mode == 1) p[0] += iy * sys->shear_shift;
+              else if (sys->shear_mode == 2) p[0] += iz * sys->shear_shift;
+              if (cond_lub (sys->pos + 3 * i, p, lubmax2) == 0)
+                calc_lub_fts_2b (sys, uoe + 11 * i, uoe + 11 * j,
+                                 sys->pos + 3 * i, p, fts + 11 * i,
+                                 fts + 11 * j);
+            }
+}
