@@ -353,6 +353,32 @@ class Stokes:
         check(self.L.stokes_hip_set_precision(self.sys, mode),
               "set_precision")
 
+    def set_lub(self, lubmin2=None, lubmax=None, nmax=None, lub_form=None):
+        """lubrication parameters of struct stokes (lubmin2, lubmax,
+        twobody_nmax, twobody_lub)"""
+        s = self.s
+        if lubmin2 is not None:
+            s.lubmin2 = float(lubmin2)
+        if lubmax is not None:
+            s.lubmax = float(lubmax)
+        if nmax is not None:
+            s.twobody_nmax = int(nmax)
+        if lub_form is not None:
+            s.twobody_lub = int(lub_form)
+
+    def lub(self, x):
+        """y = L x, the lubrication correction of the system's version
+        (stokes_hip_lub_dev through host staging)"""
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        xd = DeviceArray.from_host(x)
+        yd = DeviceArray(x.size)
+        check(self.L.stokes_hip_lub_dev(self.sys, xd.ptr, yd.ptr), "lub_dev")
+        return yd.to_host()
+
+    def lub_dev(self, x_dev, y_dev):
+        check(self.L.stokes_hip_lub_dev(self.sys, x_dev.ptr, y_dev.ptr),
+              "lub_dev")
+
     def set_recip_g_fix(self, on=True):
         check(self.L.stokes_hip_set_recip_g_fix(self.sys, 1 if on else 0),
               "set_recip_g_fix")

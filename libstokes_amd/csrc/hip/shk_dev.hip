@@ -7,6 +7,7 @@
 #include <cstring>
 #include <cstdarg>
 #include <vector>
+#include <algorithm>
 #include <mutex>
 #include <type_traits>
 #include "../shk.h"
@@ -17,6 +18,7 @@
 #include "mfma_kernels.h"
 #include "cache_kernels.h"
 #include "adj_kernels.h"
+#include "lub_kernels.h"
 
 using namespace shk;
 
@@ -63,6 +65,9 @@ struct shk_dev {
   DevBuf pos, img, imgf, kv, ks, partial, pq, xs, ys, zt;
   DevBuf cand, candf, candm, tbits, kpart, kpq, kback;
   DevBuf adjfg, adjcs, adjpart;
+  DevBuf lubcoef, lubf1, lubcnt, luboff, lublist, lubval;   // lubrication
+  int lub_N = -1;                 // order of the uploaded coefficient table
+  int lub_f1 = 0;                 // f_k(1) table uploaded
   DevBuf km, ptab;                // k lattice indices, per-particle phase tables
   unsigned long pos_gen = 0;      // positions uploads so far
   unsigned long dense_gen = 0;    // positions of the per-call dense matrix
@@ -173,7 +178,8 @@ extern "C" void shk_dev_destroy (shk_dev *d)
   for (DevBuf *b : {&d->pos, &d->img, &d->imgf, &d->kv, &d->ks, &d->partial,
                     &d->pq, &d->xs, &d->ys, &d->zt, &d->cand, &d->candf, &d->candm,
                     &d->tbits, &d->kpart, &d->kpq, &d->kback, &d->adjfg, &d->adjcs,
-                    &d->adjpart, &d->km, &d->ptab, &d->rcache, &d->rz})
+                    &d->adjpart, &d->km, &d->ptab, &d->rcache, &d->rz, &d->lubcoef,
+                    &d->lubf1, &d->lubcnt, &d->luboff, &d->lublist, &d->lubval})
     release (*b);
   if (d->blas) (void) rocblas_destroy_handle (d->blas);
   for (auto &e : d->ev)
@@ -1152,6 +1158,119 @@ extern "C" int shk_dev_atimes (shk_dev *d, const double *x, double *y, int nrhs,
       else rc = atimes_v<2> (d, x + c * n, y + c * n, prec);
       if (rc) return rc;
     }
+  return SHK_OK;
+}
+
+// ---- lubrication ----------------------------------------------------------
+
+extern "C" int shk_dev_lub (shk_dev *d, int version, const shk_lub *lp, const double *x,
+                            double *y)
+{
+  if (!d || !d->have_sys) { set_err ("lubrication: no system"); return SHK_EARG; }
+  if (version < 0 || version > 2) { set_err ("lubrication: version %d", version); return SHK_EARG; }
+  const shk_params &p = d->prm;
+  const int np = p.np, D = dimv (version);
+  LubGeom g;
+  g.np = np;
+  g.periodic = p.periodic;
+  g.poly = p.poly;
+  g.lx = p.lx; g.ly = p.ly; g.lz = p.lz;
+  g.s2x = p.periodic ? d->s2x : 0.0;
+  g.s3x = p.periodic ? d->s3x : 0.0;
+  g.lubmax2 = lp->lubmax * lp->lubmax;
+  g.ix = g.iy = g.iz = 0;
+  if (p.periodic)
+    {
+      // set_imax_lub_periodic (lub.c:129-162): lubmax, or the largest box
+      // length when there is no limit
+      double lm = lp->lubmax;
+      if (lm <= 0.0) lm = std::max (p.lx, std::max (p.ly, p.lz));
+      g.ix = (int) (lm / p.lx) + 1;
+      g.iy = (int) (lm / p.ly) + 1;
+      g.iz = (int) (lm / p.lz) + 1;
+      if (g.ix > 15 || g.iy > 15 || g.iz > 15)
+        {
+          set_err ("lubrication: more than 15 image cells per direction (lubmax %g, box %g %g %g)",
+                   lp->lubmax, p.lx, p.ly, p.lz);
+          return SHK_EUNSUP;
+        }
+    }
+  LubEval ev;
+  ev.lubmin2 = lp->lubmin2;
+  ev.nmax = lp->nmax;
+  ev.flag_lub = lp->flag_lub;
+  ev.N = lp->N;
+  ev.coef = nullptr;
+  ev.f1 = nullptr;
+  int rc;
+  if (p.poly)
+    {
+      if (!lp->coef || lp->N < lp->nmax) { set_err ("lubrication: no coefficient table"); return SHK_EARG; }
+      if (d->lub_N != lp->N)
+        {
+          const size_t per = (size_t) (lp->N + 1) * (lp->N + 2);
+          if ((rc = ensure (d, d->lubcoef, sizeof (double) * 11 * per))) return rc;
+          for (int f = 0; f < 11; f++)
+            HIPCHK (hipMemcpyAsync ((double *) d->lubcoef.p + f * per, lp->coef[f],
+                                    sizeof (double) * per, hipMemcpyHostToDevice, d->stream));
+          d->lub_N = lp->N;
+        }
+      ev.coef = (const double *) d->lubcoef.p;
+    }
+  else
+    {
+      if (!d->lub_f1)
+        {
+          if (!lp->f1) { set_err ("lubrication: no f(1) table"); return SHK_EARG; }
+          if ((rc = ensure (d, d->lubf1, sizeof (double) * 11 * 51))) return rc;
+          HIPCHK (hipMemcpyAsync (d->lubf1.p, lp->f1, sizeof (double) * 11 * 51,
+                                  hipMemcpyHostToDevice, d->stream));
+          d->lub_f1 = 1;
+        }
+      ev.f1 = (const double *) d->lubf1.p;
+    }
+  if ((rc = ensure (d, d->lubcnt, sizeof (int) * (size_t) (np > 0 ? np : 1)))) return rc;
+  if ((rc = ensure (d, d->luboff, sizeof (int) * (size_t) (np + 1)))) return rc;
+  const double4 *pos4 = (const double4 *) d->pos.p;
+  const dim3 waves ((unsigned) ((np + 3) / 4));
+  hipLaunchKernelGGL ((k_lub_pairs<0>), waves, dim3 (256), 0, d->stream, pos4, g,
+                      (int *) d->lubcnt.p, (const int *) nullptr, (int2 *) nullptr);
+  HIPCHK (hipGetLastError ());
+  hipLaunchKernelGGL (k_lub_scan, dim3 (1), dim3 (1024), 0, d->stream, (const int *) d->lubcnt.p,
+                      np, (int *) d->luboff.p);
+  HIPCHK (hipGetLastError ());
+  int total = 0;
+  HIPCHK (hipMemcpyAsync (&total, (int *) d->luboff.p + np, sizeof (int), hipMemcpyDeviceToHost,
+                          d->stream));
+  HIPCHK (hipStreamSynchronize (d->stream));
+  if (total < 0) { set_err ("lubrication: pair count overflow"); return SHK_EARG; }
+  if ((rc = ensure (d, d->lublist, sizeof (int2) * (size_t) (total > 0 ? total : 1)))) return rc;
+  if ((rc = ensure (d, d->lubval, sizeof (double) * (size_t) D * (total > 0 ? total : 1))))
+    return rc;
+  if (total > 0)
+    {
+      hipLaunchKernelGGL ((k_lub_pairs<1>), waves, dim3 (256), 0, d->stream, pos4, g,
+                          (int *) nullptr, (const int *) d->luboff.p, (int2 *) d->lublist.p);
+      HIPCHK (hipGetLastError ());
+      const dim3 pg ((unsigned) ((total + 127) / 128));
+      if (version == 0)
+        hipLaunchKernelGGL ((k_lub_eval<0>), pg, dim3 (128), 0, d->stream, pos4, g, ev,
+                            (const int *) d->luboff.p, (const int2 *) d->lublist.p, total, x,
+                            (double *) d->lubval.p);
+      else if (version == 1)
+        hipLaunchKernelGGL ((k_lub_eval<1>), pg, dim3 (128), 0, d->stream, pos4, g, ev,
+                            (const int *) d->luboff.p, (const int2 *) d->lublist.p, total, x,
+                            (double *) d->lubval.p);
+      else
+        hipLaunchKernelGGL ((k_lub_eval<2>), pg, dim3 (128), 0, d->stream, pos4, g, ev,
+                            (const int *) d->luboff.p, (const int2 *) d->lublist.p, total, x,
+                            (double *) d->lubval.p);
+      HIPCHK (hipGetLastError ());
+    }
+  const size_t nd = (size_t) np * D;
+  hipLaunchKernelGGL (k_lub_gather, dim3 ((unsigned) ((nd + 255) / 256)), dim3 (256), 0, d->stream,
+                      (const int *) d->luboff.p, (const double *) d->lubval.p, np, D, y);
+  HIPCHK (hipGetLastError ());
   return SHK_OK;
 }
 

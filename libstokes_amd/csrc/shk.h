@@ -92,6 +92,21 @@ int shk_dev_atimes (shk_dev *d, const double *x, double *y, int nrhs, int prec);
 int  shk_dev_noalloc_begin (shk_dev *d);
 void shk_dev_noalloc_end (shk_dev *d);
 
+/* Lubrication correction y = L x (lub.c:164-768) for version 0 / 1 / 2
+ * (3 / 6 / 11 values per particle, device pointers) at the uploaded
+ * positions (radii in the positions' fourth component when the system has
+ * them).  coef[11] / N: the coefficient polynomials (unequal spheres;
+ * uploaded on the first call), f1[11 * 51]: f_k(1), k <= 50 (equal
+ * spheres).  lubmin2 / lubmax in r (equal) or s = 2 r / (a1 + a2) units. */
+typedef struct shk_lub {
+  double lubmin2, lubmax;
+  int nmax, flag_lub;
+  const double *const *coef;
+  int N;
+  const double *f1;
+} shk_lub;
+int shk_dev_lub (shk_dev *d, int version, const shk_lub *lp, const double *x, double *y);
+
 /* dense (n x n) row-major extracted-form matrix, device pointer */
 int shk_dev_matrix (shk_dev *d, double *mat);
 /* y = mat . T(x) (FTS, extracted form) or mat . x (F/FT), device pointers */

@@ -707,18 +707,20 @@ orc_lub_pair (const orc_sys *sys, double lubmin2, int nmax, int flag_lub,
   const int v = sys->version, d = v == 0 ? 3 : (v == 1 ? 6 : 11);
   double xx = pj[0] - pi[0], yy = pj[1] - pi[1], zz = pj[2] - pi[2];
   double r2 = xx * xx + yy * yy + zz * zz;
-  double s11[11], s12[11], s21[11], s22[11];
+  double s11[11] = {0}, s12[11] = {0}, s21[11] = {0}, s22[11] = {0};
   if (sys->a == NULL)
     {
       /* calc_lub_fts_2b (fts.c:1758-1869): r clamped to lubmin */
       if (r2 < lubmin2) r2 = lubmin2;
-      double rr = sqrt (r2), res[22], inf[22];
+      /* the far-field inverse of the version (scalar_minv_f, f.c:183;
+       * scalar_minv_ft, ft.c:414; scalar_minv_fts, fts.c:1490) */
+      double rr = sqrt (r2), res[22], inf[44];
       orc_two_body_res (rr, res);
-      orc_minv_fts (rr, inf);
+      orc_minv_poly (v, rr, 1.0, 1.0, inf);
       for (int k = 0; k < 11; k++)
         {
-          s11[k] = s22[k] = res[2 * k] - inf[2 * k];
-          s12[k] = s21[k] = res[2 * k + 1] - inf[2 * k + 1];
+          s11[k] = s22[k] = res[2 * k] - inf[4 * k];
+          s12[k] = s21[k] = res[2 * k + 1] - inf[4 * k + 1];
         }
     }
   else
@@ -730,9 +732,9 @@ orc_lub_pair (const orc_sys *sys, double lubmin2, int nmax, int flag_lub,
       const double rs = 0.25 * (a1 + a2) * (a1 + a2);
       if (r2 / rs < lubmin2) r2 = rs * lubmin2;
       double rr = sqrt (r2), r12[22], r21[22], inf[44], lub[44];
-      orc_twobody_res (2, rr, a1, a2, nmax, flag_lub, 1, r12);
-      orc_twobody_res (2, rr, a2, a1, nmax, flag_lub, 1, r21);
-      orc_minv_poly (2, rr, a1, a2, inf);
+      orc_twobody_res (v, rr, a1, a2, nmax, flag_lub, 1, r12);
+      orc_twobody_res (v, rr, a2, a1, nmax, flag_lub, 1, r21);
+      orc_minv_poly (v, rr, a1, a2, inf);
       for (int k = 0; k < 11; k++)
         {
           lub[4 * k] = r12[2 * k] - inf[4 * k];

@@ -15,6 +15,8 @@
 #include "non-ewald.h"
 #include "chebyshev.h"
 #include "KIrand.h"
+#include "f.h"
+#include "ft.h"
 #include "fts.h"
 #include "two-body-res.h"
 #include "twobody.h"
@@ -216,8 +218,9 @@ ref_cond_lub (const double *x1, const double *x2, double a1, double a2,
               : cond_lub (x1, x2, lubmax2);
 }
 
-/* calc_lub_3fts (lub.c:576-768) for equal spheres: the reference's pair
- * loop (j >= i, image cells -imax..imax, cond_lub) over its compiled
+/* calc_lub_3f / _3ft / _3fts (lub.c:164-768) for equal spheres: the
+ * reference's pair loop (j >= i, image cells -imax..imax, cond_lub) over
+ * its compiled calc_lub_f_2b (f.c:234), calc_lub_ft_2b (ft.c:494) or
  * calc_lub_fts_2b (fts.c:1758-1869); lub.c itself also reaches the
  * unequal-sphere path, which needs LAPACK (minv-poly.c), so the loop is
  * driven here */
@@ -226,9 +229,14 @@ ref_lub_3fts_mono (struct stokes *sys, double lubmin2, double lubmax,
                    const double *uoe, double *fts)
 {
   int np = sys->np;
+  const int d = sys->version == 0 ? 3 : (sys->version == 1 ? 6 : 11);
+  void (*pair) (struct stokes *, const double *, const double *,
+                const double *, const double *, double *, double *)
+    = sys->version == 0 ? calc_lub_f_2b
+      : (sys->version == 1 ? calc_lub_ft_2b : calc_lub_fts_2b);
   sys->lubmin2 = lubmin2;
   sys->lubmax = lubmax;
-  for (int i = 0; i < np * 11; i++) fts[i] = 0.0;
+  for (int i = 0; i < np * d; i++) fts[i] = 0.0;
   double lubmax2 = lubmax * lubmax;
   int ix0 = 0, iy0 = 0, iz0 = 0;
   if (sys->periodic)
@@ -256,8 +264,7 @@ ref_lub_3fts_mono (struct stokes *sys, double lubmin2, double lubmax,
               if (sys->shear_mode == 1) p[0] += iy * sys->shear_shift;
               else if (sys->shear_mode == 2) p[0] += iz * sys->shear_shift;
               if (cond_lub (sys->pos + 3 * i, p, lubmax2) == 0)
-                calc_lub_fts_2b (sys, uoe + 11 * i, uoe + 11 * j,
-                                 sys->pos + 3 * i, p, fts + 11 * i,
-                                 fts + 11 * j);
+                pair (sys, uoe + d * i, uoe + d * j, sys->pos + 3 * i, p,
+                      fts + d * i, fts + d * j);
             }
 }

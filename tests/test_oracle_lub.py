@@ -169,11 +169,12 @@ def _near_config(n, phi, seed, L=None):
     return pos, Lb
 
 
-@pytest.mark.parametrize("periodic", [True, False])
-def test_lub_loop_equal_spheres(periodic):
+@pytest.mark.parametrize("version,periodic", [(2, True), (2, False),
+                                              (1, True), (0, True)])
+def test_lub_loop_equal_spheres(version, periodic):
     pos, L = _near_config(30, 0.2, 3)
     box = (L, L, L) if periodic else None
-    s = po.System(2, pos, box=box, tr=1.0 if periodic else None)
+    s = po.System(version, pos, box=box, tr=1.0 if periodic else None)
     uoe = np.random.default_rng(4).standard_normal(s.n)
     for lubmin2, lubmax in ((4.0 + 1e-12, 3.0), (4.0441, 2.6), (0.0, 0.0)):
         if lubmax == 0.0 and periodic:
