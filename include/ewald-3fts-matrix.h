@@ -25,6 +25,21 @@ void solve_mix_3fts_matrix_0 (struct stokes *sys, const double *f,
                               const double *ef, double *u, double *o,
                               double *s, double *ff, double *tf, double *sf);
 
+/* the same problems with lubrication by dense FP64 solves on the GPU
+ * (ewald-3fts-matrix.c:264-330, 332-380; 731-838, 840-880) */
+void solve_res_lub_3fts_matrix_0 (struct stokes *sys, const double *u,
+                                  const double *o, const double *e, double *f,
+                                  double *t, double *s);
+void solve_res_lub_3fts_matrix (struct stokes *sys, const double *u,
+                                const double *o, const double *e, double *f,
+                                double *t, double *s);
+void solve_mob_lub_3fts_matrix_0 (struct stokes *sys, const double *f,
+                                  const double *t, const double *e, double *u,
+                                  double *o, double *s);
+void solve_mob_lub_3fts_matrix (struct stokes *sys, const double *f,
+                                const double *t, const double *e, double *u,
+                                double *o, double *s);
+
 #ifdef __cplusplus
 }
 #endif

@@ -55,6 +55,21 @@ void solve_mix_3fts (struct stokes *sys, const double *f, const double *t,
                      const double *ef, double *u, double *o, double *s,
                      double *ff, double *tf, double *sf);
 
+/* natural resistance / mobility problems with lubrication (ewald-3fts.c:
+ * 676-719, 733-770; 923-951, 966-1000): _0 in the fluid-rest frame, the
+ * others shift the known velocities / strains to it and the velocities
+ * back; GMRES on the device over M and the lubrication correction */
+void solve_res_lub_3fts_0 (struct stokes *sys, const double *u,
+                           const double *o, const double *e, double *f,
+                           double *t, double *s);
+void solve_res_lub_3fts (struct stokes *sys, const double *u, const double *o,
+                         const double *e, double *f, double *t, double *s);
+void solve_mob_lub_3fts_0 (struct stokes *sys, const double *f,
+                           const double *t, const double *e, double *u,
+                           double *o, double *s);
+void solve_mob_lub_3fts (struct stokes *sys, const double *f, const double *t,
+                         const double *e, double *u, double *o, double *s);
+
 /* imposed-flow shifts of particles 0..n-1: replace f.c:569-610
  * (shift_labo_to_rest_U), f.c:621-660 (shift_rest_to_labo_U), ft.c:901-920
  * (shift_labo_to_rest_O), ft.c:935-955 (shift_rest_to_labo_O) and

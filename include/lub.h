@@ -61,6 +61,35 @@ void scalars_lub_poly_full (int version, double r, double a1, double a2,
                             struct twobody_f *f12, struct twobody_f *f21,
                             int n, int flag_lub, double *lub);
 
+/* lub-matrix.c:76-223 (F), 225-371 (FT), 373-507 (FTS): the dense
+ * extracted-form matrix of L, (np D)^2 row-major, host pointer, as the
+ * reference's (pair blocks with matrix_fts_ij's signs); built on the GPU */
+void make_matrix_lub_3f (struct stokes *sys, double *mat);
+void make_matrix_lub_3ft (struct stokes *sys, double *mat);
+void make_matrix_lub_3fts (struct stokes *sys, double *mat);
+/* the same for sys->version into device memory.  No counterpart in the
+ * reference (host pointers only). */
+int stokes_hip_make_matrix_lub_dev (struct stokes *sys, double *mat_dev);
+
+/* Lubricated FTS solves with device pointers (11 values per particle, the
+ * reference's particle-wise packing).  res: (U, O, E) -> (F, T, S) =
+ * (M^-1 + L) (U, O, E) by GMRES on M with b = (I + M L) (U, O, E)
+ * (solve_res_lub_3fts_0, ewald-3fts.c:676-719); mob: (F, T, E) -> (U, O,
+ * S) by GMRES on A (u, o, s) = -M (0, 0, s) + (I + M L) (u, o, 0)
+ * (solve_mob_lub_3fts_0, 923-951); stokes_set_iter's parameters.
+ * matrix: kind 0 / 1 the same problems by a dense LU solve of the assembled
+ * FP64 matrices (solve_res_lub_3fts_matrix_0, ewald-3fts-matrix.c:264-330;
+ * solve_mob_lub_3fts_matrix_0, 731-838).  No counterpart in the reference
+ * (host pointers only). */
+int stokes_hip_solve_res_lub_3fts_dev (struct stokes *sys, const double *uoe_dev,
+                                       double *fts_dev, int *iters,
+                                       double *relres);
+int stokes_hip_solve_mob_lub_3fts_dev (struct stokes *sys, const double *fte_dev,
+                                       double *uos_dev, int *iters,
+                                       double *relres);
+int stokes_hip_solve_lub_3fts_matrix_dev (struct stokes *sys, int kind,
+                                          const double *b_dev, double *x_dev);
+
 /* y = L x with device pointers (the operator of calc_lub_3f / 3ft / 3fts
  * for sys->version); 0 or a STOKES_HIP_E* code.  No counterpart in the
  * reference (its correction runs on the host). */

@@ -76,6 +76,17 @@ EXPORTS = [
     "stokes_hip_bd_set_fts_iter", "stokes_hip_ensemble_autotune",
     "stokes_hip_bd_set_scheme", "stokes_hip_set_dense_blocks",
     "stokes_hip_bd_set_rng_mt19937",
+    # lub.h, twobody.h
+    "calc_lub_3f", "calc_lub_3ft", "calc_lub_3fts", "make_matrix_lub_3f",
+    "make_matrix_lub_3ft", "make_matrix_lub_3fts", "stokes_hip_lub_dev",
+    "stokes_hip_make_matrix_lub_dev", "stokes_hip_solve_res_lub_3fts_dev",
+    "stokes_hip_solve_mob_lub_3fts_dev",
+    "stokes_hip_solve_lub_3fts_matrix_dev", "solve_res_lub_3fts",
+    "solve_res_lub_3fts_0", "solve_mob_lub_3fts", "solve_mob_lub_3fts_0",
+    "solve_res_lub_3fts_matrix", "solve_res_lub_3fts_matrix_0",
+    "solve_mob_lub_3fts_matrix", "solve_mob_lub_3fts_matrix_0",
+    "cond_lub", "cond_lub_poly", "twobody_f_init", "twobody_f_free",
+    "twobody_far", "twobody_lub", "scalar_two_body_res",
 ]
 
 
@@ -192,6 +203,26 @@ def lib():
                  ("solve_mix_3ft", [sp] + [_dp] * 8)):
         getattr(L, f).argtypes = a
         getattr(L, f).restype = None
+    for f in ("solve_res_lub_3fts", "solve_res_lub_3fts_0",
+              "solve_mob_lub_3fts", "solve_mob_lub_3fts_0",
+              "solve_res_lub_3fts_matrix", "solve_res_lub_3fts_matrix_0",
+              "solve_mob_lub_3fts_matrix", "solve_mob_lub_3fts_matrix_0"):
+        getattr(L, f).argtypes = [sp] + [_dp] * 6
+        getattr(L, f).restype = None
+    for f in ("calc_lub_3f", "calc_lub_3ft", "calc_lub_3fts"):
+        getattr(L, f).argtypes = [sp, _dp, _dp]
+        getattr(L, f).restype = None
+    for f in ("make_matrix_lub_3f", "make_matrix_lub_3ft",
+              "make_matrix_lub_3fts"):
+        getattr(L, f).argtypes = [sp, _dp]
+        getattr(L, f).restype = None
+    L.stokes_hip_lub_dev.argtypes = [sp, _vp, _vp]
+    L.stokes_hip_make_matrix_lub_dev.argtypes = [sp, _vp]
+    for f in ("stokes_hip_solve_res_lub_3fts_dev",
+              "stokes_hip_solve_mob_lub_3fts_dev"):
+        getattr(L, f).argtypes = [sp, _vp, _vp, C.POINTER(C.c_int), _dp]
+    L.stokes_hip_solve_lub_3fts_matrix_dev.argtypes = [sp, C.c_int, _vp,
+                                                       _vp]
     L.stokes_set_iter.argtypes = [sp, C.c_char_p, C.c_int, C.c_int,
                                   C.c_double, C.c_int, _vp]
     L.stokes_hip_adjoint_dev.argtypes = [sp, _vp, _vp, _vp, _vp]
@@ -378,6 +409,33 @@ class Stokes:
     def lub_dev(self, x_dev, y_dev):
         check(self.L.stokes_hip_lub_dev(self.sys, x_dev.ptr, y_dev.ptr),
               "lub_dev")
+
+    def lub_matrix(self):
+        """the dense extracted-form lubrication matrix of the system's
+        version (stokes_hip_make_matrix_lub_dev), (n, n) row-major"""
+        md = DeviceArray(self.n * self.n)
+        check(self.L.stokes_hip_make_matrix_lub_dev(self.sys, md.ptr),
+              "make_matrix_lub_dev")
+        return md.to_host().reshape(self.n, self.n)
+
+    def solve_lub(self, b, kind="res", matrix=False):
+        """lubricated FTS problem on the device: kind "res" (U O E -> F T
+        S) or "mob" (F T E -> U O S), per particle; GMRES or (matrix) the
+        dense solve; returns (x, iterations, relative residual), the last
+        two None for the dense solve"""
+        bd = DeviceArray.from_host(np.ascontiguousarray(b, np.float64))
+        xd = DeviceArray(bd.n)
+        if matrix:
+            check(self.L.stokes_hip_solve_lub_3fts_matrix_dev(
+                self.sys, 0 if kind == "res" else 1, bd.ptr, xd.ptr),
+                "solve_lub_3fts_matrix")
+            return xd.to_host(), None, None
+        it, rr = C.c_int(), C.c_double()
+        f = (self.L.stokes_hip_solve_res_lub_3fts_dev if kind == "res"
+             else self.L.stokes_hip_solve_mob_lub_3fts_dev)
+        check(f(self.sys, bd.ptr, xd.ptr, C.byref(it), C.byref(rr)),
+              "solve_" + kind + "_lub")
+        return xd.to_host(), it.value, rr.value
 
     def set_recip_g_fix(self, on=True):
         check(self.L.stokes_hip_set_recip_g_fix(self.sys, 1 if on else 0),

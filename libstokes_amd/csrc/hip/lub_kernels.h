@@ -133,15 +133,15 @@ struct LubEval {
   const double *f1;     // equal spheres: f_k(1), k <= 50
 };
 
+// the correction scalars of directed pair p (S11, S12 seen from the
+// receiving particle i), the unit vector e and the weight (2 for a periodic
+// self image)
 template <int V>
-__global__ __launch_bounds__ (128) void
-k_lub_eval (const double4 *__restrict__ pos, LubGeom g, LubEval ev, const int *__restrict__ off,
-            const int2 *__restrict__ list, int npairs, const double *__restrict__ x,
-            double *__restrict__ val)
+__device__ inline void lub_pair (const double4 *__restrict__ pos, const LubGeom &g,
+                                 const LubEval &ev, const int *__restrict__ off,
+                                 const int2 *__restrict__ list, int p, int &i, int &j, double *e,
+                                 double *s11, double *s12, double &w)
 {
-  constexpr int D = Dim<V>::D;
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= npairs) return;
   // the receiving particle: the last i with off[i] <= p
   int lo = 0, hi = g.np;
   while (hi - lo > 1)
@@ -150,9 +150,9 @@ k_lub_eval (const double4 *__restrict__ pos, LubGeom g, LubEval ev, const int *_
       if (off[mid] <= p) lo = mid;
       else hi = mid;
     }
-  const int i = lo;
+  i = lo;
   const int2 rec = list[p];
-  const int j = rec.x;
+  j = rec.x;
   const int a = (rec.y & 31) - 15, b = ((rec.y >> 5) & 31) - 15, c = ((rec.y >> 10) & 31) - 15;
   const double4 pi = pos[i], pj = pos[j];
   double d[3];
@@ -160,7 +160,7 @@ k_lub_eval (const double4 *__restrict__ pos, LubGeom g, LubEval ev, const int *_
   d[1] = pj.y + g.ly * b - pi.y;
   d[2] = pj.z + g.lz * c - pi.z;
   double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
-  double two[22], m[44], s11[11], s12[11];
+  double two[22], m[44];
   for (int k = 0; k < 22; k++) two[k] = 0.0;
   if (!g.poly)
     {
@@ -198,14 +198,82 @@ k_lub_eval (const double4 *__restrict__ pos, LubGeom g, LubEval ev, const int *_
     }
   for (int f = lub::nfam_of (V); f < 11; f++) s11[f] = s12[f] = 0.0;
   const double r = sqrt (r2);
-  const double ex = d[0] / r, ey = d[1] / r, ez = d[2] / r;
+  e[0] = d[0] / r;
+  e[1] = d[1] / r;
+  e[2] = d[2] / r;
+  w = (i == j) ? 2.0 : 1.0;
+}
+
+template <int V>
+__global__ __launch_bounds__ (128) void
+k_lub_eval (const double4 *__restrict__ pos, LubGeom g, LubEval ev, const int *__restrict__ off,
+            const int2 *__restrict__ list, int npairs, const double *__restrict__ x,
+            double *__restrict__ val)
+{
+  constexpr int D = Dim<V>::D;
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= npairs) return;
+  int i, j;
+  double e[3], s11[11], s12[11], w;
+  lub_pair<V> (pos, g, ev, off, list, p, i, j, e, s11, s12, w);
   double y[11];
   for (int k = 0; k < D; k++) y[k] = 0.0;
-  lub::apply_block<D> (true, x + (size_t) D * i, y, ex, ey, ez, s11);
-  lub::apply_block<D> (false, x + (size_t) D * j, y, ex, ey, ez, s12);
-  const double w = (i == j) ? 2.0 : 1.0;
+  lub::apply_block<D> (true, x + (size_t) D * i, y, e[0], e[1], e[2], s11);
+  lub::apply_block<D> (false, x + (size_t) D * j, y, e[0], e[1], e[2], s12);
   double *o = val + (size_t) p * D;
   for (int k = 0; k < D; k++) o[k] = w * y[k];
+}
+
+// per pair: e, w, S11, S12 (26 values) for the dense build
+template <int V>
+__global__ __launch_bounds__ (128) void
+k_lub_scal (const double4 *__restrict__ pos, LubGeom g, LubEval ev, const int *__restrict__ off,
+            const int2 *__restrict__ list, int npairs, double *__restrict__ sc)
+{
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= npairs) return;
+  int i, j;
+  double e[3], s11[11], s12[11], w;
+  lub_pair<V> (pos, g, ev, off, list, p, i, j, e, s11, s12, w);
+  double *o = sc + (size_t) p * 26;
+  o[0] = e[0]; o[1] = e[1]; o[2] = e[2]; o[3] = w;
+  for (int k = 0; k < 11; k++) { o[4 + k] = s11[k]; o[15 + k] = s12[k]; }
+}
+
+// the dense extracted-form matrix (make_matrix_lub_3fts, lub-matrix.c:
+// 373-507, zeroed beforehand): the workgroup of particle i adds, pair by pair
+// in list order, the self block at (i, i) and the pair block at (i, j); only
+// this workgroup writes row block i
+template <int V>
+__global__ __launch_bounds__ (128) void
+k_lub_matrix (const int *__restrict__ off, const int2 *__restrict__ list,
+              const double *__restrict__ sc, int np, double *__restrict__ mat)
+{
+  constexpr int D = Dim<V>::D;
+  __shared__ double Bs[D * D], Bp[D * D];
+  const int i = blockIdx.x, t = threadIdx.x;
+  const size_t n = (size_t) np * D;
+  const int r = t / D, c = t - r * D;
+  double acc = 0.0;
+  for (int p = off[i]; p < off[i + 1]; p++)
+    {
+      const double *o = sc + (size_t) p * 26;
+      __syncthreads ();
+      if (t == 0)
+        {
+          lub::block_dense<D> (true, o[0], o[1], o[2], o + 4, Bs);
+          lub::block_dense<D> (false, o[0], o[1], o[2], o + 15, Bp);
+        }
+      __syncthreads ();
+      if (t < D * D)
+        {
+          const double w = o[3];
+          acc += w * Bs[t];
+          double *q = mat + ((size_t) i * D + r) * n + (size_t) list[p].x * D + c;
+          *q += w * Bp[t];
+        }
+    }
+  if (t < D * D) mat[((size_t) i * D + r) * n + (size_t) i * D + c] += acc;
 }
 
 // y[i][q] = sum over i's pairs in list order

@@ -500,5 +500,70 @@ LUB_HD void apply_block (bool self, const double *x, double *y, double ex, doubl
   }
 }
 
+// the raw block B(e; s) (rows: U, O, E components; columns: F, T, S in the
+// extracted form), as matrix_fts_ij / matrix_lub_fts_2b write it: y += B
+// T(x) is apply_block.  self: the i == j signs.
+template <int D>
+LUB_HD void block_dense (bool self, double ex, double ey, double ez, const double *s,
+                         double *B /* [D * D] row-major */)
+{
+  for (int k = 0; k < D * D; k++) B[k] = 0.0;
+  const double e[3] = {ex, ey, ez};
+  for (int p = 0; p < 3; p++)
+    for (int q = 0; q < 3; q++)
+      B[p * D + q] = (p == q ? s[1] : 0.0) + (s[0] - s[1]) * e[p] * e[q];
+  if (D == 3) return;
+  // (B v)_p = b (v x e)_p: row p, column q of the cross-product matrix
+  const double X[3][3] = {{0.0, e[2], -e[1]}, {-e[2], 0.0, e[0]}, {e[1], -e[0], 0.0}};
+  const double tb = self ? -s[2] : s[2];
+  for (int p = 0; p < 3; p++)
+    for (int q = 0; q < 3; q++)
+      {
+        B[(3 + p) * D + q] += s[2] * X[p][q];
+        B[p * D + 3 + q] += tb * X[p][q];
+        B[(3 + p) * D + 3 + q] += (p == q ? s[4] : 0.0) + (s[3] - s[4]) * e[p] * e[q];
+      }
+  if (D == 6) return;
+  const int P[5] = {0, 0, 0, 1, 1}, Q[5] = {0, 1, 2, 2, 1};
+  const double xg = s[5], yg = s[6], h = s[7];
+  const double gsign = self ? 1.0 : -1.0;
+  for (int r = 0; r < 5; r++)
+    for (int k = 0; k < 3; k++)
+      {
+        const int p = P[r], q = Q[r];
+        const double g = -xg / 3.0 * (p == q) * e[k] + yg * (e[p] * (q == k) + e[q] * (p == k))
+          + (xg - 2.0 * yg) * e[p] * e[q] * e[k];
+        B[(6 + r) * D + k] += g;
+        B[k * D + 6 + r] += gsign * g;
+      }
+  const double hxx = h * e[0] * e[0], hxy = h * e[0] * e[1], hxz = h * e[0] * e[2];
+  const double hyy = h * e[1] * e[1], hyz = h * e[1] * e[2], hzz = h * e[2] * e[2];
+  const double H[5][3] = {{0.0, 2.0 * hxz, -2.0 * hxy},
+                          {-hxz, hyz, hxx - hyy},
+                          {hxy, -hxx + hzz, -hyz},
+                          {hyy - hzz, -hxy, hxz},
+                          {-2.0 * hyz, 0.0, 2.0 * hxy}};
+  for (int r = 0; r < 5; r++)
+    for (int k = 0; k < 3; k++)
+      {
+        B[(6 + r) * D + 3 + k] += H[r][k];
+        B[(3 + k) * D + 6 + r] += H[r][k];
+      }
+  const double xm = s[8], ym = s[9], zm = s[10];
+  const double m1 = 1.5 * xm - 2.0 * ym + 0.5 * zm, m2 = -0.5 * xm + 0.5 * zm;
+  const double m3 = xm / 6.0 - 0.5 * zm, m4 = 0.5 * ym - 0.5 * zm, m5 = 0.5 * zm;
+  for (int r = 0; r < 5; r++)
+    for (int c = 0; c < 5; c++)
+      {
+        const int p = P[r], q = Q[r], t = P[c], u = Q[c];
+        const double dpq = p == q, dtu = t == u, dqt = q == t, dqu = q == u;
+        const double dpt = p == t, dpu = p == u;
+        B[(6 + r) * D + 6 + c] += m1 * e[p] * e[q] * e[t] * e[u]
+          + m2 * (dpq * e[t] * e[u] + e[p] * e[q] * dtu) + m3 * dpq * dtu
+          + m4 * (e[p] * dqt * e[u] + e[p] * dqu * e[t] + e[q] * dpt * e[u] + e[q] * dpu * e[t])
+          + m5 * (dpt * dqu + dqt * dpu);
+      }
+}
+
 }  // namespace lub
 }  // namespace shk

@@ -1163,14 +1163,15 @@ extern "C" int shk_dev_atimes (shk_dev *d, const double *x, double *y, int nrhs,
 
 // ---- lubrication ----------------------------------------------------------
 
-extern "C" int shk_dev_lub (shk_dev *d, int version, const shk_lub *lp, const double *x,
-                            double *y)
+// the directed pair list of the current positions (count, scan, fill) and
+// the geometry / evaluation parameters of the lubrication kernels
+static int lub_setup (shk_dev *d, int version, const shk_lub *lp, LubGeom &g, LubEval &ev,
+                      int &total)
 {
   if (!d || !d->have_sys) { set_err ("lubrication: no system"); return SHK_EARG; }
   if (version < 0 || version > 2) { set_err ("lubrication: version %d", version); return SHK_EARG; }
   const shk_params &p = d->prm;
-  const int np = p.np, D = dimv (version);
-  LubGeom g;
+  const int np = p.np;
   g.np = np;
   g.periodic = p.periodic;
   g.poly = p.poly;
@@ -1195,7 +1196,6 @@ extern "C" int shk_dev_lub (shk_dev *d, int version, const shk_lub *lp, const do
           return SHK_EUNSUP;
         }
     }
-  LubEval ev;
   ev.lubmin2 = lp->lubmin2;
   ev.nmax = lp->nmax;
   ev.flag_lub = lp->flag_lub;
@@ -1239,19 +1239,34 @@ extern "C" int shk_dev_lub (shk_dev *d, int version, const shk_lub *lp, const do
   hipLaunchKernelGGL (k_lub_scan, dim3 (1), dim3 (1024), 0, d->stream, (const int *) d->lubcnt.p,
                       np, (int *) d->luboff.p);
   HIPCHK (hipGetLastError ());
-  int total = 0;
+  total = 0;
   HIPCHK (hipMemcpyAsync (&total, (int *) d->luboff.p + np, sizeof (int), hipMemcpyDeviceToHost,
                           d->stream));
   HIPCHK (hipStreamSynchronize (d->stream));
   if (total < 0) { set_err ("lubrication: pair count overflow"); return SHK_EARG; }
   if ((rc = ensure (d, d->lublist, sizeof (int2) * (size_t) (total > 0 ? total : 1)))) return rc;
-  if ((rc = ensure (d, d->lubval, sizeof (double) * (size_t) D * (total > 0 ? total : 1))))
-    return rc;
   if (total > 0)
     {
       hipLaunchKernelGGL ((k_lub_pairs<1>), waves, dim3 (256), 0, d->stream, pos4, g,
                           (int *) nullptr, (const int *) d->luboff.p, (int2 *) d->lublist.p);
       HIPCHK (hipGetLastError ());
+    }
+  return SHK_OK;
+}
+
+extern "C" int shk_dev_lub (shk_dev *d, int version, const shk_lub *lp, const double *x,
+                            double *y)
+{
+  LubGeom g;
+  LubEval ev;
+  int total = 0, rc = lub_setup (d, version, lp, g, ev, total);
+  if (rc) return rc;
+  const int np = g.np, D = dimv (version);
+  if ((rc = ensure (d, d->lubval, sizeof (double) * (size_t) D * (total > 0 ? total : 1))))
+    return rc;
+  const double4 *pos4 = (const double4 *) d->pos.p;
+  if (total > 0)
+    {
       const dim3 pg ((unsigned) ((total + 127) / 128));
       if (version == 0)
         hipLaunchKernelGGL ((k_lub_eval<0>), pg, dim3 (128), 0, d->stream, pos4, g, ev,
@@ -1270,6 +1285,47 @@ extern "C" int shk_dev_lub (shk_dev *d, int version, const shk_lub *lp, const do
   const size_t nd = (size_t) np * D;
   hipLaunchKernelGGL (k_lub_gather, dim3 ((unsigned) ((nd + 255) / 256)), dim3 (256), 0, d->stream,
                       (const int *) d->luboff.p, (const double *) d->lubval.p, np, D, y);
+  HIPCHK (hipGetLastError ());
+  return SHK_OK;
+}
+
+extern "C" int shk_dev_lub_matrix (shk_dev *d, int version, const shk_lub *lp, double *mat)
+{
+  LubGeom g;
+  LubEval ev;
+  int total = 0, rc = lub_setup (d, version, lp, g, ev, total);
+  if (rc) return rc;
+  const int np = g.np, D = dimv (version);
+  const size_t n = (size_t) np * D;
+  HIPCHK (hipMemsetAsync (mat, 0, sizeof (double) * n * n, d->stream));
+  if (total == 0) return SHK_OK;
+  if ((rc = ensure (d, d->lubval, sizeof (double) * 26 * (size_t) total))) return rc;
+  const double4 *pos4 = (const double4 *) d->pos.p;
+  const dim3 pg ((unsigned) ((total + 127) / 128));
+  const int *off = (const int *) d->luboff.p;
+  const int2 *list = (const int2 *) d->lublist.p;
+  double *sc = (double *) d->lubval.p;
+  if (version == 0)
+    {
+      hipLaunchKernelGGL ((k_lub_scal<0>), pg, dim3 (128), 0, d->stream, pos4, g, ev, off, list,
+                          total, sc);
+      hipLaunchKernelGGL ((k_lub_matrix<0>), dim3 (np), dim3 (128), 0, d->stream, off, list,
+                          (const double *) sc, np, mat);
+    }
+  else if (version == 1)
+    {
+      hipLaunchKernelGGL ((k_lub_scal<1>), pg, dim3 (128), 0, d->stream, pos4, g, ev, off, list,
+                          total, sc);
+      hipLaunchKernelGGL ((k_lub_matrix<1>), dim3 (np), dim3 (128), 0, d->stream, off, list,
+                          (const double *) sc, np, mat);
+    }
+  else
+    {
+      hipLaunchKernelGGL ((k_lub_scal<2>), pg, dim3 (128), 0, d->stream, pos4, g, ev, off, list,
+                          total, sc);
+      hipLaunchKernelGGL ((k_lub_matrix<2>), dim3 (np), dim3 (128), 0, d->stream, off, list,
+                          (const double *) sc, np, mat);
+    }
   HIPCHK (hipGetLastError ());
   return SHK_OK;
 }

@@ -212,3 +212,131 @@ extern "C" int shk_dev_solve_mixed (shk_dev *d, double *mat, int np, int nm, int
   HCHK (hipStreamSynchronize (s));
   return SHK_OK;
 }
+
+// ---- dense lubricated solves ------------------------------------------------
+
+namespace {
+
+// C = I + B L on the known-force (U, O) columns, -B on the stresslet columns
+__global__ void k_lub_assemble (const double *__restrict__ B, double *__restrict__ C, size_t n)
+{
+  const size_t t = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n * n) return;
+  const size_t row = t / n, col = t - row * n;
+  if (col % 11 < 6) C[t] += (row == col) ? 1.0 : 0.0;
+  else C[t] = -B[t];
+}
+
+// g = x on the components with (k % 11 >= 6) == hi, 0 elsewhere
+__global__ void k_part (const double *__restrict__ x, int n, int hi, double *__restrict__ g)
+{
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) g[k] = ((k % 11 >= 6) == (hi != 0)) ? x[k] : 0.0;
+}
+
+// LU solve of the row-major n x n A (overwritten) for one right-hand side r
+int lu_solve (hipStream_t s, double *A, int n, double *r, Scratch &sc)
+{
+  int *ipiv = nullptr, *info = nullptr;
+  int rc;
+  if ((rc = sc.get (&ipiv, n)) || (rc = sc.get (&info, 1))) return rc;
+  rocblas_handle hb = nullptr;
+  if (rocblas_create_handle (&hb) != rocblas_status_success)
+    {
+      shk_set_error ("solve: rocblas_create_handle failed");
+      return SHK_EHIP;
+    }
+  rocblas_set_stream (hb, s);
+  // A read column-major is A^T: factor it, solve with the transpose
+  rocblas_status st = rocsolver_dgetrf (hb, n, n, A, n, ipiv, info);
+  if (st == rocblas_status_success)
+    st = rocsolver_dgetrs (hb, rocblas_operation_transpose, n, 1, A, n, ipiv, r, n);
+  int hinfo = 0;
+  hipError_t he = hipMemcpyAsync (&hinfo, info, sizeof (int), hipMemcpyDeviceToHost, s);
+  if (he == hipSuccess) he = hipStreamSynchronize (s);
+  rocblas_destroy_handle (hb);
+  if (st != rocblas_status_success || he != hipSuccess)
+    {
+      shk_set_error ("solve: rocsolver getrf/getrs failed (%d)", (int) st);
+      return SHK_EHIP;
+    }
+  if (hinfo != 0)
+    {
+      shk_set_error ("solve: singular matrix (info %d)", hinfo);
+      return SHK_EARG;
+    }
+  return SHK_OK;
+}
+
+}  // namespace
+
+// solve_res_lub_3fts_matrix_0 (ewald-3fts-matrix.c:264-330): x = M^-1 (I +
+// M L) b; solve_mob_lub_3fts_matrix_0 (731-838): with B = M T and C = I + (M
+// T)(L T) the relation B (F, T, S) = C (U, O, E) solved for (U, O, S) given
+// (F, T, E): [C_K | -B_H] (U, O, S) = B_K (F, T) - C_H E.  (There: the
+// matrices split into blocks and a generalised linear solve with LAPACK's
+// inverse; here one LU solve of the assembled system on the GPU.)
+extern "C" int shk_dev_solve_lub (shk_dev *d, double *M, double *L, int np, int kind,
+                                  const double *b, double *x)
+{
+  const int n = np * 11;
+  if (np <= 0) { shk_set_error ("solve: np %d", np); return SHK_EARG; }
+  hipStream_t s = (hipStream_t) shk_dev_stream (d);
+  Scratch sc;
+  double *g1, *g2, *w1, *w2, *r;
+  int rc;
+  if ((rc = sc.get (&g1, n)) || (rc = sc.get (&g2, n)) || (rc = sc.get (&w1, n))
+      || (rc = sc.get (&w2, n)) || (rc = sc.get (&r, n)))
+    return rc;
+  const size_t nn = (size_t) n * n;
+  hipLaunchKernelGGL (k_trans_ext, dim3 (blocks (nn / 11)), dim3 (256), 0, s, M, n, np);
+  hipLaunchKernelGGL (k_trans_ext, dim3 (blocks (nn / 11)), dim3 (256), 0, s, L, n, np);
+  HCHK (hipGetLastError ());
+  if (kind == 0)
+    {
+      // r = b + B (L b); B g = r
+      if ((rc = shk_dev_gemv (d, L, n, n, b, 0, w1))) return rc;
+      if ((rc = shk_dev_gemv (d, M, n, n, w1, 0, w2))) return rc;
+      HCHK (hipMemcpyAsync (r, b, sizeof (double) * n, hipMemcpyDeviceToDevice, s));
+      if ((rc = shk_dev_axpby (d, r, w2, 1.0, 1.0, n))) return rc;
+      if ((rc = lu_solve (s, M, n, r, sc))) return rc;
+      HCHK (hipMemcpyAsync (x, r, sizeof (double) * n, hipMemcpyDeviceToDevice, s));
+      HCHK (hipStreamSynchronize (s));
+      return SHK_OK;
+    }
+  // r = B (F, T, 0) - (0, 0, E) - B L (0, 0, E)
+  hipLaunchKernelGGL (k_part, dim3 (blocks (n)), dim3 (256), 0, s, b, n, 0, g1);
+  hipLaunchKernelGGL (k_part, dim3 (blocks (n)), dim3 (256), 0, s, b, n, 1, g2);
+  HCHK (hipGetLastError ());
+  if ((rc = shk_dev_gemv (d, M, n, n, g1, 0, r))) return rc;
+  if ((rc = shk_dev_axpby (d, r, g2, -1.0, 1.0, n))) return rc;
+  if ((rc = shk_dev_gemv (d, L, n, n, g2, 0, w1))) return rc;
+  if ((rc = shk_dev_gemv (d, M, n, n, w1, 0, w2))) return rc;
+  if ((rc = shk_dev_axpby (d, r, w2, -1.0, 1.0, n))) return rc;
+  // C = B L (row-major: column-major C^T = L^T B^T), then the assembly
+  double *C;
+  if ((rc = sc.get (&C, nn))) return rc;
+  rocblas_handle hb = nullptr;
+  if (rocblas_create_handle (&hb) != rocblas_status_success)
+    {
+      shk_set_error ("solve: rocblas_create_handle failed");
+      return SHK_EHIP;
+    }
+  rocblas_set_stream (hb, s);
+  const double one = 1.0, zero = 0.0;
+  rocblas_status st = rocblas_dgemm (hb, rocblas_operation_none, rocblas_operation_none, n, n, n,
+                                     &one, L, n, M, n, &zero, C, n);
+  rocblas_destroy_handle (hb);
+  if (st != rocblas_status_success)
+    {
+      shk_set_error ("solve: rocblas_dgemm failed (%d)", (int) st);
+      return SHK_EHIP;
+    }
+  hipLaunchKernelGGL (k_lub_assemble, dim3 (blocks (nn)), dim3 (256), 0, s, (const double *) M, C,
+                      nn > 0 ? (size_t) n : 0);
+  HCHK (hipGetLastError ());
+  if ((rc = lu_solve (s, C, n, r, sc))) return rc;
+  HCHK (hipMemcpyAsync (x, r, sizeof (double) * n, hipMemcpyDeviceToDevice, s));
+  HCHK (hipStreamSynchronize (s));
+  return SHK_OK;
+}

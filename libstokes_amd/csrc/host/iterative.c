@@ -242,6 +242,28 @@ stokes_hip_mix_solve (stokes_ctx *c, int D, int qs, int kind, int nm,
   return rc;
 }
 
+int
+stokes_hip_gmres (stokes_ctx *c, size_t n, int (*op) (void *, const double *, double *),
+                  void *ud, const double *b_dev, double *x_dev, int *iters,
+                  double *relres)
+{
+  int m, maxit;
+  double eps;
+  iter_params (c->sys, NULL, &m, &maxit, &eps);
+  double *ws = NULL;
+  const size_t words = n * (size_t) (m + 1) + n + (size_t) (m + 2);
+  int rc = shk_dev_malloc (c->dev, (void **) &ws, sizeof (double) * words);
+  if (rc) return rc;
+  int it = 0;
+  double rr = 0.0;
+  rc = gmres_dev (c->dev, n, op, ud, b_dev, x_dev, m, maxit, eps, ws,
+                  ws + n * (size_t) (m + 1), ws + n * (size_t) (m + 2), &it, &rr);
+  shk_dev_free (c->dev, ws);
+  if (iters) *iters = it;
+  if (relres) *relres = rr;
+  return rc;
+}
+
 void
 stokes_hip_iter_params (const struct stokes *sys, int *m, int *maxit,
                         double *eps)

@@ -11,10 +11,12 @@
 #include <string.h>
 #include "stokes_hip.h"
 #include "lub.h"
+#include "ewald-3fts.h"
+#include "ewald-3fts-matrix.h"
 #include "stokes_internal.h"
 
-static int
-lub_params (struct stokes *sys, shk_lub *lp, const double **tab, double *f1)
+int
+stokes_hip_lub_params (struct stokes *sys, shk_lub *lp, const double **tab, double *f1)
 {
   memset (lp, 0, sizeof (*lp));
   lp->lubmin2 = sys->lubmin2;
@@ -59,7 +61,7 @@ lub_dev (struct stokes *sys, int version, const double *x_dev, double *y_dev)
   shk_lub lp;
   const double *tab[11];
   double f1[11 * 51];
-  rc = lub_params (sys, &lp, tab, f1);
+  rc = stokes_hip_lub_params (sys, &lp, tab, f1);
   if (rc) return rc;
   rc = shk_dev_lub (c->dev, version, &lp, x_dev, y_dev);
   if (rc)
@@ -104,3 +106,357 @@ lub_host (struct stokes *sys, int version, const double *x, double *y, const cha
 void calc_lub_3f (struct stokes *sys, const double *u, double *f) { lub_host (sys, 0, u, f, "calc_lub_3f"); }
 void calc_lub_3ft (struct stokes *sys, const double *uo, double *ft) { lub_host (sys, 1, uo, ft, "calc_lub_3ft"); }
 void calc_lub_3fts (struct stokes *sys, const double *uoe, double *fts) { lub_host (sys, 2, uoe, fts, "calc_lub_3fts"); }
+
+/* ------------------------------------------------------------------ */
+/* dense lubrication matrix                                             */
+/* ------------------------------------------------------------------ */
+
+static int
+lub_matrix_dev (struct stokes *sys, int version, double *mat_dev)
+{
+  stokes_ctx *c = stokes_hip_ctx (sys);
+  if (!c) return STOKES_HIP_ENODEV;
+  int rc = stokes_hip_prepare (c, c->pos_dev != NULL);
+  if (rc) return rc;
+  shk_lub lp;
+  const double *tab[11];
+  double f1[11 * 51];
+  if ((rc = stokes_hip_lub_params (sys, &lp, tab, f1))) return rc;
+  rc = shk_dev_lub_matrix (c->dev, version, &lp, mat_dev);
+  if (rc) stokes_hip_set_error ("lubrication matrix: %s", shk_last_error ());
+  return rc;
+}
+
+int
+stokes_hip_make_matrix_lub_dev (struct stokes *sys, double *mat_dev)
+{
+  if (!sys || sys->version < 0 || sys->version > 2)
+    {
+      stokes_hip_set_error ("make_matrix_lub_dev: invalid system");
+      return STOKES_HIP_EARG;
+    }
+  return lub_matrix_dev (sys, sys->version, mat_dev);
+}
+
+static void
+lub_matrix_host (struct stokes *sys, int version, double *mat, const char *name)
+{
+  const size_t n = (size_t) sys->np * (version == 0 ? 3 : (version == 1 ? 6 : 11));
+  void *md = NULL;
+  int rc = stokes_hip_malloc (&md, sizeof (double) * n * n);
+  if (!rc) rc = lub_matrix_dev (sys, version, md);
+  if (!rc) rc = stokes_hip_memcpy_d2h (mat, md, sizeof (double) * n * n);
+  if (md) stokes_hip_free (md);
+  if (rc)
+    {
+      fprintf (stderr, "%s: %s\n", name, stokes_hip_last_error ());
+      exit (1);
+    }
+}
+
+void make_matrix_lub_3f (struct stokes *sys, double *mat) { lub_matrix_host (sys, 0, mat, "make_matrix_lub_3f"); }
+void make_matrix_lub_3ft (struct stokes *sys, double *mat) { lub_matrix_host (sys, 1, mat, "make_matrix_lub_3ft"); }
+void make_matrix_lub_3fts (struct stokes *sys, double *mat) { lub_matrix_host (sys, 2, mat, "make_matrix_lub_3fts"); }
+
+/* ------------------------------------------------------------------ */
+/* lubricated FTS solves                                                */
+/* ------------------------------------------------------------------ */
+
+typedef struct {
+  stokes_ctx *c;
+  shk_lub lp;
+  int np;
+  double *t1, *t2, *t3;
+} lub_op;
+
+/* A x = M x (the resistance problem's operator) */
+static int
+op_mob (void *ud, const double *x, double *y)
+{
+  lub_op *o = ud;
+  return shk_dev_atimes (o->c->dev, x, y, 1, o->c->prec);
+}
+
+/* A (u, o, s) = -M (0, 0, s) + (I + M L) (u, o, 0) (atimes_mob_lub_3fts,
+ * ewald-3fts.c:835-907) */
+static int
+op_mob_lub (void *ud, const double *x, double *y)
+{
+  lub_op *o = ud;
+  shk_dev *d = o->c->dev;
+  const size_t n = (size_t) o->np * 11;
+  int rc = shk_dev_mix_split (d, x, o->np, o->np, 11, 6, o->t1, o->t2);
+  if (!rc) rc = shk_dev_atimes (d, o->t2, y, 1, o->c->prec);
+  if (!rc) rc = shk_dev_axpby (d, y, o->t1, 1.0, -1.0, n);
+  if (!rc) rc = shk_dev_lub (d, 2, &o->lp, o->t1, o->t3);
+  if (!rc) rc = shk_dev_atimes (d, o->t3, o->t2, 1, o->c->prec);
+  if (!rc) rc = shk_dev_axpby (d, y, o->t2, 1.0, 1.0, n);
+  return rc;
+}
+
+/* kind 0: (U, O, E) -> (F, T, S) = M^-1 (I + M L) (U, O, E); kind 1:
+ * (F, T, E) -> (U, O, S) with (F, T, S) = (M^-1 + L) (U, O, E); matrix:
+ * dense solve instead of GMRES */
+static int
+lub_solve_dev (struct stokes *sys, int kind, int matrix, const double *b_dev,
+               double *x_dev, int *iters, double *relres, const char *what)
+{
+  if (!sys || sys->version != 2 || !b_dev || !x_dev)
+    {
+      stokes_hip_set_error ("%s: an FTS system and device vectors are required",
+                            what);
+      return STOKES_HIP_EARG;
+    }
+  stokes_ctx *c = stokes_hip_ctx (sys);
+  if (!c) return STOKES_HIP_ENODEV;
+  int rc = stokes_hip_prepare (c, c->pos_dev != NULL);
+  if (rc) return rc;
+  lub_op o;
+  memset (&o, 0, sizeof (o));
+  o.c = c;
+  o.np = sys->np;
+  const double *tab[11];
+  double f1[11 * 51];
+  if ((rc = stokes_hip_lub_params (sys, &o.lp, tab, f1))) return rc;
+  shk_dev *d = c->dev;
+  const size_t n = (size_t) sys->np * 11;
+  if (matrix)
+    {
+      double *M = NULL, *L = NULL;
+      rc = shk_dev_malloc (d, (void **) &M, sizeof (double) * n * n);
+      if (!rc) rc = shk_dev_malloc (d, (void **) &L, sizeof (double) * n * n);
+      if (!rc) rc = shk_dev_matrix (d, M);
+      if (!rc) rc = shk_dev_lub_matrix (d, 2, &o.lp, L);
+      if (!rc) rc = shk_dev_solve_lub (d, M, L, sys->np, kind, b_dev, x_dev);
+      if (M) shk_dev_free (d, M);
+      if (L) shk_dev_free (d, L);
+      if (rc) stokes_hip_set_error ("%s: %s", what, shk_last_error ());
+      return rc;
+    }
+  double *ws = NULL;
+  rc = shk_dev_malloc (d, (void **) &ws, sizeof (double) * 4 * n);
+  if (rc)
+    {
+      stokes_hip_set_error ("%s: %s", what, shk_last_error ());
+      return STOKES_HIP_ENOMEM;
+    }
+  double *b = ws;
+  o.t1 = ws + n;
+  o.t2 = ws + 2 * n;
+  o.t3 = ws + 3 * n;
+  if (kind == 0)
+    {
+      /* b = (I + M L) (U, O, E) (solve_res_lub_3fts_0, ewald-3fts.c:676-719) */
+      rc = shk_dev_lub (d, 2, &o.lp, b_dev, o.t1);
+      if (!rc) rc = shk_dev_atimes (d, o.t1, b, 1, c->prec);
+      if (!rc) rc = shk_dev_axpby (d, b, b_dev, 1.0, 1.0, n);
+      if (!rc) rc = stokes_hip_gmres (c, n, op_mob, &o, b, x_dev, iters, relres);
+    }
+  else
+    {
+      /* b = M (F, T, 0) - (I + M L) (0, 0, E) (calc_b_mob_lub_3fts,
+       * ewald-3fts.c:776-833) */
+      rc = shk_dev_mix_split (d, b_dev, sys->np, sys->np, 11, 6, o.t1, o.t2);
+      if (!rc) rc = shk_dev_atimes (d, o.t1, b, 1, c->prec);
+      if (!rc) rc = shk_dev_axpby (d, b, o.t2, -1.0, 1.0, n);
+      if (!rc) rc = shk_dev_lub (d, 2, &o.lp, o.t2, o.t3);
+      if (!rc) rc = shk_dev_atimes (d, o.t3, o.t1, 1, c->prec);
+      if (!rc) rc = shk_dev_axpby (d, b, o.t1, -1.0, 1.0, n);
+      if (!rc) rc = stokes_hip_gmres (c, n, op_mob_lub, &o, b, x_dev, iters, relres);
+    }
+  shk_dev_free (d, ws);
+  if (rc) stokes_hip_set_error ("%s: %s", what, shk_last_error ());
+  return rc;
+}
+
+int
+stokes_hip_solve_res_lub_3fts_dev (struct stokes *sys, const double *uoe_dev,
+                                   double *fts_dev, int *iters, double *relres)
+{
+  return lub_solve_dev (sys, 0, 0, uoe_dev, fts_dev, iters, relres,
+                        "solve_res_lub_3fts");
+}
+
+int
+stokes_hip_solve_mob_lub_3fts_dev (struct stokes *sys, const double *fte_dev,
+                                   double *uos_dev, int *iters, double *relres)
+{
+  return lub_solve_dev (sys, 1, 0, fte_dev, uos_dev, iters, relres,
+                        "solve_mob_lub_3fts");
+}
+
+int
+stokes_hip_solve_lub_3fts_matrix_dev (struct stokes *sys, int kind,
+                                      const double *b_dev, double *x_dev)
+{
+  if (kind != 0 && kind != 1)
+    {
+      stokes_hip_set_error ("solve_lub_3fts_matrix: kind %d", kind);
+      return STOKES_HIP_EARG;
+    }
+  return lub_solve_dev (sys, kind, 1, b_dev, x_dev, NULL, NULL,
+                        "solve_lub_3fts_matrix");
+}
+
+/* ---- the reference's host-array entry points ---- */
+
+static void
+pack11 (int np, double *h, const double *a, const double *b, const double *c)
+{
+  for (int i = 0; i < np; i++)
+    {
+      memcpy (h + 11 * (size_t) i, a + 3 * (size_t) i, sizeof (double) * 3);
+      memcpy (h + 11 * (size_t) i + 3, b + 3 * (size_t) i, sizeof (double) * 3);
+      memcpy (h + 11 * (size_t) i + 6, c + 5 * (size_t) i, sizeof (double) * 5);
+    }
+}
+
+static void
+unpack11 (int np, const double *h, double *a, double *b, double *c)
+{
+  for (int i = 0; i < np; i++)
+    {
+      memcpy (a + 3 * (size_t) i, h + 11 * (size_t) i, sizeof (double) * 3);
+      memcpy (b + 3 * (size_t) i, h + 11 * (size_t) i + 3, sizeof (double) * 3);
+      memcpy (c + 5 * (size_t) i, h + 11 * (size_t) i + 6, sizeof (double) * 5);
+    }
+}
+
+/* the reference resets a wrong version with a message (ewald-3fts.c:680-685) */
+static void
+lub_host_solve (struct stokes *sys, int kind, int matrix, const double *a,
+                const double *b, const double *c, double *x1, double *x2,
+                double *x3, const char *name)
+{
+  if (sys->version != 2)
+    {
+      fprintf (stderr, "libstokes %s : the version is wrong. reset to FTS\n", name);
+      sys->version = 2;
+    }
+  const size_t n = (size_t) sys->np * 11;
+  double *h = malloc (sizeof (double) * n);
+  void *bd = NULL, *xd = NULL;
+  if (!h)
+    {
+      fprintf (stderr, "%s: allocation failed\n", name);
+      exit (1);
+    }
+  pack11 (sys->np, h, a, b, c);
+  int rc = stokes_hip_malloc (&bd, sizeof (double) * n);
+  if (!rc) rc = stokes_hip_malloc (&xd, sizeof (double) * n);
+  if (!rc) rc = stokes_hip_memcpy_h2d (bd, h, sizeof (double) * n);
+  if (!rc) rc = lub_solve_dev (sys, kind, matrix, bd, xd, NULL, NULL, name);
+  if (!rc) rc = stokes_hip_memcpy_d2h (h, xd, sizeof (double) * n);
+  if (bd) stokes_hip_free (bd);
+  if (xd) stokes_hip_free (xd);
+  if (rc)
+    {
+      fprintf (stderr, "%s: %s\n", name, stokes_hip_last_error ());
+      exit (1);
+    }
+  unpack11 (sys->np, h, x1, x2, x3);
+  free (h);
+}
+
+void
+solve_res_lub_3fts_0 (struct stokes *sys, const double *u, const double *o,
+                      const double *e, double *f, double *t, double *s)
+{
+  lub_host_solve (sys, 0, 0, u, o, e, f, t, s, "solve_res_lub_3fts_0");
+}
+
+void
+solve_mob_lub_3fts_0 (struct stokes *sys, const double *f, const double *t,
+                      const double *e, double *u, double *o, double *s)
+{
+  lub_host_solve (sys, 1, 0, f, t, e, u, o, s, "solve_mob_lub_3fts_0");
+}
+
+void
+solve_res_lub_3fts_matrix_0 (struct stokes *sys, const double *u,
+                             const double *o, const double *e, double *f,
+                             double *t, double *s)
+{
+  lub_host_solve (sys, 0, 1, u, o, e, f, t, s, "solve_res_lub_3fts_matrix_0");
+}
+
+void
+solve_mob_lub_3fts_matrix_0 (struct stokes *sys, const double *f,
+                             const double *t, const double *e, double *u,
+                             double *o, double *s)
+{
+  lub_host_solve (sys, 1, 1, f, t, e, u, o, s, "solve_mob_lub_3fts_matrix_0");
+}
+
+static double *
+lub_alloc (size_t n)
+{
+  double *p = malloc (sizeof (double) * (n ? n : 1));
+  if (!p)
+    {
+      fprintf (stderr, "lubricated solve: allocation failed\n");
+      exit (1);
+    }
+  return p;
+}
+
+/* labo frame: the known velocities / strains go to the fluid-rest frame
+ * first, the velocities come back (ewald-3fts.c:733-770, 966-1000) */
+static void
+res_labo (struct stokes *sys, int matrix, const double *u, const double *o,
+          const double *e, double *f, double *t, double *s)
+{
+  const int np = sys->np;
+  double *u0 = lub_alloc (3 * (size_t) np), *o0 = lub_alloc (3 * (size_t) np);
+  double *e0 = lub_alloc (5 * (size_t) np);
+  shift_labo_to_rest_U (sys, np, u, u0);
+  shift_labo_to_rest_O (sys, np, o, o0);
+  shift_labo_to_rest_E (sys, np, e, e0);
+  if (matrix) solve_res_lub_3fts_matrix_0 (sys, u0, o0, e0, f, t, s);
+  else solve_res_lub_3fts_0 (sys, u0, o0, e0, f, t, s);
+  free (u0); free (o0); free (e0);
+}
+
+static void
+mob_labo (struct stokes *sys, int matrix, const double *f, const double *t,
+          const double *e, double *u, double *o, double *s)
+{
+  const int np = sys->np;
+  double *e0 = lub_alloc (5 * (size_t) np);
+  shift_labo_to_rest_E (sys, np, e, e0);
+  if (matrix) solve_mob_lub_3fts_matrix_0 (sys, f, t, e0, u, o, s);
+  else solve_mob_lub_3fts_0 (sys, f, t, e0, u, o, s);
+  free (e0);
+  shift_rest_to_labo_U (sys, np, u);
+  shift_rest_to_labo_O (sys, np, o);
+}
+
+void
+solve_res_lub_3fts (struct stokes *sys, const double *u, const double *o,
+                    const double *e, double *f, double *t, double *s)
+{
+  res_labo (sys, 0, u, o, e, f, t, s);
+}
+
+void
+solve_mob_lub_3fts (struct stokes *sys, const double *f, const double *t,
+                    const double *e, double *u, double *o, double *s)
+{
+  mob_labo (sys, 0, f, t, e, u, o, s);
+}
+
+void
+solve_res_lub_3fts_matrix (struct stokes *sys, const double *u,
+                           const double *o, const double *e, double *f,
+                           double *t, double *s)
+{
+  res_labo (sys, 1, u, o, e, f, t, s);
+}
+
+void
+solve_mob_lub_3fts_matrix (struct stokes *sys, const double *f,
+                           const double *t, const double *e, double *u,
+                           double *o, double *s)
+{
+  mob_labo (sys, 1, f, t, e, u, o, s);
+}

@@ -61,6 +61,17 @@ shk_dev *stokes_hip_global_dev (void);
 void stokes_hip_shadow_split (const struct stokes *s, double tratio,
                               struct stokes *sh);
 void stokes_hip_shadow_free (struct stokes *sh);
+/* restarted GMRES over a device operator op (ud, x, y): y = A x, with the
+ * parameters of stokes_set_iter; returns an SHK_E* code */
+int stokes_hip_gmres (stokes_ctx *c, size_t n,
+                      int (*op) (void *, const double *, double *), void *ud,
+                      const double *b_dev, double *x_dev, int *iters,
+                      double *relres);
+/* the lubrication parameters of sys (tables of the coefficients or the
+ * equal-sphere f_k(1), 11 x 51 values into f1) */
+int stokes_hip_lub_params (struct stokes *sys, shk_lub *lp, const double **tab,
+                           double *f1);
+
 /* record an error message for stokes_hip_last_error() */
 void stokes_hip_set_error (const char *fmt, ...);
 int stokes_dim (int version);

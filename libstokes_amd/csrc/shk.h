@@ -106,6 +106,15 @@ typedef struct shk_lub {
   const double *f1;
 } shk_lub;
 int shk_dev_lub (shk_dev *d, int version, const shk_lub *lp, const double *x, double *y);
+/* the dense extracted-form matrix of L (make_matrix_lub_3f/3ft/3fts,
+ * lub-matrix.c), n x n row-major, overwritten */
+int shk_dev_lub_matrix (shk_dev *d, int version, const shk_lub *lp, double *mat);
+/* dense lubricated FTS solves from the dense extracted-form matrices M and
+ * L (both transformed in place): kind 0 resistance, b = (U, O, E) -> x =
+ * M^-1 (I + M L) b = (F, T, S); kind 1 mobility, b = (F, T, E) -> x = (U,
+ * O, S) with (F, T, S) = (M^-1 + L) (U, O, E) */
+int shk_dev_solve_lub (shk_dev *d, double *M, double *L, int np, int kind, const double *b,
+                       double *x);
 
 /* dense (n x n) row-major extracted-form matrix, device pointer */
 int shk_dev_matrix (shk_dev *d, double *mat);

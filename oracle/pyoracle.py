@@ -138,6 +138,8 @@ def ref():
         R.ref_fts_block.argtypes = [C.c_int, _dp, _dp, _dp, _dp]
         R.ref_lub_3fts_mono.argtypes = [C.c_void_p, C.c_double, C.c_double,
                                         _dp, _dp]
+        R.ref_lub_matrix_mono.argtypes = [C.c_void_p, C.c_double,
+                                          C.c_double, _dp]
         R.scalars_nonewald_poly_full.argtypes = [C.c_int, C.c_double,
                                                  C.c_double, C.c_double, _dp]
     return _ref
@@ -320,6 +322,23 @@ class System:
         R.ref_sys_free(h)
         return fts
 
+    def ref_lub_matrix_mono(self, lubmin2=4.0 + 1e-12, lubmax=3.0):
+        """the compiled reference's matrix_lub_{f,ft,fts}_2b over
+        lub-matrix.c's pair loop (equal spheres): the extracted-form
+        lubrication matrix, (n, n)"""
+        R = ref()
+        h = R.ref_sys_new(C.byref(self.s))
+        m = np.zeros((self.n, self.n))
+        R.ref_lub_matrix_mono(h, lubmin2, lubmax, _p(m))
+        R.ref_sys_free(h)
+        return m
+
+    def lub_columns(self, **kw):
+        """the linear map of the oracle's lubrication correction, column by
+        column: L[:, k] = lub(e_k)"""
+        eye = np.eye(self.n)
+        return np.stack([self.lub(eye[k], **kw) for k in range(self.n)], 1)
+
     def ref_matrix(self):
         R = ref()
         h = R.ref_sys_new(C.byref(self.s))
@@ -327,6 +346,12 @@ class System:
         R.ref_matrix(h, _p(m))
         R.ref_sys_free(h)
         return m
+
+
+def ext_to_map(version, np_, mat):
+    """the linear map of an extracted-form matrix: mat . T for FTS
+    (trans_mat_ext2ext), mat itself for F and FT"""
+    return trans_mat_ext2ext(np_, mat) if version == 2 else mat
 
 
 def trans_mat_ext2ext(np_, mat):

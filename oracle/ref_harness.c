@@ -268,3 +268,52 @@ ref_lub_3fts_mono (struct stokes *sys, double lubmin2, double lubmax,
                       fts + d * i, fts + d * j);
             }
 }
+
+/* make_matrix_lub_3f / 3ft / 3fts (lub-matrix.c:76-507) for equal spheres:
+ * the same pair loop over matrix_lub_{f,ft,fts}_2b (f.c:308, ft.c:579,
+ * fts.c:1871); lub-matrix.c itself also names the unequal-sphere pair
+ * routines, which need LAPACK (minv-poly.c), so the loop is driven here */
+void
+ref_lub_matrix_mono (struct stokes *sys, double lubmin2, double lubmax,
+                     double *mat)
+{
+  int np = sys->np;
+  const int d = sys->version == 0 ? 3 : (sys->version == 1 ? 6 : 11);
+  const int n = np * d;
+  void (*pair) (struct stokes *, int, int, const double *, const double *,
+                int, double *)
+    = sys->version == 0 ? matrix_lub_f_2b
+      : (sys->version == 1 ? matrix_lub_ft_2b : matrix_lub_fts_2b);
+  sys->lubmin2 = lubmin2;
+  sys->lubmax = lubmax;
+  for (long k = 0; k < (long) n * n; k++) mat[k] = 0.0;
+  double lubmax2 = lubmax * lubmax;
+  int ix0 = 0, iy0 = 0, iz0 = 0;
+  if (sys->periodic)
+    {
+      double lm = lubmax;
+      if (lm <= 0.0)
+        {
+          lm = sys->lx;
+          if (sys->ly > lm) lm = sys->ly;
+          if (sys->lz > lm) lm = sys->lz;
+        }
+      ix0 = (int) (lm / sys->lx) + 1;
+      iy0 = (int) (lm / sys->ly) + 1;
+      iz0 = (int) (lm / sys->lz) + 1;
+    }
+  for (int i = 0; i < np; i++)
+    for (int j = i; j < np; j++)
+      for (int ix = -ix0; ix <= ix0; ix++)
+        for (int iy = -iy0; iy <= iy0; iy++)
+          for (int iz = -iz0; iz <= iz0; iz++)
+            {
+              double p[3] = {sys->pos[3 * j] + sys->lx * ix,
+                             sys->pos[3 * j + 1] + sys->ly * iy,
+                             sys->pos[3 * j + 2] + sys->lz * iz};
+              if (sys->shear_mode == 1) p[0] += iy * sys->shear_shift;
+              else if (sys->shear_mode == 2) p[0] += iz * sys->shear_shift;
+              if (cond_lub (sys->pos + 3 * i, p, lubmax2) == 0)
+                pair (sys, i, j, sys->pos + 3 * i, p, n, mat);
+            }
+}

@@ -230,3 +230,26 @@ def test_lub_loop_unequal_spheres():
                         want[11 * j:11 * j + 11] += y[11:]
     assert np.linalg.norm(want) > 0.0
     assert np.linalg.norm(got - want) <= 1e-12 * np.linalg.norm(want)
+
+
+@pytest.mark.parametrize("version,case", [(2, "periodic"), (2, "open"),
+                                          (1, "periodic"), (0, "periodic"),
+                                          (2, "self_image")])
+def test_lub_matrix_is_the_operator(version, case):
+    """the compiled reference's extracted-form lubrication matrix (the pair
+    routines matrix_lub_{f,ft,fts}_2b over lub-matrix.c's loop) times T is
+    the oracle's calc_lub_3fts, column by column; "self_image": a box of
+    2.6 radii, where every sphere lubricates with its own images"""
+    if case == "self_image":
+        pos = np.array([[0.3, 0.2, 0.1], [1.55, 1.4, 1.35]])
+        L = 2.6
+    else:
+        pos, L = _near_config(12, 0.2, 8)
+    box = None if case == "open" else (L, L, L)
+    s = po.System(version, pos, box=box, tr=None if box is None else 1.0)
+    for lubmin2, lubmax in ((4.0 + 1e-12, 3.0), (4.0441, 2.6)):
+        m = po.ext_to_map(version, len(pos),
+                          s.ref_lub_matrix_mono(lubmin2, lubmax))
+        want = s.lub_columns(lubmin2=lubmin2, lubmax=lubmax)
+        assert np.linalg.norm(want) > 0.0
+        assert np.linalg.norm(m - want) <= 1e-12 * np.linalg.norm(want)
