@@ -590,12 +590,13 @@ def main():
     shadow.close()
     flops = pin * FLOP_PER_IMAGE_FTS + npairs * 2.0 * 121.0 * nb
     achieved = flops / (real_ms * 1e-3) / 1e12
-    kname = "k_real_mfma<2, 2>"
+    kname = "k_real_mfma_pc<2, 4, 4>"
     roof = dict(bound="mfma", achieved=achieved, peak=PEAK_FP32,
                 unit="TFLOP/s", frac=achieved / PEAK_FP32,
                 traffic=measured_traffic(kname, n, trb, nb),
-                kernel=f"{kname} (real-space pair blocks x {nb} columns: FP32 "
-                       f"VALU moments + FP32 MFMA products)",
+                kernel=f"{kname} (real-space pair blocks x {nb} columns: "
+                       f"producer waves sum the FP32 image moments, consumer "
+                       f"waves contract the blocks on the FP32 matrix cores)",
                 kernel_ms=real_ms, rest_of_call_ms=rest_ms,
                 pair_images=pin, pairs=npairs,
                 flop_per_pair_image=FLOP_PER_IMAGE_FTS,

@@ -569,4 +569,277 @@ k_real_mfma (const double4 *__restrict__ pos, const float *__restrict__ XT, int 
       }
 }
 
+// v from the lane given by the quad permutation CTRL (DPP quad_perm)
+template <int CTRL>
+__device__ __forceinline__ float dpp_quad (float v)
+{
+  return __builtin_bit_cast (float, __builtin_amdgcn_update_dpp (0, __builtin_bit_cast (int, v), CTRL,
+                                                                 0xF, 0xF, true));
+}
+
+template <int CTRL>
+__device__ __forceinline__ unsigned long long dpp_quad64 (unsigned long long v)
+{
+  const unsigned lo = (unsigned) v, hi = (unsigned) (v >> 32);
+  const unsigned plo = (unsigned) __builtin_amdgcn_update_dpp (0, (int) lo, CTRL, 0xF, 0xF, true);
+  const unsigned phi = (unsigned) __builtin_amdgcn_update_dpp (0, (int) hi, CTRL, 0xF, 0xF, true);
+  return ((unsigned long long) phi << 32) | plo;
+}
+
+// candidate images held in LDS by k_real_mfma_pc
+constexpr int REAL_PC_CAND = 256;
+
+// real-space pair sum for 32 NC columns by producer and consumer waves: a
+// workgroup of NP producer and NC consumer waves on 16 target particles i, a
+// range of source particles j and 32 columns per consumer wave.  Per chunk of
+// 4 source particles (64 pairs, pair P = i_l + 16 j_l):
+//   producers: the NP adjacent lanes l of producer wave w with l / NP = P -
+//     w 64/NP share pair P: they test every NP-th candidate image each, OR
+//     the hit masks, and lane phase ph = l % NP evaluates the images of rank
+//     ph, ph + NP, ... (so the images of a pair are spread evenly over its
+//     lanes), sums their moments; the NP partial sums are added by quad
+//     permutations (the same sum in every lane of the group) and written to
+//     LDS as mom[m][P];
+//   consumers: contract the 16 D x 4 D block matrix of the chunk with their
+//     32 columns on the matrix cores, rows component-major (row tile r =
+//     component, row i_l), reduction index 4 q + j_l: lane i_l + 16 j_l
+//     supplies the entry (r, q) of its own pair from the moments in LDS.
+// Producers fill chunk c while consumers contract chunk c - 1 (two moment
+// buffers, one barrier per chunk), so the vector work of the image sums and
+// the matrix-core work run side by side on every SIMD, and the image sums are
+// evaluated once for all 32 NC columns.  partial[((split np + i) D + r) B +
+// b]; XT rows of zs values.
+template <int V, int NP, int NC>
+__global__ __launch_bounds__ (64 * (NP + NC)) __attribute__ ((amdgpu_waves_per_eu (2))) void
+k_real_mfma_pc (const double4 *__restrict__ pos, const float *__restrict__ XT, int B, int zs,
+                const float4 *__restrict__ cand, const int4 *__restrict__ candm,
+                const unsigned *__restrict__ tbits, RealParams p, int jper,
+                float *__restrict__ partial)
+{
+  constexpr int D = Dim<V>::D, NM = Dim<V>::NMOM;
+  constexpr int PW = 64 / NP;                   // pairs per producer wave
+  static_assert (NP == 1 || NP == 2 || NP == 4, "phases are lanes of a quad");
+  __shared__ float mom[2][NM * 64];
+  __shared__ unsigned bits_l[64];
+  __shared__ float4 cand_l[REAL_PC_CAND];
+  __shared__ int4 candm_l[REAL_PC_CAND];
+  // reference-table masks of the candidates, per 64-candidate chunk and
+  // wrap offset (w1, w2, w3) in {-1, 0, 1}^3
+  __shared__ unsigned long long tmask_l[REAL_PC_CAND / 64][27];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int ibase = blockIdx.x * 16;
+  const int j0 = blockIdx.y * jper, j1 = min (p.np, j0 + jper);
+  const int nch = (j1 - j0 + 3) >> 2;
+  const bool bits_in_lds = p.tbits_words <= 64;
+  if (bits_in_lds && threadIdx.x < p.tbits_words) bits_l[threadIdx.x] = tbits[threadIdx.x];
+  // the candidate images in LDS when they fit (the host keeps ncand within
+  // REAL_PC_CAND for this kernel)
+  for (int c = threadIdx.x; c < p.ncand; c += 64 * (NP + NC))
+    {
+      cand_l[c] = cand[c];
+      candm_l[c] = candm[c];
+    }
+  for (int e = threadIdx.x; e < ((p.ncand + 63) >> 6) * 27; e += 64 * (NP + NC))
+    {
+      const int ch = e / 27, w = e - 27 * ch;
+      const int a = w / 9 - 1, b = (w / 3) % 3 - 1, c3 = w % 3 - 1;
+      const int cn = min (64, p.ncand - 64 * ch);
+      unsigned long long m = 0ull;
+      for (int c = 0; c < cn; c++)
+        {
+          const int4 mm = candm[64 * ch + c];
+          if (in_table (p, tbits, mm.x - a, mm.y - b, mm.z - c3)) m |= 1ull << c;
+        }
+      tmask_l[ch][w] = m;
+    }
+  __syncthreads ();
+  if (wave < NP)
+    {
+      // ---- producer
+      const unsigned *bits = bits_in_lds ? bits_l : tbits;
+      const int P = wave * PW + lane / NP, ph = lane % NP;
+      const int il = P & 15, jl = P >> 4;
+      const int ia = ibase + il;
+      const bool ivalid = ia < p.np;
+      const double4 pi = pos[ivalid ? ia : 0];
+      const float ai = float (pi.w), xi = float (p.xi), rmax2 = float (p.rmax2);
+      // the source particle of the next chunk, loaded one chunk ahead
+      double4 pjn = pos[ivalid && j0 + jl < j1 ? j0 + jl : 0];
+      for (int it = 0; it <= nch; it++)
+        {
+          if (it < nch)
+            {
+              const int j = j0 + 4 * it + jl;
+              const bool valid = ivalid && j < j1;
+              const double4 pj = pjn;
+              {
+                const int jn = j + 4;
+                pjn = pos[ivalid && jn < j1 ? jn : 0];
+              }
+              double dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
+              const double n3 = rint (dz / p.lz);
+              dx -= n3 * p.s3x; dz -= n3 * p.lz;
+              const double n2 = rint (dy / p.ly);
+              dx -= n2 * p.s2x; dy -= n2 * p.ly;
+              const double n1 = rint (dx / p.lx);
+              dx -= n1 * p.lx;
+              const int w1 = int (n1), w2 = int (n2), w3 = int (n3);
+              const int wtab = (abs (w1) <= 1 && abs (w2) <= 1 && abs (w3) <= 1)
+                ? (w1 + 1) * 9 + (w2 + 1) * 3 + (w3 + 1) : -1;
+              const float mdx = float (dx), mdy = float (dy), mdz = float (dz);
+              const float aj = float (pj.w);
+              const float rmn = (ai + aj) * float (p.rmin);
+              float M[NM];
+#pragma unroll
+              for (int t = 0; t < NM; t++) M[t] = 0.f;
+              for (int c0 = 0; c0 < p.ncand; c0 += 64)
+                {
+                  // the images of the pair among the next 64 candidates
+                  // (inside r_max and in the reference table): each lane
+                  // of the quad tests every NP-th, the quad ORs the masks
+                  const int cn = min (64, p.ncand - c0);
+                  unsigned long long mask = 0ull;
+#pragma unroll 4
+                  for (int c = ph; c < cn; c += NP)
+                    {
+                      const float4 v = cand_l[c0 + c];
+                      const float ex = mdx + v.x, ey = mdy + v.y, ez = mdz + v.z;
+                      const float rr = ex * ex + ey * ey + ez * ez;
+                      const bool hit = valid && rr != 0.f && rr <= rmax2;
+                      mask |= (unsigned long long) hit << c;
+                    }
+                  if (wtab >= 0)
+                    mask &= tmask_l[c0 >> 6][wtab];
+                  else
+                    {
+                      // wrap offsets beyond one cell: the table test image by image
+                      unsigned long long m2 = mask;
+                      while (m2)
+                        {
+                          const int c = __builtin_ctzll (m2);
+                          m2 &= m2 - 1ull;
+                          const int4 mm = candm_l[c0 + c];
+                          if (!in_table (p, bits, mm.x - w1, mm.y - w2, mm.z - w3))
+                            mask &= ~(1ull << c);
+                        }
+                    }
+                  if (NP >= 2) mask |= dpp_quad64<0xB1> (mask);
+                  if (NP >= 4) mask |= dpp_quad64<0x4E> (mask);
+                  // this lane's share: the images of rank ph, ph + NP, ...
+                  unsigned long long mine = 0ull;
+                  for (int k = 0; mask; k++)
+                    {
+                      const unsigned long long b = mask & (~mask + 1ull);
+                      if ((k & (NP - 1)) == ph) mine |= b;
+                      mask ^= b;
+                    }
+                  while (mine)
+                    {
+                      const int c = __builtin_ctzll (mine);
+                      mine &= mine - 1ull;
+                      const float4 v = cand_l[c0 + c];
+                      const float ex = mdx + v.x, ey = mdy + v.y, ez = mdz + v.z;
+                      const float rr = ex * ex + ey * ey + ez * ez;
+                      float iu = m_rsqrt (rr);
+                      float r = rr * iu;
+                      if (r < rmn)
+                        {
+                          r = rmn;
+                          iu = 1.f / rmn;
+                        }
+                      float sc[11];
+                      if (p.poly) scalars_real_poly_u<V, float> (xi, r, iu, ai, aj, sc);
+                      else scalars_real_u<V, float> (xi, r, iu, sc);
+                      moments_add<V, float> (M, ex * iu, ey * iu, ez * iu, sc);
+                    }
+                }
+              // the NP phases of a pair are adjacent lanes: pairwise sums
+              // by quad permutations (commutative, so every lane of the
+              // group holds the same value)
+#pragma unroll
+              for (int t = 0; t < NM; t++)
+                {
+                  float v = M[t];
+                  if (NP >= 2) v += dpp_quad<0xB1> (v);      // lane ^ 1
+                  if (NP >= 4) v += dpp_quad<0x4E> (v);      // lane ^ 2
+                  M[t] = v;
+                }
+              if (ph == 0)
+                {
+                  float *mw = mom[it & 1];
+#pragma unroll
+                  for (int t = 0; t < NM; t++) mw[t * 64 + P] = M[t];
+                }
+            }
+          __syncthreads ();
+        }
+      return;
+    }
+  // ---- consumer: lane ir + 16 kq holds pair (ir, kq) of a chunk
+  const int ir = lane & 15, kq = lane >> 4;
+  const int cbase = blockIdx.z * 32 * NC + 32 * (wave - NP);
+  mf4 acc[D][2];
+#pragma unroll
+  for (int t = 0; t < D; t++)
+#pragma unroll
+    for (int ct = 0; ct < 2; ct++) acc[t][ct] = mf4{0.f, 0.f, 0.f, 0.f};
+  // z[j][q][cbase + 16 ct + ir] of the next chunk, loaded one chunk ahead
+  float bzn[D][2];
+  auto load_z = [&] (int chunk) {
+    const int jz = j0 + 4 * chunk + kq;
+#pragma unroll
+    for (int ct = 0; ct < 2; ct++)
+      {
+        const int b = cbase + 16 * ct + ir;
+        const bool ok = jz < j1 && b < B;
+        const float *src = XT + (size_t) (ok ? jz : 0) * zs + (ok ? b : 0);
+#pragma unroll
+        for (int q = 0; q < D; q++) bzn[q][ct] = ok ? src[(size_t) q * B] : 0.f;
+      }
+  };
+  for (int it = 0; it <= nch; it++)
+    {
+      float bz[D][2];
+#pragma unroll
+      for (int q = 0; q < D; q++)
+#pragma unroll
+        for (int ct = 0; ct < 2; ct++) bz[q][ct] = bzn[q][ct];
+      if (it < nch) load_z (it);
+      if (it >= 1)
+        {
+          // moment t of this lane's pair at mr[64 t], read from LDS as needed
+          struct LdsM {
+            const float *b;
+            __device__ float operator[] (int t) const { return b[64 * t]; }
+          } Ml{mom[(it - 1) & 1] + lane};
+          // q outer: D independent accumulator pairs between dependent issues
+#pragma unroll
+          for (int q = 0; q < D; q++)
+#pragma unroll
+            for (int r = 0; r < D; r++)
+              {
+                const float a = block_entry<V, float> (Ml, r, q, false);
+                acc[r][0] = mfma4 (a, bz[q][0], acc[r][0]);
+                acc[r][1] = mfma4 (a, bz[q][1], acc[r][1]);
+              }
+        }
+      __syncthreads ();
+    }
+  // row i_l = 4 (lane / 16) + e of row tile r, column 16 ct + lane % 16
+#pragma unroll
+  for (int r = 0; r < D; r++)
+#pragma unroll
+    for (int e = 0; e < 4; e++)
+      {
+        const int i = ibase + 4 * kq + e;
+        if (i < p.np)
+#pragma unroll
+          for (int ct = 0; ct < 2; ct++)
+            {
+              const int b = cbase + 16 * ct + ir;
+              if (b < B) partial[(((size_t) blockIdx.y * p.np + i) * D + r) * B + b] = acc[r][ct][e];
+            }
+      }
+}
+
 }  // namespace shk

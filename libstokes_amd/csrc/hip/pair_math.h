@@ -39,6 +39,31 @@ SHK_HD float  m_rsqrt (float x)  { return rsqrtf (x); }
 
 template <typename T> SHK_HD T inv_sqrt_pi () { return T (0.56418958354775628695); }
 
+// erfc(x) and exp(-x2) (x2 = x^2, x >= 0).  FP64: libm.  FP32 on the device:
+// exp by the hardware exp2, erfc by the Chebyshev fit of Numerical Recipes
+// in C (2nd ed., sec. 6.2, erfcc: fractional error below 1.2e-7 for x >= 0),
+// erfc(x) = t exp(-x^2) exp(P(t)), t = 1 / (1 + x/2), sharing exp(-x^2).
+SHK_HD void erfc_exp (double x, double x2, double &ec, double &e)
+{
+  ec = erfc (x);
+  e = exp (-x2);
+}
+SHK_HD void erfc_exp (float x, float x2, float &ec, float &e)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+  const float l2e = 1.44269504088896341f;
+  e = __builtin_amdgcn_exp2f (-x2 * l2e);
+  const float t = __builtin_amdgcn_rcpf (1.f + 0.5f * x);
+  const float pt = -1.26551223f + t * (1.00002368f + t * (0.37409196f + t * (0.09678418f
+    + t * (-0.18628806f + t * (0.27886807f + t * (-1.13520398f + t * (1.48851587f
+    + t * (-0.82215223f + t * 0.17087277f))))))));
+  ec = t * e * __builtin_amdgcn_exp2f (pt * l2e);
+#else
+  ec = erfcf (x);
+  e = expf (-x2);
+#endif
+}
+
 // ---------------------------------------------------------------------------
 // real-space Ewald scalars, monodisperse (ewald.c:48-186)
 // ---------------------------------------------------------------------------
@@ -50,8 +75,9 @@ SHK_HD void scalars_real_u (T xi, T r, T u, T *o)
 {
   const T xr = xi * r, x2 = xr * xr;
   const T u2 = u * u, u3 = u2 * u;
-  const T ec = m_erfc (xr);
-  const T ex = xi * inv_sqrt_pi<T> () * m_exp (-x2);
+  T ec, ex;
+  erfc_exp (xr, x2, ec, ex);
+  ex *= xi * inv_sqrt_pi<T> ();
   o[1] = (T (0.75) + T (0.5) * u2) * u * ec
     + ((T (1) + x2 * (T (14) + T (4) * x2 * (T (-5) + x2))) * u2 - T (4.5) + T (3) * x2) * ex;
   const T a2 = (T (0.75) - T (1.5) * u2) * u * ec
@@ -142,8 +168,9 @@ SHK_HD void scalars_real_poly_u (T xi, T r, T u, T aa, T ab, T *o)
   const T x = xi * r, x2 = x * x;
   const T u2 = u * u, u3 = u2 * u, u4 = u2 * u2, u5 = u3 * u2;
   const T aa2 = aa * aa, ab2 = ab * ab;
-  const T ec = m_erfc (x);
-  const T ex = xi * inv_sqrt_pi<T> () * m_exp (-x2);
+  T ec, ex;
+  erfc_exp (x, x2, ec, ex);
+  ex *= xi * inv_sqrt_pi<T> ();
   const T sixth = T (1.0 / 6.0), tenth = T (0.1);
   const T a0 = T (2) * (T (-3) + x2 * T (2)) * ex + ec * u;
   const T b0 = T (2) * (T (1) + x2 * T (-2)) * ex + ec * u;
@@ -367,8 +394,8 @@ SHK_HD void moments_add (T *M, T ex, T ey, T ez, const T *s)
 // fts.c:73-111 (only the dense matrix uses them).  Rows/columns: 0-2 U/F,
 // 3-5 omega/T, 6-10 E/S in extracted order (xx xy xz yz yy); S columns act on
 // the transformed vector (fts.c:660-670).
-template <int V, typename T>
-SHK_HD T block_entry (const T *M, int p, int q, bool self)
+template <int V, typename T, typename MA>
+SHK_HD T block_entry (const MA &M, int p, int q, bool self)
 {
   // A: 0-2 x 0-2
   if (p < 3 && q < 3)

@@ -818,9 +818,17 @@ static int atimes_block_vt (shk_dev *d, const double *x, double *y, int B, bool 
   while (CT < CTMAX && 16 * CT < B) CT *= 2;
   const int nib = (p.np + 15) / 16, ncg = (B + 16 * CT - 1) / (16 * CT);
   int nj = (p.np + JW - 1) / JW, jper = 0;
+  // FTS above 32 columns: workgroups of NWG producer waves (image sums) and
+  // NWG consumer waves of 32 columns each (k_real_mfma_pc), 32 NWG columns
+  // per workgroup; one workgroup per CU
+  const int NWG = B > 64 ? 4 : 2;
+  const bool wg = MF && V == 2 && B > 32 && d->ncand <= REAL_PC_CAND;
+  const int ncw = wg ? (B + 32 * NWG - 1) / (32 * NWG) : ncg;
   if (MF)
     {
-      nj = (2048 + nib * ncg - 1) / (nib * ncg);
+      // about 2048 waves, or two workgroups per CU of the producer/consumer form
+      const int wpb = wg ? 4 * ncw : ncg;
+      nj = (2048 + nib * wpb - 1) / (nib * wpb);
       if (nj > (p.np + 15) / 16) nj = (p.np + 15) / 16;
       if (nj < 1) nj = 1;
       jper = (p.np + nj - 1) / nj;
@@ -925,6 +933,21 @@ static int atimes_block_vt (shk_dev *d, const double *x, double *y, int B, bool 
         }
       nj = 1;
         }
+    }
+  else if (wg)
+    {
+      const dim3 g (nib, nj, ncw);
+      const float4 *cf = (const float4 *) d->candf.p;
+      const int4 *cm = (const int4 *) d->candm.p;
+      const unsigned *tb = (const unsigned *) d->tbits.p;
+      const float *xf = (const float *) XT;
+      float *pf = (float *) d->partial.p;
+      if (NWG == 4)
+        hipLaunchKernelGGL ((k_real_mfma_pc<V, 4, 4>), g, dim3 (512), 0, d->stream, pos4, xf, B,
+                            zs, cf, cm, tb, rp, jper, pf);
+      else
+        hipLaunchKernelGGL ((k_real_mfma_pc<V, 2, 2>), g, dim3 (256), 0, d->stream, pos4, xf, B,
+                            zs, cf, cm, tb, rp, jper, pf);
     }
   else if (MF)
     {

@@ -171,7 +171,8 @@ def test_device_api_multi_rhs_and_extmat(golden):
 
 
 @pytest.mark.parametrize("version,poly,nrhs", [
-    (0, False, 70), (1, False, 9), (2, False, 5), (2, True, 3), (0, True, 16)])
+    (0, False, 70), (1, False, 9), (2, False, 5), (2, True, 3), (0, True, 16),
+    (2, False, 64), (2, True, 40), (2, False, 130), (2, True, 129)])
 def test_block_rhs(version, poly, nrhs):
     """several right-hand sides in one call (block kernels): every column
     equals the oracle's single matvec; MIXED within 1e-6"""
