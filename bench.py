@@ -525,7 +525,7 @@ def main():
     t64_ref = time_matvec(s64_ref, xd, yd, max(3, args.steps // 4), 1)
     y64_ref = yd.to_host()
     log(f"fp64 tr=1: {t64_ref * 1e3:.3f} ms")
-    trs = [args.tr] if args.tr else [100.0, 300.0, 1000.0, 3000.0, 1e4]
+    trs = [args.tr] if args.tr else [30.0, 50.0, 100.0, 300.0, 1000.0, 3000.0, 1e4]
 
     # ---- one right-hand side per call: splitting tuned on the mixed path
     best = None
@@ -590,13 +590,13 @@ def main():
     shadow.close()
     flops = pin * FLOP_PER_IMAGE_FTS + npairs * 2.0 * 121.0 * nb
     achieved = flops / (real_ms * 1e-3) / 1e12
-    kname = "k_real_mfma_pc<2, 4, 4>"
+    kname = "k_real_moments<2>+k_real_mfma_mom<2, 4>"
     roof = dict(bound="mfma", achieved=achieved, peak=PEAK_FP32,
                 unit="TFLOP/s", frac=achieved / PEAK_FP32,
                 traffic=measured_traffic(kname, n, trb, nb),
-                kernel=f"{kname} (real-space pair blocks x {nb} columns: "
-                       f"producer waves sum the FP32 image moments, consumer "
-                       f"waves contract the blocks on the FP32 matrix cores)",
+                kernel=f"{kname} (real-space part of the call: FP32 image "
+                       f"moments of each pair i <= j once, then the pair "
+                       f"blocks x {nb} columns on the FP32 matrix cores)",
                 kernel_ms=real_ms, rest_of_call_ms=rest_ms,
                 pair_images=pin, pairs=npairs,
                 flop_per_pair_image=FLOP_PER_IMAGE_FTS,

@@ -842,4 +842,255 @@ k_real_mfma_pc (const double4 *__restrict__ pos, const float *__restrict__ XT, i
       }
 }
 
+// ---------------------------------------------------------------------------
+// Equal spheres with a table symmetric under m -> -m: the image sums of the
+// pair (j, i) are those of (i, j) with e -> -e, so the moments of odd order
+// in e change sign and the rest are equal.  k_real_moments evaluates each
+// pair i <= j once (one lane per pair, every image) and writes both orders
+// into mt[m][i / 16][j][i % 16] (npad x npad per moment, zero where i or j
+// >= np), and k_real_mfma_mom contracts the blocks with the columns on the
+// matrix cores, reading the moments of its pairs from mt.
+// ---------------------------------------------------------------------------
+
+// sign of moment t of the pair seen the other way round
+template <int V> __device__ __forceinline__ float moment_parity (int t)
+{
+  return ((t >= M_SB && t < M_SC) || (t >= M_SG1 && t < M_SH)) ? -1.f : 1.f;
+}
+
+// workgroup: 16 i x 16 j (four waves of 16 i x 4 j); the workgroups with
+// every j below every i return at once
+template <int V>
+__global__ __launch_bounds__ (256) __attribute__ ((amdgpu_waves_per_eu (3))) void
+k_real_moments (const double4 *__restrict__ pos, const float4 *__restrict__ cand,
+                const int4 *__restrict__ candm, const unsigned *__restrict__ tbits, RealParams p,
+                int npad, float *__restrict__ mt)
+{
+  constexpr int NM = Dim<V>::NMOM;
+  if (blockIdx.y < blockIdx.x) return;
+  __shared__ float4 cand_l[REAL_PC_CAND];
+  __shared__ unsigned long long tmask_l[REAL_PC_CAND / 64][27];
+  __shared__ float tr_l[17][16 * 17];
+  for (int c = threadIdx.x; c < p.ncand; c += 256) cand_l[c] = cand[c];
+  for (int e = threadIdx.x; e < ((p.ncand + 63) >> 6) * 27; e += 256)
+    {
+      const int ch = e / 27, w = e - 27 * ch;
+      const int a = w / 9 - 1, b = (w / 3) % 3 - 1, c3 = w % 3 - 1;
+      const int cn = min (64, p.ncand - 64 * ch);
+      unsigned long long m = 0ull;
+      for (int c = 0; c < cn; c++)
+        {
+          const int4 mm = candm[64 * ch + c];
+          if (in_table (p, tbits, mm.x - a, mm.y - b, mm.z - c3)) m |= 1ull << c;
+        }
+      tmask_l[ch][w] = m;
+    }
+  __syncthreads ();
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int i = blockIdx.x * 16 + (lane & 15);
+  const int j = blockIdx.y * 16 + 4 * wave + (lane >> 4);
+  const bool valid = i <= j && j < p.np;
+  float M[NM];
+#pragma unroll
+  for (int t = 0; t < NM; t++) M[t] = 0.f;
+  if (valid)
+    {
+      const double4 pi = pos[i], pj = pos[j];
+      const float ai = float (pi.w), aj = float (pj.w), xi = float (p.xi);
+      const float rmax2 = float (p.rmax2);
+      double dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
+      const double n3 = rint (dz / p.lz);
+      dx -= n3 * p.s3x; dz -= n3 * p.lz;
+      const double n2 = rint (dy / p.ly);
+      dx -= n2 * p.s2x; dy -= n2 * p.ly;
+      const double n1 = rint (dx / p.lx);
+      dx -= n1 * p.lx;
+      const int w1 = int (n1), w2 = int (n2), w3 = int (n3);
+      const int wtab = (abs (w1) <= 1 && abs (w2) <= 1 && abs (w3) <= 1)
+        ? (w1 + 1) * 9 + (w2 + 1) * 3 + (w3 + 1) : -1;
+      const float mdx = float (dx), mdy = float (dy), mdz = float (dz);
+      const float rmn = (ai + aj) * float (p.rmin);
+      for (int c0 = 0; c0 < p.ncand; c0 += 64)
+        {
+          const int cn = min (64, p.ncand - c0);
+          unsigned long long mask = 0ull;
+#pragma unroll 4
+          for (int c = 0; c < cn; c++)
+            {
+              const float4 v = cand_l[c0 + c];
+              const float ex = mdx + v.x, ey = mdy + v.y, ez = mdz + v.z;
+              const float rr = ex * ex + ey * ey + ez * ez;
+              mask |= (unsigned long long) (rr != 0.f && rr <= rmax2) << c;
+            }
+          if (wtab >= 0)
+            mask &= tmask_l[c0 >> 6][wtab];
+          else
+            {
+              unsigned long long m2 = mask;
+              while (m2)
+                {
+                  const int c = __builtin_ctzll (m2);
+                  m2 &= m2 - 1ull;
+                  const int4 mm = candm[c0 + c];
+                  if (!in_table (p, tbits, mm.x - w1, mm.y - w2, mm.z - w3)) mask &= ~(1ull << c);
+                }
+            }
+          while (mask)
+            {
+              const int c = __builtin_ctzll (mask);
+              mask &= mask - 1ull;
+              const float4 v = cand_l[c0 + c];
+              const float ex = mdx + v.x, ey = mdy + v.y, ez = mdz + v.z;
+              const float rr = ex * ex + ey * ey + ez * ez;
+              float iu = m_rsqrt (rr);
+              float r = rr * iu;
+              if (r < rmn)
+                {
+                  r = rmn;
+                  iu = 1.f / rmn;
+                }
+              float sc[11];
+              scalars_real_u<V, float> (xi, r, iu, sc);
+              moments_add<V, float> (M, ex * iu, ey * iu, ez * iu, sc);
+            }
+        }
+    }
+  // (i, j) for i <= j (zeros for the padding pairs), coalesced
+  const size_t ms = (size_t) npad * npad;
+  if (i <= j)
+    {
+      float *o = mt + (size_t) (i >> 4) * npad * 16 + (size_t) j * 16 + (i & 15);
+#pragma unroll
+      for (int t = 0; t < NM; t++) o[t * ms] = M[t];
+    }
+  // (j, i) for i < j with the odd moments negated: this workgroup's 16 x 16
+  // block of them is 256 consecutive floats per moment (element il 16 + jl
+  // from the pair il, jl), staged through LDS in rounds of MR moments
+  constexpr int MR = 17;
+  const int il = lane & 15, jl = 4 * wave + (lane >> 4);
+  float *ob = mt + (size_t) blockIdx.y * npad * 16 + (size_t) blockIdx.x * 256;
+  const bool diag = blockIdx.x == blockIdx.y;
+#pragma unroll
+  for (int t0 = 0; t0 < NM; t0 += MR)
+    {
+#pragma unroll
+      for (int u = 0; u < MR; u++)
+        if (t0 + u < NM) tr_l[u][il * 17 + jl] = moment_parity<V> (t0 + u) * M[t0 + u];
+      __syncthreads ();
+      const int e = threadIdx.x, ei = e >> 4, ej = e & 15;
+      if (!diag || ei < ej)
+#pragma unroll
+        for (int u = 0; u < MR; u++)
+          if (t0 + u < NM) ob[(size_t) (t0 + u) * ms + e] = tr_l[u][ei * 17 + ej];
+      __syncthreads ();
+    }
+}
+
+// the block contraction of k_real_mfma_pc's consumer waves with the moments
+// read from mt: NC waves of 32 columns per workgroup share the moments of a
+// chunk through LDS (each wave loads every NC-th moment one chunk ahead; two
+// buffers, one barrier per chunk)
+template <int V, int NC>
+__global__ __launch_bounds__ (64 * NC) __attribute__ ((amdgpu_waves_per_eu (2))) void
+k_real_mfma_mom (const float *__restrict__ mt, int npad, const float *__restrict__ XT, int B,
+                 int zs, int np, int jper, float *__restrict__ partial)
+{
+  constexpr int D = Dim<V>::D, NM = Dim<V>::NMOM;
+  constexpr int TW = (NM + NC - 1) / NC;        // moments loaded per wave
+  __shared__ float mom[2][NM * 64];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int ir = lane & 15, kq = lane >> 4;
+  const int ibase = blockIdx.x * 16;
+  const int j0 = blockIdx.y * jper, j1 = min (np, j0 + jper);
+  const int cbase = blockIdx.z * 32 * NC + 32 * wave;
+  const size_t ms = (size_t) npad * npad;
+  // moment t of pair (ibase + ir, jb + kq) at mb[t ms + 16 jb]
+  const float *mb = mt + (size_t) blockIdx.x * npad * 16 + lane;
+  float pre[TW];
+  auto load_mom = [&] (int jb) {
+    const bool jok = jb + kq < j1;
+#pragma unroll
+    for (int u = 0; u < TW; u++)
+      {
+        const int t = wave + NC * u;
+        pre[u] = (t < NM && jok) ? mb[(size_t) t * ms + (size_t) jb * 16] : 0.f;
+      }
+  };
+  auto store_mom = [&] (int buf) {
+#pragma unroll
+    for (int u = 0; u < TW; u++)
+      {
+        const int t = wave + NC * u;
+        if (t < NM) mom[buf][t * 64 + lane] = pre[u];
+      }
+  };
+  mf4 acc[D][2];
+#pragma unroll
+  for (int t = 0; t < D; t++)
+#pragma unroll
+    for (int ct = 0; ct < 2; ct++) acc[t][ct] = mf4{0.f, 0.f, 0.f, 0.f};
+  // z[j][q][cbase + 16 ct + ir] of the next chunk, loaded one chunk ahead
+  float bzn[D][2];
+  auto load_z = [&] (int jb) {
+    const int jz = jb + kq;
+#pragma unroll
+    for (int ct = 0; ct < 2; ct++)
+      {
+        const int b = cbase + 16 * ct + ir;
+        const bool ok = jz < j1 && b < B;
+        const float *src = XT + (size_t) (ok ? jz : 0) * zs + (ok ? b : 0);
+#pragma unroll
+        for (int q = 0; q < D; q++) bzn[q][ct] = ok ? src[(size_t) q * B] : 0.f;
+      }
+  };
+  load_mom (j0);
+  load_z (j0);
+  store_mom (0);
+  __syncthreads ();
+  int buf = 0;
+  for (int jb = j0; jb < j1; jb += 4, buf ^= 1)
+    {
+      const bool more = jb + 4 < j1;
+      float bz[D][2];
+#pragma unroll
+      for (int q = 0; q < D; q++)
+#pragma unroll
+        for (int ct = 0; ct < 2; ct++) bz[q][ct] = bzn[q][ct];
+      if (more)
+        {
+          load_mom (jb + 4);
+          load_z (jb + 4);
+        }
+      struct LdsM {
+        const float *b;
+        __device__ float operator[] (int t) const { return b[64 * t]; }
+      } Ml{mom[buf] + lane};
+#pragma unroll
+      for (int q = 0; q < D; q++)
+#pragma unroll
+        for (int r = 0; r < D; r++)
+          {
+            const float a = block_entry<V, float> (Ml, r, q, false);
+            acc[r][0] = mfma4 (a, bz[q][0], acc[r][0]);
+            acc[r][1] = mfma4 (a, bz[q][1], acc[r][1]);
+          }
+      if (more) store_mom (buf ^ 1);
+      __syncthreads ();
+    }
+#pragma unroll
+  for (int r = 0; r < D; r++)
+#pragma unroll
+    for (int e = 0; e < 4; e++)
+      {
+        const int i = ibase + 4 * kq + e;
+        if (i < np)
+#pragma unroll
+          for (int ct = 0; ct < 2; ct++)
+            {
+              const int b = cbase + 16 * ct + ir;
+              if (b < B) partial[(((size_t) blockIdx.y * np + i) * D + r) * B + b] = acc[r][ct][e];
+            }
+      }
+}
+
 }  // namespace shk

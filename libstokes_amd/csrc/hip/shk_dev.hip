@@ -66,6 +66,8 @@ struct shk_dev {
   DevBuf cand, candf, candm, tbits, kpart, kpq, kback;
   DevBuf adjfg, adjcs, adjpart;
   DevBuf lubcoef, lubf1, lubcnt, luboff, lublist, lubval;   // lubrication
+  DevBuf momt;                    // real-space pair moments (equal spheres)
+  int tsym = 0;                   // reference table symmetric under m -> -m
   int lub_N = -1;                 // order of the uploaded coefficient table
   int lub_f1 = 0;                 // f_k(1) table uploaded
   DevBuf km, ptab;                // k lattice indices, per-particle phase tables
@@ -179,7 +181,8 @@ extern "C" void shk_dev_destroy (shk_dev *d)
                     &d->pq, &d->xs, &d->ys, &d->zt, &d->cand, &d->candf, &d->candm,
                     &d->tbits, &d->kpart, &d->kpq, &d->kback, &d->adjfg, &d->adjcs,
                     &d->adjpart, &d->km, &d->ptab, &d->rcache, &d->rz, &d->lubcoef,
-                    &d->lubf1, &d->lubcnt, &d->luboff, &d->lublist, &d->lubval})
+                    &d->lubf1, &d->lubcnt, &d->luboff, &d->lublist, &d->lubval,
+                    &d->momt})
     release (*b);
   if (d->blas) (void) rocblas_destroy_handle (d->blas);
   for (auto &e : d->ev)
@@ -391,6 +394,20 @@ extern "C" int shk_dev_set_system (shk_dev *d, const shk_params *p, const double
         }
       d->ncand = t->ncand; d->ncore = t->ncore; d->tbits_words = t->tbits_words;
       d->tx = t->tx; d->ty = t->ty; d->tz = t->tz;
+      // the image set of (j, i) is that of (i, j) negated when the table
+      // holds m and -m alike
+      d->tsym = 1;
+      {
+        const int ny = 2 * t->ty + 1, nz = 2 * t->tz + 1;
+        const long ntot = (long) (2 * t->tx + 1) * ny * nz;
+        for (long idx = 0; idx < ntot && d->tsym; idx++)
+          {
+            const long jdx = ntot - 1 - idx;      // (m1, m2, m3) -> (-m1, -m2, -m3)
+            const int a = (t->tbits[idx >> 5] >> (idx & 31)) & 1;
+            const int b = (t->tbits[jdx >> 5] >> (jdx & 31)) & 1;
+            if (a != b) d->tsym = 0;
+          }
+      }
       d->s2x = t->s2x; d->s3x = t->s3x;
     }
   // the host arrays may be freed by the caller right after return
@@ -824,10 +841,16 @@ static int atimes_block_vt (shk_dev *d, const double *x, double *y, int B, bool 
   const int NWG = B > 64 ? 4 : 2;
   const bool wg = MF && V == 2 && B > 32 && d->ncand <= REAL_PC_CAND;
   const int ncw = wg ? (B + 32 * NWG - 1) / (32 * NWG) : ncg;
+  // equal spheres with a symmetric image table: each pair's moments once
+  // (k_real_moments, both orders into momt), then NWG waves of 32 columns
+  // per workgroup contract them (k_real_mfma_mom); momt holds
+  // NMOM npad^2 floats, used up to 2 GiB
+  const bool mom = wg && !p.poly && d->tsym
+                   && (double) Dim<V>::NMOM * npad * npad * 4.0 <= 2.0 * 1024 * 1024 * 1024;
   if (MF)
     {
       // about 2048 waves, or two workgroups per CU of the producer/consumer form
-      const int wpb = wg ? 4 * ncw : ncg;
+      const int wpb = mom ? NWG * ncw : (wg ? 4 * ncw : ncg);
       nj = (2048 + nib * wpb - 1) / (nib * wpb);
       if (nj > (p.np + 15) / 16) nj = (p.np + 15) / 16;
       if (nj < 1) nj = 1;
@@ -933,6 +956,25 @@ static int atimes_block_vt (shk_dev *d, const double *x, double *y, int B, bool 
         }
       nj = 1;
         }
+    }
+  else if (mom)
+    {
+      const size_t mbytes = sizeof (float) * Dim<V>::NMOM * (size_t) npad * npad;
+      if ((rc = ensure (d, d->momt, mbytes))) return rc;
+      float *mt = (float *) d->momt.p;
+      const int nt = npad / 16;
+      hipLaunchKernelGGL ((k_real_moments<V>), dim3 (nt, nt), dim3 (256), 0, d->stream, pos4,
+                          (const float4 *) d->candf.p, (const int4 *) d->candm.p,
+                          (const unsigned *) d->tbits.p, rp, npad, mt);
+      HIPCHK (hipGetLastError ());
+      const dim3 g (nib, nj, ncw);
+      float *pf = (float *) d->partial.p;
+      if (NWG == 4)
+        hipLaunchKernelGGL ((k_real_mfma_mom<V, 4>), g, dim3 (256), 0, d->stream,
+                            (const float *) mt, npad, (const float *) XT, B, zs, p.np, jper, pf);
+      else
+        hipLaunchKernelGGL ((k_real_mfma_mom<V, 2>), g, dim3 (128), 0, d->stream,
+                            (const float *) mt, npad, (const float *) XT, B, zs, p.np, jper, pf);
     }
   else if (wg)
     {
