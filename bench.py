@@ -4,10 +4,10 @@ Workload (the 11N x 11N FTS configuration the headline metric is quoted
 on): N = 1024 monodisperse spheres (a = 1) at volume fraction 0.3 in a
 cubic box, Ewald tolerance 1e-12, F/T/S ~ N(0,1), seeded synthetic
 positions (non-overlapping, gap >= 2.1a).  One step = one call of the
-batched matvec Y = M(x) . X on B = 32 right-hand sides (11N x 32), from
+batched matvec Y = M(x) . X on B = 128 right-hand sides (11N x 128), from
 device-resident positions and forces (positions re-read every call).
 
-value   = matvecs/s (32 per step) of the MIXED batched path (FP32 real
+value   = matvecs/s (128 per step) of the MIXED batched path (FP32 real
           space and reciprocal sums on the matrix cores, FP64 k-blocks,
           partial sums and self term), at the Ewald splitting chosen by
           timing the batched call, with the largest per-column relative
@@ -17,13 +17,17 @@ fp64    = matvecs/s of the FP64 GPU path: batched at the same splitting,
           reference's default splitting (tr = 1); mixed_over_fp64 divides
           value by the best of these.
 single  = the same figures for one right-hand side per call.
-roofline: the real-space kernel of the batched MIXED call (k_real_mfma,
-          FP32 VALU pair scalars + FP32 matrix-core block products);
-          algorithmic flops = pair-images inside r_max x 300 + pairs with
-          an image inside r_max x 2 x 121 x B; kernel time from HIP events
-          around that launch on the library stream.
+roofline: the real-space part of the batched MIXED call (for equal
+          spheres k_real_moments + k_real_mfma_mom: FP32 image moments per
+          pair, then the pair blocks times the columns on the FP32 matrix
+          cores); algorithmic flops = pair-images inside r_max x 300 +
+          pairs with an image inside r_max x 2 x 121 x B; time from HIP
+          events around that part on the library stream.
 cpu_baseline: the oracle (FP64 C restatement of the reference loops, one
-          core) timed on a bounded row sample of the same workload.
+          core) timed on a bounded row sample of the same workload, with
+          the oracle's time over the compiled reference's on the
+          reference's own CPU case measured beside it.
+bd, ensemble: the BD step (N = 2048 F) and the 256-box ensemble figures.
 """
 import argparse
 import json
@@ -127,6 +131,33 @@ def cpu_baseline(pos, L, x, tr, budget_s):
                 sample=f"oracle FP64 rows 0..{rows - 1} of the N={n} FTS "
                        f"matvec (tr={tr:g}), {t:.1f} s; rate = rows/N/time",
                 host=host_cpu())
+
+
+def oracle_over_reference():
+    """the oracle's time over the compiled reference's (oracle/_ref) for one
+    FP64 FTS matvec of the reference's CPU-runnable case (N = 64, phi = 0.1,
+    tr = 1), both on one core of this host; None without oracle/_ref"""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as po
+    if not po.ref_available():
+        return None
+    pos, L = workloads.configuration(64, 0.1, seed=21)
+    x = np.random.default_rng(3).standard_normal(11 * 64)
+    o = po.System(2, pos, box=(L, L, L), tr=1.0)
+    o.atimes(x)
+    o.ref_atimes(x)
+    reps = 3
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        y1 = o.atimes(x)
+    ta = (time.perf_counter() - t0) / reps
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        y2 = o.ref_atimes(x)
+    tb = (time.perf_counter() - t0) / reps
+    return {"case": "FTS N=64 phi=0.1 tr=1, one core", "oracle_s": ta,
+            "reference_s": tb, "oracle_over_reference": ta / tb,
+            "max_rel_diff": float(np.abs(y1 - y2).max() / np.abs(y2).max())}
 
 
 def host_cpu():
@@ -333,6 +364,41 @@ def bench_bd(args, log):
         log(f"cheb block 64 {name}: {tc * 1e3:.3f} ms (1/sqrt), "
             f"{ts * 1e3:.3f} ms (sqrt)")
         s.close()
+    # the same steps with the overlap check on (sys->rmin = 0: no contact
+    # clamp; an overlap after a step halves dt or redraws the noise,
+    # brownian.c:2436-2560), MIXED, at the splitting of the run above
+    tr_ol = out.get("mixed", {}).get("tr", 1000.0)
+    s = ls.Stokes(0, pos, box=(L, L, L), tr=1.0, precision=ls.MIXED, rmin=0.0)
+    s.set_split(tr_ol)
+    xd = ls.DeviceArray.from_host(pos.reshape(-1))
+    bd = ls.BD(s, peclet=1.0, n_cheb=30, eps=5e-2, dt_lim=1e-6,
+               rmin_ol=1.0, seed=11)
+    try:
+        for _ in range(2):
+            bd.step(xd, 1e-3)
+        s.sync()
+        att = halv = redr = 0
+        t0 = time.perf_counter()
+        for _ in range(args.bd_steps):
+            info = bd.step(xd, 1e-3)
+            att += info.attempts
+            halv += info.halvings
+            redr += info.redraws
+        s.sync()
+        t = time.perf_counter() - t0
+        out["mixed_overlap_check"] = {
+            "rmin": 0.0, "tr": tr_ol, "steps_per_s": args.bd_steps / t,
+            "ms_per_step": t / args.bd_steps * 1e3,
+            "attempts_per_step": att / args.bd_steps, "halvings": halv,
+            "redraws": redr}
+        log(f"BD mixed, overlap check on: {t / args.bd_steps * 1e3:.3f} "
+            f"ms/step, {att / args.bd_steps:.2f} attempts/step")
+    except ls.StokesError as e:
+        out["mixed_overlap_check"] = {"rmin": 0.0, "error": str(e)}
+        log(f"BD mixed, overlap check on: {e}")
+    bd.close()
+    xd.free()
+    s.close()
     return out
 
 
@@ -341,14 +407,11 @@ def _poly_box(args):
     return workloads.configuration_poly(n, 0.25, seed=seed)
 
 
-def bench_ensemble(args, log):
-    """Batched ensemble: nbox independent FTS boxes, N log-uniform on
-    [50, 2000], radii log-uniform on [0.5, 2], phi = 0.25, MIXED precision,
-    one graph launch per ensemble product (each box at the splitting its
-    own autotuned product was fastest at, results those of the box's own
-    tr = 1 splitting)"""
+def ensemble_boxes(nbox, log):
+    """the ensemble's configurations: N log-uniform on [50, 2000], radii
+    log-uniform on [0.5, 2], phi = 0.25 (generated by worker processes
+    before this process first touches the GPU)"""
     import multiprocessing as mp
-    nbox = args.ens_boxes
     rng = np.random.default_rng(0)
     ns = np.exp(rng.uniform(math.log(50), math.log(2000), nbox)).astype(int)
     t0 = time.perf_counter()
@@ -356,6 +419,15 @@ def bench_ensemble(args, log):
         boxes = pool.map(_poly_box, [(int(n), 1000 + k) for k, n in
                                      enumerate(ns)])
     log(f"ensemble: {nbox} configurations in {time.perf_counter() - t0:.1f} s")
+    return boxes
+
+
+def bench_ensemble(args, boxes, log):
+    """Batched ensemble: nbox independent FTS boxes (ensemble_boxes), MIXED
+    precision, one graph launch per ensemble product (each box at the
+    splitting its own autotuned product was fastest at, results those of
+    the box's own tr = 1 splitting)"""
+    nbox = len(boxes)
     systems = []
     for pos, a, L in boxes:
         s = ls.Stokes(2, pos, box=(L, L, L), tr=1.0, a=a, precision=ls.MIXED)
@@ -482,7 +554,7 @@ def main():
                     help="right-hand sides per launch for the batched figures")
     ap.add_argument("--ens-boxes", type=int, default=256,
                     help="boxes of the batched-ensemble figure (0: skip)")
-    ap.add_argument("--bd-steps", type=int, default=50,
+    ap.add_argument("--bd-steps", type=int, default=100,
                     help="timed BD steps per precision (0: skip BD)")
     ap.add_argument("--profile-batch", type=int, default=1,
                     help="right-hand sides per call in --profile mode")
@@ -503,13 +575,16 @@ def main():
                           "--gpus must be 1"}))
         sys.exit(2)
 
+    log = lambda m: print(m, file=sys.stderr, flush=True)
+    # CPU-only set-up first: the worker processes start before this process
+    # touches the GPU
+    boxes = ensemble_boxes(args.ens_boxes, log) if args.ens_boxes > 0 else None
     n = args.n
     pos, L = workloads.configuration(n, args.phi, seed=1)
     x = np.random.default_rng(2).standard_normal(11 * n)
     posd = ls.DeviceArray.from_host(pos)
     xd = ls.DeviceArray.from_host(x)
     yd = ls.DeviceArray(11 * n)
-    log = lambda m: print(m, file=sys.stderr, flush=True)
 
     def make(tr, prec):
         # the system keeps the reference's default splitting (tr = 1); the
@@ -566,7 +641,18 @@ def main():
     smixb = make(trb, ls.MIXED)
     tb_mix = time_matvec(smixb, xb, yb, args.steps, args.warmup, nrhs=nb)
     Yb_mix = yb.to_host().reshape(nb, -1)
-    s64b = make(trb, ls.FP64)
+    # FP64 batched at its own best splitting (the balance of its real-space
+    # and reciprocal kernels differs from the mixed one)
+    best64 = None
+    for tr in trs:
+        s = make(tr, ls.FP64)
+        t = time_matvec(s, xb, yb, 2, 1, nrhs=nb)
+        log(f"batch {nb} fp64 tr={tr:g}: {t * 1e3:.3f} ms")
+        if best64 is None or t < best64[1]:
+            best64 = (tr, t)
+        s.close()
+    tr64b = best64[0]
+    s64b = make(tr64b, ls.FP64)
     tb_64 = time_matvec(s64b, xb, yb, max(3, args.steps // 4), 1, nrhs=nb)
     Yb_64 = yb.to_host().reshape(nb, -1)
     s64b.close()
@@ -578,7 +664,7 @@ def main():
     y0 = yd.to_host()
     err_b_tr1 = float(np.linalg.norm(Yb_mix[0] - y0) / np.linalg.norm(y0))
     log(f"batch {nb}: mixed {tb_mix * 1e3:.3f} ms (tr={trb:g}), fp64 "
-        f"{tb_64 * 1e3:.3f} ms, max col err {max(errs):.2e}")
+        f"{tb_64 * 1e3:.3f} ms (tr={tr64b:g}), max col err {max(errs):.2e}")
     fp64_best = max(1.0 / t64, 1.0 / t64_ref, nb / tb_64)
 
     # roofline of the dominant kernel of the batched MIXED call: the
@@ -665,6 +751,7 @@ def main():
         "rel_err_mixed_vs_fp64_max_column": max(errs),
         "rel_err_mixed_vs_fp64_tr1_column0": err_b_tr1,
         "fp64_matvecs_per_s_batched": nb / tb_64,
+        "fp64_batched_tr": tr64b,
         "fp64_matvecs_per_s_single": 1.0 / t64,
         "fp64_matvecs_per_s_single_tr1": 1.0 / t64_ref,
         "mixed_over_fp64": (nb / tb_mix) / fp64_best,
@@ -680,9 +767,12 @@ def main():
     if args.bd_steps > 0:
         out["bd"] = bench_bd(args, log)
     if args.ens_boxes > 0:
-        out["ensemble"] = bench_ensemble(args, log)
+        out["ensemble"] = bench_ensemble(args, boxes, log)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(pos, L, x, 1.0, args.cpu_budget)
+        ratio = oracle_over_reference()
+        if ratio is not None:
+            out["cpu_baseline"]["oracle_vs_reference"] = ratio
         thr = max(1, min(16, os.cpu_count() or 1))
         out["cpu_baseline_threads"] = cpu_baseline_threads(
             pos, L, x, 1.0, args.cpu_budget, thr)

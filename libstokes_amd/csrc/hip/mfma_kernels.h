@@ -860,17 +860,18 @@ template <int V> __device__ __forceinline__ float moment_parity (int t)
 
 // workgroup: 16 i x 16 j (four waves of 16 i x 4 j); the workgroups with
 // every j below every i return at once
-template <int V>
-__global__ __launch_bounds__ (256) __attribute__ ((amdgpu_waves_per_eu (3))) void
-k_real_moments (const double4 *__restrict__ pos, const float4 *__restrict__ cand,
+template <int V, typename T>
+__global__ __launch_bounds__ (256) __attribute__ ((amdgpu_waves_per_eu (sizeof (T) == 4 ? 3 : 2))) void
+k_real_moments (const double4 *__restrict__ pos, const typename Vec4<T>::type *__restrict__ cand,
                 const int4 *__restrict__ candm, const unsigned *__restrict__ tbits, RealParams p,
-                int npad, float *__restrict__ mt)
+                int npad, T *__restrict__ mt)
 {
   constexpr int NM = Dim<V>::NMOM;
+  using T4 = typename Vec4<T>::type;
   if (blockIdx.y < blockIdx.x) return;
-  __shared__ float4 cand_l[REAL_PC_CAND];
+  __shared__ T4 cand_l[REAL_PC_CAND];
   __shared__ unsigned long long tmask_l[REAL_PC_CAND / 64][27];
-  __shared__ float tr_l[17][16 * 17];
+  __shared__ T tr_l[17][16 * 17];
   for (int c = threadIdx.x; c < p.ncand; c += 256) cand_l[c] = cand[c];
   for (int e = threadIdx.x; e < ((p.ncand + 63) >> 6) * 27; e += 256)
     {
@@ -890,14 +891,14 @@ k_real_moments (const double4 *__restrict__ pos, const float4 *__restrict__ cand
   const int i = blockIdx.x * 16 + (lane & 15);
   const int j = blockIdx.y * 16 + 4 * wave + (lane >> 4);
   const bool valid = i <= j && j < p.np;
-  float M[NM];
+  T M[NM];
 #pragma unroll
-  for (int t = 0; t < NM; t++) M[t] = 0.f;
+  for (int t = 0; t < NM; t++) M[t] = T (0);
   if (valid)
     {
       const double4 pi = pos[i], pj = pos[j];
-      const float ai = float (pi.w), aj = float (pj.w), xi = float (p.xi);
-      const float rmax2 = float (p.rmax2);
+      const T ai = T (pi.w), aj = T (pj.w), xi = T (p.xi);
+      const T rmax2 = T (p.rmax2);
       double dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
       const double n3 = rint (dz / p.lz);
       dx -= n3 * p.s3x; dz -= n3 * p.lz;
@@ -908,8 +909,8 @@ k_real_moments (const double4 *__restrict__ pos, const float4 *__restrict__ cand
       const int w1 = int (n1), w2 = int (n2), w3 = int (n3);
       const int wtab = (abs (w1) <= 1 && abs (w2) <= 1 && abs (w3) <= 1)
         ? (w1 + 1) * 9 + (w2 + 1) * 3 + (w3 + 1) : -1;
-      const float mdx = float (dx), mdy = float (dy), mdz = float (dz);
-      const float rmn = (ai + aj) * float (p.rmin);
+      const T mdx = T (dx), mdy = T (dy), mdz = T (dz);
+      const T rmn = (ai + aj) * T (p.rmin);
       for (int c0 = 0; c0 < p.ncand; c0 += 64)
         {
           const int cn = min (64, p.ncand - c0);
@@ -917,10 +918,10 @@ k_real_moments (const double4 *__restrict__ pos, const float4 *__restrict__ cand
 #pragma unroll 4
           for (int c = 0; c < cn; c++)
             {
-              const float4 v = cand_l[c0 + c];
-              const float ex = mdx + v.x, ey = mdy + v.y, ez = mdz + v.z;
-              const float rr = ex * ex + ey * ey + ez * ez;
-              mask |= (unsigned long long) (rr != 0.f && rr <= rmax2) << c;
+              const T4 v = cand_l[c0 + c];
+              const T ex = mdx + v.x, ey = mdy + v.y, ez = mdz + v.z;
+              const T rr = ex * ex + ey * ey + ez * ez;
+              mask |= (unsigned long long) (rr != T (0) && rr <= rmax2) << c;
             }
           if (wtab >= 0)
             mask &= tmask_l[c0 >> 6][wtab];
@@ -939,19 +940,19 @@ k_real_moments (const double4 *__restrict__ pos, const float4 *__restrict__ cand
             {
               const int c = __builtin_ctzll (mask);
               mask &= mask - 1ull;
-              const float4 v = cand_l[c0 + c];
-              const float ex = mdx + v.x, ey = mdy + v.y, ez = mdz + v.z;
-              const float rr = ex * ex + ey * ey + ez * ez;
-              float iu = m_rsqrt (rr);
-              float r = rr * iu;
+              const T4 v = cand_l[c0 + c];
+              const T ex = mdx + v.x, ey = mdy + v.y, ez = mdz + v.z;
+              const T rr = ex * ex + ey * ey + ez * ez;
+              T iu = m_rsqrt (rr);
+              T r = rr * iu;
               if (r < rmn)
                 {
                   r = rmn;
-                  iu = 1.f / rmn;
+                  iu = T (1) / rmn;
                 }
-              float sc[11];
-              scalars_real_u<V, float> (xi, r, iu, sc);
-              moments_add<V, float> (M, ex * iu, ey * iu, ez * iu, sc);
+              T sc[11];
+              scalars_real_u<V, T> (xi, r, iu, sc);
+              moments_add<V, T> (M, ex * iu, ey * iu, ez * iu, sc);
             }
         }
     }
@@ -959,7 +960,7 @@ k_real_moments (const double4 *__restrict__ pos, const float4 *__restrict__ cand
   const size_t ms = (size_t) npad * npad;
   if (i <= j)
     {
-      float *o = mt + (size_t) (i >> 4) * npad * 16 + (size_t) j * 16 + (i & 15);
+      T *o = mt + (size_t) (i >> 4) * npad * 16 + (size_t) j * 16 + (i & 15);
 #pragma unroll
       for (int t = 0; t < NM; t++) o[t * ms] = M[t];
     }
@@ -968,14 +969,14 @@ k_real_moments (const double4 *__restrict__ pos, const float4 *__restrict__ cand
   // from the pair il, jl), staged through LDS in rounds of MR moments
   constexpr int MR = 17;
   const int il = lane & 15, jl = 4 * wave + (lane >> 4);
-  float *ob = mt + (size_t) blockIdx.y * npad * 16 + (size_t) blockIdx.x * 256;
+  T *ob = mt + (size_t) blockIdx.y * npad * 16 + (size_t) blockIdx.x * 256;
   const bool diag = blockIdx.x == blockIdx.y;
 #pragma unroll
   for (int t0 = 0; t0 < NM; t0 += MR)
     {
 #pragma unroll
       for (int u = 0; u < MR; u++)
-        if (t0 + u < NM) tr_l[u][il * 17 + jl] = moment_parity<V> (t0 + u) * M[t0 + u];
+        if (t0 + u < NM) tr_l[u][il * 17 + jl] = T (moment_parity<V> (t0 + u)) * M[t0 + u];
       __syncthreads ();
       const int e = threadIdx.x, ei = e >> 4, ej = e & 15;
       if (!diag || ei < ej)
@@ -1090,6 +1091,98 @@ k_real_mfma_mom (const float *__restrict__ mt, int npad, const float *__restrict
               const int b = cbase + 16 * ct + ir;
               if (b < B) partial[(((size_t) blockIdx.y * np + i) * D + r) * B + b] = acc[r][ct][e];
             }
+      }
+}
+
+typedef double md4 __attribute__ ((ext_vector_type (4)));
+
+__device__ __forceinline__ md4 mfma4d (double a, double b, md4 c)
+{
+  return __builtin_amdgcn_mfma_f64_16x16x4f64 (a, b, c, 0, 0, 0);
+}
+
+// FP64 form of k_real_mfma_mom: v_mfma_f64_16x16x4f64 (A[i][k] in lane i +
+// 16 k, B[k][j] in lane j + 16 k, D[i][j] in element i / 4 of lane j + 16
+// (i % 4)), 16 columns per wave, NC waves per workgroup
+template <int V, int NC>
+__global__ __launch_bounds__ (64 * NC) __attribute__ ((amdgpu_waves_per_eu (2))) void
+k_real_mfma_mom64 (const double *__restrict__ mt, int npad, const double *__restrict__ XT,
+                   int B, int zs, int np, int jper, double *__restrict__ partial)
+{
+  constexpr int D = Dim<V>::D, NM = Dim<V>::NMOM;
+  constexpr int TW = (NM + NC - 1) / NC;
+  __shared__ double mom[2][NM * 64];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int ir = lane & 15, kq = lane >> 4;
+  const int ibase = blockIdx.x * 16;
+  const int j0 = blockIdx.y * jper, j1 = min (np, j0 + jper);
+  const int cbase = blockIdx.z * 16 * NC + 16 * wave;
+  const size_t ms = (size_t) npad * npad;
+  const double *mb = mt + (size_t) blockIdx.x * npad * 16 + lane;
+  double pre[TW];
+  auto load_mom = [&] (int jb) {
+    const bool jok = jb + kq < j1;
+#pragma unroll
+    for (int u = 0; u < TW; u++)
+      {
+        const int t = wave + NC * u;
+        pre[u] = (t < NM && jok) ? mb[(size_t) t * ms + (size_t) jb * 16] : 0.0;
+      }
+  };
+  auto store_mom = [&] (int buf) {
+#pragma unroll
+    for (int u = 0; u < TW; u++)
+      {
+        const int t = wave + NC * u;
+        if (t < NM) mom[buf][t * 64 + lane] = pre[u];
+      }
+  };
+  double bzn[D];
+  auto load_z = [&] (int jb) {
+    const int jz = jb + kq, b = cbase + ir;
+    const bool ok = jz < j1 && b < B;
+    const double *src = XT + (size_t) (ok ? jz : 0) * zs + (ok ? b : 0);
+#pragma unroll
+    for (int q = 0; q < D; q++) bzn[q] = ok ? src[(size_t) q * B] : 0.0;
+  };
+  md4 acc[D];
+#pragma unroll
+  for (int t = 0; t < D; t++) acc[t] = md4{0.0, 0.0, 0.0, 0.0};
+  load_mom (j0);
+  load_z (j0);
+  store_mom (0);
+  __syncthreads ();
+  int buf = 0;
+  for (int jb = j0; jb < j1; jb += 4, buf ^= 1)
+    {
+      const bool more = jb + 4 < j1;
+      double bz[D];
+#pragma unroll
+      for (int q = 0; q < D; q++) bz[q] = bzn[q];
+      if (more)
+        {
+          load_mom (jb + 4);
+          load_z (jb + 4);
+        }
+      struct LdsM {
+        const double *b;
+        __device__ double operator[] (int t) const { return b[64 * t]; }
+      } Ml{mom[buf] + lane};
+#pragma unroll
+      for (int q = 0; q < D; q++)
+#pragma unroll
+        for (int r = 0; r < D; r++)
+          acc[r] = mfma4d (block_entry<V, double> (Ml, r, q, false), bz[q], acc[r]);
+      if (more) store_mom (buf ^ 1);
+      __syncthreads ();
+    }
+#pragma unroll
+  for (int r = 0; r < D; r++)
+#pragma unroll
+    for (int e = 0; e < 4; e++)
+      {
+        const int i = ibase + kq + 4 * e, b = cbase + ir;
+        if (i < np && b < B) partial[(((size_t) blockIdx.y * np + i) * D + r) * B + b] = acc[r][e];
       }
 }
 

@@ -847,10 +847,19 @@ static int atimes_block_vt (shk_dev *d, const double *x, double *y, int B, bool 
   // NMOM npad^2 floats, used up to 2 GiB
   const bool mom = wg && !p.poly && d->tsym
                    && (double) Dim<V>::NMOM * npad * npad * 4.0 <= 2.0 * 1024 * 1024 * 1024;
-  if (MF)
+  // the same for FP64 blocks of 16 columns or more: moments in FP64, then
+  // NC64 waves of 16 columns per workgroup on the FP64 matrix cores
+  // (k_real_mfma_mom64); momt holds NMOM npm^2 doubles, used up to 2 GiB
+  const int npm = (p.np + 63) & ~63;
+  const int NC64 = B > 64 ? 8 : (B > 32 ? 4 : 2);
+  const int ncw64 = (B + 16 * NC64 - 1) / (16 * NC64);
+  const bool mom64 = !MF && V == 2 && B >= 16 && !p.poly && d->tsym && p.periodic
+                     && d->ncand <= REAL_PC_CAND
+                     && (double) Dim<V>::NMOM * npm * npm * 8.0 <= 2.0 * 1024 * 1024 * 1024;
+  if (MF || mom64)
     {
       // about 2048 waves, or two workgroups per CU of the producer/consumer form
-      const int wpb = mom ? NWG * ncw : (wg ? 4 * ncw : ncg);
+      const int wpb = mom64 ? NC64 * ncw64 : (mom ? NWG * ncw : (wg ? 4 * ncw : ncg));
       nj = (2048 + nib * wpb - 1) / (nib * wpb);
       if (nj > (p.np + 15) / 16) nj = (p.np + 15) / 16;
       if (nj < 1) nj = 1;
@@ -957,13 +966,37 @@ static int atimes_block_vt (shk_dev *d, const double *x, double *y, int B, bool 
       nj = 1;
         }
     }
+  else if (mom64)
+    {
+      const size_t mbytes = sizeof (double) * Dim<V>::NMOM * (size_t) npm * npm;
+      if ((rc = ensure (d, d->momt, mbytes))) return rc;
+      double *mt = (double *) d->momt.p;
+      const int nt = npm / 16;
+      hipLaunchKernelGGL ((k_real_moments<V, double>), dim3 (nt, nt), dim3 (256), 0, d->stream,
+                          pos4, (const double4 *) d->cand.p, (const int4 *) d->candm.p,
+                          (const unsigned *) d->tbits.p, rp, npm, mt);
+      HIPCHK (hipGetLastError ());
+      const dim3 g (nib, nj, ncw64);
+      double *pd = (double *) d->partial.p;
+      const double *xd = (const double *) XT;
+      const int zd = D * B;
+      if (NC64 == 8)
+        hipLaunchKernelGGL ((k_real_mfma_mom64<V, 8>), g, dim3 (512), 0, d->stream,
+                            (const double *) mt, npm, xd, B, zd, p.np, jper, pd);
+      else if (NC64 == 4)
+        hipLaunchKernelGGL ((k_real_mfma_mom64<V, 4>), g, dim3 (256), 0, d->stream,
+                            (const double *) mt, npm, xd, B, zd, p.np, jper, pd);
+      else
+        hipLaunchKernelGGL ((k_real_mfma_mom64<V, 2>), g, dim3 (128), 0, d->stream,
+                            (const double *) mt, npm, xd, B, zd, p.np, jper, pd);
+    }
   else if (mom)
     {
       const size_t mbytes = sizeof (float) * Dim<V>::NMOM * (size_t) npad * npad;
       if ((rc = ensure (d, d->momt, mbytes))) return rc;
       float *mt = (float *) d->momt.p;
       const int nt = npad / 16;
-      hipLaunchKernelGGL ((k_real_moments<V>), dim3 (nt, nt), dim3 (256), 0, d->stream, pos4,
+      hipLaunchKernelGGL ((k_real_moments<V, float>), dim3 (nt, nt), dim3 (256), 0, d->stream, pos4,
                           (const float4 *) d->candf.p, (const int4 *) d->candm.p,
                           (const unsigned *) d->tbits.p, rp, npad, mt);
       HIPCHK (hipGetLastError ());

@@ -202,6 +202,34 @@ def test_block_rhs(version, poly, nrhs):
     s.close()
 
 
+@pytest.mark.parametrize("nrhs", [16, 40, 130])
+def test_block_pair_moments(nrhs):
+    """equal spheres in an unsheared box (symmetric image table): the block
+    path through the per-pair moments (each pair i <= j evaluated once, the
+    other order by parity) and the matrix cores, FP64 (v_mfma_f64) and
+    MIXED, every sampled column against the oracle's single matvec"""
+    rng = np.random.default_rng(70 + nrhs)
+    n = 200
+    pos, L = workloads.configuration(n, 0.2, seed=71)
+    X = rng.standard_normal((nrhs, n * 11))
+    s = ls.Stokes(2, pos, box=(L,) * 3, tr=20.0)
+    o = po.System(2, pos, box=(L,) * 3, tr=20.0)
+    xd = ls.DeviceArray.from_host(X)
+    yd = ls.DeviceArray(nrhs * n * 11)
+    s.atimes_dev(xd, yd, nrhs=nrhs)
+    Y = yd.to_host().reshape(nrhs, -1)
+    s.set_precision(ls.MIXED)
+    s.atimes_dev(xd, yd, nrhs=nrhs)
+    Ym = yd.to_host().reshape(nrhs, -1)
+    for c in sorted({0, 15, nrhs // 2, nrhs - 1}):
+        yo = o.atimes(X[c])
+        assert relerr(Y[c], yo) <= TOL64, c
+        assert relerr(Ym[c], yo) <= TOLMIX, c
+    xd.free()
+    yd.free()
+    s.close()
+
+
 @pytest.mark.parametrize("tr,B,dense", [(3000.0, 32, False), (300.0, 7, False),
                                         (300.0, 128, False), (300.0, 128, True),
                                         (1000.0, 200, True)])
