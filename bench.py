@@ -414,61 +414,118 @@ def _poly_box(args):
     return workloads.configuration_poly(n, 0.25, seed=seed)
 
 
-def ensemble_boxes(nbox, log):
+def ensemble_boxes(nbox, log, workers=8):
     """the ensemble's configurations: N log-uniform on [50, 2000], radii
     log-uniform on [0.5, 2], phi = 0.25 (generated by worker processes
-    before this process first touches the GPU)"""
+    before this process first touches the GPU; workers 0: in this process)"""
     import multiprocessing as mp
     rng = np.random.default_rng(0)
     ns = np.exp(rng.uniform(math.log(50), math.log(2000), nbox)).astype(int)
     t0 = time.perf_counter()
-    with mp.get_context("spawn").Pool(8) as pool:
-        boxes = pool.map(_poly_box, [(int(n), 1000 + k) for k, n in
-                                     enumerate(ns)])
+    work = [(int(n), 1000 + k) for k, n in enumerate(ns)]
+    if workers > 0:
+        with mp.get_context("spawn").Pool(workers) as pool:
+            boxes = pool.map(_poly_box, work)
+    else:
+        boxes = [_poly_box(w) for w in work]
     log(f"ensemble: {nbox} configurations in {time.perf_counter() - t0:.1f} s")
     return boxes
 
 
+def pair_images_poly(pos, L, rmax):
+    """exact count of (i, j, image) with 0 < r <= rmax in a cubic periodic
+    box of side L (every lattice image: the reference's table holds all
+    cells within reach for an unsheared cubic box)"""
+    from scipy.spatial import cKDTree
+    n = int(np.ceil(rmax / L))
+    shifts = np.array([(a, b, c) for a in range(-n, n + 1)
+                       for b in range(-n, n + 1)
+                       for c in range(-n, n + 1)], float) * L
+    cloud = (pos[None, :, :] + shifts[:, None, :]).reshape(-1, 3)
+    lo, hi = pos.min(0) - rmax, pos.max(0) + rmax
+    keep = np.all((cloud >= lo) & (cloud <= hi), axis=1)
+    t_pos, t_cloud = cKDTree(pos), cKDTree(cloud[keep])
+    # pairs at distance 0 are each particle with itself (primary image)
+    return int(t_pos.count_neighbors(t_cloud, rmax)) - len(pos)
+
+
 def bench_ensemble(args, boxes, log):
-    """Batched ensemble: nbox independent FTS boxes (ensemble_boxes), MIXED
-    precision, one graph launch per ensemble product (each box at the
-    splitting its own autotuned product was fastest at, results those of
-    the box's own tr = 1 splitting)"""
+    """Batched ensemble (C4): nbox independent FTS boxes (ensemble_boxes),
+    MIXED mobility product and the FP64 lubrication correction (lubmax 3 in
+    units of the mean radius: gaps below one radius, series to order 100)
+    in one graph launch of batched kernels, every box at the splitting and
+    the kernels at the tile shape the ensemble autotuner chose (results
+    those of each box's own tr = 1 splitting)"""
     nbox = len(boxes)
     systems = []
     for pos, a, L in boxes:
         s = ls.Stokes(2, pos, box=(L, L, L), tr=1.0, a=a, precision=ls.MIXED)
-        s.set_split(1000.0)
+        s.set_lub(4.0 + 1e-12, 3.0, 100, 1)
         systems.append(s)
     ens = ls.Ensemble(systems)
     t0 = time.perf_counter()
     trs = ens.autotune(trs=[10.0, 30.0, 100.0, 300.0, 1000.0, 3000.0])
     vals, cnt = np.unique(trs, return_counts=True)
     tuned = {f"{v:g}": int(c) for v, c in zip(vals, cnt)}
-    log(f"ensemble: per-box splittings tuned in "
-        f"{time.perf_counter() - t0:.1f} s: {tuned}")
+    tiles = ens.tiles()
+    log(f"ensemble: splittings and tiles tuned in "
+        f"{time.perf_counter() - t0:.1f} s: {tuned}, tiles {tiles}")
     tot = ens.offsets[-1]
     posd = ls.DeviceArray.from_host(
         np.concatenate([b[0].reshape(-1) for b in boxes]))
     xd = ls.DeviceArray.from_host(
         np.random.default_rng(1).standard_normal(11 * tot))
     yd = ls.DeviceArray(11 * tot)
-    t0 = time.perf_counter()
-    ens.atimes_dev(posd, xd, yd)          # capture + first launch
-    log(f"ensemble: capture + first launch {time.perf_counter() - t0:.2f} s")
+    yl = ls.DeviceArray(11 * tot)
     reps = max(3, args.steps // 4)
+
+    def timed(fn):
+        fn()                               # capture + first launch
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        return (time.perf_counter() - t0) / reps
+
+    t_mob = timed(lambda: ens.atimes_dev(posd, xd, yd))
+    t_all = timed(lambda: ens.atimes_lub_dev(posd, xd, yd, yl))
+    log(f"ensemble: {t_all * 1e3:.2f} ms per launch with lubrication, "
+        f"{t_mob * 1e3:.2f} ms mobility only")
+    ms = np.min([ens.phase_ms(posd, xd, yd, yl) for _ in range(3)], axis=0)
+    # algorithmic work of the real-space phase: every pair image inside
+    # each box's own r_max at its tuned splitting
     t0 = time.perf_counter()
-    for _ in range(reps):
-        ens.atimes_dev(posd, xd, yd)
-    t = (time.perf_counter() - t0) / reps
-    log(f"ensemble: {t * 1e3:.2f} ms per launch")
+    pin = 0
+    for (pos, a, L), tr in zip(boxes, trs):
+        sh = ls.Stokes(2, pos, box=(L, L, L), tr=float(tr) if tr > 0 else 1.0,
+                       a=a)
+        pin += pair_images_poly(pos, L, math.sqrt(sh.s.rmax2))
+        sh.close()
+    log(f"ensemble: {pin} pair images counted in "
+        f"{time.perf_counter() - t0:.1f} s")
+    peak_valu = PEAK_FP64          # FP32 FMA without packed math, per lane
+    achieved = pin * FLOP_PER_IMAGE_FTS / (ms[0] * 1e-3) / 1e12
     out = {"workload": f"{nbox} FTS boxes, N log-uniform [50, 2000] "
                        f"(sum {tot}), radii log-uniform [0.5, 2], phi 0.25, "
-                       f"MIXED, one graph launch",
-           "nbox": nbox, "sum_n": int(tot), "ms_per_launch": t * 1e3,
+                       f"MIXED mobility + FP64 lubrication (lubmax 3), one "
+                       f"graph launch of batched kernels",
+           "nbox": nbox, "sum_n": int(tot), "ms_per_launch": t_all * 1e3,
+           "ms_per_launch_mobility_only": t_mob * 1e3,
+           "phase_ms_eager": {"real": ms[0], "reciprocal": ms[1],
+                              "combine": ms[2], "lubrication": ms[3]},
            "tuned_tr_counts": tuned,
-           "box_matvecs_per_s": nbox / t,
-           "particle_matvecs_per_s": tot / t}
+           "tiles": {"chunks_per_wave": tiles[0],
+                     "particles_per_sf_split": tiles[1],
+                     "k_per_back_split": tiles[2]},
+           "box_products_per_s": nbox / t_all,
+           "particle_products_per_s": tot / t_all,
+           "roofline": {"bound": "valu", "achieved": achieved,
+                        "peak": peak_valu, "unit": "TFLOP/s",
+                        "frac": achieved / peak_valu, "traffic": None,
+                        "kernel": "k_ens_real<2, float> (FP32 pair scalars "
+                                  "and contraction, all boxes in one launch;"
+                                  " peak: FP32 FMA without packed math)",
+                        "kernel_ms": ms[0], "pair_images": pin,
+                        "flop_per_pair_image": FLOP_PER_IMAGE_FTS}}
     ens.close()
     for s in systems:
         s.close()
@@ -546,6 +603,43 @@ def profile_bd_run(args):
         s.close()
 
 
+def profile_ens_run(args):
+    """short run for rocprofv3: the ensemble workload (--ens-boxes boxes,
+    MIXED mobility + FP64 lubrication) at one splitting for every box (--tr,
+    default 300) and the tile shape (4, 1024, 256): one graph launch, then
+    --steps eager launches on one stream (kernels one after another)"""
+    log = lambda m: print(m, file=sys.stderr, flush=True)
+    boxes = ensemble_boxes(args.ens_boxes, log, workers=0)
+    systems = []
+    for pos, a, L in boxes:
+        s = ls.Stokes(2, pos, box=(L, L, L), tr=1.0, a=a, precision=ls.MIXED)
+        s.set_lub(4.0 + 1e-12, 3.0, 100, 1)
+        systems.append(s)
+    ens = ls.Ensemble(systems)
+    tr = args.tr if args.tr else 300.0
+    ens.set_split(tr)
+    ens.set_tiles(4, 1024, 256)
+    tot = ens.offsets[-1]
+    posd = ls.DeviceArray.from_host(
+        np.concatenate([b[0].reshape(-1) for b in boxes]))
+    xd = ls.DeviceArray.from_host(
+        np.random.default_rng(1).standard_normal(11 * tot))
+    yd = ls.DeviceArray(11 * tot)
+    yl = ls.DeviceArray(11 * tot)
+    ens.atimes_lub_dev(posd, xd, yd, yl)
+    # eager launches on one stream (each kernel's duration its own)
+    ms = np.min([ens.phase_ms(posd, xd, yd, yl) for _ in range(args.steps)],
+                axis=0)
+    print(json.dumps({"ensemble_boxes": len(boxes), "sum_n": int(tot),
+                      "tr": tr, "tiles": [4, 1024, 256],
+                      "phase_ms": {"real": ms[0], "reciprocal": ms[1],
+                                   "combine": ms[2], "lubrication": ms[3]},
+                      "steps": args.steps}), flush=True)
+    ens.close()
+    for s in systems:
+        s.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -572,7 +666,12 @@ def main():
                     help="short run for rocprofv3: --steps BD steps of each "
                          "precision at --tr, one cached 64-column Chebyshev "
                          "block of each")
+    ap.add_argument("--profile-ens", action="store_true",
+                    help="short run for rocprofv3: --steps launches of the "
+                         "ensemble (mobility + lubrication) at --tr")
     args = ap.parse_args()
+    if args.profile_ens:
+        return profile_ens_run(args)
     if args.profile:
         return profile_run(args)
     if args.profile_bd:

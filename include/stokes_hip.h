@@ -192,27 +192,64 @@ int stokes_hip_lanczos_bounds (struct stokes *sys, int niter,
                                double *lmin, double *lmax);
 
 /* ---- ensembles of independent systems -----------------------------------
- * The operator of nbox systems (uneven np, radii, boxes, precisions) as one
- * graph launch: every box runs on one of a pool of streams forked from and
- * joined back to the launch stream.  Box b reads positions at
- * pos_dev + 3 off_b and x, y at x_dev / y_dev + dim_b off_b, off_b the sum
- * of np over the boxes before b (stokes_hip_ensemble_offset).  Positions
- * are re-read on every launch; the other parameters of the systems are
- * those at the first launch until stokes_hip_ensemble_refresh().  Calls on
- * the member systems must not overlap an ensemble launch.  No counterpart
- * in the reference. */
+ * The operator of nbox periodic systems (uneven np, radii and box length;
+ * one version, all with radii or all without) as one graph launch of
+ * batched kernels: each phase of the operator (real-space pair sum, phase
+ * tables, structure factors, k-blocks, back-projection, combine) and of the
+ * FP64 lubrication correction (near-pair count, scan, fill, evaluate,
+ * gather) is one kernel over the work items of every box.  Box b reads
+ * positions at pos_dev + 3 off_b and x, y at x_dev / y_dev + dim off_b,
+ * off_b the sum of np over the boxes before b (stokes_hip_ensemble_offset).
+ * Positions are re-read on every launch; the other parameters of the
+ * systems (box, radii, Ewald parameters, lubrication parameters) are those
+ * of the first launch until stokes_hip_ensemble_refresh().  A member
+ * system's splitting (stokes_hip_set_split), G-sign fix and, for box 0, its
+ * precision are taken at creation.  Each box's result is that of its own
+ * product to the precision's bar; calls on the member systems must not
+ * overlap an ensemble launch.  No counterpart in the reference. */
 typedef struct stokes_hip_ensemble stokes_hip_ensemble;
 int stokes_hip_ensemble_create (struct stokes *const *sys, int nbox,
                                 stokes_hip_ensemble **out);
+/* y = M x for every box (the precision of the ensemble) */
 int stokes_hip_ensemble_atimes (stokes_hip_ensemble *e, const double *pos_dev,
                                 const double *x_dev, double *y_dev);
+/* ylub = L x, the lubrication correction of calc_lub_3f / 3ft / 3fts
+ * (lub.c:164-768, FP64) for every box with its own lubmin2, lubmax,
+ * twobody_nmax and twobody_lub */
+int stokes_hip_ensemble_lub (stokes_hip_ensemble *e, const double *pos_dev,
+                             const double *x_dev, double *ylub_dev);
+/* both in one launch */
+int stokes_hip_ensemble_atimes_lub (stokes_hip_ensemble *e, const double *pos_dev,
+                                    const double *x_dev, double *y_dev,
+                                    double *ylub_dev);
 int stokes_hip_ensemble_refresh (stokes_hip_ensemble *e);
-/* the autotuner per box: stokes_hip_autotune of each member system (one
- * column, candidates trs[ntr] or the default list when NULL, reps timed
- * calls each), best_tr[nbox] (may be NULL) receives the splittings kept;
- * the next launch recaptures */
+/* STOKES_HIP_FP64 or STOKES_HIP_MIXED for every box */
+int stokes_hip_ensemble_set_precision (stokes_hip_ensemble *e, int mode);
+/* splitting tratio (0: the box's own xi) of box `box`, or of every box
+ * when box = -1; results stay those of the box's own xi */
+int stokes_hip_ensemble_set_split (stokes_hip_ensemble *e, int box, double tratio);
+/* tile shape of the batched kernels: jw 64-particle chunks per real-space
+ * wave, jper particles per structure-factor split, kper k vectors per
+ * back-projection split */
+int stokes_hip_ensemble_set_tiles (stokes_hip_ensemble *e, int jw, int jper, int kper);
+int stokes_hip_ensemble_get_tiles (const stokes_hip_ensemble *e, int *jw, int *jper,
+                                   int *kper);
+/* the autotuner: times the batch at every candidate splitting trs[ntr]
+ * (NULL: 10, 30, 100, 300, 1000, 3000) for all boxes alike, fits the
+ * real-space and reciprocal phase times to the boxes' work at each
+ * splitting, gives each box its cheapest splitting under that fit (kept
+ * when the whole launch beats the best common splitting), then sweeps the
+ * tile shape; best of reps timed launches each.  best_tr[nbox] (may be
+ * NULL) receives the splittings kept.  Uses the member systems' host
+ * positions. */
 int stokes_hip_ensemble_autotune (stokes_hip_ensemble *e, const double *trs,
                                   int ntr, int reps, double *best_tr);
+/* one eager launch on one stream with events between the phases (y_dev
+ * or ylub_dev may be NULL to skip that product): ms[0] real space, ms[1]
+ * reciprocal space, ms[2] combine, ms[3] lubrication (0 when skipped) */
+int stokes_hip_ensemble_phase_ms (stokes_hip_ensemble *e, const double *pos_dev,
+                                  const double *x_dev, double *y_dev, double *ylub_dev,
+                                  double *ms);
 size_t stokes_hip_ensemble_offset (const stokes_hip_ensemble *e, int box);
 void stokes_hip_ensemble_destroy (stokes_hip_ensemble *e);
 

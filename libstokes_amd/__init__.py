@@ -65,6 +65,10 @@ EXPORTS = [
     "shift_labo_to_rest_U", "shift_rest_to_labo_U", "shift_labo_to_rest_O",
     "shift_rest_to_labo_O", "shift_labo_to_rest_E",
     "stokes_hip_ensemble_create", "stokes_hip_ensemble_atimes",
+    "stokes_hip_ensemble_lub", "stokes_hip_ensemble_atimes_lub",
+    "stokes_hip_ensemble_set_precision", "stokes_hip_ensemble_set_split",
+    "stokes_hip_ensemble_set_tiles", "stokes_hip_ensemble_get_tiles",
+    "stokes_hip_ensemble_phase_ms",
     "stokes_hip_ensemble_refresh", "stokes_hip_ensemble_offset",
     "stokes_hip_ensemble_destroy",
     "stokes_hip_bd_create", "stokes_hip_bd_set_bounds",
@@ -257,6 +261,14 @@ def lib():
     L.stokes_hip_ensemble_create.argtypes = [C.POINTER(sp), C.c_int,
                                              C.POINTER(_vp)]
     L.stokes_hip_ensemble_atimes.argtypes = [_vp, _vp, _vp, _vp]
+    L.stokes_hip_ensemble_lub.argtypes = [_vp, _vp, _vp, _vp]
+    L.stokes_hip_ensemble_atimes_lub.argtypes = [_vp, _vp, _vp, _vp, _vp]
+    L.stokes_hip_ensemble_set_precision.argtypes = [_vp, C.c_int]
+    L.stokes_hip_ensemble_set_split.argtypes = [_vp, C.c_int, C.c_double]
+    L.stokes_hip_ensemble_set_tiles.argtypes = [_vp, C.c_int, C.c_int,
+                                                C.c_int]
+    L.stokes_hip_ensemble_get_tiles.argtypes = [_vp, _ip, _ip, _ip]
+    L.stokes_hip_ensemble_phase_ms.argtypes = [_vp, _vp, _vp, _vp, _vp, _dp]
     L.stokes_hip_ensemble_refresh.argtypes = [_vp]
     L.stokes_hip_ensemble_autotune.argtypes = [_vp, _dp, C.c_int, C.c_int,
                                                _dp]
@@ -615,11 +627,50 @@ class Ensemble:
         check(self.L.stokes_hip_ensemble_atimes(self.h, pos_dev.ptr, x_dev.ptr,
                                                 y_dev.ptr), "ensemble_atimes")
 
+    def lub_dev(self, pos_dev, x_dev, ylub_dev):
+        """the FP64 lubrication correction of every box"""
+        check(self.L.stokes_hip_ensemble_lub(self.h, pos_dev.ptr, x_dev.ptr,
+                                             ylub_dev.ptr), "ensemble_lub")
+
+    def atimes_lub_dev(self, pos_dev, x_dev, y_dev, ylub_dev):
+        """M x and the lubrication correction L x in one launch"""
+        check(self.L.stokes_hip_ensemble_atimes_lub(
+            self.h, pos_dev.ptr, x_dev.ptr, y_dev.ptr, ylub_dev.ptr),
+            "ensemble_atimes_lub")
+
+    def set_precision(self, mode):
+        check(self.L.stokes_hip_ensemble_set_precision(self.h, mode),
+              "ensemble_set_precision")
+
+    def set_split(self, tr, box=-1):
+        check(self.L.stokes_hip_ensemble_set_split(self.h, box, float(tr)),
+              "ensemble_set_split")
+
+    def set_tiles(self, jw, jper, kper):
+        check(self.L.stokes_hip_ensemble_set_tiles(self.h, jw, jper, kper),
+              "ensemble_set_tiles")
+
+    def tiles(self):
+        v = [C.c_int(), C.c_int(), C.c_int()]
+        check(self.L.stokes_hip_ensemble_get_tiles(self.h, *[C.byref(t)
+                                                             for t in v]),
+              "ensemble_get_tiles")
+        return tuple(t.value for t in v)
+
+    def phase_ms(self, pos_dev, x_dev, y_dev, ylub_dev=None):
+        """(real, reciprocal, combine, lubrication) ms of one eager launch
+        (y_dev or ylub_dev None: that product skipped)"""
+        ms = np.zeros(4)
+        check(self.L.stokes_hip_ensemble_phase_ms(
+            self.h, pos_dev.ptr, x_dev.ptr, y_dev.ptr if y_dev else None,
+            ylub_dev.ptr if ylub_dev else None, _p(ms)), "ensemble_phase_ms")
+        return ms
+
     def refresh(self):
         check(self.L.stokes_hip_ensemble_refresh(self.h), "ensemble_refresh")
 
     def autotune(self, trs=None, reps=3):
-        """per-box splitting autotuner; returns the splittings kept"""
+        """splittings per box and tile shape; returns the splittings kept"""
         nb = len(self.systems)
         best = np.zeros(nb)
         if trs is None:

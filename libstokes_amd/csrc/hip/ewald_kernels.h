@@ -238,6 +238,33 @@ __device__ __forceinline__ void kblock_apply_s (double ex, double ey, double ez,
   contract_item<V, double> (ex, ey, ez, ss, C, Q);
 }
 
+// kblock_apply_s for scalar sets whose possibly nonzero entries are the
+// bits of MC (sc) and MS (ss): the contractions skip the zero scalars
+template <int V, unsigned MC, unsigned MS>
+__device__ __forceinline__ void kblock_apply_m (double ex, double ey, double ez, const double *sc,
+                                                const double *ss, const double *C,
+                                                const double *S, double *P, double *Q)
+{
+  double nss[11];
+#pragma unroll
+  for (int t = 0; t < 11; t++) nss[t] = -ss[t];
+  if (MC) contract_item_m<V, double, MC> (ex, ey, ez, sc, C, P);
+  if (MS) contract_item_m<V, double, MS> (ex, ey, ez, nss, S, P);
+  if (MC) contract_item_m<V, double, MC> (ex, ey, ez, sc, S, Q);
+  if (MS) contract_item_m<V, double, MS> (ex, ey, ez, ss, C, Q);
+}
+
+// the nonzero scalars of the reciprocal blocks: equal spheres ya, yc, yh,
+// ym (even) and yb, yg (odd); unequal spheres per power of a_i
+// (poly_kscalars): sc1, ss1, sc2, ss2
+constexpr unsigned KM_MONO_C = 0x292u, KM_MONO_S = 0x44u;
+template <int P> struct KMask;
+template <> struct KMask<1> { static constexpr unsigned c1 = 0x2u, s1 = 0u, c2 = 0x2u, s2 = 0u; };
+template <> struct KMask<2> { static constexpr unsigned c1 = 0u, s1 = 0x44u, c2 = 0u, s2 = 0x40u; };
+template <> struct KMask<3> { static constexpr unsigned c1 = 0x292u, s1 = 0u, c2 = 0x200u, s2 = 0u; };
+template <> struct KMask<4> { static constexpr unsigned c1 = 0u, s1 = 0x40u, c2 = 0u, s2 = 0u; };
+template <> struct KMask<5> { static constexpr unsigned c1 = 0x200u, s1 = 0u, c2 = 0u, s2 = 0u; };
+
 // P, Q (D each) of one k from its structure factors CS = [C | S] for equal
 // spheres: the paired-k weights w of the k table (ya yb yc yg yh ym)
 template <int V>
@@ -251,7 +278,7 @@ __device__ __forceinline__ void kblock_pq_mono (const double4 kvec, const double
   const double ss[11] = {0.0, 0.0, w[1], 0.0, 0.0, 0.0, w[3], 0.0, 0.0, 0.0, 0.0};
 #pragma unroll
   for (int q = 0; q < D; q++) P[q] = Q[q] = 0.0;
-  kblock_apply_s<V> (ex, ey, ez, sc, ss, CS, CS + D, P, Q);
+  kblock_apply_m<V, KM_MONO_C, KM_MONO_S> (ex, ey, ez, sc, ss, CS, CS + D, P, Q);
 }
 
 // P, Q of power PW (1..5) of a_i for unequal spheres: CS = [C | S | C2 | S2]
@@ -268,8 +295,9 @@ __device__ __forceinline__ void kblock_pq_poly (const double4 kvec, const double
   poly_kscalars<PW> (kvec.w, w[6], w[7], w[3], sc1, ss1, sc2, ss2);
 #pragma unroll
   for (int q = 0; q < D; q++) P[q] = Q[q] = 0.0;
-  kblock_apply_s<V> (ex, ey, ez, sc1, ss1, CS, CS + D, P, Q);
-  kblock_apply_s<V> (ex, ey, ez, sc2, ss2, CS + 2 * D, CS + 3 * D, P, Q);
+  kblock_apply_m<V, KMask<PW>::c1, KMask<PW>::s1> (ex, ey, ez, sc1, ss1, CS, CS + D, P, Q);
+  kblock_apply_m<V, KMask<PW>::c2, KMask<PW>::s2> (ex, ey, ez, sc2, ss2, CS + 2 * D, CS + 3 * D,
+                                                    P, Q);
 }
 
 // the same for a power known only at run time (1..5)
@@ -364,7 +392,7 @@ k_matrix (const double4 *__restrict__ pos, const double4 *__restrict__ img,
 }
 
 // y[row] = sum_col mat[row][col] * z[col], one wave per row
-__global__ __launch_bounds__ (256) void
+static __global__ __launch_bounds__ (256) void
 k_gemv_rows (const double *__restrict__ mat, const double *__restrict__ z,
              size_t n, double *__restrict__ y)
 {

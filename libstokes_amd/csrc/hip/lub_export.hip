@@ -25,6 +25,10 @@ struct HostF {
     p[ZM] = f->ZM;
   }
   double operator() (int fam, int k) const { return p[fam][k]; }
+  void four (int fam, int k, double *f) const
+  {
+    for (int i = 0; i < 4; i++) f[i] = p[fam][k + i];
+  }
 };
 
 // the reference's n: made even, the series to m < n

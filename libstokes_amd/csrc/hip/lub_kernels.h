@@ -21,7 +21,20 @@ struct LubGeom {
   double s2x, s3x;          // shear: x shift per y-image / z-image
   int ix, iy, iz;           // image range
   double lubmax2;           // 0: no limit
+  int minimg;               // the cut-off is at most half of every box length
+                            // (unsheared): only the nearest image can be near
 };
+
+// the nearest-image search applies: periodic and unsheared, a cut-off
+// (lubmax, in r or in s = 2 r / (a_i + a_j) units: at most lubmax a_max in
+// r) of at most half of every box length
+inline int lub_minimg (const LubGeom &g, double lubmax, double amax)
+{
+  if (!g.periodic || g.s2x != 0.0 || g.s3x != 0.0 || !(lubmax > 0.0) || !(amax > 0.0)) return 0;
+  const double rcut = g.poly ? lubmax * amax : lubmax;
+  const double lmin = g.lx < g.ly ? (g.lx < g.lz ? g.lx : g.lz) : (g.ly < g.lz ? g.ly : g.lz);
+  return rcut <= 0.5 * lmin ? 1 : 0;
+}
 
 // image cell code: 5 bits per direction, offset 15
 SHK_HD int lub_img_code (int a, int b, int c) { return (a + 15) | ((b + 15) << 5) | ((c + 15) << 10); }
@@ -39,17 +52,45 @@ __device__ inline bool lub_hit (const LubGeom &g, const double4 &pi, const doubl
 }
 
 // one wave per receiving particle i, lanes over j; FILL = 0 counts, 1
-// writes the (j, cell) records from off[i] in (j chunk, cell, lane) order
+// writes the (j, cell) records from off[i] in (j chunk, cell, lane) order,
+// those with an index below cap
 template <int FILL>
-__global__ __launch_bounds__ (256) void
-k_lub_pairs (const double4 *__restrict__ pos, LubGeom g, int *__restrict__ cnt,
-             const int *__restrict__ off, int2 *__restrict__ list)
+__device__ __forceinline__ void
+lub_pairs_wave (int i, int lane, const double4 *__restrict__ pos, const LubGeom &g,
+                int *__restrict__ cnt, const int *__restrict__ off, int2 *__restrict__ list,
+                int cap)
 {
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int i = blockIdx.x * 4 + wave;
   if (i >= g.np) return;
   const double4 pi = pos[i];
   int base = FILL ? off[i] : 0;
+  if (g.minimg)
+    {
+      // every image but the nearest is at least half a box length away in
+      // one direction, beyond the cut-off: one test per pair, the nearest
+      // image when it is inside the enumerated cells
+      for (int j0 = 0; j0 < g.np; j0 += 64)
+        {
+          const int j = j0 + lane;
+          const bool valid = j < g.np;
+          const double4 pj = pos[valid ? j : i];
+          const int a = -(int) rint ((pj.x - pi.x) / g.lx);
+          const int b = -(int) rint ((pj.y - pi.y) / g.ly);
+          const int c = -(int) rint ((pj.z - pi.z) / g.lz);
+          double d[3];
+          const bool hit = valid && abs (a) <= g.ix && abs (b) <= g.iy && abs (c) <= g.iz
+            && lub_hit (g, pi, pj, a, b, c, d);
+          const unsigned long long m = __ballot (hit);
+          if (FILL && hit)
+            {
+              const unsigned long long below = lane ? (m & ((~0ull) >> (64 - lane))) : 0ull;
+              const int at = base + __popcll (below);
+              if (at < cap) list[at] = make_int2 (j, lub_img_code (a, b, c));
+            }
+          base += __popcll (m);
+        }
+      if (!FILL && lane == 0) cnt[i] = base;
+      return;
+    }
   for (int j0 = 0; j0 < g.np; j0 += 64)
     {
       const int j = j0 + lane;
@@ -65,7 +106,8 @@ k_lub_pairs (const double4 *__restrict__ pos, LubGeom g, int *__restrict__ cnt,
               if (FILL && hit)
                 {
                   const unsigned long long below = lane ? (m & ((~0ull) >> (64 - lane))) : 0ull;
-                  list[base + __popcll (below)] = make_int2 (j, lub_img_code (a, b, c));
+                  const int at = base + __popcll (below);
+                  if (at < cap) list[at] = make_int2 (j, lub_img_code (a, b, c));
                 }
               base += __popcll (m);
             }
@@ -73,8 +115,17 @@ k_lub_pairs (const double4 *__restrict__ pos, LubGeom g, int *__restrict__ cnt,
   if (!FILL && lane == 0) cnt[i] = base;
 }
 
+template <int FILL>
+__global__ __launch_bounds__ (256) void
+k_lub_pairs (const double4 *__restrict__ pos, LubGeom g, int *__restrict__ cnt,
+             const int *__restrict__ off, int2 *__restrict__ list)
+{
+  lub_pairs_wave<FILL> (blockIdx.x * 4 + (threadIdx.x >> 6), threadIdx.x & 63, pos, g, cnt, off,
+                        list, 0x7fffffff);
+}
+
 // exclusive scan of cnt[0..n) into off[0..n] (one workgroup; n moderate)
-__global__ __launch_bounds__ (1024) void
+static __global__ __launch_bounds__ (1024) void
 k_lub_scan (const int *__restrict__ cnt, int n, int *__restrict__ off)
 {
   __shared__ int part[1024];
@@ -106,6 +157,10 @@ k_lub_scan (const int *__restrict__ cnt, int n, int *__restrict__ off)
 }
 
 // f values of the device coefficient table at one lambda: f_k = sum_q c_q l^q
+// by Horner's rule from q = k + 1 down; the loop takes eight coefficients
+// per step once the remaining count is a multiple of eight, so that the
+// loads of a step can be issued together (same order of operations as the
+// plain loop)
 struct DevPolyF {
   const double *c;   // [fam][k (N + 2) + q]
   int N;
@@ -114,8 +169,85 @@ struct DevPolyF {
   {
     const double *p = c + ((size_t) fam * (N + 1) + k) * (N + 2);
     double v = 0.0;
-    for (int q = k + 1; q >= 0; q--) v = v * l + p[q];
+    int q = k + 1;
+    for (; (q + 1) & 7; q--) v = v * l + p[q];
+    if (q < 7) return v;
+    // the next step's eight coefficients are loaded before this step's
+    // products (the prefetch index stays within the row: q - 8 >= 7 or 7)
+    double n7 = p[q], n6 = p[q - 1], n5 = p[q - 2], n4 = p[q - 3];
+    double n3 = p[q - 4], n2 = p[q - 5], n1 = p[q - 6], n0 = p[q - 7];
+    for (; q >= 7; q -= 8)
+      {
+        const double c7 = n7, c6 = n6, c5 = n5, c4 = n4, c3 = n3, c2 = n2, c1 = n1, c0 = n0;
+        const int qn = q - 8 >= 7 ? q - 8 : 7;
+        n7 = p[qn]; n6 = p[qn - 1]; n5 = p[qn - 2]; n4 = p[qn - 3];
+        n3 = p[qn - 4]; n2 = p[qn - 5]; n1 = p[qn - 6]; n0 = p[qn - 7];
+        v = v * l + c7;
+        v = v * l + c6;
+        v = v * l + c5;
+        v = v * l + c4;
+        v = v * l + c3;
+        v = v * l + c2;
+        v = v * l + c1;
+        v = v * l + c0;
+      }
     return v;
+  }
+  // f_k .. f_k+3 together (k + 3 <= N): four Horner recurrences side by
+  // side from the top degree of row k + 3, the shorter rows reading their
+  // zero coefficients above degree k + 1 (0 l + 0 = 0 exactly, so each
+  // value is the one operator() gives)
+  __device__ void four (int fam, int k, double *f) const
+  {
+    const size_t R = (size_t) (N + 2);
+    horner4 (c + ((size_t) fam * (N + 1) + k) * R, R, k + 4, l, f);
+  }
+  // rows p0, p0 + R, p0 + 2R, p0 + 3R from degree qtop down
+  __device__ static void horner4 (const double *p0, size_t R, int qtop, double l, double *f)
+  {
+    const double *p1 = p0 + R, *p2 = p1 + R, *p3 = p2 + R;
+    double v0 = 0.0, v1 = 0.0, v2 = 0.0, v3 = 0.0;
+    int q = qtop;
+    for (; (q + 1) & 3; q--)
+      {
+        v0 = v0 * l + p0[q]; v1 = v1 * l + p1[q];
+        v2 = v2 * l + p2[q]; v3 = v3 * l + p3[q];
+      }
+    for (; q >= 3; q -= 4)
+      {
+        const double a0 = p0[q], a1 = p0[q - 1], a2 = p0[q - 2], a3 = p0[q - 3];
+        const double b0 = p1[q], b1 = p1[q - 1], b2 = p1[q - 2], b3 = p1[q - 3];
+        const double c0 = p2[q], c1 = p2[q - 1], c2 = p2[q - 2], c3 = p2[q - 3];
+        const double d0 = p3[q], d1 = p3[q - 1], d2 = p3[q - 2], d3 = p3[q - 3];
+        v0 = v0 * l + a0; v1 = v1 * l + b0; v2 = v2 * l + c0; v3 = v3 * l + d0;
+        v0 = v0 * l + a1; v1 = v1 * l + b1; v2 = v2 * l + c1; v3 = v3 * l + d1;
+        v0 = v0 * l + a2; v1 = v1 * l + b2; v2 = v2 * l + c2; v3 = v3 * l + d2;
+        v0 = v0 * l + a3; v1 = v1 * l + b3; v2 = v2 * l + c3; v3 = v3 * l + d3;
+      }
+    f[0] = v0; f[1] = v1; f[2] = v2; f[3] = v3;
+  }
+};
+
+// The same values with the four rows of each step staged in LDS by the whole
+// workgroup: every thread of the block must make the same sequence of
+// four() calls (uniform control flow; a thread without a pair of its own
+// evaluates a neighbour's), so the rows are read from memory once per block
+// and each thread's recurrences read them back as broadcasts.  sh holds
+// 4 (N + 2) doubles.
+struct LdsPolyF {
+  const double *c;
+  int N;
+  double l;
+  double *sh;
+  __device__ double operator() (int fam, int k) const { return DevPolyF{c, N, l} (fam, k); }
+  __device__ void four (int fam, int k, double *f) const
+  {
+    const int R = N + 2;
+    const double *g = c + ((size_t) fam * (N + 1) + k) * R;   // rows k .. k + 3, contiguous
+    __syncthreads ();
+    for (int t = threadIdx.x; t < 4 * R; t += blockDim.x) sh[t] = g[t];
+    __syncthreads ();
+    DevPolyF::horner4 (sh, (size_t) R, k + 4, l, f);
   }
 };
 
@@ -123,6 +255,10 @@ struct DevPolyF {
 struct DevTabF {
   const double *t;
   __device__ double operator() (int fam, int k) const { return t[fam * 51 + k]; }
+  __device__ void four (int fam, int k, double *f) const
+  {
+    for (int i = 0; i < 4; i++) f[i] = t[fam * 51 + k + i];
+  }
 };
 
 struct LubEval {
@@ -135,12 +271,13 @@ struct LubEval {
 
 // the correction scalars of directed pair p (S11, S12 seen from the
 // receiving particle i), the unit vector e and the weight (2 for a periodic
-// self image)
+// self image).  sh != nullptr: the coefficient rows go through that LDS
+// buffer (LdsPolyF; block-uniform calls)
 template <int V>
 __device__ inline void lub_pair (const double4 *__restrict__ pos, const LubGeom &g,
                                  const LubEval &ev, const int *__restrict__ off,
                                  const int2 *__restrict__ list, int p, int &i, int &j, double *e,
-                                 double *s11, double *s12, double &w)
+                                 double *s11, double *s12, double &w, double *sh = nullptr)
 {
   // the receiving particle: the last i with off[i] <= p
   int lo = 0, hi = g.np;
@@ -183,9 +320,18 @@ __device__ inline void lub_pair (const double4 *__restrict__ pos, const LubGeom 
       if (r2 / rs < ev.lubmin2) r2 = rs * ev.lubmin2;
       const double r = sqrt (r2), l = aj / ai, s = 2.0 * r / (ai + aj);
       const int n = ev.nmax % 2 ? ev.nmax - 1 : ev.nmax;
-      const DevPolyF fv{ev.coef, ev.N, l};
-      if (ev.flag_lub) lub::lub_form (V, fv, n - 1, l, s, 1.0, two);
-      else lub::far_form (V, fv, n, l, s, two);
+      if (sh)
+        {
+          const LdsPolyF fv{ev.coef, ev.N, l, sh};
+          if (ev.flag_lub) lub::lub_form (V, fv, n - 1, l, s, 1.0, two);
+          else lub::far_form (V, fv, n, l, s, two);
+        }
+      else
+        {
+          const DevPolyF fv{ev.coef, ev.N, l};
+          if (ev.flag_lub) lub::lub_form (V, fv, n - 1, l, s, 1.0, two);
+          else lub::far_form (V, fv, n, l, s, two);
+        }
       lub::scale (V, two, ai, l, false);
       lub::minv (V, r, ai, aj, m);
       for (int f = 0; f < lub::nfam_of (V); f++)
@@ -204,24 +350,47 @@ __device__ inline void lub_pair (const double4 *__restrict__ pos, const LubGeom 
   w = (i == j) ? 2.0 : 1.0;
 }
 
+// directed pair p: its D values into val[p] (store false: evaluate only);
+// pos, off and x of the pair's own system (off[0..np], pair indices p in
+// [off[0], off[np])); sh: as for lub_pair
 template <int V>
-__global__ __launch_bounds__ (128) void
-k_lub_eval (const double4 *__restrict__ pos, LubGeom g, LubEval ev, const int *__restrict__ off,
-            const int2 *__restrict__ list, int npairs, const double *__restrict__ x,
-            double *__restrict__ val)
+__device__ __forceinline__ void
+lub_eval_item (int p, const double4 *__restrict__ pos, const LubGeom &g, const LubEval &ev,
+               const int *__restrict__ off, const int2 *__restrict__ list,
+               const double *__restrict__ x, double *__restrict__ val, double *sh = nullptr,
+               bool store = true)
 {
   constexpr int D = Dim<V>::D;
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= npairs) return;
   int i, j;
   double e[3], s11[11], s12[11], w;
-  lub_pair<V> (pos, g, ev, off, list, p, i, j, e, s11, s12, w);
+  lub_pair<V> (pos, g, ev, off, list, p, i, j, e, s11, s12, w, sh);
+  if (!store) return;
   double y[11];
   for (int k = 0; k < D; k++) y[k] = 0.0;
   lub::apply_block<D> (true, x + (size_t) D * i, y, e[0], e[1], e[2], s11);
   lub::apply_block<D> (false, x + (size_t) D * j, y, e[0], e[1], e[2], s12);
   double *o = val + (size_t) p * D;
   for (int k = 0; k < D; k++) o[k] = w * y[k];
+}
+
+// POLY: the kind of radii as a compile-time constant.  Unequal spheres
+// stage the coefficient rows in LDS (dynamic, 4 (N + 2) doubles): every
+// thread of the block runs the evaluation, those past the last pair on the
+// last pair without storing.
+template <int V, int POLY>
+__global__ __launch_bounds__ (128) void
+k_lub_eval (const double4 *__restrict__ pos, LubGeom g, LubEval ev, const int *__restrict__ off,
+            const int2 *__restrict__ list, int npairs, const double *__restrict__ x,
+            double *__restrict__ val)
+{
+  extern __shared__ double lub_sh[];
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  g.poly = POLY;
+  if (POLY)
+    lub_eval_item<V> (p < npairs ? p : npairs - 1, pos, g, ev, off, list, x, val, lub_sh,
+                      p < npairs);
+  else if (p < npairs)
+    lub_eval_item<V> (p, pos, g, ev, off, list, x, val);
 }
 
 // per pair: e, w, S11, S12 (26 values) for the dense build
@@ -276,15 +445,17 @@ k_lub_matrix (const int *__restrict__ off, const int2 *__restrict__ list,
   if (t < D * D) mat[((size_t) i * D + r) * n + (size_t) i * D + c] += acc;
 }
 
-// y[i][q] = sum over i's pairs in list order
-__global__ void k_lub_gather (const int *__restrict__ off, const double *__restrict__ val,
-                              int np, int D, double *__restrict__ y)
+// y[i][q] = sum over i's pairs in list order (pairs below cap: a list
+// that overflowed leaves y incomplete, never reads past it)
+static __global__ void k_lub_gather (const int *__restrict__ off, const double *__restrict__ val,
+                                     int np, int D, int cap, double *__restrict__ y)
 {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= np * D) return;
   const int i = t / D, q = t - i * D;
   double acc = 0.0;
-  for (int p = off[i]; p < off[i + 1]; p++) acc += val[(size_t) p * D + q];
+  const int p1 = min (off[i + 1], cap);
+  for (int p = off[i]; p < p1; p++) acc += val[(size_t) p * D + q];
   y[t] = acc;
 }
 

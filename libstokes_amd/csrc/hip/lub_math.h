@@ -30,6 +30,46 @@ enum { XA = 0, YA, YB, XC, YC, XG, YG, YH, XM, YM, ZM, NFAM };
 LUB_HD int nfam_of (int version) { return version == 0 ? 2 : (version == 1 ? 5 : 11); }
 
 // ---------------------------------------------------------------------------
+// sums of f_m t^m over m in [m0, m1], even m into se and odd m into so.
+// The functor evaluates four consecutive rows together (fv.four), which
+// lets it interleave their recurrences; t^m by repeated products.
+// ---------------------------------------------------------------------------
+template <class FV>
+LUB_HD void fsums (const FV &fv, int fam, int m0, int m1, double t, double &se, double &so)
+{
+  se = so = 0.0;
+  double tm = 1.0;
+  for (int m = 0; m < m0; m++) tm *= t;
+  int m = m0;
+#pragma unroll 1
+  for (; m + 3 <= m1; m += 4)
+    {
+      double f[4];
+      fv.four (fam, m, f);
+      const double t0 = tm, t1 = t0 * t, t2 = t1 * t, t3 = t2 * t;
+      tm = t3 * t;
+      if (m & 1)
+        {
+          so += f[0] * t0; se += f[1] * t1;
+          so += f[2] * t2; se += f[3] * t3;
+        }
+      else
+        {
+          se += f[0] * t0; so += f[1] * t1;
+          se += f[2] * t2; so += f[3] * t3;
+        }
+    }
+#pragma unroll 1
+  for (; m <= m1; m++)
+    {
+      const double fm = fv (fam, m);
+      if (m & 1) so += fm * tm;
+      else se += fm * tm;
+      tm *= t;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // far form: the 1/(s (1 + l)) series, even k for one of (11, 12), odd for the
 // other, the 12 part times its prefactor (n even: k < n)
 // ---------------------------------------------------------------------------
@@ -46,13 +86,8 @@ LUB_HD void far_form (int version, const FV &fv, int n, double l, double s, doub
   const int nf = nfam_of (version);
   for (int f = 0; f < nf; f++)
     {
-      double ev = 0.0, od = 0.0, tk = 1.0;
-      for (int k = 0; k < n; k++)
-        {
-          if (k & 1) od += fv (f, k) * tk;
-          else ev += fv (f, k) * tk;
-          tk *= t;
-        }
+      double ev, od;
+      fsums (fv, f, 0, n - 1, t, ev, od);
       two[2 * f] = odd11[f] ? od : ev;
       two[2 * f + 1] = pre[f] * (odd11[f] ? ev : od);
     }
@@ -65,23 +100,22 @@ LUB_HD void far_form (int version, const FV &fv, int n, double l, double s, doub
 // parity, f_m (s (1 + l))^-m + a_m (2 / s)^m with a_m the m-th coefficient
 // of the subtracted near-field part, a_m = -c1 - 2 c2 / m + 4 c3 / d_m,
 // d_m = m (m - 2) (-2 m at m = 2, as twobody.c:1055-1062) or m (m + 2) for
-// the G, H, M forms.  sf: sign of each term.
-template <class FV>
-LUB_HD double rest (const FV &fv, int fam, int mmax, int parity, int mlo, double s, double l,
-                    double c1, double c2, double c3, bool plus2, double sf)
+// the G, H, M forms.  sf: sign of each term.  fsum: the f_m part of that
+// parity (fsums over m = 1 .. mmax).
+LUB_HD double rest (double fsum, int mmax, int parity, int mlo, double s, double c1, double c2,
+                    double c3, bool plus2, double sf)
 {
-  const double t = 1.0 / (s * (1.0 + l)), u = 2.0 / s;
-  double tm = 1.0, um = 1.0, acc = 0.0;
+  const double u = 2.0 / s;
+  double um = 1.0, acc = 0.0;
   for (int m = 1; m <= mmax; m++)
     {
-      tm *= t;
       um *= u;
       if ((m & 1) != parity || m < mlo) continue;
       const double dm = plus2 ? double (m * (m + 2)) : (m == 2 ? -4.0 : double (m * (m - 2)));
       const double am = -c1 - 2.0 * c2 / m + 4.0 * c3 / dm;
-      acc += sf * (fv (fam, m) * tm + am * um);
+      acc += am * um;
     }
-  return acc;
+  return sf * (fsum + acc);
 }
 
 // twobody_lub_with_f (twobody.c:1977-2644) with terms m <= mmax; yc12_g5
@@ -94,57 +128,63 @@ LUB_HD void lub_form (int version, const FV &fv, int mmax, double l, double s, d
   const double l1 = 1.0 + l, l12 = l1 * l1, l13 = l12 * l1, l14 = l12 * l12;
   const double ll = l * l, l3 = ll * l, l4 = ll * ll;
   const double S1 = 1.0 - 4.0 / (s * s);
+  const double tt = 1.0 / (s * l1);
+  double fe, fo;
   const double ls1 = log (S1), ls2 = log ((s + 2.0) / (s - 2.0));
   double g1, g2, g3, g4, g5, g6;
   // XA
   g1 = 2.0 * ll / l13;
   g2 = 0.2 * l * (1.0 + 7.0 * l + ll) / l13;
   g3 = (1.0 + 18.0 * l - 29.0 * ll + 18.0 * l3 + l4) / (42.0 * l13);
+  fsums (fv, XA, 1, mmax, tt, fe, fo);
   two[0] = g1 / S1 - g2 * ls1 - g3 * S1 * ls1 + fv (XA, 0) - g1
-    + rest (fv, XA, mmax, 0, 2, s, l, g1, g2, g3, false, 1.0);
+    + rest (fe, mmax, 0, 2, s, g1, g2, g3, false, 1.0);
   two[1] = -2.0 / l1 * (2.0 / s * g1 / S1 + g2 * ls2 + g3 * S1 * ls2 + 4.0 * g3 / s
-                        + rest (fv, XA, mmax, 1, 1, s, l, g1, g2, g3, false, 1.0));
+                        + rest (fo, mmax, 1, 1, s, g1, g2, g3, false, 1.0));
   // YA
   g2 = 4.0 / 15.0 * l * (2.0 + l + 2.0 * ll) / l13;
   g3 = 2.0 / 375.0 * (16.0 - 45.0 * l + 58.0 * ll - 45.0 * l3 + 16.0 * l4) / l13;
+  fsums (fv, YA, 1, mmax, tt, fe, fo);
   two[2] = -g2 * ls1 - g3 * S1 * ls1 + fv (YA, 0)
-    + rest (fv, YA, mmax, 0, 2, s, l, 0.0, g2, g3, false, 1.0);
+    + rest (fe, mmax, 0, 2, s, 0.0, g2, g3, false, 1.0);
   two[3] = -2.0 / l1 * (g2 * ls2 + g3 * S1 * ls2 + 4.0 * g3 / s
-                        + rest (fv, YA, mmax, 1, 1, s, l, 0.0, g2, g3, false, 1.0));
+                        + rest (fo, mmax, 1, 1, s, 0.0, g2, g3, false, 1.0));
   if (version == 0) return;
   // YB
   g2 = -0.2 * l * (4.0 + l) / l12;
   g3 = -0.004 * (32.0 - 33.0 * l + 83.0 * ll + 43.0 * l3) / l12;
+  fsums (fv, YB, 1, mmax, tt, fe, fo);
   two[4] = g2 * ls2 + g3 * S1 * ls2 + 4.0 * g3 / s
-    + rest (fv, YB, mmax, 1, 1, s, l, 0.0, g2, g3, false, 1.0);
+    + rest (fo, mmax, 1, 1, s, 0.0, g2, g3, false, 1.0);
   two[5] = -4.0 / l12 * (-g2 * ls1 - g3 * S1 * ls1
-                         + rest (fv, YB, mmax, 0, 2, s, l, 0.0, g2, g3, false, 1.0));
+                         + rest (fe, mmax, 0, 2, s, 0.0, g2, g3, false, 1.0));
   // XC: a_m = -l^2 / ((1 + l) m (m - 1)) from m = 2
   {
-    const double c = ll / l1, t = 1.0 / (s * l1), u = 2.0 / s;
+    const double c = ll / l1, u = 2.0 / s;
     double v11 = c * (0.5 * ls1 + ls2 / s) + 1.0;
     double v12 = -c * (0.5 * ls2 + (ls1 - 2.0) / s);
-    double tm = t, um = u;
+    double a11 = 0.0, a12 = 0.0, um = u;
     for (int m = 2; m <= mmax; m++)
       {
-        tm *= t;
         um *= u;
-        const double term = fv (XC, m) * tm - c / double (m * (m - 1)) * um;
-        if (m & 1) v12 += term;
-        else v11 += term;
+        const double term = -c / double (m * (m - 1)) * um;
+        if (m & 1) a12 += term;
+        else a11 += term;
       }
-    two[6] = v11;
-    two[7] = -8.0 / l13 * v12;
+    fsums (fv, XC, 2, mmax, tt, fe, fo);
+    two[6] = v11 + (fe + a11);
+    two[7] = -8.0 / l13 * (v12 + (fo + a12));
   }
   // YC
   g2 = 0.4 * l / l1;
   g3 = 0.008 * (8.0 + 6.0 * l + 33.0 * ll) / l1;
   g4 = 0.8 * ll / l14;
   g5 = yc12_g5 * 0.016 * l * (43.0 - 24.0 * l + 43.0 * ll) / l14;
+  fsums (fv, YC, 1, mmax, tt, fe, fo);
   two[8] = -g2 * ls1 - g3 * S1 * ls1 + fv (YC, 0)
-    + rest (fv, YC, mmax, 0, 2, s, l, 0.0, g2, g3, false, 1.0);
+    + rest (fe, mmax, 0, 2, s, 0.0, g2, g3, false, 1.0);
   two[9] = 8.0 / l13 * (g4 * ls2 + g5 * S1 * ls2 + 4.0 * g5 / s
-                        + rest (fv, YC, mmax, 1, 1, s, l, 0.0, g4, g5, false, 1.0));
+                        + rest (fo, mmax, 1, 1, s, 0.0, g4, g5, false, 1.0));
   if (version == 1) return;
   // G, H, M: forms in s^2 - 4
   const double q = s * s - 4.0;
@@ -153,26 +193,29 @@ LUB_HD void lub_form (int version, const FV &fv, int mmax, double l, double s, d
   g1 = 3.0 * ll / l13;
   g2 = 0.3 * l * (1.0 + 12.0 * l - 4.0 * ll) / l13;
   g3 = (5.0 + 181.0 * l - 453.0 * ll + 566.0 * l3 - 65.0 * l4) / (140.0 * l13);
+  fsums (fv, XG, 1, mmax, tt, fe, fo);
   two[10] = g1 * T1 + (g2 + g3 * T3) * ls2 - g3 * s
-    + rest (fv, XG, mmax, 1, 1, s, l, g1, g2, g3, true, 1.0);
+    + rest (fo, mmax, 1, 1, s, g1, g2, g3, true, 1.0);
   two[11] = 4.0 / l12 * (-g1 * T4 - (g2 + g3 * T3) * ls5 + g3
-                         + rest (fv, XG, mmax, 0, 2, s, l, g1, g2, g3, true, -1.0));
+                         + rest (fe, mmax, 0, 2, s, g1, g2, g3, true, -1.0));
   // YG
   g2 = 0.1 * l * (4.0 - l + 7.0 * ll) / l13;
   g3 = (32.0 - 179.0 * l + 532.0 * ll - 356.0 * l3 + 221.0 * l4) / (500.0 * l13);
+  fsums (fv, YG, 1, mmax, tt, fe, fo);
   two[12] = (g2 + g3 * T3) * ls2 - g3 * s
-    + rest (fv, YG, mmax, 1, 1, s, l, 0.0, g2, g3, true, 1.0);
+    + rest (fo, mmax, 1, 1, s, 0.0, g2, g3, true, 1.0);
   two[13] = 4.0 / l12 * (-(g2 + g3 * T3) * ls5 + g3
-                         + rest (fv, YG, mmax, 0, 2, s, l, 0.0, g2, g3, true, -1.0));
+                         + rest (fe, mmax, 0, 2, s, 0.0, g2, g3, true, -1.0));
   // YH
   g2 = 0.1 * l * (2.0 - l) / l12;
   g3 = (16.0 - 61.0 * l + 180.0 * ll + 2.0 * l3) / (500.0 * l12);
   g5 = 0.05 * ll * (1.0 + 7.0 * l) / l12;
   g6 = l * (43.0 + 147.0 * l - 185.0 * ll + 221.0 * l3) / (1000.0 * l12);
+  fsums (fv, YH, 1, mmax, tt, fe, fo);
   two[14] = (g2 + g3 * T3) * ls5 - g3
-    + rest (fv, YH, mmax, 0, 2, s, l, 0.0, g2, g3, true, 1.0);
+    + rest (fe, mmax, 0, 2, s, 0.0, g2, g3, true, 1.0);
   two[15] = 8.0 / l13 * ((g5 + g6 * T3) * ls2 - g6 * s
-                         + rest (fv, YH, mmax, 1, 1, s, l, 0.0, g5, g6, true, 1.0));
+                         + rest (fo, mmax, 1, 1, s, 0.0, g5, g6, true, 1.0));
   // XM
   g1 = 1.2 * ll / l13;
   g2 = 0.12 * l * (1.0 + 17.0 * l - 9.0 * ll) / l13;
@@ -180,24 +223,27 @@ LUB_HD void lub_form (int version, const FV &fv, int mmax, double l, double s, d
   g4 = g1 * l;
   g5 = 0.12 * ll * (-4.0 + 17.0 * l - 4.0 * ll) / l13;
   g6 = l * (-65.0 + 832.0 * l - 1041.0 * ll + 832.0 * l3 - 65.0 * l4) / (350.0 * l13);
+  fsums (fv, XM, 1, mmax, tt, fe, fo);
   two[16] = g1 * T4 + (g2 + g3 * T3) * ls5 - g3 + 1.0
-    + rest (fv, XM, mmax, 0, 2, s, l, g1, g2, g3, true, 1.0);
+    + rest (fe, mmax, 0, 2, s, g1, g2, g3, true, 1.0);
   two[17] = 8.0 / l13 * (g4 * T1 + (g5 + g6 * T3) * ls2 - g6 * s
-                         + rest (fv, XM, mmax, 1, 1, s, l, g4, g5, g6, true, 1.0));
+                         + rest (fo, mmax, 1, 1, s, g4, g5, g6, true, 1.0));
   // YM
   g2 = 0.24 * l * (1.0 - l + 4.0 * ll) / l13;
   g3 = (24.0 - 201.0 * l + 882.0 * ll - 1182.0 * l3 + 591.0 * l4) / (625.0 * l13);
   g5 = 0.06 * ll * (7.0 - 10.0 * l + 7.0 * ll) / l13;
   g6 = 3.0 * l * (221.0 - 728.0 * l + 1902.0 * ll - 728.0 * l3 + 221.0 * l4) / (2500.0 * l13);
+  fsums (fv, YM, 1, mmax, tt, fe, fo);
   two[18] = (g2 + g3 * T3) * ls5 - g3 + 1.0
-    + rest (fv, YM, mmax, 0, 2, s, l, 0.0, g2, g3, true, 1.0);
+    + rest (fe, mmax, 0, 2, s, 0.0, g2, g3, true, 1.0);
   two[19] = 8.0 / l13 * ((g5 + g6 * T3) * ls2 - g6 * s
-                         + rest (fv, YM, mmax, 1, 1, s, l, 0.0, g5, g6, true, 1.0));
+                         + rest (fo, mmax, 1, 1, s, 0.0, g5, g6, true, 1.0));
   // ZM
   g3 = -0.3 * ll * (1.0 + ll) / l13;
-  two[20] = g3 * T3 * ls5 - g3 + 1.0 + rest (fv, ZM, mmax, 0, 2, s, l, 0.0, 0.0, g3, true, 1.0);
+  fsums (fv, ZM, 1, mmax, tt, fe, fo);
+  two[20] = g3 * T3 * ls5 - g3 + 1.0 + rest (fe, mmax, 0, 2, s, 0.0, 0.0, g3, true, 1.0);
   two[21] = 8.0 / l13 * (-g3 * T3 * ls2 + g3 * s
-                         + rest (fv, ZM, mmax, 1, 1, s, l, 0.0, 0.0, g3, true, -1.0));
+                         + rest (fo, mmax, 1, 1, s, 0.0, 0.0, g3, true, -1.0));
 }
 
 // twobody_scale (sd = false: dimensional with radius a1, twobody.c:2709-2772)

@@ -577,4 +577,100 @@ SHK_HD void contract_item (T ex, T ey, T ez, const T *s, const T *z, T *acc)
   acc[10] += cee * ey * ey + cdd + T (2) * ey * vy + mm5x2 * zyy;  // yy
 }
 
+// contract_item with the scalars known to be zero left out: bit t of M set
+// when s[t] may be nonzero; every product with a scalar outside M is
+// skipped (the terms it would add are zero)
+template <int V, typename T, unsigned M>
+SHK_HD void contract_item_m (T ex, T ey, T ez, const T *s, const T *z, T *acc)
+{
+  constexpr bool SA = (M & 0x3u) != 0, SB = (M & 0x4u) != 0, SC = (M & 0x18u) != 0;
+  constexpr bool SG = (M & 0x60u) != 0, SH = (M & 0x80u) != 0, SM = (M & 0x700u) != 0;
+  const T s0 = (M & 0x1u) ? s[0] : T (0), s1 = (M & 0x2u) ? s[1] : T (0);
+  const T s3 = (M & 0x8u) ? s[3] : T (0), s4 = (M & 0x10u) ? s[4] : T (0);
+  const T s5 = (M & 0x20u) ? s[5] : T (0), s6 = (M & 0x40u) ? s[6] : T (0);
+  const T s8 = (M & 0x100u) ? s[8] : T (0), s9 = (M & 0x200u) ? s[9] : T (0);
+  const T s10 = (M & 0x400u) ? s[10] : T (0);
+  const T eF = ex * z[0] + ey * z[1] + ez * z[2];
+  if (SA)
+    {
+      const T a1 = s1, a2 = s0 - s1;
+      acc[0] += a1 * z[0] + a2 * ex * eF;
+      acc[1] += a1 * z[1] + a2 * ey * eF;
+      acc[2] += a1 * z[2] + a2 * ez * eF;
+    }
+  if (V == 0) return;
+  const T tex = z[4] * ez - z[5] * ey, tey = z[5] * ex - z[3] * ez, tez = z[3] * ey - z[4] * ex;
+  if (SB)
+    {
+      const T b1 = s[2];
+      const T fex = z[1] * ez - z[2] * ey, fey = z[2] * ex - z[0] * ez, fez = z[0] * ey - z[1] * ex;
+      acc[0] += b1 * tex; acc[1] += b1 * tey; acc[2] += b1 * tez;
+      acc[3] += b1 * fex; acc[4] += b1 * fey; acc[5] += b1 * fez;
+    }
+  if (SC)
+    {
+      const T eT = ex * z[3] + ey * z[4] + ez * z[5];
+      const T c1 = s4, c2 = s3 - s4;
+      acc[3] += c1 * z[3] + c2 * ex * eT;
+      acc[4] += c1 * z[4] + c2 * ey * eT;
+      acc[5] += c1 * z[5] + c2 * ez * eT;
+    }
+  if (V == 1) return;
+  if (!(SG || SH || SM)) return;
+  const T zxx = z[6], zxy = T (0.5) * z[7], zxz = T (0.5) * z[8], zyz = T (0.5) * z[9];
+  const T zyy = z[10];
+  const T Zx = zxx * ex + zxy * ey + zxz * ez;
+  const T Zy = zxy * ex + zyy * ey + zyz * ez;
+  const T Zz = zxz * ex + zyz * ey;
+  const T Q = Zx * ex + Zy * ey + Zz * ez;
+  const T tr = zxx + zyy;
+  const T g1 = T (-1.0 / 3.0) * s5, g2 = s6, g3 = s5 - T (2) * s6;
+  if (SG)
+    {
+      const T gu = g1 * tr + g3 * Q;
+      acc[0] -= gu * ex + T (2) * g2 * Zx;
+      acc[1] -= gu * ey + T (2) * g2 * Zy;
+      acc[2] -= gu * ez + T (2) * g2 * Zz;
+    }
+  if (SH)
+    {
+      const T h2 = T (2) * s[7];
+      acc[3] += h2 * (ey * Zz - ez * Zy);
+      acc[4] += h2 * (ez * Zx - ex * Zz);
+      acc[5] += h2 * (ex * Zy - ey * Zx);
+    }
+  const T mm1 = T (1.5) * s8 - T (2) * s9 + T (0.5) * s10;
+  const T mm2 = T (-0.5) * s8 + T (0.5) * s10;
+  const T mm3 = T (1.0 / 6.0) * s8 - T (0.5) * s10;
+  const T mm4x2 = s9 - s10;
+  const T mm5x2 = s10;
+  const T h1 = SH ? s[7] : T (0);
+  T cee = T (0), cdd = T (0), vx = T (0), vy = T (0), vz = T (0);
+  if (SG)
+    {
+      cee = g3 * eF; cdd = g1 * eF;
+      vx = g2 * z[0]; vy = g2 * z[1]; vz = g2 * z[2];
+    }
+  if (SH)
+    {
+      vx += h1 * tex; vy += h1 * tey; vz += h1 * tez;
+    }
+  if (SM)
+    {
+      cee += mm1 * Q + mm2 * tr;
+      cdd += mm2 * Q + mm3 * tr;
+      vx += mm4x2 * Zx; vy += mm4x2 * Zy; vz += mm4x2 * Zz;
+    }
+  acc[6] += cee * ex * ex + cdd + T (2) * ex * vx;
+  acc[7] += cee * ex * ey + ex * vy + ey * vx;
+  acc[8] += cee * ex * ez + ex * vz + ez * vx;
+  acc[9] += cee * ey * ez + ey * vz + ez * vy;
+  acc[10] += cee * ey * ey + cdd + T (2) * ey * vy;
+  if (M & 0x400u)
+    {
+      acc[6] += mm5x2 * zxx; acc[7] += mm5x2 * zxy; acc[8] += mm5x2 * zxz;
+      acc[9] += mm5x2 * zyz; acc[10] += mm5x2 * zyy;
+    }
+}
+
 }  // namespace shk

@@ -79,30 +79,18 @@ __device__ __forceinline__ bool in_table (const RealParams &p, const unsigned *b
   return (bits[idx >> 5] >> (idx & 31)) & 1u;
 }
 
+// The pair images of output particle i and the 64 sources of chunk jc (one
+// per lane): loads the chunk into the wave's LDS, then the core and shell
+// loops add every image inside r_max to acc (per lane, not yet reduced).
+// Every lane of the wave calls it with the same (i, jc).
 template <int V, typename T>
-__global__ __launch_bounds__ (256) void
-k_real_compact (const double4 *__restrict__ pos, const double *__restrict__ x,
-                const typename Vec4<T>::type *__restrict__ cand,
-                const int4 *__restrict__ candm, const unsigned *__restrict__ tbits,
-                RealParams p, double *__restrict__ partial)
+__device__ __forceinline__ void
+real_chunk (const double4 *__restrict__ pos, const double *__restrict__ x,
+            const typename Vec4<T>::type *__restrict__ cand, const int4 *__restrict__ candm,
+            const unsigned *bits, const RealParams &p, int i, int jc, RealLds<V, T> &L,
+            int lane, T *acc)
 {
   constexpr int D = Dim<V>::D;
-  __shared__ RealLds<V, T> lds_all[4];
-  __shared__ unsigned bits_lds[1024];
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  RealLds<V, T> &L = lds_all[wave];
-  const int nj = (p.np + 63) / 64;
-  const int gw = blockIdx.x * 4 + wave;
-  const int i = gw / nj, jc = gw - i * nj;
-
-  // table bitmap to LDS (whole block; every wave passes this barrier)
-  const bool bits_in_lds = p.tbits_words <= 1024;
-  if (bits_in_lds)
-    for (int w = threadIdx.x; w < p.tbits_words; w += blockDim.x) bits_lds[w] = tbits[w];
-  __syncthreads ();
-  const unsigned *bits = bits_in_lds ? bits_lds : tbits;
-  if (i >= p.np) return;
-
   const int j = jc * 64 + lane;
   const bool valid = j < p.np;
   const double4 pi = pos[i];
@@ -116,6 +104,7 @@ k_real_compact (const double4 *__restrict__ pos, const double *__restrict__ x,
   const double n1 = rint (dx / p.lx);
   dx -= n1 * p.lx;
   const int w1 = int (n1), w2 = int (n2), w3 = int (n3);
+  wave_sync ();                   // the previous chunk's LDS reads are done
   L.d[lane][0] = T (dx); L.d[lane][1] = T (dy); L.d[lane][2] = T (dz);
   {
     T zz[D];
@@ -128,9 +117,6 @@ k_real_compact (const double4 *__restrict__ pos, const double *__restrict__ x,
 
   const T ai = T (pi.w), xi = T (p.xi), rmax2 = T (p.rmax2);
   const T mdx = L.d[lane][0], mdy = L.d[lane][1], mdz = L.d[lane][2];
-  T acc[D];
-#pragma unroll
-  for (int q = 0; q < D; q++) acc[q] = T (0);
 
   // core: inside r_max for every lane
   for (int c = 0; c < p.ncore; c++)
@@ -181,7 +167,12 @@ k_real_compact (const double4 *__restrict__ pos, const double *__restrict__ x,
       const T ex = L.d[src][0] + w.x, ey = L.d[src][1] + w.y, ez = L.d[src][2] + w.z;
       real_item<V, T> (p, ai, xi, L, src, ex, ey, ez, ex * ex + ey * ey + ez * ez, acc);
     }
+}
 
+// sum of acc over the wave (fixed butterfly), into every lane
+template <int D, typename T>
+__device__ __forceinline__ void wave_reduce (T *acc)
+{
 #pragma unroll
   for (int q = 0; q < D; q++)
     {
@@ -190,6 +181,37 @@ k_real_compact (const double4 *__restrict__ pos, const double *__restrict__ x,
       for (int off = 32; off > 0; off >>= 1) v += __shfl_xor (v, off, 64);
       acc[q] = v;
     }
+}
+
+template <int V, typename T>
+__global__ __launch_bounds__ (256) void
+k_real_compact (const double4 *__restrict__ pos, const double *__restrict__ x,
+                const typename Vec4<T>::type *__restrict__ cand,
+                const int4 *__restrict__ candm, const unsigned *__restrict__ tbits,
+                RealParams p, double *__restrict__ partial)
+{
+  constexpr int D = Dim<V>::D;
+  __shared__ RealLds<V, T> lds_all[4];
+  __shared__ unsigned bits_lds[1024];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  RealLds<V, T> &L = lds_all[wave];
+  const int nj = (p.np + 63) / 64;
+  const int gw = blockIdx.x * 4 + wave;
+  const int i = gw / nj, jc = gw - i * nj;
+
+  // table bitmap to LDS (whole block; every wave passes this barrier)
+  const bool bits_in_lds = p.tbits_words <= 1024;
+  if (bits_in_lds)
+    for (int w = threadIdx.x; w < p.tbits_words; w += blockDim.x) bits_lds[w] = tbits[w];
+  __syncthreads ();
+  const unsigned *bits = bits_in_lds ? bits_lds : tbits;
+  if (i >= p.np) return;
+
+  T acc[D];
+#pragma unroll
+  for (int q = 0; q < D; q++) acc[q] = T (0);
+  real_chunk<V, T> (pos, x, cand, candm, bits, p, i, jc, L, lane, acc);
+  wave_reduce<D> (acc);
   if (lane == 0)
     {
       double *out = partial + ((size_t) jc * p.np + i) * D;

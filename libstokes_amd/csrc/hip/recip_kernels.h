@@ -84,17 +84,34 @@ k_recip_sf (const double4 *__restrict__ pos, const double *__restrict__ x,
     }
 }
 
+// one thread per k: the splits summed in a fixed order, then P and Q of
+// every power of a_i (unequal spheres: five, each with the scalars of its
+// power only)
+template <int NPOW> __host__ __device__ inline long recip_blocks_threads (int nk)
+{
+  return (long) nk;
+}
+
+template <int V, typename T, int PW>
+__device__ __forceinline__ void recip_blocks_pw (const double4 &kk, const double *w,
+                                                 const double *CS, T *out)
+{
+  constexpr int D = Dim<V>::D;
+  double P[D], Q[D];
+  kblock_pq_poly<V, PW> (kk, w, CS, P, Q);
+#pragma unroll
+  for (int q = 0; q < D; q++) { out[q] = T (P[q]); out[D + q] = T (Q[q]); }
+}
+
 template <int V, typename T, int NPOW>
-__global__ __launch_bounds__ (128) void
-k_recip_blocks (const double4 *__restrict__ kv, const double *__restrict__ ks, int nk,
-                int nsplit, double gsign, const T *__restrict__ part, T *__restrict__ PQ)
+__device__ __forceinline__ void
+recip_blocks_item (int t, const double4 *__restrict__ kv, const double *__restrict__ ks,
+                   int nk, int nsplit, const T *__restrict__ part, T *__restrict__ PQ)
 {
   constexpr int D = Dim<V>::D;
   constexpr int NS = (NPOW > 1) ? 4 : 2;
-  // one thread per (k, power of a_i)
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= nk * NPOW) return;
-  const int k = t / NPOW, pw = t - k * NPOW + 1;
+  const int k = t;
+  if (k >= nk) return;
   double CS[NS * D];
 #pragma unroll
   for (int q = 0; q < NS * D; q++) CS[q] = 0.0;
@@ -116,11 +133,21 @@ k_recip_blocks (const double4 *__restrict__ kv, const double *__restrict__ ks, i
       for (int q = 0; q < D; q++) { out[q] = T (P[q]); out[D + q] = T (Q[q]); }
       return;
     }
-  double P[D], Q[D];
-  kblock_pq_poly_rt<V> (pw, kk, w, CS, P, Q);
-  T *out = PQ + (((size_t) k * NPOW + (pw - 1)) * 2) * D;
-#pragma unroll
-  for (int q = 0; q < D; q++) { out[q] = T (P[q]); out[D + q] = T (Q[q]); }
+  T *out = PQ + (size_t) k * NPOW * 2 * D;
+  recip_blocks_pw<V, T, 1> (kk, w, CS, out);
+  recip_blocks_pw<V, T, 2> (kk, w, CS, out + 2 * D);
+  recip_blocks_pw<V, T, 3> (kk, w, CS, out + 4 * D);
+  recip_blocks_pw<V, T, 4> (kk, w, CS, out + 6 * D);
+  recip_blocks_pw<V, T, 5> (kk, w, CS, out + 8 * D);
+}
+
+template <int V, typename T, int NPOW>
+__global__ __launch_bounds__ (128) void
+k_recip_blocks (const double4 *__restrict__ kv, const double *__restrict__ ks, int nk,
+                int nsplit, double gsign, const T *__restrict__ part, T *__restrict__ PQ)
+{
+  recip_blocks_item<V, T, NPOW> (blockIdx.x * blockDim.x + threadIdx.x, kv, ks, nk, nsplit,
+                                 part, PQ);
 }
 
 template <int V, typename T, int NPOW>
@@ -182,14 +209,14 @@ k_recip_back (const double4 *__restrict__ pos, const double4 *__restrict__ kv,
 // the summation order (and the result) does not depend on scheduling
 constexpr int COMBINE_LANES = 16;
 
+// thread gt of np * D * COMBINE_LANES (all lanes of a wave take part)
 template <int V, typename T>
-__global__ __launch_bounds__ (256) void
-k_combine (const double4 *__restrict__ pos, const double *__restrict__ x,
-           const double *__restrict__ rpart, int nreal, const T *__restrict__ kpart,
-           int nrecip, DevParams p, int periodic, double *__restrict__ y)
+__device__ __forceinline__ void
+combine_item (size_t gt, const double4 *__restrict__ pos, const double *__restrict__ x,
+              const double *__restrict__ rpart, int nreal, const T *__restrict__ kpart,
+              int nrecip, const DevParams &p, int periodic, double *__restrict__ y)
 {
   constexpr int D = Dim<V>::D;
-  const size_t gt = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
   const size_t t = gt / COMBINE_LANES;
   const int g = (int) (gt % COMBINE_LANES);
   const size_t n = (size_t) p.np * D;
@@ -209,6 +236,16 @@ k_combine (const double4 *__restrict__ pos, const double *__restrict__ x,
       const int i = (int) (tt / D), q = (int) (tt - (size_t) i * D);
       y[tt] = self_diag (p, pos[i].w, periodic != 0, V, q) * x[tt] + acc;
     }
+}
+
+template <int V, typename T>
+__global__ __launch_bounds__ (256) void
+k_combine (const double4 *__restrict__ pos, const double *__restrict__ x,
+           const double *__restrict__ rpart, int nreal, const T *__restrict__ kpart,
+           int nrecip, DevParams p, int periodic, double *__restrict__ y)
+{
+  combine_item<V, T> ((size_t) blockIdx.x * blockDim.x + threadIdx.x, pos, x, rpart, nreal,
+                      kpart, nrecip, p, periodic, y);
 }
 
 // ---- separable phases ---------------------------------------------------
@@ -237,12 +274,12 @@ __device__ __forceinline__ void phase_angles (const PhaseBasis &b, const double4
   t1 = b.k0x * p.x - b.s1 * t2 - b.s2 * t3;
 }
 
-// tab[j ld + e], e < W (ld >= W: row stride)
+// tab[j ld + e], e < W (ld >= W: row stride); thread t of np * W
 template <typename T>
-__global__ void k_phase_tab (const double4 *__restrict__ pos, int np, PhaseBasis b, int ld,
-                             Cplx<T> *__restrict__ tab)
+__device__ __forceinline__ void
+phase_tab_item (size_t t, const double4 *__restrict__ pos, int np, const PhaseBasis &b, int ld,
+                Cplx<T> *__restrict__ tab)
 {
-  const size_t t = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (size_t) np * b.W) return;
   const int j = (int) (t / b.W), e = (int) (t - (size_t) j * b.W);
   double t1, t2, t3;
@@ -255,6 +292,13 @@ __global__ void k_phase_tab (const double4 *__restrict__ pos, int np, PhaseBasis
   double s, c;
   sincos (double (m) * th, &s, &c);
   tab[(size_t) j * ld + e] = Cplx<T>{T (c), T (s)};
+}
+
+template <typename T>
+__global__ void k_phase_tab (const double4 *__restrict__ pos, int np, PhaseBasis b, int ld,
+                             Cplx<T> *__restrict__ tab)
+{
+  phase_tab_item<T> ((size_t) blockIdx.x * blockDim.x + threadIdx.x, pos, np, b, ld, tab);
 }
 
 template <typename T>
@@ -275,11 +319,12 @@ template <typename T> struct Quad;
 template <> struct Quad<float> { using type = float4; };
 template <> struct Quad<double> { using type = double4; };
 
+// block (bx, by): k block bx of 64, particle split by
 template <int V, typename T, int NPOW>
-__global__ __launch_bounds__ (256) void
-k_recip_sf_tab (const double4 *__restrict__ pos, const double *__restrict__ x,
-                const int4 *__restrict__ km, const Cplx<T> *__restrict__ tab, PhaseBasis b,
-                int np, int nk, int jper, int JT, T *__restrict__ part)
+__device__ __forceinline__ void
+recip_sf_tab_block (int bx, int by, const double4 *__restrict__ pos, const double *__restrict__ x,
+                    const int4 *__restrict__ km, const Cplx<T> *__restrict__ tab,
+                    const PhaseBasis &b, int np, int nk, int jper, int JT, T *__restrict__ part)
 {
   constexpr int D = Dim<V>::D;
   constexpr int DP = ZPad<V>::P;
@@ -291,8 +336,8 @@ k_recip_sf_tab (const double4 *__restrict__ pos, const double *__restrict__ x,
   Cplx<T> *stab = (Cplx<T> *) (sa2 + JT);
   T *red = (T *) smem_sf;                        // reused after the j loop
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int k = blockIdx.x * 64 + lane;
-  const int j0 = blockIdx.y * jper;
+  const int k = bx * 64 + lane;
+  const int j0 = by * jper;
   const int j1 = min (np, j0 + jper);
   const int4 m = km[min (k, nk - 1)];
   const int o1 = m.x, o2 = b.K1 + 1 + b.K2 + m.y, o3 = b.K1 + 2 * b.K2 + 2 + b.K3 + m.z;
@@ -368,10 +413,20 @@ k_recip_sf_tab (const double4 *__restrict__ pos, const double *__restrict__ x,
   if (wave == 0 && k < nk)
     {
       // part[(split * NS*D + q) * nk + k]: consecutive lanes, consecutive k
-      T *out = part + (size_t) blockIdx.y * (NS * D) * nk + k;
+      T *out = part + (size_t) by * (NS * D) * nk + k;
 #pragma unroll
       for (int q = 0; q < NS * D; q++) out[(size_t) q * nk] = acc[q];
     }
+}
+
+template <int V, typename T, int NPOW>
+__global__ __launch_bounds__ (256) void
+k_recip_sf_tab (const double4 *__restrict__ pos, const double *__restrict__ x,
+                const int4 *__restrict__ km, const Cplx<T> *__restrict__ tab, PhaseBasis b,
+                int np, int nk, int jper, int JT, T *__restrict__ part)
+{
+  recip_sf_tab_block<V, T, NPOW> (blockIdx.x, blockIdx.y, pos, x, km, tab, b, np, nk, jper, JT,
+                                  part);
 }
 
 // back-projection walking lattice rows.  A block is 64 i (one per lane) x
@@ -379,11 +434,13 @@ k_recip_sf_tab (const double4 *__restrict__ pos, const double *__restrict__ x,
 // quarters, one per wave, each with its own LDS tiles.  km[k].w != 0 marks
 // the first k of a row (direct phase); the other k rotate the FP64 phase by
 // e^{i th3}.  The four partial sums are added in LDS in wave order.
+// block (bx, by): particle block bx of 64, k split by
 template <int V, typename T, int NPOW>
-__global__ __launch_bounds__ (256) void
-k_recip_back_rows (const double4 *__restrict__ pos, const double4 *__restrict__ kv,
-                   const int4 *__restrict__ km, PhaseBasis b, const T *__restrict__ PQ, int np,
-                   int nk, int kper, T *__restrict__ part)
+__device__ __forceinline__ void
+recip_back_rows_block (int bx, int by, const double4 *__restrict__ pos,
+                       const double4 *__restrict__ kv, const int4 *__restrict__ km,
+                       const PhaseBasis &b, const T *__restrict__ PQ, int np, int nk, int kper,
+                       T *__restrict__ part)
 {
   constexpr int D = Dim<V>::D;
   constexpr int KT = NPOW > 1 ? 8 : 16;          // LDS stays under 64 KB
@@ -393,8 +450,8 @@ k_recip_back_rows (const double4 *__restrict__ pos, const double4 *__restrict__ 
   __shared__ T spq[4][KT][PW];
   __shared__ T red[3][D][64];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int i = blockIdx.x * 64 + lane;
-  const int kb0 = blockIdx.y * kper;
+  const int i = bx * 64 + lane;
+  const int kb0 = by * kper;
   const int kb1 = min (nk, kb0 + kper);
   const int kq = (kb1 - kb0 + 3) / 4;
   const int k0 = min (kb1, kb0 + wave * kq);
@@ -460,10 +517,20 @@ k_recip_back_rows (const double4 *__restrict__ pos, const double4 *__restrict__ 
   __syncthreads ();
   if (wave == 0 && i < np)
     {
-      T *out = part + ((size_t) blockIdx.y * np + i) * D;
+      T *out = part + ((size_t) by * np + i) * D;
 #pragma unroll
       for (int q = 0; q < D; q++) out[q] = acc[q] + red[0][q][lane] + red[1][q][lane] + red[2][q][lane];
     }
+}
+
+template <int V, typename T, int NPOW>
+__global__ __launch_bounds__ (256) void
+k_recip_back_rows (const double4 *__restrict__ pos, const double4 *__restrict__ kv,
+                   const int4 *__restrict__ km, PhaseBasis b, const T *__restrict__ PQ, int np,
+                   int nk, int kper, T *__restrict__ part)
+{
+  recip_back_rows_block<V, T, NPOW> (blockIdx.x, blockIdx.y, pos, kv, km, b, PQ, np, nk, kper,
+                                     part);
 }
 
 }  // namespace shk

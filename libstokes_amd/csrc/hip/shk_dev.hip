@@ -547,7 +547,8 @@ static int recip_launch (shk_dev *d, const double *x, int &nrecip, const void *&
                       d->stream, pos4, x, km, (const Cplx<T> *) d->ptab.p, b, p.np, nk, jper, jt,
                       (T *) d->kpart.p);
   HIPCHK (hipGetLastError ());
-  hipLaunchKernelGGL ((k_recip_blocks<V, T, NPOW>), dim3 ((nk * NPOW + 127) / 128), dim3 (128), 0,
+  hipLaunchKernelGGL ((k_recip_blocks<V, T, NPOW>),
+                      dim3 ((unsigned) ((recip_blocks_threads<NPOW> (nk) + 127) / 128)), dim3 (128), 0,
                       d->stream, kv, (const double *) d->ks.p, nk, sj, p.gsign,
                       (const T *) d->kpart.p, (T *) d->kpq.p);
   HIPCHK (hipGetLastError ());
@@ -1294,6 +1295,7 @@ static int lub_setup (shk_dev *d, int version, const shk_lub *lp, LubGeom &g, Lu
           return SHK_EUNSUP;
         }
     }
+  g.minimg = lub_minimg (g, lp->lubmax, lp->amax);
   ev.lubmin2 = lp->lubmin2;
   ev.nmax = lp->nmax;
   ev.flag_lub = lp->flag_lub;
@@ -1366,23 +1368,23 @@ extern "C" int shk_dev_lub (shk_dev *d, int version, const shk_lub *lp, const do
   if (total > 0)
     {
       const dim3 pg ((unsigned) ((total + 127) / 128));
-      if (version == 0)
-        hipLaunchKernelGGL ((k_lub_eval<0>), pg, dim3 (128), 0, d->stream, pos4, g, ev,
-                            (const int *) d->luboff.p, (const int2 *) d->lublist.p, total, x,
-                            (double *) d->lubval.p);
-      else if (version == 1)
-        hipLaunchKernelGGL ((k_lub_eval<1>), pg, dim3 (128), 0, d->stream, pos4, g, ev,
-                            (const int *) d->luboff.p, (const int2 *) d->lublist.p, total, x,
-                            (double *) d->lubval.p);
-      else
-        hipLaunchKernelGGL ((k_lub_eval<2>), pg, dim3 (128), 0, d->stream, pos4, g, ev,
-                            (const int *) d->luboff.p, (const int2 *) d->lublist.p, total, x,
-                            (double *) d->lubval.p);
+      const int *lo = (const int *) d->luboff.p;
+      const int2 *li = (const int2 *) d->lublist.p;
+      double *val = (double *) d->lubval.p;
+      const size_t lsh = sizeof (double) * 4 * (size_t) (ev.N + 2);
+#define LUB_EVAL(VV, PP)                                                                  \
+      hipLaunchKernelGGL ((k_lub_eval<VV, PP>), pg, dim3 (128), PP ? lsh : 0, d->stream, pos4, \
+                          g, ev, lo, li, total, x, val)
+      if (version == 0) { if (g.poly) LUB_EVAL (0, 1); else LUB_EVAL (0, 0); }
+      else if (version == 1) { if (g.poly) LUB_EVAL (1, 1); else LUB_EVAL (1, 0); }
+      else { if (g.poly) LUB_EVAL (2, 1); else LUB_EVAL (2, 0); }
+#undef LUB_EVAL
       HIPCHK (hipGetLastError ());
     }
   const size_t nd = (size_t) np * D;
   hipLaunchKernelGGL (k_lub_gather, dim3 ((unsigned) ((nd + 255) / 256)), dim3 (256), 0, d->stream,
-                      (const int *) d->luboff.p, (const double *) d->lubval.p, np, D, y);
+                      (const int *) d->luboff.p, (const double *) d->lubval.p, np, D,
+                      0x7fffffff, y);
   HIPCHK (hipGetLastError ());
   return SHK_OK;
 }

@@ -109,8 +109,8 @@ stokes_hip_ctx (struct stokes *sys)
 }
 
 /* the context of sys if one exists (none is created) */
-static stokes_ctx *
-ctx_find (struct stokes *sys)
+stokes_ctx *
+stokes_hip_ctx_find (struct stokes *sys)
 {
   stokes_ctx *c = NULL;
   pthread_mutex_lock (&g_lock);
@@ -437,6 +437,114 @@ prepare_params (stokes_ctx *c, shk_params *pup)
   return full;
 }
 
+/* the device parameters p and host lattice tables t of s (user parameters
+ * pu from prepare_params) computed at the splitting of split_tr (0: s's own
+ * xi); gcorr: restore the reference-sign G term of that other splitting.
+ * Release t with stokes_hip_tables_free. */
+static int
+build_tables (const struct stokes *s, const shk_params *pu, double split_tr,
+              int gcorr, shk_params *p_out, shk_tables *t_out)
+{
+  double *img = NULL, *kv = NULL, *ks = NULL, *cand = NULL;
+  int *candm = NULL, *km = NULL;
+  unsigned *tbits = NULL;
+  shk_tables t;
+  memset (&t, 0, sizeof (t));
+  struct stokes sh;
+  const struct stokes *src = s;
+  shk_params p = *pu;
+  const int split = split_tr > 0.0;
+  if (split)
+    {
+      stokes_hip_shadow_split (s, split_tr, &sh);
+      src = &sh;
+      p.xi = sh.xi; p.xi2 = sh.xi2; p.xiaspi = sh.xiaspi; p.xia2 = sh.xia2;
+      p.rmax2 = sh.rmax2;
+      p.self_a = sh.self_a; p.self_c = sh.self_c; p.self_m = sh.self_m;
+      p.nr = sh.nr;
+      if (gcorr) p.gsign = -1.0;
+    }
+  if (p.periodic)
+    {
+      if (p.nr <= 0 || src->rlx == NULL)
+        {
+          if (split) stokes_hip_shadow_free (&sh);
+          stokes_hip_set_error ("periodic system without image table"
+                                " (call stokes_set_xi)");
+          return STOKES_HIP_EARG;
+        }
+      img = malloc (sizeof (double) * 4 * p.nr);
+      for (int m = 0; img && m < p.nr; m++)
+        {
+          double sx = 0.0;
+          if (s->shear_mode == 1) sx = (double) src->rmy[m] * s->shear_shift;
+          else if (s->shear_mode == 2) sx = (double) src->rmz[m] * s->shear_shift;
+          img[4 * m + 0] = src->rlx[m] + sx;
+          img[4 * m + 1] = src->rly[m];
+          img[4 * m + 2] = src->rlz[m];
+          img[4 * m + 3] = 0.0;
+        }
+      p.nk = build_kvectors (src, p.gsign, &kv, &ks, &km);
+      if (split && gcorr && p.nk >= 0)
+        p.nk = add_g_correction (s, &kv, &ks, &km, p.nk);
+      t.km = km;
+      t.kbasis_s1 = s->shear_mode == 1 ? s->shear_shift / s->lx : 0.0;
+      t.kbasis_s2 = s->shear_mode == 2 ? s->shear_shift / s->lx : 0.0;
+      for (int k = 0; k < p.nk && km; k++)
+        {
+          if (km[4 * k] > t.kmax1) t.kmax1 = km[4 * k];
+          if (abs (km[4 * k + 1]) > t.kmax2) t.kmax2 = abs (km[4 * k + 1]);
+          if (abs (km[4 * k + 2]) > t.kmax3) t.kmax3 = abs (km[4 * k + 2]);
+        }
+      t.img = img;
+      t.kv = kv;
+      t.ks = ks;
+      t.s2x = s->shear_mode == 1 ? s->shear_shift : 0.0;
+      t.s3x = s->shear_mode == 2 ? s->shear_shift : 0.0;
+      t.ncand = build_candidates (src, t.s2x, t.s3x, &cand, &candm, &t.ncore);
+      t.cand = cand;
+      t.candm = candm;
+      t.tbits_words = build_table_bits (src, &tbits, &t.tx, &t.ty, &t.tz);
+      t.tbits = tbits;
+      if (!img || t.ncand < 0 || t.tbits_words < 0 || p.nk < 0)
+        {
+          stokes_hip_tables_free (&t);
+          if (split) stokes_hip_shadow_free (&sh);
+          stokes_hip_set_error ("lattice tables: allocation failed");
+          return STOKES_HIP_ENOMEM;
+        }
+    }
+  else
+    p.nk = 0;
+  if (split) stokes_hip_shadow_free (&sh);
+  *p_out = p;
+  *t_out = t;
+  return 0;
+}
+
+void
+stokes_hip_tables_free (shk_tables *t)
+{
+  free ((void *) t->img); free ((void *) t->kv); free ((void *) t->ks);
+  free ((void *) t->km); free ((void *) t->cand); free ((void *) t->candm);
+  free ((void *) t->tbits);
+  memset (t, 0, sizeof (*t));
+}
+
+int
+stokes_hip_tables (const struct stokes *s, double split_tr, int fix_g,
+                   shk_params *p, shk_tables *t)
+{
+  stokes_ctx c;
+  memset (&c, 0, sizeof (c));
+  c.sys = (struct stokes *) s;
+  c.fix_recip_g = fix_g;
+  shk_params pu;
+  prepare_params (&c, &pu);
+  const int split = split_tr > 0.0 && pu.periodic && s->shear_mode == 0;
+  return build_tables (s, &pu, split ? split_tr : 0.0, s->version == 2 && !fix_g, p, t);
+}
+
 int
 stokes_hip_prepare (stokes_ctx *c, int positions_on_device)
 {
@@ -464,82 +572,12 @@ stokes_hip_prepare (stokes_ctx *c, int positions_on_device)
   const int split = c->split_tr > 0.0 && pu.periodic && s->shear_mode == 0;
   if (full)
     {
-      double *img = NULL, *kv = NULL, *ks = NULL, *cand = NULL;
-      int *candm = NULL, *km = NULL;
-      unsigned *tbits = NULL;
+      shk_params p;
       shk_tables t;
-      memset (&t, 0, sizeof (t));
-      struct stokes sh;
-      const struct stokes *src = s;
-      shk_params p = pu;
-      if (split)
-        {
-          stokes_hip_shadow_split (s, c->split_tr, &sh);
-          src = &sh;
-          p.xi = sh.xi; p.xi2 = sh.xi2; p.xiaspi = sh.xiaspi; p.xia2 = sh.xia2;
-          p.rmax2 = sh.rmax2;
-          p.self_a = sh.self_a; p.self_c = sh.self_c; p.self_m = sh.self_m;
-          p.nr = sh.nr;
-          if (gcorr) p.gsign = -1.0;
-        }
-      if (p.periodic)
-        {
-          if (p.nr <= 0 || src->rlx == NULL)
-            {
-              if (split) stokes_hip_shadow_free (&sh);
-              stokes_hip_set_error ("periodic system without image table"
-                                    " (call stokes_set_xi)");
-              return STOKES_HIP_EARG;
-            }
-          img = malloc (sizeof (double) * 4 * p.nr);
-          for (int m = 0; m < p.nr; m++)
-            {
-              double sx = 0.0;
-              if (s->shear_mode == 1) sx = (double) src->rmy[m] * s->shear_shift;
-              else if (s->shear_mode == 2) sx = (double) src->rmz[m] * s->shear_shift;
-              img[4 * m + 0] = src->rlx[m] + sx;
-              img[4 * m + 1] = src->rly[m];
-              img[4 * m + 2] = src->rlz[m];
-              img[4 * m + 3] = 0.0;
-            }
-          p.nk = build_kvectors (src, p.gsign, &kv, &ks, &km);
-          if (split && gcorr)
-            p.nk = add_g_correction (s, &kv, &ks, &km, p.nk);
-          t.km = km;
-          t.kbasis_s1 = s->shear_mode == 1 ? s->shear_shift / s->lx : 0.0;
-          t.kbasis_s2 = s->shear_mode == 2 ? s->shear_shift / s->lx : 0.0;
-          for (int k = 0; k < p.nk && km; k++)
-            {
-              if (km[4 * k] > t.kmax1) t.kmax1 = km[4 * k];
-              if (abs (km[4 * k + 1]) > t.kmax2) t.kmax2 = abs (km[4 * k + 1]);
-              if (abs (km[4 * k + 2]) > t.kmax3) t.kmax3 = abs (km[4 * k + 2]);
-            }
-          t.img = img;
-          t.kv = kv;
-          t.ks = ks;
-          t.s2x = s->shear_mode == 1 ? s->shear_shift : 0.0;
-          t.s3x = s->shear_mode == 2 ? s->shear_shift : 0.0;
-          t.ncand = build_candidates (src, t.s2x, t.s3x, &cand, &candm, &t.ncore);
-          t.cand = cand;
-          t.candm = candm;
-          t.tbits_words = build_table_bits (src, &tbits, &t.tx, &t.ty, &t.tz);
-          t.tbits = tbits;
-          if (t.ncand < 0 || t.tbits_words < 0 || p.nk < 0)
-            {
-              free (img); free (kv); free (ks); free (km); free (cand);
-              free (candm); free (tbits);
-              if (split) stokes_hip_shadow_free (&sh);
-              stokes_hip_set_error ("lattice tables: allocation failed");
-              return STOKES_HIP_ENOMEM;
-            }
-        }
-      else
-        p.nk = 0;
-      int rc = shk_dev_set_system (c->dev, &p, s->pos, s->a,
-                                   p.periodic ? &t : NULL);
-      free (img); free (kv); free (ks); free (km); free (cand); free (candm);
-      free (tbits);
-      if (split) stokes_hip_shadow_free (&sh);
+      int rc = build_tables (s, &pu, split ? c->split_tr : 0.0, gcorr, &p, &t);
+      if (rc) return rc;
+      rc = shk_dev_set_system (c->dev, &p, s->pos, s->a, p.periodic ? &t : NULL);
+      stokes_hip_tables_free (&t);
       if (rc) return dev_fail (rc, "upload");
       c->prm = p;
       pu.nk = p.nk;
@@ -848,7 +886,7 @@ int
 stokes_hip_atimes_dev_noalloc (struct stokes *sys, const double *x_dev,
                                double *y_dev)
 {
-  stokes_ctx *c = sys ? ctx_find (sys) : NULL;
+  stokes_ctx *c = sys ? stokes_hip_ctx_find (sys) : NULL;
   if (!c || !c->valid)
     {
       stokes_hip_set_error ("atimes_dev_noalloc: no device set-up for this"

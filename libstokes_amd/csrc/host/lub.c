@@ -23,6 +23,10 @@ stokes_hip_lub_params (struct stokes *sys, shk_lub *lp, const double **tab, doub
   lp->lubmax = sys->lubmax;
   lp->nmax = sys->twobody_nmax;
   lp->flag_lub = sys->twobody_lub;
+  lp->amax = 1.0;
+  if (sys->a != NULL)
+    for (int i = 0; i < sys->np; i++)
+      if (i == 0 || sys->a[i] > lp->amax) lp->amax = sys->a[i];
   if (sys->a != NULL)
     {
       if (lp->nmax < 2)

@@ -49,6 +49,15 @@ typedef struct stokes_ctx {
 
 /* find or create the context of sys; NULL and a message on failure */
 stokes_ctx *stokes_hip_ctx (struct stokes *sys);
+/* the context of sys if one exists (none is created) */
+stokes_ctx *stokes_hip_ctx_find (struct stokes *sys);
+/* the device parameters p and host lattice tables t of sys computed at the
+ * splitting of split_tr (0 or a non-periodic / sheared system: sys's own
+ * xi), the reciprocal G term with the reference's sign unless fix_g;
+ * release t with stokes_hip_tables_free */
+int stokes_hip_tables (const struct stokes *sys, double split_tr, int fix_g,
+                       shk_params *p, shk_tables *t);
+void stokes_hip_tables_free (shk_tables *t);
 /* upload the parameters / positions of sys for the next operator call */
 int stokes_hip_prepare (stokes_ctx *c, int positions_on_device);
 void stokes_hip_release_context (struct stokes *sys);

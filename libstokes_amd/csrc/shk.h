@@ -104,6 +104,7 @@ typedef struct shk_lub {
   const double *const *coef;
   int N;
   const double *f1;
+  double amax;              /* largest radius (1 for equal spheres; 0: unknown) */
 } shk_lub;
 int shk_dev_lub (shk_dev *d, int version, const shk_lub *lp, const double *x, double *y);
 /* the dense extracted-form matrix of L (make_matrix_lub_3f/3ft/3fts,
@@ -115,6 +116,48 @@ int shk_dev_lub_matrix (shk_dev *d, int version, const shk_lub *lp, double *mat)
  * O, S) with (F, T, S) = (M^-1 + L) (U, O, E) */
 int shk_dev_solve_lub (shk_dev *d, double *M, double *L, int np, int kind, const double *b,
                        double *x);
+
+/* ---- ensembles: many periodic systems, one launch per phase ---------
+ * (ens_dev.hip).  A box is given by its parameters and host tables (as
+ * for shk_dev_set_system), its radii (NULL: 1) and its tile shape: jw
+ * 64-particle chunks per real-space wave, sj particle splits of the
+ * structure factors, sk k splits of the back-projection.  Every box must
+ * be periodic with box 0's version and kind of radii. */
+typedef struct shk_ens shk_ens;
+typedef struct shk_ens_box {
+  shk_params p;
+  shk_tables t;
+  const double *a;
+  int jw, sj, sk;
+} shk_ens_box;
+shk_ens *shk_ens_create (void);
+void shk_ens_destroy (shk_ens *e);
+void *shk_ens_stream (shk_ens *e);
+/* changes whenever a device buffer of e moves (captured graphs are stale) */
+unsigned long shk_ens_generation (shk_ens *e);
+/* upload the boxes (waits); prec SHK_FP64 or SHK_MIXED */
+int shk_ens_set (shk_ens *e, int nbox, const shk_ens_box *boxes, int prec);
+/* lubrication parameters per box (lp[nbox]; the coefficient table of box
+ * 0 serves all boxes) */
+int shk_ens_set_lub (shk_ens *e, const shk_lub *lp);
+/* count the near pairs at the packed positions pos[3 ntot] (waits) and
+ * size the pair list for them */
+int shk_ens_lub_prepare (shk_ens *e, const double *pos, int *total);
+/* launches only (may be captured): y = M x for every box (y may be NULL),
+ * ylub = L x, the lubrication correction (may be NULL; needs
+ * shk_ens_lub_prepare); timed: on one stream with events between the
+ * phases (shk_ens_phase_ms) */
+int shk_ens_launch (shk_ens *e, const double *pos, const double *x, double *y,
+                    double *ylub, int timed);
+/* pair total of the last completed launch, and the list capacity: a total
+ * above the capacity means that launch's ylub is incomplete */
+int shk_ens_lub_last_total (shk_ens *e);
+int shk_ens_lub_capacity (shk_ens *e);
+/* ms[0..3]: real space, reciprocal space, combine, lubrication of the last
+ * timed launch (0 for a part it skipped) */
+int shk_ens_phase_ms (shk_ens *e, double *ms);
+/* info[8]: np, candidates, core candidates, nk, phase-table width, jw, sj, sk */
+int shk_ens_box_info (shk_ens *e, int b, int *info);
 
 /* dense (n x n) row-major extracted-form matrix, device pointer */
 int shk_dev_matrix (shk_dev *d, double *mat);
