@@ -2,7 +2,6 @@
 // internal C ABI of shk.h.  One context per struct stokes, one HIP stream
 // per context (default priority), deterministic reduction order.
 #include <hip/hip_runtime.h>
-#include <rocblas/rocblas.h>
 #include <cstdio>
 #include <cstring>
 #include <cstdarg>
@@ -19,6 +18,7 @@
 #include "cache_kernels.h"
 #include "adj_kernels.h"
 #include "lub_kernels.h"
+#include "dgemm_kernels.h"
 
 using namespace shk;
 
@@ -67,6 +67,7 @@ struct shk_dev {
   DevBuf adjfg, adjcs, adjpart;
   DevBuf lubcoef, lubf1, lubcnt, luboff, lublist, lubval;   // lubrication
   DevBuf momt;                    // real-space pair moments (equal spheres)
+  DevBuf eph, gpart;              // FP64 phase matrices; split-K GEMM partials
   int tsym = 0;                   // reference table symmetric under m -> -m
   int lub_N = -1;                 // order of the uploaded coefficient table
   int lub_f1 = 0;                 // f_k(1) table uploaded
@@ -78,7 +79,6 @@ struct shk_dev {
   unsigned long ptab_gen = 0;     // positions the single-column table was built
   int ptab_kind = 0;              // for (1 FP32, 2 FP64; 0 none)
   DevBuf rcache, rz;              // dense real-space matrix, its vector(s)
-  rocblas_handle blas = nullptr;  // block products with the cached matrix
   int rc_use = 0, rc_prec = 0;    // products use rcache (built in rc_prec)
   PhaseBasis basis{};
   int ncand = 0, ncore = 0, tbits_words = 0;
@@ -182,9 +182,8 @@ extern "C" void shk_dev_destroy (shk_dev *d)
                     &d->tbits, &d->kpart, &d->kpq, &d->kback, &d->adjfg, &d->adjcs,
                     &d->adjpart, &d->km, &d->ptab, &d->rcache, &d->rz, &d->lubcoef,
                     &d->lubf1, &d->lubcnt, &d->luboff, &d->lublist, &d->lubval,
-                    &d->momt})
+                    &d->momt, &d->eph, &d->gpart})
     release (*b);
-  if (d->blas) (void) rocblas_destroy_handle (d->blas);
   for (auto &e : d->ev)
     if (e) (void) hipEventDestroy (e);
   if (d->stream) (void) hipStreamDestroy (d->stream);
@@ -810,6 +809,126 @@ static int recip_block_launch_mfma (shk_dev *d, const float *Zt, int zs, int npa
   return SHK_OK;
 }
 
+// C = A B in FP64 on the matrix cores (k_dgemm), row-major; K split so that
+// about 512 workgroups run (slices of at least 256), the slices summed in
+// order (k_dgemm_sum) through d->gpart
+static int dgemm (shk_dev *d, int M, int N, int K, const double *A, int lda, const double *B,
+                  int ldb, double *C, int ldc)
+{
+  const int tiles = ((M + 63) / 64) * ((N + 63) / 64);
+  int sz = 1;
+  while (tiles * sz < 512 && K / (2 * sz) >= 256) sz *= 2;
+  int kper = ((K + sz - 1) / sz + 15) & ~15;
+  sz = (K + kper - 1) / kper;
+  const dim3 g ((unsigned) ((N + 63) / 64), (unsigned) ((M + 63) / 64), (unsigned) sz);
+  if (sz == 1)
+    hipLaunchKernelGGL (k_dgemm, g, dim3 (256), 0, d->stream, A, lda, B, ldb, C, ldc, (size_t) 0,
+                        M, N, K, kper);
+  else
+    {
+      const size_t cz = (size_t) M * N;
+      int rc = ensure (d, d->gpart, sizeof (double) * cz * sz);
+      if (rc) return rc;
+      double *P = (double *) d->gpart.p;
+      hipLaunchKernelGGL (k_dgemm, g, dim3 (256), 0, d->stream, A, lda, B, ldb, P, N, cz, M, N, K,
+                          kper);
+      hipLaunchKernelGGL (k_dgemm_sum, dim3 ((unsigned) ((cz + 255) / 256)), dim3 (256), 0,
+                          d->stream, (const double *) P, N, cz, sz, M, N, C, ldc);
+    }
+  HIPCHK (hipGetLastError ());
+  return SHK_OK;
+}
+
+// the phase matrix of the half-space k: E[k][j] = cos k.x_j, E[nk + k][j]
+// = sin k.x_j (2 nk x np), or transposed, ET[j][k], ET[j][nk + k]
+// (np x 2 nk); FP64 sincos of the FP64 phase
+template <bool TRANS>
+__global__ void k_phase_E (const double4 *__restrict__ pos, const double4 *__restrict__ kv,
+                           int nk, int np, double *__restrict__ E)
+{
+  const size_t t = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (size_t) nk * np) return;
+  int k, j;
+  if (TRANS) { j = (int) (t / nk); k = (int) (t - (size_t) j * nk); }
+  else { k = (int) (t / np); j = (int) (t - (size_t) k * np); }
+  const double4 kk = kv[k], pj = pos[j];
+  double sn, cs;
+  sincos (kk.x * pj.x + kk.y * pj.y + kk.z * pj.z, &sn, &cs);
+  if (TRANS)
+    {
+      E[(size_t) j * 2 * nk + k] = cs;
+      E[(size_t) j * 2 * nk + nk + k] = sn;
+    }
+  else
+    {
+      E[(size_t) k * np + j] = cs;
+      E[(size_t) (nk + k) * np + j] = sn;
+    }
+}
+
+// P, Q of the equal-sphere k-blocks from SF = [C; S] (2 nk x D B rows,
+// column q B + b) into PQ = [P; Q] in the same layout; thread per (k, b)
+template <int V>
+__global__ __launch_bounds__ (128) void
+k_blk64 (const double4 *__restrict__ kv, const double *__restrict__ ks, int nk, int B,
+         const double *__restrict__ SF, double *__restrict__ PQ)
+{
+  constexpr int D = Dim<V>::D;
+  const size_t t = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (size_t) nk * B) return;
+  const int k = (int) (t / B), b = (int) (t - (size_t) k * B);
+  const size_t C = (size_t) D * B;
+  double CS[2 * D];
+#pragma unroll
+  for (int q = 0; q < D; q++)
+    {
+      CS[q] = SF[(size_t) k * C + (size_t) q * B + b];
+      CS[D + q] = SF[(size_t) (nk + k) * C + (size_t) q * B + b];
+    }
+  double P[D], Q[D];
+  kblock_pq_mono<V> (kv[k], ks + (size_t) k * 8, CS, P, Q);
+#pragma unroll
+  for (int q = 0; q < D; q++)
+    {
+      PQ[(size_t) k * C + (size_t) q * B + b] = P[q];
+      PQ[(size_t) (nk + k) * C + (size_t) q * B + b] = Q[q];
+    }
+}
+
+// FP64 reciprocal sums of a block of equal spheres on the matrix cores: the
+// structure factors SF = E Z (2 nk x D B), the k-blocks, the
+// back-projection Y = E^T PQ (np x D B, the layout of one reciprocal
+// partial); Z = XT (np rows of D B)
+template <int V>
+static int recip_block_launch_dgemm (shk_dev *d, const double *Z, int B, int &nrecip,
+                                     const void *&kpart)
+{
+  constexpr int D = Dim<V>::D;
+  const shk_params &p = d->prm;
+  const int nk = p.nk, np = p.np, C = D * B;
+  int rc;
+  if ((rc = ensure (d, d->eph, sizeof (double) * 4 * (size_t) nk * np))) return rc;
+  if ((rc = ensure (d, d->kpart, sizeof (double) * 2 * (size_t) nk * C))) return rc;
+  if ((rc = ensure (d, d->kpq, sizeof (double) * 2 * (size_t) nk * C))) return rc;
+  if ((rc = ensure (d, d->kback, sizeof (double) * (size_t) np * C))) return rc;
+  const double4 *pos4 = (const double4 *) d->pos.p;
+  const double4 *kv = (const double4 *) d->kv.p;
+  double *E = (double *) d->eph.p, *ET = E + 2 * (size_t) nk * np;
+  const dim3 ge ((unsigned) (((size_t) nk * np + 255) / 256));
+  hipLaunchKernelGGL ((k_phase_E<false>), ge, dim3 (256), 0, d->stream, pos4, kv, nk, np, E);
+  hipLaunchKernelGGL ((k_phase_E<true>), ge, dim3 (256), 0, d->stream, pos4, kv, nk, np, ET);
+  HIPCHK (hipGetLastError ());
+  double *SF = (double *) d->kpart.p, *PQ = (double *) d->kpq.p, *Y = (double *) d->kback.p;
+  if ((rc = dgemm (d, 2 * nk, C, np, E, np, Z, C, SF, C))) return rc;
+  hipLaunchKernelGGL ((k_blk64<V>), dim3 ((unsigned) (((size_t) nk * B + 127) / 128)), dim3 (128),
+                      0, d->stream, kv, (const double *) d->ks.p, nk, B, (const double *) SF, PQ);
+  HIPCHK (hipGetLastError ());
+  if ((rc = dgemm (d, np, C, 2 * nk, ET, 2 * nk, PQ, C, Y, C))) return rc;
+  nrecip = 1;
+  kpart = d->kback.p;
+  return SHK_OK;
+}
+
 template <int V, typename T>
 static int atimes_block_vt (shk_dev *d, const double *x, double *y, int B, bool dense)
 {
@@ -939,32 +1058,12 @@ static int atimes_block_vt (shk_dev *d, const double *x, double *y, int B, bool 
         }
       else
         {
-      if (!d->blas)
-        {
-          if (rocblas_create_handle (&d->blas) != rocblas_status_success
-              || rocblas_set_atomics_mode (d->blas, rocblas_atomics_not_allowed)
-                   != rocblas_status_success)
-            {
-              set_err ("rocblas handle: creation failed");
-              return SHK_EHIP;
-            }
-        }
-      if (rocblas_set_stream (d->blas, d->stream) != rocblas_status_success)
-        {
-          set_err ("rocblas_set_stream failed");
-          return SHK_EHIP;
-        }
-      const double one = 1, zero = 0;
-      rocblas_status st =
-        rocblas_dgemm (d->blas, rocblas_operation_none, rocblas_operation_none, B, ND, ND,
-                       &one, (const double *) Z, B, (const double *) d->rcache.p, ND, &zero,
-                       (double *) d->partial.p, B);
-      if (st != rocblas_status_success)
-        {
-          set_err ("rocblas gemm (cached real space): status %d", (int) st);
-          return SHK_EHIP;
-        }
-      nj = 1;
+          // FP64: Y = Mr Z on the FP64 matrix cores (rows of Mr are the
+          // output components, Z the transposed block)
+          if ((rc = dgemm (d, (int) ND, B, (int) ND, (const double *) d->rcache.p, (int) ND,
+                           (const double *) Z, B, (double *) d->partial.p, B)))
+            return rc;
+          nj = 1;
         }
     }
   else if (mom64)
@@ -1061,9 +1160,10 @@ static int atimes_block_vt (shk_dev *d, const double *x, double *y, int B, bool 
                                                       kpart)
                     : recip_block_launch_mfma<V, 1> (d, (const float *) XT, zs, npad, B, nrecip,
                                                       kpart);
+      else if (!p.poly)
+        rc = recip_block_launch_dgemm<V> (d, (const double *) XT, B, nrecip, kpart);
       else
-        rc = p.poly ? recip_block_launch<V, T, 5> (d, XT, B, nrecip, kpart)
-                    : recip_block_launch<V, T, 1> (d, XT, B, nrecip, kpart);
+        rc = recip_block_launch<V, T, 5> (d, XT, B, nrecip, kpart);
       if (rc) return rc;
     }
   hipLaunchKernelGGL ((k_combine_block<V, T>), dim3 ((unsigned) ((nb * D + 255) / 256)), dim3 (256), 0,
