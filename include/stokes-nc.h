@@ -5,15 +5,16 @@
  * stokes_nc_init defines, and read back by the functions of the
  * reference's stokes-nc-read.h (any classic or 64-bit-offset file).
  * Errors print a message and exit, as the reference's stokes_nc_error
- * does.  Not provided: stokes_nc_print_actives, the BD parameter files
- * (stokes_nc_set_by_params and its readers), and stokes_nc_set_rng /
- * stokes_nc_get_rng, which take the reference's struct KIrand
- * (stokes_nc_set_rng_state / stokes_nc_get_rng_state use the same
- * variables with a raw MT19937 state). */
+ * does.  Not provided: stokes_nc_set_rng / stokes_nc_get_rng, which take
+ * the reference's struct KIrand (stokes_nc_set_rng_state /
+ * stokes_nc_get_rng_state use the same variables with a raw MT19937
+ * state). */
 #ifndef LIBSTOKES_HIP_STOKES_NC_H
 #define LIBSTOKES_HIP_STOKES_NC_H
 
 #include <stdint.h>
+#include <stdio.h>
+#include "stokes.h"
 
 #ifdef __cplusplus
 extern "C" {
@@ -153,6 +154,51 @@ double stokes_nc_get_time_step (const struct stokes_nc *nc, int step);
 void stokes_nc_get_rng_state (const struct stokes_nc *nc, int step,
                               uint32_t *mt, int *mti, int *gauss_has_saved,
                               double *gauss_saved);
+
+/* ---- BD parameter files ---- */
+
+/* replaces stokes-nc.c:52-115: the table of the defined entries and the
+ * shear mode */
+void stokes_nc_print_actives (struct stokes_nc *nc, FILE *out);
+
+/* replaces stokes-nc.c:2731-2841: a new file (stokes_nc_init with the
+ * constant imposed flow Ui0, Oi0, Ei0) holding the parameters of a BD run:
+ * sys->version, np, nm and radii a (NULL: equal spheres), the lattice lat
+ * when sys->periodic, the forces F (and torques T, strains E) of the nm
+ * mobile particles, for np > nm the fixed particles' velocities uf (and
+ * of, ef) and positions xf, and the shear rate for shear_mode != 0 */
+struct stokes_nc *stokes_nc_set_by_params (const char *out_file, const struct stokes *sys,
+                                           int flag_Q, const double *Ui, const double *Oi,
+                                           const double *Ei, const double *F,
+                                           const double *T, const double *E,
+                                           const double *uf, const double *of,
+                                           const double *ef, const double *xf,
+                                           const double *lat, int shear_mode,
+                                           double shear_rate, int flag_rng);
+
+/* replaces stokes-nc.c:2869-3188: 0 when the file holds these parameters
+ * (entries compared to within tiny, the expected entries defined), else
+ * the number of mismatches (each printed to stderr) */
+int stokes_nc_check_params (const struct stokes_nc *nc, const struct stokes *sys,
+                            int flag_Q, const double *Ui, const double *Oi,
+                            const double *Ei, const double *F, const double *T,
+                            const double *E, const double *uf, const double *of,
+                            const double *ef, const double *xf, const double *lat,
+                            double tiny);
+
+/* replaces stokes-nc.c:3191-3342: 0 F, 1 FT, 2 FTS from the entries the
+ * file defines (0 when they match none) */
+int stokes_nc_get_version (const struct stokes_nc *nc);
+
+/* replaces stokes-nc.c:3369-3540: the parameters of stokes_nc_set_by_params
+ * back; sys gets version, np / nm (stokes_set_np), radii, periodic and
+ * the lattice (stokes_set_l), shear_mode and shear_rate, not the Ewald or
+ * iteration parameters */
+void stokes_nc_get_params (const struct stokes_nc *nc, struct stokes *sys, int *flag_Q,
+                           double *Ui, double *Oi, double *Ei, double *F, double *T,
+                           double *E, double *uf, double *of, double *ef, double *xf,
+                           double *lat, int *shear_mode, double *shear_rate,
+                           int *flag_rng);
 
 #ifdef __cplusplus
 }

@@ -22,7 +22,9 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <math.h>
 #include "stokes-nc.h"
+#include "stokes.h"
 
 enum { NCT_CHAR = 2, NCT_INT = 4, NCT_DOUBLE = 6 };
 enum { NC_TAG_DIM = 0x0A, NC_TAG_VAR = 0x0B, NC_TAG_ATT = 0x0C };
@@ -338,7 +340,7 @@ nc_create (const char *filename, int nm, int nf, const nc_flags *fl,
   if (!nc) nc_die (filename, "allocation failed", NULL);
   snprintf (nc->fname, sizeof (nc->fname), "%s", filename);
   /* NC_NOCLOBBER: an existing file is an error */
-  nc->fp = fopen (filename, "wxb");
+  nc->fp = fopen (filename, "w+xb");
   if (!nc->fp) nc_die (filename, "cannot create (exists, or no access)", NULL);
   int dv[3];
   const int p_dim = def_dim (nc, "p", nm);
@@ -916,4 +918,286 @@ stokes_nc_get_rng_state (const struct stokes_nc *nc, int step, uint32_t *mt,
   read_values (f, v, v->begin + r, NULL, gauss_has_saved);
   v = want_var (f, "mt_Gs", 1, 1, "stokes_nc_get_rng_state");
   read_values (f, v, v->begin + r, gauss_saved, NULL);
+}
+
+/* ---- BD parameter files (stokes-nc.c:52-115, 2731-3540) ---- */
+
+/* the scalar int / double variable name of the file (0 / 0.0 and a message
+ * when the file has no such variable) */
+static int
+scalar_int (const struct stokes_nc *nc, const char *name, const char *who)
+{
+  nc_file *f = NCF (nc);
+  const int k = var_index (f, name);
+  if (k < 0)
+    {
+      fprintf (stderr, "at reading %s in %s\n", name, who);
+      return 0;
+    }
+  int v = 0;
+  read_values (f, &f->vars[k], f->vars[k].begin, NULL, &v);
+  return v;
+}
+
+static double
+scalar_double (const struct stokes_nc *nc, const char *name, const char *who)
+{
+  nc_file *f = NCF (nc);
+  const int k = var_index (f, name);
+  if (k < 0)
+    {
+      fprintf (stderr, "at reading %s in %s\n", name, who);
+      return 0.0;
+    }
+  double v = 0.0;
+  read_values (f, &f->vars[k], f->vars[k].begin, &v, NULL);
+  return v;
+}
+
+void
+stokes_nc_print_actives (struct stokes_nc *nc, FILE *out)
+{
+  fprintf (out, "    | x | u o e | f t s | ui oi ei\n");
+  fprintf (out, "----+---+-------+-------+----------\n");
+  fprintf (out, "x0  | %d | %d %d %d | %d %d %d | %d  %d  %d\n", nc->flag_x0, nc->flag_u0,
+           nc->flag_o0, nc->flag_e0, nc->flag_f0, nc->flag_t0, nc->flag_s0, nc->flag_ui0,
+           nc->flag_oi0, nc->flag_ei0);
+  fprintf (out, "xf0 | %d | %d %d %d | %d %d %d |\n", nc->flag_xf0, nc->flag_uf0,
+           nc->flag_of0, nc->flag_ef0, nc->flag_ff0, nc->flag_tf0, nc->flag_sf0);
+  fprintf (out, "----+---+-------+-------+----------\n");
+  fprintf (out, "x   | %d | %d %d %d | %d %d %d | %d  %d  %d\n", nc->flag_x, nc->flag_u,
+           nc->flag_o, nc->flag_e, nc->flag_f, nc->flag_t, nc->flag_s, nc->flag_ui,
+           nc->flag_oi, nc->flag_ei);
+  fprintf (out, "xf  | %d | %d %d %d | %d %d %d |\n", nc->flag_xf, nc->flag_uf, nc->flag_of,
+           nc->flag_ef, nc->flag_ff, nc->flag_tf, nc->flag_sf);
+  fprintf (out, "----+---+-------+-------+----------\n");
+  fprintf (out, "q   | %d |\n", nc->flag_q);
+  fprintf (out, "a   | %d |\n", nc->flag_a);
+  fprintf (out, "af  | %d |\n", nc->flag_af);
+  fprintf (out, "----+---+-------+------------------\n");
+  fprintf (out, "shear_mode      | %d\n",
+           scalar_int (nc, "shear_mode", "stokes_nc_print_actives"));
+  fprintf (out, "----------------+------------------\n");
+  fprintf (out, "states of RNG   | %d\n", nc->flag_rng);
+}
+
+struct stokes_nc *
+stokes_nc_set_by_params (const char *out_file, const struct stokes *sys, int flag_Q,
+                         const double *Ui, const double *Oi, const double *Ei,
+                         const double *F, const double *T, const double *E,
+                         const double *uf, const double *of, const double *ef,
+                         const double *xf, const double *lat, int shear_mode,
+                         double shear_rate, int flag_rng)
+{
+  const int nm = sys->nm, nf = sys->np - nm;
+  const int poly = sys->a != NULL;
+  /* constant imposed flow (Ui0, Oi0, Ei0) */
+  struct stokes_nc *nc = stokes_nc_init (out_file, nm, nf, sys->version, poly, flag_Q, 0,
+                                         shear_mode, flag_rng);
+  stokes_nc_set_ui0 (nc, Ui);
+  stokes_nc_set_oi0 (nc, Oi);
+  stokes_nc_set_ei0 (nc, Ei);
+  if (sys->periodic == 1) stokes_nc_set_l (nc, lat);
+  if (poly) stokes_nc_set_a (nc, sys->a);
+  stokes_nc_set_f0 (nc, F);
+  if (sys->version >= 1) stokes_nc_set_t0 (nc, T);
+  if (sys->version == 2) stokes_nc_set_e0 (nc, E);
+  if (nf > 0)
+    {
+      if (poly) stokes_nc_set_af (nc, sys->a + nm);
+      stokes_nc_set_uf0 (nc, uf);
+      if (sys->version >= 1) stokes_nc_set_of0 (nc, of);
+      if (sys->version == 2) stokes_nc_set_ef0 (nc, ef);
+      stokes_nc_set_xf0 (nc, xf);
+    }
+  if (shear_mode != 0) stokes_nc_set_shear_rate (nc, shear_rate);
+  return nc;
+}
+
+/* 1 if some |x[i] - y[i]| >= tiny */
+static int
+differ (const double *x, const double *y, int n, double tiny)
+{
+  for (int i = 0; i < n; i++)
+    if (fabs (x[i] - y[i]) >= tiny) return 1;
+  return 0;
+}
+
+int
+stokes_nc_check_params (const struct stokes_nc *nc, const struct stokes *sys, int flag_Q,
+                        const double *Ui, const double *Oi, const double *Ei,
+                        const double *F, const double *T, const double *E,
+                        const double *uf, const double *of, const double *ef,
+                        const double *xf, const double *lat, double tiny)
+{
+  (void) flag_Q;
+  const int nm = sys->nm, nf = sys->np - nm, v = sys->version;
+  int bad = 0;
+  if (nm != nc->np) { fprintf (stderr, "mismatch np %d != %d\n", nm, nc->np); bad++; }
+  if (nf != nc->npf) { fprintf (stderr, "mismatch npf %d != %d\n", nf, nc->npf); bad++; }
+  if (nc->flag_ui0 != 1 || nc->flag_oi0 != 1 || nc->flag_ei0 != 1)
+    {
+      fprintf (stderr, "imposed flows are not set (%d,%d,%d)\n", nc->flag_ui0, nc->flag_oi0,
+               nc->flag_ei0);
+      bad++;
+    }
+  /* the entries the version needs, without (nf == 0) or with fixed
+   * particles (then also their radii flag) */
+  int want = nc->flag_f0 == 1 && nc->flag_x == 1 && nc->flag_u == 1;
+  if (v >= 1) want = want && nc->flag_t0 == 1 && nc->flag_o == 1;
+  if (v == 2) want = want && nc->flag_e0 == 1 && nc->flag_s == 1;
+  if (nf == 0)
+    {
+      if ((sys->a == NULL) != (nc->flag_a == 0)) { fprintf (stderr, "mismatch flag_a\n"); bad++; }
+      if (!want)
+        {
+          fprintf (stderr, "%s params are not set properly.\n",
+                   v == 0 ? "F" : (v == 1 ? "FT" : "FTS"));
+          bad++;
+        }
+    }
+  else
+    {
+      if ((sys->a == NULL) != (nc->flag_af == 0)) { fprintf (stderr, "mismatch flag_af\n"); bad++; }
+      want = want && nc->flag_uf0 == 1 && nc->flag_xf0 == 1 && nc->flag_ff == 1;
+      if (v >= 1) want = want && nc->flag_of0 == 1 && nc->flag_tf == 1;
+      if (v == 2) want = want && nc->flag_ef0 == 1 && nc->flag_sf == 1;
+      if (!want)
+        {
+          fprintf (stderr, "%s-fix params are not set properly.\n",
+                   v == 0 ? "F" : (v == 1 ? "FT" : "FTS"));
+          bad++;
+        }
+    }
+  double c[5];
+  stokes_nc_get_array1d (nc, "Ui0", c);
+  if (differ (Ui, c, 3, tiny)) { fprintf (stderr, "Ui mismatch\n"); bad++; }
+  stokes_nc_get_array1d (nc, "Oi0", c);
+  if (differ (Oi, c, 3, tiny)) { fprintf (stderr, "Oi mismatch\n"); bad++; }
+  stokes_nc_get_array1d (nc, "Ei0", c);
+  if (differ (Ei, c, 5, tiny)) { fprintf (stderr, "Ei mismatch\n"); bad++; }
+  if (sys->periodic == 1)
+    {
+      stokes_nc_get_array1d (nc, "l", c);
+      if (differ (lat, c, 3, tiny)) { fprintf (stderr, "l mismatch\n"); bad++; }
+    }
+  if (sys->shear_mode != 0)
+    {
+      if (sys->shear_mode != scalar_int (nc, "shear_mode", "stokes_nc_check_params"))
+        { fprintf (stderr, "shear_mode mismatch\n"); bad++; }
+      c[0] = scalar_double (nc, "shear_rate", "stokes_nc_check_params");
+      if (differ (&sys->shear_rate, c, 1, tiny)) { fprintf (stderr, "shear_rate mismatch\n"); bad++; }
+    }
+  double *w = malloc (sizeof (double) * 5 * (size_t) (sys->np > 0 ? sys->np : 1));
+  if (!w) nc_die (NCF (nc)->fname, "allocation failed", NULL);
+  if (sys->a != NULL)
+    {
+      stokes_nc_get_array1d (nc, "a", w);
+      if (differ (sys->a, w, nm, tiny)) { fprintf (stderr, "a mismatch\n"); bad++; }
+    }
+  stokes_nc_get_data0 (nc, "F0", w);
+  if (differ (F, w, 3 * nm, tiny)) { fprintf (stderr, "F0 mismatch\n"); bad++; }
+  if (v >= 1)
+    {
+      stokes_nc_get_data0 (nc, "T0", w);
+      if (differ (T, w, 3 * nm, tiny)) { fprintf (stderr, "T0 mismatch\n"); bad++; }
+    }
+  if (v == 2)
+    {
+      stokes_nc_get_data0 (nc, "E0", w);
+      if (differ (E, w, 5 * nm, tiny)) { fprintf (stderr, "E0 mismatch\n"); bad++; }
+    }
+  if (nf > 0)
+    {
+      if (sys->a != NULL)
+        {
+          stokes_nc_get_array1d (nc, "af", w);
+          if (differ (sys->a + nm, w, nf, tiny)) { fprintf (stderr, "af mismatch\n"); bad++; }
+        }
+      stokes_nc_get_data0 (nc, "Uf0", w);
+      if (differ (uf, w, 3 * nf, tiny)) { fprintf (stderr, "Uf0 mismatch\n"); bad++; }
+      if (v >= 1)
+        {
+          stokes_nc_get_data0 (nc, "Of0", w);
+          if (differ (of, w, 3 * nf, tiny)) { fprintf (stderr, "Of0 mismatch\n"); bad++; }
+        }
+      if (v == 2)
+        {
+          stokes_nc_get_data0 (nc, "Ef0", w);
+          if (differ (ef, w, 5 * nf, tiny)) { fprintf (stderr, "Ef0 mismatch\n"); bad++; }
+        }
+      stokes_nc_get_data0 (nc, "xf0", w);
+      if (differ (xf, w, 3 * nf, tiny)) { fprintf (stderr, "xf0 mismatch\n"); bad++; }
+    }
+  free (w);
+  return bad;
+}
+
+/* the version from the flags: exactly the entries stokes_nc_init defines
+ * for F, FT or FTS (with or without fixed particles), else 0 */
+int
+stokes_nc_get_version (const struct stokes_nc *nc)
+{
+  const int fix = nc->npf != 0;
+  for (int v = 2; v >= 0; v--)
+    {
+      const int t = v >= 1, s = v == 2;
+      if (nc->flag_f0 == 1 && nc->flag_t0 == t && nc->flag_e0 == s && nc->flag_x == 1
+          && nc->flag_u == 1 && nc->flag_o == t && nc->flag_s == s && nc->flag_uf0 == fix
+          && nc->flag_of0 == (fix && t) && nc->flag_ef0 == (fix && s) && nc->flag_xf0 == fix
+          && nc->flag_ff == fix && nc->flag_tf == (fix && t) && nc->flag_sf == (fix && s))
+        return v;
+    }
+  return 0;
+}
+
+void
+stokes_nc_get_params (const struct stokes_nc *nc, struct stokes *sys, int *flag_Q,
+                      double *Ui, double *Oi, double *Ei, double *F, double *T, double *E,
+                      double *uf, double *of, double *ef, double *xf, double *lat,
+                      int *shear_mode, double *shear_rate, int *flag_rng)
+{
+  *flag_Q = nc->flag_q;
+  *flag_rng = nc->flag_rng;
+  sys->version = stokes_nc_get_version (nc);
+  const int nm = nc->np, nf = nc->npf, np = nm + nf;
+  stokes_set_np (sys, np, nm);
+  if (nc->flag_a != 0)
+    {
+      double *a = malloc (sizeof (double) * (size_t) (np > 0 ? np : 1));
+      if (!a) nc_die (NCF (nc)->fname, "allocation failed", NULL);
+      stokes_nc_get_array1d (nc, "a", a);
+      if (nc->flag_af != 0) stokes_nc_get_array1d (nc, "af", a + nm);
+      stokes_set_radius (sys, a);
+      free (a);
+    }
+  /* the lattice is (0, 0, 0) until set: an open system */
+  stokes_nc_get_array1d (nc, "l", lat);
+  if (lat[0] == 0.0 && lat[1] == 0.0 && lat[2] == 0.0)
+    sys->periodic = 0;
+  else
+    {
+      sys->periodic = 1;
+      stokes_set_l (sys, lat[0], lat[1], lat[2]);
+    }
+  stokes_nc_get_array1d (nc, "Ui0", Ui);
+  stokes_nc_get_array1d (nc, "Oi0", Oi);
+  stokes_nc_get_array1d (nc, "Ei0", Ei);
+  stokes_nc_get_data0 (nc, "F0", F);
+  if (sys->version >= 1) stokes_nc_get_data0 (nc, "T0", T);
+  if (sys->version == 2) stokes_nc_get_data0 (nc, "E0", E);
+  if (nf > 0)
+    {
+      stokes_nc_get_data0 (nc, "xf0", xf);
+      stokes_nc_get_data0 (nc, "Uf0", uf);
+      if (sys->version >= 1) stokes_nc_get_data0 (nc, "Of0", of);
+      if (sys->version == 2) stokes_nc_get_data0 (nc, "Ef0", ef);
+    }
+  *shear_mode = scalar_int (nc, "shear_mode", "stokes_nc_get_params");
+  sys->shear_mode = *shear_mode;
+  if (sys->shear_mode != 0)
+    {
+      *shear_rate = scalar_double (nc, "shear_rate", "stokes_nc_get_params");
+      sys->shear_rate = *shear_rate;
+    }
 }

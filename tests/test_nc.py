@@ -329,3 +329,202 @@ def test_reader_and_append(tmp_path):
     assert np.array_equal(out, np.arange(6.0) + 10)
     assert L.stokes_nc_get_time_step(r, 1) == 2.0
     L.stokes_nc_free(r)
+
+
+# ---- BD parameter files (stokes_nc_set_by_params and its readers) ----
+
+class _Params:
+    """the arguments of stokes_nc_set_by_params for one system"""
+
+    def __init__(self, version, np_, nm, poly, periodic, shear_mode, seed):
+        r = np.random.default_rng(seed)
+        nf = np_ - nm
+        self.version, self.np, self.nm, self.nf = version, np_, nm, nf
+        self.a = r.uniform(0.5, 2.0, np_) if poly else None
+        self.periodic = periodic
+        self.Ui, self.Oi, self.Ei = r.normal(size=3), r.normal(size=3), \
+            r.normal(size=5)
+        self.F, self.T, self.E = r.normal(size=3 * nm), \
+            r.normal(size=3 * nm), r.normal(size=5 * nm)
+        self.uf, self.of, self.ef = r.normal(size=3 * max(nf, 1)), \
+            r.normal(size=3 * max(nf, 1)), r.normal(size=5 * max(nf, 1))
+        self.xf = r.normal(size=3 * max(nf, 1))
+        self.lat = np.array([10.0, 11.0, 12.0]) if periodic else np.zeros(3)
+        self.shear_mode = shear_mode
+        self.shear_rate = 0.75 if shear_mode else 0.0
+
+    def system(self):
+        s = _new_sys(self.version, self.np, self.nm)
+        st = s.contents
+        if self.a is not None:
+            ls.lib().stokes_set_radius(s, _p(self.a))
+        if self.periodic:
+            ls.lib().stokes_set_l(s, *self.lat)
+            st.periodic = 1
+        st.shear_mode = self.shear_mode
+        st.shear_rate = self.shear_rate
+        return s
+
+    def args(self):
+        return [_p(v) for v in (self.Ui, self.Oi, self.Ei, self.F, self.T,
+                                self.E, self.uf, self.of, self.ef, self.xf,
+                                self.lat)]
+
+
+def _new_sys(version, np_, nm):
+    L = ls.lib()
+    s = L.stokes_init()
+    s.contents.version = version
+    L.stokes_set_np(s, np_, nm)
+    return s
+
+
+def _param_lib():
+    L = _lib()
+    P = C.POINTER(C.c_double)
+    L.stokes_nc_set_by_params.restype = C.c_void_p
+    L.stokes_nc_set_by_params.argtypes = ([C.c_char_p, C.c_void_p, C.c_int]
+                                          + [P] * 11
+                                          + [C.c_int, C.c_double, C.c_int])
+    L.stokes_nc_check_params.restype = C.c_int
+    L.stokes_nc_check_params.argtypes = ([C.c_void_p, C.c_void_p, C.c_int]
+                                         + [P] * 11 + [C.c_double])
+    L.stokes_nc_get_version.restype = C.c_int
+    L.stokes_nc_get_version.argtypes = [C.c_void_p]
+    L.stokes_nc_get_params.argtypes = ([C.c_void_p, C.c_void_p,
+                                        C.POINTER(C.c_int)] + [P] * 11
+                                       + [C.POINTER(C.c_int), P,
+                                          C.POINTER(C.c_int)])
+    L.stokes_nc_open.restype = C.c_void_p
+    L.stokes_nc_open.argtypes = [C.c_char_p]
+    return L
+
+
+CASES = [(0, 6, 6, False, True, 0), (1, 7, 5, True, True, 0),
+         (2, 5, 5, True, False, 0), (2, 8, 6, False, True, 1),
+         (1, 4, 4, False, True, 2), (0, 9, 3, True, False, 0)]
+
+
+@pytest.mark.parametrize("version,np_,nm,poly,periodic,shear", CASES)
+def test_params_file_round_trip(tmp_path, version, np_, nm, poly, periodic,
+                                shear):
+    """stokes_nc_set_by_params writes the BD parameters with the constant
+    imposed flow and the flag table of stokes_nc_init (stokes-nc.c:
+    2731-2841); scipy reads the entries back bit for bit; the library's
+    stokes_nc_check_params (stokes-nc.c:2869-3188) finds no mismatch, and
+    one for each entry changed; stokes_nc_get_version and
+    stokes_nc_get_params (stokes-nc.c:3191-3540) return the version, sizes,
+    radii, lattice, shear and every vector written"""
+    L = _param_lib()
+    prm = _Params(version, np_, nm, poly, periodic, shear, seed=np_ + nm)
+    sys = prm.system()
+    fn = str(tmp_path / "params.nc").encode()
+    nc = L.stokes_nc_set_by_params(fn, C.cast(sys, C.c_void_p), 1, *prm.args(),
+                                   shear, prm.shear_rate, 0)
+    assert nc
+    L.stokes_nc_free(nc)
+    nf = np_ - nm
+    with netcdf_file(fn.decode(), "r", mmap=False) as f:
+        v = f.variables
+        assert np.array_equal(v["Ui0"][:], prm.Ui)
+        assert np.array_equal(v["Oi0"][:], prm.Oi)
+        assert np.array_equal(v["Ei0"][:], prm.Ei)
+        assert np.array_equal(v["F0"][:].reshape(-1), prm.F)
+        assert ("T0" in v) == (version >= 1) and ("E0" in v) == (version == 2)
+        if version >= 1:
+            assert np.array_equal(v["T0"][:].reshape(-1), prm.T)
+        if version == 2:
+            assert np.array_equal(v["E0"][:].reshape(-1), prm.E)
+        assert np.array_equal(v["l"][:], prm.lat)
+        assert "q" in v and "Ui" not in v
+        if poly:
+            assert np.array_equal(v["a"][:], prm.a[:nm])
+        if nf:
+            assert np.array_equal(v["xf0"][:].reshape(-1), prm.xf[:3 * nf])
+            assert np.array_equal(v["Uf0"][:].reshape(-1), prm.uf[:3 * nf])
+            if poly:
+                assert np.array_equal(v["af"][:], prm.a[nm:])
+        assert int(v["shear_mode"].getValue()) == shear
+        if shear:
+            assert float(v["shear_rate"].getValue()) == prm.shear_rate
+    nc = L.stokes_nc_open(fn)
+    sp = C.cast(sys, C.c_void_p)
+    assert L.stokes_nc_check_params(nc, sp, 1, *prm.args(), 1e-12) == 0
+    assert L.stokes_nc_get_version(nc) == version
+    f0, e4 = prm.F[0], prm.Ei[4]
+    prm.F[0] += 1.0
+    assert L.stokes_nc_check_params(nc, sp, 1, *prm.args(), 1e-12) == 1
+    prm.F[0] = f0
+    prm.Ei[4] += 1e-3
+    assert L.stokes_nc_check_params(nc, sp, 1, *prm.args(), 1e-12) == 1
+    assert L.stokes_nc_check_params(nc, sp, 1, *prm.args(), 1e-2) == 0
+    prm.Ei[4] = e4
+    # read everything back into a fresh system and fresh arrays
+    got = _new_sys(0, 1, 1)
+    q, sm, rng = C.c_int(), C.c_int(), C.c_int()
+    sr = C.c_double()
+    out = _Params(version, np_, nm, poly, periodic, shear, seed=99)
+    for name in ("Ui", "Oi", "Ei", "F", "T", "E", "uf", "of", "ef", "xf",
+                 "lat"):
+        setattr(out, name, np.zeros_like(getattr(out, name)))
+    L.stokes_nc_get_params(nc, C.cast(got, C.c_void_p), C.byref(q),
+                           *out.args(), C.byref(sm), C.byref(sr),
+                           C.byref(rng))
+    g = got.contents
+    assert (g.version, g.np, g.nm, g.periodic) == (version, np_, nm,
+                                                  int(periodic))
+    assert q.value == 1 and rng.value == 0 and sm.value == shear
+    assert g.shear_mode == shear
+    if shear:
+        assert sr.value == prm.shear_rate and g.shear_rate == prm.shear_rate
+    if periodic:
+        assert (g.lx, g.ly, g.lz) == tuple(prm.lat)
+    if poly:
+        assert np.array_equal(np.ctypeslib.as_array(g.a, (np_,)), prm.a)
+    else:
+        assert not g.a
+    for name, n in (("Ui", 3), ("Oi", 3), ("Ei", 5), ("F", 3 * nm),
+                    ("xf", 3 * nf), ("uf", 3 * nf)):
+        assert np.array_equal(getattr(out, name)[:n], getattr(prm, name)[:n])
+    if version >= 1:
+        assert np.array_equal(out.T, prm.T)
+    if version == 2:
+        assert np.array_equal(out.E, prm.E)
+    L.stokes_nc_free(nc)
+    ls.lib().stokes_free(got)
+    ls.lib().stokes_free(sys)
+
+
+def test_print_actives(tmp_path):
+    """stokes_nc_print_actives (stokes-nc.c:52-115): the table of the defined
+    entries, the shear mode and the random-generator flag"""
+    L = _lib()
+    L.stokes_nc_print_actives.argtypes = [C.c_void_p, C.c_void_p]
+    libc = C.CDLL(None)
+    libc.fopen.restype = C.c_void_p
+    libc.fopen.argtypes = [C.c_char_p, C.c_char_p]
+    libc.fclose.argtypes = [C.c_void_p]
+    fn = str(tmp_path / "act.nc").encode()
+    txt = str(tmp_path / "act.txt").encode()
+    nc = L.stokes_nc_init(fn, 4, 2, 1, 1, 1, 0, 2, 1)
+    out = libc.fopen(txt, b"w")
+    L.stokes_nc_print_actives(nc, out)
+    libc.fclose(out)
+    L.stokes_nc_free(nc)
+    text = open(txt.decode()).read().splitlines()
+    assert text == [
+        "    | x | u o e | f t s | ui oi ei",
+        "----+---+-------+-------+----------",
+        "x0  | 0 | 0 0 0 | 1 1 0 | 1  1  1",
+        "xf0 | 1 | 1 1 0 | 0 0 0 |",
+        "----+---+-------+-------+----------",
+        "x   | 1 | 1 1 0 | 0 0 0 | 0  0  0",
+        "xf  | 0 | 0 0 0 | 1 1 0 |",
+        "----+---+-------+-------+----------",
+        "q   | 1 |",
+        "a   | 1 |",
+        "af  | 1 |",
+        "----+---+-------+------------------",
+        "shear_mode      | 2",
+        "----------------+------------------",
+        "states of RNG   | 1"]
