@@ -475,14 +475,21 @@ def bench_dense_build(args, pos, L, posd, log):
     s.close()
     err = float(np.linalg.norm(ym - yf) / np.linalg.norm(yf))
     nbytes = 8.0 * nd * nd
-    # algorithmic work at the splitting kept: every pair image inside r_max
-    # (300 flop) and, per (i, j, k), the even and odd 11 x 11 k-blocks
-    # scaled by cos and sin (4 x 121 flop)
+    # algorithmic work of the symmetric build at the splitting kept: each
+    # unordered pair image inside r_max once (300 flop; the images of (j, i)
+    # are those of (i, j) reflected) and, per unordered pair (i <= j) and k,
+    # cos and sin of k.(x_j - x_i) from the phase table (6 flop) and the 68
+    # moments accumulated (136 flop); the 11 x 11 blocks are a fixed linear
+    # map of the moments, written twice
     sh = ls.Stokes(2, pos, box=(L, L, L), tr=tr)
     pin, _ = pair_images_in_cutoff(pos, sh)
+    rl, _ = sh.image_table()
+    r2 = np.einsum("ij,ij->i", rl, rl)
+    pself = n * int(np.count_nonzero((r2 > 0.0) & (r2 <= sh.s.rmax2)))
     nk = half_space_k(sh)
     sh.close()
-    flop = pin * FLOP_PER_IMAGE_FTS + float(n) * n * nk * 4 * 121
+    upairs = (pin - pself) / 2 + pself
+    flop = upairs * FLOP_PER_IMAGE_FTS + n * (n + 1) / 2 * nk * 142.0
     achieved = flop / t / 1e12
     mat.free()
     return {"workload": f"dense FP64 FTS mobility matrix N={n} phi={args.phi} "
@@ -494,10 +501,11 @@ def bench_dense_build(args, pos, L, posd, log):
             "roofline": {"bound": "fp64", "achieved": achieved,
                          "peak": PEAK_FP64, "unit": "TFLOP/s",
                          "frac": achieved / PEAK_FP64, "traffic": None,
-                         "kernel": "k_matrix<2, 0> (FP64 vector: pair images "
-                                   "x 300 + N^2 nk x 484 flop; the 1.015 GB "
+                         "kernel": "k_matrix_sym<2> (FP64 vector, pairs i <= j "
+                                   "once: unordered pair images x 300 + "
+                                   "N (N + 1) / 2 nk x 142 flop; the 1.015 GB "
                                    "write alone would take 0.13 ms at 8 TB/s)",
-                         "pair_images": pin, "nk": nk}}
+                         "unordered_pair_images": upairs, "nk": nk}}
 
 
 def half_space_k(s):

@@ -858,6 +858,103 @@ template <int V> __device__ __forceinline__ float moment_parity (int t)
   return ((t >= M_SB && t < M_SC) || (t >= M_SG1 && t < M_SH)) ? -1.f : 1.f;
 }
 
+// the workgroup's copy of the lattice candidates and, per chunk of 64
+// candidates and wrap offset (w1, w2, w3) in {-1, 0, 1}^3, the bit mask of
+// those whose reference image m - w lies in the table (every thread calls
+// it; a barrier before use)
+template <typename T4>
+__device__ __forceinline__ void
+stage_candidates (const RealParams &p, const T4 *__restrict__ cand, const int4 *__restrict__ candm,
+                  const unsigned *__restrict__ tbits, T4 *cand_l,
+                  unsigned long long (*tmask_l)[27])
+{
+  for (int c = threadIdx.x; c < p.ncand; c += blockDim.x) cand_l[c] = cand[c];
+  for (int e = threadIdx.x; e < ((p.ncand + 63) >> 6) * 27; e += blockDim.x)
+    {
+      const int ch = e / 27, w = e - 27 * ch;
+      const int a = w / 9 - 1, b = (w / 3) % 3 - 1, c3 = w % 3 - 1;
+      const int cn = min (64, p.ncand - 64 * ch);
+      unsigned long long m = 0ull;
+      for (int c = 0; c < cn; c++)
+        {
+          const int4 mm = candm[64 * ch + c];
+          if (in_table (p, tbits, mm.x - a, mm.y - b, mm.z - c3)) m |= 1ull << c;
+        }
+      tmask_l[ch][w] = m;
+    }
+}
+
+// M += the real-space moments of the equal-sphere pair (pi, pj) summed over
+// its images inside r_max (the reference's table, candidates staged by
+// stage_candidates)
+template <int V, typename T>
+__device__ __forceinline__ void
+pair_real_moments (const double4 pi, const double4 pj, const RealParams &p,
+                   const typename Vec4<T>::type *cand_l, const int4 *__restrict__ candm,
+                   const unsigned *__restrict__ tbits, const unsigned long long (*tmask_l)[27],
+                   T *M)
+{
+  using T4 = typename Vec4<T>::type;
+  const T ai = T (pi.w), aj = T (pj.w), xi = T (p.xi);
+  const T rmax2 = T (p.rmax2);
+  double dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
+  const double n3 = rint (dz / p.lz);
+  dx -= n3 * p.s3x; dz -= n3 * p.lz;
+  const double n2 = rint (dy / p.ly);
+  dx -= n2 * p.s2x; dy -= n2 * p.ly;
+  const double n1 = rint (dx / p.lx);
+  dx -= n1 * p.lx;
+  const int w1 = int (n1), w2 = int (n2), w3 = int (n3);
+  const int wtab = (abs (w1) <= 1 && abs (w2) <= 1 && abs (w3) <= 1)
+    ? (w1 + 1) * 9 + (w2 + 1) * 3 + (w3 + 1) : -1;
+  const T mdx = T (dx), mdy = T (dy), mdz = T (dz);
+  const T rmn = (ai + aj) * T (p.rmin);
+  for (int c0 = 0; c0 < p.ncand; c0 += 64)
+    {
+      const int cn = min (64, p.ncand - c0);
+      unsigned long long mask = 0ull;
+#pragma unroll 4
+      for (int c = 0; c < cn; c++)
+        {
+          const T4 v = cand_l[c0 + c];
+          const T ex = mdx + v.x, ey = mdy + v.y, ez = mdz + v.z;
+          const T rr = ex * ex + ey * ey + ez * ez;
+          mask |= (unsigned long long) (rr != T (0) && rr <= rmax2) << c;
+        }
+      if (wtab >= 0)
+        mask &= tmask_l[c0 >> 6][wtab];
+      else
+        {
+          unsigned long long m2 = mask;
+          while (m2)
+            {
+              const int c = __builtin_ctzll (m2);
+              m2 &= m2 - 1ull;
+              const int4 mm = candm[c0 + c];
+              if (!in_table (p, tbits, mm.x - w1, mm.y - w2, mm.z - w3)) mask &= ~(1ull << c);
+            }
+        }
+      while (mask)
+        {
+          const int c = __builtin_ctzll (mask);
+          mask &= mask - 1ull;
+          const T4 v = cand_l[c0 + c];
+          const T ex = mdx + v.x, ey = mdy + v.y, ez = mdz + v.z;
+          const T rr = ex * ex + ey * ey + ez * ez;
+          T iu = m_rsqrt (rr);
+          T r = rr * iu;
+          if (r < rmn)
+            {
+              r = rmn;
+              iu = T (1) / rmn;
+            }
+          T sc[11];
+          scalars_real_u<V, T> (xi, r, iu, sc);
+          moments_add<V, T> (M, ex * iu, ey * iu, ez * iu, sc);
+        }
+    }
+}
+
 // workgroup: 16 i x 16 j (four waves of 16 i x 4 j); the workgroups with
 // every j below every i return at once
 template <int V, typename T>
@@ -872,20 +969,7 @@ k_real_moments (const double4 *__restrict__ pos, const typename Vec4<T>::type *_
   __shared__ T4 cand_l[REAL_PC_CAND];
   __shared__ unsigned long long tmask_l[REAL_PC_CAND / 64][27];
   __shared__ T tr_l[17][16 * 17];
-  for (int c = threadIdx.x; c < p.ncand; c += 256) cand_l[c] = cand[c];
-  for (int e = threadIdx.x; e < ((p.ncand + 63) >> 6) * 27; e += 256)
-    {
-      const int ch = e / 27, w = e - 27 * ch;
-      const int a = w / 9 - 1, b = (w / 3) % 3 - 1, c3 = w % 3 - 1;
-      const int cn = min (64, p.ncand - 64 * ch);
-      unsigned long long m = 0ull;
-      for (int c = 0; c < cn; c++)
-        {
-          const int4 mm = candm[64 * ch + c];
-          if (in_table (p, tbits, mm.x - a, mm.y - b, mm.z - c3)) m |= 1ull << c;
-        }
-      tmask_l[ch][w] = m;
-    }
+  stage_candidates<T4> (p, cand, candm, tbits, cand_l, tmask_l);
   __syncthreads ();
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int i = blockIdx.x * 16 + (lane & 15);
@@ -894,68 +978,7 @@ k_real_moments (const double4 *__restrict__ pos, const typename Vec4<T>::type *_
   T M[NM];
 #pragma unroll
   for (int t = 0; t < NM; t++) M[t] = T (0);
-  if (valid)
-    {
-      const double4 pi = pos[i], pj = pos[j];
-      const T ai = T (pi.w), aj = T (pj.w), xi = T (p.xi);
-      const T rmax2 = T (p.rmax2);
-      double dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
-      const double n3 = rint (dz / p.lz);
-      dx -= n3 * p.s3x; dz -= n3 * p.lz;
-      const double n2 = rint (dy / p.ly);
-      dx -= n2 * p.s2x; dy -= n2 * p.ly;
-      const double n1 = rint (dx / p.lx);
-      dx -= n1 * p.lx;
-      const int w1 = int (n1), w2 = int (n2), w3 = int (n3);
-      const int wtab = (abs (w1) <= 1 && abs (w2) <= 1 && abs (w3) <= 1)
-        ? (w1 + 1) * 9 + (w2 + 1) * 3 + (w3 + 1) : -1;
-      const T mdx = T (dx), mdy = T (dy), mdz = T (dz);
-      const T rmn = (ai + aj) * T (p.rmin);
-      for (int c0 = 0; c0 < p.ncand; c0 += 64)
-        {
-          const int cn = min (64, p.ncand - c0);
-          unsigned long long mask = 0ull;
-#pragma unroll 4
-          for (int c = 0; c < cn; c++)
-            {
-              const T4 v = cand_l[c0 + c];
-              const T ex = mdx + v.x, ey = mdy + v.y, ez = mdz + v.z;
-              const T rr = ex * ex + ey * ey + ez * ez;
-              mask |= (unsigned long long) (rr != T (0) && rr <= rmax2) << c;
-            }
-          if (wtab >= 0)
-            mask &= tmask_l[c0 >> 6][wtab];
-          else
-            {
-              unsigned long long m2 = mask;
-              while (m2)
-                {
-                  const int c = __builtin_ctzll (m2);
-                  m2 &= m2 - 1ull;
-                  const int4 mm = candm[c0 + c];
-                  if (!in_table (p, tbits, mm.x - w1, mm.y - w2, mm.z - w3)) mask &= ~(1ull << c);
-                }
-            }
-          while (mask)
-            {
-              const int c = __builtin_ctzll (mask);
-              mask &= mask - 1ull;
-              const T4 v = cand_l[c0 + c];
-              const T ex = mdx + v.x, ey = mdy + v.y, ez = mdz + v.z;
-              const T rr = ex * ex + ey * ey + ez * ez;
-              T iu = m_rsqrt (rr);
-              T r = rr * iu;
-              if (r < rmn)
-                {
-                  r = rmn;
-                  iu = T (1) / rmn;
-                }
-              T sc[11];
-              scalars_real_u<V, T> (xi, r, iu, sc);
-              moments_add<V, T> (M, ex * iu, ey * iu, ez * iu, sc);
-            }
-        }
-    }
+  if (valid) pair_real_moments<V, T> (pos[i], pos[j], p, cand_l, candm, tbits, tmask_l, M);
   // (i, j) for i <= j (zeros for the padding pairs), coalesced
   const size_t ms = (size_t) npad * npad;
   if (i <= j)

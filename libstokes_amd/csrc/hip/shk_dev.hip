@@ -19,6 +19,7 @@
 #include "adj_kernels.h"
 #include "lub_kernels.h"
 #include "dgemm_kernels.h"
+#include "matrix_kernels.h"
 
 using namespace shk;
 
@@ -1537,7 +1538,38 @@ static int matrix_v (shk_dev *d, double *mat)
   DevParams q = dev_params (p);
   dim3 grid ((p.np + 63) / 64, p.np);
   if (d->timing) HIPCHK (hipEventRecord (d->ev[0], d->stream));
-  if (p.periodic)
+  if (p.periodic && !p.poly && d->tsym && d->ncand <= REAL_PC_CAND)
+    {
+      // each pair i <= j once: real-space moments over the candidates, the
+      // k sum from the phase table and the per-k moment weights, both
+      // blocks written
+      const int nk = p.nk, np = p.np;
+      int rc;
+      if ((rc = ensure (d, d->eph, sizeof (double) * 2 * (size_t) (nk > 0 ? nk : 1) * np)))
+        return rc;
+      if ((rc = ensure (d, d->kpart, sizeof (double) * Dim<V>::NMOM * (size_t) (nk > 0 ? nk : 1))))
+        return rc;
+      const double4 *pos4 = (const double4 *) d->pos.p;
+      const double4 *kv = (const double4 *) d->kv.p;
+      if (nk > 0)
+        {
+          hipLaunchKernelGGL ((k_phase_E<false>), dim3 ((unsigned) (((size_t) nk * np + 255) / 256)),
+                              dim3 (256), 0, d->stream, pos4, kv, nk, np, (double *) d->eph.p);
+          hipLaunchKernelGGL ((k_kmom_weights<V>), dim3 ((nk + 127) / 128), dim3 (128), 0,
+                              d->stream, kv, (const double *) d->ks.p, nk, (double *) d->kpart.p);
+        }
+      RealParams rp;
+      rp.np = p.np; rp.ncand = d->ncand; rp.ncore = d->ncore; rp.poly = 0;
+      rp.xi = p.xi; rp.rmax2 = p.rmax2; rp.rmin = p.rmin;
+      rp.lx = p.lx; rp.ly = p.ly; rp.lz = p.lz; rp.s2x = d->s2x; rp.s3x = d->s3x;
+      rp.tx = d->tx; rp.ty = d->ty; rp.tz = d->tz; rp.tbits_words = d->tbits_words;
+      const int nt = (np + 15) / 16;
+      hipLaunchKernelGGL ((k_matrix_sym<V>), dim3 (nt, nt), dim3 (256), 0, d->stream, pos4,
+                          (const double4 *) d->cand.p, (const int4 *) d->candm.p,
+                          (const unsigned *) d->tbits.p, rp, q, (const double *) d->eph.p,
+                          (const double *) d->kpart.p, nk, mat);
+    }
+  else if (p.periodic)
     hipLaunchKernelGGL ((k_matrix<V, 0>), grid, dim3 (64), 0, d->stream,
                         (const double4 *) d->pos.p, (const double4 *) d->img.p,
                         (const double4 *) d->kv.p, (const double *) d->ks.p, q, mat);
