@@ -884,6 +884,88 @@ stage_candidates (const RealParams &p, const T4 *__restrict__ cand, const int4 *
     }
 }
 
+// the separation of the equal-sphere pair (pi, pj) wrapped into the primary
+// cell, as the reference's minimum image, and the index of its wrap offset
+// among the 27 reference-table masks of stage_candidates (-1 beyond them)
+struct PairSep {
+  double dx, dy, dz;
+  int w1, w2, w3, wtab;
+};
+
+__device__ __forceinline__ PairSep pair_sep (const double4 pi, const double4 pj,
+                                             const RealParams &p)
+{
+  PairSep s;
+  double dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
+  const double n3 = rint (dz / p.lz);
+  dx -= n3 * p.s3x; dz -= n3 * p.lz;
+  const double n2 = rint (dy / p.ly);
+  dx -= n2 * p.s2x; dy -= n2 * p.ly;
+  const double n1 = rint (dx / p.lx);
+  dx -= n1 * p.lx;
+  s.dx = dx; s.dy = dy; s.dz = dz;
+  s.w1 = int (n1); s.w2 = int (n2); s.w3 = int (n3);
+  s.wtab = (abs (s.w1) <= 1 && abs (s.w2) <= 1 && abs (s.w3) <= 1)
+    ? (s.w1 + 1) * 9 + (s.w2 + 1) * 3 + (s.w3 + 1) : -1;
+  return s;
+}
+
+// bit c set: candidate c0 + c is an image of the pair inside r_max (r > 0)
+// that the reference's table holds
+template <typename T>
+__device__ __forceinline__ unsigned long long
+pair_chunk_mask (const PairSep &s, T mdx, T mdy, T mdz, int c0, const RealParams &p,
+                 const typename Vec4<T>::type *cand_l, const int4 *__restrict__ candm,
+                 const unsigned *__restrict__ tbits, const unsigned long long (*tmask_l)[27])
+{
+  using T4 = typename Vec4<T>::type;
+  const T rmax2 = T (p.rmax2);
+  const int cn = min (64, p.ncand - c0);
+  unsigned long long mask = 0ull;
+#pragma unroll 4
+  for (int c = 0; c < cn; c++)
+    {
+      const T4 v = cand_l[c0 + c];
+      const T ex = mdx + v.x, ey = mdy + v.y, ez = mdz + v.z;
+      const T rr = ex * ex + ey * ey + ez * ez;
+      mask |= (unsigned long long) (rr != T (0) && rr <= rmax2) << c;
+    }
+  if (s.wtab >= 0)
+    mask &= tmask_l[c0 >> 6][s.wtab];
+  else
+    {
+      unsigned long long m2 = mask;
+      while (m2)
+        {
+          const int c = __builtin_ctzll (m2);
+          m2 &= m2 - 1ull;
+          const int4 mm = candm[c0 + c];
+          if (!in_table (p, tbits, mm.x - s.w1, mm.y - s.w2, mm.z - s.w3)) mask &= ~(1ull << c);
+        }
+    }
+  return mask;
+}
+
+// M += the moments of one image (separation (mdx, mdy, mdz) + v, clamped
+// at rmn like the reference's minimum distance)
+template <int V, typename T>
+__device__ __forceinline__ void
+image_moments_add (T mdx, T mdy, T mdz, const typename Vec4<T>::type v, T xi, T rmn, T *M)
+{
+  const T ex = mdx + v.x, ey = mdy + v.y, ez = mdz + v.z;
+  const T rr = ex * ex + ey * ey + ez * ez;
+  T iu = m_rsqrt (rr);
+  T r = rr * iu;
+  if (r < rmn)
+    {
+      r = rmn;
+      iu = T (1) / rmn;
+    }
+  T sc[11];
+  scalars_real_u<V, T> (xi, r, iu, sc);
+  moments_add<V, T> (M, ex * iu, ey * iu, ez * iu, sc);
+}
+
 // M += the real-space moments of the equal-sphere pair (pi, pj) summed over
 // its images inside r_max (the reference's table, candidates staged by
 // stage_candidates)
@@ -894,69 +976,34 @@ pair_real_moments (const double4 pi, const double4 pj, const RealParams &p,
                    const unsigned *__restrict__ tbits, const unsigned long long (*tmask_l)[27],
                    T *M)
 {
-  using T4 = typename Vec4<T>::type;
-  const T ai = T (pi.w), aj = T (pj.w), xi = T (p.xi);
-  const T rmax2 = T (p.rmax2);
-  double dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
-  const double n3 = rint (dz / p.lz);
-  dx -= n3 * p.s3x; dz -= n3 * p.lz;
-  const double n2 = rint (dy / p.ly);
-  dx -= n2 * p.s2x; dy -= n2 * p.ly;
-  const double n1 = rint (dx / p.lx);
-  dx -= n1 * p.lx;
-  const int w1 = int (n1), w2 = int (n2), w3 = int (n3);
-  const int wtab = (abs (w1) <= 1 && abs (w2) <= 1 && abs (w3) <= 1)
-    ? (w1 + 1) * 9 + (w2 + 1) * 3 + (w3 + 1) : -1;
-  const T mdx = T (dx), mdy = T (dy), mdz = T (dz);
-  const T rmn = (ai + aj) * T (p.rmin);
+  const PairSep s = pair_sep (pi, pj, p);
+  const T mdx = T (s.dx), mdy = T (s.dy), mdz = T (s.dz);
+  const T xi = T (p.xi), rmn = (T (pi.w) + T (pj.w)) * T (p.rmin);
   for (int c0 = 0; c0 < p.ncand; c0 += 64)
     {
-      const int cn = min (64, p.ncand - c0);
-      unsigned long long mask = 0ull;
-#pragma unroll 4
-      for (int c = 0; c < cn; c++)
-        {
-          const T4 v = cand_l[c0 + c];
-          const T ex = mdx + v.x, ey = mdy + v.y, ez = mdz + v.z;
-          const T rr = ex * ex + ey * ey + ez * ez;
-          mask |= (unsigned long long) (rr != T (0) && rr <= rmax2) << c;
-        }
-      if (wtab >= 0)
-        mask &= tmask_l[c0 >> 6][wtab];
-      else
-        {
-          unsigned long long m2 = mask;
-          while (m2)
-            {
-              const int c = __builtin_ctzll (m2);
-              m2 &= m2 - 1ull;
-              const int4 mm = candm[c0 + c];
-              if (!in_table (p, tbits, mm.x - w1, mm.y - w2, mm.z - w3)) mask &= ~(1ull << c);
-            }
-        }
+      unsigned long long mask =
+        pair_chunk_mask<T> (s, mdx, mdy, mdz, c0, p, cand_l, candm, tbits, tmask_l);
       while (mask)
         {
           const int c = __builtin_ctzll (mask);
           mask &= mask - 1ull;
-          const T4 v = cand_l[c0 + c];
-          const T ex = mdx + v.x, ey = mdy + v.y, ez = mdz + v.z;
-          const T rr = ex * ex + ey * ey + ez * ez;
-          T iu = m_rsqrt (rr);
-          T r = rr * iu;
-          if (r < rmn)
-            {
-              r = rmn;
-              iu = T (1) / rmn;
-            }
-          T sc[11];
-          scalars_real_u<V, T> (xi, r, iu, sc);
-          moments_add<V, T> (M, ex * iu, ey * iu, ez * iu, sc);
+          image_moments_add<V, T> (mdx, mdy, mdz, cand_l[c0 + c], xi, rmn, M);
         }
     }
 }
 
-// workgroup: 16 i x 16 j (four waves of 16 i x 4 j); the workgroups with
-// every j below every i return at once
+// workgroup: the 16 i x 16 j pairs of one tile (blockIdx.x <= blockIdx.y;
+// the others return at once), in three phases.  (1) thread q = 16 jl + il
+// finds the images of the pair (16 bx + il, 16 by + jl) (one mask per 64
+// candidates, kept in LDS) and counts them; (2) the pairs are ranked by
+// count (a counting sort in LDS), so that the 64 lanes of a wave take pairs
+// of nearly equal image counts; (3) thread k evaluates the moments of the
+// k-th pair in that order, one loop over the set bits of all its masks.
+// Each pair's sum runs over its images in candidate order whichever lane
+// takes it, so the results do not depend on the ranking.  The moments go
+// out through LDS in rounds of MR: (i, j) to mt[m][i / 16][j][i % 16] and
+// (j, i), odd moments negated, to mt[m][j / 16][i][j % 16], both as 256
+// consecutive elements per tile and moment.
 template <int V, typename T>
 __global__ __launch_bounds__ (256) __attribute__ ((amdgpu_waves_per_eu (sizeof (T) == 4 ? 3 : 2))) void
 k_real_moments (const double4 *__restrict__ pos, const typename Vec4<T>::type *__restrict__ cand,
@@ -964,48 +1011,121 @@ k_real_moments (const double4 *__restrict__ pos, const typename Vec4<T>::type *_
                 int npad, T *__restrict__ mt)
 {
   constexpr int NM = Dim<V>::NMOM;
+  constexpr int NCH = REAL_PC_CAND / 64;
   using T4 = typename Vec4<T>::type;
   if (blockIdx.y < blockIdx.x) return;
   __shared__ T4 cand_l[REAL_PC_CAND];
-  __shared__ unsigned long long tmask_l[REAL_PC_CAND / 64][27];
-  __shared__ T tr_l[17][16 * 17];
+  __shared__ unsigned long long tmask_l[NCH][27];
+  __shared__ unsigned long long mk_l[NCH][256];
+  __shared__ int bin_l[REAL_PC_CAND + 2];
+  __shared__ unsigned short perm_l[256];
+  constexpr int MR = 17;
+  __shared__ T tr_l[MR][16 * 17];
   stage_candidates<T4> (p, cand, candm, tbits, cand_l, tmask_l);
+  for (int e = threadIdx.x; e < REAL_PC_CAND + 2; e += 256) bin_l[e] = 0;
   __syncthreads ();
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int i = blockIdx.x * 16 + (lane & 15);
-  const int j = blockIdx.y * 16 + 4 * wave + (lane >> 4);
-  const bool valid = i <= j && j < p.np;
+  const int nch = (p.ncand + 63) >> 6;
+  const int bx = blockIdx.x, by = blockIdx.y;
+  // (1) masks and image count of pair q = threadIdx.x
+  int slot, cnt = 0;
+  {
+    const int q = threadIdx.x, i = bx * 16 + (q & 15), j = by * 16 + (q >> 4);
+    const bool valid = i <= j && j < p.np;
+    PairSep s;
+    if (valid) s = pair_sep (pos[i], pos[j], p);
+    for (int ch = 0; ch < nch; ch++)
+      {
+        unsigned long long m = 0ull;
+        if (valid)
+          m = pair_chunk_mask<T> (s, T (s.dx), T (s.dy), T (s.dz), 64 * ch, p, cand_l, candm,
+                                  tbits, tmask_l);
+        mk_l[ch][q] = m;
+        cnt += __builtin_popcountll (m);
+      }
+    slot = atomicAdd (&bin_l[cnt + 1], 1);
+  }
+  __syncthreads ();
+  // (2) exclusive prefix over the counts (wave 0: five bins per lane)
+  if (threadIdx.x < 64)
+    {
+      constexpr int PB = (REAL_PC_CAND + 2 + 63) / 64;
+      const int b0 = threadIdx.x * PB;
+      int v[PB], sum = 0;
+#pragma unroll
+      for (int u = 0; u < PB; u++)
+        {
+          v[u] = (b0 + u < REAL_PC_CAND + 2) ? bin_l[b0 + u] : 0;
+          sum += v[u];
+        }
+      int incl = sum;
+#pragma unroll
+      for (int off = 1; off < 64; off <<= 1)
+        {
+          const int o = __shfl_up (incl, off, 64);
+          if ((int) threadIdx.x >= off) incl += o;
+        }
+      int run = incl - sum;
+#pragma unroll
+      for (int u = 0; u < PB; u++)
+        {
+          run += v[u];
+          if (b0 + u < REAL_PC_CAND + 2) bin_l[b0 + u] = run;
+        }
+    }
+  __syncthreads ();
+  // bin_l[c] now counts the pairs with fewer than c images
+  perm_l[bin_l[cnt] + slot] = (unsigned short) threadIdx.x;
+  __syncthreads ();
+  // (3) moments of the pair ranked threadIdx.x
+  const int q = perm_l[threadIdx.x], il = q & 15, jl = q >> 4;
+  const int i = bx * 16 + il, j = by * 16 + jl;
   T M[NM];
 #pragma unroll
   for (int t = 0; t < NM; t++) M[t] = T (0);
-  if (valid) pair_real_moments<V, T> (pos[i], pos[j], p, cand_l, candm, tbits, tmask_l, M);
-  // (i, j) for i <= j (zeros for the padding pairs), coalesced
-  const size_t ms = (size_t) npad * npad;
-  if (i <= j)
+  if (i <= j && j < p.np)
     {
-      T *o = mt + (size_t) (i >> 4) * npad * 16 + (size_t) j * 16 + (i & 15);
-#pragma unroll
-      for (int t = 0; t < NM; t++) o[t * ms] = M[t];
+      const double4 pi = pos[i], pj = pos[j];
+      const PairSep s = pair_sep (pi, pj, p);
+      const T mdx = T (s.dx), mdy = T (s.dy), mdz = T (s.dz);
+      const T xi = T (p.xi), rmn = (T (pi.w) + T (pj.w)) * T (p.rmin);
+      int ch = 0;
+      unsigned long long m = nch > 0 ? mk_l[0][q] : 0ull;
+      while (true)
+        {
+          while (m == 0ull && ++ch < nch) m = mk_l[ch][q];
+          if (m == 0ull) break;
+          const int c = 64 * ch + __builtin_ctzll (m);
+          m &= m - 1ull;
+          image_moments_add<V, T> (mdx, mdy, mdz, cand_l[c], xi, rmn, M);
+        }
     }
-  // (j, i) for i < j with the odd moments negated: this workgroup's 16 x 16
-  // block of them is 256 consecutive floats per moment (element il 16 + jl
-  // from the pair il, jl), staged through LDS in rounds of MR moments
-  constexpr int MR = 17;
-  const int il = lane & 15, jl = 4 * wave + (lane >> 4);
-  T *ob = mt + (size_t) blockIdx.y * npad * 16 + (size_t) blockIdx.x * 256;
-  const bool diag = blockIdx.x == blockIdx.y;
+  // out through LDS: (i, j) for i <= j (zeros for the padding pairs) and
+  // (j, i) for i < j with the odd moments negated
+  const size_t ms = (size_t) npad * npad;
+  T *od = mt + (size_t) bx * npad * 16 + (size_t) by * 256;
+  T *ob = mt + (size_t) by * npad * 16 + (size_t) bx * 256;
+  const bool diag = bx == by;
+  const int e = threadIdx.x;
+  // direct element e: pair (e % 16, e / 16); transposed element e: pair
+  // (e / 16, e % 16) seen from its j
+  const bool wd = !diag || (e & 15) <= (e >> 4);
+  const bool wt = !diag || (e >> 4) < (e & 15);
 #pragma unroll
   for (int t0 = 0; t0 < NM; t0 += MR)
     {
 #pragma unroll
       for (int u = 0; u < MR; u++)
-        if (t0 + u < NM) tr_l[u][il * 17 + jl] = T (moment_parity<V> (t0 + u)) * M[t0 + u];
+        if (t0 + u < NM) tr_l[u][il * 17 + jl] = M[t0 + u];
       __syncthreads ();
-      const int e = threadIdx.x, ei = e >> 4, ej = e & 15;
-      if (!diag || ei < ej)
 #pragma unroll
-        for (int u = 0; u < MR; u++)
-          if (t0 + u < NM) ob[(size_t) (t0 + u) * ms + e] = tr_l[u][ei * 17 + ej];
+      for (int u = 0; u < MR; u++)
+        if (t0 + u < NM)
+          {
+            if (wd) od[(size_t) (t0 + u) * ms + e] = tr_l[u][(e & 15) * 17 + (e >> 4)];
+            if (wt)
+              ob[(size_t) (t0 + u) * ms + e] =
+                T (moment_parity<V> (t0 + u)) * tr_l[u][(e >> 4) * 17 + (e & 15)];
+          }
       __syncthreads ();
     }
 }
