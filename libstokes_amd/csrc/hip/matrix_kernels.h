@@ -15,8 +15,9 @@
 //                   -sin(k.d) W_k[t] for odd t (moments_add is linear in
 //                   its scalars)
 //   k_matrix_sym    workgroup 16 i x 16 j (j-tile >= i-tile): per pair the
-//                   real-space moments (pair_real_moments over the staged
-//                   candidates), the self term on the diagonal, then the k
+//                   image masks over the staged candidates, then the
+//                   real-space moments (pair_images_add), the self term on
+//                   the diagonal, then the k
 //                   sum from the phase table E (cos and sin of k.x per
 //                   particle, k.d = k.x_j - k.x_i by two products) and W,
 //                   both staged in LDS 32 k at a time; blocks (i, j) and
@@ -70,19 +71,28 @@ k_matrix_sym (const double4 *__restrict__ pos, const double4 *__restrict__ cand,
   const int jl = threadIdx.x & 15, il = threadIdx.x >> 4;
   const int i = blockIdx.x * 16 + il, j = blockIdx.y * 16 + jl;
   const bool valid = i <= j && j < np;
+  // the image masks first, the moments after them
+  PairSep sp;
+  unsigned long long mk[REAL_NCH] = {};
+  double4 pi, pj;
+  if (valid)
+    {
+      pi = pos[i]; pj = pos[j];
+      sp = pair_sep (pi, pj, rp);
+      pair_masks<double> (sp, rp, cand_l, candm, tbits, tmask_l, mk);
+    }
   double M[NM];
 #pragma unroll
   for (int t = 0; t < NM; t++) M[t] = 0.0;
   if (valid)
     {
-      const double4 pi = pos[i], pj = pos[j];
       if (i == j)
         {
           double s[11];
           self_scalars<double> (q, pi.w, true, s);
           moments_add<V, double> (M, 0.0, 0.0, 0.0, s);
         }
-      pair_real_moments<V, double> (pi, pj, rp, cand_l, candm, tbits, tmask_l, M);
+      pair_images_add<V, double> (sp, rp.xi, (pi.w + pj.w) * rp.rmin, cand_l, mk, M);
     }
   for (int k0 = 0; k0 < nk; k0 += MS_KC)
     {

@@ -858,17 +858,30 @@ template <int V> __device__ __forceinline__ float moment_parity (int t)
   return ((t >= M_SB && t < M_SC) || (t >= M_SG1 && t < M_SH)) ? -1.f : 1.f;
 }
 
-// the workgroup's copy of the lattice candidates and, per chunk of 64
-// candidates and wrap offset (w1, w2, w3) in {-1, 0, 1}^3, the bit mask of
-// those whose reference image m - w lies in the table (every thread calls
-// it; a barrier before use)
+// the workgroup's copy of the lattice candidates v with |v|^2 / 2 in w,
+// padded to a multiple of 32 with vectors beyond reach, and, per chunk of
+// 64 candidates and wrap offset (w1, w2, w3) in {-1, 0, 1}^3, the bit mask
+// of those whose reference image m - w lies in the table (every thread
+// calls it; a barrier before use)
 template <typename T4>
 __device__ __forceinline__ void
 stage_candidates (const RealParams &p, const T4 *__restrict__ cand, const int4 *__restrict__ candm,
                   const unsigned *__restrict__ tbits, T4 *cand_l,
                   unsigned long long (*tmask_l)[27])
 {
-  for (int c = threadIdx.x; c < p.ncand; c += blockDim.x) cand_l[c] = cand[c];
+  const int n32 = (p.ncand + 31) & ~31;
+  for (int c = threadIdx.x; c < n32; c += blockDim.x)
+    {
+      T4 v;
+      if (c < p.ncand)
+        {
+          v = cand[c];
+          v.w = 0.5 * ((double) v.x * v.x + (double) v.y * v.y + (double) v.z * v.z);
+        }
+      else
+        v.x = v.w = 1e30, v.y = v.z = 0.0;
+      cand_l[c] = v;
+    }
   for (int e = threadIdx.x; e < ((p.ncand + 63) >> 6) * 27; e += blockDim.x)
     {
       const int ch = e / 27, w = e - 27 * ch;
@@ -910,8 +923,26 @@ __device__ __forceinline__ PairSep pair_sep (const double4 pi, const double4 pj,
   return s;
 }
 
+// 0 < t <= h for t, h >= 0 or t negative: one unsigned compare of the bit
+// patterns (t = +0 wraps to the largest pattern, a negative t has its sign
+// bit set; both compare false)
+__device__ __forceinline__ bool in_open_closed (float t, float h)
+{
+  return __float_as_uint (t) - 1u < __float_as_uint (h);
+}
+__device__ __forceinline__ bool in_open_closed (double t, double h)
+{
+  return (unsigned long long) __double_as_longlong (t) - 1ull
+         < (unsigned long long) __double_as_longlong (h);
+}
+
 // bit c set: candidate c0 + c is an image of the pair inside r_max (r > 0)
-// that the reference's table holds
+// that the reference's table holds.  With d the wrapped separation,
+// |d + v|^2 / 2 = d.v + |v|^2 / 2 + |d|^2 / 2 (three FMAs and an add per
+// candidate); for v = 0 it is |d|^2 / 2 itself, and for v != 0 the
+// wrapped d keeps |d + v| >= L / 2, so only images at r_max can round to
+// the other side.  32 candidates per unrolled group (the staged list is
+// padded to a multiple of 32).
 template <typename T>
 __device__ __forceinline__ unsigned long long
 pair_chunk_mask (const PairSep &s, T mdx, T mdy, T mdz, int c0, const RealParams &p,
@@ -919,16 +950,21 @@ pair_chunk_mask (const PairSep &s, T mdx, T mdy, T mdz, int c0, const RealParams
                  const unsigned *__restrict__ tbits, const unsigned long long (*tmask_l)[27])
 {
   using T4 = typename Vec4<T>::type;
-  const T rmax2 = T (p.rmax2);
+  const T h2 = T (0.5 * p.rmax2);
+  const T d2h = T (0.5) * (mdx * mdx + mdy * mdy + mdz * mdz);
   const int cn = min (64, p.ncand - c0);
   unsigned long long mask = 0ull;
-#pragma unroll 4
-  for (int c = 0; c < cn; c++)
+  for (int g = 0; g < cn; g += 32)
     {
-      const T4 v = cand_l[c0 + c];
-      const T ex = mdx + v.x, ey = mdy + v.y, ez = mdz + v.z;
-      const T rr = ex * ex + ey * ey + ez * ez;
-      mask |= (unsigned long long) (rr != T (0) && rr <= rmax2) << c;
+      unsigned bits = 0u;
+#pragma unroll
+      for (int b = 0; b < 32; b++)
+        {
+          const T4 v = cand_l[c0 + g + b];
+          const T t = fma (mdx, v.x, fma (mdy, v.y, fma (mdz, v.z, v.w + d2h)));
+          bits |= in_open_closed (t, h2) ? 1u << b : 0u;
+        }
+      mask |= (unsigned long long) bits << g;
     }
   if (s.wtab >= 0)
     mask &= tmask_l[c0 >> 6][s.wtab];
@@ -966,29 +1002,46 @@ image_moments_add (T mdx, T mdy, T mdz, const typename Vec4<T>::type v, T xi, T 
   moments_add<V, T> (M, ex * iu, ey * iu, ez * iu, sc);
 }
 
-// M += the real-space moments of the equal-sphere pair (pi, pj) summed over
-// its images inside r_max (the reference's table, candidates staged by
-// stage_candidates)
+constexpr int REAL_NCH = REAL_PC_CAND / 64;      // candidate chunks of 64
+
+// the image masks of the pair, one per chunk of 64 candidates (zero past
+// the last chunk)
+template <typename T>
+__device__ __forceinline__ void
+pair_masks (const PairSep &s, const RealParams &p, const typename Vec4<T>::type *cand_l,
+            const int4 *__restrict__ candm, const unsigned *__restrict__ tbits,
+            const unsigned long long (*tmask_l)[27], unsigned long long (&mk)[REAL_NCH])
+{
+#pragma unroll
+  for (int ch = 0; ch < REAL_NCH; ch++)
+    mk[ch] = 64 * ch < p.ncand
+      ? pair_chunk_mask<T> (s, T (s.dx), T (s.dy), T (s.dz), 64 * ch, p, cand_l, candm, tbits,
+                            tmask_l)
+      : 0ull;
+}
+
+// M += the moments of the images selected by the masks, in candidate order,
+// one loop over the set bits of all chunks
 template <int V, typename T>
 __device__ __forceinline__ void
-pair_real_moments (const double4 pi, const double4 pj, const RealParams &p,
-                   const typename Vec4<T>::type *cand_l, const int4 *__restrict__ candm,
-                   const unsigned *__restrict__ tbits, const unsigned long long (*tmask_l)[27],
-                   T *M)
+pair_images_add (const PairSep &s, T xi, T rmn, const typename Vec4<T>::type *cand_l,
+                 const unsigned long long (&mk)[REAL_NCH], T *M)
 {
-  const PairSep s = pair_sep (pi, pj, p);
+  static_assert (REAL_NCH == 4, "four candidate chunks");
   const T mdx = T (s.dx), mdy = T (s.dy), mdz = T (s.dz);
-  const T xi = T (p.xi), rmn = (T (pi.w) + T (pj.w)) * T (p.rmin);
-  for (int c0 = 0; c0 < p.ncand; c0 += 64)
+  unsigned long long m0 = mk[0], m1 = mk[1], m2 = mk[2], m3 = mk[3];
+  int base = 0;
+  while (true)
     {
-      unsigned long long mask =
-        pair_chunk_mask<T> (s, mdx, mdy, mdz, c0, p, cand_l, candm, tbits, tmask_l);
-      while (mask)
+      while (m0 == 0ull && (m1 | m2 | m3) != 0ull)
         {
-          const int c = __builtin_ctzll (mask);
-          mask &= mask - 1ull;
-          image_moments_add<V, T> (mdx, mdy, mdz, cand_l[c0 + c], xi, rmn, M);
+          m0 = m1; m1 = m2; m2 = m3; m3 = 0ull;
+          base += 64;
         }
+      if (m0 == 0ull) break;
+      const int c = base + __builtin_ctzll (m0);
+      m0 &= m0 - 1ull;
+      image_moments_add<V, T> (mdx, mdy, mdz, cand_l[c], xi, rmn, M);
     }
 }
 
@@ -1110,6 +1163,8 @@ k_real_moments (const double4 *__restrict__ pos, const typename Vec4<T>::type *_
   // (e / 16, e % 16) seen from its j
   const bool wd = !diag || (e & 15) <= (e >> 4);
   const bool wt = !diag || (e >> 4) < (e & 15);
+  T *pd = od + e, *pt = ob + e;
+  const T *sd = &tr_l[0][(e & 15) * 17 + (e >> 4)], *st = &tr_l[0][(e >> 4) * 17 + (e & 15)];
 #pragma unroll
   for (int t0 = 0; t0 < NM; t0 += MR)
     {
@@ -1117,15 +1172,14 @@ k_real_moments (const double4 *__restrict__ pos, const typename Vec4<T>::type *_
       for (int u = 0; u < MR; u++)
         if (t0 + u < NM) tr_l[u][il * 17 + jl] = M[t0 + u];
       __syncthreads ();
+      if (wd)
 #pragma unroll
-      for (int u = 0; u < MR; u++)
-        if (t0 + u < NM)
-          {
-            if (wd) od[(size_t) (t0 + u) * ms + e] = tr_l[u][(e & 15) * 17 + (e >> 4)];
-            if (wt)
-              ob[(size_t) (t0 + u) * ms + e] =
-                T (moment_parity<V> (t0 + u)) * tr_l[u][(e >> 4) * 17 + (e & 15)];
-          }
+        for (int u = 0; u < MR; u++)
+          if (t0 + u < NM) pd[(size_t) (t0 + u) * ms] = sd[u * 16 * 17];
+      if (wt)
+#pragma unroll
+        for (int u = 0; u < MR; u++)
+          if (t0 + u < NM) pt[(size_t) (t0 + u) * ms] = T (moment_parity<V> (t0 + u)) * st[u * 16 * 17];
       __syncthreads ();
     }
 }
