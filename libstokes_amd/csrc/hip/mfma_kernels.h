@@ -1186,40 +1186,42 @@ k_real_moments (const double4 *__restrict__ pos, const typename Vec4<T>::type *_
 
 // the block contraction of k_real_mfma_pc's consumer waves with the moments
 // read from mt: NC waves of 32 columns per workgroup share the moments of a
-// chunk through LDS (each wave loads every NC-th moment one chunk ahead; two
-// buffers, one barrier per chunk)
+// chunk of JC sources (8 for four waves, 4 for two: KS = JC / 4 MFMA
+// k-steps; two workgroups per CU either way) through LDS.  The next
+// chunk's moments go straight from mt to the other LDS buffer
+// (global_load_lds_dwordx4, 512 contiguous bytes per moment, no registers),
+// each k-step's z rows are reloaded as soon as its products have issued,
+// and one barrier per chunk hands the buffers over.  jper and j0 are
+// multiples of 8 and npad of 64, so a chunk never reads past row npad of
+// mt; its rows at or past j1 hold finite moments (zeros past np) and meet
+// zero z rows.
+
 template <int V, int NC>
 __global__ __launch_bounds__ (64 * NC) __attribute__ ((amdgpu_waves_per_eu (2))) void
 k_real_mfma_mom (const float *__restrict__ mt, int npad, const float *__restrict__ XT, int B,
                  int zs, int np, int jper, float *__restrict__ partial)
 {
   constexpr int D = Dim<V>::D, NM = Dim<V>::NMOM;
-  constexpr int TW = (NM + NC - 1) / NC;        // moments loaded per wave
-  __shared__ float mom[2][NM * 64];
+  constexpr int JC = NC >= 4 ? 8 : 4, KS = JC / 4;
+  constexpr int VPM = JC * 16 / 4;              // 16-byte vectors per moment and chunk
+  __shared__ __attribute__ ((aligned (16))) float mom[2][NM * JC * 16];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int ir = lane & 15, kq = lane >> 4;
   const int ibase = blockIdx.x * 16;
   const int j0 = blockIdx.y * jper, j1 = min (np, j0 + jper);
   const int cbase = blockIdx.z * 32 * NC + 32 * wave;
   const size_t ms = (size_t) npad * npad;
-  // moment t of pair (ibase + ir, jb + kq) at mb[t ms + 16 jb]
-  const float *mb = mt + (size_t) blockIdx.x * npad * 16 + lane;
-  float pre[TW];
-  auto load_mom = [&] (int jb) {
-    const bool jok = jb + kq < j1;
-#pragma unroll
-    for (int u = 0; u < TW; u++)
+  // moment t of the pair (ibase + i, j) at mrow[t ms + 16 j + i]
+  const float *mrow = mt + (size_t) blockIdx.x * npad * 16;
+  auto load_mom = [&] (int jb, int buf) {
+    const float *src = mrow + (size_t) jb * 16;
+    char *dst = (char *) mom[buf];
+    for (int v0 = wave * 64; v0 < NM * VPM; v0 += 64 * NC)
       {
-        const int t = wave + NC * u;
-        pre[u] = (t < NM && jok) ? mb[(size_t) t * ms + (size_t) jb * 16] : 0.f;
-      }
-  };
-  auto store_mom = [&] (int buf) {
-#pragma unroll
-    for (int u = 0; u < TW; u++)
-      {
-        const int t = wave + NC * u;
-        if (t < NM) mom[buf][t * 64 + lane] = pre[u];
+        const int v = v0 + lane;
+        if (v < NM * VPM)
+          __builtin_amdgcn_global_load_lds (src + (size_t) (v / VPM) * ms + 4 * (v % VPM),
+                                            (lds_void *) (dst + 16 * v0), 16, 0, 0);
       }
   };
   mf4 acc[D][2];
@@ -1227,10 +1229,10 @@ k_real_mfma_mom (const float *__restrict__ mt, int npad, const float *__restrict
   for (int t = 0; t < D; t++)
 #pragma unroll
     for (int ct = 0; ct < 2; ct++) acc[t][ct] = mf4{0.f, 0.f, 0.f, 0.f};
-  // z[j][q][cbase + 16 ct + ir] of the next chunk, loaded one chunk ahead
-  float bzn[D][2];
-  auto load_z = [&] (int jb) {
-    const int jz = jb + kq;
+  // z[j][q][cbase + 16 ct + ir] of k-step s (j = jb + 4 s + kq)
+  float bz[KS][D][2];
+  auto load_z = [&] (int jb, int s) {
+    const int jz = jb + 4 * s + kq;
 #pragma unroll
     for (int ct = 0; ct < 2; ct++)
       {
@@ -1238,42 +1240,37 @@ k_real_mfma_mom (const float *__restrict__ mt, int npad, const float *__restrict
         const bool ok = jz < j1 && b < B;
         const float *src = XT + (size_t) (ok ? jz : 0) * zs + (ok ? b : 0);
 #pragma unroll
-        for (int q = 0; q < D; q++) bzn[q][ct] = ok ? src[(size_t) q * B] : 0.f;
+        for (int q = 0; q < D; q++) bz[s][q][ct] = ok ? src[(size_t) q * B] : 0.f;
       }
   };
-  load_mom (j0);
-  load_z (j0);
-  store_mom (0);
-  __syncthreads ();
+  load_mom (j0, 0);
+#pragma unroll
+  for (int s = 0; s < KS; s++) load_z (j0, s);
+  async_wait_all ();
   int buf = 0;
-  for (int jb = j0; jb < j1; jb += 4, buf ^= 1)
+  for (int jb = j0; jb < j1; jb += JC, buf ^= 1)
     {
-      const bool more = jb + 4 < j1;
-      float bz[D][2];
+      const bool more = jb + JC < j1;
+      if (more) load_mom (jb + JC, buf ^ 1);
 #pragma unroll
-      for (int q = 0; q < D; q++)
-#pragma unroll
-        for (int ct = 0; ct < 2; ct++) bz[q][ct] = bzn[q][ct];
-      if (more)
+      for (int s = 0; s < KS; s++)
         {
-          load_mom (jb + 4);
-          load_z (jb + 4);
+          struct LdsM {
+            const float *b;
+            __device__ float operator[] (int t) const { return b[JC * 16 * t]; }
+          } Ml{mom[buf] + 64 * s + lane};
+#pragma unroll
+          for (int q = 0; q < D; q++)
+#pragma unroll
+            for (int r = 0; r < D; r++)
+              {
+                const float a = block_entry<V, float> (Ml, r, q, false);
+                acc[r][0] = mfma4 (a, bz[s][q][0], acc[r][0]);
+                acc[r][1] = mfma4 (a, bz[s][q][1], acc[r][1]);
+              }
+          if (more) load_z (jb + JC, s);
         }
-      struct LdsM {
-        const float *b;
-        __device__ float operator[] (int t) const { return b[64 * t]; }
-      } Ml{mom[buf] + lane};
-#pragma unroll
-      for (int q = 0; q < D; q++)
-#pragma unroll
-        for (int r = 0; r < D; r++)
-          {
-            const float a = block_entry<V, float> (Ml, r, q, false);
-            acc[r][0] = mfma4 (a, bz[q][0], acc[r][0]);
-            acc[r][1] = mfma4 (a, bz[q][1], acc[r][1]);
-          }
-      if (more) store_mom (buf ^ 1);
-      __syncthreads ();
+      async_wait_all ();
     }
 #pragma unroll
   for (int r = 0; r < D; r++)
