@@ -340,8 +340,14 @@ k_recip_back_block (const double4 *__restrict__ pos, const double4 *__restrict__
     }
 }
 
-// one thread per (i, q, b), b fastest: the partial chunks are read with
-// consecutive threads on consecutive addresses and summed in chunk order
+// y = self x + real partials + reciprocal partials, summed in that order
+// (partials in chunk order).  x and y hold the columns one after another
+// and the partials [chunk][i][q][b] have b fastest, so a workgroup takes a
+// tile of CB_I consecutive (i, q) rows and CB_B columns through LDS: the x
+// tile read along (i, q), the sums formed along b, y written along (i, q);
+// every global access is coalesced.
+constexpr int CB_I = 16, CB_B = 64;         // CB_I CB_B a multiple of 256
+
 template <int V, typename T>
 __global__ __launch_bounds__ (256) void
 k_combine_block (const double4 *__restrict__ pos, const double *__restrict__ x, int B,
@@ -349,17 +355,48 @@ k_combine_block (const double4 *__restrict__ pos, const double *__restrict__ x, 
                  int nrecip, DevParams p, int periodic, double *__restrict__ y)
 {
   constexpr int D = Dim<V>::D;
+  __shared__ double xt[CB_B][CB_I + 1];
+  __shared__ double yt[CB_I][CB_B + 1];
   const size_t n = (size_t) p.np * D;
-  const size_t t = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= n * B) return;
-  const int b = (int) (t % B);
-  const size_t iq = t / B;
-  const int i = (int) (iq / D), q = (int) (iq - (size_t) i * D);
+  const size_t iq0 = (size_t) blockIdx.x * CB_I;
+  const int b0 = blockIdx.y * CB_B;
   const size_t stride = n * B;
-  double acc = self_diag (p, pos[i].w, periodic != 0, V, q) * x[(size_t) b * n + iq];
-  for (int ch = 0; ch < nreal; ch++) acc += double (rpart[(size_t) ch * stride + t]);
-  for (int ch = 0; ch < nrecip; ch++) acc += double (kpart[(size_t) ch * stride + t]);
-  y[(size_t) b * n + iq] = acc;
+  for (int e = threadIdx.x; e < CB_I * CB_B; e += 256)
+    {
+      const int il = e % CB_I, bl = e / CB_I;
+      const size_t iq = iq0 + il;
+      const int b = b0 + bl;
+      xt[bl][il] = (iq < n && b < B) ? x[(size_t) b * n + iq] : 0.0;
+    }
+  __syncthreads ();
+  // eight elements per thread, their partial loads interleaved
+#pragma unroll
+  for (int u = 0; u < CB_I * CB_B / 256; u++)
+    {
+      const int e = threadIdx.x + 256 * u;
+      const int bl = e % CB_B, il = e / CB_B;
+      const size_t iq = iq0 + il;
+      const int b = b0 + bl;
+      if (iq < n && b < B)
+        {
+          const int i = (int) (iq / D), q = (int) (iq - (size_t) i * D);
+          const size_t t = iq * B + b;
+          double acc = self_diag (p, pos[i].w, periodic != 0, V, q) * xt[bl][il];
+#pragma unroll 4
+          for (int ch = 0; ch < nreal; ch++) acc += double (rpart[(size_t) ch * stride + t]);
+#pragma unroll 4
+          for (int ch = 0; ch < nrecip; ch++) acc += double (kpart[(size_t) ch * stride + t]);
+          yt[il][bl] = acc;
+        }
+    }
+  __syncthreads ();
+  for (int e = threadIdx.x; e < CB_I * CB_B; e += 256)
+    {
+      const int il = e % CB_I, bl = e / CB_I;
+      const size_t iq = iq0 + il;
+      const int b = b0 + bl;
+      if (iq < n && b < B) y[(size_t) b * n + iq] = yt[il][bl];
+    }
 }
 
 }  // namespace shk

@@ -1169,9 +1169,11 @@ static int atimes_block_vt (shk_dev *d, const double *x, double *y, int B, bool 
         rc = recip_block_launch<V, T, 5> (d, XT, B, nrecip, kpart);
       if (rc) return rc;
     }
-  hipLaunchKernelGGL ((k_combine_block<V, T>), dim3 ((unsigned) ((nb * D + 255) / 256)), dim3 (256), 0,
-                      d->stream, pos4, x, B, (const T *) d->partial.p, nj, (const T *) kpart,
-                      nrecip, q, 1, y);
+  hipLaunchKernelGGL ((k_combine_block<V, T>),
+                      dim3 ((unsigned) (((size_t) p.np * D + CB_I - 1) / CB_I),
+                            (unsigned) ((B + CB_B - 1) / CB_B)),
+                      dim3 (256), 0, d->stream, pos4, x, B, (const T *) d->partial.p, nj,
+                      (const T *) kpart, nrecip, q, 1, y);
   HIPCHK (hipGetLastError ());
   if (d->timing)
     {
