@@ -369,7 +369,7 @@ k_combine_block (const double4 *__restrict__ pos, const double *__restrict__ x, 
       xt[bl][il] = (iq < n && b < B) ? x[(size_t) b * n + iq] : 0.0;
     }
   __syncthreads ();
-  // eight elements per thread, their partial loads interleaved
+  // CB_I CB_B / 256 elements per thread, their partial loads interleaved
 #pragma unroll
   for (int u = 0; u < CB_I * CB_B / 256; u++)
     {
