@@ -138,12 +138,21 @@ __device__ __forceinline__ Cplx<float> lds_c (const unsigned char *base, int byt
 // Zt rows of zs floats (16-byte rows, npad rows, zero past np), tables
 // tab[j LD + e].  grid (ceil(cz/64), ceil(nk/64), splits of j, each a
 // multiple of 64 particles): the column blocks of one k block run side by
-// side and share the tables through L2.  Per tile of 64 particles the
-// tables ([64][LD]) and the z slice ([64][64]) go to LDS by direct global ->
-// LDS loads; wave w takes quads w, w + 4, w + 8, w + 12, whose LDS
-// addresses differ from the first by constants, so the inner loop spends
-// its vector instructions on the phase products only.  The 16 lanes of a
-// row read consecutive entries (consecutive m3): no bank conflicts.
+// side and share the tables through L2.  The particles go through LDS in
+// stages of SP (32 for LD = 96, 16 for LD = 128 and 160), double-buffered: the
+// next stage's tables ([SP][LD]) and z slice ([SP][64]) are copied by
+// direct global -> LDS loads while the current one is multiplied, and one
+// barrier per stage hands the buffers over.  Wave w takes quads w, w + 4 of
+// a stage, whose LDS addresses differ from the first by constants, so the
+// inner loop spends its vector instructions on the phase products only.
+// The 16 lanes of a row read consecutive entries (consecutive m3): no bank
+// conflicts.
+template <int LD> struct SfStage {
+  static constexpr int SP = LD <= 96 ? 32 : 16;             // particles per stage
+  static constexpr int TB = SP * LD * 8;                    // table bytes of a stage
+  static constexpr int BYTES = TB + SP * 64 * 4;            // + the z slice
+};
+
 template <int LD>
 __global__ __launch_bounds__ (256, 2) void
 k_sf_mfma (const float *__restrict__ Zt, int cz, int zs, const int4 *__restrict__ km,
@@ -151,7 +160,7 @@ k_sf_mfma (const float *__restrict__ Zt, int cz, int zs, const int4 *__restrict_
            float *__restrict__ part)
 {
   extern __shared__ __align__ (16) unsigned char smem_mf[];
-  constexpr int TB = 64 * LD * (int) sizeof (Cplx<float>);
+  constexpr int SP = SfStage<LD>::SP, TB = SfStage<LD>::TB, SB = SfStage<LD>::BYTES;
   float *red = (float *) smem_mf;                          // [4][64][64], after the loop
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int ir = lane & 15, kq = lane >> 4;
@@ -179,31 +188,38 @@ k_sf_mfma (const float *__restrict__ Zt, int cz, int zs, const int4 *__restrict_
         accs[kt][ct] = mf4{0.f, 0.f, 0.f, 0.f};
       }
   const int ncv = min (16, (zs - cbase) >> 2);           // 16-byte vectors per z row
-  for (int jb = j0; jb < j1; jb += 64)
+  auto stage = [&] (int jb, int buf) {
+    unsigned char *dst = smem_mf + buf * SB;
+    const Cplx<float> *ts = tab + (size_t) jb * LD;
+    async_copy16 (dst, SP * LD / 2, [&] (int v) { return (const void *) (ts + 2 * v); });
+    async_copy16 (dst + TB, SP * 16, [&] (int v) -> const void * {
+      const int row = v >> 4, cv = v & 15;
+      return Zt + (size_t) (jb + row) * zs + cbase + 4 * min (cv, ncv - 1);
+    });
+  };
+  stage (j0, 0);
+  async_wait_all ();
+  int buf = 0;
+  for (int jb = j0; jb < j1; jb += SP, buf ^= 1)
     {
-      const Cplx<float> *ts = tab + (size_t) jb * LD;
-      async_copy16 (smem_mf, 32 * LD, [&] (int v) { return (const void *) (ts + 2 * v); });
-      async_copy16 (smem_mf + TB, 64 * 16, [&] (int v) -> const void * {
-        const int row = v >> 4, cv = v & 15;
-        return Zt + (size_t) (jb + row) * zs + cbase + 4 * min (cv, ncv - 1);
-      });
-      async_wait_all ();
+      if (jb + SP < j1) stage (jb + SP, buf ^ 1);
+      const unsigned char *sb = smem_mf + buf * SB;
 #pragma unroll
-      for (int u = 0; u < 4; u++)
+      for (int u = 0; u < SP / 16; u++)
         {
           float ac[4], as[4], bz[4];
 #pragma unroll
           for (int kt = 0; kt < 4; kt++)
             {
-              const Cplx<float> e = cmul (cmul (lds_c (smem_mf, ax[kt] + 128 * LD * u),
-                                                lds_c (smem_mf, ay[kt] + 128 * LD * u)),
-                                          lds_c (smem_mf, az[kt] + 128 * LD * u));
+              const Cplx<float> e = cmul (cmul (lds_c (sb, ax[kt] + 128 * LD * u),
+                                                lds_c (sb, ay[kt] + 128 * LD * u)),
+                                          lds_c (sb, az[kt] + 128 * LD * u));
               ac[kt] = e.re;
               as[kt] = e.im;
             }
 #pragma unroll
           for (int ct = 0; ct < 4; ct++)
-            bz[ct] = *(const float *) (smem_mf + zb + 4096 * u + 64 * ct);
+            bz[ct] = *(const float *) (sb + zb + 4096 * u + 64 * ct);
 #pragma unroll
           for (int kt = 0; kt < 4; kt++)
 #pragma unroll
@@ -213,7 +229,7 @@ k_sf_mfma (const float *__restrict__ Zt, int cz, int zs, const int4 *__restrict_
                 accs[kt][ct] = mfma4 (as[kt], bz[ct], accs[kt][ct]);
               }
         }
-      __syncthreads ();
+      async_wait_all ();
     }
 #pragma unroll
   for (int s = 0; s < 2; s++)

@@ -86,6 +86,7 @@ struct shk_dev {
   int tx = 0, ty = 0, tz = 0;
   double s2x = 0.0, s3x = 0.0;
   int timing = 0;
+  int ncu = 256;                  // compute units of the device
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   double last_ms[3] = {0, 0, 0};
   unsigned long generation = 0;   // bumped whenever a work buffer moves
@@ -159,6 +160,11 @@ extern "C" shk_dev *shk_dev_create (void)
       return nullptr;
     }
   shk_dev *d = new shk_dev ();
+  int dev = 0, ncu = 0;
+  if (hipGetDevice (&dev) == hipSuccess
+      && hipDeviceGetAttribute (&ncu, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess
+      && ncu > 0)
+    d->ncu = ncu;
   if (hipStreamCreateWithFlags (&d->stream, hipStreamNonBlocking) != hipSuccess)
     {
       set_err ("hipStreamCreate failed");
@@ -713,6 +719,29 @@ static int recip_block_launch (shk_dev *d, const T *XT, int B, int &nrecip,
   return SHK_OK;
 }
 
+// number of splits s (1 <= s <= smax) of a launch with `base` workgroups
+// per split when `slots` workgroups are resident at once: the smallest s
+// whose grid fills at least 90 % of the rounds of slots it takes (a last
+// round left mostly empty costs a whole round), else the best filling
+static int balanced_splits (long base, long slots, int smax)
+{
+  int best = smax;
+  double best_eff = -1.0;
+  for (int s = 1; s <= smax; s++)
+    {
+      const long w = base * s;
+      const long rounds = (w + slots - 1) / slots;
+      const double eff = (double) w / (double) (rounds * slots);
+      if (eff >= 0.9) return s;
+      if (eff > best_eff)
+        {
+          best_eff = eff;
+          best = s;
+        }
+    }
+  return best;
+}
+
 // FP32 matrix-core reciprocal sums for a block (MIXED precision); Zt rows
 // of zs floats, npad = np rounded up to 64 rows (zero past np)
 template <int V, int NPOW>
@@ -726,32 +755,35 @@ static int recip_block_launch_mfma (shk_dev *d, const float *Zt, int zs, int npa
   const int nk = p.nk, np = p.np;
   const int cz = NH * D * B, C = D * B;
   const PhaseBasis &b = d->basis;
-  // LDS: tables of 64 particles (64 x LD entries for the structure factors
-  // with LD = 96 or 128 >= W, W x 64 for the back-projection) [+ the 16 KB
-  // z slice], the 64 KB end-of-block reduction aliased over them
-  const int LD = b.W <= 96 ? 96 : 128;
-  if (b.W > 128)
+  // LDS: two stages of the structure factors' tables and z slices (LD = 96,
+  // 128 or 160 >= W entries per particle), the back-projection's W x 64
+  // tables (at most 80 KB), the 64 KB end-of-block reduction aliased over
+  // them
+  const int LD = b.W <= 96 ? 96 : (b.W <= 128 ? 128 : 160);
+  if (b.W > 160)
     {
-      set_err ("batched reciprocal sum: phase tables of %d entries exceed LDS", b.W);
+      set_err ("batched MIXED reciprocal sum: phase tables of %d entries (more than 160: "
+               "k_max spans more than 31 lattice steps) exceed LDS",
+               b.W);
       return SHK_EARG;
     }
-  size_t lds_sf = (size_t) 64 * LD * 8 + 16384, lds_bk = (size_t) b.W * 512;
+  size_t lds_sf = 2 * (size_t) (LD == 96 ? SfStage<96>::BYTES
+                                         : (LD == 128 ? SfStage<128>::BYTES : SfStage<160>::BYTES)),
+         lds_bk = (size_t) b.W * 512;
   if (lds_sf < (size_t) MF_RED_BYTES) lds_sf = MF_RED_BYTES;
   if (lds_bk < (size_t) MF_RED_BYTES) lds_bk = MF_RED_BYTES;
-  // structure factors: splits of j, multiples of 64
+  // structure factors: splits of j, multiples of 64, filling the rounds of
+  // two resident workgroups per CU
   const int nkb = (nk + 63) / 64, ncz = (cz + 63) / 64;
-  int sj = (1024 + nkb * ncz - 1) / (nkb * ncz);
-  if (sj > npad / 64) sj = npad / 64;
-  if (sj < 1) sj = 1;
+  const long slots = 2L * d->ncu;
+  int sj = balanced_splits ((long) nkb * ncz, slots, std::max (1, std::min (npad / 64, 64)));
   int jper = (npad + sj - 1) / sj;
   jper = (jper + 63) & ~63;
   sj = (npad + jper - 1) / jper;
   // back-projection: splits of kper k (a multiple of 4); P, Q rows padded
   // with zeros to sk kper, columns to a multiple of 64
   const int nib = (np + 63) / 64, ncc = (C + 63) / 64, Cp = ncc * 64;
-  int sk = (1024 + nib * ncc - 1) / (nib * ncc);
-  if (sk > (nk + 63) / 64) sk = (nk + 63) / 64;
-  if (sk < 1) sk = 1;
+  int sk = balanced_splits ((long) nib * ncc, slots, std::max (1, std::min ((nk + 63) / 64, 64)));
   int kper = (nk + sk - 1) / sk;
   kper = (kper + 3) & ~3;
   sk = (nk + kper - 1) / kper;
@@ -770,8 +802,11 @@ static int recip_block_launch_mfma (shk_dev *d, const float *Zt, int zs, int npa
   if (LD == 96)
     hipLaunchKernelGGL ((k_phase_tab2<96>), gt, dim3 (256), 0, d->stream, pos4, np, npad, b, tab,
                         tabT);
-  else
+  else if (LD == 128)
     hipLaunchKernelGGL ((k_phase_tab2<128>), gt, dim3 (256), 0, d->stream, pos4, np, npad, b, tab,
+                        tabT);
+  else
+    hipLaunchKernelGGL ((k_phase_tab2<160>), gt, dim3 (256), 0, d->stream, pos4, np, npad, b, tab,
                         tabT);
   HIPCHK (hipGetLastError ());
   static bool attr_sf = false, attr_bk = false;
@@ -780,6 +815,8 @@ static int recip_block_launch_mfma (shk_dev *d, const float *Zt, int zs, int npa
       HIPCHK (hipFuncSetAttribute ((const void *) k_sf_mfma<96>,
                                    hipFuncAttributeMaxDynamicSharedMemorySize, 80 * 1024));
       HIPCHK (hipFuncSetAttribute ((const void *) k_sf_mfma<128>,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, 80 * 1024));
+      HIPCHK (hipFuncSetAttribute ((const void *) k_sf_mfma<160>,
                                    hipFuncAttributeMaxDynamicSharedMemorySize, 80 * 1024));
       attr_sf = true;
     }
@@ -792,8 +829,11 @@ static int recip_block_launch_mfma (shk_dev *d, const float *Zt, int zs, int npa
   if (LD == 96)
     hipLaunchKernelGGL ((k_sf_mfma<96>), dim3 (ncz, nkb, sj), dim3 (256), lds_sf, d->stream, Zt,
                         cz, zs, km, tab, npad, b, nk, jper, (float *) d->kpart.p);
-  else
+  else if (LD == 128)
     hipLaunchKernelGGL ((k_sf_mfma<128>), dim3 (ncz, nkb, sj), dim3 (256), lds_sf, d->stream, Zt,
+                        cz, zs, km, tab, npad, b, nk, jper, (float *) d->kpart.p);
+  else
+    hipLaunchKernelGGL ((k_sf_mfma<160>), dim3 (ncz, nkb, sj), dim3 (256), lds_sf, d->stream, Zt,
                         cz, zs, km, tab, npad, b, nk, jper, (float *) d->kpart.p);
   HIPCHK (hipGetLastError ());
   const size_t nkB = (size_t) nkpad * NPOW * B;

@@ -266,6 +266,35 @@ def test_block_mixed_full_size(tr, B, dense):
     s.close()
 
 
+@pytest.mark.parametrize("tr", [3e3, 3e4, 1e5])
+def test_block_mixed_phase_table_widths(tr):
+    """MIXED block path at splittings whose phase tables hold 78 (tr = 3e3),
+    113 (3e4) and 138 (1e5) entries per particle, i.e. the 96-, 128- and
+    160-entry layouts of the matrix-core structure factors: every column
+    against the FP64 block path, 1e-6, and column 0 of the FP64 block
+    against the single-column FP64 path (other kernels; both are pinned to
+    the oracle at smaller splittings, where the oracle runs in seconds);
+    N = 100, phi = 0.05 FTS, 40 columns"""
+    n, B = 100, 40
+    pos, L = workloads.configuration(n, 0.05, seed=5)
+    X = np.random.default_rng(9).standard_normal(B * 11 * n)
+    s = ls.Stokes(2, pos, box=(L,) * 3, tr=tr)
+    xd = ls.DeviceArray.from_host(X)
+    yd = ls.DeviceArray(X.size)
+    s.atimes_dev(xd, yd, nrhs=B)
+    Y = yd.to_host().reshape(B, -1)
+    s.set_precision(ls.MIXED)
+    s.atimes_dev(xd, yd, nrhs=B)
+    Ym = yd.to_host().reshape(B, -1)
+    errs = [relerr(Ym[c], Y[c]) for c in range(B)]
+    _say(f"  tr={tr:g}: MIXED block vs FP64 block max {max(errs):.2e}")
+    assert max(errs) <= TOLMIX
+    assert relerr(s_single(s, X[:11 * n]), Y[0]) <= TOL64
+    xd.free()
+    yd.free()
+    s.close()
+
+
 def s_single(s, x):
     s.set_precision(ls.FP64)
     return s.atimes(x)
