@@ -1313,78 +1313,73 @@ __device__ __forceinline__ md4 mfma4d (double a, double b, md4 c)
 
 // FP64 form of k_real_mfma_mom: v_mfma_f64_16x16x4f64 (A[i][k] in lane i +
 // 16 k, B[k][j] in lane j + 16 k, D[i][j] in element i / 4 of lane j + 16
-// (i % 4)), 16 columns per wave, NC waves per workgroup
+// (i % 4)), 16 columns per wave, NC waves per workgroup; chunks of JC
+// sources (8 for eight waves, whose registers allow one workgroup per CU
+// anyway, 4 otherwise) copied from mt to LDS by global_load_lds_dwordx4
+// (jper and j0 multiples of 8, npad of 64: a chunk never reads past row
+// npad; rows at or past j1 hold finite moments and meet zero z rows)
 template <int V, int NC>
 __global__ __launch_bounds__ (64 * NC) __attribute__ ((amdgpu_waves_per_eu (2))) void
 k_real_mfma_mom64 (const double *__restrict__ mt, int npad, const double *__restrict__ XT,
                    int B, int zs, int np, int jper, double *__restrict__ partial)
 {
   constexpr int D = Dim<V>::D, NM = Dim<V>::NMOM;
-  constexpr int TW = (NM + NC - 1) / NC;
-  __shared__ double mom[2][NM * 64];
+  constexpr int JC = NC >= 8 ? 8 : 4, KS = JC / 4;
+  constexpr int VPM = JC * 16 / 2;              // 16-byte vectors per moment and chunk
+  __shared__ __attribute__ ((aligned (16))) double mom[2][NM * JC * 16];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int ir = lane & 15, kq = lane >> 4;
   const int ibase = blockIdx.x * 16;
   const int j0 = blockIdx.y * jper, j1 = min (np, j0 + jper);
   const int cbase = blockIdx.z * 16 * NC + 16 * wave;
   const size_t ms = (size_t) npad * npad;
-  const double *mb = mt + (size_t) blockIdx.x * npad * 16 + lane;
-  double pre[TW];
-  auto load_mom = [&] (int jb) {
-    const bool jok = jb + kq < j1;
-#pragma unroll
-    for (int u = 0; u < TW; u++)
+  const double *mrow = mt + (size_t) blockIdx.x * npad * 16;
+  auto load_mom = [&] (int jb, int buf) {
+    const double *src = mrow + (size_t) jb * 16;
+    char *dst = (char *) mom[buf];
+    for (int v0 = wave * 64; v0 < NM * VPM; v0 += 64 * NC)
       {
-        const int t = wave + NC * u;
-        pre[u] = (t < NM && jok) ? mb[(size_t) t * ms + (size_t) jb * 16] : 0.0;
+        const int v = v0 + lane;
+        if (v < NM * VPM)
+          __builtin_amdgcn_global_load_lds (src + (size_t) (v / VPM) * ms + 2 * (v % VPM),
+                                            (lds_void *) (dst + 16 * v0), 16, 0, 0);
       }
   };
-  auto store_mom = [&] (int buf) {
-#pragma unroll
-    for (int u = 0; u < TW; u++)
-      {
-        const int t = wave + NC * u;
-        if (t < NM) mom[buf][t * 64 + lane] = pre[u];
-      }
-  };
-  double bzn[D];
-  auto load_z = [&] (int jb) {
-    const int jz = jb + kq, b = cbase + ir;
+  double bz[KS][D];
+  auto load_z = [&] (int jb, int s) {
+    const int jz = jb + 4 * s + kq, b = cbase + ir;
     const bool ok = jz < j1 && b < B;
     const double *src = XT + (size_t) (ok ? jz : 0) * zs + (ok ? b : 0);
 #pragma unroll
-    for (int q = 0; q < D; q++) bzn[q] = ok ? src[(size_t) q * B] : 0.0;
+    for (int q = 0; q < D; q++) bz[s][q] = ok ? src[(size_t) q * B] : 0.0;
   };
   md4 acc[D];
 #pragma unroll
   for (int t = 0; t < D; t++) acc[t] = md4{0.0, 0.0, 0.0, 0.0};
-  load_mom (j0);
-  load_z (j0);
-  store_mom (0);
-  __syncthreads ();
+  load_mom (j0, 0);
+#pragma unroll
+  for (int s = 0; s < KS; s++) load_z (j0, s);
+  async_wait_all ();
   int buf = 0;
-  for (int jb = j0; jb < j1; jb += 4, buf ^= 1)
+  for (int jb = j0; jb < j1; jb += JC, buf ^= 1)
     {
-      const bool more = jb + 4 < j1;
-      double bz[D];
+      const bool more = jb + JC < j1;
+      if (more) load_mom (jb + JC, buf ^ 1);
 #pragma unroll
-      for (int q = 0; q < D; q++) bz[q] = bzn[q];
-      if (more)
+      for (int s = 0; s < KS; s++)
         {
-          load_mom (jb + 4);
-          load_z (jb + 4);
+          struct LdsM {
+            const double *b;
+            __device__ double operator[] (int t) const { return b[JC * 16 * t]; }
+          } Ml{mom[buf] + 64 * s + lane};
+#pragma unroll
+          for (int q = 0; q < D; q++)
+#pragma unroll
+            for (int r = 0; r < D; r++)
+              acc[r] = mfma4d (block_entry<V, double> (Ml, r, q, false), bz[s][q], acc[r]);
+          if (more) load_z (jb + JC, s);
         }
-      struct LdsM {
-        const double *b;
-        __device__ double operator[] (int t) const { return b[64 * t]; }
-      } Ml{mom[buf] + lane};
-#pragma unroll
-      for (int q = 0; q < D; q++)
-#pragma unroll
-        for (int r = 0; r < D; r++)
-          acc[r] = mfma4d (block_entry<V, double> (Ml, r, q, false), bz[q], acc[r]);
-      if (more) store_mom (buf ^ 1);
-      __syncthreads ();
+      async_wait_all ();
     }
 #pragma unroll
   for (int r = 0; r < D; r++)

@@ -1025,8 +1025,8 @@ static int atimes_block_vt (shk_dev *d, const double *x, double *y, int B, bool 
       if (nj > (p.np + 15) / 16) nj = (p.np + 15) / 16;
       if (nj < 1) nj = 1;
       jper = (p.np + nj - 1) / nj;
-      // multiples of 4 sources (8 for the FP32 moment contraction's chunks)
-      const int ja = (mom && !mom64) ? 7 : 3;
+      // multiples of 4 sources (8 for the moment contractions' chunks)
+      const int ja = (mom || mom64) ? 7 : 3;
       jper = (jper + ja) & ~ja;
       nj = (p.np + jper - 1) / jper;
     }
