@@ -1061,6 +1061,49 @@ pair_images_add (const PairSep &s, T xi, T rmn, const typename Vec4<T>::type *ca
     }
 }
 
+// the 256 threads of a workgroup rank their values cnt (0 <= cnt <= 256)
+// by a counting sort in LDS; returns the thread index holding the value of
+// rank threadIdx.x (ties in an arbitrary order).  bin_l: REAL_PC_CAND + 2
+// ints zeroed before the call (and a barrier); perm_l: 256 entries.  Every
+// thread calls it.
+__device__ __forceinline__ int rank_by_count (int cnt, int *bin_l, unsigned short *perm_l)
+{
+  const int slot = atomicAdd (&bin_l[cnt + 1], 1);
+  __syncthreads ();
+  // prefix sums over the counts (wave 0: PB bins per lane)
+  if (threadIdx.x < 64)
+    {
+      constexpr int PB = (REAL_PC_CAND + 2 + 63) / 64;
+      const int b0 = threadIdx.x * PB;
+      int v[PB], sum = 0;
+#pragma unroll
+      for (int u = 0; u < PB; u++)
+        {
+          v[u] = (b0 + u < REAL_PC_CAND + 2) ? bin_l[b0 + u] : 0;
+          sum += v[u];
+        }
+      int incl = sum;
+#pragma unroll
+      for (int off = 1; off < 64; off <<= 1)
+        {
+          const int o = __shfl_up (incl, off, 64);
+          if ((int) threadIdx.x >= off) incl += o;
+        }
+      int run = incl - sum;
+#pragma unroll
+      for (int u = 0; u < PB; u++)
+        {
+          run += v[u];
+          if (b0 + u < REAL_PC_CAND + 2) bin_l[b0 + u] = run;
+        }
+    }
+  __syncthreads ();
+  // bin_l[c] now counts the values below c
+  perm_l[bin_l[cnt] + slot] = (unsigned short) threadIdx.x;
+  __syncthreads ();
+  return perm_l[threadIdx.x];
+}
+
 // workgroup: the 16 i x 16 j pairs of one tile (blockIdx.x <= blockIdx.y;
 // the others return at once), in three phases.  (1) thread q = 16 jl + il
 // finds the images of the pair (16 bx + il, 16 by + jl) (one mask per 64
@@ -1096,7 +1139,7 @@ k_real_moments (const double4 *__restrict__ pos, const typename Vec4<T>::type *_
   const int nch = (p.ncand + 63) >> 6;
   const int bx = blockIdx.x, by = blockIdx.y;
   // (1) masks and image count of pair q = threadIdx.x
-  int slot, cnt = 0;
+  int cnt = 0;
   {
     const int q = threadIdx.x, i = bx * 16 + (q & 15), j = by * 16 + (q >> 4);
     const bool valid = i <= j && j < p.np;
@@ -1111,42 +1154,11 @@ k_real_moments (const double4 *__restrict__ pos, const typename Vec4<T>::type *_
         mk_l[ch][q] = m;
         cnt += __builtin_popcountll (m);
       }
-    slot = atomicAdd (&bin_l[cnt + 1], 1);
   }
-  __syncthreads ();
-  // (2) exclusive prefix over the counts (wave 0: five bins per lane)
-  if (threadIdx.x < 64)
-    {
-      constexpr int PB = (REAL_PC_CAND + 2 + 63) / 64;
-      const int b0 = threadIdx.x * PB;
-      int v[PB], sum = 0;
-#pragma unroll
-      for (int u = 0; u < PB; u++)
-        {
-          v[u] = (b0 + u < REAL_PC_CAND + 2) ? bin_l[b0 + u] : 0;
-          sum += v[u];
-        }
-      int incl = sum;
-#pragma unroll
-      for (int off = 1; off < 64; off <<= 1)
-        {
-          const int o = __shfl_up (incl, off, 64);
-          if ((int) threadIdx.x >= off) incl += o;
-        }
-      int run = incl - sum;
-#pragma unroll
-      for (int u = 0; u < PB; u++)
-        {
-          run += v[u];
-          if (b0 + u < REAL_PC_CAND + 2) bin_l[b0 + u] = run;
-        }
-    }
-  __syncthreads ();
-  // bin_l[c] now counts the pairs with fewer than c images
-  perm_l[bin_l[cnt] + slot] = (unsigned short) threadIdx.x;
-  __syncthreads ();
+  // (2) rank by count
+  const int q = rank_by_count (cnt, bin_l, perm_l);
   // (3) moments of the pair ranked threadIdx.x
-  const int q = perm_l[threadIdx.x], il = q & 15, jl = q >> 4;
+  const int il = q & 15, jl = q >> 4;
   const int i = bx * 16 + il, j = by * 16 + jl;
   T M[NM];
 #pragma unroll
