@@ -1,6 +1,5 @@
-/* ewald-3f.h -- iterative F resistance, mobility and mixed problems
- * (drop-in for the reference's ewald-3f.h without the lubrication
- * variants).  The resistance and mixed problems are solved by restarted
+/* ewald-3f.h -- iterative F resistance, mobility and mixed problems,
+ * without and with lubrication (drop-in for the reference's ewald-3f.h).  The resistance and mixed problems are solved by restarted
  * GMRES over the GPU operator with the parameters kept by stokes_set_iter
  * (sys->it: restart, max, eps on |b - A x| / |b|; defaults 30, 2000,
  * 1e-12); the mobility problem is one product.  A wrong sys->version is
@@ -33,6 +32,28 @@ void solve_mob_3f (struct stokes *sys, const double *f, double *u);
  * 0..nf-1.  nm == np is the mobility problem */
 void solve_mix_3f (struct stokes *sys, const double *f, const double *uf,
                    double *u, double *ff);
+
+/* with lubrication (the correction L of calc_lub_3f, lub.h).  Replace
+ * ewald-3f.c:373-406 (solve_res_lub_3f_0: M f = (I + M L) u, GMRES on M)
+ * and 334-362 (solve_res_lub_3f: u shifted to the fluid-rest frame first) */
+void solve_res_lub_3f_0 (struct stokes *sys, const double *u, double *f);
+void solve_res_lub_3f (struct stokes *sys, const double *u, double *f);
+
+/* replaces ewald-3f.c:410-436: y = (I + M L) x, n = 3 np, user_data the
+ * struct stokes */
+void atimes_mob_lub_3f (int n, const double *x, double *y, void *user_data);
+
+/* replaces ewald-3f.c:446-481: (I + M L) u = M f by GMRES, u shifted to
+ * the labo frame */
+void solve_mob_lub_3f (struct stokes *sys, const double *f, double *u);
+
+/* replaces ewald-3f.c:646-705: fixed particles nm..np-1; unknowns (U_m,
+ * F_f) of A x = (I + M L) [U_m, 0_f] - M [0_m, F_f] = M [F_m, 0_f] -
+ * (I + M L) [0_m, U_f] (ewald-3f.c:495-644); uf shifted with the positions
+ * of particles 0..nf-1 as the reference does (ewald-3f.c:685).  nm == np
+ * is solve_mob_lub_3f */
+void solve_mix_lub_3f (struct stokes *sys, const double *f, const double *uf,
+                       double *u, double *ff);
 
 #ifdef __cplusplus
 }

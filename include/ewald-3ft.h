@@ -1,5 +1,5 @@
 /* ewald-3ft.h -- iterative FT resistance, mobility and mixed problems
- * (drop-in for the reference's ewald-3ft.h without the lubrication
+ * (drop-in for the reference's ewald-3ft.h, with the lubrication
  * variants).  The resistance and mixed problems are solved by restarted
  * GMRES over the GPU operator with the parameters kept by stokes_set_iter
  * (sys->it: restart, max, eps on |b - A x| / |b|; defaults 30, 2000,
@@ -38,6 +38,22 @@ void solve_mob_3ft (struct stokes *sys, const double *f, const double *t,
 void solve_mix_3ft (struct stokes *sys, const double *f, const double *t,
                     const double *uf, const double *of, double *u, double *o,
                     double *ff, double *tf);
+
+/* with lubrication (calc_lub_3ft, lub.h), six values per particle as the F
+ * forms: replace ewald-3ft.c:441-485 (solve_res_lub_3ft_0), 395-428
+ * (solve_res_lub_3ft), 527-569 (solve_mob_lub_3ft: (I + M L) (u, o) = M
+ * (f, t), velocities shifted to the labo frame) and 747-805
+ * (solve_mix_lub_3ft: fixed particles nm..np-1, uf and of shifted with the
+ * positions of particles 0..nf-1 as ewald-3ft.c:782-783 do) */
+void solve_res_lub_3ft_0 (struct stokes *sys, const double *u, const double *o,
+                          double *f, double *t);
+void solve_res_lub_3ft (struct stokes *sys, const double *u, const double *o,
+                        double *f, double *t);
+void solve_mob_lub_3ft (struct stokes *sys, const double *f, const double *t,
+                        double *u, double *o);
+void solve_mix_lub_3ft (struct stokes *sys, const double *f, const double *t,
+                        const double *uf, const double *of, double *u, double *o,
+                        double *ff, double *tf);
 
 #ifdef __cplusplus
 }

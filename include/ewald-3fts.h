@@ -1,5 +1,5 @@
 /* ewald-3fts.h -- iterative FTS resistance, mobility and mixed problems
- * (drop-in for the reference's ewald-3fts.h without the lubrication
+ * (drop-in for the reference's ewald-3fts.h, with the lubrication
  * variants), solved by restarted GMRES over the GPU operator.  The
  * iteration parameters are those kept by stokes_set_iter (sys->it):
  * restart, max and eps (relative residual |b - A x| / |b|); defaults 30,
@@ -69,6 +69,24 @@ void solve_mob_lub_3fts_0 (struct stokes *sys, const double *f,
                            double *o, double *s);
 void solve_mob_lub_3fts (struct stokes *sys, const double *f, const double *t,
                          const double *e, double *u, double *o, double *s);
+
+/* replaces ewald-3fts.c:1192-1234 and 1255-1306: the mixed problem with
+ * lubrication, fixed particles nm..np-1.  Unknowns (U, O, S)_mobile,
+ * (F, T, S)_fixed by GMRES on A x = (I + M L) [(U, O, 0)_m, 0_f] -
+ * M [(0, 0, S)_m, (F, T, S)_f] with b = M [(F, T, 0)_m, 0_f] - (I + M L)
+ * [(0, 0, E)_m, (U, O, E)_f] (ewald-3fts.c:1015-1190).  _0: fluid-rest
+ * frame; the other shifts uf, of, ef with the positions of particles
+ * 0..nf-1 and e with those of 0..nm-1, as the reference does
+ * (ewald-3fts.c:1288-1291), and u, o back.  nm == np is the mobility
+ * problem with lubrication */
+void solve_mix_lub_3fts_0 (struct stokes *sys, const double *f, const double *t,
+                           const double *e, const double *uf, const double *of,
+                           const double *ef, double *u, double *o, double *s,
+                           double *ff, double *tf, double *sf);
+void solve_mix_lub_3fts (struct stokes *sys, const double *f, const double *t,
+                         const double *e, const double *uf, const double *of,
+                         const double *ef, double *u, double *o, double *s,
+                         double *ff, double *tf, double *sf);
 
 /* imposed-flow shifts of particles 0..n-1: replace f.c:569-610
  * (shift_labo_to_rest_U), f.c:621-660 (shift_rest_to_labo_U), ft.c:901-920

@@ -90,6 +90,20 @@ int stokes_hip_solve_mob_lub_3fts_dev (struct stokes *sys, const double *fte_dev
 int stokes_hip_solve_lub_3fts_matrix_dev (struct stokes *sys, int kind,
                                           const double *b_dev, double *x_dev);
 
+/* the lubricated solves of sys->version (D = 3, 6, 11 values per particle)
+ * with device pointers, fluid-rest frame.  res: velocities of every
+ * particle -> forces, M f = (I + M L) v by GMRES (solve_res_lub_3f_0,
+ * ewald-3f.c:373-406, and its FT / FTS forms); mix: known [(F, T, E)_m,
+ * (U, O, E)_f] -> unknowns [(U, O, S)_m, (F, T, S)_f], fixed particles
+ * sys->nm..np-1 (solve_mix_lub_3f, ewald-3f.c:646-705; solve_mix_lub_3ft,
+ * ewald-3ft.c:747-805; solve_mix_lub_3fts_0, ewald-3fts.c:1192-1234; the
+ * mobility problem when nm == np).  No counterpart in the reference (host
+ * pointers only). */
+int stokes_hip_solve_res_lub_dev (struct stokes *sys, const double *v_dev,
+                                  double *f_dev, int *iters, double *relres);
+int stokes_hip_solve_mix_lub_dev (struct stokes *sys, const double *b_dev,
+                                  double *x_dev, int *iters, double *relres);
+
 /* y = L x with device pointers (the operator of calc_lub_3f / 3ft / 3fts
  * for sys->version); 0 or a STOKES_HIP_E* code.  No counterpart in the
  * reference (its correction runs on the host). */

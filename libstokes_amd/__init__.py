@@ -89,6 +89,11 @@ EXPORTS = [
     "solve_res_lub_3fts_0", "solve_mob_lub_3fts", "solve_mob_lub_3fts_0",
     "solve_res_lub_3fts_matrix", "solve_res_lub_3fts_matrix_0",
     "solve_mob_lub_3fts_matrix", "solve_mob_lub_3fts_matrix_0",
+    "stokes_hip_solve_res_lub_dev", "stokes_hip_solve_mix_lub_dev",
+    "solve_res_lub_3f", "solve_res_lub_3f_0", "atimes_mob_lub_3f",
+    "solve_mob_lub_3f", "solve_mix_lub_3f", "solve_res_lub_3ft",
+    "solve_res_lub_3ft_0", "solve_mob_lub_3ft", "solve_mix_lub_3ft",
+    "solve_mix_lub_3fts", "solve_mix_lub_3fts_0",
     "cond_lub", "cond_lub_poly", "twobody_f_init", "twobody_f_free",
     "twobody_far", "twobody_lub", "scalar_two_body_res",
 ]
@@ -227,6 +232,22 @@ def lib():
         getattr(L, f).argtypes = [sp, _vp, _vp, C.POINTER(C.c_int), _dp]
     L.stokes_hip_solve_lub_3fts_matrix_dev.argtypes = [sp, C.c_int, _vp,
                                                        _vp]
+    for f in ("stokes_hip_solve_res_lub_dev", "stokes_hip_solve_mix_lub_dev"):
+        getattr(L, f).argtypes = [sp, _vp, _vp, C.POINTER(C.c_int), _dp]
+    for f, a in (("solve_res_lub_3f", [sp] + [_dp] * 2),
+                 ("solve_res_lub_3f_0", [sp] + [_dp] * 2),
+                 ("solve_mob_lub_3f", [sp] + [_dp] * 2),
+                 ("solve_mix_lub_3f", [sp] + [_dp] * 4),
+                 ("solve_res_lub_3ft", [sp] + [_dp] * 4),
+                 ("solve_res_lub_3ft_0", [sp] + [_dp] * 4),
+                 ("solve_mob_lub_3ft", [sp] + [_dp] * 4),
+                 ("solve_mix_lub_3ft", [sp] + [_dp] * 8),
+                 ("solve_mix_lub_3fts", [sp] + [_dp] * 12),
+                 ("solve_mix_lub_3fts_0", [sp] + [_dp] * 12)):
+        getattr(L, f).argtypes = a
+        getattr(L, f).restype = None
+    L.atimes_mob_lub_3f.argtypes = [C.c_int, _dp, _dp, sp]
+    L.atimes_mob_lub_3f.restype = None
     L.stokes_set_iter.argtypes = [sp, C.c_char_p, C.c_int, C.c_int,
                                   C.c_double, C.c_int, _vp]
     L.stokes_hip_adjoint_dev.argtypes = [sp, _vp, _vp, _vp, _vp]
@@ -448,6 +469,25 @@ class Stokes:
         check(f(self.sys, bd.ptr, xd.ptr, C.byref(it), C.byref(rr)),
               "solve_" + kind + "_lub")
         return xd.to_host(), it.value, rr.value
+
+    def solve_lub_any(self, b, kind="mix"):
+        """lubricated problem of the system's version on the device, in
+        the fluid-rest frame: kind "res" (velocities of every particle ->
+        forces, M f = (I + M L) v) or "mix" (known [(F, T, E)_mobile, (U, O,
+        E)_fixed] -> unknowns [(U, O, S)_mobile, (F, T, S)_fixed], the
+        system's nm mobile particles); per particle as the reference packs
+        them; returns (x, iterations, relative residual)"""
+        bd = DeviceArray.from_host(np.ascontiguousarray(b, np.float64))
+        xd = DeviceArray(bd.n)
+        it, rr = C.c_int(), C.c_double()
+        f = (self.L.stokes_hip_solve_res_lub_dev if kind == "res"
+             else self.L.stokes_hip_solve_mix_lub_dev)
+        check(f(self.sys, bd.ptr, xd.ptr, C.byref(it), C.byref(rr)),
+              "solve_" + kind + "_lub")
+        x = xd.to_host()
+        bd.free()
+        xd.free()
+        return x, it.value, rr.value
 
     def set_recip_g_fix(self, on=True):
         check(self.L.stokes_hip_set_recip_g_fix(self.sys, 1 if on else 0),

@@ -11,6 +11,9 @@
 #include <string.h>
 #include "stokes_hip.h"
 #include "lub.h"
+#include "ewald.h"
+#include "ewald-3f.h"
+#include "ewald-3ft.h"
 #include "ewald-3fts.h"
 #include "ewald-3fts-matrix.h"
 #include "stokes_internal.h"
@@ -163,13 +166,15 @@ void make_matrix_lub_3ft (struct stokes *sys, double *mat) { lub_matrix_host (sy
 void make_matrix_lub_3fts (struct stokes *sys, double *mat) { lub_matrix_host (sys, 2, mat, "make_matrix_lub_3fts"); }
 
 /* ------------------------------------------------------------------ */
-/* lubricated FTS solves                                                */
+/* lubricated solves                                                    */
 /* ------------------------------------------------------------------ */
 
 typedef struct {
   stokes_ctx *c;
   shk_lub lp;
-  int np;
+  int np, nm;                 /* particles nm..np-1 are fixed */
+  int version, D, qs;         /* D values per particle, the first qs of a
+                                 mobile particle's are its known forces */
   double *t1, *t2, *t3;
 } lub_op;
 
@@ -181,34 +186,53 @@ op_mob (void *ud, const double *x, double *y)
   return shk_dev_atimes (o->c->dev, x, y, 1, o->c->prec);
 }
 
-/* A (u, o, s) = -M (0, 0, s) + (I + M L) (u, o, 0) (atimes_mob_lub_3fts,
- * ewald-3fts.c:835-907) */
+/* A x = (I + M L) lo(x) - M hi(x), with lo(x) = [(u, o, 0)_mobile,
+ * 0_fixed] and hi(x) = [(0, 0, s)_mobile, (f, t, s)_fixed] for the
+ * unknowns x = [(u, o, s)_mobile, (f, t, s)_fixed] (F: u / f, FT: u, o /
+ * f, t; atimes_mob_lub_3f, ewald-3f.c:410-436; atimes_mix_lub_3f, 566-644;
+ * atimes_mix_lub_3ft, ewald-3ft.c:657-746; atimes_mob_lub_3fts,
+ * ewald-3fts.c:835-907; atimes_mix_lub_3fts, 1088-1190) */
 static int
-op_mob_lub (void *ud, const double *x, double *y)
+op_mix_lub (void *ud, const double *x, double *y)
 {
   lub_op *o = ud;
   shk_dev *d = o->c->dev;
-  const size_t n = (size_t) o->np * 11;
-  int rc = shk_dev_mix_split (d, x, o->np, o->np, 11, 6, o->t1, o->t2);
+  const size_t n = (size_t) o->np * o->D;
+  int rc = shk_dev_mix_split (d, x, o->np, o->nm, o->D, o->qs, o->t1, o->t2);
   if (!rc) rc = shk_dev_atimes (d, o->t2, y, 1, o->c->prec);
   if (!rc) rc = shk_dev_axpby (d, y, o->t1, 1.0, -1.0, n);
-  if (!rc) rc = shk_dev_lub (d, 2, &o->lp, o->t1, o->t3);
+  if (!rc) rc = shk_dev_lub (d, o->version, &o->lp, o->t1, o->t3);
   if (!rc) rc = shk_dev_atimes (d, o->t3, o->t2, 1, o->c->prec);
   if (!rc) rc = shk_dev_axpby (d, y, o->t2, 1.0, 1.0, n);
   return rc;
 }
 
-/* kind 0: (U, O, E) -> (F, T, S) = M^-1 (I + M L) (U, O, E); kind 1:
- * (F, T, E) -> (U, O, S) with (F, T, S) = (M^-1 + L) (U, O, E); matrix:
- * dense solve instead of GMRES */
+/* sys->version, nm mobile particles.  kind 0: velocities (U, O, E) of
+ * every particle -> (F, T, S) = M^-1 (I + M L) (U, O, E), GMRES on M with
+ * b = (I + M L) (U, O, E) (solve_res_lub_3f_0, ewald-3f.c:373-406;
+ * solve_res_lub_3ft_0, ewald-3ft.c:441-485; solve_res_lub_3fts_0,
+ * ewald-3fts.c:676-719); kind 1: known [(F, T, E)_mobile, (U, O,
+ * E)_fixed] -> unknowns [(U, O, S)_mobile, (F, T, S)_fixed], GMRES on
+ * op_mix_lub with b = M lo(known) - (I + M L) hi(known) (calc_b_mix_lub_3f,
+ * ewald-3f.c:495-564; calc_b_mix_lub_3ft, ewald-3ft.c:585-655;
+ * calc_b_mob_lub_3fts, ewald-3fts.c:776-833; calc_b_mix_lub_3fts,
+ * 1015-1076), nm = np being the mobility problem.  matrix (FTS, nm = np):
+ * dense solve instead of GMRES.  All in the fluid-rest frame. */
 static int
-lub_solve_dev (struct stokes *sys, int kind, int matrix, const double *b_dev,
-               double *x_dev, int *iters, double *relres, const char *what)
+lub_solve_dev (struct stokes *sys, int version, int kind, int nm, int matrix,
+               const double *b_dev, double *x_dev, int *iters, double *relres,
+               const char *what)
 {
-  if (!sys || sys->version != 2 || !b_dev || !x_dev)
+  static const char *vn[3] = {"F", "FT", "FTS"};
+  if (!sys || sys->version != version || !b_dev || !x_dev)
     {
-      stokes_hip_set_error ("%s: an FTS system and device vectors are required",
-                            what);
+      stokes_hip_set_error ("%s: an %s system and device vectors are required", what,
+                            vn[version]);
+      return STOKES_HIP_EARG;
+    }
+  if (nm < 0 || nm > sys->np || (matrix && (version != 2 || nm != sys->np)))
+    {
+      stokes_hip_set_error ("%s: %d mobile particles of %d", what, nm, sys->np);
       return STOKES_HIP_EARG;
     }
   stokes_ctx *c = stokes_hip_ctx (sys);
@@ -219,11 +243,15 @@ lub_solve_dev (struct stokes *sys, int kind, int matrix, const double *b_dev,
   memset (&o, 0, sizeof (o));
   o.c = c;
   o.np = sys->np;
+  o.nm = nm;
+  o.version = version;
+  o.D = version == 0 ? 3 : (version == 1 ? 6 : 11);
+  o.qs = version == 2 ? 6 : o.D;
   const double *tab[11];
   double f1[11 * 51];
   if ((rc = stokes_hip_lub_params (sys, &o.lp, tab, f1))) return rc;
   shk_dev *d = c->dev;
-  const size_t n = (size_t) sys->np * 11;
+  const size_t n = (size_t) sys->np * o.D;
   if (matrix)
     {
       double *M = NULL, *L = NULL;
@@ -250,23 +278,20 @@ lub_solve_dev (struct stokes *sys, int kind, int matrix, const double *b_dev,
   o.t3 = ws + 3 * n;
   if (kind == 0)
     {
-      /* b = (I + M L) (U, O, E) (solve_res_lub_3fts_0, ewald-3fts.c:676-719) */
-      rc = shk_dev_lub (d, 2, &o.lp, b_dev, o.t1);
+      rc = shk_dev_lub (d, version, &o.lp, b_dev, o.t1);
       if (!rc) rc = shk_dev_atimes (d, o.t1, b, 1, c->prec);
       if (!rc) rc = shk_dev_axpby (d, b, b_dev, 1.0, 1.0, n);
       if (!rc) rc = stokes_hip_gmres (c, n, op_mob, &o, b, x_dev, iters, relres);
     }
   else
     {
-      /* b = M (F, T, 0) - (I + M L) (0, 0, E) (calc_b_mob_lub_3fts,
-       * ewald-3fts.c:776-833) */
-      rc = shk_dev_mix_split (d, b_dev, sys->np, sys->np, 11, 6, o.t1, o.t2);
+      rc = shk_dev_mix_split (d, b_dev, sys->np, nm, o.D, o.qs, o.t1, o.t2);
       if (!rc) rc = shk_dev_atimes (d, o.t1, b, 1, c->prec);
       if (!rc) rc = shk_dev_axpby (d, b, o.t2, -1.0, 1.0, n);
-      if (!rc) rc = shk_dev_lub (d, 2, &o.lp, o.t2, o.t3);
+      if (!rc) rc = shk_dev_lub (d, version, &o.lp, o.t2, o.t3);
       if (!rc) rc = shk_dev_atimes (d, o.t3, o.t1, 1, c->prec);
       if (!rc) rc = shk_dev_axpby (d, b, o.t1, -1.0, 1.0, n);
-      if (!rc) rc = stokes_hip_gmres (c, n, op_mob_lub, &o, b, x_dev, iters, relres);
+      if (!rc) rc = stokes_hip_gmres (c, n, op_mix_lub, &o, b, x_dev, iters, relres);
     }
   shk_dev_free (d, ws);
   if (rc) stokes_hip_set_error ("%s: %s", what, shk_last_error ());
@@ -277,7 +302,7 @@ int
 stokes_hip_solve_res_lub_3fts_dev (struct stokes *sys, const double *uoe_dev,
                                    double *fts_dev, int *iters, double *relres)
 {
-  return lub_solve_dev (sys, 0, 0, uoe_dev, fts_dev, iters, relres,
+  return lub_solve_dev (sys, 2, 0, sys ? sys->np : 0, 0, uoe_dev, fts_dev, iters, relres,
                         "solve_res_lub_3fts");
 }
 
@@ -285,7 +310,7 @@ int
 stokes_hip_solve_mob_lub_3fts_dev (struct stokes *sys, const double *fte_dev,
                                    double *uos_dev, int *iters, double *relres)
 {
-  return lub_solve_dev (sys, 1, 0, fte_dev, uos_dev, iters, relres,
+  return lub_solve_dev (sys, 2, 1, sys ? sys->np : 0, 0, fte_dev, uos_dev, iters, relres,
                         "solve_mob_lub_3fts");
 }
 
@@ -298,8 +323,34 @@ stokes_hip_solve_lub_3fts_matrix_dev (struct stokes *sys, int kind,
       stokes_hip_set_error ("solve_lub_3fts_matrix: kind %d", kind);
       return STOKES_HIP_EARG;
     }
-  return lub_solve_dev (sys, kind, 1, b_dev, x_dev, NULL, NULL,
+  return lub_solve_dev (sys, 2, kind, sys ? sys->np : 0, 1, b_dev, x_dev, NULL, NULL,
                         "solve_lub_3fts_matrix");
+}
+
+int
+stokes_hip_solve_res_lub_dev (struct stokes *sys, const double *v_dev, double *f_dev,
+                              int *iters, double *relres)
+{
+  if (!sys || sys->version < 0 || sys->version > 2)
+    {
+      stokes_hip_set_error ("solve_res_lub: no system");
+      return STOKES_HIP_EARG;
+    }
+  return lub_solve_dev (sys, sys->version, 0, sys->np, 0, v_dev, f_dev, iters, relres,
+                        "solve_res_lub");
+}
+
+int
+stokes_hip_solve_mix_lub_dev (struct stokes *sys, const double *b_dev, double *x_dev,
+                              int *iters, double *relres)
+{
+  if (!sys || sys->version < 0 || sys->version > 2)
+    {
+      stokes_hip_set_error ("solve_mix_lub: no system");
+      return STOKES_HIP_EARG;
+    }
+  return lub_solve_dev (sys, sys->version, 1, sys->nm, 0, b_dev, x_dev, iters, relres,
+                        "solve_mix_lub");
 }
 
 /* ---- the reference's host-array entry points ---- */
@@ -349,7 +400,7 @@ lub_host_solve (struct stokes *sys, int kind, int matrix, const double *a,
   int rc = stokes_hip_malloc (&bd, sizeof (double) * n);
   if (!rc) rc = stokes_hip_malloc (&xd, sizeof (double) * n);
   if (!rc) rc = stokes_hip_memcpy_h2d (bd, h, sizeof (double) * n);
-  if (!rc) rc = lub_solve_dev (sys, kind, matrix, bd, xd, NULL, NULL, name);
+  if (!rc) rc = lub_solve_dev (sys, 2, kind, sys->np, matrix, bd, xd, NULL, NULL, name);
   if (!rc) rc = stokes_hip_memcpy_d2h (h, xd, sizeof (double) * n);
   if (bd) stokes_hip_free (bd);
   if (xd) stokes_hip_free (xd);
@@ -463,4 +514,239 @@ solve_mob_lub_3fts_matrix (struct stokes *sys, const double *f,
                            double *o, double *s)
 {
   mob_labo (sys, 1, f, t, e, u, o, s);
+}
+
+/* ---- lubricated F and FT solves, the FTS mixed problem (host arrays) ---- */
+
+/* h[D np] in (known values), out (solution); exits with a message on a
+ * failure, as the reference's solvers have no error return */
+static void
+lub_host_v (struct stokes *sys, int version, int kind, int nm, double *h,
+            const char *name)
+{
+  static const char *vn[3] = {"F", "FT", "FTS"};
+  if (sys->version != version)
+    {
+      fprintf (stderr, "libstokes %s : the version is wrong. reset to %s\n", name,
+               vn[version]);
+      sys->version = version;
+    }
+  const size_t n = (size_t) sys->np * (version == 0 ? 3 : (version == 1 ? 6 : 11));
+  void *bd = NULL, *xd = NULL;
+  int rc = stokes_hip_malloc (&bd, sizeof (double) * n);
+  if (!rc) rc = stokes_hip_malloc (&xd, sizeof (double) * n);
+  if (!rc) rc = stokes_hip_memcpy_h2d (bd, h, sizeof (double) * n);
+  if (!rc) rc = lub_solve_dev (sys, version, kind, nm, 0, bd, xd, NULL, NULL, name);
+  if (!rc) rc = stokes_hip_memcpy_d2h (h, xd, sizeof (double) * n);
+  if (bd) stokes_hip_free (bd);
+  if (xd) stokes_hip_free (xd);
+  if (rc)
+    {
+      fprintf (stderr, "%s: %s\n", name, stokes_hip_last_error ());
+      exit (1);
+    }
+}
+
+/* (a_i, b_i) per particle of 6 from two arrays of 3 and back */
+static void
+pack_ft (int n, double *h, const double *a, const double *b)
+{
+  for (int i = 0; i < n; i++)
+    {
+      memcpy (h + 6 * (size_t) i, a + 3 * (size_t) i, sizeof (double) * 3);
+      memcpy (h + 6 * (size_t) i + 3, b + 3 * (size_t) i, sizeof (double) * 3);
+    }
+}
+
+static void
+unpack_ft (int n, const double *h, double *a, double *b)
+{
+  for (int i = 0; i < n; i++)
+    {
+      memcpy (a + 3 * (size_t) i, h + 6 * (size_t) i, sizeof (double) * 3);
+      memcpy (b + 3 * (size_t) i, h + 6 * (size_t) i + 3, sizeof (double) * 3);
+    }
+}
+
+void
+solve_res_lub_3f_0 (struct stokes *sys, const double *u, double *f)
+{
+  const size_t n = (size_t) sys->np * 3;
+  double *h = lub_alloc (n);
+  memcpy (h, u, sizeof (double) * n);
+  lub_host_v (sys, 0, 0, sys->np, h, "solve_res_lub_3f_0");
+  memcpy (f, h, sizeof (double) * n);
+  free (h);
+}
+
+void
+solve_res_lub_3f (struct stokes *sys, const double *u, double *f)
+{
+  double *u0 = lub_alloc (3 * (size_t) sys->np);
+  shift_labo_to_rest_U (sys, sys->np, u, u0);
+  solve_res_lub_3f_0 (sys, u0, f);
+  free (u0);
+}
+
+/* y = (I + M L) x (ewald-3f.c:410-436) */
+void
+atimes_mob_lub_3f (int n, const double *x, double *y, void *user_data)
+{
+  struct stokes *sys = user_data;
+  if (sys->version != 0)
+    {
+      fprintf (stderr, "libstokes: version is wrong. reset to F.\n");
+      sys->version = 0;
+    }
+  double *l = lub_alloc ((size_t) n), *ml = lub_alloc ((size_t) n);
+  calc_lub_3f (sys, x, l);
+  atimes_3all (n, l, ml, sys);
+  for (int i = 0; i < n; i++) y[i] = x[i] + ml[i];
+  free (l);
+  free (ml);
+}
+
+void
+solve_mob_lub_3f (struct stokes *sys, const double *f, double *u)
+{
+  const size_t n = (size_t) sys->np * 3;
+  double *h = lub_alloc (n);
+  memcpy (h, f, sizeof (double) * n);
+  lub_host_v (sys, 0, 1, sys->np, h, "solve_mob_lub_3f");
+  memcpy (u, h, sizeof (double) * n);
+  free (h);
+  shift_rest_to_labo_U (sys, sys->np, u);
+}
+
+void
+solve_mix_lub_3f (struct stokes *sys, const double *f, const double *uf, double *u,
+                  double *ff)
+{
+  const int np = sys->np, nm = sys->nm, nf = np - nm;
+  if (nm == np)
+    {
+      solve_mob_lub_3f (sys, f, u);
+      return;
+    }
+  double *h = lub_alloc ((size_t) np * 3);
+  memcpy (h, f, sizeof (double) * 3 * (size_t) nm);
+  /* the shift takes the positions of particles 0..nf-1, as the reference
+   * does (ewald-3f.c:685) */
+  shift_labo_to_rest_U (sys, nf, uf, h + 3 * (size_t) nm);
+  lub_host_v (sys, 0, 1, nm, h, "solve_mix_lub_3f");
+  memcpy (u, h, sizeof (double) * 3 * (size_t) nm);
+  memcpy (ff, h + 3 * (size_t) nm, sizeof (double) * 3 * (size_t) nf);
+  free (h);
+  shift_rest_to_labo_U (sys, nm, u);
+}
+
+void
+solve_res_lub_3ft_0 (struct stokes *sys, const double *u, const double *o, double *f,
+                     double *t)
+{
+  double *h = lub_alloc ((size_t) sys->np * 6);
+  pack_ft (sys->np, h, u, o);
+  lub_host_v (sys, 1, 0, sys->np, h, "solve_res_lub_3ft_0");
+  unpack_ft (sys->np, h, f, t);
+  free (h);
+}
+
+void
+solve_res_lub_3ft (struct stokes *sys, const double *u, const double *o, double *f,
+                   double *t)
+{
+  const size_t n3 = (size_t) sys->np * 3;
+  double *u0 = lub_alloc (2 * n3), *o0 = u0 + n3;
+  shift_labo_to_rest_U (sys, sys->np, u, u0);
+  shift_labo_to_rest_O (sys, sys->np, o, o0);
+  solve_res_lub_3ft_0 (sys, u0, o0, f, t);
+  free (u0);
+}
+
+void
+solve_mob_lub_3ft (struct stokes *sys, const double *f, const double *t, double *u,
+                   double *o)
+{
+  double *h = lub_alloc ((size_t) sys->np * 6);
+  pack_ft (sys->np, h, f, t);
+  lub_host_v (sys, 1, 1, sys->np, h, "solve_mob_lub_3ft");
+  unpack_ft (sys->np, h, u, o);
+  free (h);
+  shift_rest_to_labo_U (sys, sys->np, u);
+  shift_rest_to_labo_O (sys, sys->np, o);
+}
+
+void
+solve_mix_lub_3ft (struct stokes *sys, const double *f, const double *t,
+                   const double *uf, const double *of, double *u, double *o, double *ff,
+                   double *tf)
+{
+  const int np = sys->np, nm = sys->nm, nf = np - nm;
+  if (nm == np)
+    {
+      solve_mob_lub_3ft (sys, f, t, u, o);
+      return;
+    }
+  const size_t nf3 = (size_t) nf * 3;
+  double *h = lub_alloc ((size_t) np * 6), *w = lub_alloc (2 * nf3 + 1);
+  pack_ft (nm, h, f, t);
+  /* positions of particles 0..nf-1 for the shift, as the reference's
+   * (ewald-3ft.c:782-783) */
+  shift_labo_to_rest_U (sys, nf, uf, w);
+  shift_labo_to_rest_O (sys, nf, of, w + nf3);
+  pack_ft (nf, h + 6 * (size_t) nm, w, w + nf3);
+  lub_host_v (sys, 1, 1, nm, h, "solve_mix_lub_3ft");
+  unpack_ft (nm, h, u, o);
+  unpack_ft (nf, h + 6 * (size_t) nm, ff, tf);
+  free (h);
+  free (w);
+  shift_rest_to_labo_U (sys, nm, u);
+  shift_rest_to_labo_O (sys, nm, o);
+}
+
+void
+solve_mix_lub_3fts_0 (struct stokes *sys, const double *f, const double *t,
+                      const double *e, const double *uf, const double *of,
+                      const double *ef, double *u, double *o, double *s, double *ff,
+                      double *tf, double *sf)
+{
+  const int np = sys->np, nm = sys->nm, nf = np - nm;
+  if (nm == np)
+    {
+      solve_mob_lub_3fts_0 (sys, f, t, e, u, o, s);
+      return;
+    }
+  double *h = lub_alloc ((size_t) np * 11);
+  pack11 (nm, h, f, t, e);
+  pack11 (nf, h + 11 * (size_t) nm, uf, of, ef);
+  lub_host_v (sys, 2, 1, nm, h, "solve_mix_lub_3fts_0");
+  unpack11 (nm, h, u, o, s);
+  unpack11 (nf, h + 11 * (size_t) nm, ff, tf, sf);
+  free (h);
+}
+
+void
+solve_mix_lub_3fts (struct stokes *sys, const double *f, const double *t, const double *e,
+                    const double *uf, const double *of, const double *ef, double *u,
+                    double *o, double *s, double *ff, double *tf, double *sf)
+{
+  const int np = sys->np, nm = sys->nm, nf = np - nm;
+  if (nm == np)
+    {
+      solve_mob_lub_3fts (sys, f, t, e, u, o, s);
+      return;
+    }
+  const size_t nf3 = (size_t) nf * 3;
+  double *w = lub_alloc (2 * nf3 + 5 * (size_t) nf + 5 * (size_t) nm);
+  double *uf0 = w, *of0 = w + nf3, *ef0 = w + 2 * nf3, *e0 = ef0 + 5 * (size_t) nf;
+  /* positions of particles 0..nf-1 (and 0..nm-1 for e), as the
+   * reference's shifts (ewald-3fts.c:1288-1291) */
+  shift_labo_to_rest_U (sys, nf, uf, uf0);
+  shift_labo_to_rest_O (sys, nf, of, of0);
+  shift_labo_to_rest_E (sys, nf, ef, ef0);
+  shift_labo_to_rest_E (sys, nm, e, e0);
+  solve_mix_lub_3fts_0 (sys, f, t, e0, uf0, of0, ef0, u, o, s, ff, tf, sf);
+  free (w);
+  shift_rest_to_labo_U (sys, nm, u);
+  shift_rest_to_labo_O (sys, nm, o);
 }

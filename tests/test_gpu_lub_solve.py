@@ -214,3 +214,152 @@ def test_labo_frame_wrappers(matrix):
     _say(f"labo-frame lubricated wrappers matrix={matrix}: mob {err:.2e}")
     assert err < 1e-12
     s.close()
+
+
+def _mix_case(version, case, nfix):
+    """a lubricating configuration with its last nfix particles fixed"""
+    a = None
+    if case == "poly":
+        pos, a, L = workloads.configuration_poly(12, 0.25, seed=5, gap=1.01,
+                                                 a_lo=0.5, a_hi=2.0)
+    else:
+        pos, L = workloads.configuration(16, 0.2, seed=3, gap=2.02)
+    kw = {} if case == "open" else {"box": (L, L, L), "tr": 1.0}
+    s = ls.Stokes(version, pos, a=a, nm=len(pos) - nfix, **kw)
+    o = po.System(version, pos, a=a, **kw)
+    s.set_lub(4.0 + 1e-12, 3.0, 100, 1)
+    s.set_iter(max_it=2000, restart=40, eps=1e-12)
+    return pos, s, o
+
+
+def _mix_restated(M, Lm, b, npart, nm, version):
+    """unknowns [(U, O, S)_m, (F, T, S)_f] of M g = (I + M L) v with the
+    known [(F, T, E)_m, (U, O, E)_f] in b (K: the mobile particles' first
+    qs values, unknown velocities / known forces; H: the rest)"""
+    D = (3, 6, 11)[version]
+    qs = 6 if version == 2 else D
+    n = D * npart
+    P = np.eye(n) + M @ Lm
+    q, i = np.arange(n) % D, np.arange(n) // D
+    kmask = (i < nm) & (q < qs)
+    K, H = np.flatnonzero(kmask), np.flatnonzero(~kmask)
+    A = np.concatenate([P[:, K], -M[:, H]], axis=1)
+    rhs = M[:, K] @ b[K] - P[:, H] @ b[H]
+    z = np.linalg.solve(A, rhs)
+    x = np.empty(n)
+    x[K] = z[:K.size]
+    x[H] = z[K.size:]
+    return x
+
+
+@pytest.mark.parametrize("version,case,nfix", [
+    (0, "periodic", 0), (0, "periodic", 3), (0, "open", 2), (1, "periodic", 0),
+    (1, "periodic", 3), (1, "poly", 2), (2, "periodic", 3), (2, "poly", 2)])
+def test_solve_mix_lub_any_version(version, case, nfix):
+    """lubricated mobility (nfix = 0) and mixed problems of every version
+    on the device (GMRES, stokes_hip_solve_mix_lub_dev: solve_mob_lub_3f,
+    solve_mix_lub_3f, ewald-3f.c:446-705; the FT forms, ewald-3ft.c:
+    527-805; solve_mix_lub_3fts_0, ewald-3fts.c:1192-1234) against numpy's
+    dense solve of the oracle's maps, 1e-9; the F and FT resistance forms
+    (solve_res_lub_3f_0, ewald-3f.c:373-406; solve_res_lub_3ft_0,
+    ewald-3ft.c:441-485) likewise"""
+    pos, s, o = _mix_case(version, case, nfix)
+    npart = len(pos)
+    nm = npart - nfix
+    Mx = o.matrix()
+    M = po.trans_mat_ext2ext(npart, Mx) if version == 2 else Mx
+    Lm = o.lub_columns()
+    rng = np.random.default_rng(31 + version + nfix)
+    b = rng.standard_normal(s.n)
+    want = _mix_restated(M, Lm, b, npart, nm, version)
+    x, it, rr = s.solve_lub_any(b, "mix")
+    e = relerr(x, want)
+    _say(f"mix lub v{version} {case} nfix={nfix}: GMRES {it} it residual "
+         f"{rr:.2e} err {e:.2e}")
+    assert rr < 1e-12 and e <= 1e-9
+    if version < 2 and nfix == 0:
+        v = rng.standard_normal(s.n)
+        want = np.linalg.solve(M, v + M @ (Lm @ v))
+        x, it, rr = s.solve_lub_any(v, "res")
+        e = relerr(x, want)
+        _say(f"res lub v{version} {case}: GMRES {it} it err {e:.2e}")
+        assert rr < 1e-12 and e <= 1e-9
+    s.close()
+
+
+def test_lub_host_entry_points_f_ft_fts():
+    """the host-array forms (solve_res_lub_3f(_0), atimes_mob_lub_3f,
+    solve_mob_lub_3f, solve_mix_lub_3f, the FT forms and
+    solve_mix_lub_3fts(_0)) equal the device solves in a fluid at rest,
+    where the labo and fluid-rest frames coincide"""
+    P = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))
+    n, nfix = 16, 3
+    nm = n - nfix
+    rng = np.random.default_rng(37)
+    # F
+    pos, s, o = _mix_case(0, "periodic", nfix)
+    f, uf = rng.standard_normal(3 * nm), rng.standard_normal(3 * nfix)
+    u, ff = np.zeros(3 * nm), np.zeros(3 * nfix)
+    s.L.solve_mix_lub_3f(s.sys, P(f), P(uf), P(u), P(ff))
+    x, _, _ = s.solve_lub_any(np.concatenate([f, uf]), "mix")
+    assert relerr(np.concatenate([u, ff]), x) < 1e-12
+    s.close()
+    pos, s, o = _mix_case(0, "periodic", 0)
+    v = rng.standard_normal(3 * n)
+    f0, f1 = np.zeros(3 * n), np.zeros(3 * n)
+    s.L.solve_res_lub_3f_0(s.sys, P(v), P(f0))
+    s.L.solve_res_lub_3f(s.sys, P(v), P(f1))
+    x, _, _ = s.solve_lub_any(v, "res")
+    assert relerr(f0, x) < 1e-12 and relerr(f1, x) < 1e-12
+    y = np.zeros(3 * n)
+    s.L.atimes_mob_lub_3f(3 * n, P(v), P(y), s.sys)
+    Lm = o.lub_columns()
+    assert relerr(y, v + o.atimes(Lm @ v)) <= 1e-12
+    u = np.zeros(3 * n)
+    s.L.solve_mob_lub_3f(s.sys, P(v), P(u))
+    x, _, _ = s.solve_lub_any(v, "mix")
+    assert relerr(u, x) < 1e-12
+    s.close()
+    # FT
+    pos, s, o = _mix_case(1, "periodic", nfix)
+    f, t = rng.standard_normal(3 * nm), rng.standard_normal(3 * nm)
+    uf, of = rng.standard_normal(3 * nfix), rng.standard_normal(3 * nfix)
+    u, om = np.zeros(3 * nm), np.zeros(3 * nm)
+    ff, tf = np.zeros(3 * nfix), np.zeros(3 * nfix)
+    s.L.solve_mix_lub_3ft(s.sys, P(f), P(t), P(uf), P(of), P(u), P(om),
+                          P(ff), P(tf))
+    pk = lambda a, c, k: np.concatenate([a.reshape(k, 3), c.reshape(k, 3)],
+                                        1).reshape(-1)
+    x, _, _ = s.solve_lub_any(np.concatenate([pk(f, t, nm), pk(uf, of, nfix)]),
+                              "mix")
+    assert relerr(np.concatenate([pk(u, om, nm), pk(ff, tf, nfix)]), x) < 1e-12
+    s.close()
+    pos, s, o = _mix_case(1, "periodic", 0)
+    uu, oo = rng.standard_normal(3 * n), rng.standard_normal(3 * n)
+    fr, tr = np.zeros(3 * n), np.zeros(3 * n)
+    s.L.solve_res_lub_3ft(s.sys, P(uu), P(oo), P(fr), P(tr))
+    x, _, _ = s.solve_lub_any(pk(uu, oo, n), "res")
+    assert relerr(pk(fr, tr, n), x) < 1e-12
+    um, omm = np.zeros(3 * n), np.zeros(3 * n)
+    s.L.solve_mob_lub_3ft(s.sys, P(uu), P(oo), P(um), P(omm))
+    x, _, _ = s.solve_lub_any(pk(uu, oo, n), "mix")
+    assert relerr(pk(um, omm, n), x) < 1e-12
+    s.close()
+    # FTS mixed
+    pos, s, o = _mix_case(2, "periodic", nfix)
+    f, t, e = (rng.standard_normal(k * nm) for k in (3, 3, 5))
+    uf, of, ef = (rng.standard_normal(k * nfix) for k in (3, 3, 5))
+    outs = [np.zeros(k * nm) for k in (3, 3, 5)] + \
+        [np.zeros(k * nfix) for k in (3, 3, 5)]
+    pk3 = lambda a, c, d, k: np.concatenate(
+        [a.reshape(k, 3), c.reshape(k, 3), d.reshape(k, 5)], 1).reshape(-1)
+    x, _, _ = s.solve_lub_any(np.concatenate([pk3(f, t, e, nm),
+                                              pk3(uf, of, ef, nfix)]), "mix")
+    for name in ("solve_mix_lub_3fts_0", "solve_mix_lub_3fts"):
+        for a in outs:
+            a[:] = 0.0
+        getattr(s.L, name)(s.sys, P(f), P(t), P(e), P(uf), P(of), P(ef),
+                           *[P(a) for a in outs])
+        got = np.concatenate([pk3(*outs[:3], nm), pk3(*outs[3:], nfix)])
+        assert relerr(got, x) < 1e-12, name
+    s.close()
