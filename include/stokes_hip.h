@@ -369,6 +369,16 @@ int stokes_hip_bd_set_rng_mt19937 (stokes_hip_bd *bd, unsigned long seed);
  * evaluated at the intermediate configuration, as the reference does,
  * 2770-2816), 2 Ball-Melrose 1997 (BD_evolve_BM97, 2970-3190) */
 int stokes_hip_bd_set_scheme (stokes_hip_bd *bd, int scheme, double bb_n);
+/* BD_params_init's flag_lub (brownian.c:154): with enable the drift of
+ * every stage is the lubricated mixed problem of solve_mix_3all
+ * (solve_mix_lub_3f / 3ft / 3fts, ode.c:41-160; stokes_hip_solve_mix_lub_dev
+ * over the correction of calc_lub_3f / 3ft / 3fts with sys's lubrication
+ * parameters), and the attempts run eagerly (GMRES steered from the host).
+ * The noise stays M^-1/2 y, as in the reference, whose lubricated noise
+ * branch (flag_lub_B, brownian.c:1734-1849) is never taken.  0 or a
+ * STOKES_HIP_E* code */
+int stokes_hip_bd_set_lub (stokes_hip_bd *bd, int enable);
+
 /* FT / FTS: constant torques [3 nm] (BD->T, brownian.c:2296-2299) or NULL,
  * read on every step */
 int stokes_hip_bd_set_torque (stokes_hip_bd *bd, const double *T_dev);

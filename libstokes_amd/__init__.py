@@ -78,7 +78,7 @@ EXPORTS = [
     "stokes_hip_bd_step_q", "stokes_hip_bd_set_torque",
     "stokes_hip_bd_step_t", "stokes_hip_bd_set_shear_ref",
     "stokes_hip_bd_set_fts_iter", "stokes_hip_ensemble_autotune",
-    "stokes_hip_bd_set_scheme", "stokes_hip_set_dense_blocks",
+    "stokes_hip_bd_set_scheme", "stokes_hip_bd_set_lub", "stokes_hip_set_dense_blocks",
     "stokes_hip_bd_set_rng_mt19937",
     # lub.h, twobody.h
     "calc_lub_3f", "calc_lub_3ft", "calc_lub_3fts", "make_matrix_lub_3f",
@@ -312,6 +312,7 @@ def lib():
     L.stokes_hip_bd_set_shear_ref.argtypes = [_vp, C.c_double, C.c_double]
     L.stokes_hip_bd_set_torque.argtypes = [_vp, _vp]
     L.stokes_hip_bd_set_scheme.argtypes = [_vp, C.c_int, C.c_double]
+    L.stokes_hip_bd_set_lub.argtypes = [_vp, C.c_int]
     L.stokes_hip_bd_set_rng_mt19937.argtypes = [_vp, C.c_ulong]
     L.stokes_hip_bd_set_fts_iter.argtypes = [_vp, C.c_int, C.c_int,
                                              C.c_double]
@@ -775,6 +776,12 @@ class BD:
         """draw the noise from the reference's MT19937 + polar stream"""
         check(self.L.stokes_hip_bd_set_rng_mt19937(self.h, int(seed)),
               "bd_set_rng_mt19937")
+
+    def set_lub(self, on=True):
+        """the drift of each stage with lubrication (BD_params_init's
+        flag_lub): the lubricated mixed problem, attempts run eagerly"""
+        check(self.L.stokes_hip_bd_set_lub(self.h, 1 if on else 0),
+              "bd_set_lub")
 
     def set_scheme(self, scheme, bb_n=1.0):
         """0 mid-point, 1 Banchio-Brady 2003 (step parameter bb_n), 2

@@ -25,6 +25,7 @@
 #include <stdlib.h>
 #include <string.h>
 #include "chebyshev.h"
+#include "lub.h"
 #include "stokes_hip.h"
 #include "stokes_internal.h"
 
@@ -61,6 +62,7 @@ struct stokes_hip_bd {
   int scheme;                  /* 0 mid-point, 1 BB03, 2 BM97 */
   double bb_n;                 /* BB03 step parameter */
   int real_cache;              /* dense real-space matrix at x (32 products) */
+  int lub;                     /* flag_lub: the drift with lubrication */
   int have_bounds;
   int fits;                    /* Lanczos fits so far */
   double eig[2];
@@ -395,6 +397,35 @@ stokes_hip_bd_set_scheme (stokes_hip_bd *bd, int scheme, double bb_n)
 }
 
 int
+stokes_hip_bd_set_lub (stokes_hip_bd *bd, int enable)
+{
+  if (!bd)
+    {
+      stokes_hip_set_error ("bd_set_lub: no integrator");
+      return STOKES_HIP_EARG;
+    }
+  if (enable && !bd->fts_buf)
+    {
+      /* the known values and the solution of the lubricated mixed problem */
+      const int dmix = bd->ver == 2 ? 11 : (bd->ver == 0 ? 3 : 6);
+      const size_t len = 2 * (size_t) dmix * bd->np;
+      int rc = shk_dev_malloc (bd->c->dev, (void **) &bd->fts_buf, sizeof (double) * len);
+      if (!rc) rc = shk_dev_scal (bd->c->dev, bd->fts_buf, NULL, 0.0, len);
+      if (rc)
+        {
+          stokes_hip_set_error ("bd_set_lub: allocation failed: %s", shk_last_error ());
+          return STOKES_HIP_ENOMEM;
+        }
+      bd->fts_b = bd->fts_buf;
+      bd->fts_x = bd->fts_b + (size_t) dmix * bd->np;
+    }
+  bd->lub = enable != 0;
+  shk_dev_graph_destroy (bd->graph);
+  bd->graph = NULL;
+  return 0;
+}
+
+int
 stokes_hip_bd_set_torque (stokes_hip_bd *bd, const double *T_dev)
 {
   if (!bd || bd->ver == 0)
@@ -473,6 +504,39 @@ bd_mob (stokes_hip_bd *bd, shk_dev *d, int prec, const double *x, double *y,
    * frame */
   const int flow = !noise && bd->has_flow;
   const int nf = bd->np - bd->nm;
+  if (!noise && bd->lub)
+    {
+      /* the drift with lubrication (flag_lub: solve_mix_3all ->
+       * solve_mix_lub_3f / 3ft / 3fts, ode.c:41-160): the known values
+       * [(F, T, E = 0)_mobile, (U, O, E)_fixed] in the fluid-rest frame
+       * (the fixed particles at rest; with an imposed flow the shifts of
+       * the drift without lubrication), the lubricated mixed problem by
+       * GMRES with stokes_set_iter's parameters, the mobile velocities
+       * back in the labo frame */
+      const int D = bd->ver == 0 ? 3 : (bd->ver == 1 ? 6 : 11);
+      const int dv = bd->ver == 2 ? 6 : D;
+      int rc = 0, it = 0;
+      double rr = 0.0;
+      if (bd->ver == 2) rc = shk_dev_ft_fts (d, x, bd->nm, 0, bd->fts_b);
+      else rc = shk_dev_scal (d, bd->fts_b, x, 1.0, bd->n);
+      if (!rc && nf > 0)
+        rc = shk_dev_scal (d, bd->fts_b + (size_t) D * bd->nm, NULL, 0.0, (size_t) D * nf);
+      if (!rc && flow && bd->ver == 2)
+        rc = shk_dev_bd_imposed (d, bd->fts_b, bd->pcur, bd->nm, 11, 6, 11, &bd->flow, -1.0);
+      if (!rc && flow && nf > 0)
+        rc = shk_dev_bd_imposed (d, bd->fts_b + (size_t) D * bd->nm, bd->pcur, nf, D, 0, D,
+                                 &bd->flow, -1.0);
+      if (!rc) rc = stokes_hip_solve_mix_lub_here (bd->sys, bd->fts_b, bd->fts_x, &it, &rr);
+      bd->fts_iters += it;
+      if (!rc)
+        {
+          if (bd->ver == 2) rc = shk_dev_ft_fts (d, bd->fts_x, bd->nm, 1, y);
+          else rc = shk_dev_scal (d, y, bd->fts_x, 1.0, bd->n);
+        }
+      if (!rc && flow)
+        rc = shk_dev_bd_imposed (d, y, bd->pcur, bd->nm, dv, 0, dv, &bd->flow, 1.0);
+      return rc;
+    }
   if (bd->ver != 2)
     {
       const int D = bd->ver == 0 ? 3 : 6;
@@ -872,7 +936,7 @@ bd_step (stokes_hip_bd *bd, double *x_dev, double *q_dev, int have_t,
   double fact = bd_fact (bd->peclet, dt_local);
   /* an attempt that steers GMRES from the host runs eagerly (FTS, and the
    * F / FT drift with fixed particles); the others replay a graph */
-  const int eager = bd->ver == 2 || bd->nm < bd->np || bd->shear;
+  const int eager = bd->ver == 2 || bd->nm < bd->np || bd->shear || bd->lub;
   /* the cell shift at t (stokes_set_shear_shift at REDO, brownian.c:2393) */
   const double sh_t = bd->shear
     ? stokes_get_shear_shift (bd->sys, t, bd->t0, bd->s0) : 0.0;

@@ -219,9 +219,25 @@ op_mix_lub (void *ud, const double *x, double *y)
  * 1015-1076), nm = np being the mobility problem.  matrix (FTS, nm = np):
  * dense solve instead of GMRES.  All in the fluid-rest frame. */
 static int
+lub_solve_dev_pos (struct stokes *sys, int version, int kind, int nm, int matrix,
+                   const double *b_dev, double *x_dev, int *iters, double *relres,
+                   const char *what, int keep_pos);
+
+static int
 lub_solve_dev (struct stokes *sys, int version, int kind, int nm, int matrix,
                const double *b_dev, double *x_dev, int *iters, double *relres,
                const char *what)
+{
+  return lub_solve_dev_pos (sys, version, kind, nm, matrix, b_dev, x_dev, iters, relres,
+                            what, 0);
+}
+
+/* keep_pos: the positions the device context holds now (set by the
+ * caller, e.g. a Brownian step's stage), no set-up or upload */
+static int
+lub_solve_dev_pos (struct stokes *sys, int version, int kind, int nm, int matrix,
+                   const double *b_dev, double *x_dev, int *iters, double *relres,
+                   const char *what, int keep_pos)
 {
   static const char *vn[3] = {"F", "FT", "FTS"};
   if (!sys || sys->version != version || !b_dev || !x_dev)
@@ -237,7 +253,7 @@ lub_solve_dev (struct stokes *sys, int version, int kind, int nm, int matrix,
     }
   stokes_ctx *c = stokes_hip_ctx (sys);
   if (!c) return STOKES_HIP_ENODEV;
-  int rc = stokes_hip_prepare (c, c->pos_dev != NULL);
+  int rc = keep_pos ? 0 : stokes_hip_prepare (c, c->pos_dev != NULL);
   if (rc) return rc;
   lub_op o;
   memset (&o, 0, sizeof (o));
@@ -338,6 +354,14 @@ stokes_hip_solve_res_lub_dev (struct stokes *sys, const double *v_dev, double *f
     }
   return lub_solve_dev (sys, sys->version, 0, sys->np, 0, v_dev, f_dev, iters, relres,
                         "solve_res_lub");
+}
+
+int
+stokes_hip_solve_mix_lub_here (struct stokes *sys, const double *b_dev, double *x_dev,
+                               int *iters, double *relres)
+{
+  return lub_solve_dev_pos (sys, sys->version, 1, sys->nm, 0, b_dev, x_dev, iters, relres,
+                            "solve_mix_lub", 1);
 }
 
 int

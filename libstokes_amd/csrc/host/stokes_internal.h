@@ -130,4 +130,10 @@ typedef struct stokes_hip_mt {
 void stokes_hip_mt_seed (stokes_hip_mt *r, unsigned long seed);
 double stokes_hip_mt_gaussian (stokes_hip_mt *r);
 
+/* stokes_hip_solve_mix_lub_dev at the positions the device context holds
+ * now (a Brownian step's stage sets them), without the set-up of the
+ * public entry point (lub.c) */
+int stokes_hip_solve_mix_lub_here (struct stokes *sys, const double *b_dev,
+                                   double *x_dev, int *iters, double *relres);
+
 #endif

@@ -869,3 +869,100 @@ def test_bd_sheared_retries_match_oracle(scheme, halvings):
     _say(f"  x err / disp {ex:.2e}")
     assert ex < 1e-10
     bd.close()
+
+
+@pytest.mark.parametrize("version,n", [(0, 16), (1, 16), (2, 12)])
+def test_bd_step_with_lubrication(version, n):
+    """flag_lub (stokes_hip_bd_set_lub): the drift of both stages of the
+    mid-point step is the lubricated mobility problem (solve_mix_lub_3f ->
+    solve_mob_lub_3f, ewald-3f.c:446-705; the FT form, ewald-3ft.c:527-805;
+    solve_mix_lub_3fts, ewald-3fts.c:1255-1306, with E = 0).  Composed in
+    numpy from the device's noise z and the oracle's maps M, L at x and at
+    the mid-point (brownian.c:2416-2510): f = (F, T) + fact z, U1 from
+    (I + M L) v = M f at x (FTS: the mixed problem, known f and E = 0,
+    unknown U, O, S), x_mid = x + dt/2 U1, U2 the same at x_mid, x_new = x +
+    dt U2; within 1e-9 of the displacement, and apart from the step without
+    lubrication"""
+    pe, dt, nch = 1.0, 1e-3, 30
+    D = (3, 6, 11)[version]
+    dn = 3 if version == 0 else 6
+    pos, L = workloads.configuration(n, 0.2, seed=3, gap=2.02)
+    s = ls.Stokes(version, pos, box=(L, L, L), tr=1.0)
+    s.set_lub(4.0 + 1e-12, 3.0, 100, 1)
+    s.set_iter(max_it=2000, restart=40, eps=1e-13)
+    o = po.System(version, pos, box=(L, L, L), tr=1.0)
+    if version == 0:
+        w, _ = _spectrum(o)
+    else:
+        G = o.matrix() if version == 1 else po.ft_mobility_dense(o)
+        w = np.linalg.eigvalsh(0.5 * (G + G.T))
+    e0, e1 = 0.95 * w[0], 1.05 * w[-1]
+    rng = np.random.default_rng(5 + version)
+    F = rng.standard_normal(3 * n)
+    T = rng.standard_normal(3 * n)
+    Fd = ls.DeviceArray.from_host(F)
+    Td = ls.DeviceArray.from_host(T)
+    bd = ls.BD(s, peclet=pe, n_cheb=nch, eps=1e-6, dt_lim=1e-6, rmin_ol=1.0,
+               seed=7, force_dev=Fd, bounds=(e0, e1))
+    if version > 0:
+        bd.set_torque(Td)
+    if version == 2:
+        bd.set_fts_iter(30, 2000, 1e-13)
+    bd.set_lub(True)
+    x = pos.reshape(-1).copy()
+    xd = ls.DeviceArray.from_host(x)
+    info = bd.step(xd, dt)
+    assert info.attempts == 1 and info.dt == dt
+    y, z = bd.last_noise()
+    assert z.size == dn * n
+    f = np.zeros((n, dn))
+    f[:, :3] = F.reshape(n, 3)
+    if version > 0:
+        f[:, 3:] = T.reshape(n, 3)
+    f = f.reshape(-1) + math.sqrt(2.0 / (pe * dt)) * z
+
+    def maps(xp):
+        o.pos[:] = xp
+        Mx = o.matrix()
+        M = po.trans_mat_ext2ext(n, Mx) if version == 2 else Mx
+        return M, o.lub_columns()
+
+    def drift(xp, lub=True):
+        M, Lm = maps(xp)
+        assert np.linalg.norm(Lm) > 0
+        if not lub:
+            Lm = np.zeros_like(Lm)
+        if version < 2:
+            v = np.linalg.solve(np.eye(D * n) + M @ Lm, M @ f)
+        else:
+            b = np.zeros((n, 11))
+            b[:, :6] = f.reshape(n, 6)
+            P = np.eye(D * n) + M @ Lm
+            q = np.arange(D * n) % D
+            K, H = np.flatnonzero(q < 6), np.flatnonzero(q >= 6)
+            A = np.concatenate([P[:, K], -M[:, H]], axis=1)
+            bb = b.reshape(-1)
+            sol = np.linalg.solve(A, M[:, K] @ bb[K] - P[:, H] @ bb[H])
+            v = np.empty(D * n)
+            v[K] = sol[:K.size]
+            v[H] = sol[K.size:]
+        return v.reshape(n, D)[:, :3].reshape(-1)
+
+    xm = x + 0.5 * dt * drift(x)
+    xn = x + dt * drift(xm)
+    xg = xd.to_host()
+    disp = np.abs(xn - x).max()
+    e = np.abs(xg - xn).max() / disp
+    # the same noise without lubrication moves the particles elsewhere
+    xm0 = x + 0.5 * dt * drift(x, False)
+    x0 = x + dt * drift(xm0, False)
+    d0 = np.abs(x0 - xn).max() / disp
+    _say(f"  lubricated step version={version} N={n}: |x_gpu - x_numpy| / disp "
+         f"= {e:.2e}, without lubrication {d0:.2e}")
+    assert e < 1e-9
+    assert d0 > 1e-3
+    bd.close()
+    Fd.free()
+    Td.free()
+    xd.free()
+    s.close()
