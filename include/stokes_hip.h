@@ -74,6 +74,13 @@ int stokes_hip_set_real_cache (struct stokes *sys, int enable);
  * of evaluating every pair once per 32-column group matrix-free.  Off by
  * default (matrix-free).  No counterpart in the reference. */
 int stokes_hip_set_dense_blocks (struct stokes *sys, int enable);
+/* enable (the default): FP64 products of more than one column run the
+ * reciprocal (wave-space) part on a second stream of the library beside the
+ * real-space part, joined before the two are added; 0 runs them one after
+ * the other on the library's stream.  MIXED products always run them one
+ * after the other (faster there).  The results do not depend on it.  No
+ * counterpart in the reference. */
+int stokes_hip_set_overlap (struct stokes *sys, int enable);
 /* time reps operator calls of nrhs columns at each candidate tratio (NULL:
  * 0 (= sys->xi), 10, 100, 300, 1000, 3000, 1e4) and keep the fastest as
  * the splitting of sys */

@@ -79,6 +79,7 @@ EXPORTS = [
     "stokes_hip_bd_step_t", "stokes_hip_bd_set_shear_ref",
     "stokes_hip_bd_set_fts_iter", "stokes_hip_ensemble_autotune",
     "stokes_hip_bd_set_scheme", "stokes_hip_bd_set_lub", "stokes_hip_set_dense_blocks",
+    "stokes_hip_set_overlap",
     "stokes_hip_bd_set_rng_mt19937",
     # lub.h, twobody.h
     "calc_lub_3f", "calc_lub_3ft", "calc_lub_3fts", "make_matrix_lub_3f",
@@ -190,6 +191,7 @@ def lib():
     L.stokes_hip_set_split.argtypes = [sp, C.c_double]
     L.stokes_hip_set_real_cache.argtypes = [sp, C.c_int]
     L.stokes_hip_set_dense_blocks.argtypes = [sp, C.c_int]
+    L.stokes_hip_set_overlap.argtypes = [sp, C.c_int]
     L.stokes_hip_solve_mix_dev.argtypes = [sp, _vp, _vp]
     L.stokes_hip_solve_mob_3fts_dev.argtypes = [sp, _vp, _vp,
                                                 C.POINTER(C.c_int), _dp]
@@ -564,6 +566,11 @@ class Stokes:
         following products (keep the positions fixed)"""
         check(self.L.stokes_hip_set_real_cache(self.sys, int(bool(on))),
               "set_real_cache")
+
+    def set_overlap(self, on=True):
+        """FP64 block products: reciprocal part on a second stream (default on)"""
+        check(self.L.stokes_hip_set_overlap(self.sys, int(bool(on))),
+              "set_overlap")
 
     def set_dense_blocks(self, on=True):
         """per-call dense real-space matrix for wide MIXED blocks"""

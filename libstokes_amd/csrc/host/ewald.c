@@ -679,6 +679,15 @@ stokes_hip_set_dense_blocks (struct stokes *sys, int enable)
 }
 
 int
+stokes_hip_set_overlap (struct stokes *sys, int enable)
+{
+  stokes_ctx *c = ctx_or_error (sys);
+  if (!c) return STOKES_HIP_ENODEV;
+  int rc = shk_dev_set_overlap (c->dev, enable);
+  return rc ? dev_fail (rc, "set_overlap") : 0;
+}
+
+int
 stokes_hip_set_split (struct stokes *sys, double tratio)
 {
   stokes_ctx *c = ctx_or_error (sys);

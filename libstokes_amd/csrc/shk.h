@@ -220,6 +220,9 @@ int shk_dev_adjoint_x (shk_dev *d, const double *f, const double *g,
 /* allow (on != 0) the per-call dense FP32 real-space matrix for MIXED
  * blocks wider than one wave's column tiles */
 int shk_dev_set_dense_blocks (shk_dev *d, int on);
+/* on != 0 (the default): FP64 block products run their reciprocal part on
+ * a second stream beside the real-space part */
+int shk_dev_set_overlap (shk_dev *d, int on);
 int shk_dev_real_cache (shk_dev *d, int on, int prec);
 
 /* y = a x + b y (len doubles) */
