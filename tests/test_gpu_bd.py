@@ -871,35 +871,42 @@ def test_bd_sheared_retries_match_oracle(scheme, halvings):
     bd.close()
 
 
-@pytest.mark.parametrize("version,n", [(0, 16), (1, 16), (2, 12)])
-def test_bd_step_with_lubrication(version, n):
+@pytest.mark.parametrize("version,n,nm", [(0, 16, 16), (1, 16, 16), (2, 12, 12),
+                                          (0, 16, 12), (2, 12, 9)])
+def test_bd_step_with_lubrication(version, n, nm):
     """flag_lub (stokes_hip_bd_set_lub): the drift of both stages of the
-    mid-point step is the lubricated mobility problem (solve_mix_lub_3f ->
-    solve_mob_lub_3f, ewald-3f.c:446-705; the FT form, ewald-3ft.c:527-805;
-    solve_mix_lub_3fts, ewald-3fts.c:1255-1306, with E = 0).  Composed in
-    numpy from the device's noise z and the oracle's maps M, L at x and at
-    the mid-point (brownian.c:2416-2510): f = (F, T) + fact z, U1 from
-    (I + M L) v = M f at x (FTS: the mixed problem, known f and E = 0,
-    unknown U, O, S), x_mid = x + dt/2 U1, U2 the same at x_mid, x_new = x +
-    dt U2; within 1e-9 of the displacement, and apart from the step without
-    lubrication"""
+    mid-point step is the lubricated mixed problem of solve_mix_3all
+    (solve_mix_lub_3f, ewald-3f.c:646-705 -> solve_mob_lub_3f when nothing
+    is fixed; the FT form, ewald-3ft.c:527-805; solve_mix_lub_3fts,
+    ewald-3fts.c:1255-1306, with E = 0), particles nm..n-1 fixed at rest.
+    Composed in numpy from the device's noise z and the oracle's maps M, L
+    at x and at the mid-point (brownian.c:2416-2510): f = (F, T) + fact z
+    on the mobile particles, v from (I + M L) v = M g with the mobile
+    particles' (F, T) and everything else's velocity / strain known (zero)
+    and the rest unknown, x_mid = x + dt/2 U1, U2 the same at x_mid, x_new
+    = x + dt U2; within 1e-9 of the displacement, and apart from the step
+    without lubrication"""
     pe, dt, nch = 1.0, 1e-3, 30
     D = (3, 6, 11)[version]
     dn = 3 if version == 0 else 6
+    qs = 6 if version == 2 else D
     pos, L = workloads.configuration(n, 0.2, seed=3, gap=2.02)
-    s = ls.Stokes(version, pos, box=(L, L, L), tr=1.0)
+    s = ls.Stokes(version, pos, box=(L, L, L), tr=1.0, nm=nm)
     s.set_lub(4.0 + 1e-12, 3.0, 100, 1)
     s.set_iter(max_it=2000, restart=40, eps=1e-13)
     o = po.System(version, pos, box=(L, L, L), tr=1.0)
-    if version == 0:
-        w, _ = _spectrum(o)
+    if version < 2:
+        G = o.matrix()[:dn * nm, :dn * nm]
+    elif nm == n:
+        G = po.ft_mobility_dense(o)
     else:
-        G = o.matrix() if version == 1 else po.ft_mobility_dense(o)
-        w = np.linalg.eigvalsh(0.5 * (G + G.T))
+        G = np.stack([o.fts_mix_ft(nm, np.eye(6 * nm)[c])
+                      for c in range(6 * nm)], axis=1)
+    w = np.linalg.eigvalsh(0.5 * (G + G.T))
     e0, e1 = 0.95 * w[0], 1.05 * w[-1]
     rng = np.random.default_rng(5 + version)
-    F = rng.standard_normal(3 * n)
-    T = rng.standard_normal(3 * n)
+    F = rng.standard_normal(3 * nm)
+    T = rng.standard_normal(3 * nm)
     Fd = ls.DeviceArray.from_host(F)
     Td = ls.DeviceArray.from_host(T)
     bd = ls.BD(s, peclet=pe, n_cheb=nch, eps=1e-6, dt_lim=1e-6, rmin_ol=1.0,
@@ -909,44 +916,42 @@ def test_bd_step_with_lubrication(version, n):
     if version == 2:
         bd.set_fts_iter(30, 2000, 1e-13)
     bd.set_lub(True)
-    x = pos.reshape(-1).copy()
+    full = pos.reshape(-1).copy()
+    x = full[:3 * nm].copy()
     xd = ls.DeviceArray.from_host(x)
     info = bd.step(xd, dt)
     assert info.attempts == 1 and info.dt == dt
     y, z = bd.last_noise()
-    assert z.size == dn * n
-    f = np.zeros((n, dn))
-    f[:, :3] = F.reshape(n, 3)
+    assert z.size == dn * nm
+    f = np.zeros((nm, dn))
+    f[:, :3] = F.reshape(nm, 3)
     if version > 0:
-        f[:, 3:] = T.reshape(n, 3)
+        f[:, 3:] = T.reshape(nm, 3)
     f = f.reshape(-1) + math.sqrt(2.0 / (pe * dt)) * z
 
-    def maps(xp):
-        o.pos[:] = xp
+    def drift(xp, lub=True):
+        p = full.copy()
+        p[:3 * nm] = xp
+        o.pos[:] = p
         Mx = o.matrix()
         M = po.trans_mat_ext2ext(n, Mx) if version == 2 else Mx
-        return M, o.lub_columns()
-
-    def drift(xp, lub=True):
-        M, Lm = maps(xp)
+        Lm = o.lub_columns()
         assert np.linalg.norm(Lm) > 0
         if not lub:
             Lm = np.zeros_like(Lm)
-        if version < 2:
-            v = np.linalg.solve(np.eye(D * n) + M @ Lm, M @ f)
-        else:
-            b = np.zeros((n, 11))
-            b[:, :6] = f.reshape(n, 6)
-            P = np.eye(D * n) + M @ Lm
-            q = np.arange(D * n) % D
-            K, H = np.flatnonzero(q < 6), np.flatnonzero(q >= 6)
-            A = np.concatenate([P[:, K], -M[:, H]], axis=1)
-            bb = b.reshape(-1)
-            sol = np.linalg.solve(A, M[:, K] @ bb[K] - P[:, H] @ bb[H])
-            v = np.empty(D * n)
-            v[K] = sol[:K.size]
-            v[H] = sol[K.size:]
-        return v.reshape(n, D)[:, :3].reshape(-1)
+        b = np.zeros((n, D))
+        b[:nm, :dn] = f.reshape(nm, dn)
+        b = b.reshape(-1)
+        P = np.eye(D * n) + M @ Lm
+        q, i = np.arange(D * n) % D, np.arange(D * n) // D
+        K = np.flatnonzero((i < nm) & (q < qs))
+        H = np.flatnonzero(~((i < nm) & (q < qs)))
+        A = np.concatenate([P[:, K], -M[:, H]], axis=1)
+        sol = np.linalg.solve(A, M[:, K] @ b[K] - P[:, H] @ b[H])
+        v = np.empty(D * n)
+        v[K] = sol[:K.size]
+        v[H] = sol[K.size:]
+        return v.reshape(n, D)[:nm, :3].reshape(-1)
 
     xm = x + 0.5 * dt * drift(x)
     xn = x + dt * drift(xm)
@@ -957,8 +962,8 @@ def test_bd_step_with_lubrication(version, n):
     xm0 = x + 0.5 * dt * drift(x, False)
     x0 = x + dt * drift(xm0, False)
     d0 = np.abs(x0 - xn).max() / disp
-    _say(f"  lubricated step version={version} N={n}: |x_gpu - x_numpy| / disp "
-         f"= {e:.2e}, without lubrication {d0:.2e}")
+    _say(f"  lubricated step version={version} N={n} nm={nm}: |x_gpu - x_numpy| "
+         f"/ disp = {e:.2e}, without lubrication {d0:.2e}")
     assert e < 1e-9
     assert d0 > 1e-3
     bd.close()
