@@ -871,9 +871,10 @@ def test_bd_sheared_retries_match_oracle(scheme, halvings):
     bd.close()
 
 
-@pytest.mark.parametrize("version,n,nm", [(0, 16, 16), (1, 16, 16), (2, 12, 12),
-                                          (0, 16, 12), (2, 12, 9)])
-def test_bd_step_with_lubrication(version, n, nm):
+@pytest.mark.parametrize("version,n,nm,scheme", [
+    (0, 16, 16, 0), (1, 16, 16, 0), (2, 12, 12, 0), (0, 16, 12, 0), (2, 12, 9, 0),
+    (0, 16, 16, 2), (1, 16, 16, 2)])
+def test_bd_step_with_lubrication(version, n, nm, scheme):
     """flag_lub (stokes_hip_bd_set_lub): the drift of both stages of the
     mid-point step is the lubricated mixed problem of solve_mix_3all
     (solve_mix_lub_3f, ewald-3f.c:646-705 -> solve_mob_lub_3f when nothing
@@ -884,8 +885,9 @@ def test_bd_step_with_lubrication(version, n, nm):
     on the mobile particles, v from (I + M L) v = M g with the mobile
     particles' (F, T) and everything else's velocity / strain known (zero)
     and the rest unknown, x_mid = x + dt/2 U1, U2 the same at x_mid, x_new
-    = x + dt U2; within 1e-9 of the displacement, and apart from the step
-    without lubrication"""
+    = x + dt U2 (Ball-Melrose 1997, scheme 2: x1 = x + dt U1, x_new = x +
+    dt (U1 + U(x1)) / 2); within 1e-9 of the displacement, and apart from
+    the step without lubrication"""
     pe, dt, nch = 1.0, 1e-3, 30
     D = (3, 6, 11)[version]
     dn = 3 if version == 0 else 6
@@ -916,6 +918,8 @@ def test_bd_step_with_lubrication(version, n, nm):
     if version == 2:
         bd.set_fts_iter(30, 2000, 1e-13)
     bd.set_lub(True)
+    if scheme:
+        bd.set_scheme(scheme)
     full = pos.reshape(-1).copy()
     x = full[:3 * nm].copy()
     xd = ls.DeviceArray.from_host(x)
@@ -953,16 +957,21 @@ def test_bd_step_with_lubrication(version, n, nm):
         v[H] = sol[K.size:]
         return v.reshape(n, D)[:nm, :3].reshape(-1)
 
-    xm = x + 0.5 * dt * drift(x)
-    xn = x + dt * drift(xm)
+    def step(lub):
+        u1 = drift(x, lub)
+        if scheme == 0:
+            return x + dt * drift(x + 0.5 * dt * u1, lub)
+        return x + 0.5 * dt * (u1 + drift(x + dt * u1, lub))
+
+    xn = step(True)
     xg = xd.to_host()
     disp = np.abs(xn - x).max()
     e = np.abs(xg - xn).max() / disp
     # the same noise without lubrication moves the particles elsewhere
-    xm0 = x + 0.5 * dt * drift(x, False)
-    x0 = x + dt * drift(xm0, False)
+    x0 = step(False)
     d0 = np.abs(x0 - xn).max() / disp
-    _say(f"  lubricated step version={version} N={n} nm={nm}: |x_gpu - x_numpy| "
+    _say(f"  lubricated step version={version} N={n} nm={nm} scheme={scheme}: "
+         f"|x_gpu - x_numpy| "
          f"/ disp = {e:.2e}, without lubrication {d0:.2e}")
     assert e < 1e-9
     assert d0 > 1e-3
