@@ -873,7 +873,7 @@ def test_bd_sheared_retries_match_oracle(scheme, halvings):
 
 @pytest.mark.parametrize("version,n,nm,scheme", [
     (0, 16, 16, 0), (1, 16, 16, 0), (2, 12, 12, 0), (0, 16, 12, 0), (2, 12, 9, 0),
-    (0, 16, 16, 2), (1, 16, 16, 2)])
+    (0, 16, 16, 2), (1, 16, 16, 2), (0, 16, 16, 1), (1, 16, 12, 1)])
 def test_bd_step_with_lubrication(version, n, nm, scheme):
     """flag_lub (stokes_hip_bd_set_lub): the drift of both stages of the
     mid-point step is the lubricated mixed problem of solve_mix_3all
@@ -886,8 +886,10 @@ def test_bd_step_with_lubrication(version, n, nm, scheme):
     particles' (F, T) and everything else's velocity / strain known (zero)
     and the rest unknown, x_mid = x + dt/2 U1, U2 the same at x_mid, x_new
     = x + dt U2 (Ball-Melrose 1997, scheme 2: x1 = x + dt U1, x_new = x +
-    dt (U1 + U(x1)) / 2); within 1e-9 of the displacement, and apart from
-    the step without lubrication"""
+    dt (U1 + U(x1)) / 2; Banchio-Brady 2003, scheme 1 with n = 4: u0 the
+    drift of the noise force alone at x, x1 = x + dt u0 / n, x_new = x + dt
+    (U_F(x) + u0 + n/2 (U_z(x1) - u0)), brownian.c:2579-2920); within 1e-9
+    of the displacement, and apart from the step without lubrication"""
     pe, dt, nch = 1.0, 1e-3, 30
     D = (3, 6, 11)[version]
     dn = 3 if version == 0 else 6
@@ -919,7 +921,7 @@ def test_bd_step_with_lubrication(version, n, nm, scheme):
         bd.set_fts_iter(30, 2000, 1e-13)
     bd.set_lub(True)
     if scheme:
-        bd.set_scheme(scheme)
+        bd.set_scheme(scheme, 4.0)
     full = pos.reshape(-1).copy()
     x = full[:3 * nm].copy()
     xd = ls.DeviceArray.from_host(x)
@@ -927,13 +929,14 @@ def test_bd_step_with_lubrication(version, n, nm, scheme):
     assert info.attempts == 1 and info.dt == dt
     y, z = bd.last_noise()
     assert z.size == dn * nm
-    f = np.zeros((nm, dn))
-    f[:, :3] = F.reshape(nm, 3)
+    fdet = np.zeros((nm, dn))
+    fdet[:, :3] = F.reshape(nm, 3)
     if version > 0:
-        f[:, 3:] = T.reshape(nm, 3)
-    f = f.reshape(-1) + math.sqrt(2.0 / (pe * dt)) * z
+        fdet[:, 3:] = T.reshape(nm, 3)
+    fdet = fdet.reshape(-1)
+    fz = math.sqrt(2.0 / (pe * dt)) * z
 
-    def drift(xp, lub=True):
+    def drift(xp, lub=True, f=fdet + fz):
         p = full.copy()
         p[:3 * nm] = xp
         o.pos[:] = p
@@ -958,6 +961,10 @@ def test_bd_step_with_lubrication(version, n, nm, scheme):
         return v.reshape(n, D)[:nm, :3].reshape(-1)
 
     def step(lub):
+        if scheme == 1:
+            u0 = drift(x, lub, fz)
+            u = drift(x + 0.25 * dt * u0, lub, fz)
+            return x + dt * (drift(x, lub, fdet) + u0 + 2.0 * (u - u0))
         u1 = drift(x, lub)
         if scheme == 0:
             return x + dt * drift(x + 0.5 * dt * u1, lub)
